@@ -1,0 +1,139 @@
+"""Loader and prototypes of libdmd_scatt.so (the HIP library behind the C-ABI
+declared in include/dmd_scatt.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (or ``make -C
+denmat_dynm_development_amd/csrc``).  There is no fallback: when the shared
+object is missing every entry point raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdmd_scatt.so")
+
+DMD_OK = 0
+DMD_ERR_ARG = -1
+DMD_ERR_HIP = -2
+DMD_ERR_STATE = -3
+DMD_ERR_UNSUPPORTED = -4
+DMD_ERR_IO = -5
+DMD_ERR_NOMEM = -6
+
+DMD_PREC_FP64 = 0
+DMD_PREC_C64 = 1
+DMD_PREC_BF16 = 2
+
+DMD_KERNEL_AUTO = 0
+DMD_KERNEL_ROWS = 1
+DMD_KERNEL_TILES = 2
+
+
+class DMDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class SynthParams(ctypes.Structure):
+    """dmd_synth_params (include/dmd_scatt.h)."""
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("nb", ctypes.c_int),
+        ("n_modes", ctypes.c_int),
+        ("nk", ctypes.c_int64),
+        ("partners", ctypes.c_int64),
+        ("partners_min", ctypes.c_int64),
+        ("partners_max", ctypes.c_int64),
+        ("powerlaw_alpha", ctypes.c_double),
+        ("e_max", ctypes.c_double),
+        ("split_max", ctypes.c_double),
+        ("gap_min", ctypes.c_double),
+        ("gap_max", ctypes.c_double),
+        ("omega_min", ctypes.c_double),
+        ("omega_max", ctypes.c_double),
+        ("sigma_g", ctypes.c_double),
+        ("degauss", ctypes.c_double),
+        ("temperature", ctypes.c_double),
+        ("sigma_imp", ctypes.c_double),
+        ("imp_fraction", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_pd = ctypes.POINTER(ctypes.c_double)
+_pSP = ctypes.POINTER(SynthParams)
+
+# name -> argtypes (all return int unless listed in _RESTYPE)
+PROTOTYPES = {
+    "dmd_scatt_last_error": [],
+    "dmd_scatt_version": [],
+    "dmd_scatt_create": [_i64, _i, _i64, _vp, _vp, _vp, _d, _i, ctypes.POINTER(_vp)],
+    "dmd_scatt_destroy": [_vp],
+    "dmd_scatt_upload_P": [_vp, _vp, _vp, _d],
+    "dmd_scatt_add_P": [_vp, _vp, _vp, _d],
+    "dmd_scatt_download_P": [_vp, _i64, _i64, _vp, _vp],
+    "dmd_scatt_generate_P": [_vp, _pSP],
+    "dmd_scatt_set_flags": [_vp, _i, _i, _i],
+    "dmd_scatt_set_ddmdt_eq": [_vp, _vp],
+    "dmd_scatt_compute_ddmdt_eq": [_vp, _vp],
+    "dmd_scatt_eval": [_vp, _d, _vp, _vp, _vp],
+    "dmd_scatt_eval_batched": [_vp, _d, _i, _vp, _vp],
+    "dmd_scatt_compute_ddm": [_vp, _vp, _vp, _vp],
+    "dmd_scatt_set_window": [_vp, _i, _i, _vp],
+    "dmd_scatt_evolve_driver": [_vp, _d, _vp, _vp, _vp],
+    "dmd_scatt_set_coherent": [_vp, _vp],
+    "dmd_scatt_set_observable": [_vp, _vp, _vp, _i, _i, _d],
+    "dmd_scatt_measure": [_vp, _d, _vp, _pd],
+    "dmd_scatt_integrate_rk4": [_vp, _d, _d, _i64, _i, _vp, _vp],
+    "dmd_scatt_integrate_rk4_batched": [_vp, _i, _d, _d, _i64, _i, _vp, _vp],
+    "dmd_scatt_info": [_vp, _pi64, _pi64, _pd],
+    "dmd_scatt_time_eval": [_vp, _d, _i, _pd, _pd],
+    "dmd_synth_preset": [_i, _pSP],
+    "dmd_synth_energies": [_pSP, _vp],
+    "dmd_synth_kpairs": [_pSP, _vp, _vp, _pi64],
+    "dmd_synth_P": [_pSP, _vp, _i64, _vp, _vp, _vp, _vp, _i],
+    "dmd_synth_couplings": [_pSP, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp],
+    "dmd_synth_spin": [_pSP, _vp],
+    "dmd_ldbd_write_synth": [ctypes.c_char_p, _pSP, _i],
+}
+_RESTYPE = {"dmd_scatt_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def load():
+    """Loads libdmd_scatt.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(there is no CPU fallback for the scattering operator)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, args in PROTOTYPES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, ctypes.c_int)
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().dmd_scatt_last_error().decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != DMD_OK:
+        raise DMDError(rc, f"{what}: {last_error()}")
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(load(), name)(*args), name)

@@ -1,0 +1,775 @@
+// dmd_scatt.hip -- MI355X (gfx950) Lindblad scattering operator for DMD.
+//
+// Hot path: electronphonon::evolve (src_v4.5.7/evolve_eph.cpp:34-166).  For
+// every stored pair (k <= k') with P1, P2 (nb^4 complex each) it accumulates
+//   Pdm_k  += P1t . vec(rho_k')           dm1P_k  += vec(1-rho_k')^T . P2t
+// and, for k < k', the reverse with P1_next = conj(P2t), P2_next = conj(P1t)
+//   Pdm_k' += conj(P2t) . vec(rho_k)      dm1P_k' += vec(1-rho_k)^T . conj(P1t)
+// then per k: C = (1-rho_k) mat(Pdm_k) - mat(dm1P_k) rho_k (zhemm, upper
+// triangle), ddmdt_k = prefac/2 (C + C^dagger), minus ddmdt_eq (optional).
+//
+// MI355X design.
+//  * P lives in HBM once, per stored pair [P1 | P2] = 2 nb^4 complex128
+//    contiguous, pairs sorted by (k, k') (CSR rows = k).
+//  * The interaction-picture phase of compute_Pt (evolve_eph.cpp:169-187)
+//    factorises: P1t[(i1 i2),(i3 i4)] = P1 u_k[i1] conj(u_k[i2]) conj(u_k'[i3])
+//    u_k'[i4] with u = e^{i e t}.  So rho is rotated to the Schroedinger
+//    picture once per k (nk nb^2 sincos instead of 2 nb^4 per pair), P is
+//    streamed phase free, and the result is rotated back in the epilogue.
+//  * Row kernel: one 64-wide wavefront per destination row; lanes map to the
+//    elements of [P1 | P2] so each load instruction moves 1 KiB of P fully
+//    coalesced (nb = 2: two pairs per instruction); partner rho rows are
+//    gathered from L2; per-lane FP64 partial sums are reduced once per row in
+//    LDS and the Hermitian epilogue is fused.  Deterministic: fixed order.
+//  * Time enters through a device-resident clock so a whole RK4 step is
+//    captured once in a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dmd_internal.h"
+#include "synth.h"
+
+namespace dmd {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+// Hermitian completion from the upper triangle with a real diagonal, the
+// way CBLAS zhemm(CblasUpper) reads A.
+template <int NB>
+__device__ __forceinline__ double2 herm_up(const double2 *A, int i, int j)
+{
+    if (i < j) return A[i * NB + j];
+    if (i > j) return cconj(A[j * NB + i]);
+    return make_double2(A[i * NB + i].x, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// rho -> Schroedinger picture: rhoS[i,j] = rho[i,j] cis(-(e_i - e_j) t)
+// (window rows/cols [b0, b0+NB) of an nbd x nbd state).  With a separate
+// 1-rho array (the reference's dm1 argument) it is rotated alike.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ void k_rotate_in(int64_t nk, int nbd, int b0, const double2 *__restrict__ rho,
+                            const double2 *__restrict__ rho1, const double *__restrict__ e,
+                            const DClock *__restrict__ clk, double a, int expt,
+                            double2 *__restrict__ rhoS, double2 *__restrict__ rho1S)
+{
+    constexpr int NB2 = NB * NB;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nk * NB2) return;
+    int64_t k = idx / NB2;
+    int ij = (int)(idx - k * NB2), i = ij / NB, j = ij - i * NB;
+    int64_t src = k * (int64_t)nbd * nbd + (int64_t)(b0 + i) * nbd + (b0 + j);
+    double2 v = rho[src];
+    double2 ph = make_double2(1.0, 0.0);
+    bool rot = expt && i != j;
+    if (rot) ph = ccis(-(e[k * NB + i] - e[k * NB + j]) * clock_t_at(clk, a));
+    rhoS[idx] = rot ? cmul(v, ph) : v;
+    if (rho1) {
+        double2 w = rho1[src];
+        rho1S[idx] = rot ? cmul(w, ph) : w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel (deterministic): one wavefront per destination row k.
+// Lane roles: element e of a pair's [P1 | P2] (2 NB^4 complex).  For
+// 2 NB^4 <= 64 a load instruction covers 64/(2 NB^4) pairs; otherwise a
+// pair spans J = ceil(2 NB^4 / 64) registers per lane.
+// Forward pair (k, k'):  P1 lane (i12,i34): af += P1 * rhoS_k'[i34]      -> Pdm[i12]
+//                        P2 lane (i12,i34): af += (1-rhoS_k')[i12] * P2  -> dm1P[i34]
+// Reverse pair (k'', k): P1 lane: ar += (1-rhoS_k'')[i12] * conj(P1)     -> dm1P[i34]
+//                        P2 lane: ar += conj(P2) * rhoS_k''[i34]         -> Pdm[i12]
+// ---------------------------------------------------------------------------
+template <int NB>
+struct Shape {
+    static constexpr int NB2 = NB * NB, NB4 = NB2 * NB2, EP = 2 * NB4;
+    static constexpr int PPI = EP <= 64 ? 64 / EP : 1;  // pairs per load instruction
+    static constexpr int J = EP <= 64 ? 1 : (EP + 63) / 64;
+    static constexpr int SLOTS = EP <= 64 ? PPI * EP : 64 * J;  // LDS slots per wave
+};
+
+template <int NB, bool DM1>
+__device__ __forceinline__ double2 rho_elem(const double2 *__restrict__ rhoS, const double2 *__restrict__ rho1S,
+                                            int64_t kp, int idx, bool one)
+{
+    constexpr int NB2 = NB * NB;
+    if (DM1 && one) return rho1S[kp * NB2 + idx];
+    double2 x = rhoS[kp * NB2 + idx];
+    if (!one) return x;
+    bool diag = (idx % (NB + 1)) == 0;
+    return make_double2((diag ? 1.0 : 0.0) - x.x, -x.y);
+}
+
+template <int NB, bool DM1>
+__global__ __launch_bounds__(256) void k_rows(int64_t nk, const double2 *__restrict__ P,
+                                              const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                              const int64_t *__restrict__ revptr,
+                                              const int64_t *__restrict__ revpos,
+                                              const int32_t *__restrict__ revcol,
+                                              const double2 *__restrict__ rhoS, const double2 *__restrict__ rho1S,
+                                              const double *__restrict__ e, const DClock *__restrict__ clk,
+                                              double a, int expt, int schrodinger, double prefac,
+                                              const double2 *__restrict__ eq, double2 *__restrict__ out)
+{
+    using S = Shape<NB>;
+    constexpr int NB2 = S::NB2, NB4 = S::NB4, EP = S::EP, PPI = S::PPI, J = S::J, SLOTS = S::SLOTS;
+    __shared__ double2 red[4][2][SLOTS];
+    __shared__ double2 vec[4][4][NB2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t k = (int64_t)blockIdx.x * 4 + wv;
+    const bool active = k < nk;
+
+    const int slot = EP <= 64 ? lane / EP : 0;
+    int eidx[J], fidx[J], ridx[J];
+    bool fone[J], rone[J], valid[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        int el = EP <= 64 ? lane % EP : lane + 64 * j;
+        valid[j] = EP <= 64 ? true : el < EP;
+        if (!valid[j]) el = 0;
+        int which = el / NB4, r = el - which * NB4, i12 = r / NB2, i34 = r - i12 * NB2;
+        eidx[j] = el;
+        fidx[j] = which == 0 ? i34 : i12;
+        fone[j] = which == 1;
+        ridx[j] = which == 0 ? i12 : i34;
+        rone[j] = which == 0;
+    }
+    double2 af[J], ar[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) { af[j] = make_double2(0, 0); ar[j] = make_double2(0, 0); }
+
+    if (active) {
+        // forward pairs of row k: contiguous P
+        const int64_t p0 = rowptr[k], p1 = rowptr[k + 1];
+        constexpr int U = EP <= 64 ? 4 : 1;
+        for (int64_t q = p0; q < p1; q += PPI * U) {
+            double2 pv[U][J];
+            double2 rv[U][J];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                int64_t pair = q + (int64_t)u * PPI + slot;
+                ok[u] = pair < p1;
+                int64_t kp = ok[u] ? (int64_t)col[pair] : 0;
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    bool v = ok[u] && valid[j];
+                    pv[u][j] = v ? P[pair * EP + eidx[j]] : make_double2(0, 0);
+                    rv[u][j] = v ? rho_elem<NB, DM1>(rhoS, rho1S, kp, fidx[j], fone[j]) : make_double2(0, 0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int j = 0; j < J; j++) af[j] = cadd(af[j], cmul(pv[u][j], rv[u][j]));
+        }
+        // reverse pairs (k'', k), k'' < k: P at their own rows
+        const int64_t r0 = revptr[k], r1 = revptr[k + 1];
+        for (int64_t q = r0; q < r1; q += PPI * U) {
+            double2 pv[U][J];
+            double2 rv[U][J];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                int64_t it = q + (int64_t)u * PPI + slot;
+                bool ok = it < r1;
+                int64_t pos = ok ? revpos[it] : 0;
+                int64_t kpp = ok ? (int64_t)revcol[it] : 0;
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    bool v = ok && valid[j];
+                    pv[u][j] = v ? P[pos * EP + eidx[j]] : make_double2(0, 0);
+                    rv[u][j] = v ? rho_elem<NB, DM1>(rhoS, rho1S, kpp, ridx[j], rone[j]) : make_double2(0, 0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int j = 0; j < J; j++) ar[j] = cadd(ar[j], cmulc(rv[u][j], pv[u][j]));
+        }
+    }
+    // per-row reduction of the lane partials
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        int s = EP <= 64 ? lane : lane + 64 * j;
+        if (s < SLOTS) { red[wv][0][s] = af[j]; red[wv][1][s] = ar[j]; }
+    }
+    __syncthreads();
+    if (lane < NB2) {
+        double2 pdm = make_double2(0, 0), d1p = make_double2(0, 0);
+        for (int sl = 0; sl < PPI; sl++) {
+            const int base = sl * EP;
+            for (int m = 0; m < NB2; m++) {
+                // Pdm[lane=i12]: forward P1 (i12, m) + reverse P2 (i12, m)
+                pdm = cadd(pdm, cadd(red[wv][0][base + lane * NB2 + m], red[wv][1][base + NB4 + lane * NB2 + m]));
+                // dm1P[lane=i34]: forward P2 (m, i34) + reverse P1 (m, i34)
+                d1p = cadd(d1p, cadd(red[wv][0][base + NB4 + m * NB2 + lane], red[wv][1][base + m * NB2 + lane]));
+            }
+        }
+        vec[wv][0][lane] = pdm;
+        vec[wv][1][lane] = d1p;
+        if (active) {
+            vec[wv][2][lane] = rhoS[k * NB2 + lane];
+            vec[wv][3][lane] = rho_elem<NB, DM1>(rhoS, rho1S, k, lane, true);
+        }
+    }
+    __syncthreads();
+    if (!active || lane >= NB2) return;
+    // epilogue: C = (1-rho) Pdm - dm1P rho (zhemm upper), Hermitian part, rotate back
+    const int i = lane / NB, j = lane - i * NB;
+    const double2 *pdm = vec[wv][0], *d1p = vec[wv][1], *rk = vec[wv][2], *r1k = vec[wv][3];
+    double2 cij = make_double2(0, 0), cji = make_double2(0, 0);
+#pragma unroll
+    for (int m = 0; m < NB; m++) {
+        cij = cadd(cij, cmul(herm_up<NB>(r1k, i, m), pdm[m * NB + j]));
+        cji = cadd(cji, cmul(herm_up<NB>(r1k, j, m), pdm[m * NB + i]));
+    }
+#pragma unroll
+    for (int m = 0; m < NB; m++) {
+        cij = csub(cij, cmul(d1p[i * NB + m], herm_up<NB>(rk, m, j)));
+        cji = csub(cji, cmul(d1p[j * NB + m], herm_up<NB>(rk, m, i)));
+    }
+    double2 d = cadd(cij, cconj(cji));
+    d = make_double2(d.x * (prefac * 0.5), d.y * (prefac * 0.5));
+    const double t = clock_t_at(clk, a);
+    const double phase_arg = (e[k * NB + i] - e[k * NB + j]) * t;
+    if (expt && i != j) d = cmul(d, ccis(phase_arg));
+    if (eq) {
+        double2 q = eq[k * NB2 + lane];
+        if (i != j && !schrodinger) q = cmul(q, ccis(phase_arg));
+        d = csub(d, q);
+    }
+    out[k * NB2 + lane] = d;
+}
+
+// ---------------------------------------------------------------------------
+// P placement: file order -> sorted layout, scaled; mode 0 set, 1 add.
+// ---------------------------------------------------------------------------
+__global__ void k_place_P(int64_t n, int64_t first, int nb4, const double2 *__restrict__ P1,
+                          const double2 *__restrict__ P2, const int64_t *__restrict__ inv, double scale, int add,
+                          double2 *__restrict__ Pd)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = n * 2 * nb4;
+    if (idx >= tot) return;
+    int64_t f = idx / (2 * nb4);
+    int el = (int)(idx - f * 2 * nb4);
+    int64_t pos = inv ? inv[first + f] : first + f;
+    double2 v = el < nb4 ? P1[f * nb4 + el] : P2[f * nb4 + el - nb4];
+    v = make_double2(v.x * scale, v.y * scale);
+    double2 *d = Pd + pos * 2 * nb4 + el;
+    *d = add ? cadd(*d, v) : v;
+}
+__global__ void k_fetch_P(int64_t n, int64_t first, int nb4, const double2 *__restrict__ Pd,
+                          const int64_t *__restrict__ inv, double2 *__restrict__ P1, double2 *__restrict__ P2)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = n * 2 * nb4;
+    if (idx >= tot) return;
+    int64_t f = idx / (2 * nb4);
+    int el = (int)(idx - f * 2 * nb4);
+    int64_t pos = inv ? inv[first + f] : first + f;
+    double2 v = Pd[pos * 2 * nb4 + el];
+    if (el < nb4) P1[f * nb4 + el] = v;
+    else P2[f * nb4 + el - nb4] = v;
+}
+
+template <int NB>
+__global__ void k_generate_P(int64_t np, dmd_synth_params p, const int32_t *__restrict__ k1s,
+                             const int32_t *__restrict__ col, const double *__restrict__ e, double2 *__restrict__ Pd)
+{
+    constexpr int NB4 = NB * NB * NB * NB;
+    int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= np) return;
+    int64_t k = k1s[pos], kp = col[pos];
+    double P1[2 * NB4], P2[2 * NB4];
+    synth_pair_P_nb<NB>(p, e + k * NB, e + kp * NB, k, kp, P1, P2);
+    double2 *d = Pd + pos * 2 * NB4;
+    for (int i = 0; i < NB4; i++) {
+        d[i] = make_double2(P1[2 * i], P1[2 * i + 1]);
+        d[NB4 + i] = make_double2(P2[2 * i], P2[2 * i + 1]);
+    }
+}
+
+}  // namespace dmd
+
+using namespace dmd;
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+namespace dmd {
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+}  // namespace dmd
+dmd_scatt::~dmd_scatt()
+{
+    dfree(d_e); dfree(d_edm); dfree(d_P); dfree(d_inv); dfree(d_rowptr); dfree(d_revptr); dfree(d_revpos);
+    dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
+    dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
+    dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part);
+    dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
+    for (void *p : extra) dfree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (ev2) (void)hipEventDestroy(ev2);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+namespace dmd {
+// Copies n elements of host-or-device src into device dst (stream ordered).
+int to_device(dmd_scatt *h, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return DMD_OK;
+    DMD_HIP(hipMemcpyAsync(dst, src, bytes, is_device_ptr(src) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                           h->st));
+    return DMD_OK;
+}
+int from_device(dmd_scatt *h, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return DMD_OK;
+    DMD_HIP(hipMemcpyAsync(dst, src, bytes, is_device_ptr(dst) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           h->st));
+    return DMD_OK;
+}
+int set_clock(dmd_scatt *h, double t0, double dt, long long step)
+{
+    h->h_clk.t0 = t0; h->h_clk.dt = dt; h->h_clk.step = step; h->h_clk.pad = 0;
+    DMD_HIP(hipMemcpyAsync(h->d_clk, &h->h_clk, sizeof(DClock), hipMemcpyHostToDevice, h->st));
+    return DMD_OK;
+}
+
+
+}  // namespace dmd
+
+extern "C" const char *dmd_scatt_last_error(void) { return g_err.c_str(); }
+extern "C" int dmd_scatt_version(void) { return 1; }
+
+extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint64_t *k1st, const uint64_t *k2nd,
+                                const double *e, double nk_full, int precision, dmd_scatt **out)
+{
+    if (!out) return fail(DMD_ERR_ARG, "dmd_scatt_create: out is null");
+    *out = nullptr;
+    if (nk < 1 || nk >= ((int64_t)1 << 31)) return fail(DMD_ERR_ARG, "dmd_scatt_create: nk out of range");
+    if (nb < 1 || nb > 4) return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_create: nb must be 1..4 (got " + std::to_string(nb) + ")");
+    if (n_pairs < 0 || (n_pairs > 0 && (!k1st || !k2nd))) return fail(DMD_ERR_ARG, "dmd_scatt_create: bad k-pair lists");
+    if (!e) return fail(DMD_ERR_ARG, "dmd_scatt_create: e is null");
+    if (!(nk_full > 0)) return fail(DMD_ERR_ARG, "dmd_scatt_create: nk_full must be positive");
+    if (precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_create: precision not available");
+    for (int64_t i = 0; i < n_pairs; i++)
+        if (k1st[i] > k2nd[i] || k2nd[i] >= (uint64_t)nk)
+            return fail(DMD_ERR_ARG, "dmd_scatt_create: pair " + std::to_string(i) + " is not k1st <= k2nd < nk");
+
+    std::unique_ptr<dmd_scatt> h(new dmd_scatt());
+    DMD_HIP(hipGetDevice(&h->dev));
+    DMD_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    DMD_HIP(hipEventCreate(&h->ev0));
+    DMD_HIP(hipEventCreate(&h->ev1));
+    DMD_HIP(hipEventCreate(&h->ev2));
+    h->nk = nk; h->nb = nb; h->nb2 = nb * nb; h->nb4 = nb * nb * nb * nb; h->np = n_pairs;
+    h->prefac = 2.0 * M_PI / nk_full;  // ElectronPhonon.h:61
+    h->precision = precision;
+    h->nbd = nb; h->b0 = 0;
+
+    // sorted order by (k1, k2): counting sort by k1, then by k2 within rows
+    std::vector<int64_t> cnt(nk + 1, 0);
+    bool sorted = true;
+    for (int64_t i = 0; i < n_pairs; i++) {
+        cnt[k1st[i] + 1]++;
+        if (i && (k1st[i] < k1st[i - 1] || (k1st[i] == k1st[i - 1] && k2nd[i] < k2nd[i - 1]))) sorted = false;
+    }
+    for (int64_t k = 0; k < nk; k++) cnt[k + 1] += cnt[k];
+    std::vector<int64_t> order;
+    h->identity_order = sorted;
+    if (!sorted) {
+        order.resize(n_pairs);
+        std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+        for (int64_t i = 0; i < n_pairs; i++) order[cur[k1st[i]]++] = i;
+        for (int64_t k = 0; k < nk; k++)
+            std::stable_sort(order.begin() + cnt[k], order.begin() + cnt[k + 1],
+                             [&](int64_t a, int64_t b) { return k2nd[a] < k2nd[b]; });
+    }
+    auto F = [&](int64_t pos) { return sorted ? pos : order[pos]; };
+    std::vector<int32_t> col(n_pairs), k1s(n_pairs);
+    std::vector<int64_t> rcnt(nk + 1, 0);
+    for (int64_t pos = 0; pos < n_pairs; pos++) {
+        int64_t f = F(pos);
+        col[pos] = (int32_t)k2nd[f];
+        k1s[pos] = (int32_t)k1st[f];
+        if (k1st[f] < k2nd[f]) rcnt[k2nd[f] + 1]++;
+    }
+    for (int64_t k = 0; k < nk; k++) rcnt[k + 1] += rcnt[k];
+    h->nrev = rcnt[nk];
+    std::vector<int64_t> revpos(h->nrev);
+    std::vector<int32_t> revcol(h->nrev);
+    {
+        std::vector<int64_t> cur(rcnt.begin(), rcnt.end() - 1);
+        for (int64_t pos = 0; pos < n_pairs; pos++)
+            if (k1s[pos] < col[pos]) {
+                int64_t c = cur[col[pos]]++;
+                revpos[c] = pos;
+                revcol[c] = k1s[pos];
+            }
+    }
+    int rc;
+    const size_t ns = (size_t)nk * h->nb2;
+    if ((rc = dalloc_t(&h->d_e, (size_t)nk * nb, "e")) || (rc = dalloc_t(&h->d_edm, (size_t)nk * nb, "e_dm")) ||
+        (rc = dalloc_t(&h->d_P, (size_t)n_pairs * 2 * h->nb4, "P")) ||
+        (rc = dalloc_t(&h->d_rowptr, nk + 1, "rowptr")) || (rc = dalloc_t(&h->d_revptr, nk + 1, "revptr")) ||
+        (rc = dalloc_t(&h->d_revpos, h->nrev, "revpos")) || (rc = dalloc_t(&h->d_revcol, h->nrev, "revcol")) ||
+        (rc = dalloc_t(&h->d_col, n_pairs, "col")) || (rc = dalloc_t(&h->d_k1, n_pairs, "k1")) ||
+        (rc = dalloc_t(&h->d_rhoS, ns, "rhoS")) || (rc = dalloc_t(&h->d_rho1S, ns, "rho1S")) ||
+        (rc = dalloc_t(&h->d_out, ns, "out")) || (rc = dalloc_t(&h->d_in, ns, "in")) ||
+        (rc = dalloc_t(&h->d_in1, ns, "in1")) || (rc = dalloc_t(&h->d_clk, 1, "clock")))
+        return rc;
+    if (!sorted) {
+        std::vector<int64_t> inv(n_pairs);
+        for (int64_t pos = 0; pos < n_pairs; pos++) inv[order[pos]] = pos;
+        if ((rc = dalloc_t(&h->d_inv, n_pairs, "inv"))) return rc;
+        DMD_HIP(hipMemcpy(h->d_inv, inv.data(), n_pairs * 8, hipMemcpyHostToDevice));
+    }
+    DMD_HIP(hipMemcpy(h->d_e, e, (size_t)nk * nb * 8, is_device_ptr(e) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    DMD_HIP(hipMemcpy(h->d_edm, h->d_e, (size_t)nk * nb * 8, hipMemcpyDeviceToDevice));
+    DMD_HIP(hipMemcpy(h->d_rowptr, cnt.data(), (nk + 1) * 8, hipMemcpyHostToDevice));
+    DMD_HIP(hipMemcpy(h->d_revptr, rcnt.data(), (nk + 1) * 8, hipMemcpyHostToDevice));
+    if (h->nrev) {
+        DMD_HIP(hipMemcpy(h->d_revpos, revpos.data(), h->nrev * 8, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_revcol, revcol.data(), h->nrev * 4, hipMemcpyHostToDevice));
+    }
+    if (n_pairs) {
+        DMD_HIP(hipMemcpy(h->d_col, col.data(), n_pairs * 4, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_k1, k1s.data(), n_pairs * 4, hipMemcpyHostToDevice));
+    }
+    DMD_HIP(hipMemset(h->d_P, 0, (size_t)std::max<int64_t>(n_pairs, 1) * 2 * h->nb4 * sizeof(double2)));
+    for (double2 *b : {h->d_rhoS, h->d_rho1S, h->d_out, h->d_in, h->d_in1}) DMD_HIP(hipMemset(b, 0, ns * sizeof(double2)));
+    DMD_HIP(hipDeviceSynchronize());
+    *out = h.release();
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_destroy(dmd_scatt *h)
+{
+    if (!h) return DMD_OK;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    delete h;
+    return DMD_OK;
+}
+
+static int put_P(dmd_scatt *h, const double *P1, const double *P2, double scale, int add)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (h->np == 0) return DMD_OK;
+    if (!P1 || !P2) return fail(DMD_ERR_ARG, "P1/P2 null");
+    const size_t per = (size_t)h->nb4;  // complex per P array per pair
+    const bool dev1 = is_device_ptr(P1), dev2 = is_device_ptr(P2);
+    // staging chunk: 64 Mi complex per array at most
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)((64ll << 20) / per));
+    double2 *s1 = nullptr, *s2 = nullptr;
+    int rc;
+    if (!dev1 || !dev2) {
+        int64_t c = std::min<int64_t>(chunk, h->np);
+        if ((rc = dalloc_t(&s1, (size_t)c * per, "P staging"))) return rc;
+        if ((rc = dalloc_t(&s2, (size_t)c * per, "P staging"))) { dfree(s1); return rc; }
+    }
+    for (int64_t f0 = 0; f0 < h->np; f0 += chunk) {
+        int64_t n = std::min<int64_t>(chunk, h->np - f0);
+        const double2 *a = (const double2 *)P1 + f0 * per, *b = (const double2 *)P2 + f0 * per;
+        if (!dev1) {
+            hipError_t e = hipMemcpyAsync(s1, a, n * per * 16, hipMemcpyHostToDevice, h->st);
+            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+            a = s1;
+        }
+        if (!dev2) {
+            hipError_t e = hipMemcpyAsync(s2, b, n * per * 16, hipMemcpyHostToDevice, h->st);
+            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+            b = s2;
+        }
+        int64_t tot = n * 2 * (int64_t)per;
+        hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, f0, h->nb4, a, b, h->d_inv, scale,
+                           add, h->d_P);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+        if (!dev1 || !dev2) {
+            e = hipStreamSynchronize(h->st);
+            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+        }
+    }
+    DMD_HIP(hipStreamSynchronize(h->st));
+    dfree(s1); dfree(s2);
+    h->has_P = true;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_upload_P(dmd_scatt *h, const double *P1, const double *P2, double scale)
+{
+    return put_P(h, P1, P2, scale, 0);
+}
+extern "C" int dmd_scatt_add_P(dmd_scatt *h, const double *P1, const double *P2, double scale)
+{
+    return put_P(h, P1, P2, scale, 1);
+}
+
+extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2)
+{
+    if (!h || !P1 || !P2 || first < 0 || count < 0 || first + count > h->np)
+        return fail(DMD_ERR_ARG, "dmd_scatt_download_P: bad arguments");
+    if (count == 0) return DMD_OK;
+    const size_t per = (size_t)h->nb4;
+    int rc;
+    double2 *s1 = nullptr, *s2 = nullptr;
+    if ((rc = dalloc_t(&s1, count * per, "download"))) return rc;
+    if ((rc = dalloc_t(&s2, count * per, "download"))) { dfree(s1); return rc; }
+    int64_t tot = count * 2 * (int64_t)per;
+    hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, count, first, h->nb4, h->d_P, h->d_inv,
+                       s1, s2);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(P1, s1, count * per * 16, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(P2, s2, count * per * 16, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    dfree(s1); dfree(s2);
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
+{
+    if (!h || !p) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: null argument");
+    if (p->nb != h->nb || p->nk != h->nk) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: params do not match the handle");
+    if (h->np == 0) { h->has_P = true; return DMD_OK; }
+    const unsigned g = nblk(h->np, 64);
+    switch (h->nb) {
+    case 1: hipLaunchKernelGGL(k_generate_P<1>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
+    case 2: hipLaunchKernelGGL(k_generate_P<2>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
+    case 3: hipLaunchKernelGGL(k_generate_P<3>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
+    case 4: hipLaunchKernelGGL(k_generate_P<4>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
+    }
+    DMD_HIP(hipGetLastError());
+    DMD_HIP(hipStreamSynchronize(h->st));
+    h->has_P = true;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int kernel)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (kernel < DMD_KERNEL_AUTO || kernel > DMD_KERNEL_TILES) return fail(DMD_ERR_ARG, "unknown kernel");
+    if (schrodinger && expt) return fail(DMD_ERR_ARG, "in the schrodinger picture expt must be 0 (parameters.cpp:452-453)");
+    h->expt = expt ? 1 : 0;
+    h->schrodinger = schrodinger ? 1 : 0;
+    h->kernel = kernel;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_set_ddmdt_eq(dmd_scatt *h, const double *eq)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (!eq) { h->has_eq = false; return DMD_OK; }
+    int rc;
+    if (!h->d_eq && (rc = dalloc_t(&h->d_eq, (size_t)h->nk * h->nb2, "ddmdt_eq"))) return rc;
+    if ((rc = to_device(h, h->d_eq, eq, (size_t)h->nk * h->nb2 * 16))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    h->has_eq = true;
+    return DMD_OK;
+}
+
+namespace dmd {
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+// rotate the window of state `in` (nbd x nbd per k, offset b0) to rhoS
+int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a)
+{
+    int64_t n = h->nk * h->nb2;
+    dim3 g(nblk(n, 256)), b(256);
+    switch (h->nb) {
+    case 1: hipLaunchKernelGGL(k_rotate_in<1>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
+    case 2: hipLaunchKernelGGL(k_rotate_in<2>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
+    case 3: hipLaunchKernelGGL(k_rotate_in<3>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
+    case 4: hipLaunchKernelGGL(k_rotate_in<4>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
+    }
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+template <int NB>
+static void launch_rows_nb(dmd_scatt *h, bool dm1, double a, double2 *out)
+{
+    dim3 g(nblk(h->nk, 4)), b(256);
+    const double2 *eq = h->has_eq ? h->d_eq : nullptr;
+    if (dm1)
+        hipLaunchKernelGGL((k_rows<NB, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_revptr,
+                           h->d_revpos, h->d_revcol, h->d_rhoS, h->d_rho1S, h->d_e, h->d_clk, a, h->expt,
+                           h->schrodinger, h->prefac, eq, out);
+    else
+        hipLaunchKernelGGL((k_rows<NB, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_revptr,
+                           h->d_revpos, h->d_revcol, h->d_rhoS, h->d_rho1S, h->d_e, h->d_clk, a, h->expt,
+                           h->schrodinger, h->prefac, eq, out);
+}
+
+// the contraction + epilogue on the current rhoS (and rho1S when dm1)
+int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
+{
+    switch (h->nb) {
+    case 1: launch_rows_nb<1>(h, dm1, a, out); break;
+    case 2: launch_rows_nb<2>(h, dm1, a, out); break;
+    case 3: launch_rows_nb<3>(h, dm1, a, out); break;
+    case 4: launch_rows_nb<4>(h, dm1, a, out); break;
+    }
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+}  // namespace dmd
+
+static int eval_core(dmd_scatt *h, double t, const double *rho, const double *rho1, double *ddmdt, bool timed)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (!rho || !ddmdt) return fail(DMD_ERR_ARG, "rho/ddmdt null");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    const size_t bytes = (size_t)h->nk * h->nb2 * 16;
+    int rc;
+    const double2 *in = (const double2 *)rho, *in1 = (const double2 *)rho1;
+    if (!is_device_ptr(rho)) {
+        if ((rc = to_device(h, h->d_in, rho, bytes))) return rc;
+        in = h->d_in;
+    }
+    if (rho1 && !is_device_ptr(rho1)) {
+        if ((rc = to_device(h, h->d_in1, rho1, bytes))) return rc;
+        in1 = h->d_in1;
+    }
+    if ((rc = set_clock(h, t, 0.0, 0))) return rc;
+    if ((rc = launch_rotate(h, in, in1, h->nb, 0, 0.0))) return rc;
+    double2 *out = is_device_ptr(ddmdt) ? (double2 *)ddmdt : h->d_out;
+    if (timed) DMD_HIP(hipEventRecord(h->ev0, h->st));
+    if ((rc = launch_contract(h, rho1 != nullptr, 0.0, out))) return rc;
+    if (timed) DMD_HIP(hipEventRecord(h->ev1, h->st));
+    if (out == h->d_out && (rc = from_device(h, ddmdt, h->d_out, bytes))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    if (timed) {
+        float ms = 0;
+        DMD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_ms = ms;
+    }
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_eval(dmd_scatt *h, double t, const double *rho, const double *rho1, double *ddmdt)
+{
+    return eval_core(h, t, rho, rho1, ddmdt, true);
+}
+
+extern "C" int dmd_scatt_compute_ddm(dmd_scatt *h, const double *rho, const double *rho1, double *ddm)
+{
+    if (!h || !ddm) return fail(DMD_ERR_ARG, "dmd_scatt_compute_ddm: bad arguments");
+    int save_expt = h->expt;
+    bool save_eq = h->has_eq;
+    h->expt = 0;
+    h->has_eq = false;
+    const size_t n = (size_t)h->nk * h->nb2 * 2;
+    std::vector<double> tmp(n);
+    int rc = eval_core(h, 0.0, rho, rho1, tmp.data(), false);
+    h->expt = save_expt;
+    h->has_eq = save_eq;
+    if (rc) return rc;
+    if (is_device_ptr(ddm)) {
+        std::vector<double> cur(n);
+        DMD_HIP(hipMemcpy(cur.data(), ddm, n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) cur[i] += tmp[i];
+        DMD_HIP(hipMemcpy(ddm, cur.data(), n * 8, hipMemcpyHostToDevice));
+    } else {
+        for (size_t i = 0; i < n; i++) ddm[i] += tmp[i];
+    }
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_compute_ddmdt_eq(dmd_scatt *h, const double *f)
+{
+    // compute_ddmdt_eq (evolve_eph.cpp:3-19): evolve at t=0 with diag(f), diag(1-f)
+    if (!h || !f) return fail(DMD_ERR_ARG, "dmd_scatt_compute_ddmdt_eq: bad arguments");
+    const size_t n = (size_t)h->nk * h->nb2;
+    std::vector<double> fh((size_t)h->nk * h->nb);
+    if (is_device_ptr(f)) DMD_HIP(hipMemcpy(fh.data(), f, fh.size() * 8, hipMemcpyDeviceToHost));
+    else std::memcpy(fh.data(), f, fh.size() * 8);
+    std::vector<double> dm(2 * n, 0.0), dm1(2 * n, 0.0), out(2 * n);
+    for (int64_t k = 0; k < h->nk; k++)
+        for (int i = 0; i < h->nb; i++) {
+            size_t o = 2 * (k * h->nb2 + i * h->nb + i);
+            dm[o] = fh[k * h->nb + i];
+            dm1[o] = 1 - fh[k * h->nb + i];
+        }
+    bool save_eq = h->has_eq;
+    h->has_eq = false;
+    int rc = eval_core(h, 0.0, dm.data(), dm1.data(), out.data(), false);
+    h->has_eq = save_eq;
+    if (rc) return rc;
+    return dmd_scatt_set_ddmdt_eq(h, out.data());
+}
+
+extern "C" int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, double *last_kernel_ms)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (n_pairs) *n_pairs = h->np;
+    if (p_bytes) *p_bytes = h->np * 2 * h->nb4 * 16;
+    if (last_kernel_ms) *last_kernel_ms = h->last_ms;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel_ms, double *eval_ms)
+{
+    if (!h || n < 1) return fail(DMD_ERR_ARG, "dmd_scatt_time_eval: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    int rc;
+    if ((rc = set_clock(h, t, 0.0, 0))) return rc;
+    // d_in holds the last host-staged rho (or zeros); use it as the state
+    if ((rc = launch_rotate(h, h->d_in, nullptr, h->nb, 0, 0.0))) return rc;
+    if ((rc = launch_contract(h, false, 0.0, h->d_out))) return rc;  // warm
+    DMD_HIP(hipStreamSynchronize(h->st));
+    float kms = 0, ems = 0;
+    DMD_HIP(hipEventRecord(h->ev2, h->st));
+    for (int i = 0; i < n; i++) {
+        if ((rc = launch_rotate(h, h->d_in, nullptr, h->nb, 0, 0.0))) return rc;
+        if (i == 0) DMD_HIP(hipEventRecord(h->ev0, h->st));
+        if ((rc = launch_contract(h, false, 0.0, h->d_out))) return rc;
+    }
+    DMD_HIP(hipEventRecord(h->ev1, h->st));
+    DMD_HIP(hipEventSynchronize(h->ev1));
+    DMD_HIP(hipEventElapsedTime(&ems, h->ev2, h->ev1));
+    // kernel-only: time n back-to-back contractions
+    DMD_HIP(hipEventRecord(h->ev0, h->st));
+    for (int i = 0; i < n; i++)
+        if ((rc = launch_contract(h, false, 0.0, h->d_out))) return rc;
+    DMD_HIP(hipEventRecord(h->ev1, h->st));
+    DMD_HIP(hipEventSynchronize(h->ev1));
+    DMD_HIP(hipEventElapsedTime(&kms, h->ev0, h->ev1));
+    if (kernel_ms) *kernel_ms = kms / n;
+    if (eval_ms) *eval_ms = ems / n;
+    h->last_ms = kms / n;
+    return DMD_OK;
+}

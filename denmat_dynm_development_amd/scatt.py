@@ -1,0 +1,297 @@
+"""Host-side mirror of the reference's scattering operator interface.
+
+``ElectronPhonon`` follows the members of the reference's ``electronphonon``
+class that the dynamics driver touches (src_v4.5.7/ElectronPhonon.h:150-165,
+src_v4.5.7/DensityMatrix_Dynamics.h:59-63,268): ``evolve``,
+``evolve_driver``, ``compute_ddmdt_eq``, plus the loaders ``set_eph`` (P and
+k-pairs, ElectronPhonon.cpp:3-13,182-233) and the e-i fold-in
+``add_scatt_contrib`` (ElectronPhonon.cpp:44-113).  Every call goes through
+the C-ABI of libdmd_scatt.so to the HIP kernels; there is no CPU path.
+
+Arrays are numpy complex128/float64 (host) or torch tensors on the GPU; the
+library detects device pointers and then works in place on them.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SynthParams, call
+
+try:  # torch is optional: only used when tensors are passed
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _is_tensor(a):
+    return torch is not None and isinstance(a, torch.Tensor)
+
+
+def _ptr(a, dtype):
+    """(pointer, keepalive) for numpy arrays or torch tensors."""
+    if a is None:
+        return None, None
+    if _is_tensor(a):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        want = torch.complex128 if dtype == np.complex128 else (torch.float64 if dtype == np.float64 else torch.uint64)
+        if a.dtype != want:
+            raise TypeError(f"tensor dtype {a.dtype}, expected {want}")
+        return ctypes.c_void_p(a.data_ptr()), a
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    return ctypes.c_void_p(arr.ctypes.data), arr
+
+
+def _empty_like_state(ref, shape):
+    if _is_tensor(ref):
+        return torch.empty(shape, dtype=torch.complex128, device=ref.device)
+    return np.empty(shape, np.complex128)
+
+
+class ScatteringOperator:
+    """One dmd_scatt handle: k-pairs + P resident in HBM."""
+
+    def __init__(self, nk, nb, k1st, k2nd, e, nk_full=None, precision=_lib.DMD_PREC_FP64):
+        self.nk, self.nb = int(nk), int(nb)
+        self.nb_dm, self.b0 = self.nb, 0
+        k1 = np.ascontiguousarray(k1st, dtype=np.uint64)
+        k2 = np.ascontiguousarray(k2nd, dtype=np.uint64)
+        self.n_pairs = len(k1)
+        e = np.ascontiguousarray(e, dtype=np.float64).reshape(self.nk, self.nb)
+        h = ctypes.c_void_p()
+        call("dmd_scatt_create", self.nk, self.nb, self.n_pairs, k1.ctypes.data, k2.ctypes.data, e.ctypes.data,
+             float(nk_full if nk_full is not None else nk), int(precision), ctypes.byref(h))
+        self._h = h
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().dmd_scatt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- P ------------------------------------------------------------------------
+    def upload_P(self, P1, P2, scale=1.0):
+        p1, k1 = _ptr(P1, np.complex128)
+        p2, k2 = _ptr(P2, np.complex128)
+        call("dmd_scatt_upload_P", self._h, p1, p2, float(scale))
+
+    def add_P(self, P1, P2, scale=1.0):
+        p1, k1 = _ptr(P1, np.complex128)
+        p2, k2 = _ptr(P2, np.complex128)
+        call("dmd_scatt_add_P", self._h, p1, p2, float(scale))
+
+    def download_P(self, first=0, count=None):
+        count = self.n_pairs - first if count is None else count
+        n4 = self.nb ** 4
+        P1 = np.empty((count, n4), np.complex128)
+        P2 = np.empty((count, n4), np.complex128)
+        call("dmd_scatt_download_P", self._h, int(first), int(count), P1.ctypes.data, P2.ctypes.data)
+        return P1, P2
+
+    def generate_P(self, params):
+        call("dmd_scatt_generate_P", self._h, ctypes.byref(params))
+
+    # -- flags / eq ----------------------------------------------------------------
+    def set_flags(self, expt=True, schrodinger=False, kernel=_lib.DMD_KERNEL_AUTO):
+        call("dmd_scatt_set_flags", self._h, int(bool(expt)), int(bool(schrodinger)), int(kernel))
+
+    def set_ddmdt_eq(self, ddmdt_eq):
+        p, keep = _ptr(ddmdt_eq, np.complex128)
+        call("dmd_scatt_set_ddmdt_eq", self._h, p)
+
+    def compute_ddmdt_eq(self, f):
+        p, keep = _ptr(f, np.float64)
+        call("dmd_scatt_compute_ddmdt_eq", self._h, p)
+
+    # -- hot path ---------------------------------------------------------------
+    def eval(self, t, rho, rho1=None, out=None):
+        """dRho/dt|scatt at time t for rho (nk x nb^2), evolve_eph.cpp:34-166."""
+        shape = (self.nk, self.nb * self.nb)
+        pr, kr = _ptr(rho, np.complex128)
+        p1, k1 = _ptr(rho1, np.complex128)
+        if out is None:
+            out = _empty_like_state(rho, shape)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_eval", self._h, float(t), pr, p1, po)
+        if not _is_tensor(out) and ko is not out:
+            out[...] = ko.reshape(out.shape)
+        return out
+
+    def eval_batched(self, t, rho_B, out=None):
+        B = rho_B.shape[0]
+        if out is None:
+            out = _empty_like_state(rho_B, (B, self.nk, self.nb * self.nb))
+        pr, kr = _ptr(rho_B, np.complex128)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_eval_batched", self._h, float(t), int(B), pr, po)
+        return out
+
+    def compute_ddm(self, rho, rho1=None, ddm=None):
+        if ddm is None:
+            ddm = np.zeros((self.nk, self.nb * self.nb), np.complex128)
+        pr, kr = _ptr(rho, np.complex128)
+        p1, k1 = _ptr(rho1, np.complex128)
+        pd, kd = _ptr(ddm, np.complex128)
+        call("dmd_scatt_compute_ddm", self._h, pr, p1, pd)
+        if not _is_tensor(ddm) and kd is not ddm:
+            ddm[...] = kd.reshape(ddm.shape)
+        return ddm
+
+    # -- driver pieces --------------------------------------------------------------
+    def set_window(self, nb_dm, b0, e_dm):
+        p, keep = _ptr(e_dm, np.float64)
+        call("dmd_scatt_set_window", self._h, int(nb_dm), int(b0), p)
+        self.nb_dm, self.b0 = int(nb_dm), int(b0)
+
+    def evolve_driver(self, t, dm_expand, dm1_expand=None, out=None):
+        shape = (self.nk, self.nb_dm * self.nb_dm)
+        pr, kr = _ptr(dm_expand, np.complex128)
+        p1, k1 = _ptr(dm1_expand, np.complex128)
+        if out is None:
+            out = _empty_like_state(dm_expand, shape)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_evolve_driver", self._h, float(t), pr, p1, po)
+        return out
+
+    def set_coherent(self, H):
+        p, keep = _ptr(H, np.complex128)
+        call("dmd_scatt_set_coherent", self._h, p)
+
+    def set_observable(self, s, f, i0, i1, prefac):
+        ps, ks = _ptr(s, np.complex128)
+        pf, kf = _ptr(f, np.float64)
+        call("dmd_scatt_set_observable", self._h, ps, pf, int(i0), int(i1), float(prefac))
+
+    def measure(self, t, rho):
+        pr, kr = _ptr(rho, np.complex128)
+        v = ctypes.c_double()
+        call("dmd_scatt_measure", self._h, float(t), pr, ctypes.byref(v))
+        return v.value
+
+    def integrate_rk4(self, rho0, t0, dt, n_steps, report_every=0):
+        """Fixed-step RK4 on the device; returns (rho(t0 + n dt), observable samples)."""
+        rho = np.array(rho0, dtype=np.complex128, copy=True).reshape(self.nk, self.nb_dm * self.nb_dm) \
+            if not _is_tensor(rho0) else rho0.clone()
+        nobs = (n_steps // report_every + 1) if report_every > 0 else 0
+        obs = np.zeros(max(nobs, 1))
+        pr, kr = _ptr(rho, np.complex128)
+        call("dmd_scatt_integrate_rk4", self._h, float(t0), float(dt), int(n_steps), int(report_every), pr,
+             obs.ctypes.data)
+        return rho, obs[:nobs]
+
+    def integrate_rk4_batched(self, rho0_B, t0, dt, n_steps, report_every=0):
+        B = rho0_B.shape[0]
+        rho = np.array(rho0_B, dtype=np.complex128, copy=True).reshape(B, self.nk, self.nb_dm * self.nb_dm)
+        nobs = (n_steps // report_every + 1) if report_every > 0 else 0
+        obs = np.zeros((B, max(nobs, 1)))
+        call("dmd_scatt_integrate_rk4_batched", self._h, int(B), float(t0), float(dt), int(n_steps),
+             int(report_every), rho.ctypes.data, obs.ctypes.data)
+        return rho, obs[:, :nobs]
+
+    # -- introspection --------------------------------------------------------------
+    def info(self):
+        n = ctypes.c_int64()
+        b = ctypes.c_int64()
+        ms = ctypes.c_double()
+        call("dmd_scatt_info", self._h, ctypes.byref(n), ctypes.byref(b), ctypes.byref(ms))
+        return {"n_pairs": n.value, "p_bytes": b.value, "last_kernel_ms": ms.value}
+
+    def time_eval(self, t, n):
+        k = ctypes.c_double()
+        e = ctypes.c_double()
+        call("dmd_scatt_time_eval", self._h, float(t), int(n), ctypes.byref(k), ctypes.byref(e))
+        return k.value, e.value
+
+
+class ElectronPhonon(ScatteringOperator):
+    """Drop-in shaped like the reference's electronphonon for the driver.
+
+    evolve(t, dm, dm1)          -> ddmdt        (evolve_eph.cpp:34-166)
+    evolve_driver(t, dm, dm1)   -> ddmdt_expand (evolve_eph.cpp:21-32)
+    compute_ddmdt_eq(f)                          (evolve_eph.cpp:3-19)
+    set_eph(P1, P2, scale_eph)                   (ElectronPhonon.cpp:209-233)
+    add_scatt_contrib(P1, P2, factor)            (ElectronPhonon.cpp:44-113, "eimp")
+    """
+
+    def __init__(self, nk, nb, k1st, k2nd, e, nk_full, alg_expt=True, alg_ddmdteq=False,
+                 picture="interaction", precision=_lib.DMD_PREC_FP64):
+        super().__init__(nk, nb, k1st, k2nd, e, nk_full, precision)
+        self.prefac_eph = 2 * np.pi / nk_full
+        self.alg_ddmdteq = alg_ddmdteq
+        self.set_flags(expt=alg_expt, schrodinger=(picture == "schrodinger"))
+
+    def set_eph(self, P1, P2, scale_eph=1.0):
+        self.upload_P(P1, P2, scale_eph)
+
+    def add_scatt_contrib(self, P1add, P2add, factor):
+        self.add_P(P1add, P2add, factor)
+
+    def evolve(self, t, dm, dm1=None):
+        return self.eval(t, dm, dm1)
+
+
+def synth_params(preset):
+    """dmd_synth_params of a named workload preset (0 tiny, 1..5 benchmark shapes)."""
+    p = SynthParams()
+    call("dmd_synth_preset", int(preset), ctypes.byref(p))
+    return p
+
+
+def synth_energies(p):
+    e = np.empty((p.nk, p.nb), np.float64)
+    call("dmd_synth_energies", ctypes.byref(p), e.ctypes.data)
+    return e
+
+
+def synth_kpairs(p):
+    n = ctypes.c_int64(0)
+    call("dmd_synth_kpairs", ctypes.byref(p), None, None, ctypes.byref(n))
+    k1 = np.empty(n.value, np.uint64)
+    k2 = np.empty(n.value, np.uint64)
+    call("dmd_synth_kpairs", ctypes.byref(p), k1.ctypes.data, k2.ctypes.data, ctypes.byref(n))
+    return k1, k2
+
+
+def synth_P(p, e, k1, k2, nthreads=0):
+    n = len(k1)
+    n4 = p.nb ** 4
+    P1 = np.empty((n, n4), np.complex128)
+    P2 = np.empty((n, n4), np.complex128)
+    e = np.ascontiguousarray(e, np.float64)
+    k1 = np.ascontiguousarray(k1, np.uint64)
+    k2 = np.ascontiguousarray(k2, np.uint64)
+    call("dmd_synth_P", ctypes.byref(p), e.ctypes.data, n, k1.ctypes.data, k2.ctypes.data, P1.ctypes.data,
+         P2.ctypes.data, int(nthreads))
+    return P1, P2
+
+
+def synth_couplings(p, k, kp):
+    n2 = p.nb * p.nb
+    omega = np.empty(p.n_modes)
+    gm = np.empty((p.n_modes, n2), np.complex128)
+    gp = np.empty((p.n_modes, n2), np.complex128)
+    call("dmd_synth_couplings", ctypes.byref(p), int(k), int(kp), omega.ctypes.data, gm.ctypes.data, gp.ctypes.data)
+    return omega, gm, gp
+
+
+def synth_spin(p):
+    s3 = np.empty((p.nk, 3, p.nb * p.nb), np.complex128)
+    call("dmd_synth_spin", ctypes.byref(p), s3.ctypes.data)
+    return s3
+
+
+def write_ldbd(dirname, p, nthreads=0):
+    call("dmd_ldbd_write_synth", str(dirname).encode(), ctypes.byref(p), int(nthreads))

@@ -1,0 +1,194 @@
+/*
+ * dmd_scatt.h -- C-ABI of the MI355X Lindblad scattering operator for DMD.
+ *
+ * Drop-in for the reference's scattering term dRho/dt|scatt
+ * (electronphonon::evolve_driver / evolve, src_v4.5.7/evolve_eph.cpp:21-166,
+ * and the initializer's per-pair ElectronPhonon::compute_ddm,
+ * src_FeynWann_forDMD/lindbladInit_for-DMD-4.5.6/ElectronPhonon.h:855-862),
+ * with the e-i term of the same form folded into P
+ * (src_v4.5.7/ElectronPhonon.cpp:44-113).
+ *
+ * Conventions
+ *   complex arrays  interleaved (re, im) float64, i.e. the reference's
+ *                   struct complex {double x, y} (src_v4.5.7/comm/scalar.h:67-69);
+ *   2-D arrays      flat row-major pools, dm[0] of the reference's row-pointer
+ *                   tables (src_v4.5.7/comm/myarray.cpp:145-155);
+ *   k-pairs         stored pairs k1st[i] <= k2nd[i] as in
+ *                   ldbd_kpair_k1st/k2nd.bin (ElectronPhonon.cpp:182-204);
+ *   P1, P2          n_pairs x nb^4 complex, element [(i1*nb+i2)*nb^2 + i3*nb+i4]
+ *                   (ldbd_P1/P2_lindblad.bin, ElectronPhonon.cpp:209-233);
+ *   time            Hartree atomic units (src_v4.5.7/comm/Units.h:62).
+ * Pointers to state arrays (rho, ddmdt, ...) may be host or device memory;
+ * the library detects which.  Every call returns 0 on success and a negative
+ * DMD_ERR_* code otherwise; dmd_scatt_last_error() gives the message (thread
+ * local).  One host thread per handle; calls are ordered on the handle's
+ * HIP stream and return after the results are in the caller's buffers.
+ */
+#ifndef DMD_SCATT_H
+#define DMD_SCATT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMD_OK 0
+#define DMD_ERR_ARG (-1)
+#define DMD_ERR_HIP (-2)
+#define DMD_ERR_STATE (-3)
+#define DMD_ERR_UNSUPPORTED (-4)
+#define DMD_ERR_IO (-5)
+#define DMD_ERR_NOMEM (-6)
+
+/* precision of the streamed P */
+#define DMD_PREC_FP64 0 /* complex128 P, FP64 arithmetic: the drop-in           */
+#define DMD_PREC_C64 1  /* complex64 P, FP32 products, FP64 row sums and state  */
+#define DMD_PREC_BF16 2 /* bf16 re/im P with a per-pair FP32 scale              */
+
+/* contraction kernel */
+#define DMD_KERNEL_AUTO 0   /* fastest available for the shape                  */
+#define DMD_KERNEL_ROWS 1   /* deterministic: every row gathers its own pairs     */
+#define DMD_KERNEL_TILES 2  /* single P read: 2-D tiles with LDS accumulation   */
+
+typedef struct dmd_scatt dmd_scatt;
+
+/* Synthetic ldbd_data description (energies in Hartree).  See dmd_synth_*. */
+typedef struct {
+    uint64_t seed;
+    int nb;               /* bands in the e-ph window                          */
+    int n_modes;          /* phonon branches                                   */
+    int64_t nk;           /* k points                                          */
+    int64_t partners;     /* partners drawn per k (uniform mode)               */
+    int64_t partners_min; /* power-law mode bounds; partners_max = 0: uniform   */
+    int64_t partners_max;
+    double powerlaw_alpha;/* pdf ~ c^-alpha on [partners_min, partners_max]    */
+    double e_max;         /* band-edge energies E_k ~ U(0, e_max) above CBM     */
+    double split_max;     /* Kramers splitting ~ U(0, split_max)               */
+    double gap_min, gap_max; /* second Kramers pair offset (nb = 4)            */
+    double omega_min, omega_max; /* phonon energies per branch ~ U(.,.)         */
+    double sigma_g;       /* e-ph coupling std dev per component                */
+    double degauss;       /* gaussian smearing of the energy delta             */
+    double temperature;   /* k_B T                                             */
+    double sigma_imp;     /* e-i coupling std dev per component                */
+    double imp_fraction;  /* |n_i V_cell| * scale_ei; 0 disables the e-i term   */
+} dmd_synth_params;
+
+/* ---- errors --------------------------------------------------------------- */
+const char *dmd_scatt_last_error(void);
+int dmd_scatt_version(void);
+
+/* ---- operator lifetime ------------------------------------------------------ */
+/* Replaces electronphonon's constructor + alloc_ephmat + read_ldbd_kpair
+ * (src_v4.5.7/ElectronPhonon.h:53-93, ElectronPhonon.cpp:182-204,255-304):
+ * takes the stored k-pair lists and band energies e (nk x nb, the e-ph window
+ * of e_dm, ElectronPhonon.h:84), builds the CSR structure-of-arrays layout
+ * grouped by k and allocates HBM.  prefac = 2*pi/nk_full (ElectronPhonon.h:61). */
+int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint64_t *k1st, const uint64_t *k2nd,
+                     const double *e, double nk_full, int precision, dmd_scatt **out);
+int dmd_scatt_destroy(dmd_scatt *h);
+
+/* P = scale * (P1, P2), in ldbd_P file order; replaces read_ldbd_eph
+ * (ElectronPhonon.cpp:209-233, scale = scale_eph). */
+int dmd_scatt_upload_P(dmd_scatt *h, const double *P1, const double *P2, double scale);
+/* P += scale * (P1, P2); the e-i fold-in add_scatt_contrib("eimp")
+ * (ElectronPhonon.cpp:87-97; ab_neutral scale = |n_i V_cell| * scale_ei,
+ * ElectronImpurity.cpp:38-39 and ElectronImpurity.h:17-18). */
+int dmd_scatt_add_P(dmd_scatt *h, const double *P1, const double *P2, double scale);
+/* P of stored pairs [first, first+count) in file order, as complex128. */
+int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2);
+/* Fills P on the device from the synthetic description (the handle's e must
+ * be the one dmd_synth_energies gives for p); no host traffic. */
+int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p);
+
+/* alg.expt (interaction-picture phases on P, evolve_eph.cpp:91-98),
+ * alg.picture == "schrodinger" (ddmdt_eq subtracted without phase,
+ * evolve_eph.cpp:158; -i[e, rho] joins the integrator's right-hand side,
+ * DenMat.cpp:16-24,45-46) and the contraction kernel (DMD_KERNEL_*).
+ * Defaults: expt 1, interaction picture, DMD_KERNEL_AUTO. */
+int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int kernel);
+/* ddmdt_eq subtracted after the assembly (alg.ddmdteq, evolve_eph.cpp:154-162);
+ * NULL clears it.  nk x nb^2 complex. */
+int dmd_scatt_set_ddmdt_eq(dmd_scatt *h, const double *ddmdt_eq);
+/* compute_ddmdt_eq (evolve_eph.cpp:3-19): ddmdt_eq = evolve(t=0, diag f,
+ * diag 1-f) and stores it.  f: nk x nb occupations of the window. */
+int dmd_scatt_compute_ddmdt_eq(dmd_scatt *h, const double *f);
+
+/* ---- the hot path --------------------------------------------------------- */
+/* electronphonon::evolve (evolve_eph.cpp:34-166): ddmdt (nk x nb^2, fully
+ * overwritten) from rho (nk x nb^2) at time t.  one_minus_rho may be NULL,
+ * then 1-rho is formed as set_oneminusdm does (DenMat.cpp:249-261). */
+int dmd_scatt_eval(dmd_scatt *h, double t, const double *rho, const double *one_minus_rho, double *ddmdt);
+/* B independent states sharing one pass over P: rho_B, ddmdt_B are
+ * B x nk x nb^2 (one_minus_rho formed internally). */
+int dmd_scatt_eval_batched(dmd_scatt *h, double t, int B, const double *rho_B, double *ddmdt_B);
+/* The initializer's one-step form over all stored pairs
+ * (lindbladInit.../ElectronPhonon.h:774-862 without the file reads): ddm +=
+ * sum of per-pair compute_ddm, no phases.  Same stream as eval at t=0 with
+ * expt off; ddm accumulated. */
+int dmd_scatt_compute_ddm(dmd_scatt *h, const double *rho, const double *one_minus_rho, double *ddm);
+
+/* ---- driver-side pieces (DenMat, observables, integrator) -------------------- */
+/* Band window of evolve_driver (evolve_eph.cpp:21-32): the dynamics state is
+ * nb_dm x nb_dm per k, the scattering acts on rows/cols [b0, b0+nb).  e_dm is
+ * nk x nb_dm.  Default: nb_dm = nb, b0 = 0, e_dm = e. */
+int dmd_scatt_set_window(dmd_scatt *h, int nb_dm, int b0, const double *e_dm);
+/* evolve_driver itself: cut, evolve, expand (nk x nb_dm^2 arrays). */
+int dmd_scatt_evolve_driver(dmd_scatt *h, double t, const double *dm_expand, const double *dm1_expand,
+                            double *ddmdt_expand);
+/* Coherent Hamiltonian H (nk x nb_dm^2, the H_BS(+H_Ez) of init_Hcoh,
+ * DenMat.cpp:7-28); -i[H(t), rho] joins the right-hand side (DenMat.cpp:29-55).
+ * NULL removes it. */
+int dmd_scatt_set_coherent(dmd_scatt *h, const double *H);
+/* Observable of ob_1dmk (observable.h:301-366): S(t) = prefac * sum_k
+ * sum_{i in [i0,i1)} sum_b Re(s_t[i,b] (rho[b,i] - f_i delta_bi)).
+ * s: nk x nb_dm^2 (one component of ldbd_smat.bin), f: nk x nb_dm. */
+int dmd_scatt_set_observable(dmd_scatt *h, const double *s, const double *f, int i0, int i1, double prefac);
+int dmd_scatt_measure(dmd_scatt *h, double t, const double *rho, double *value);
+/* On-device fixed-step RK4 over compute() (DensityMatrix_Dynamics.h:243-278:
+ * coherent + scattering), with re-Hermitization after every step, replacing
+ * the GSL loop of evolve_gsl (:189-225).  rho_inout: nk x nb_dm^2 at t0, state
+ * at t0 + n_steps*dt on return.  If report_every > 0 and an observable is set,
+ * obs_out receives S at steps 0, report_every, 2*report_every, ...
+ * (n_steps/report_every + 1 values). */
+int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps, int report_every,
+                            double *rho_inout, double *obs_out);
+/* B trajectories in one pass over P per stage (rho_inout: B x nk x nb_dm^2,
+ * obs_out: B x (n_steps/report_every + 1)). */
+int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
+                                    int report_every, double *rho_inout, double *obs_out);
+
+/* ---- introspection ---------------------------------------------------------- */
+/* Bytes of P streamed per eval and stored pairs; the last kernel time in ms
+ * measured with HIP events on the handle's stream (0 if none). */
+int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, double *last_kernel_ms);
+/* Times n evals of the contraction at time t on the current device state
+ * (rho from the last eval); returns the mean ms per eval of the dominant
+ * kernel and of the whole eval, measured with HIP events on the handle's stream. */
+int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel_ms, double *eval_ms);
+
+/* ---- synthetic ldbd_data ------------------------------------------------------- */
+/* Presets: 1..5 = the five benchmark workloads (see the package docs); 0 = a
+ * tiny case.  Returns DMD_ERR_ARG for an unknown preset. */
+int dmd_synth_preset(int which, dmd_synth_params *p);
+int dmd_synth_energies(const dmd_synth_params *p, double *e /* nk x nb */);
+/* Stored pairs: pass k1st = k2nd = NULL to get the count in *n_pairs. */
+int dmd_synth_kpairs(const dmd_synth_params *p, uint64_t *k1st, uint64_t *k2nd, int64_t *n_pairs);
+/* P1/P2 of the given pairs on the host (nthreads <= 0: all cores). */
+int dmd_synth_P(const dmd_synth_params *p, const double *e, int64_t n_pairs, const uint64_t *k1st,
+                const uint64_t *k2nd, double *P1, double *P2, int nthreads);
+/* Couplings of one stored pair: omega (n_modes), gm, gp (n_modes x nb^2). */
+int dmd_synth_couplings(const dmd_synth_params *p, uint64_t k, uint64_t kp, double *omega, double *gm,
+                        double *gp);
+/* Random Hermitian spin matrices, nk x 3 x nb^2 (ldbd_smat.bin layout). */
+int dmd_synth_spin(const dmd_synth_params *p, double *s3);
+
+/* ---- ldbd_data files (reference formats) ---------------------------------------- */
+/* Writes ldbd_size.dat, ldbd_ek.bin, ldbd_kvec.bin, ldbd_smat.bin,
+ * ldbd_kpair_k1st/k2nd.bin and ldbd_P1/P2_lindblad.bin into dir
+ * (layouts of lindbladInit.../electron_save.cpp:16-65 and ElectronPhonon.h:252-284). */
+int dmd_ldbd_write_synth(const char *dir, const dmd_synth_params *p, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

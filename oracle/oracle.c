@@ -1,0 +1,589 @@
+/*
+ * oracle.c -- CPU restatement of the DMD Lindblad scattering hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke()
+ * entry and the cpu_baseline leg of bench.py load it (as oracle/liboracle.so)
+ * and compare the HIP library against it.  Nothing in the product library
+ * links, loads or calls it, and it is never the thing that is measured as the
+ * GPU result.
+ *
+ * Every routine follows one reference routine (paths relative to the reference
+ * root) in loop order and operation order, with the reference's interleaved
+ * complex128 layout (struct {double x, y}, src_v4.5.7/comm/scalar.h:67-106) and
+ * flat row-major 2-D pools (src_v4.5.7/comm/myarray.cpp:145-155).
+ *
+ * Pinning (what checks each routine; the tests hold them):
+ *   oracle_compute_Pt      t=0 identity, factorised-phase identity, periodicity.
+ *   oracle_evolve          brute-force Eq. 9 over an explicit both-direction
+ *                          pair list (tests/test_oracle.py), Hermiticity,
+ *                          particle conservation, rho=0 / rho=I zeros,
+ *                          interaction vs Schroedinger picture, per-pair
+ *                          initializer form (oracle_compute_ddm_pairs).
+ *   oracle_compute_ddm_pairs  equality with oracle_evolve at t=0 (linearity).
+ *   oracle_compute_P       Gram-form positivity, the P1(k,k')/P2(k',k)
+ *                          conjugate identity, ImSigma from the P diagonal
+ *                          against a direct golden-rule sum.
+ *   oracle_coherent        two-level precession closed form, tracelessness.
+ *   oracle_rk4             1+z+z^2/2+z^3/6+z^4/24 amplification, O(h^4).
+ *   oracle_spin            brute force, zero at rho=f, linearity, and the
+ *                          reference's own GaAs restart/sz_elec_tot.out pair.
+ *   oracle_dm_Bpert_1st    closed form of the first-order perturbation,
+ *                          degenerate limit, Hermiticity.
+ * The bit-level rounding of the reference's CBLAS zgemm/zhemm (GSL 2.6 or MKL)
+ * and libm sincos is not reproduced bit for bit ("parity unpinned" at the last
+ * ulp); summation orders follow the reference's loops so results agree to a
+ * few ulp.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double x, y; } zc;
+
+static inline zc zmk(double x, double y) { zc r; r.x = x; r.y = y; return r; }
+/* complex product as in src_v4.5.7/comm/scalar.h:89 */
+static inline zc zmul(zc a, zc b) { return zmk(a.x * b.x - a.y * b.y, a.y * b.x + a.x * b.y); }
+static inline zc zadd(zc a, zc b) { return zmk(a.x + b.x, a.y + b.y); }
+static inline zc zsub(zc a, zc b) { return zmk(a.x - b.x, a.y - b.y); }
+static inline zc zconj(zc a) { return zmk(a.x, -a.y); }
+static inline zc zscale(zc a, double s) { return zmk(a.x * s, a.y * s); }
+/* cis(x) = cos x + i sin x, src_v4.5.7/comm/scalar.h:122-125 */
+static inline zc zcis(double x) { return zmk(cos(x), sin(x)); }
+
+#define Z(p, i) (((const zc *)(p))[i])
+#define ZW(p, i) (((zc *)(p))[i])
+
+/* ------------------------------------------------------------------------ */
+/* compute_Pt: interaction-picture P, src_v4.5.7/evolve_eph.cpp:169-187.     */
+/* P1t[(i1 i2),(i3 i4)] = P1 * cis((ek[i1] - ekp[i3] - ek[i2] + ekp[i4]) t)  */
+/* P2t (minus)          = P2 * cis((ek[i3] - ekp[i1] - ek[i4] + ekp[i2]) t)  */
+/* ------------------------------------------------------------------------ */
+void oracle_compute_Pt(int nb, double t, const double *ek, const double *ekp,
+                       const double *P, double *Pt, int minus)
+{
+    for (int i1 = 0; i1 < nb; i1++) {
+        int n1 = i1 * nb;
+        for (int i2 = 0; i2 < nb; i2++) {
+            int n12 = (n1 + i2) * nb;
+            for (int i3 = 0; i3 < nb; i3++) {
+                int n123 = (n12 + i3) * nb;
+                for (int i4 = 0; i4 < nb; i4++) {
+                    double arg = !minus ? (ek[i1] - ekp[i3] - ek[i2] + ekp[i4]) * t
+                                        : (ek[i3] - ekp[i1] - ek[i4] + ekp[i2]) * t;
+                    ZW(Pt, n123 + i4) = zmul(Z(P, n123 + i4), zcis(arg));
+                }
+            }
+        }
+    }
+}
+
+/* C(m x 1) += A(m x m) * b, the zgemm_interface call at evolve_eph.cpp:99
+ * (src_v4.5.7/comm/mymatrix.cpp:82-86).  Row-major; the k loop is the outer
+ * accumulation order of a reference CBLAS zgemm (C += A[i,k]*B[k,j]). */
+static void gemv_Ab(zc *C, const zc *A, const zc *b, int m)
+{
+    for (int i = 0; i < m; i++)
+        for (int k = 0; k < m; k++)
+            C[i] = zadd(C[i], zmul(A[i * m + k], b[k]));
+}
+/* C(1 x m) += a(1 x m) * B(m x m), the call at evolve_eph.cpp:100. */
+static void gemv_aB(zc *C, const zc *a, const zc *B, int m)
+{
+    for (int k = 0; k < m; k++) {
+        zc ak = a[k];
+        for (int j = 0; j < m; j++)
+            C[j] = zadd(C[j], zmul(ak, B[k * m + j]));
+    }
+}
+
+/* Element of a Hermitian matrix stored in its upper triangle, as CBLAS zhemm
+ * with CblasUpper reads it: A[i][j] for i<=j, conj(A[j][i]) below, and only
+ * the real part on the diagonal. */
+static inline zc herm_upper(const zc *A, int n, int i, int j)
+{
+    if (i < j) return A[i * n + j];
+    if (i > j) return zconj(A[j * n + i]);
+    return zmk(A[i * n + i].x, 0.0);
+}
+
+/* zhemm_interface(C, left, A, B, n, alpha, beta), src_v4.5.7/comm/mymatrix.cpp:73-76:
+ * left:  C = alpha*A*B + beta*C;  right: C = alpha*B*A + beta*C; A Hermitian
+ * (upper triangle), row-major n x n. */
+static void zhemm_upper(zc *C, int left, const zc *A, const zc *B, int n, zc alpha, zc beta)
+{
+    zc tmp[64];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            zc s = zmk(0, 0);
+            for (int k = 0; k < n; k++) {
+                if (left) s = zadd(s, zmul(herm_upper(A, n, i, k), B[k * n + j]));
+                else      s = zadd(s, zmul(B[i * n + k], herm_upper(A, n, k, j)));
+            }
+            tmp[i * n + j] = s;
+        }
+    /* BLAS semantics: with beta == 0, C is not read on input */
+    int beta0 = (beta.x == 0.0 && beta.y == 0.0);
+    for (int i = 0; i < n * n; i++)
+        C[i] = beta0 ? zmul(alpha, tmp[i]) : zadd(zmul(alpha, tmp[i]), zmul(beta, C[i]));
+}
+
+/* The per-pair accumulation body of electronphonon::evolve for pairs
+ * [p0, p1): src_v4.5.7/evolve_eph.cpp:49-106 (dense summode branch, the one the
+ * reference's default flags select: alg.summode=1, alg.sparseP=0). */
+static void evolve_pairs(int64_t p0, int64_t p1, int nb, const uint64_t *k1, const uint64_t *k2,
+                         const double *P1, const double *P2, const double *e,
+                         const double *dm, const double *dm1, double t, int expt,
+                         zc *Pdm, zc *dm1P)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    zc *P1t = (zc *)malloc(sizeof(zc) * nb4), *P2t = (zc *)malloc(sizeof(zc) * nb4);
+    zc *P1n = (zc *)malloc(sizeof(zc) * nb4), *P2n = (zc *)malloc(sizeof(zc) * nb4);
+    for (int64_t ip = p0; ip < p1; ip++) {
+        int64_t ik = (int64_t)k1[ip], ikp = (int64_t)k2[ip];
+        const double *p1 = P1 + (size_t)ip * nb4 * 2, *p2 = P2 + (size_t)ip * nb4 * 2;
+        if (!expt) {                                   /* evolve_eph.cpp:91-94 */
+            memcpy(P1t, p1, sizeof(zc) * nb4);
+            memcpy(P2t, p2, sizeof(zc) * nb4);
+        } else {                                       /* evolve_eph.cpp:95-98 */
+            oracle_compute_Pt(nb, t, e + ik * nb, e + ikp * nb, p1, (double *)P1t, 0);
+            oracle_compute_Pt(nb, t, e + ik * nb, e + ikp * nb, p2, (double *)P2t, 1);
+        }
+        gemv_Ab(Pdm + ik * nb2, P1t, (const zc *)dm + ikp * nb2, nb2);    /* :99  */
+        gemv_aB(dm1P + ik * nb2, (const zc *)dm1 + ikp * nb2, P2t, nb2);  /* :100 */
+        if (ik < ikp) {                                                    /* :101 */
+            for (int i = 0; i < nb4; i++) { P2n[i] = zconj(P1t[i]); P1n[i] = zconj(P2t[i]); }
+            gemv_Ab(Pdm + ikp * nb2, P1n, (const zc *)dm + ik * nb2, nb2);   /* :104 */
+            gemv_aB(dm1P + ikp * nb2, (const zc *)dm1 + ik * nb2, P2n, nb2); /* :105 */
+        }
+    }
+    free(P1t); free(P2t); free(P1n); free(P2n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* electronphonon::evolve, src_v4.5.7/evolve_eph.cpp:34-166.                 */
+/*   Pdm_k  = sum_k' P1t_kk' . vec(rho_k')      (+ reverse from stored pairs) */
+/*   dm1P_k = sum_k' vec(1-rho_k')^T . P2t_kk'                                */
+/*   C_k = (1-rho_k) mat(Pdm_k) - mat(dm1P_k) rho_k   (zhemm, :140-143)       */
+/*   ddmdt_k = prefac * 0.5 * (C_k + C_k^dagger)      (:149-151)              */
+/*   optional ddmdteq subtraction (:154-162).                                 */
+/* nthreads <= 1 reproduces the reference loop order on one rank.  With      */
+/* nthreads > 1 the stored pairs are block-partitioned over threads like     */
+/* MPI ranks (src_v4.5.7/comm/mymp.cpp:8-21), each thread keeps private      */
+/* Pdm/dm1P and the partials are summed in thread order (the allreduce at    */
+/* :137 without its per-row collectives).                                    */
+/* ------------------------------------------------------------------------ */
+int oracle_evolve(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                  const double *P1, const double *P2, const double *e,
+                  const double *dm, const double *dm1, double t, double prefac, int expt,
+                  const double *ddmdt_eq, int schrodinger, int nthreads, double *ddmdt)
+{
+    int nb2 = nb * nb;
+    if (nb < 1 || nb > 8 || nk < 0 || npairs < 0) return -1;
+    for (int64_t ip = 0; ip < npairs; ip++)
+        if (k1[ip] > k2[ip] || (int64_t)k2[ip] >= nk) return -2;
+    size_t nrow = (size_t)nk * nb2;
+    zc *Pdm = (zc *)calloc(nrow, sizeof(zc)), *dm1P = (zc *)calloc(nrow, sizeof(zc));
+    if (!Pdm || !dm1P) { free(Pdm); free(dm1P); return -3; }
+    if (nthreads <= 1) {
+        evolve_pairs(0, npairs, nb, k1, k2, P1, P2, e, dm, dm1, t, expt, Pdm, dm1P);
+    } else {
+        zc **pPdm = (zc **)calloc(nthreads, sizeof(zc *)), **pdm1P = (zc **)calloc(nthreads, sizeof(zc *));
+        int fail = 0;
+        for (int it = 0; it < nthreads; it++) {
+            pPdm[it] = (zc *)calloc(nrow, sizeof(zc));
+            pdm1P[it] = (zc *)calloc(nrow, sizeof(zc));
+            if (!pPdm[it] || !pdm1P[it]) fail = 1;
+        }
+        if (!fail) {
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+            for (int it = 0; it < nthreads; it++) {
+                int64_t p0 = npairs * it / nthreads, p1 = npairs * (it + 1) / nthreads;
+                evolve_pairs(p0, p1, nb, k1, k2, P1, P2, e, dm, dm1, t, expt, pPdm[it], pdm1P[it]);
+            }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+            for (int64_t i = 0; i < (int64_t)nrow; i++)
+                for (int it = 0; it < nthreads; it++) {
+                    Pdm[i] = zadd(Pdm[i], pPdm[it][i]);
+                    dm1P[i] = zadd(dm1P[i], pdm1P[it][i]);
+                }
+        }
+        for (int it = 0; it < nthreads; it++) { free(pPdm[it]); free(pdm1P[it]); }
+        free(pPdm); free(pdm1P);
+        if (fail) { free(Pdm); free(dm1P); return -3; }
+    }
+    /* final assembly, evolve_eph.cpp:140-152 */
+    const zc c1 = zmk(1, 0), cm1 = zmk(-1, 0), c0 = zmk(0, 0);
+    for (int64_t ik = 0; ik < nk; ik++) {
+        zc contrib[64];
+        for (int i = 0; i < nb2; i++) contrib[i] = c0;
+        zhemm_upper(contrib, 1, (const zc *)dm1 + ik * nb2, Pdm + ik * nb2, nb, c1, c0);
+        zhemm_upper(contrib, 0, (const zc *)dm + ik * nb2, dm1P + ik * nb2, nb, cm1, c1);
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++)
+                ZW(ddmdt, ik * nb2 + i * nb + j) =
+                    zscale(zadd(contrib[i * nb + j], zconj(contrib[j * nb + i])), prefac * 0.5);
+    }
+    if (ddmdt_eq) {                                    /* evolve_eph.cpp:154-162 */
+        for (int64_t ik = 0; ik < nk; ik++)
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    size_t o = ik * nb2 + i * nb + j;
+                    if (i == j || schrodinger)
+                        ZW(ddmdt, o) = zsub(Z(ddmdt, o), Z(ddmdt_eq, o));
+                    else
+                        ZW(ddmdt, o) = zsub(Z(ddmdt, o), zmul(Z(ddmdt_eq, o), zcis((e[ik * nb + i] - e[ik * nb + j]) * t)));
+                }
+    }
+    free(Pdm); free(dm1P);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initializer one-step form: ElectronPhonon::compute_ddm and its caller's   */
+/* pair loop, lindbladInit_for-DMD-4.5.6/ElectronPhonon.h:755-773 (term1_P,  */
+/* term2_P), :815-837 (reverse with P1_next=conj(P2), P2_next=conj(P1)) and  */
+/* :855-862 (per-pair Hermitian assembly).  The e-i twin                     */
+/* (ElectronImpurity.h:663-771) is the same with P scaled by the defect      */
+/* fraction.  ddm is accumulated (+=), as in the reference.                  */
+/* ------------------------------------------------------------------------ */
+static void compute_ddm_one(int nb, const zc *dmk, const zc *dmkp, const zc *dm1k, const zc *dm1kp,
+                            const zc *p1, const zc *p2, double prefac, zc *ddmk)
+{
+    int nb2 = nb * nb;
+    zc maux1[64] = {{0, 0}}, maux2[64] = {{0, 0}}, contrib[64];
+    const zc c1 = zmk(1, 0), c0 = zmk(0, 0);
+    for (int i = 0; i < nb2; i++) contrib[i] = c0;
+    /* term1_P: + (1-dm_k) . mat(P1 . vec dm_k') */
+    for (int i = 0; i < nb2; i++) maux1[i] = c0;
+    gemv_Ab(maux1, p1, dmkp, nb2);
+    zhemm_upper(maux2, 1, dm1k, maux1, nb, c1, c0);
+    for (int i = 0; i < nb2; i++) contrib[i] = zadd(contrib[i], maux2[i]);
+    /* term2_P: - mat(vec(1-dm_k')^T . P2) . dm_k */
+    for (int i = 0; i < nb2; i++) maux1[i] = c0;
+    gemv_aB(maux1, dm1kp, p2, nb2);
+    zhemm_upper(maux2, 0, dmk, maux1, nb, c1, c0);
+    for (int i = 0; i < nb2; i++) contrib[i] = zsub(contrib[i], maux2[i]);
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++)
+            ddmk[i * nb + j] = zadd(ddmk[i * nb + j],
+                                    zscale(zadd(contrib[i * nb + j], zconj(contrib[j * nb + i])), prefac * 0.5));
+}
+
+int oracle_compute_ddm_pairs(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                             const double *P1, const double *P2, const double *dm, const double *dm1,
+                             double prefac, double *ddm)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 8) return -1;
+    zc *P1n = (zc *)malloc(sizeof(zc) * nb4), *P2n = (zc *)malloc(sizeof(zc) * nb4);
+    for (int64_t ip = 0; ip < npairs; ip++) {
+        int64_t ik = (int64_t)k1[ip], ikp = (int64_t)k2[ip];
+        if (ik > ikp || ikp >= nk) { free(P1n); free(P2n); return -2; }
+        const zc *p1 = (const zc *)P1 + (size_t)ip * nb4, *p2 = (const zc *)P2 + (size_t)ip * nb4;
+        compute_ddm_one(nb, (const zc *)dm + ik * nb2, (const zc *)dm + ikp * nb2,
+                        (const zc *)dm1 + ik * nb2, (const zc *)dm1 + ikp * nb2, p1, p2, prefac,
+                        (zc *)ddm + ik * nb2);
+        if (ik < ikp) {
+            for (int i = 0; i < nb4; i++) { P1n[i] = zconj(p2[i]); P2n[i] = zconj(p1[i]); }
+            compute_ddm_one(nb, (const zc *)dm + ikp * nb2, (const zc *)dm + ik * nb2,
+                            (const zc *)dm1 + ikp * nb2, (const zc *)dm1 + ik * nb2, P1n, P2n, prefac,
+                            (zc *)ddm + ikp * nb2);
+        }
+    }
+    free(P1n); free(P2n);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* P build of the initializer, lindbladInit_for-DMD-4.5.6/ElectronPhonon.h:  */
+/* 432-575 (lindblad=true, gaussian smearing, no energy-window cut, optional */
+/* detailed balance).  Inputs per branch a: omega[a] (= wqp = wqm), gm[a]    */
+/* (nb x nb, g^-_kk') and gp[a] (nb x nb, g^+_kk' = dagger(g^-_k'k)), all    */
+/* interleaved complex.  P1/P2 are overwritten.                              */
+/* ------------------------------------------------------------------------ */
+static double bose(double x) { return 1.0 / (exp(x) - 1.0); }
+
+int oracle_compute_P(int nb, const double *Ek, const double *Ekp, int nm, const double *omega,
+                     const double *gm, const double *gp, double T, double sigma, int detail_balance,
+                     double *P1, double *P2)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 8) return -1;
+    zc App[64], Amm[64], Amp[64], Apm[64], A2pp[64], A2mm[64], A2mp[64], A2pm[64];
+    for (int i = 0; i < nb4; i++) { ZW(P1, i) = zmk(0, 0); ZW(P2, i) = zmk(0, 0); }
+    double sigma2 = sigma * sigma;
+    double prefac_exp = -0.5 / sigma2;
+    double prefac_delta = 1. / (sigma * sqrt(2. * M_PI));
+    double prefac_sqrtdelta = sqrt(prefac_delta);
+    for (int a = 0; a < nm; a++) {
+        double wqm = omega[a], wqp = omega[a];
+        double nqm = bose(fmax(1e-3, wqm / T)), nqp = bose(fmax(1e-3, wqp / T));
+        const zc *gpa = (const zc *)gp + a * nb2, *gma = (const zc *)gm + a * nb2;
+        for (int b1 = 0; b1 < nb; b1++)
+            for (int b2 = 0; b2 < nb; b2++) {
+                double dE = Ek[b1] - Ekp[b2];
+                /* phonon emission, :475-487 and :497-503 */
+                double deltaplus = exp(prefac_exp * pow(dE + wqp, 2));
+                zc G2p = zscale(gpa[b1 * nb + b2], prefac_sqrtdelta * sqrt(deltaplus));
+                double dEbyT = dE / T;
+                double nq1_DB = (detail_balance && -dEbyT < 46) ? (exp(-dEbyT) * nqp) : nqp + 1;
+                App[b1 * nb + b2] = zscale(G2p, sqrt(nq1_DB));
+                Apm[b1 * nb + b2] = zscale(G2p, sqrt(nqp));
+                A2pp[b1 * nb + b2] = zscale(G2p, sqrt(nq1_DB));
+                A2pm[b1 * nb + b2] = zscale(G2p, sqrt(nqp));
+                /* phonon absorption, :507-519 and :535-540 */
+                double deltaminus = exp(prefac_exp * pow(dE - wqm, 2));
+                zc G2m = zscale(gma[b1 * nb + b2], prefac_sqrtdelta * sqrt(deltaminus));
+                nq1_DB = (detail_balance && dEbyT < 46) ? (exp(dEbyT) * nqm) : nqm + 1;
+                Amp[b1 * nb + b2] = zscale(G2m, sqrt(nq1_DB));
+                Amm[b1 * nb + b2] = zscale(G2m, sqrt(nqm));
+                A2mp[b1 * nb + b2] = zscale(G2m, sqrt(nq1_DB));
+                A2mm[b1 * nb + b2] = zscale(G2m, sqrt(nqm));
+            }
+        /* :558-571 */
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                int n12 = (i1 * nb + i2) * nb2;
+                for (int i3 = 0; i3 < nb; i3++) {
+                    int i13 = i1 * nb + i3, i31 = i3 * nb + i1;
+                    for (int i4 = 0; i4 < nb; i4++) {
+                        zc t1 = zadd(zmul(App[i13], zconj(A2pp[i2 * nb + i4])), zmul(Amm[i13], zconj(A2mm[i2 * nb + i4])));
+                        zc t2 = zadd(zmul(Amp[i31], zconj(A2mp[i4 * nb + i2])), zmul(Apm[i31], zconj(A2pm[i4 * nb + i2])));
+                        ZW(P1, n12 + i3 * nb + i4) = zadd(Z(P1, n12 + i3 * nb + i4), t1);
+                        ZW(P2, n12 + i3 * nb + i4) = zadd(Z(P2, n12 + i3 * nb + i4), t2);
+                    }
+                }
+            }
+    }
+    return 0;
+}
+
+/* e-i P in the model form without detailed balance,                        */
+/* src_v4.5.7/scattering_model/ElecImp_Model.h:185-214:                     */
+/* P1[(i1 i2),(i3 i4)] = A1[i1,i3] conj(A2[i2,i4]);  P2 = A1[i3,i1] conj(A2[i4,i2]). */
+int oracle_compute_P_eimp(int nb, const double *A1, const double *A2, double *P1, double *P2)
+{
+    int nb2 = nb * nb;
+    for (int i1 = 0; i1 < nb; i1++)
+        for (int i2 = 0; i2 < nb; i2++) {
+            int n12 = (i1 * nb + i2) * nb2;
+            for (int i3 = 0; i3 < nb; i3++) {
+                int i13 = i1 * nb + i3, i31 = i3 * nb + i1;
+                for (int i4 = 0; i4 < nb; i4++) {
+                    ZW(P1, n12 + i3 * nb + i4) = zmul(Z(A1, i13), zconj(Z(A2, i2 * nb + i4)));
+                    ZW(P2, n12 + i3 * nb + i4) = zmul(Z(A1, i31), zconj(Z(A2, i4 * nb + i2)));
+                }
+            }
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coherent term, singdenmat_k::evolve_coh + compute_Hcoht,                  */
+/* src_v4.5.7/DenMat.cpp:29-55: ddmdt = -i (H(t) dm - dm H(t)),              */
+/* H(t)_ij = H_ij cis((e_i - e_j) t) off the diagonal in the interaction      */
+/* picture.  Output overwritten.                                             */
+/* ------------------------------------------------------------------------ */
+int oracle_coherent(int64_t nk, int nb, const double *H, const double *e, const double *dm,
+                    double t, int interaction, double *out)
+{
+    int nb2 = nb * nb;
+    const zc c1 = zmk(1, 0), cm1 = zmk(-1, 0), c0 = zmk(0, 0), cmi = zmk(0, -1);
+    zc Ht[64];
+    for (int64_t ik = 0; ik < nk; ik++) {
+        const zc *Hk = (const zc *)H + ik * nb2;
+        const double *ek = e + ik * nb;
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++)
+                Ht[i * nb + j] = (i == j || !interaction) ? Hk[i * nb + j] : zmul(Hk[i * nb + j], zcis((ek[i] - ek[j]) * t));
+        zc *o = (zc *)out + ik * nb2;
+        for (int i = 0; i < nb2; i++) o[i] = c0;
+        zhemm_upper(o, 0, Ht, (const zc *)dm + ik * nb2, nb, c1, c0);   /* dm*H      */
+        zhemm_upper(o, 1, Ht, (const zc *)dm + ik * nb2, nb, c1, cm1);  /* H*dm - .. */
+        for (int i = 0; i < nb2; i++) o[i] = zmul(o[i], cmi);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spin-type observable, ob_1dmk::measure_brange for sx/sy/sz with diff=true, */
+/* src_v4.5.7/observable.h:301-336 and totals :356-366:                      */
+/*   S = prefac * sum_k sum_{i in [i0,i1)} sum_b Re(s_t[i,b] * dmk_bi),       */
+/*   s_t[i,b] = s[i,b] cis((e_i - e_b) t) (i != b, interaction picture),      */
+/*   dmk_bi = dm[b,i] - f_i delta_bi.                                         */
+/* s is one component (nk x nb x nb); returns S, and S_amp (no phase) via     */
+/* *amp when non-null (the third column of sz_elec_tot.out).                  */
+/* ------------------------------------------------------------------------ */
+double oracle_spin(int64_t nk, int nb, int i0, int i1, const double *s, const double *e,
+                   const double *f, const double *dm, double t, int interaction, double prefac, double *amp)
+{
+    int nb2 = nb * nb;
+    double tot = 0, tot_amp = 0;
+    for (int64_t ik = 0; ik < nk; ik++) {
+        for (int i = i0; i < i1; i++) {
+            double ob = 0, ob_amp = 0;
+            for (int b = 0; b < nb; b++) {
+                zc ob_kib = Z(s, ik * nb2 + i * nb + b);
+                zc ob_kib_t = (i == b || !interaction) ? ob_kib : zmul(ob_kib, zcis((e[ik * nb + i] - e[ik * nb + b]) * t));
+                zc dm_kbi = Z(dm, ik * nb2 + b * nb + i);
+                if (i == b) dm_kbi.x -= f[ik * nb + i];
+                ob += zmul(ob_kib_t, dm_kbi).x;
+                ob_amp += zmul(ob_kib, dm_kbi).x;
+            }
+            tot += ob; tot_amp += ob_amp;
+        }
+    }
+    if (amp) *amp = tot_amp * prefac;
+    return tot * prefac;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Initial state rho0 = f + (df/de) H1, electron::compute_dm_Bpert_1st,       */
+/* src_v4.5.7/electron.cpp:62-98 (degthr 1e-8, degenerate limit             */
+/* favg(favg-1)/T, interaction-picture phase at t0).  s holds 3 components   */
+/* per k (nk x 3 x nb x nb), as ldbd_smat.bin.                               */
+/* ------------------------------------------------------------------------ */
+int oracle_dm_Bpert_1st(int64_t nk, int nb, const double *e, const double *f, const double *s,
+                        const double *Bpert, double T, double t0, int interaction, double *dm)
+{
+    int nb2 = nb * nb;
+    const double degthr = 1e-8;
+    for (int64_t ik = 0; ik < nk; ik++) {
+        zc *d = (zc *)dm + ik * nb2;
+        for (int i = 0; i < nb2; i++) d[i] = zmk(0, 0);
+        for (int i = 0; i < nb; i++)
+            for (int j = i; j < nb; j++) {
+                zc H1 = zmk(0, 0);
+                for (int c = 0; c < 3; c++)
+                    H1 = zadd(H1, zscale(Z(s, (ik * 3 + c) * nb2 + i * nb + j), Bpert[c]));
+                double ei = e[ik * nb + i], ej = e[ik * nb + j];
+                double fi = f[ik * nb + i], fj = f[ik * nb + j];
+                double dfde;
+                if (fabs(ei - ej) < degthr) {
+                    double favg = 0.5 * (fi + fj);
+                    dfde = favg * (favg - 1.) / T;
+                } else
+                    dfde = (fi - fj) / (ei - ej);
+                d[i * nb + j] = zscale(H1, dfde);
+                if (i != j && interaction) d[i * nb + j] = zmul(d[i * nb + j], zcis((ei - ej) * t0));
+                if (i < j) d[j * nb + i] = zconj(d[i * nb + j]);
+            }
+        for (int i = 0; i < nb; i++) d[i * nb + i].x += f[ik * nb + i];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Right-hand side of dm_dynamics::compute for the configuration this path  */
+/* covers, src_v4.5.7/DensityMatrix_Dynamics.h:243-278: 1-rho                */
+/* (set_oneminusdm, DenMat.cpp:249-261), optional coherent term, the         */
+/* scattering term through evolve_driver's band window (evolve_eph.cpp:21-32: */
+/* rows/cols [b0, b0+nb) of the nb_dm x nb_dm state), summed.               */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t nk; int nb_dm, b0, nb;
+    int64_t npairs; const uint64_t *k1, *k2; const double *P1, *P2;
+    const double *e_dm;     /* nk x nb_dm */
+    const double *H;        /* nk x nb_dm^2 or NULL */
+    const double *ddmdt_eq; /* nk x nb^2 (e-ph window) or NULL */
+    double prefac; int expt;
+    int nthreads;
+} oracle_system;
+
+static int oracle_rhs(const oracle_system *S, double t, const double *dm, double *ddmdt,
+                      double *w_dm, double *w_dm1, double *w_ddm, double *w_e, double *w_coh)
+{
+    int64_t nk = S->nk; int nbd = S->nb_dm, nb = S->nb, b0 = S->b0;
+    int nbd2 = nbd * nbd, nb2 = nb * nb;
+    memset(ddmdt, 0, sizeof(double) * 2 * nk * nbd2);
+    if (S->H) {
+        oracle_coherent(nk, nbd, S->H, S->e_dm, dm, t, 1, w_coh);
+        for (int64_t i = 0; i < 2 * nk * nbd2; i++) ddmdt[i] += w_coh[i];
+    }
+    for (int64_t ik = 0; ik < nk; ik++) {
+        for (int i = 0; i < nb; i++) {
+            w_e[ik * nb + i] = S->e_dm[ik * nbd + b0 + i];
+            for (int j = 0; j < nb; j++) {
+                zc v = Z(dm, ik * nbd2 + (b0 + i) * nbd + b0 + j);
+                ZW(w_dm, ik * nb2 + i * nb + j) = v;
+                ZW(w_dm1, ik * nb2 + i * nb + j) = (i == j) ? zmk(1.0 - v.x, -v.y) : zmk(-v.x, -v.y);
+            }
+        }
+    }
+    int rc = oracle_evolve(nk, nb, S->npairs, S->k1, S->k2, S->P1, S->P2, w_e, w_dm, w_dm1, t,
+                           S->prefac, S->expt, S->ddmdt_eq, 0, S->nthreads, w_ddm);
+    if (rc) return rc;
+    for (int64_t ik = 0; ik < nk; ik++)
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) {
+                size_t o = ik * nbd2 + (b0 + i) * nbd + b0 + j;
+                ZW(ddmdt, o) = zadd(Z(ddmdt, o), Z(w_ddm, ik * nb2 + i * nb + j));
+            }
+    return 0;
+}
+
+/* Classical fourth-order Runge-Kutta on the compute() right-hand side, with */
+/* the state re-Hermitized after every step as update_dm_euler does for the  */
+/* reference's Euler step (DenMat.cpp:236-248).  The reference has no fixed- */
+/* step RK4 (its integrators are GSL rkf45 and Euler,                        */
+/* DensityMatrix_Dynamics.h:177-234); this is the textbook method.           */
+/* Observable (if s_obs) recorded at step 0 and every report_every steps.    */
+int oracle_rk4(int64_t nk, int nb_dm, int b0, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+               const double *P1, const double *P2, const double *e_dm, const double *H, const double *ddmdt_eq,
+               double prefac, int expt, double t0, double dt, int64_t nsteps, int report_every,
+               const double *s_obs, const double *f_dm, int i0, int i1, double obs_prefac,
+               int nthreads, double *dm, double *obs_out)
+{
+    oracle_system S = { nk, nb_dm, b0, nb, npairs, k1, k2, P1, P2, e_dm, H, ddmdt_eq, prefac, expt, nthreads };
+    size_t n = (size_t)2 * nk * nb_dm * nb_dm, nw = (size_t)2 * nk * nb * nb;
+    double *kk = (double *)malloc(sizeof(double) * n), *acc = (double *)malloc(sizeof(double) * n);
+    double *y = (double *)malloc(sizeof(double) * n), *coh = (double *)malloc(sizeof(double) * n);
+    double *wdm = (double *)malloc(sizeof(double) * nw), *wdm1 = (double *)malloc(sizeof(double) * nw);
+    double *wddm = (double *)malloc(sizeof(double) * nw), *we = (double *)malloc(sizeof(double) * nk * nb);
+    int rc = 0, iobs = 0;
+    double t = t0;
+    if (s_obs && report_every > 0) obs_out[iobs++] = oracle_spin(nk, nb_dm, i0, i1, s_obs, e_dm, f_dm, dm, t, 1, obs_prefac, NULL);
+    static const double a[4] = { 0.0, 0.5, 0.5, 1.0 }, w[4] = { 1.0, 2.0, 2.0, 1.0 };
+    for (int64_t it = 0; it < nsteps && !rc; it++) {
+        memcpy(y, dm, sizeof(double) * n);
+        memset(acc, 0, sizeof(double) * n);
+        for (int st = 0; st < 4 && !rc; st++) {
+            if (st > 0)
+                for (size_t i = 0; i < n; i++) y[i] = dm[i] + a[st] * dt * kk[i];
+            rc = oracle_rhs(&S, t + a[st] * dt, y, kk, wdm, wdm1, wddm, we, coh);
+            for (size_t i = 0; i < n; i++) acc[i] += w[st] * kk[i];
+        }
+        for (size_t i = 0; i < n; i++) dm[i] += dt / 6.0 * acc[i];
+        /* re-Hermitize: dm = (dm + dm^dagger)/2 per k */
+        for (int64_t ik = 0; ik < nk; ik++) {
+            zc *d = (zc *)dm + ik * nb_dm * nb_dm;
+            for (int i = 0; i < nb_dm; i++) {
+                d[i * nb_dm + i].y = 0.0;
+                for (int j = i + 1; j < nb_dm; j++) {
+                    zc u = d[i * nb_dm + j], l = d[j * nb_dm + i];
+                    zc h = zmk(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+                    d[i * nb_dm + j] = h; d[j * nb_dm + i] = zconj(h);
+                }
+            }
+        }
+        t = t0 + (double)(it + 1) * dt;
+        if (s_obs && report_every > 0 && (it + 1) % report_every == 0)
+            obs_out[iobs++] = oracle_spin(nk, nb_dm, i0, i1, s_obs, e_dm, f_dm, dm, t, 1, obs_prefac, NULL);
+    }
+    free(kk); free(acc); free(y); free(coh); free(wdm); free(wdm1); free(wddm); free(we);
+    return rc;
+}
+
+/* Number of OpenMP threads this build can use (1 without OpenMP). */
+int oracle_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

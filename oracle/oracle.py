@@ -1,0 +1,201 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product library
+never loads it.
+
+Arrays are numpy complex128 (interleaved re/im, the reference's complex
+layout, src_v4.5.7/comm/scalar.h:67-69) or float64 / uint64.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+_i64 = ctypes.c_int64
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.oracle_compute_Pt.argtypes = [ctypes.c_int, ctypes.c_double, _d, _d, _d, _d, ctypes.c_int]
+        L.oracle_evolve.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, ctypes.c_int,
+                                    ctypes.c_int, _d]
+        L.oracle_compute_ddm_pairs.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d,
+                                               ctypes.c_double, _d]
+        L.oracle_compute_P.argtypes = [ctypes.c_int, _d, _d, ctypes.c_int, _d, _d, _d, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, _d, _d]
+        L.oracle_compute_P_eimp.argtypes = [ctypes.c_int, _d, _d, _d, _d]
+        L.oracle_coherent.argtypes = [_i64, ctypes.c_int, _d, _d, _d, ctypes.c_double, ctypes.c_int, _d]
+        L.oracle_spin.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_double, _d]
+        L.oracle_spin.restype = ctypes.c_double
+        L.oracle_dm_Bpert_1st.argtypes = [_i64, ctypes.c_int, _d, _d, _d, _d, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, _d]
+        L.oracle_rk4.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d,
+                                 _d, _d, _d, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_double, _i64, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_int, _d, _d]
+        L.oracle_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_d if a.dtype != np.uint64 else _u64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def compute_Pt(nb, t, ek, ekp, P, minus):
+    P = _c128(P)
+    out = np.empty_like(P)
+    lib().oracle_compute_Pt(nb, t, _p(_f64(ek)), _p(_f64(ekp)), _p(P), _p(out), int(minus))
+    return out
+
+
+def evolve(k1, k2, P1, P2, e, dm, dm1=None, t=0.0, prefac=1.0, expt=True, ddmdt_eq=None,
+           schrodinger=False, nthreads=1):
+    """electronphonon::evolve (src_v4.5.7/evolve_eph.cpp:34-166).
+
+    P1, P2: (npairs, nb^4) complex; e: (nk, nb); dm, dm1: (nk, nb^2) complex.
+    dm1 defaults to 1 - dm as set_oneminusdm forms it (DenMat.cpp:249-261).
+    """
+    e = _f64(e)
+    nk, nb = e.shape
+    dm = _c128(dm).reshape(nk, nb * nb)
+    if dm1 is None:
+        dm1 = one_minus(dm, nb)
+    dm1 = _c128(dm1).reshape(nk, nb * nb)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    P1 = _c128(P1)
+    P2 = _c128(P2)
+    out = np.zeros((nk, nb * nb), np.complex128)
+    eq = None if ddmdt_eq is None else _c128(ddmdt_eq)
+    rc = lib().oracle_evolve(nk, nb, len(k1), _p(k1), _p(k2), _p(P1), _p(P2), _p(e), _p(dm), _p(dm1),
+                             float(t), float(prefac), int(bool(expt)), _p(eq), int(bool(schrodinger)),
+                             int(nthreads), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_evolve failed: {rc}")
+    return out
+
+
+def compute_ddm_pairs(k1, k2, P1, P2, dm, dm1=None, prefac=1.0, nk=None, nb=None):
+    dm = _c128(dm)
+    nk = dm.shape[0] if nk is None else nk
+    nb = int(round(np.sqrt(dm.shape[1]))) if nb is None else nb
+    if dm1 is None:
+        dm1 = one_minus(dm, nb)
+    dm1 = _c128(dm1)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    out = np.zeros((nk, nb * nb), np.complex128)
+    rc = lib().oracle_compute_ddm_pairs(nk, nb, len(k1), _p(k1), _p(k2), _p(_c128(P1)), _p(_c128(P2)),
+                                        _p(dm), _p(dm1), float(prefac), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_compute_ddm_pairs failed: {rc}")
+    return out
+
+
+def compute_P(Ek, Ekp, omega, gm, gp, T, sigma, detail_balance=False):
+    nb = len(Ek)
+    nm = len(omega)
+    P1 = np.zeros(nb ** 4, np.complex128)
+    P2 = np.zeros(nb ** 4, np.complex128)
+    lib().oracle_compute_P(nb, _p(_f64(Ek)), _p(_f64(Ekp)), nm, _p(_f64(omega)), _p(_c128(gm)),
+                           _p(_c128(gp)), float(T), float(sigma), int(detail_balance), _p(P1), _p(P2))
+    return P1, P2
+
+
+def compute_P_eimp(A1, A2):
+    A1 = _c128(A1)
+    nb = int(round(np.sqrt(A1.size)))
+    P1 = np.zeros(nb ** 4, np.complex128)
+    P2 = np.zeros(nb ** 4, np.complex128)
+    lib().oracle_compute_P_eimp(nb, _p(A1), _p(_c128(A2)), _p(P1), _p(P2))
+    return P1, P2
+
+
+def coherent(H, e, dm, t, interaction=True):
+    e = _f64(e)
+    nk, nb = e.shape
+    out = np.zeros((nk, nb * nb), np.complex128)
+    lib().oracle_coherent(nk, nb, _p(_c128(H)), _p(e), _p(_c128(dm)), float(t), int(interaction), _p(out))
+    return out
+
+
+def spin(s, e, f, dm, t, i0, i1, prefac, interaction=True, with_amp=False):
+    e = _f64(e)
+    nk, nb = e.shape
+    amp = np.zeros(1)
+    v = lib().oracle_spin(nk, nb, i0, i1, _p(_c128(s)), _p(e), _p(_f64(f)), _p(_c128(dm)), float(t),
+                          int(interaction), float(prefac), _p(amp))
+    return (v, float(amp[0])) if with_amp else v
+
+
+def dm_Bpert_1st(e, f, s3, Bpert, T, t0=0.0, interaction=True):
+    e = _f64(e)
+    nk, nb = e.shape
+    out = np.zeros((nk, nb * nb), np.complex128)
+    lib().oracle_dm_Bpert_1st(nk, nb, _p(e), _p(_f64(f)), _p(_c128(s3)), _p(_f64(Bpert)), float(T),
+                              float(t0), int(interaction), _p(out))
+    return out
+
+
+def rk4(k1, k2, P1, P2, e_dm, dm0, dt, nsteps, b0=0, nb=None, prefac=1.0, expt=True, t0=0.0, H=None,
+        ddmdt_eq=None, report_every=0, s_obs=None, f_dm=None, i0=0, i1=None, obs_prefac=1.0,
+        nthreads=1):
+    e_dm = _f64(e_dm)
+    nk, nbd = e_dm.shape
+    nb = nbd if nb is None else nb
+    i1 = nbd if i1 is None else i1
+    dm = _c128(dm0).copy().reshape(nk, nbd * nbd)
+    nobs = (nsteps // report_every + 1) if (report_every and s_obs is not None) else 0
+    obs = np.zeros(max(nobs, 1))
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    rc = lib().oracle_rk4(nk, nbd, b0, nb, len(k1), _p(k1), _p(k2), _p(_c128(P1)), _p(_c128(P2)), _p(e_dm),
+                          _p(None if H is None else _c128(H)), _p(None if ddmdt_eq is None else _c128(ddmdt_eq)),
+                          float(prefac), int(expt), float(t0), float(dt), int(nsteps), int(report_every),
+                          _p(None if s_obs is None else _c128(s_obs)), _p(None if f_dm is None else _f64(f_dm)),
+                          i0, i1, float(obs_prefac), int(nthreads), _p(dm), _p(obs))
+    if rc != 0:
+        raise ValueError(f"oracle_rk4 failed: {rc}")
+    return dm, obs[:nobs]
+
+
+def one_minus(dm, nb):
+    """1 - rho exactly as set_oneminusdm forms it (src_v4.5.7/DenMat.cpp:249-261)."""
+    dm = _c128(dm)
+    out = -dm.reshape(-1, nb, nb)
+    idx = np.arange(nb)
+    out[:, idx, idx] = 1.0 - dm.reshape(-1, nb, nb)[:, idx, idx]
+    return out.reshape(dm.shape)
+
+
+def max_threads():
+    return lib().oracle_max_threads()

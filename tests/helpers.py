@@ -1,0 +1,55 @@
+"""Shared builders for test inputs (seeded)."""
+import numpy as np
+
+FS = 41.341373335182  # 1 fs in Hartree atomic time (src_v4.5.7/comm/Units.h:62)
+
+
+def random_pairs(nk, npart, rng, diag=False):
+    s = set()
+    for k in range(nk):
+        for _ in range(npart):
+            kp = int(rng.integers(nk))
+            if kp == k and not diag:
+                continue
+            s.add((min(k, kp), max(k, kp)))
+    if diag:
+        for k in range(0, nk, 3):
+            s.add((k, k))
+    pr = sorted(s)
+    k1 = np.array([a for a, _ in pr], np.uint64)
+    k2 = np.array([b for _, b in pr], np.uint64)
+    return k1, k2
+
+
+def random_energies(nk, nb, rng, emax=3.7e-3):
+    return np.sort(rng.uniform(0, emax, (nk, nb)), axis=1)
+
+
+def random_P(npairs, nb, rng, structured=True):
+    """P1/P2 either in the Lindblad Gram form sum_l A (x) conj(A) (the form the
+    initializer writes) or unstructured complex numbers."""
+    n2, n4 = nb * nb, nb ** 4
+    if not structured:
+        P1 = rng.normal(size=(npairs, n4)) + 1j * rng.normal(size=(npairs, n4))
+        P2 = rng.normal(size=(npairs, n4)) + 1j * rng.normal(size=(npairs, n4))
+        return 1e-3 * P1, 1e-3 * P2
+    P1 = np.zeros((npairs, nb, nb, nb, nb), np.complex128)
+    P2 = np.zeros_like(P1)
+    for _ in range(3):
+        A = (rng.normal(size=(npairs, nb, nb)) + 1j * rng.normal(size=(npairs, nb, nb))) * 0.03
+        B = (rng.normal(size=(npairs, nb, nb)) + 1j * rng.normal(size=(npairs, nb, nb))) * 0.03
+        # P1[(i1 i2),(i3 i4)] = A[i1,i3] conj(A[i2,i4]); P2 = B[i3,i1] conj(B[i4,i2])
+        P1 += np.einsum("pac,pbd->pabcd", A, A.conj())
+        P2 += np.einsum("pca,pdb->pabcd", B, B.conj())
+    return P1.reshape(npairs, n4), P2.reshape(npairs, n4)
+
+
+def random_rho(nk, nb, rng, scale=0.05, diag=0.3):
+    A = rng.normal(size=(nk, nb, nb)) + 1j * rng.normal(size=(nk, nb, nb))
+    H = scale * (A + A.conj().transpose(0, 2, 1)) / 2
+    H += np.eye(nb) * diag
+    return H.reshape(nk, nb * nb)
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
