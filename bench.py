@@ -128,6 +128,12 @@ def main():
     alg_bytes = n_pairs * (2 * p.nb ** 4 * 16 + 4) + 2 * p.nk * p.nb ** 2 * 16
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     log(f"contraction kernel {kernel_ms:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
+    if args.kernel == 1 or p.nb > 2:
+        kind, kname = "rows", f"k_rows<{p.nb}>"
+    elif args.kernel == 2 or p.nb == 1:
+        kind, kname = "single_read", f"k_fwd_atomic<{p.nb}> (+k_assemble)"
+    else:
+        kind, kname = "tiles", "k_tiles<false> (+k_assemble)"
     out = {
         "metric": "k-pair contractions/s",
         "value": value,
@@ -144,12 +150,12 @@ def main():
         "config": {"workload": f"plain FP64 Lindblad e-ph, preset {args.preset}: nb={p.nb} nk={p.nk} "
                                f"stored_pairs={n_pairs} RK4 dt=1fs",
                    "nb": p.nb, "nk": p.nk, "stored_pairs": n_pairs, "precision": "fp64",
-                   "kernel": ["auto", "rows", "tiles"][args.kernel]},
+                   "kernel": kind},
         "hbm_GBps": achieved,
         "traj_steps_per_s": args.steps / el,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "k_rows<2,false>" if args.kernel != 2 else "k_tiles",
+                     "kernel": kname,
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:

@@ -25,7 +25,8 @@ DMD_PREC_BF16 = 2
 
 DMD_KERNEL_AUTO = 0
 DMD_KERNEL_ROWS = 1
-DMD_KERNEL_TILES = 2
+DMD_KERNEL_SINGLE_READ = 2
+DMD_KERNEL_TILES = 3
 
 
 class DMDError(RuntimeError):

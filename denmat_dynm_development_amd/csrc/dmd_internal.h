@@ -27,6 +27,9 @@ int hw_threads();
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// rows per tile of the tile kernel (k mod R and k' mod R fit in 8 bits)
+#define DMD_TILE_R 256
+
 // Device clock: t(a) = (t0 + step*dt) + a*dt, the same expression as the RK4
 // stage times of the oracle's driver.
 struct DClock {
@@ -83,13 +86,23 @@ struct dmd_scatt {
     double *d_e = nullptr, *d_edm = nullptr;
     double2 *d_P = nullptr;
     int64_t *d_inv = nullptr, *d_rowptr = nullptr, *d_revptr = nullptr, *d_revpos = nullptr;
-    int32_t *d_col = nullptr, *d_revcol = nullptr, *d_k1 = nullptr;
+    int32_t *d_col = nullptr, *d_revcol = nullptr, *d_k1 = nullptr, *d_k2 = nullptr;
+    int64_t *d_fwdpos = nullptr;         // forward list entry -> physical position of P
+    uint16_t *d_pk = nullptr;            // per physical pair: (k mod R) | (k' mod R) << 8
+    int64_t *d_cbeg = nullptr;           // tile chunks: first pair, length, tile (I, J)
+    int32_t *d_clen = nullptr;
+    int2 *d_cij = nullptr;
+    int64_t nchunks = 0;
+    int64_t nslots = 0;                  // physical pair slots (stored pairs + tile padding)
+    int tile_R = 0;
+    int soa = 0;                         // P stored as SoA granules of 8 pairs (tile kernel)
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
     // driver: coherent H, observable, RK4 buffers (allocated on first use)
     double2 *d_H = nullptr, *d_s = nullptr;
     double *d_f = nullptr, *d_obs = nullptr, *d_part = nullptr;
+    double *d_acc_rows = nullptr;  // nk x [Pdm | dm1P] accumulator of the single-read kernel
     int obs_i0 = 0, obs_i1 = 0;
     double obs_prefac = 0;
     bool has_H = false, has_obs = false;
@@ -126,4 +139,6 @@ int set_clock(dmd_scatt *h, double t0, double dt, long long step);
 int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a);
 // contraction + Hermitian epilogue on h->d_rhoS (and h->d_rho1S when dm1) into out (nk x nb^2)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
+// DMD_KERNEL_ROWS or DMD_KERNEL_SINGLE_READ: the kernel launch_contract uses
+int contract_kind(const dmd_scatt *h);
 }  // namespace dmd

@@ -129,6 +129,7 @@ __device__ __forceinline__ double2 rho_elem(const double2 *__restrict__ rhoS, co
 template <int NB, bool DM1>
 __global__ __launch_bounds__(256) void k_rows(int64_t nk, const double2 *__restrict__ P,
                                               const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                              const int64_t *__restrict__ fwdpos,
                                               const int64_t *__restrict__ revptr,
                                               const int64_t *__restrict__ revpos,
                                               const int32_t *__restrict__ revcol,
@@ -174,9 +175,10 @@ __global__ __launch_bounds__(256) void k_rows(int64_t nk, const double2 *__restr
             bool ok[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                int64_t pair = q + (int64_t)u * PPI + slot;
-                ok[u] = pair < p1;
-                int64_t kp = ok[u] ? (int64_t)col[pair] : 0;
+                int64_t it = q + (int64_t)u * PPI + slot;
+                ok[u] = it < p1;
+                int64_t kp = ok[u] ? (int64_t)col[it] : 0;
+                int64_t pair = ok[u] ? fwdpos[it] : 0;
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     bool v = ok[u] && valid[j];
@@ -268,11 +270,295 @@ __global__ __launch_bounds__(256) void k_rows(int64_t nk, const double2 *__restr
 }
 
 // ---------------------------------------------------------------------------
+// Single-read kernel: each stored pair's P is streamed once.  One wavefront
+// per row k walks the forward pairs (k, k') (contiguous P):
+//   forward contributions to row k stay in registers (as in the row kernel);
+//   reverse contributions to row k' (k < k') are reduced across the pair's
+//   lanes with shuffles to 2 NB^2 complex and added with FP64 atomics into a
+//   per-row accumulator acc[k'] = [Pdm | dm1P] (NB = 2: one 128-byte line),
+//   which stays resident in L2 / Infinity Cache.
+// k_assemble then forms the Hermitian epilogue per row and re-zeros acc, so
+// no memset is needed between evaluations (safe inside a captured graph).
+// Summation order of the atomics is not fixed: results agree with the
+// deterministic row kernel to rounding.
+// ---------------------------------------------------------------------------
+// Reverse contributions of one pair: summed over the pair's lanes, then added
+// with FP64 atomics into row k' of the accumulator ([Pdm | dm1P], 4 NB^2
+// doubles).  `row` is that row (null for lanes whose pair has no reverse term).
+template <int NB>
+struct RevAdd;
+// NB = 1 (2 lanes per pair): lane e=0 holds P1 -> dm1P[0], e=1 holds P2 -> Pdm[0].
+template <>
+struct RevAdd<1> {
+    __device__ static void run(double2 v, int lane, double *row)
+    {
+        const int e = lane & 1;
+        if (!row) return;
+        double *a = row + (e ? 0 : 2);
+        unsafeAtomicAdd(a, v.x);
+        unsafeAtomicAdd(a + 1, v.y);
+    }
+};
+// NB = 2 (32 lanes per pair): P1 lanes (e<16) sum over i12 (xor 4, 8) ->
+// dm1P[i34] held at e = i34; P2 lanes sum over i34 (xor 1, 2) -> Pdm[i12]
+// held at e = 16 + 4 i12.  Lane L < 16 then takes component L&1 of target
+// L>>1 so one atomic instruction covers the row's 128-byte line.
+template <>
+struct RevAdd<2> {
+    __device__ static void run(double2 x, int lane, double *row)
+    {
+        const int e = lane & 31, base = lane & 32;
+        const bool p1 = e < 16;
+        const int m1 = p1 ? 4 : 1, m2 = p1 ? 8 : 2;
+        x.x += __shfl(x.x, lane ^ m1); x.y += __shfl(x.y, lane ^ m1);
+        x.x += __shfl(x.x, lane ^ m2); x.y += __shfl(x.y, lane ^ m2);
+        const int t = (e & 15) >> 1;  // target complex 0..7: Pdm[0..3], dm1P[0..3]
+        const int src = base + (t < 4 ? 16 + 4 * t : t - 4);
+        double sx = __shfl(x.x, src), sy = __shfl(x.y, src);
+        if (row && e < 16) unsafeAtomicAdd(row + e, (e & 1) ? sy : sx);
+    }
+};
+
+template <int NB, bool DM1>
+__global__ __launch_bounds__(256) void k_fwd_atomic(int64_t nk, const double2 *__restrict__ P,
+                                                    const int64_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const int64_t *__restrict__ fwdpos,
+                                                    const double2 *__restrict__ rhoS,
+                                                    const double2 *__restrict__ rho1S, double *__restrict__ acc)
+{
+    using S = Shape<NB>;
+    constexpr int NB2 = S::NB2, NB4 = S::NB4, EP = S::EP, PPI = S::PPI, SLOTS = S::SLOTS;
+    static_assert(EP <= 64, "single-read kernel covers nb <= 2; nb = 4 uses k_fwd_atomic4");
+    __shared__ double2 red[4][SLOTS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t k = (int64_t)blockIdx.x * 4 + wv;
+    const bool active = k < nk;
+    const int slot = lane / EP, el = lane % EP;
+    const int which = el / NB4, r = el - which * NB4, i12 = r / NB2, i34 = r - i12 * NB2;
+    const int fidx = which == 0 ? i34 : i12, ridx = which == 0 ? i12 : i34;
+    const bool fone = which == 1, rone = which == 0;
+    double2 af = make_double2(0, 0);
+    if (active) {
+        // reverse factor: rho of this row (the pair's k), fixed for the row
+        const double2 rrow = rho_elem<NB, DM1>(rhoS, rho1S, k, ridx, rone);
+        const int64_t p0 = rowptr[k], p1 = rowptr[k + 1];
+        constexpr int U = 4;
+        for (int64_t q = p0; q < p1; q += PPI * U) {
+            double2 pv[U], rv[U];
+            int64_t kp[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                int64_t it = q + (int64_t)u * PPI + slot;
+                bool ok = it < p1;
+                kp[u] = ok ? (int64_t)col[it] : -1;
+                int64_t pair = ok ? fwdpos[it] : 0;
+                pv[u] = ok ? P[pair * EP + el] : make_double2(0, 0);
+                rv[u] = ok ? rho_elem<NB, DM1>(rhoS, rho1S, kp[u], fidx, fone) : make_double2(0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                af = cadd(af, cmul(pv[u], rv[u]));
+                // reverse: P1 lane -> (1-rho_k)[i12] conj(P1), P2 lane -> conj(P2) rho_k[i34]
+                double2 vr = kp[u] > k ? cmulc(rrow, pv[u]) : make_double2(0, 0);
+                RevAdd<NB>::run(vr, lane, kp[u] > k ? acc + kp[u] * (4 * NB2) : nullptr);
+            }
+        }
+    }
+    red[wv][lane] = af;  // SLOTS == 64 for EP <= 64
+    __syncthreads();
+    if (!active || lane >= 2 * NB2) return;
+    // forward sums: lane < NB2 -> Pdm[lane] (P1 rows), else dm1P[lane-NB2] (P2 columns)
+    double2 s = make_double2(0, 0);
+    for (int sl = 0; sl < PPI; sl++)
+        for (int m = 0; m < NB2; m++)
+            s = cadd(s, lane < NB2 ? red[wv][sl * EP + lane * NB2 + m] : red[wv][sl * EP + NB4 + m * NB2 + (lane - NB2)]);
+    double *a = acc + k * (4 * NB2) + 2 * lane;
+    unsafeAtomicAdd(a, s.x);
+    unsafeAtomicAdd(a + 1, s.y);
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel (nb = 2), lane per pair.  Rows are cut into blocks of
+// R = DMD_TILE_R; the stored pairs of tile (I, J) (k in block I, k' in block
+// J) are contiguous in HBM, padded to granules of 8 pairs, and each granule
+// is stored structure-of-arrays: element j of its 8 pairs is one 128-byte
+// line.  A wavefront takes 64 consecutive pairs (8 granules), one per lane,
+// and loads them with 32 fully coalesced 1 KiB instructions (32 KiB in flight
+// per wave).  Each lane then forms its pair's four GEMVs in registers
+//   forward  Pdm_k[i]  = sum_m P1[i,m] rho_k'[m]   dm1P_k[j]  = sum_m (1-rho_k')[m] P2[m,j]
+//   reverse  Pdm_k'[i] = sum_m conj(P2[i,m]) rho_k[m]  dm1P_k'[j] = sum_m (1-rho_k)[m] conj(P1[m,j])
+// (no cross-lane traffic) and adds them into LDS row accumulators of block I
+// and block J (ds_add_f64).  The chunk's two R-row blocks are flushed once
+// with global FP64 atomics (~2R lines per chunk instead of one per pair).
+// Padding pairs carry P = 0 and add exact zeros.
+// ---------------------------------------------------------------------------
+template <bool DM1>
+__global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
+                                                   const uint16_t *__restrict__ pk,
+                                                   const int64_t *__restrict__ cbeg,
+                                                   const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
+                                                   const double2 *__restrict__ rhoS,
+                                                   const double2 *__restrict__ rho1S, double *__restrict__ acc)
+{
+    constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R, RW = 16;  // RW: doubles per row [Pdm | dm1P]
+    constexpr int RS = RW + 1;  // LDS row stride: the odd pad spreads rows over the banks
+    constexpr int W = 4;
+    __shared__ double accA[R * RS], accB[R * RS];
+    const int c = blockIdx.x;
+    const int64_t b = cbeg[c];
+    const int n = clen[c];
+    const int m = (n + 63) / 64;  // blocks of 64 slots; lane l owns chunk pairs l*m .. l*m+m-1
+    const int2 ij = cij[c];
+    const int I = ij.x, J = ij.y;
+    double *aJ = (I == J) ? accA : accB;
+    for (int i = threadIdx.x; i < R * RS; i += blockDim.x) { accA[i] = 0.0; accB[i] = 0.0; }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t I0 = (int64_t)I * R, J0 = (int64_t)J * R;
+    const int mq = (m + W - 1) / W;
+    const int i0 = wv * mq, i1 = min(m, i0 + mq);
+    for (int it = i0; it < i1; it++) {
+        const bool ok = lane * m + it < n;
+        const int64_t s = b + (int64_t)it * 64 + lane;
+        const unsigned code = ok ? pk[s] : 0u;
+        const int lo1 = code & 255u, lo2 = code >> 8;
+        const int64_t k = I0 + lo1, kp = J0 + lo2;
+        const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
+        double2 p[EP];
+#pragma unroll
+        for (int el = 0; el < EP; el++) p[el] = ok ? g[el * 8] : make_double2(0, 0);
+        double2 rk[NB2], r1k[NB2], rkp[NB2], r1kp[NB2];
+#pragma unroll
+        for (int q = 0; q < NB2; q++) {
+            rk[q] = ok ? rhoS[k * NB2 + q] : make_double2(0, 0);
+            rkp[q] = ok ? rhoS[kp * NB2 + q] : make_double2(0, 0);
+            r1k[q] = ok ? rho_elem<NB, DM1>(rhoS, rho1S, k, q, true) : make_double2(0, 0);
+            r1kp[q] = ok ? rho_elem<NB, DM1>(rhoS, rho1S, kp, q, true) : make_double2(0, 0);
+        }
+        const bool rev = ok && k < kp;
+#pragma unroll
+        for (int i = 0; i < NB2; i++) {
+            double2 f1 = make_double2(0, 0), f2 = make_double2(0, 0), g1 = make_double2(0, 0), g2 = make_double2(0, 0);
+#pragma unroll
+            for (int q = 0; q < NB2; q++) {
+                f1 = cadd(f1, cmul(p[i * NB2 + q], rkp[q]));             // Pdm_k[i]
+                f2 = cadd(f2, cmul(r1kp[q], p[NB4 + q * NB2 + i]));      // dm1P_k[i]
+                g1 = cadd(g1, cmulc(rk[q], p[NB4 + i * NB2 + q]));       // Pdm_k'[i]
+                g2 = cadd(g2, cmulc(r1k[q], p[q * NB2 + i]));            // dm1P_k'[i]
+            }
+            if (ok) {
+                atomicAdd(&accA[lo1 * RS + 2 * i], f1.x);
+                atomicAdd(&accA[lo1 * RS + 2 * i + 1], f1.y);
+                atomicAdd(&accA[lo1 * RS + 2 * (NB2 + i)], f2.x);
+                atomicAdd(&accA[lo1 * RS + 2 * (NB2 + i) + 1], f2.y);
+            }
+            if (rev) {
+                atomicAdd(&aJ[lo2 * RS + 2 * i], g1.x);
+                atomicAdd(&aJ[lo2 * RS + 2 * i + 1], g1.y);
+                atomicAdd(&aJ[lo2 * RS + 2 * (NB2 + i)], g2.x);
+                atomicAdd(&aJ[lo2 * RS + 2 * (NB2 + i) + 1], g2.y);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < R * RW; i += blockDim.x) {
+        const int row = i / RW, d = i - row * RW;
+        const double v = accA[row * RS + d];
+        if (v != 0.0 && I0 + row < nk) unsafeAtomicAdd(acc + (I0 + row) * RW + d, v);
+        if (I != J) {
+            const double w = accB[row * RS + d];
+            if (w != 0.0 && J0 + row < nk) unsafeAtomicAdd(acc + (J0 + row) * RW + d, w);
+        }
+    }
+}
+
+// In-place transpose of every 8-pair granule between pair-contiguous
+// ([pair][element]) and structure-of-arrays ([element][pair]) storage.
+__global__ void k_relayout(int64_t ngran, int ep, int to_soa, double2 *__restrict__ P)
+{
+    extern __shared__ double2 buf[];
+    const int64_t g = blockIdx.x;
+    if (g >= ngran) return;
+    double2 *base = P + g * 8 * ep;
+    const int n = 8 * ep;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) buf[i] = base[i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        // i indexes the destination layout
+        int pr, el;
+        if (to_soa) { el = i / 8; pr = i - el * 8; }
+        else { pr = i / ep; el = i - pr * ep; }
+        base[i] = to_soa ? buf[pr * ep + el] : buf[el * 8 + pr];
+    }
+}
+
+// Epilogue from the accumulator: C = (1-rho) Pdm - dm1P rho (zhemm upper),
+// ddm = prefac/2 (C + C^+), rotated back, minus ddmdt_eq; zeroes acc[k].
+template <int NB, bool DM1>
+__global__ __launch_bounds__(256) void k_assemble(int64_t nk, double *__restrict__ acc,
+                                                  const double2 *__restrict__ rhoS,
+                                                  const double2 *__restrict__ rho1S, const double *__restrict__ e,
+                                                  const DClock *__restrict__ clk, double a, int expt, int schrodinger,
+                                                  double prefac, const double2 *__restrict__ eq,
+                                                  double2 *__restrict__ out)
+{
+    constexpr int NB2 = NB * NB;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double2 pdm[NB2], d1p[NB2], rk[NB2], r1k[NB2];
+    double *ak = acc + k * (4 * NB2);
+#pragma unroll
+    for (int m = 0; m < NB2; m++) {
+        pdm[m] = make_double2(ak[2 * m], ak[2 * m + 1]);
+        d1p[m] = make_double2(ak[2 * (NB2 + m)], ak[2 * (NB2 + m) + 1]);
+        rk[m] = rhoS[k * NB2 + m];
+        r1k[m] = rho_elem<NB, DM1>(rhoS, rho1S, k, m, true);
+    }
+#pragma unroll
+    for (int m = 0; m < 4 * NB2; m++) ak[m] = 0.0;
+    const double t = clock_t_at(clk, a);
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            double2 cij = make_double2(0, 0), cji = make_double2(0, 0);
+#pragma unroll
+            for (int m = 0; m < NB; m++) {
+                cij = cadd(cij, cmul(herm_up<NB>(r1k, i, m), pdm[m * NB + j]));
+                cji = cadd(cji, cmul(herm_up<NB>(r1k, j, m), pdm[m * NB + i]));
+            }
+#pragma unroll
+            for (int m = 0; m < NB; m++) {
+                cij = csub(cij, cmul(d1p[i * NB + m], herm_up<NB>(rk, m, j)));
+                cji = csub(cji, cmul(d1p[j * NB + m], herm_up<NB>(rk, m, i)));
+            }
+            double2 d = cadd(cij, cconj(cji));
+            d = make_double2(d.x * (prefac * 0.5), d.y * (prefac * 0.5));
+            const double arg = (e[k * NB + i] - e[k * NB + j]) * t;
+            if (expt && i != j) d = cmul(d, ccis(arg));
+            if (eq) {
+                double2 q = eq[k * NB2 + i * NB + j];
+                if (i != j && !schrodinger) q = cmul(q, ccis(arg));
+                d = csub(d, q);
+            }
+            out[k * NB2 + i * NB + j] = d;
+        }
+}
+
+// ---------------------------------------------------------------------------
 // P placement: file order -> sorted layout, scaled; mode 0 set, 1 add.
 // ---------------------------------------------------------------------------
+// Address of element el of the pair in slot s: pair-contiguous, or the
+// structure-of-arrays granules of 8 pairs the tile kernel streams.
+__device__ __forceinline__ int64_t paddr(int64_t s, int el, int ep, int soa)
+{
+    return soa ? (s >> 3) * (8 * (int64_t)ep) + (int64_t)el * 8 + (s & 7) : s * ep + el;
+}
+
 __global__ void k_place_P(int64_t n, int64_t first, int nb4, const double2 *__restrict__ P1,
                           const double2 *__restrict__ P2, const int64_t *__restrict__ inv, double scale, int add,
-                          double2 *__restrict__ Pd)
+                          int soa, double2 *__restrict__ Pd)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t tot = n * 2 * nb4;
@@ -282,11 +568,12 @@ __global__ void k_place_P(int64_t n, int64_t first, int nb4, const double2 *__re
     int64_t pos = inv ? inv[first + f] : first + f;
     double2 v = el < nb4 ? P1[f * nb4 + el] : P2[f * nb4 + el - nb4];
     v = make_double2(v.x * scale, v.y * scale);
-    double2 *d = Pd + pos * 2 * nb4 + el;
+    double2 *d = Pd + paddr(pos, el, 2 * nb4, soa);
     *d = add ? cadd(*d, v) : v;
 }
 __global__ void k_fetch_P(int64_t n, int64_t first, int nb4, const double2 *__restrict__ Pd,
-                          const int64_t *__restrict__ inv, double2 *__restrict__ P1, double2 *__restrict__ P2)
+                          const int64_t *__restrict__ inv, int soa, double2 *__restrict__ P1,
+                          double2 *__restrict__ P2)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t tot = n * 2 * nb4;
@@ -294,25 +581,29 @@ __global__ void k_fetch_P(int64_t n, int64_t first, int nb4, const double2 *__re
     int64_t f = idx / (2 * nb4);
     int el = (int)(idx - f * 2 * nb4);
     int64_t pos = inv ? inv[first + f] : first + f;
-    double2 v = Pd[pos * 2 * nb4 + el];
+    double2 v = Pd[paddr(pos, el, 2 * nb4, soa)];
     if (el < nb4) P1[f * nb4 + el] = v;
     else P2[f * nb4 + el - nb4] = v;
 }
 
 template <int NB>
-__global__ void k_generate_P(int64_t np, dmd_synth_params p, const int32_t *__restrict__ k1s,
-                             const int32_t *__restrict__ col, const double *__restrict__ e, double2 *__restrict__ Pd)
+__global__ void k_generate_P(int64_t nslots, dmd_synth_params p, const int32_t *__restrict__ k1s,
+                             const int32_t *__restrict__ k2s, const double *__restrict__ e, int soa,
+                             double2 *__restrict__ Pd)
 {
     constexpr int NB4 = NB * NB * NB * NB;
-    int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= np) return;
-    int64_t k = k1s[pos], kp = col[pos];
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    int64_t k = k1s[s], kp = k2s[s];
     double P1[2 * NB4], P2[2 * NB4];
-    synth_pair_P_nb<NB>(p, e + k * NB, e + kp * NB, k, kp, P1, P2);
-    double2 *d = Pd + pos * 2 * NB4;
+    if (k >= 0) {
+        synth_pair_P_nb<NB>(p, e + k * NB, e + kp * NB, k, kp, P1, P2);
+    } else {  // padding slot
+        for (int i = 0; i < 2 * NB4; i++) { P1[i] = 0.0; P2[i] = 0.0; }
+    }
     for (int i = 0; i < NB4; i++) {
-        d[i] = make_double2(P1[2 * i], P1[2 * i + 1]);
-        d[NB4 + i] = make_double2(P2[2 * i], P2[2 * i + 1]);
+        Pd[paddr(s, i, 2 * NB4, soa)] = make_double2(P1[2 * i], P1[2 * i + 1]);
+        Pd[paddr(s, NB4 + i, 2 * NB4, soa)] = make_double2(P2[2 * i], P2[2 * i + 1]);
     }
 }
 
@@ -332,9 +623,10 @@ void dfree(void *p)
 dmd_scatt::~dmd_scatt()
 {
     dfree(d_e); dfree(d_edm); dfree(d_P); dfree(d_inv); dfree(d_rowptr); dfree(d_revptr); dfree(d_revpos);
-    dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
+    dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_k2); dfree(d_fwdpos); dfree(d_pk); dfree(d_cbeg);
+    dfree(d_clen); dfree(d_cij); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
     dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
-    dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part);
+    dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -398,7 +690,11 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     h->precision = precision;
     h->nbd = nb; h->b0 = 0;
 
-    // sorted order by (k1, k2): counting sort by k1, then by k2 within rows
+    // Layouts (all O(n_pairs) counting sorts):
+    //  kk order   stored pairs sorted by (k, k'): CSR rows of the forward lists
+    //  physical   order of P in HBM; for nb <= 2 sorted by (k/R, k'/R, k, k')
+    //             (R = DMD_TILE_R row tiles) so each tile's pairs are
+    //             contiguous for the tile kernel; otherwise the kk order.
     std::vector<int64_t> cnt(nk + 1, 0);
     bool sorted = true;
     for (int64_t i = 0; i < n_pairs; i++) {
@@ -406,24 +702,90 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         if (i && (k1st[i] < k1st[i - 1] || (k1st[i] == k1st[i - 1] && k2nd[i] < k2nd[i - 1]))) sorted = false;
     }
     for (int64_t k = 0; k < nk; k++) cnt[k + 1] += cnt[k];
-    std::vector<int64_t> order;
-    h->identity_order = sorted;
+    std::vector<int64_t> kk;  // kk position -> file index (empty: identity)
     if (!sorted) {
-        order.resize(n_pairs);
+        kk.resize(n_pairs);
         std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
-        for (int64_t i = 0; i < n_pairs; i++) order[cur[k1st[i]]++] = i;
+        for (int64_t i = 0; i < n_pairs; i++) kk[cur[k1st[i]]++] = i;
         for (int64_t k = 0; k < nk; k++)
-            std::stable_sort(order.begin() + cnt[k], order.begin() + cnt[k + 1],
+            std::stable_sort(kk.begin() + cnt[k], kk.begin() + cnt[k + 1],
                              [&](int64_t a, int64_t b) { return k2nd[a] < k2nd[b]; });
     }
-    auto F = [&](int64_t pos) { return sorted ? pos : order[pos]; };
-    std::vector<int32_t> col(n_pairs), k1s(n_pairs);
+    auto KK = [&](int64_t i) { return sorted ? i : kk[i]; };
+    const int R = nb <= 2 ? DMD_TILE_R : 0;
+    h->tile_R = R;
+    // Slots: physical positions of P.  With tiles, each tile's pairs (in
+    // (k, k') order) are cut into chunks of at most cmax pairs; a chunk of n
+    // pairs occupies m = ceil(n/64) blocks of 64 slots, pair L of the chunk
+    // going to slot (L % m) * 64 + L / m.  So lane l of a wavefront walking
+    // the chunk block by block meets pairs l*m, l*m+1, ... (one k for long
+    // runs; the 64 lanes sit on different rows), while each block of 64 slots
+    // is still read as 8 coalesced SoA granules.  Unused slots hold P = 0 and
+    // are never read.
+    std::vector<int64_t> phys;  // slot -> file index, -1 for padding
+    std::vector<int64_t> cbeg;
+    std::vector<int32_t> clen;
+    std::vector<int2> cij;
+    int64_t nslots = n_pairs;
+    if (R) {
+        const int64_t nbk = (nk + R - 1) / R;
+        const int64_t cmax = std::min<int64_t>(4096, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64;
+        std::vector<int64_t> tc(nbk * nbk + 1, 0);
+        for (int64_t i = 0; i < n_pairs; i++) {
+            int64_t f = KK(i);
+            tc[(int64_t)(k1st[f] / R) * nbk + (int64_t)(k2nd[f] / R) + 1]++;
+        }
+        for (int64_t b = 0; b < nbk * nbk; b++) tc[b + 1] += tc[b];
+        std::vector<int64_t> byt(n_pairs);  // file indices grouped by tile, (k, k') order inside
+        {
+            std::vector<int64_t> cur(tc.begin(), tc.end() - 1);
+            for (int64_t i = 0; i < n_pairs; i++) {
+                int64_t f = KK(i);
+                byt[cur[(int64_t)(k1st[f] / R) * nbk + (int64_t)(k2nd[f] / R)]++] = f;
+            }
+        }
+        nslots = 0;
+        for (int64_t t = 0; t < nbk * nbk; t++)
+            for (int64_t a = tc[t]; a < tc[t + 1]; a += cmax) {
+                int64_t n = std::min<int64_t>(cmax, tc[t + 1] - a);
+                cbeg.push_back(nslots);
+                clen.push_back((int32_t)n);
+                cij.push_back(make_int2((int)(t / nbk), (int)(t % nbk)));
+                nslots += (n + 63) / 64 * 64;
+            }
+        phys.assign(nslots, -1);
+        int64_t c = 0;
+        for (int64_t t = 0; t < nbk * nbk; t++)
+            for (int64_t a = tc[t]; a < tc[t + 1]; a += cmax, c++) {
+                const int64_t n = clen[c], m = (n + 63) / 64;
+                for (int64_t L = 0; L < n; L++) phys[cbeg[c] + (L % m) * 64 + L / m] = byt[a + L];
+            }
+    }
+    h->nslots = nslots;
+    auto PH = [&](int64_t pos) { return R ? phys[pos] : KK(pos); };
+    const bool identity = !R && sorted;
+    std::vector<int64_t> inv;
+    if (!identity) {
+        inv.resize(n_pairs);
+        for (int64_t pos = 0; pos < nslots; pos++)
+            if (PH(pos) >= 0) inv[PH(pos)] = pos;
+    }
+    auto INV = [&](int64_t f) { return identity ? f : inv[f]; };
+    h->identity_order = identity;
+    // forward lists (CSR by k) and reverse lists (by k', then k)
+    std::vector<int64_t> fwdpos(n_pairs);
+    std::vector<int32_t> col(n_pairs), k1p(nslots), k2p(nslots);
     std::vector<int64_t> rcnt(nk + 1, 0);
-    for (int64_t pos = 0; pos < n_pairs; pos++) {
-        int64_t f = F(pos);
-        col[pos] = (int32_t)k2nd[f];
-        k1s[pos] = (int32_t)k1st[f];
+    for (int64_t i = 0; i < n_pairs; i++) {
+        int64_t f = KK(i);
+        fwdpos[i] = INV(f);
+        col[i] = (int32_t)k2nd[f];
         if (k1st[f] < k2nd[f]) rcnt[k2nd[f] + 1]++;
+    }
+    for (int64_t pos = 0; pos < nslots; pos++) {
+        int64_t f = PH(pos);
+        k1p[pos] = f >= 0 ? (int32_t)k1st[f] : -1;
+        k2p[pos] = f >= 0 ? (int32_t)k2nd[f] : -1;
     }
     for (int64_t k = 0; k < nk; k++) rcnt[k + 1] += rcnt[k];
     h->nrev = rcnt[nk];
@@ -431,27 +793,39 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     std::vector<int32_t> revcol(h->nrev);
     {
         std::vector<int64_t> cur(rcnt.begin(), rcnt.end() - 1);
-        for (int64_t pos = 0; pos < n_pairs; pos++)
-            if (k1s[pos] < col[pos]) {
-                int64_t c = cur[col[pos]]++;
-                revpos[c] = pos;
-                revcol[c] = k1s[pos];
+        for (int64_t i = 0; i < n_pairs; i++) {
+            int64_t f = KK(i);
+            if (k1st[f] < k2nd[f]) {
+                int64_t c = cur[k2nd[f]]++;
+                revpos[c] = INV(f);
+                revcol[c] = (int32_t)k1st[f];
             }
+        }
     }
+    std::vector<uint16_t> pk;
+    if (R) {
+        pk.assign(nslots, 0);  // padding slots: (0, 0), never read
+        for (int64_t pos = 0; pos < nslots; pos++)
+            if (k1p[pos] >= 0) pk[pos] = (uint16_t)((k1p[pos] % R) | ((k2p[pos] % R) << 8));
+    }
+    h->nchunks = (int64_t)cbeg.size();
     int rc;
     const size_t ns = (size_t)nk * h->nb2;
     if ((rc = dalloc_t(&h->d_e, (size_t)nk * nb, "e")) || (rc = dalloc_t(&h->d_edm, (size_t)nk * nb, "e_dm")) ||
-        (rc = dalloc_t(&h->d_P, (size_t)n_pairs * 2 * h->nb4, "P")) ||
+        (rc = dalloc_t(&h->d_P, (size_t)nslots * 2 * h->nb4, "P")) ||
         (rc = dalloc_t(&h->d_rowptr, nk + 1, "rowptr")) || (rc = dalloc_t(&h->d_revptr, nk + 1, "revptr")) ||
         (rc = dalloc_t(&h->d_revpos, h->nrev, "revpos")) || (rc = dalloc_t(&h->d_revcol, h->nrev, "revcol")) ||
-        (rc = dalloc_t(&h->d_col, n_pairs, "col")) || (rc = dalloc_t(&h->d_k1, n_pairs, "k1")) ||
+        (rc = dalloc_t(&h->d_col, n_pairs, "col")) || (rc = dalloc_t(&h->d_fwdpos, n_pairs, "fwdpos")) ||
+        (rc = dalloc_t(&h->d_k1, nslots, "k1")) || (rc = dalloc_t(&h->d_k2, nslots, "k2")) ||
         (rc = dalloc_t(&h->d_rhoS, ns, "rhoS")) || (rc = dalloc_t(&h->d_rho1S, ns, "rho1S")) ||
         (rc = dalloc_t(&h->d_out, ns, "out")) || (rc = dalloc_t(&h->d_in, ns, "in")) ||
-        (rc = dalloc_t(&h->d_in1, ns, "in1")) || (rc = dalloc_t(&h->d_clk, 1, "clock")))
+        (rc = dalloc_t(&h->d_in1, ns, "in1")) || (rc = dalloc_t(&h->d_clk, 1, "clock")) ||
+        (rc = dalloc_t(&h->d_acc_rows, (size_t)nk * 4 * h->nb2, "row accumulator")))
         return rc;
-    if (!sorted) {
-        std::vector<int64_t> inv(n_pairs);
-        for (int64_t pos = 0; pos < n_pairs; pos++) inv[order[pos]] = pos;
+    if (R && ((rc = dalloc_t(&h->d_pk, nslots, "tile offsets")) || (rc = dalloc_t(&h->d_cbeg, h->nchunks, "chunks")) ||
+              (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks"))))
+        return rc;
+    if (!identity) {
         if ((rc = dalloc_t(&h->d_inv, n_pairs, "inv"))) return rc;
         DMD_HIP(hipMemcpy(h->d_inv, inv.data(), n_pairs * 8, hipMemcpyHostToDevice));
     }
@@ -465,10 +839,20 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     }
     if (n_pairs) {
         DMD_HIP(hipMemcpy(h->d_col, col.data(), n_pairs * 4, hipMemcpyHostToDevice));
-        DMD_HIP(hipMemcpy(h->d_k1, k1s.data(), n_pairs * 4, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_fwdpos, fwdpos.data(), n_pairs * 8, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_k1, k1p.data(), nslots * 4, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_k2, k2p.data(), nslots * 4, hipMemcpyHostToDevice));
     }
-    DMD_HIP(hipMemset(h->d_P, 0, (size_t)std::max<int64_t>(n_pairs, 1) * 2 * h->nb4 * sizeof(double2)));
+    if (R && n_pairs) {
+        DMD_HIP(hipMemcpy(h->d_pk, pk.data(), nslots * 2, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_cbeg, cbeg.data(), h->nchunks * 8, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_clen, clen.data(), h->nchunks * 4, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_cij, cij.data(), h->nchunks * 8, hipMemcpyHostToDevice));
+    }
+    DMD_HIP(hipMemset(h->d_P, 0, (size_t)std::max<int64_t>(nslots, 1) * 2 * h->nb4 * sizeof(double2)));
+    h->soa = (nb == 2 && R) ? 1 : 0;  // the default kernel (tiles) streams the SoA granules
     for (double2 *b : {h->d_rhoS, h->d_rho1S, h->d_out, h->d_in, h->d_in1}) DMD_HIP(hipMemset(b, 0, ns * sizeof(double2)));
+    DMD_HIP(hipMemset(h->d_acc_rows, 0, (size_t)nk * 4 * h->nb2 * sizeof(double)));
     DMD_HIP(hipDeviceSynchronize());
     *out = h.release();
     return DMD_OK;
@@ -513,7 +897,7 @@ static int put_P(dmd_scatt *h, const double *P1, const double *P2, double scale,
         }
         int64_t tot = n * 2 * (int64_t)per;
         hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, f0, h->nb4, a, b, h->d_inv, scale,
-                           add, h->d_P);
+                           add, h->soa, h->d_P);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
         if (!dev1 || !dev2) {
@@ -548,7 +932,7 @@ extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, 
     if ((rc = dalloc_t(&s2, count * per, "download"))) { dfree(s1); return rc; }
     int64_t tot = count * 2 * (int64_t)per;
     hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, count, first, h->nb4, h->d_P, h->d_inv,
-                       s1, s2);
+                       h->soa, s1, s2);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(P1, s1, count * per * 16, hipMemcpyDefault, h->st);
     if (e == hipSuccess) e = hipMemcpyAsync(P2, s2, count * per * 16, hipMemcpyDefault, h->st);
@@ -563,12 +947,12 @@ extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
     if (!h || !p) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: null argument");
     if (p->nb != h->nb || p->nk != h->nk) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: params do not match the handle");
     if (h->np == 0) { h->has_P = true; return DMD_OK; }
-    const unsigned g = nblk(h->np, 64);
+    const unsigned g = nblk(h->nslots, 64);
     switch (h->nb) {
-    case 1: hipLaunchKernelGGL(k_generate_P<1>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
-    case 2: hipLaunchKernelGGL(k_generate_P<2>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
-    case 3: hipLaunchKernelGGL(k_generate_P<3>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
-    case 4: hipLaunchKernelGGL(k_generate_P<4>, dim3(g), dim3(64), 0, h->st, h->np, *p, h->d_k1, h->d_col, h->d_e, h->d_P); break;
+    case 1: hipLaunchKernelGGL(k_generate_P<1>, dim3(g), dim3(64), 0, h->st, h->nslots, *p, h->d_k1, h->d_k2, h->d_e, h->soa, h->d_P); break;
+    case 2: hipLaunchKernelGGL(k_generate_P<2>, dim3(g), dim3(64), 0, h->st, h->nslots, *p, h->d_k1, h->d_k2, h->d_e, h->soa, h->d_P); break;
+    case 3: hipLaunchKernelGGL(k_generate_P<3>, dim3(g), dim3(64), 0, h->st, h->nslots, *p, h->d_k1, h->d_k2, h->d_e, h->soa, h->d_P); break;
+    case 4: hipLaunchKernelGGL(k_generate_P<4>, dim3(g), dim3(64), 0, h->st, h->nslots, *p, h->d_k1, h->d_k2, h->d_e, h->soa, h->d_P); break;
     }
     DMD_HIP(hipGetLastError());
     DMD_HIP(hipStreamSynchronize(h->st));
@@ -584,6 +968,18 @@ extern "C" int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int 
     h->expt = expt ? 1 : 0;
     h->schrodinger = schrodinger ? 1 : 0;
     h->kernel = kernel;
+    // the tile kernel streams SoA granules, the row kernels pair-contiguous P
+    const int want_soa = contract_kind(h) == DMD_KERNEL_TILES ? 1 : 0;
+    if (want_soa != h->soa) {
+        const int64_t ngran = h->nslots / 8;
+        const int ep = 2 * h->nb4;
+        if (ngran > 0)
+            hipLaunchKernelGGL(k_relayout, dim3((unsigned)ngran), dim3(256), 8 * ep * sizeof(double2), h->st, ngran, ep,
+                               want_soa, h->d_P);
+        DMD_HIP(hipGetLastError());
+        DMD_HIP(hipStreamSynchronize(h->st));
+        h->soa = want_soa;
+    }
     return DMD_OK;
 }
 
@@ -624,23 +1020,78 @@ static void launch_rows_nb(dmd_scatt *h, bool dm1, double a, double2 *out)
     dim3 g(nblk(h->nk, 4)), b(256);
     const double2 *eq = h->has_eq ? h->d_eq : nullptr;
     if (dm1)
-        hipLaunchKernelGGL((k_rows<NB, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_revptr,
+        hipLaunchKernelGGL((k_rows<NB, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, h->d_revptr,
                            h->d_revpos, h->d_revcol, h->d_rhoS, h->d_rho1S, h->d_e, h->d_clk, a, h->expt,
                            h->schrodinger, h->prefac, eq, out);
     else
-        hipLaunchKernelGGL((k_rows<NB, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_revptr,
+        hipLaunchKernelGGL((k_rows<NB, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, h->d_revptr,
                            h->d_revpos, h->d_revcol, h->d_rhoS, h->d_rho1S, h->d_e, h->d_clk, a, h->expt,
                            h->schrodinger, h->prefac, eq, out);
+}
+
+template <int NB>
+static void launch_single_nb(dmd_scatt *h, bool dm1, double a, double2 *out)
+{
+    const double2 *eq = h->has_eq ? h->d_eq : nullptr;
+    dim3 g(nblk(h->nk, 4)), b(256), ga(nblk(h->nk, 256));
+    if (dm1) {
+        hipLaunchKernelGGL((k_fwd_atomic<NB, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, h->d_rhoS,
+                           h->d_rho1S, h->d_acc_rows);
+        hipLaunchKernelGGL((k_assemble<NB, true>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S, h->d_e,
+                           h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
+    } else {
+        hipLaunchKernelGGL((k_fwd_atomic<NB, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, h->d_rhoS,
+                           h->d_rho1S, h->d_acc_rows);
+        hipLaunchKernelGGL((k_assemble<NB, false>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S,
+                           h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
+    }
+}
+
+static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
+{
+    const double2 *eq = h->has_eq ? h->d_eq : nullptr;
+    dim3 b(256), ga(nblk(h->nk, 256));
+    if (h->nchunks > 0) {
+        if (dm1)
+            hipLaunchKernelGGL(k_tiles_soa<true>, dim3((unsigned)h->nchunks), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+        else
+            hipLaunchKernelGGL(k_tiles_soa<false>, dim3((unsigned)h->nchunks), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+    }
+    if (dm1)
+        hipLaunchKernelGGL((k_assemble<2, true>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S, h->d_e,
+                           h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
+    else
+        hipLaunchKernelGGL((k_assemble<2, false>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S,
+                           h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
+}
+
+// Which contraction kernel runs.  AUTO: the tile kernel for nb = 2, the
+// per-pair atomic kernel for nb = 1, the row kernel for nb >= 3.
+int contract_kind(const dmd_scatt *h)
+{
+    if (h->kernel == DMD_KERNEL_ROWS || h->nb > 2) return DMD_KERNEL_ROWS;
+    if (h->kernel == DMD_KERNEL_SINGLE_READ) return DMD_KERNEL_SINGLE_READ;
+    return h->nb == 2 && h->tile_R ? DMD_KERNEL_TILES : DMD_KERNEL_SINGLE_READ;
 }
 
 // the contraction + epilogue on the current rhoS (and rho1S when dm1)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
-    switch (h->nb) {
-    case 1: launch_rows_nb<1>(h, dm1, a, out); break;
-    case 2: launch_rows_nb<2>(h, dm1, a, out); break;
-    case 3: launch_rows_nb<3>(h, dm1, a, out); break;
-    case 4: launch_rows_nb<4>(h, dm1, a, out); break;
+    const int kind = contract_kind(h);
+    if (kind == DMD_KERNEL_TILES) {
+        launch_tiles(h, dm1, a, out);
+    } else if (kind == DMD_KERNEL_SINGLE_READ) {
+        if (h->nb == 1) launch_single_nb<1>(h, dm1, a, out);
+        else launch_single_nb<2>(h, dm1, a, out);
+    } else {
+        switch (h->nb) {
+        case 1: launch_rows_nb<1>(h, dm1, a, out); break;
+        case 2: launch_rows_nb<2>(h, dm1, a, out); break;
+        case 3: launch_rows_nb<3>(h, dm1, a, out); break;
+        case 4: launch_rows_nb<4>(h, dm1, a, out); break;
+        }
     }
     DMD_HIP(hipGetLastError());
     return DMD_OK;

@@ -46,9 +46,13 @@ extern "C" {
 #define DMD_PREC_BF16 2 /* bf16 re/im P with a per-pair FP32 scale              */
 
 /* contraction kernel */
-#define DMD_KERNEL_AUTO 0   /* fastest available for the shape                  */
-#define DMD_KERNEL_ROWS 1   /* deterministic: every row gathers its own pairs     */
-#define DMD_KERNEL_TILES 2  /* single P read: 2-D tiles with LDS accumulation   */
+#define DMD_KERNEL_AUTO 0        /* fastest available for the shape               */
+#define DMD_KERNEL_ROWS 1        /* deterministic: every row gathers its own pairs  */
+#define DMD_KERNEL_SINGLE_READ 2 /* P read once; the reverse direction is summed  */
+                                 /* with FP64 atomics (order not fixed), nb <= 2   */
+#define DMD_KERNEL_TILES 3       /* P read once; both directions summed in LDS per  */
+                                 /* (k, k') row-block tile, one atomic flush per   */
+                                 /* tile chunk (order not fixed), nb = 2           */
 
 typedef struct dmd_scatt dmd_scatt;
 

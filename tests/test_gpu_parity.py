@@ -32,23 +32,74 @@ def _case(nk, nb, npart, seed, diag=False, structured=True):
     return rng, k1, k2, e, P1, P2, rho
 
 
+KERNELS = [1, 2, 3]  # DMD_KERNEL_ROWS (deterministic), _SINGLE_READ, _TILES
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("nb", [1, 2, 3, 4])
 @pytest.mark.parametrize("t_fs", [0.0, 37.0, 2500.0])
-def test_eval_matches_oracle(nb, t_fs):
+def test_eval_matches_oracle(nb, t_fs, kernel):
     scatt, oracle = _mods()
-    print(f"case eval nb={nb} t={t_fs}fs", flush=True)
+    print(f"case eval nb={nb} t={t_fs}fs kernel={kernel}", flush=True)
     rng, k1, k2, e, P1, P2, rho = _case(40, nb, 5, 10 + nb, diag=True)
     t = t_fs * FS
     prefac = 2 * np.pi / 40
     ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=prefac)
     with scatt.ScatteringOperator(40, nb, k1, k2, e, nk_full=40) as op:
         op.upload_P(P1, P2)
+        op.set_flags(kernel=kernel)
         got = op.eval(t, rho)
+        again = op.eval(t, rho)  # the accumulator is re-zeroed by every evaluation
+    assert rel_err(got, ref) < TOL
+    assert rel_err(again, got) < 1e-14
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("nb", [1, 2])
+def test_eval_many_tiles_unsorted(kernel, nb):
+    """nk spans several 256-row tiles, tiles split into several chunks, pairs
+    given in shuffled order (exercises the tile layout and the index lists)."""
+    scatt, oracle = _mods()
+    print(f"case tiles nb={nb} kernel={kernel}", flush=True)
+    rng, k1, k2, e, P1, P2, rho = _case(700, nb, 9, 40 + nb, diag=True)
+    perm = rng.permutation(len(k1))
+    k1, k2, P1, P2 = k1[perm], k2[perm], P1[perm], P2[perm]
+    t = 211 * FS
+    ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=2 * np.pi / 700)
+    with scatt.ScatteringOperator(700, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        op.set_flags(kernel=kernel)
+        got = op.eval(t, rho)
+        D1, D2 = op.download_P()
+    assert np.array_equal(D1, P1) and np.array_equal(D2, P2)
     assert rel_err(got, ref) < TOL
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_eval_edge_cases(kernel):
+    """rows without pairs, a k with only reverse pairs, diagonal-only pairs, one pair."""
+    scatt, oracle = _mods()
+    rng = np.random.default_rng(31)
+    nb, nk = 2, 9
+    for k1, k2 in ([[2], [7]], [[0, 3, 3, 8], [0, 3, 5, 8]], [[0, 1, 1, 4], [8, 2, 8, 8]]):
+        k1 = np.array(k1, np.uint64)
+        k2 = np.array(k2, np.uint64)
+        e = random_energies(nk, nb, rng)
+        P1, P2 = random_P(len(k1), nb, rng)
+        rho = random_rho(nk, nb, rng)
+        ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=9 * FS, prefac=0.5)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e, nk_full=4 * np.pi) as op:
+            op.upload_P(P1, P2)
+            op.set_flags(kernel=kernel)
+            got = op.eval(9 * FS, rho)
+        assert rel_err(got, ref) < TOL
+        empty = np.abs(ref).sum(axis=1) == 0
+        assert np.all(got[empty] == 0)
+
+
 @pytest.mark.parametrize("nb", [2, 4])
-def test_eval_unstructured_explicit_dm1_eq_unsorted(nb):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_eval_unstructured_explicit_dm1_eq_unsorted(nb, kernel):
     scatt, oracle = _mods()
     print(f"case eval-variants nb={nb}", flush=True)
     rng, k1, k2, e, P1, P2, rho = _case(33, nb, 4, 20 + nb, diag=True, structured=False)
@@ -62,14 +113,14 @@ def test_eval_unstructured_explicit_dm1_eq_unsorted(nb):
         op.upload_P(P1, P2)
         for expt in (True, False):
             for use_eq in (False, True):
-                op.set_flags(expt=expt)
+                op.set_flags(expt=expt, kernel=kernel)
                 op.set_ddmdt_eq(eq if use_eq else None)
                 ref = oracle.evolve(k1, k2, P1, P2, e, rho, dm1, t=t, prefac=prefac, expt=expt,
                                     ddmdt_eq=eq if use_eq else None)
                 got = op.eval(t, rho, dm1)
                 assert rel_err(got, ref) < TOL, (expt, use_eq)
         # Schroedinger picture: eq subtracted without phase, no P phases
-        op.set_flags(expt=False, schrodinger=True)
+        op.set_flags(expt=False, schrodinger=True, kernel=kernel)
         op.set_ddmdt_eq(eq)
         ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=prefac, expt=False, ddmdt_eq=eq, schrodinger=True)
         assert rel_err(op.eval(t, rho), ref) < TOL
@@ -184,6 +235,8 @@ def test_rk4_trajectory_matches_oracle(with_H):
 
 
 def test_rk4_graph_replays_match_eager(monkeypatch):
+    """Graph replays equal the eager stream: bit for bit with the deterministic
+    row kernel, to rounding with the single-read kernel (atomic order)."""
     scatt, oracle = _mods()
     nk, nb = 26, 2
     rng = np.random.default_rng(13)
@@ -193,13 +246,15 @@ def test_rk4_graph_replays_match_eager(monkeypatch):
     rho0 = random_rho(nk, nb, rng)
     with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
         op.upload_P(P1, P2)
-        monkeypatch.setenv("DMD_GRAPH", "0")
-        eager, _ = op.integrate_rk4(rho0, 0.0, FS, 12)
-        monkeypatch.setenv("DMD_GRAPH", "1")
-        for _ in range(3):  # several replays, eager work in between
-            g, _ = op.integrate_rk4(rho0, 0.0, FS, 12)
-            np.testing.assert_array_equal(g, eager)
-            op.eval(0.0, rho0)
+        for kernel, tol in ((1, 0.0), (2, 1e-13), (3, 1e-13)):
+            op.set_flags(kernel=kernel)
+            monkeypatch.setenv("DMD_GRAPH", "0")
+            eager, _ = op.integrate_rk4(rho0, 0.0, FS, 12)
+            monkeypatch.setenv("DMD_GRAPH", "1")
+            for _ in range(3):  # several replays, eager work in between
+                g, _ = op.integrate_rk4(rho0, 0.0, FS, 12)
+                assert rel_err(g, eager) <= tol
+                op.eval(0.0, rho0)
 
 
 def test_window_rk4_with_coherent_matches_oracle():
@@ -209,6 +264,7 @@ def test_window_rk4_with_coherent_matches_oracle():
     k1, k2 = random_pairs(nk, 4, rng)
     e_dm = random_energies(nk, nbd, rng)
     P1, P2 = random_P(len(k1), nb, rng)
+    P1, P2 = 0.1 * P1, 0.1 * P2  # keeps dt * rate well inside RK4's stability region
     rho0 = random_rho(nk, nbd, rng)
     H = random_rho(nk, nbd, rng, scale=1e-4, diag=0.0)
     s = random_rho(nk, nbd, rng, scale=0.5, diag=0.0)
@@ -237,3 +293,37 @@ def test_generate_on_device_matches_host_generator():
         D1, D2 = op.download_P()
     # same formula on host and device; transcendental functions may differ by an ulp
     assert rel_err(D1, P1) < 1e-13 and rel_err(D2, P2) < 1e-13
+
+
+def test_kernel_switch_relayouts_P_in_place():
+    """Switching between the tile kernel (SoA granules) and the row kernels
+    (pair-contiguous P) transposes P on the device; results and the
+    downloaded P stay the same through several switches."""
+    scatt, oracle = _mods()
+    rng, k1, k2, e, P1, P2, rho = _case(600, 2, 7, 77, diag=True)
+    t = 55 * FS
+    ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=2 * np.pi / 600)
+    with scatt.ScatteringOperator(600, 2, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        for kernel in (3, 1, 2, 3, 0, 1, 3):
+            op.set_flags(kernel=kernel)
+            assert rel_err(op.eval(t, rho), ref) < TOL, kernel
+            D1, D2 = op.download_P()
+            assert np.array_equal(D1, P1) and np.array_equal(D2, P2), kernel
+
+
+def test_generate_P_in_both_layouts():
+    scatt, oracle = _mods()
+    p = scatt.synth_params(0)
+    p.nk, p.partners = 600, 6
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    out = []
+    for kernel in (3, 1):
+        with scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e) as op:
+            op.set_flags(kernel=kernel)
+            op.generate_P(p)
+            out.append(op.download_P())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    P1, P2 = scatt.synth_P(p, e, k1, k2)
+    assert rel_err(out[0][0], P1) < 1e-13 and rel_err(out[0][1], P2) < 1e-13
