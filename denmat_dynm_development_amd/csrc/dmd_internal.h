@@ -29,6 +29,8 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 // rows per tile of the tile kernel (k mod R and k' mod R fit in 8 bits)
 #define DMD_TILE_R 256
+// chunks of one row block a tile-kernel workgroup takes in sequence
+#define DMD_TILE_GROUP 1
 
 // Device clock: t(a) = (t0 + step*dt) + a*dt, the same expression as the RK4
 // stage times of the oracle's driver.
@@ -92,7 +94,8 @@ struct dmd_scatt {
     int64_t *d_cbeg = nullptr;           // tile chunks: first pair, length, tile (I, J)
     int32_t *d_clen = nullptr;
     int2 *d_cij = nullptr;
-    int64_t nchunks = 0;
+    int32_t *d_gbeg = nullptr;           // workgroup -> first chunk of its group
+    int64_t nchunks = 0, ngroups = 0;
     int64_t nslots = 0;                  // physical pair slots (stored pairs + tile padding)
     int tile_R = 0;
     int soa = 0;                         // P stored as SoA granules of 8 pairs (tile kernel)

@@ -398,78 +398,124 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                                                    const uint16_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
+                                                   const int32_t *__restrict__ gbeg,
                                                    const double2 *__restrict__ rhoS,
                                                    const double2 *__restrict__ rho1S, double *__restrict__ acc)
 {
     constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R, RW = 16;  // RW: doubles per row [Pdm | dm1P]
-    constexpr int RS = RW + 1;  // LDS row stride: the odd pad spreads rows over the banks
     constexpr int W = 4;
-    __shared__ double accA[R * RS], accB[R * RS];
-    const int c = blockIdx.x;
-    const int64_t b = cbeg[c];
-    const int n = clen[c];
-    const int m = (n + 63) / 64;  // blocks of 64 slots; lane l owns chunk pairs l*m .. l*m+m-1
-    const int2 ij = cij[c];
-    const int I = ij.x, J = ij.y;
-    double *aJ = (I == J) ? accA : accB;
-    for (int i = threadIdx.x; i < R * RS; i += blockDim.x) { accA[i] = 0.0; accB[i] = 0.0; }
-    __syncthreads();
+    // LDS: row accumulators of blocks I and J (double d of row r at
+    // r*RW + (d ^ (r & 15)): the swizzle spreads rows over the banks) and the
+    // rho rows of block J, gathered per pair from LDS instead of L2.
+    __shared__ double accA[R * RW], accB[R * RW];
+    __shared__ double2 rhoJ[R * NB2];
+    // a workgroup takes a group of consecutive chunks of one row block I
+    // (tiles (I, J1), (I, J2), ...): block I's accumulators stay in LDS for
+    // the whole group, block J's are flushed at the end of every chunk
+    const int c0 = gbeg[blockIdx.x], c1 = gbeg[blockIdx.x + 1];
+    const int I = cij[c0].x;
+    const int64_t I0 = (int64_t)I * R;
+    for (int i = threadIdx.x; i < R * RW; i += blockDim.x) { accA[i] = 0.0; accB[i] = 0.0; }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t I0 = (int64_t)I * R, J0 = (int64_t)J * R;
-    const int mq = (m + W - 1) / W;
-    const int i0 = wv * mq, i1 = min(m, i0 + mq);
-    for (int it = i0; it < i1; it++) {
-        const bool ok = lane * m + it < n;
-        const int64_t s = b + (int64_t)it * 64 + lane;
-        const unsigned code = ok ? pk[s] : 0u;
-        const int lo1 = code & 255u, lo2 = code >> 8;
-        const int64_t k = I0 + lo1, kp = J0 + lo2;
-        const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
-        double2 p[EP];
-#pragma unroll
-        for (int el = 0; el < EP; el++) p[el] = ok ? g[el * 8] : make_double2(0, 0);
-        double2 rk[NB2], r1k[NB2], rkp[NB2], r1kp[NB2];
-#pragma unroll
-        for (int q = 0; q < NB2; q++) {
-            rk[q] = ok ? rhoS[k * NB2 + q] : make_double2(0, 0);
-            rkp[q] = ok ? rhoS[kp * NB2 + q] : make_double2(0, 0);
-            r1k[q] = ok ? rho_elem<NB, DM1>(rhoS, rho1S, k, q, true) : make_double2(0, 0);
-            r1kp[q] = ok ? rho_elem<NB, DM1>(rhoS, rho1S, kp, q, true) : make_double2(0, 0);
+    for (int c = c0; c < c1; c++) {
+        const int64_t b = cbeg[c];
+        const int n = clen[c];
+        const int m = (n + 63) / 64;  // blocks of 64 slots; lane l owns chunk pairs l*m .. l*m+m-1
+        const int J = cij[c].y;
+        const int64_t J0 = (int64_t)J * R;
+        double *aJ = (I == J) ? accA : accB;
+        if (!DM1) {
+            for (int i = threadIdx.x; i < R * NB2; i += blockDim.x)
+                rhoJ[i] = J0 * NB2 + i < nk * NB2 ? rhoS[J0 * NB2 + i] : make_double2(0, 0);
         }
-        const bool rev = ok && k < kp;
+        __syncthreads();
+        const int mq = (m + W - 1) / W;
+        const int i0 = wv * mq, i1 = min(m, i0 + mq);
+        unsigned code_next = (i0 < i1 && lane * m + i0 < n) ? pk[b + (int64_t)i0 * 64 + lane] : 0u;
+        // rho of the lane's current row k: a lane walks a run of one k, so it is
+        // reloaded only where k changes
+        int lo1_cur = -1;
+        double2 rk[NB2], r1k[NB2];
 #pragma unroll
-        for (int i = 0; i < NB2; i++) {
-            double2 f1 = make_double2(0, 0), f2 = make_double2(0, 0), g1 = make_double2(0, 0), g2 = make_double2(0, 0);
+        for (int q = 0; q < NB2; q++) { rk[q] = make_double2(0, 0); r1k[q] = make_double2(0, 0); }
+        for (int it = i0; it < i1; it++) {
+            const bool ok = lane * m + it < n;
+            const int64_t s = b + (int64_t)it * 64 + lane;
+            const unsigned code = code_next;
+            code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? pk[s + 64] : 0u;
+            const int lo1 = code & 255u, lo2 = code >> 8;
+            const int64_t k = I0 + lo1, kp = J0 + lo2;
+            const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
+            double2 p[EP];
 #pragma unroll
-            for (int q = 0; q < NB2; q++) {
-                f1 = cadd(f1, cmul(p[i * NB2 + q], rkp[q]));             // Pdm_k[i]
-                f2 = cadd(f2, cmul(r1kp[q], p[NB4 + q * NB2 + i]));      // dm1P_k[i]
-                g1 = cadd(g1, cmulc(rk[q], p[NB4 + i * NB2 + q]));       // Pdm_k'[i]
-                g2 = cadd(g2, cmulc(r1k[q], p[q * NB2 + i]));            // dm1P_k'[i]
+            for (int el = 0; el < EP; el++) p[el] = ok ? g[el * 8] : make_double2(0, 0);
+            double2 rkp[NB2], r1kp[NB2];
+#pragma unroll
+            for (int q = 0; q < NB2; q++)
+                rkp[q] = !ok ? make_double2(0, 0) : (DM1 ? rhoS[kp * NB2 + q] : rhoJ[lo2 * NB2 + q]);
+            if (ok && lo1 != lo1_cur) {
+#pragma unroll
+                for (int q = 0; q < NB2; q++) rk[q] = rhoS[k * NB2 + q];
+                if (DM1) {
+#pragma unroll
+                    for (int q = 0; q < NB2; q++) r1k[q] = rho1S[k * NB2 + q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NB2; q++)
+                        r1k[q] = make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rk[q].x, -rk[q].y);
+                }
+                lo1_cur = lo1;
             }
-            if (ok) {
-                atomicAdd(&accA[lo1 * RS + 2 * i], f1.x);
-                atomicAdd(&accA[lo1 * RS + 2 * i + 1], f1.y);
-                atomicAdd(&accA[lo1 * RS + 2 * (NB2 + i)], f2.x);
-                atomicAdd(&accA[lo1 * RS + 2 * (NB2 + i) + 1], f2.y);
+            if (DM1) {
+#pragma unroll
+                for (int q = 0; q < NB2; q++) r1kp[q] = ok ? rho1S[kp * NB2 + q] : make_double2(0, 0);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NB2; q++)
+                    r1kp[q] = make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rkp[q].x, -rkp[q].y);
             }
-            if (rev) {
-                atomicAdd(&aJ[lo2 * RS + 2 * i], g1.x);
-                atomicAdd(&aJ[lo2 * RS + 2 * i + 1], g1.y);
-                atomicAdd(&aJ[lo2 * RS + 2 * (NB2 + i)], g2.x);
-                atomicAdd(&aJ[lo2 * RS + 2 * (NB2 + i) + 1], g2.y);
+            const bool rev = ok && k < kp;
+#pragma unroll
+            for (int i = 0; i < NB2; i++) {
+                double2 f1 = make_double2(0, 0), f2 = make_double2(0, 0), g1 = make_double2(0, 0),
+                        g2 = make_double2(0, 0);
+#pragma unroll
+                for (int q = 0; q < NB2; q++) {
+                    f1 = cadd(f1, cmul(p[i * NB2 + q], rkp[q]));         // Pdm_k[i]
+                    f2 = cadd(f2, cmul(r1kp[q], p[NB4 + q * NB2 + i]));  // dm1P_k[i]
+                    g1 = cadd(g1, cmulc(rk[q], p[NB4 + i * NB2 + q]));   // Pdm_k'[i]
+                    g2 = cadd(g2, cmulc(r1k[q], p[q * NB2 + i]));        // dm1P_k'[i]
+                }
+                const int sw1 = lo1 & 15, sw2 = lo2 & 15;
+                if (ok) {
+                    atomicAdd(&accA[lo1 * RW + ((2 * i) ^ sw1)], f1.x);
+                    atomicAdd(&accA[lo1 * RW + ((2 * i + 1) ^ sw1)], f1.y);
+                    atomicAdd(&accA[lo1 * RW + ((2 * (NB2 + i)) ^ sw1)], f2.x);
+                    atomicAdd(&accA[lo1 * RW + ((2 * (NB2 + i) + 1) ^ sw1)], f2.y);
+                }
+                if (rev) {
+                    atomicAdd(&aJ[lo2 * RW + ((2 * i) ^ sw2)], g1.x);
+                    atomicAdd(&aJ[lo2 * RW + ((2 * i + 1) ^ sw2)], g1.y);
+                    atomicAdd(&aJ[lo2 * RW + ((2 * (NB2 + i)) ^ sw2)], g2.x);
+                    atomicAdd(&aJ[lo2 * RW + ((2 * (NB2 + i) + 1) ^ sw2)], g2.y);
+                }
+            }
+        }
+        __syncthreads();  // block J done (and rhoJ free for the next chunk)
+        if (I != J) {  // flush and clear block J
+            for (int i = threadIdx.x; i < R * RW; i += blockDim.x) {
+                const int row = i / RW, d = i - row * RW;
+                const double w = accB[row * RW + (d ^ (row & 15))];
+                accB[row * RW + (d ^ (row & 15))] = 0.0;
+                if (w != 0.0 && J0 + row < nk) unsafeAtomicAdd(acc + (J0 + row) * RW + d, w);
             }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < R * RW; i += blockDim.x) {
         const int row = i / RW, d = i - row * RW;
-        const double v = accA[row * RS + d];
+        const double v = accA[row * RW + (d ^ (row & 15))];
         if (v != 0.0 && I0 + row < nk) unsafeAtomicAdd(acc + (I0 + row) * RW + d, v);
-        if (I != J) {
-            const double w = accB[row * RS + d];
-            if (w != 0.0 && J0 + row < nk) unsafeAtomicAdd(acc + (J0 + row) * RW + d, w);
-        }
     }
 }
 
@@ -624,7 +670,7 @@ dmd_scatt::~dmd_scatt()
 {
     dfree(d_e); dfree(d_edm); dfree(d_P); dfree(d_inv); dfree(d_rowptr); dfree(d_revptr); dfree(d_revpos);
     dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_k2); dfree(d_fwdpos); dfree(d_pk); dfree(d_cbeg);
-    dfree(d_clen); dfree(d_cij); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
+    dfree(d_clen); dfree(d_cij); dfree(d_gbeg); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
     dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
     dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
@@ -802,6 +848,12 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
             }
         }
     }
+    std::vector<int32_t> gbeg;  // workgroup -> first chunk (groups of chunks of one row block)
+    for (int64_t c = 0; c < (int64_t)cbeg.size(); c++)
+        if (gbeg.empty() || cij[c].x != cij[gbeg.back()].x || c - gbeg.back() >= DMD_TILE_GROUP)
+            gbeg.push_back((int32_t)c);
+    gbeg.push_back((int32_t)cbeg.size());
+    h->ngroups = (int64_t)gbeg.size() - 1;
     std::vector<uint16_t> pk;
     if (R) {
         pk.assign(nslots, 0);  // padding slots: (0, 0), never read
@@ -823,7 +875,8 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         (rc = dalloc_t(&h->d_acc_rows, (size_t)nk * 4 * h->nb2, "row accumulator")))
         return rc;
     if (R && ((rc = dalloc_t(&h->d_pk, nslots, "tile offsets")) || (rc = dalloc_t(&h->d_cbeg, h->nchunks, "chunks")) ||
-              (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks"))))
+              (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks")) ||
+              (rc = dalloc_t(&h->d_gbeg, h->ngroups + 1, "chunk groups"))))
         return rc;
     if (!identity) {
         if ((rc = dalloc_t(&h->d_inv, n_pairs, "inv"))) return rc;
@@ -848,6 +901,7 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         DMD_HIP(hipMemcpy(h->d_cbeg, cbeg.data(), h->nchunks * 8, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_clen, clen.data(), h->nchunks * 4, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_cij, cij.data(), h->nchunks * 8, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_gbeg, gbeg.data(), (h->ngroups + 1) * 4, hipMemcpyHostToDevice));
     }
     DMD_HIP(hipMemset(h->d_P, 0, (size_t)std::max<int64_t>(nslots, 1) * 2 * h->nb4 * sizeof(double2)));
     h->soa = (nb == 2 && R) ? 1 : 0;  // the default kernel (tiles) streams the SoA granules
@@ -1051,13 +1105,13 @@ static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const double2 *eq = h->has_eq ? h->d_eq : nullptr;
     dim3 b(256), ga(nblk(h->nk, 256));
-    if (h->nchunks > 0) {
+    if (h->ngroups > 0) {
         if (dm1)
-            hipLaunchKernelGGL(k_tiles_soa<true>, dim3((unsigned)h->nchunks), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+            hipLaunchKernelGGL(k_tiles_soa<true>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
         else
-            hipLaunchKernelGGL(k_tiles_soa<false>, dim3((unsigned)h->nchunks), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+            hipLaunchKernelGGL(k_tiles_soa<false>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
     }
     if (dm1)
         hipLaunchKernelGGL((k_assemble<2, true>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S, h->d_e,
