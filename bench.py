@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the scattering operator runs on a single MI355X here "
@@ -133,7 +135,7 @@ def main():
     elif args.kernel == 2 or p.nb == 1:
         kind, kname = "single_read", f"k_fwd_atomic<{p.nb}> (+k_assemble)"
     else:
-        kind, kname = "tiles", "k_tiles<false> (+k_assemble)"
+        kind, kname = "tiles", "k_tiles_soa<false> (+k_assemble_h)"
     out = {
         "metric": "k-pair contractions/s",
         "value": value,
@@ -158,6 +160,13 @@ def main():
                      "kernel": kname,
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
     }
+    if args.copy_calib:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, e, k1, k2, rho0)

@@ -58,6 +58,22 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b)  // a * conj(b)
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// acc += a * b and acc += a * conj(b) as four FMAs each (the sum is
+// accumulated term by term rather than product first)
+__device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b)
+{
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.y, b.x, acc.y);
+    acc.y = fma(a.x, b.y, acc.y);
+}
+__device__ __forceinline__ void cfmac(double2 &acc, double2 a, double2 b)
+{
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.y, b.x, acc.y);
+    acc.y = fma(-a.x, b.y, acc.y);
+}
 __device__ __forceinline__ double2 ccis(double x)
 {
     double s, c;
@@ -106,6 +122,7 @@ struct dmd_scatt {
     double2 *d_H = nullptr, *d_s = nullptr;
     double *d_f = nullptr, *d_obs = nullptr, *d_part = nullptr;
     double *d_acc_rows = nullptr;  // nk x [Pdm | dm1P] accumulator of the single-read kernel
+    double *d_acc_herm = nullptr;  // nk x [H00, H11, Re H01, Im H01] accumulator of the tile kernel
     int obs_i0 = 0, obs_i1 = 0;
     double obs_prefac = 0;
     bool has_H = false, has_obs = false;

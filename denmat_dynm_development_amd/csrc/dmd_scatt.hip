@@ -393,6 +393,97 @@ __global__ __launch_bounds__(256) void k_fwd_atomic(int64_t nk, const double2 *_
 // with global FP64 atomics (~2R lines per chunk instead of one per pair).
 // Padding pairs carry P = 0 and add exact zeros.
 // ---------------------------------------------------------------------------
+// Flush of LDS accumulator rows (nb = 2).  Thread r of the block takes row r:
+// with Pdm, dm1P the row's partial sums from this chunk it forms
+// H = C + C^+, C = (1-rho) Pdm - dm1P rho (zhemm upper), the four doubles
+// [H00, H11, Re H01, Im H01].  C is linear in (Pdm, dm1P) for a fixed rho_k,
+// so summing the chunks' H gives the row's H (k_assemble_h scales it by
+// prefac/2).  The wave then regroups the values so every atomic instruction
+// covers 16 consecutive rows' 32-byte records (4 full 128-byte lines).
+template <bool DM1>
+__device__ __forceinline__ void herm_row(const double *__restrict__ accrow_base, int r, const double2 *rr,
+                                         const double2 *__restrict__ rho1S, int64_t k, double hv[4])
+{
+    constexpr int NB = 2, NB2 = 4, RW = 16;
+    double v[RW];
+#pragma unroll
+    for (int d = 0; d < RW; d++) v[d] = accrow_base[r * RW + (d ^ (r & 15))];
+    double2 pdm[NB2], d1p[NB2], r1[NB2];
+#pragma unroll
+    for (int q = 0; q < NB2; q++) {
+        pdm[q] = make_double2(v[2 * q], v[2 * q + 1]);
+        d1p[q] = make_double2(v[2 * (NB2 + q)], v[2 * (NB2 + q) + 1]);
+        r1[q] = DM1 ? rho1S[k * NB2 + q]
+                    : make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rr[q].x, -rr[q].y);
+    }
+    double2 c[NB2];
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            double2 x = make_double2(0, 0);
+#pragma unroll
+            for (int q = 0; q < NB; q++) x = cadd(x, cmul(herm_up<NB>(r1, i, q), pdm[q * NB + j]));
+#pragma unroll
+            for (int q = 0; q < NB; q++) x = csub(x, cmul(d1p[i * NB + q], herm_up<NB>(rr, q, j)));
+            c[i * NB + j] = x;
+        }
+    hv[0] = 2.0 * c[0].x;      // H00 = C00 + conj(C00)
+    hv[1] = 2.0 * c[3].x;      // H11
+    hv[2] = c[1].x + c[2].x;   // H01 = C01 + conj(C10)
+    hv[3] = c[1].y - c[2].y;
+}
+// Rows [row0, row0 + 64) of one wave: lane l holds hv of row row0 + l (zeros
+// for rows past nk); adds them into hacc[(base + row) * 4 + d].
+__device__ __forceinline__ void flush_herm_wave(const double hv[4], int lane, int64_t base_row, int64_t nk,
+                                                double *__restrict__ hacc)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int src = 16 * q + (lane >> 2), d = lane & 3;
+        const double a0 = __shfl(hv[0], src), a1 = __shfl(hv[1], src), a2 = __shfl(hv[2], src),
+                     a3 = __shfl(hv[3], src);
+        const double v = d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
+        const int64_t row = base_row + src;
+        if (v != 0.0 && row < nk) unsafeAtomicAdd(hacc + row * 4 + d, v);
+    }
+}
+
+// Epilogue from the Hermitian accumulator (nb = 2): ddm = prefac/2 H,
+// rotated back to the interaction picture, minus ddmdt_eq; zeroes hacc[k].
+__global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restrict__ hacc,
+                                                    const double *__restrict__ e, const DClock *__restrict__ clk,
+                                                    double a, int expt, int schrodinger, double prefac,
+                                                    const double2 *__restrict__ eq, double2 *__restrict__ out)
+{
+    constexpr int NB = 2, NB2 = 4;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    double *h = hacc + k * 4;
+    const double h00 = h[0], h11 = h[1], hr = h[2], hi = h[3];
+    h[0] = 0.0; h[1] = 0.0; h[2] = 0.0; h[3] = 0.0;
+    const double s = prefac * 0.5;
+    double2 d[NB2] = { make_double2(h00 * s, 0.0), make_double2(hr * s, hi * s), make_double2(hr * s, -hi * s),
+                       make_double2(h11 * s, 0.0) };
+    const double t = clock_t_at(clk, a);
+    const double arg = (e[k * NB] - e[k * NB + 1]) * t;  // (e_0 - e_1) t for element (0,1)
+    if (expt) {
+        d[1] = cmul(d[1], ccis(arg));
+        d[2] = cmul(d[2], ccis(-arg));
+    }
+    if (eq) {
+#pragma unroll
+        for (int ij = 0; ij < NB2; ij++) {
+            double2 q = eq[k * NB2 + ij];
+            if (ij == 1 && !schrodinger) q = cmul(q, ccis(arg));
+            if (ij == 2 && !schrodinger) q = cmul(q, ccis(-arg));
+            d[ij] = csub(d[ij], q);
+        }
+    }
+#pragma unroll
+    for (int ij = 0; ij < NB2; ij++) out[k * NB2 + ij] = d[ij];
+}
+
 template <bool DM1>
 __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint16_t *__restrict__ pk,
@@ -400,7 +491,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
                                                    const int32_t *__restrict__ gbeg,
                                                    const double2 *__restrict__ rhoS,
-                                                   const double2 *__restrict__ rho1S, double *__restrict__ acc)
+                                                   const double2 *__restrict__ rho1S, double *__restrict__ hacc)
 {
     constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R, RW = 16;  // RW: doubles per row [Pdm | dm1P]
     constexpr int W = 4;
@@ -440,19 +531,19 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
         for (int q = 0; q < NB2; q++) { rk[q] = make_double2(0, 0); r1k[q] = make_double2(0, 0); }
         for (int it = i0; it < i1; it++) {
             const bool ok = lane * m + it < n;
-            const int64_t s = b + (int64_t)it * 64 + lane;
+            // idle lanes read the chunk's first slot (always valid) and add nothing
+            const int64_t s = b + (ok ? (int64_t)it * 64 + lane : 0);
             const unsigned code = code_next;
-            code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? pk[s + 64] : 0u;
+            code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? pk[b + (int64_t)(it + 1) * 64 + lane] : 0u;
             const int lo1 = code & 255u, lo2 = code >> 8;
             const int64_t k = I0 + lo1, kp = J0 + lo2;
             const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
             double2 p[EP];
 #pragma unroll
-            for (int el = 0; el < EP; el++) p[el] = ok ? g[el * 8] : make_double2(0, 0);
+            for (int el = 0; el < EP; el++) p[el] = g[el * 8];
             double2 rkp[NB2], r1kp[NB2];
 #pragma unroll
-            for (int q = 0; q < NB2; q++)
-                rkp[q] = !ok ? make_double2(0, 0) : (DM1 ? rhoS[kp * NB2 + q] : rhoJ[lo2 * NB2 + q]);
+            for (int q = 0; q < NB2; q++) rkp[q] = DM1 ? rhoS[kp * NB2 + q] : rhoJ[lo2 * NB2 + q];
             if (ok && lo1 != lo1_cur) {
 #pragma unroll
                 for (int q = 0; q < NB2; q++) rk[q] = rhoS[k * NB2 + q];
@@ -468,7 +559,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             }
             if (DM1) {
 #pragma unroll
-                for (int q = 0; q < NB2; q++) r1kp[q] = ok ? rho1S[kp * NB2 + q] : make_double2(0, 0);
+                for (int q = 0; q < NB2; q++) r1kp[q] = rho1S[kp * NB2 + q];
             } else {
 #pragma unroll
                 for (int q = 0; q < NB2; q++)
@@ -481,10 +572,10 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                         g2 = make_double2(0, 0);
 #pragma unroll
                 for (int q = 0; q < NB2; q++) {
-                    f1 = cadd(f1, cmul(p[i * NB2 + q], rkp[q]));         // Pdm_k[i]
-                    f2 = cadd(f2, cmul(r1kp[q], p[NB4 + q * NB2 + i]));  // dm1P_k[i]
-                    g1 = cadd(g1, cmulc(rk[q], p[NB4 + i * NB2 + q]));   // Pdm_k'[i]
-                    g2 = cadd(g2, cmulc(r1k[q], p[q * NB2 + i]));        // dm1P_k'[i]
+                    cfma(f1, p[i * NB2 + q], rkp[q]);           // Pdm_k[i]   += P1[i,q] rho_k'[q]
+                    cfma(f2, r1kp[q], p[NB4 + q * NB2 + i]);    // dm1P_k[i]  += (1-rho_k')[q] P2[q,i]
+                    cfmac(g1, rk[q], p[NB4 + i * NB2 + q]);     // Pdm_k'[i]  += rho_k[q] conj(P2[i,q])
+                    cfmac(g2, r1k[q], p[q * NB2 + i]);          // dm1P_k'[i] += (1-rho_k)[q] conj(P1[q,i])
                 }
                 const int sw1 = lo1 & 15, sw2 = lo2 & 15;
                 if (ok) {
@@ -501,21 +592,33 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                 }
             }
         }
-        __syncthreads();  // block J done (and rhoJ free for the next chunk)
-        if (I != J) {  // flush and clear block J
-            for (int i = threadIdx.x; i < R * RW; i += blockDim.x) {
-                const int row = i / RW, d = i - row * RW;
-                const double w = accB[row * RW + (d ^ (row & 15))];
-                accB[row * RW + (d ^ (row & 15))] = 0.0;
-                if (w != 0.0 && J0 + row < nk) unsafeAtomicAdd(acc + (J0 + row) * RW + d, w);
+        __syncthreads();  // block J done
+        static_assert(R == 256, "one accumulator row per thread in the flush");
+        if (I != J) {  // flush block J as its Hermitian part (rho rows from LDS) and clear it
+            const int r = threadIdx.x;
+            double hv[4] = { 0.0, 0.0, 0.0, 0.0 };
+            if (J0 + r < nk) {
+                double2 rr[NB2];
+#pragma unroll
+                for (int q = 0; q < NB2; q++) rr[q] = rhoJ[r * NB2 + q];
+                herm_row<DM1>(accB, r, rr, rho1S, J0 + r, hv);
             }
+#pragma unroll
+            for (int d = 0; d < RW; d++) accB[r * RW + d] = 0.0;
+            flush_herm_wave(hv, lane, J0 + (r & ~63), nk, hacc);
         }
+        __syncthreads();  // rhoJ and accB free for the next chunk
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < R * RW; i += blockDim.x) {
-        const int row = i / RW, d = i - row * RW;
-        const double v = accA[row * RW + (d ^ (row & 15))];
-        if (v != 0.0 && I0 + row < nk) unsafeAtomicAdd(acc + (I0 + row) * RW + d, v);
+    {
+        const int r = threadIdx.x;
+        double hv[4] = { 0.0, 0.0, 0.0, 0.0 };
+        if (I0 + r < nk) {
+            double2 rr[NB2];
+#pragma unroll
+            for (int q = 0; q < NB2; q++) rr[q] = rhoS[(I0 + r) * NB2 + q];
+            herm_row<DM1>(accA, r, rr, rho1S, I0 + r, hv);
+        }
+        flush_herm_wave(hv, lane, I0 + (r & ~63), nk, hacc);
     }
 }
 
@@ -672,7 +775,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_k2); dfree(d_fwdpos); dfree(d_pk); dfree(d_cbeg);
     dfree(d_clen); dfree(d_cij); dfree(d_gbeg); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
     dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
-    dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows);
+    dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows); dfree(d_acc_herm);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -872,7 +975,8 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         (rc = dalloc_t(&h->d_rhoS, ns, "rhoS")) || (rc = dalloc_t(&h->d_rho1S, ns, "rho1S")) ||
         (rc = dalloc_t(&h->d_out, ns, "out")) || (rc = dalloc_t(&h->d_in, ns, "in")) ||
         (rc = dalloc_t(&h->d_in1, ns, "in1")) || (rc = dalloc_t(&h->d_clk, 1, "clock")) ||
-        (rc = dalloc_t(&h->d_acc_rows, (size_t)nk * 4 * h->nb2, "row accumulator")))
+        (rc = dalloc_t(&h->d_acc_rows, (size_t)nk * 4 * h->nb2, "row accumulator")) ||
+        (rc = dalloc_t(&h->d_acc_herm, (size_t)nk * h->nb2, "Hermitian accumulator")))
         return rc;
     if (R && ((rc = dalloc_t(&h->d_pk, nslots, "tile offsets")) || (rc = dalloc_t(&h->d_cbeg, h->nchunks, "chunks")) ||
               (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks")) ||
@@ -907,6 +1011,7 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     h->soa = (nb == 2 && R) ? 1 : 0;  // the default kernel (tiles) streams the SoA granules
     for (double2 *b : {h->d_rhoS, h->d_rho1S, h->d_out, h->d_in, h->d_in1}) DMD_HIP(hipMemset(b, 0, ns * sizeof(double2)));
     DMD_HIP(hipMemset(h->d_acc_rows, 0, (size_t)nk * 4 * h->nb2 * sizeof(double)));
+    DMD_HIP(hipMemset(h->d_acc_herm, 0, (size_t)nk * h->nb2 * sizeof(double)));
     DMD_HIP(hipDeviceSynchronize());
     *out = h.release();
     return DMD_OK;
@@ -1104,21 +1209,16 @@ static void launch_single_nb(dmd_scatt *h, bool dm1, double a, double2 *out)
 static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const double2 *eq = h->has_eq ? h->d_eq : nullptr;
-    dim3 b(256), ga(nblk(h->nk, 256));
     if (h->ngroups > 0) {
         if (dm1)
             hipLaunchKernelGGL(k_tiles_soa<true>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
         else
             hipLaunchKernelGGL(k_tiles_soa<false>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_rows);
+                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
     }
-    if (dm1)
-        hipLaunchKernelGGL((k_assemble<2, true>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S, h->d_e,
-                           h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
-    else
-        hipLaunchKernelGGL((k_assemble<2, false>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S,
-                           h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
+    hipLaunchKernelGGL(k_assemble_h, dim3(nblk(h->nk, 256)), dim3(256), 0, h->st, h->nk, h->d_acc_herm, h->d_e,
+                       h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
 }
 
 // Which contraction kernel runs.  AUTO: the tile kernel for nb = 2, the
