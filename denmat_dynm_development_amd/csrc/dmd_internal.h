@@ -27,8 +27,8 @@ int hw_threads();
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// rows per tile of the tile kernel (k mod R and k' mod R fit in 8 bits)
-#define DMD_TILE_R 256
+// rows per tile of the tile kernel (a multiple of 256)
+#define DMD_TILE_R 512
 // chunks of one row block a tile-kernel workgroup takes in sequence
 #define DMD_TILE_GROUP 1
 
@@ -106,7 +106,7 @@ struct dmd_scatt {
     int64_t *d_inv = nullptr, *d_rowptr = nullptr, *d_revptr = nullptr, *d_revpos = nullptr;
     int32_t *d_col = nullptr, *d_revcol = nullptr, *d_k1 = nullptr, *d_k2 = nullptr;
     int64_t *d_fwdpos = nullptr;         // forward list entry -> physical position of P
-    uint16_t *d_pk = nullptr;            // per physical pair: (k mod R) | (k' mod R) << 8
+    uint32_t *d_pk = nullptr;            // per physical pair: (k mod R) + (k' mod R) * R
     int64_t *d_cbeg = nullptr;           // tile chunks: first pair, length, tile (I, J)
     int32_t *d_clen = nullptr;
     int2 *d_cij = nullptr;

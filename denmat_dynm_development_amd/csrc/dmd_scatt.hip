@@ -393,48 +393,48 @@ __global__ __launch_bounds__(256) void k_fwd_atomic(int64_t nk, const double2 *_
 // with global FP64 atomics (~2R lines per chunk instead of one per pair).
 // Padding pairs carry P = 0 and add exact zeros.
 // ---------------------------------------------------------------------------
-// Flush of LDS accumulator rows (nb = 2).  Thread r of the block takes row r:
-// with Pdm, dm1P the row's partial sums from this chunk it forms
-// H = C + C^+, C = (1-rho) Pdm - dm1P rho (zhemm upper), the four doubles
+// Hermitian increment of one pair's contribution to one row (nb = 2): with
+// pdm, d1p the pair's Pdm and dm1P terms for row k and r, r1 = rho_k, 1-rho_k,
+// C = (1-rho) mat(pdm) - mat(d1p) rho (zhemm upper) and H = C + C^+ as
 // [H00, H11, Re H01, Im H01].  C is linear in (Pdm, dm1P) for a fixed rho_k,
-// so summing the chunks' H gives the row's H (k_assemble_h scales it by
-// prefac/2).  The wave then regroups the values so every atomic instruction
-// covers 16 consecutive rows' 32-byte records (4 full 128-byte lines).
-template <bool DM1>
-__device__ __forceinline__ void herm_row(const double *__restrict__ accrow_base, int r, const double2 *rr,
-                                         const double2 *__restrict__ rho1S, int64_t k, double hv[4])
+// so the sum of the pairs' H is the row's H (the reference sums Pdm, dm1P
+// first and assembles once, evolve_eph.cpp:140-151; the initializer assembles
+// per pair, lindbladInit.../ElectronPhonon.h:855-862).
+__device__ __forceinline__ double2 one_minus(const double2 *r, int q)
 {
-    constexpr int NB = 2, NB2 = 4, RW = 16;
-    double v[RW];
+    return make_double2((q % 3 == 0 ? 1.0 : 0.0) - r[q].x, -r[q].y);  // nb = 2: diagonal q = 0, 3
+}
+__device__ __forceinline__ void herm_inc(const double2 *pdm, const double2 *d1p, const double2 *r,
+                                         const double2 *r1, double hv[4])
+{
+    constexpr int NB = 2;
+    double2 r1loc[4];
+    if (!r1) {
 #pragma unroll
-    for (int d = 0; d < RW; d++) v[d] = accrow_base[r * RW + (d ^ (r & 15))];
-    double2 pdm[NB2], d1p[NB2], r1[NB2];
-#pragma unroll
-    for (int q = 0; q < NB2; q++) {
-        pdm[q] = make_double2(v[2 * q], v[2 * q + 1]);
-        d1p[q] = make_double2(v[2 * (NB2 + q)], v[2 * (NB2 + q) + 1]);
-        r1[q] = DM1 ? rho1S[k * NB2 + q]
-                    : make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rr[q].x, -rr[q].y);
+        for (int q = 0; q < 4; q++) r1loc[q] = one_minus(r, q);
+        r1 = r1loc;
     }
-    double2 c[NB2];
+    double2 c[4];
 #pragma unroll
     for (int i = 0; i < NB; i++)
 #pragma unroll
         for (int j = 0; j < NB; j++) {
             double2 x = make_double2(0, 0);
 #pragma unroll
-            for (int q = 0; q < NB; q++) x = cadd(x, cmul(herm_up<NB>(r1, i, q), pdm[q * NB + j]));
+            for (int q = 0; q < NB; q++) cfma(x, herm_up<NB>(r1, i, q), pdm[q * NB + j]);
 #pragma unroll
-            for (int q = 0; q < NB; q++) x = csub(x, cmul(d1p[i * NB + q], herm_up<NB>(rr, q, j)));
+            for (int q = 0; q < NB; q++) cfma(x, make_double2(-d1p[i * NB + q].x, -d1p[i * NB + q].y), herm_up<NB>(r, q, j));
             c[i * NB + j] = x;
         }
-    hv[0] = 2.0 * c[0].x;      // H00 = C00 + conj(C00)
-    hv[1] = 2.0 * c[3].x;      // H11
-    hv[2] = c[1].x + c[2].x;   // H01 = C01 + conj(C10)
+    hv[0] = 2.0 * c[0].x;     // H00 = C00 + conj(C00)
+    hv[1] = 2.0 * c[3].x;     // H11
+    hv[2] = c[1].x + c[2].x;  // H01 = C01 + conj(C10)
     hv[3] = c[1].y - c[2].y;
 }
-// Rows [row0, row0 + 64) of one wave: lane l holds hv of row row0 + l (zeros
-// for rows past nk); adds them into hacc[(base + row) * 4 + d].
+
+// Rows [base_row, base_row + 64) of one wave: lane l holds the 4 doubles of
+// row base_row + l (zeros past nk); the wave regroups them so every atomic
+// instruction covers 16 consecutive rows' 32-byte records (4 full lines).
 __device__ __forceinline__ void flush_herm_wave(const double hv[4], int lane, int64_t base_row, int64_t nk,
                                                 double *__restrict__ hacc)
 {
@@ -486,27 +486,24 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
 
 template <bool DM1>
 __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
-                                                   const uint16_t *__restrict__ pk,
+                                                   const uint32_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
                                                    const int32_t *__restrict__ gbeg,
                                                    const double2 *__restrict__ rhoS,
                                                    const double2 *__restrict__ rho1S, double *__restrict__ hacc)
 {
-    constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R, RW = 16;  // RW: doubles per row [Pdm | dm1P]
-    constexpr int W = 4;
-    // LDS: row accumulators of blocks I and J (double d of row r at
-    // r*RW + (d ^ (r & 15)): the swizzle spreads rows over the banks) and the
+    constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R;
+    constexpr int W = 4, RPT = R / 256;  // rows per thread in the flush
+    // LDS: Hermitian row increments of blocks I and J, structure of arrays
+    // (double d of row r at d*R + r, conflict-free for the flush), and the
     // rho rows of block J, gathered per pair from LDS instead of L2.
-    __shared__ double accA[R * RW], accB[R * RW];
+    __shared__ double accA[4 * R], accB[4 * R];
     __shared__ double2 rhoJ[R * NB2];
-    // a workgroup takes a group of consecutive chunks of one row block I
-    // (tiles (I, J1), (I, J2), ...): block I's accumulators stay in LDS for
-    // the whole group, block J's are flushed at the end of every chunk
     const int c0 = gbeg[blockIdx.x], c1 = gbeg[blockIdx.x + 1];
     const int I = cij[c0].x;
     const int64_t I0 = (int64_t)I * R;
-    for (int i = threadIdx.x; i < R * RW; i += blockDim.x) { accA[i] = 0.0; accB[i] = 0.0; }
+    for (int i = threadIdx.x; i < 4 * R; i += blockDim.x) { accA[i] = 0.0; accB[i] = 0.0; }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int c = c0; c < c1; c++) {
         const int64_t b = cbeg[c];
@@ -522,7 +519,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
         __syncthreads();
         const int mq = (m + W - 1) / W;
         const int i0 = wv * mq, i1 = min(m, i0 + mq);
-        unsigned code_next = (i0 < i1 && lane * m + i0 < n) ? pk[b + (int64_t)i0 * 64 + lane] : 0u;
+        uint32_t code_next = (i0 < i1 && lane * m + i0 < n) ? pk[b + (int64_t)i0 * 64 + lane] : 0u;
         // rho of the lane's current row k: a lane walks a run of one k, so it is
         // reloaded only where k changes
         int lo1_cur = -1;
@@ -533,15 +530,12 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             const bool ok = lane * m + it < n;
             // idle lanes read the chunk's first slot (always valid) and add nothing
             const int64_t s = b + (ok ? (int64_t)it * 64 + lane : 0);
-            const unsigned code = code_next;
+            const uint32_t code = code_next;
             code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? pk[b + (int64_t)(it + 1) * 64 + lane] : 0u;
-            const int lo1 = code & 255u, lo2 = code >> 8;
+            const int lo1 = code % R, lo2 = code / R;
             const int64_t k = I0 + lo1, kp = J0 + lo2;
             const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
-            double2 p[EP];
-#pragma unroll
-            for (int el = 0; el < EP; el++) p[el] = g[el * 8];
-            double2 rkp[NB2], r1kp[NB2];
+            double2 rkp[NB2];
 #pragma unroll
             for (int q = 0; q < NB2; q++) rkp[q] = DM1 ? rhoS[kp * NB2 + q] : rhoJ[lo2 * NB2 + q];
             if (ok && lo1 != lo1_cur) {
@@ -550,74 +544,83 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                 if (DM1) {
 #pragma unroll
                     for (int q = 0; q < NB2; q++) r1k[q] = rho1S[k * NB2 + q];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < NB2; q++)
-                        r1k[q] = make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rk[q].x, -rk[q].y);
                 }
                 lo1_cur = lo1;
             }
+            double2 r1kp[NB2];
             if (DM1) {
 #pragma unroll
                 for (int q = 0; q < NB2; q++) r1kp[q] = rho1S[kp * NB2 + q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NB2; q++)
-                    r1kp[q] = make_double2((q % (NB + 1) == 0 ? 1.0 : 0.0) - rkp[q].x, -rkp[q].y);
             }
+            // 1-rho: explicit rows with DM1, else formed from rho where used
+            auto om_k = [&](int q) { return DM1 ? r1k[q] : one_minus(rk, q); };
+            auto om_kp = [&](int q) { return DM1 ? r1kp[q] : one_minus(rkp, q); };
             const bool rev = ok && k < kp;
+            double2 p1[NB4], p2[NB4];
 #pragma unroll
-            for (int i = 0; i < NB2; i++) {
-                double2 f1 = make_double2(0, 0), f2 = make_double2(0, 0), g1 = make_double2(0, 0),
-                        g2 = make_double2(0, 0);
+            for (int el = 0; el < NB4; el++) p1[el] = g[el * 8];
 #pragma unroll
-                for (int q = 0; q < NB2; q++) {
-                    cfma(f1, p[i * NB2 + q], rkp[q]);           // Pdm_k[i]   += P1[i,q] rho_k'[q]
-                    cfma(f2, r1kp[q], p[NB4 + q * NB2 + i]);    // dm1P_k[i]  += (1-rho_k')[q] P2[q,i]
-                    cfmac(g1, rk[q], p[NB4 + i * NB2 + q]);     // Pdm_k'[i]  += rho_k[q] conj(P2[i,q])
-                    cfmac(g2, r1k[q], p[q * NB2 + i]);          // dm1P_k'[i] += (1-rho_k)[q] conj(P1[q,i])
+            for (int el = 0; el < NB4; el++) p2[el] = g[(NB4 + el) * 8];
+            // forward (row k): Pdm_k = P1 rho_k', dm1P_k = (1-rho_k') P2
+            {
+                double2 f1[NB2], f2[NB2];
+#pragma unroll
+                for (int i = 0; i < NB2; i++) {
+                    f1[i] = make_double2(0, 0);
+                    f2[i] = make_double2(0, 0);
+#pragma unroll
+                    for (int q = 0; q < NB2; q++) {
+                        cfma(f1[i], p1[i * NB2 + q], rkp[q]);      // Pdm_k[i]  += P1[i,q] rho_k'[q]
+                        cfma(f2[i], om_kp(q), p2[q * NB2 + i]);    // dm1P_k[i] += (1-rho_k')[q] P2[q,i]
+                    }
                 }
-                const int sw1 = lo1 & 15, sw2 = lo2 & 15;
+                double hf[4];
+                herm_inc(f1, f2, rk, DM1 ? r1k : nullptr, hf);
                 if (ok) {
-                    atomicAdd(&accA[lo1 * RW + ((2 * i) ^ sw1)], f1.x);
-                    atomicAdd(&accA[lo1 * RW + ((2 * i + 1) ^ sw1)], f1.y);
-                    atomicAdd(&accA[lo1 * RW + ((2 * (NB2 + i)) ^ sw1)], f2.x);
-                    atomicAdd(&accA[lo1 * RW + ((2 * (NB2 + i) + 1) ^ sw1)], f2.y);
+#pragma unroll
+                    for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
                 }
+            }
+            // reverse (row k'): Pdm_k' = conj(P2) rho_k, dm1P_k' = (1-rho_k) conj(P1)
+            {
+                double2 g1[NB2], g2[NB2];
+#pragma unroll
+                for (int i = 0; i < NB2; i++) {
+                    g1[i] = make_double2(0, 0);
+                    g2[i] = make_double2(0, 0);
+#pragma unroll
+                    for (int q = 0; q < NB2; q++) {
+                        cfmac(g1[i], rk[q], p2[i * NB2 + q]);      // Pdm_k'[i]  += rho_k[q] conj(P2[i,q])
+                        cfmac(g2[i], om_k(q), p1[q * NB2 + i]);    // dm1P_k'[i] += (1-rho_k)[q] conj(P1[q,i])
+                    }
+                }
+                double hr[4];
+                herm_inc(g1, g2, rkp, DM1 ? r1kp : nullptr, hr);
                 if (rev) {
-                    atomicAdd(&aJ[lo2 * RW + ((2 * i) ^ sw2)], g1.x);
-                    atomicAdd(&aJ[lo2 * RW + ((2 * i + 1) ^ sw2)], g1.y);
-                    atomicAdd(&aJ[lo2 * RW + ((2 * (NB2 + i)) ^ sw2)], g2.x);
-                    atomicAdd(&aJ[lo2 * RW + ((2 * (NB2 + i) + 1) ^ sw2)], g2.y);
+#pragma unroll
+                    for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
                 }
             }
         }
         __syncthreads();  // block J done
-        static_assert(R == 256, "one accumulator row per thread in the flush");
-        if (I != J) {  // flush block J as its Hermitian part (rho rows from LDS) and clear it
-            const int r = threadIdx.x;
-            double hv[4] = { 0.0, 0.0, 0.0, 0.0 };
-            if (J0 + r < nk) {
-                double2 rr[NB2];
+        if (I != J) {  // flush and clear block J
 #pragma unroll
-                for (int q = 0; q < NB2; q++) rr[q] = rhoJ[r * NB2 + q];
-                herm_row<DM1>(accB, r, rr, rho1S, J0 + r, hv);
+            for (int t = 0; t < RPT; t++) {
+                const int r = t * 256 + threadIdx.x;
+                double hv[4];
+#pragma unroll
+                for (int d = 0; d < 4; d++) { hv[d] = accB[d * R + r]; accB[d * R + r] = 0.0; }
+                flush_herm_wave(hv, lane, J0 + (r & ~63), nk, hacc);
             }
-#pragma unroll
-            for (int d = 0; d < RW; d++) accB[r * RW + d] = 0.0;
-            flush_herm_wave(hv, lane, J0 + (r & ~63), nk, hacc);
         }
         __syncthreads();  // rhoJ and accB free for the next chunk
     }
-    {
-        const int r = threadIdx.x;
-        double hv[4] = { 0.0, 0.0, 0.0, 0.0 };
-        if (I0 + r < nk) {
-            double2 rr[NB2];
 #pragma unroll
-            for (int q = 0; q < NB2; q++) rr[q] = rhoS[(I0 + r) * NB2 + q];
-            herm_row<DM1>(accA, r, rr, rho1S, I0 + r, hv);
-        }
+    for (int t = 0; t < RPT; t++) {
+        const int r = t * 256 + threadIdx.x;
+        double hv[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) hv[d] = accA[d * R + r];
         flush_herm_wave(hv, lane, I0 + (r & ~63), nk, hacc);
     }
 }
@@ -878,7 +881,7 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     int64_t nslots = n_pairs;
     if (R) {
         const int64_t nbk = (nk + R - 1) / R;
-        const int64_t cmax = std::min<int64_t>(4096, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64;
+        const int64_t cmax = std::min<int64_t>(16384, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64;
         std::vector<int64_t> tc(nbk * nbk + 1, 0);
         for (int64_t i = 0; i < n_pairs; i++) {
             int64_t f = KK(i);
@@ -957,11 +960,11 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
             gbeg.push_back((int32_t)c);
     gbeg.push_back((int32_t)cbeg.size());
     h->ngroups = (int64_t)gbeg.size() - 1;
-    std::vector<uint16_t> pk;
+    std::vector<uint32_t> pk;
     if (R) {
         pk.assign(nslots, 0);  // padding slots: (0, 0), never read
         for (int64_t pos = 0; pos < nslots; pos++)
-            if (k1p[pos] >= 0) pk[pos] = (uint16_t)((k1p[pos] % R) | ((k2p[pos] % R) << 8));
+            if (k1p[pos] >= 0) pk[pos] = (uint32_t)(k1p[pos] % R) + (uint32_t)(k2p[pos] % R) * (uint32_t)R;
     }
     h->nchunks = (int64_t)cbeg.size();
     int rc;
@@ -1001,13 +1004,13 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         DMD_HIP(hipMemcpy(h->d_k2, k2p.data(), nslots * 4, hipMemcpyHostToDevice));
     }
     if (R && n_pairs) {
-        DMD_HIP(hipMemcpy(h->d_pk, pk.data(), nslots * 2, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_pk, pk.data(), nslots * 4, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_cbeg, cbeg.data(), h->nchunks * 8, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_clen, clen.data(), h->nchunks * 4, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_cij, cij.data(), h->nchunks * 8, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_gbeg, gbeg.data(), (h->ngroups + 1) * 4, hipMemcpyHostToDevice));
     }
-    DMD_HIP(hipMemset(h->d_P, 0, (size_t)std::max<int64_t>(nslots, 1) * 2 * h->nb4 * sizeof(double2)));
+    if (nslots) DMD_HIP(hipMemset(h->d_P, 0, (size_t)nslots * 2 * h->nb4 * sizeof(double2)));
     h->soa = (nb == 2 && R) ? 1 : 0;  // the default kernel (tiles) streams the SoA granules
     for (double2 *b : {h->d_rhoS, h->d_rho1S, h->d_out, h->d_in, h->d_in1}) DMD_HIP(hipMemset(b, 0, ns * sizeof(double2)));
     DMD_HIP(hipMemset(h->d_acc_rows, 0, (size_t)nk * 4 * h->nb2 * sizeof(double)));
@@ -1028,7 +1031,7 @@ extern "C" int dmd_scatt_destroy(dmd_scatt *h)
 static int put_P(dmd_scatt *h, const double *P1, const double *P2, double scale, int add)
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
-    if (h->np == 0) return DMD_OK;
+    if (h->np == 0) { h->has_P = true; return DMD_OK; }
     if (!P1 || !P2) return fail(DMD_ERR_ARG, "P1/P2 null");
     const size_t per = (size_t)h->nb4;  // complex per P array per pair
     const bool dev1 = is_device_ptr(P1), dev2 = is_device_ptr(P2);
@@ -1261,10 +1264,9 @@ static int eval_core(dmd_scatt *h, double t, const double *rho, const double *rh
     const size_t bytes = (size_t)h->nk * h->nb2 * 16;
     int rc;
     const double2 *in = (const double2 *)rho, *in1 = (const double2 *)rho1;
-    if (!is_device_ptr(rho)) {
-        if ((rc = to_device(h, h->d_in, rho, bytes))) return rc;
-        in = h->d_in;
-    }
+    // the state always lands in d_in (dmd_scatt_time_eval replays it)
+    if ((rc = to_device(h, h->d_in, rho, bytes))) return rc;
+    in = h->d_in;
     if (rho1 && !is_device_ptr(rho1)) {
         if ((rc = to_device(h, h->d_in1, rho1, bytes))) return rc;
         in1 = h->d_in1;
@@ -1352,7 +1354,7 @@ extern "C" int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     int rc;
     if ((rc = set_clock(h, t, 0.0, 0))) return rc;
-    // d_in holds the last host-staged rho (or zeros); use it as the state
+    // d_in holds the state of the last dmd_scatt_eval (zeros before any)
     if ((rc = launch_rotate(h, h->d_in, nullptr, h->nb, 0, 0.0))) return rc;
     if ((rc = launch_contract(h, false, 0.0, h->d_out))) return rc;  // warm
     DMD_HIP(hipStreamSynchronize(h->st));

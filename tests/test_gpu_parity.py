@@ -327,3 +327,24 @@ def test_generate_P_in_both_layouts():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     P1, P2 = scatt.synth_P(p, e, k1, k2)
     assert rel_err(out[0][0], P1) < 1e-13 and rel_err(out[0][1], P2) < 1e-13
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_no_pairs_gives_minus_eq_only(kernel):
+    """An operator without stored pairs: ddmdt is zero, or -ddmdt_eq (rotated)."""
+    scatt, oracle = _mods()
+    rng = np.random.default_rng(3)
+    nk, nb = 300, 2
+    k = np.zeros(0, np.uint64)
+    e = random_energies(nk, nb, rng)
+    rho = random_rho(nk, nb, rng)
+    eq = random_rho(nk, nb, rng, scale=1e-3)
+    P = np.zeros((0, nb ** 4), np.complex128)
+    with scatt.ScatteringOperator(nk, nb, k, k, e) as op:
+        op.upload_P(P, P)
+        op.set_flags(kernel=kernel)
+        assert np.abs(op.eval(7 * FS, rho)).max() == 0.0
+        op.set_ddmdt_eq(eq)
+        got = op.eval(7 * FS, rho)
+    ref = oracle.evolve(k, k, P, P, e, rho, t=7 * FS, prefac=2 * np.pi / nk, ddmdt_eq=eq)
+    assert rel_err(got, ref) < TOL
