@@ -25,6 +25,12 @@ sys.path.insert(0, ROOT)
 
 FS = 41.341373335182
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script (calibrated against a 1 GiB elementwise kernel:
+# FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); key (preset, kernel kind) -> (bytes, profile file)
+MEASURED_TRAFFIC = {
+    (2, "tiles"): (31.697e9, "profiles/k_tiles_soa_nb2_nk65536_58Mpairs.txt"),
+}
 
 
 def log(msg):
@@ -137,6 +143,7 @@ def main():
         kind, kname = "single_read", f"k_fwd_atomic<{p.nb}> (+k_assemble)"
     else:
         kind, kname = "tiles", "k_tiles_soa<false> (+k_assemble_h)"
+    traffic = MEASURED_TRAFFIC.get((args.preset, kind))
     out = {
         "metric": "k-pair contractions/s",
         "value": value,
@@ -157,7 +164,9 @@ def main():
         "hbm_GBps": achieved,
         "traj_steps_per_s": args.steps / el,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "kernel": kname,
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
     }
