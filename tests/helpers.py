@@ -53,3 +53,23 @@ def random_rho(nk, nb, rng, scale=0.05, diag=0.3):
 
 def rel_err(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def gaas_spin_fixture():
+    """The reference GaAs run's final state and S_z (tests/golden/make_gaas_spin.py).
+
+    Returns (e_dm, f, s_z, dm, i0, i1, t, prefac, sz, sz_amp) with s_z and dm
+    full nb_dm x nb_dm per k (zero outside what the electron S_z reads).  The
+    run's sz_elec_tot.out holds totals per unit cell: prefac = 1/nk_full.
+    """
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gaas_spin_restart.npz"))
+    e = d["e_dm"]
+    nk, nb, nv = e.shape[0], int(d["nb_dm"]), int(d["nv_dm"])
+    f = 1.0 / (np.exp((e - float(d["mu"])) / float(d["T"])) + 1.0)  # electron.h:119-124
+    s = np.zeros((nk, nb, nb), np.complex128)
+    s[:, nv:, :] = d["sz_rows"]
+    dm = np.zeros((nk, nb, nb), np.complex128)
+    dm[:, :, nv:] = d["dm_cols"]
+    return (e, f, s.reshape(nk, nb * nb), dm.reshape(nk, nb * nb), nv, nb, float(d["t"]),
+            1.0 / float(d["nk_full"]), float(d["sz"]), float(d["sz_amp"]))

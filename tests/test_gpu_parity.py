@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import FS, random_P, random_energies, random_pairs, random_rho, rel_err
+from helpers import FS, gaas_spin_fixture, random_P, random_energies, random_pairs, random_rho, rel_err
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -348,3 +348,18 @@ def test_no_pairs_gives_minus_eq_only(kernel):
         got = op.eval(7 * FS, rho)
     ref = oracle.evolve(k, k, P, P, e, rho, t=7 * FS, prefac=2 * np.pi / nk, ddmdt_eq=eq)
     assert rel_err(got, ref) < TOL
+
+
+def test_spin_observable_on_reference_gaas_state():
+    """dmd_scatt_measure on the reference GaAs run's final state reproduces the
+    last S_z of its sz_elec_tot.out (the CPU twin of this check pins the oracle)."""
+    scatt, oracle = _mods()
+    e, f, s, dm, i0, i1, t, prefac, sz, sz_amp = gaas_spin_fixture()
+    nk, nbd = e.shape
+    k1 = np.zeros(0, np.uint64)
+    with scatt.ScatteringOperator(nk, i1 - i0, k1, k1, e[:, i0:i1]) as op:
+        op.upload_P(np.zeros((0, (i1 - i0) ** 4), np.complex128), np.zeros((0, (i1 - i0) ** 4), np.complex128))
+        op.set_window(nbd, i0, e)
+        op.set_observable(s, f, i0, i1, prefac)
+        v = op.measure(t, dm)
+    assert abs(v / sz - 1) < 5e-12

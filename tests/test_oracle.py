@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 import scipy.linalg
 
-from helpers import FS, random_P, random_energies, random_pairs, random_rho, rel_err
+from helpers import FS, gaas_spin_fixture, random_P, random_energies, random_pairs, random_rho, rel_err
 from oracle import oracle
 
 
@@ -298,3 +298,18 @@ def test_spin_observable_brute_and_zero_at_f():
     assert abs(got - ref) < 1e-14 * abs(ref)
     feq = np.einsum("ki,ij->kij", f, np.eye(nb)).reshape(nk, nb * nb).astype(complex)
     assert oracle.spin(s, e, f, feq, t, 0, nb, 1.0) == 0.0
+
+
+def test_spin_matches_reference_gaas_run():
+    """S_z of the reference's own GaAs e-ph run (Examples/RealTime_Example_GaAs_20230105/
+    ld-dmd/t300/mult8_gauss0.005/dBz/only-eph/n2e16): the final density matrix
+    (restart/denmat_restart.bin) with that run's S_z matrices, band energies,
+    T and mu reproduces the last line of its sz_elec_tot.out, with and without
+    the interaction-picture phase.  The printed mu has 8 digits; its rounding
+    moves S_z by ~1e-12 relative."""
+    e, f, s, dm, i0, i1, t, prefac, sz, sz_amp = gaas_spin_fixture()
+    v, a = oracle.spin(s, e, f, dm, t, i0, i1, prefac, with_amp=True)
+    assert abs(v / sz - 1) < 5e-12 and abs(a / sz_amp - 1) < 5e-12
+    # a planted mistake: the phase sign flipped must miss the reference value
+    v_bad = oracle.spin(s, e, f, dm, -t, i0, i1, prefac)
+    assert abs(v_bad / sz - 1) > 1e-6
