@@ -320,6 +320,7 @@ extern "C" int dmd_scatt_evolve_driver(dmd_scatt *h, double t, const double *dm,
     if (!is_device_ptr(dm)) { if ((rc = to_device(h, h->d_in, dm, bytes))) return rc; in = h->d_in; }
     if (dm1 && !is_device_ptr(dm1)) { if ((rc = to_device(h, h->d_in1, dm1, bytes))) return rc; in1 = h->d_in1; }
     if (!h->d_w && (rc = dalloc_t(&h->d_w, (size_t)h->nk * h->nb2, "window ddm"))) return rc;
+    if ((rc = tiles4_prepare(h))) return rc;
     if ((rc = set_clock(h, t, 0.0, 0))) return rc;
     if ((rc = launch_rotate(h, in, in1, h->nbd, h->b0, 0.0))) return rc;
     if ((rc = launch_contract(h, dm1 != nullptr, 0.0, h->d_w))) return rc;
@@ -388,6 +389,7 @@ extern "C" int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64
     if (want_obs && !obs_out) return fail(DMD_ERR_ARG, "obs_out is null");
     int rc;
     if ((rc = ensure_rk(h))) return rc;
+    if ((rc = tiles4_prepare(h))) return rc;
     const int64_t nobs = want_obs ? n_steps / report_every + 1 : 0;
     double *d_obs = nullptr;
     if (nobs && (rc = dalloc_t(&d_obs, nobs, "obs series"))) return rc;

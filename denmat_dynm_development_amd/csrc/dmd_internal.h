@@ -31,6 +31,20 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 #define DMD_TILE_R 512
 // chunks of one row block a tile-kernel workgroup takes in sequence
 #define DMD_TILE_GROUP 1
+// nb = 4 tile kernel: rows per tile, chunks per workgroup, pairs per chunk at most
+#define DMD_TILE4_R 32
+#define DMD_TILE4_GROUP 8
+#define DMD_TILE4_CMAX 4096
+
+// nb = 4 blocked pair record (512 complex elements, P1 then P2): offset of
+// P_mat[row][col] for U elements per 16-byte unit.  Lane l = 4 (row/4) +
+// col/4 of a 16-lane group holds the 4 x 4 block; its element e = 4 (row%4) +
+// col%4 sits in unit e / U.
+__host__ __device__ inline int blk_off(int mat, int row, int col, int U)
+{
+    const int l = 4 * (row >> 2) + (col >> 2), e = 4 * (row & 3) + (col & 3);
+    return mat * 256 + ((e / U) * 16 + l) * U + e % U;
+}
 
 // Device clock: t(a) = (t0 + step*dt) + a*dt, the same expression as the RK4
 // stage times of the oracle's driver.
@@ -114,7 +128,17 @@ struct dmd_scatt {
     int64_t nchunks = 0, ngroups = 0;
     int64_t nslots = 0;                  // physical pair slots (stored pairs + tile padding)
     int tile_R = 0;
-    int soa = 0;                         // P stored as SoA granules of 8 pairs (tile kernel)
+    int soa = 0;                         // P layout: 0 pair-contiguous, 1 SoA granules of 8 pairs
+                                         // (nb = 2 tiles), 2 blocked 4x4 per lane (nb = 4 tiles)
+    // nb = 4 tiles and the reduced-precision forms (DMD_PREC_C64 / _BF16 store P
+    // only in the blocked layout; d_P then holds float2 or packed bf16 re/im words)
+    size_t p_esize = 16;                 // bytes per stored complex element of P
+    float *d_pscale = nullptr;           // BF16: per-slot power-of-two scale
+    double *d_fsplit = nullptr;          // nk x nb occupations f of the split rho = f + drho
+    double *d_feq = nullptr;             // nk x 64: F1 | F2 row sums at rho = diag(f)
+    float2 *d_x32 = nullptr;             // nk x 16: drho_S in FP32 for the reduced forms
+    double *d_hacc16 = nullptr;          // nk x 16 packed Hermitian row accumulator (nb = 4)
+    bool feq_dirty = true;
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
@@ -159,6 +183,13 @@ int set_clock(dmd_scatt *h, double t0, double dt, long long step);
 int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a);
 // contraction + Hermitian epilogue on h->d_rhoS (and h->d_rho1S when dm1) into out (nk x nb^2)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
-// DMD_KERNEL_ROWS or DMD_KERNEL_SINGLE_READ: the kernel launch_contract uses
+// DMD_KERNEL_ROWS, _SINGLE_READ or _TILES: the kernel launch_contract uses
 int contract_kind(const dmd_scatt *h);
+// nb = 4 tile path (dmd_tiles4.hip)
+int tiles4_place(dmd_scatt *h, const double2 *P1, const double2 *P2, int64_t n, int64_t first, double scale, int add);
+int tiles4_fetch(dmd_scatt *h, int64_t first, int64_t n, double2 *P1, double2 *P2);
+int tiles4_generate(dmd_scatt *h, const dmd_synth_params &p);
+int tiles4_relayout(dmd_scatt *h, int to_blocked);
+int tiles4_prepare(dmd_scatt *h);  // F_eq of the reduced forms when P or f changed (not capturable)
+int launch_tiles4(dmd_scatt *h, bool dm1, double a, double2 *out);
 }  // namespace dmd

@@ -705,6 +705,7 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nk, double *__restrict
 // structure-of-arrays granules of 8 pairs the tile kernel streams.
 __device__ __forceinline__ int64_t paddr(int64_t s, int el, int ep, int soa)
 {
+    if (soa == 2) return s * 512 + blk_off(el >> 8, (el >> 4) & 15, el & 15, 1);  // nb = 4 blocked records
     return soa ? (s >> 3) * (8 * (int64_t)ep) + (int64_t)el * 8 + (s & 7) : s * ep + el;
 }
 
@@ -780,6 +781,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
     dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows); dfree(d_acc_herm);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
+    dfree(d_pscale); dfree(d_fsplit); dfree(d_feq); dfree(d_x32); dfree(d_hacc16);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -826,7 +828,10 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     if (n_pairs < 0 || (n_pairs > 0 && (!k1st || !k2nd))) return fail(DMD_ERR_ARG, "dmd_scatt_create: bad k-pair lists");
     if (!e) return fail(DMD_ERR_ARG, "dmd_scatt_create: e is null");
     if (!(nk_full > 0)) return fail(DMD_ERR_ARG, "dmd_scatt_create: nk_full must be positive");
-    if (precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_create: precision not available");
+    if (precision != DMD_PREC_FP64 && precision != DMD_PREC_C64 && precision != DMD_PREC_BF16)
+        return fail(DMD_ERR_ARG, "dmd_scatt_create: unknown precision");
+    if (precision != DMD_PREC_FP64 && nb != 4)
+        return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_create: the reduced-precision forms are built for nb = 4");
     for (int64_t i = 0; i < n_pairs; i++)
         if (k1st[i] > k2nd[i] || k2nd[i] >= (uint64_t)nk)
             return fail(DMD_ERR_ARG, "dmd_scatt_create: pair " + std::to_string(i) + " is not k1st <= k2nd < nk");
@@ -864,7 +869,8 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
                              [&](int64_t a, int64_t b) { return k2nd[a] < k2nd[b]; });
     }
     auto KK = [&](int64_t i) { return sorted ? i : kk[i]; };
-    const int R = nb <= 2 ? DMD_TILE_R : 0;
+    const int R = nb <= 2 ? DMD_TILE_R : (nb == 4 ? DMD_TILE4_R : 0);
+    const bool lane_perm = nb <= 2;  // nb = 4: a chunk is walked by quarters, no slot permutation
     h->tile_R = R;
     // Slots: physical positions of P.  With tiles, each tile's pairs (in
     // (k, k') order) are cut into chunks of at most cmax pairs; a chunk of n
@@ -881,7 +887,9 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
     int64_t nslots = n_pairs;
     if (R) {
         const int64_t nbk = (nk + R - 1) / R;
-        const int64_t cmax = std::min<int64_t>(16384, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64;
+        const int64_t cmax = lane_perm ? std::min<int64_t>(16384, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64
+                                       : DMD_TILE4_CMAX;
+        const int64_t pad = lane_perm ? 64 : 4;
         std::vector<int64_t> tc(nbk * nbk + 1, 0);
         for (int64_t i = 0; i < n_pairs; i++) {
             int64_t f = KK(i);
@@ -903,14 +911,16 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
                 cbeg.push_back(nslots);
                 clen.push_back((int32_t)n);
                 cij.push_back(make_int2((int)(t / nbk), (int)(t % nbk)));
-                nslots += (n + 63) / 64 * 64;
+                nslots += (n + pad - 1) / pad * pad;
             }
         phys.assign(nslots, -1);
         int64_t c = 0;
         for (int64_t t = 0; t < nbk * nbk; t++)
             for (int64_t a = tc[t]; a < tc[t + 1]; a += cmax, c++) {
                 const int64_t n = clen[c], m = (n + 63) / 64;
-                for (int64_t L = 0; L < n; L++) phys[cbeg[c] + (L % m) * 64 + L / m] = byt[a + L];
+                for (int64_t L = 0; L < n; L++)
+                    phys[cbeg[c] + (lane_perm ? (L % m) * 64 + L / m : L)] = byt[a + L];
+                if (!lane_perm) clen[c] = (int32_t)((n + pad - 1) / pad * pad);  // padded slots carry no pair
             }
     }
     h->nslots = nslots;
@@ -955,22 +965,26 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         }
     }
     std::vector<int32_t> gbeg;  // workgroup -> first chunk (groups of chunks of one row block)
+    const int group = lane_perm ? DMD_TILE_GROUP : DMD_TILE4_GROUP;
     for (int64_t c = 0; c < (int64_t)cbeg.size(); c++)
-        if (gbeg.empty() || cij[c].x != cij[gbeg.back()].x || c - gbeg.back() >= DMD_TILE_GROUP)
+        if (gbeg.empty() || cij[c].x != cij[gbeg.back()].x || c - gbeg.back() >= group)
             gbeg.push_back((int32_t)c);
     gbeg.push_back((int32_t)cbeg.size());
     h->ngroups = (int64_t)gbeg.size() - 1;
     std::vector<uint32_t> pk;
     if (R) {
-        pk.assign(nslots, 0);  // padding slots: (0, 0), never read
+        // padding slots: (0, 0) for nb <= 2 (never read), a marker for nb = 4
+        pk.assign(nslots, lane_perm ? 0u : 0xFFFFFFFFu);
         for (int64_t pos = 0; pos < nslots; pos++)
             if (k1p[pos] >= 0) pk[pos] = (uint32_t)(k1p[pos] % R) + (uint32_t)(k2p[pos] % R) * (uint32_t)R;
     }
     h->nchunks = (int64_t)cbeg.size();
     int rc;
     const size_t ns = (size_t)nk * h->nb2;
+    h->p_esize = precision == DMD_PREC_FP64 ? 16 : (precision == DMD_PREC_C64 ? 8 : 4);
+    const size_t p_words = ((size_t)nslots * 2 * h->nb4 * h->p_esize + 15) / 16;
     if ((rc = dalloc_t(&h->d_e, (size_t)nk * nb, "e")) || (rc = dalloc_t(&h->d_edm, (size_t)nk * nb, "e_dm")) ||
-        (rc = dalloc_t(&h->d_P, (size_t)nslots * 2 * h->nb4, "P")) ||
+        (rc = dalloc_t(&h->d_P, p_words, "P")) ||
         (rc = dalloc_t(&h->d_rowptr, nk + 1, "rowptr")) || (rc = dalloc_t(&h->d_revptr, nk + 1, "revptr")) ||
         (rc = dalloc_t(&h->d_revpos, h->nrev, "revpos")) || (rc = dalloc_t(&h->d_revcol, h->nrev, "revcol")) ||
         (rc = dalloc_t(&h->d_col, n_pairs, "col")) || (rc = dalloc_t(&h->d_fwdpos, n_pairs, "fwdpos")) ||
@@ -985,6 +999,8 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
               (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks")) ||
               (rc = dalloc_t(&h->d_gbeg, h->ngroups + 1, "chunk groups"))))
         return rc;
+    if (precision == DMD_PREC_BF16 && (rc = dalloc_t(&h->d_pscale, nslots, "P scales"))) return rc;
+    if (nb == 4 && (rc = dalloc_t(&h->d_hacc16, (size_t)nk * 16, "Hermitian accumulator"))) return rc;
     if (!identity) {
         if ((rc = dalloc_t(&h->d_inv, n_pairs, "inv"))) return rc;
         DMD_HIP(hipMemcpy(h->d_inv, inv.data(), n_pairs * 8, hipMemcpyHostToDevice));
@@ -1010,8 +1026,11 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         DMD_HIP(hipMemcpy(h->d_cij, cij.data(), h->nchunks * 8, hipMemcpyHostToDevice));
         DMD_HIP(hipMemcpy(h->d_gbeg, gbeg.data(), (h->ngroups + 1) * 4, hipMemcpyHostToDevice));
     }
-    if (nslots) DMD_HIP(hipMemset(h->d_P, 0, (size_t)nslots * 2 * h->nb4 * sizeof(double2)));
-    h->soa = (nb == 2 && R) ? 1 : 0;  // the default kernel (tiles) streams the SoA granules
+    if (nslots) DMD_HIP(hipMemset(h->d_P, 0, p_words * sizeof(double2)));
+    if (h->d_pscale) DMD_HIP(hipMemset(h->d_pscale, 0, (size_t)nslots * sizeof(float)));
+    if (h->d_hacc16) DMD_HIP(hipMemset(h->d_hacc16, 0, (size_t)nk * 16 * sizeof(double)));
+    // the default kernel (tiles) streams SoA granules (nb = 2) or blocked records (nb = 4)
+    h->soa = (nb == 2 && R) ? 1 : ((nb == 4 && R) ? 2 : 0);
     for (double2 *b : {h->d_rhoS, h->d_rho1S, h->d_out, h->d_in, h->d_in1}) DMD_HIP(hipMemset(b, 0, ns * sizeof(double2)));
     DMD_HIP(hipMemset(h->d_acc_rows, 0, (size_t)nk * 4 * h->nb2 * sizeof(double)));
     DMD_HIP(hipMemset(h->d_acc_herm, 0, (size_t)nk * h->nb2 * sizeof(double)));
@@ -1058,8 +1077,12 @@ static int put_P(dmd_scatt *h, const double *P1, const double *P2, double scale,
             b = s2;
         }
         int64_t tot = n * 2 * (int64_t)per;
-        hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, f0, h->nb4, a, b, h->d_inv, scale,
-                           add, h->soa, h->d_P);
+        if (h->precision != DMD_PREC_FP64) {
+            if ((rc = tiles4_place(h, a, b, n, f0, scale, add))) { dfree(s1); dfree(s2); return rc; }
+        } else {
+            hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, f0, h->nb4, a, b, h->d_inv,
+                               scale, add, h->soa, h->d_P);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
         if (!dev1 || !dev2) {
@@ -1093,8 +1116,12 @@ extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, 
     if ((rc = dalloc_t(&s1, count * per, "download"))) return rc;
     if ((rc = dalloc_t(&s2, count * per, "download"))) { dfree(s1); return rc; }
     int64_t tot = count * 2 * (int64_t)per;
-    hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, count, first, h->nb4, h->d_P, h->d_inv,
-                       h->soa, s1, s2);
+    if (h->precision != DMD_PREC_FP64) {
+        if ((rc = tiles4_fetch(h, first, count, s1, s2))) { dfree(s1); dfree(s2); return rc; }
+    } else {
+        hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, count, first, h->nb4, h->d_P, h->d_inv,
+                           h->soa, s1, s2);
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(P1, s1, count * per * 16, hipMemcpyDefault, h->st);
     if (e == hipSuccess) e = hipMemcpyAsync(P2, s2, count * per * 16, hipMemcpyDefault, h->st);
@@ -1109,6 +1136,12 @@ extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
     if (!h || !p) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: null argument");
     if (p->nb != h->nb || p->nk != h->nk) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: params do not match the handle");
     if (h->np == 0) { h->has_P = true; return DMD_OK; }
+    if (h->precision != DMD_PREC_FP64) {
+        int rc = tiles4_generate(h, *p);
+        if (rc) return rc;
+        h->has_P = true;
+        return DMD_OK;
+    }
     const unsigned g = nblk(h->nslots, 64);
     switch (h->nb) {
     case 1: hipLaunchKernelGGL(k_generate_P<1>, dim3(g), dim3(64), 0, h->st, h->nslots, *p, h->d_k1, h->d_k2, h->d_e, h->soa, h->d_P); break;
@@ -1127,11 +1160,24 @@ extern "C" int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int 
     if (!h) return fail(DMD_ERR_ARG, "null handle");
     if (kernel < DMD_KERNEL_AUTO || kernel > DMD_KERNEL_TILES) return fail(DMD_ERR_ARG, "unknown kernel");
     if (schrodinger && expt) return fail(DMD_ERR_ARG, "in the schrodinger picture expt must be 0 (parameters.cpp:452-453)");
+    if (h->precision != DMD_PREC_FP64 && kernel != DMD_KERNEL_AUTO && kernel != DMD_KERNEL_TILES)
+        return fail(DMD_ERR_UNSUPPORTED, "the reduced-precision forms run on the tile kernel only");
     h->expt = expt ? 1 : 0;
     h->schrodinger = schrodinger ? 1 : 0;
     h->kernel = kernel;
-    // the tile kernel streams SoA granules, the row kernels pair-contiguous P
-    const int want_soa = contract_kind(h) == DMD_KERNEL_TILES ? 1 : 0;
+    // the tile kernels stream SoA granules (nb = 2) or blocked records (nb = 4),
+    // the row kernels pair-contiguous P
+    const bool tiles = contract_kind(h) == DMD_KERNEL_TILES;
+    if (h->nb == 4) {
+        const int want = tiles ? 2 : 0;
+        if (want != h->soa) {
+            int rc = tiles4_relayout(h, want == 2);
+            if (rc) return rc;
+            h->soa = want;
+        }
+        return DMD_OK;
+    }
+    const int want_soa = tiles ? 1 : 0;
     if (want_soa != h->soa) {
         const int64_t ngran = h->nslots / 8;
         const int ep = 2 * h->nb4;
@@ -1142,6 +1188,23 @@ extern "C" int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int 
         DMD_HIP(hipStreamSynchronize(h->st));
         h->soa = want_soa;
     }
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_set_split(dmd_scatt *h, const double *f)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (h->precision == DMD_PREC_FP64) return DMD_OK;  // the FP64 form does not split rho
+    int rc;
+    if (!f) {
+        dfree(h->d_fsplit);
+        h->d_fsplit = nullptr;
+    } else {
+        if (!h->d_fsplit && (rc = dalloc_t(&h->d_fsplit, (size_t)h->nk * h->nb, "split occupations"))) return rc;
+        if ((rc = to_device(h, h->d_fsplit, f, (size_t)h->nk * h->nb * 8))) return rc;
+        DMD_HIP(hipStreamSynchronize(h->st));
+    }
+    h->feq_dirty = true;
     return DMD_OK;
 }
 
@@ -1228,6 +1291,8 @@ static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
 // per-pair atomic kernel for nb = 1, the row kernel for nb >= 3.
 int contract_kind(const dmd_scatt *h)
 {
+    if (h->nb == 4 && h->tile_R && (h->kernel == DMD_KERNEL_AUTO || h->kernel == DMD_KERNEL_TILES))
+        return DMD_KERNEL_TILES;
     if (h->kernel == DMD_KERNEL_ROWS || h->nb > 2) return DMD_KERNEL_ROWS;
     if (h->kernel == DMD_KERNEL_SINGLE_READ) return DMD_KERNEL_SINGLE_READ;
     return h->nb == 2 && h->tile_R ? DMD_KERNEL_TILES : DMD_KERNEL_SINGLE_READ;
@@ -1237,7 +1302,9 @@ int contract_kind(const dmd_scatt *h)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const int kind = contract_kind(h);
-    if (kind == DMD_KERNEL_TILES) {
+    if (kind == DMD_KERNEL_TILES && h->nb == 4) {
+        return launch_tiles4(h, dm1, a, out);
+    } else if (kind == DMD_KERNEL_TILES) {
         launch_tiles(h, dm1, a, out);
     } else if (kind == DMD_KERNEL_SINGLE_READ) {
         if (h->nb == 1) launch_single_nb<1>(h, dm1, a, out);
@@ -1271,6 +1338,7 @@ static int eval_core(dmd_scatt *h, double t, const double *rho, const double *rh
         if ((rc = to_device(h, h->d_in1, rho1, bytes))) return rc;
         in1 = h->d_in1;
     }
+    if ((rc = tiles4_prepare(h))) return rc;
     if ((rc = set_clock(h, t, 0.0, 0))) return rc;
     if ((rc = launch_rotate(h, in, in1, h->nb, 0, 0.0))) return rc;
     double2 *out = is_device_ptr(ddmdt) ? (double2 *)ddmdt : h->d_out;
@@ -1343,7 +1411,7 @@ extern "C" int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, 
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
     if (n_pairs) *n_pairs = h->np;
-    if (p_bytes) *p_bytes = h->np * 2 * h->nb4 * 16;
+    if (p_bytes) *p_bytes = h->np * 2 * h->nb4 * (int64_t)h->p_esize + (h->d_pscale ? h->np * 4 : 0);
     if (last_kernel_ms) *last_kernel_ms = h->last_ms;
     return DMD_OK;
 }
@@ -1353,6 +1421,7 @@ extern "C" int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel
     if (!h || n < 1) return fail(DMD_ERR_ARG, "dmd_scatt_time_eval: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     int rc;
+    if ((rc = tiles4_prepare(h))) return rc;
     if ((rc = set_clock(h, t, 0.0, 0))) return rc;
     // d_in holds the state of the last dmd_scatt_eval (zeros before any)
     if ((rc = launch_rotate(h, h->d_in, nullptr, h->nb, 0, 0.0))) return rc;

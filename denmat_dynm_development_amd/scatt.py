@@ -108,6 +108,15 @@ class ScatteringOperator:
     def set_flags(self, expt=True, schrodinger=False, kernel=_lib.DMD_KERNEL_AUTO):
         call("dmd_scatt_set_flags", self._h, int(bool(expt)), int(bool(schrodinger)), int(kernel))
 
+    def set_split(self, f):
+        """Occupations f (nk x nb) of the split rho = diag(f) + drho used by the
+        reduced-precision forms (None: f = 0)."""
+        if f is None:
+            call("dmd_scatt_set_split", self._h, None)
+            return
+        p, keep = _ptr(f, np.float64)
+        call("dmd_scatt_set_split", self._h, p)
+
     def set_ddmdt_eq(self, ddmdt_eq):
         p, keep = _ptr(ddmdt_eq, np.complex128)
         call("dmd_scatt_set_ddmdt_eq", self._h, p)

@@ -42,8 +42,8 @@ extern "C" {
 
 /* precision of the streamed P */
 #define DMD_PREC_FP64 0 /* complex128 P, FP64 arithmetic: the drop-in           */
-#define DMD_PREC_C64 1  /* complex64 P, FP32 products, FP64 row sums and state  */
-#define DMD_PREC_BF16 2 /* bf16 re/im P with a per-pair FP32 scale              */
+#define DMD_PREC_C64 1  /* complex64 P, FP32 products, FP64 row sums and state (nb = 4) */
+#define DMD_PREC_BF16 2 /* bf16 re/im P with a per-pair FP32 scale (nb = 4)           */
 
 /* contraction kernel */
 #define DMD_KERNEL_AUTO 0        /* fastest available for the shape               */
@@ -116,6 +116,12 @@ int dmd_scatt_set_ddmdt_eq(dmd_scatt *h, const double *ddmdt_eq);
 /* compute_ddmdt_eq (evolve_eph.cpp:3-19): ddmdt_eq = evolve(t=0, diag f,
  * diag 1-f) and stores it.  f: nk x nb occupations of the window. */
 int dmd_scatt_compute_ddmdt_eq(dmd_scatt *h, const double *f);
+/* Reduced-precision forms (DMD_PREC_C64 / _BF16): the diagonal occupations f
+ * (nk x nb) around which rho is split, rho = diag(f) + drho.  The rows sums at
+ * diag(f) are formed once in FP64 from the stored P, and only drho enters the
+ * FP32 products, as the reference's ddmdt_eq idea separates the equilibrium
+ * part (evolve_eph.cpp:3-19,154-162).  NULL: f = 0.  No effect for FP64. */
+int dmd_scatt_set_split(dmd_scatt *h, const double *f);
 
 /* ---- the hot path --------------------------------------------------------- */
 /* electronphonon::evolve (evolve_eph.cpp:34-166): ddmdt (nk x nb^2, fully

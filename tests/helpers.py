@@ -73,3 +73,26 @@ def gaas_spin_fixture():
     dm[:, :, nv:] = d["dm_cols"]
     return (e, f, s.reshape(nk, nb * nb), dm.reshape(nk, nb * nb), nv, nb, float(d["t"]),
             1.0 / float(d["nk_full"]), float(d["sz"]), float(d["sz_amp"]))
+
+
+def quantise_P(P1, P2, precision):
+    """Stored values of the reduced-precision forms: complex64 (round to
+    nearest), or BF16 re/im (round to nearest even) of P / s with the per-pair
+    power-of-two scale s = 2^e, max |re|, |im| of the pair in [2^(e-1), 2^e)."""
+    P1 = np.asarray(P1, np.complex128)
+    P2 = np.asarray(P2, np.complex128)
+    if precision == 1:
+        return P1.astype(np.complex64).astype(np.complex128), P2.astype(np.complex64).astype(np.complex128)
+    both = np.concatenate([P1, P2], axis=1)
+    mx = np.maximum(np.abs(both.real), np.abs(both.imag)).max(axis=1)
+    _, ex = np.frexp(np.where(mx > 0, mx, 1.0))
+    sc = np.where(mx > 0, np.ldexp(1.0, ex), 1.0).astype(np.float32).astype(np.float64)[:, None]
+
+    def bf16(x):
+        u = (x / sc).astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+        return u.astype(np.uint32).view(np.float32).astype(np.float64) * sc
+
+    q = bf16(both.real) + 1j * bf16(both.imag)
+    n = P1.shape[1]
+    return q[:, :n], q[:, n:]

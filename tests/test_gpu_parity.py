@@ -55,18 +55,19 @@ def test_eval_matches_oracle(nb, t_fs, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("nb", [1, 2])
+@pytest.mark.parametrize("nb", [1, 2, 4])
 def test_eval_many_tiles_unsorted(kernel, nb):
-    """nk spans several 256-row tiles, tiles split into several chunks, pairs
+    """nk spans several row tiles (512 rows at nb <= 2, 32 at nb = 4), pairs
     given in shuffled order (exercises the tile layout and the index lists)."""
     scatt, oracle = _mods()
     print(f"case tiles nb={nb} kernel={kernel}", flush=True)
-    rng, k1, k2, e, P1, P2, rho = _case(700, nb, 9, 40 + nb, diag=True)
+    nk = 700 if nb < 4 else 150
+    rng, k1, k2, e, P1, P2, rho = _case(nk, nb, 9, 40 + nb, diag=True)
     perm = rng.permutation(len(k1))
     k1, k2, P1, P2 = k1[perm], k2[perm], P1[perm], P2[perm]
     t = 211 * FS
-    ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=2 * np.pi / 700)
-    with scatt.ScatteringOperator(700, nb, k1, k2, e) as op:
+    ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=2 * np.pi / nk)
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
         op.upload_P(P1, P2)
         op.set_flags(kernel=kernel)
         got = op.eval(t, rho)
