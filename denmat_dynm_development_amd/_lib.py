@@ -97,6 +97,7 @@ PROTOTYPES = {
     "dmd_scatt_integrate_rk4_batched": [_vp, _i, _d, _d, _i64, _i, _vp, _vp],
     "dmd_scatt_info": [_vp, _pi64, _pi64, _pd],
     "dmd_scatt_time_eval": [_vp, _d, _i, _pd, _pd],
+    "dmd_scatt_time_eval_batched": [_vp, _d, _i, _pd],
     "dmd_synth_preset": [_i, _pSP],
     "dmd_synth_energies": [_pSP, _vp],
     "dmd_synth_kpairs": [_pSP, _vp, _vp, _pi64],

@@ -1,0 +1,385 @@
+// dmd_batched.hip -- B trajectories sharing one pass over P (nb = 4, BF16 P):
+// the per-pair GEMVs of electronphonon::evolve (src_v4.5.7/evolve_eph.cpp:99-106)
+// become GEMMs over the trajectory index and run on the matrix cores.
+//
+// For a destination row k and each of its visits v (a stored pair (k, q) in
+// the forward direction, or (q, k) in the reverse direction):
+//   F1_k[b] += A1_v vec(X_q[b])          A1 = P1      (forward) | conj(P2)   (reverse)
+//   G_k[b]  += A2_v^T vec(X_q[b])        A2 = P2      (forward) | conj(P1)   (reverse)
+// with X = drho_S = rho_S - diag(f) the FP32-class part of the state; the
+// equilibrium part F(diag f, 1 - diag f) is the FP64 F_eq shared with the
+// unbatched reduced forms, and dm1P = F2_eq - G (1 - rho = (1 - f) - drho).
+//
+// Matrix-core mapping (v_mfma_f32_32x32x16_bf16, FP32 accumulation): a
+// complex 16 x 16 matrix times 16 x 32 complex columns is the real 32 x 32
+// product [[Ar, -Ai], [Ai, Ar]] [Xr; Xi].  A is exact in BF16 (the stored
+// P); X is carried as two BF16 terms (hi + lo, a 16-bit significand), so
+// each visit issues 2 K-halves x 2 terms x 2 products = 8 MFMAs per 32
+// trajectories.
+//
+// Layouts in HBM.
+//  * Visit records, CSR by destination row: 2 x 256 words (BF16 re | im << 16)
+//    per visit, A1 row-major then A2^T row-major, conj and the per-pair
+//    power-of-two scale applied (2 KiB per visit, 4 KiB per stored pair).
+//  * X: per partner row q, [term][re/im][k-half][trajectory][8] BF16, so a
+//    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
+//    bytes.
+//  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
+// Uneven rows: every row's visits are cut into segments of at most
+// DMD_SEG visits, one wavefront each, so a 5000-partner row is as many
+// independent work items; segments of one row meet in FP32 atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "dmd_internal.h"
+
+namespace dmd {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float floatx16;
+
+__device__ __forceinline__ uint32_t f2bf(float f)  // round to nearest even (finite)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ __forceinline__ float bflo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bfhi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+// Builds visit records from the blocked BF16 store.  One 64-lane group per visit.
+__global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const int64_t *__restrict__ vslot,
+                                                     const uint8_t *__restrict__ vdir, const uint32_t *__restrict__ Pq,
+                                                     const float *__restrict__ pscale, uint32_t *__restrict__ vrec)
+{
+    const int64_t v = blockIdx.x;
+    if (v >= nvis) return;
+    const int64_t s = vslot[v];
+    const bool rev = vdir[v] != 0;
+    const uint32_t *rec = Pq + s * 512;
+    const float sc = pscale[s];
+    uint32_t *out = vrec + v * 512;
+    for (int t = threadIdx.x; t < 512; t += 64) {
+        const int mat = t >> 8, r = (t >> 4) & 15, c = t & 15;
+        // A1 = P1 | conj(P2); A2^T[r][c] = A2[c][r], A2 = P2 | conj(P1)
+        const int src_mat = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
+        const int row = mat == 0 ? r : c, col = mat == 0 ? c : r;
+        const uint32_t w = rec[blk_off(src_mat, row, col, 4)];
+        const float re = bflo(w) * sc, im = bfhi(w) * sc;
+        out[t] = f2bf(re) | (f2bf(rev ? -im : im) << 16);
+    }
+}
+
+// drho_S of every trajectory as the hi/lo BF16 operand, and rho_S in FP64
+// for the assembly.  rho: B x nk x 16 (interaction picture).  One thread per
+// (b, k, element).
+__global__ void k_prepare_x(int64_t nk, int B, int Bp, const double2 *__restrict__ rho, const double *__restrict__ e,
+                            const DClock *__restrict__ clk, double a, int expt, const double *__restrict__ f,
+                            double2 *__restrict__ rhoS, uint16_t *__restrict__ xb)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)Bp * nk * 16) return;
+    const int64_t bk = idx >> 4;
+    const int b = (int)(bk / nk);
+    const int64_t k = bk - (int64_t)b * nk;
+    const int l = (int)(idx & 15), i = l >> 2, j = l & 3;
+    double2 v = make_double2(0, 0);
+    if (b < B) {
+        v = rho[idx];
+        if (expt && i != j) v = cmul(v, ccis(-(e[k * 4 + i] - e[k * 4 + j]) * clock_t_at(clk, a)));
+        rhoS[idx] = v;
+        if (f && i == j) v.x -= f[k * 4 + i];
+    }
+    // [q][term][re/im][k-half][b][8]
+    const int half = l >> 3, el = l & 7;
+    const float re = (float)v.x, im = (float)v.y;
+    const uint32_t rh = f2bf(re), ih = f2bf(im);
+    const uint32_t rl = f2bf((float)(v.x - (double)__uint_as_float(rh << 16)));
+    const uint32_t il = f2bf((float)(v.y - (double)__uint_as_float(ih << 16)));
+    const int64_t base = k * 8;  // 2 terms x 2 parts x 2 halves
+    auto at = [&](int term, int part) { return ((((base + term * 4 + part * 2 + half) * Bp) + b) * 8 + el); };
+    xb[at(0, 0)] = (uint16_t)rh;
+    xb[at(0, 1)] = (uint16_t)ih;
+    xb[at(1, 0)] = (uint16_t)rl;
+    xb[at(1, 1)] = (uint16_t)il;
+}
+
+__device__ __forceinline__ bf16x8 as_bf8(uint4 u)
+{
+    bf16x8 r;
+    __builtin_memcpy(&r, &u, 16);
+    return r;
+}
+
+// One wavefront per segment; NBLK blocks of 32 trajectories per pass (the
+// grid's y index selects the pass).
+template <int NBLK>
+__global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
+                                                 const uint32_t *__restrict__ vrec, const int32_t *__restrict__ vq,
+                                                 const uint16_t *__restrict__ xb, float *__restrict__ F)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sg >= nseg) return;
+    const int4 S = seg[sg];  // (k, v0 lo, v0 hi, count)
+    const int64_t k = S.x;
+    const int64_t v0 = (int64_t)(uint32_t)S.y | ((int64_t)S.z << 32);
+    const int nv = S.w;
+    const int i = lane & 31, h = lane >> 5;  // A row (real-ified), K half
+    const int r = i & 15, im_row = i >> 4;
+    const int b0 = blockIdx.y * NBLK * 32;
+    floatx16 f1[NBLK], g2[NBLK];
+#pragma unroll
+    for (int nb = 0; nb < NBLK; nb++)
+#pragma unroll
+        for (int t = 0; t < 16; t++) { f1[nb][t] = 0.0f; g2[nb][t] = 0.0f; }
+    for (int u = 0; u < nv; u++) {
+        const int64_t v = v0 + u;
+        const uint32_t *rec = vrec + v * 512;
+        // the lane's 8 elements of row r of A1 and of A2^T (K range 8h .. 8h+7)
+        const uint4 a1a = *(const uint4 *)(rec + r * 16 + 8 * h), a1b = *(const uint4 *)(rec + r * 16 + 8 * h + 4);
+        const uint4 a2a = *(const uint4 *)(rec + 256 + r * 16 + 8 * h),
+                    a2b = *(const uint4 *)(rec + 256 + r * 16 + 8 * h + 4);
+        const int64_t q = vq[v];
+        // split words into re / im BF16 vectors; K chunk 0 multiplies Xr, chunk 1 Xi:
+        // re rows: [Ar | -Ai], im rows: [Ai | Ar]
+        const uint32_t w1[8] = { a1a.x, a1a.y, a1a.z, a1a.w, a1b.x, a1b.y, a1b.z, a1b.w };
+        const uint32_t w2[8] = { a2a.x, a2a.y, a2a.z, a2a.w, a2b.x, a2b.y, a2b.z, a2b.w };
+        uint32_t c0a[4], c1a[4], c0b[4], c1b[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const uint32_t x0 = w1[2 * t], x1 = w1[2 * t + 1], y0 = w2[2 * t], y1 = w2[2 * t + 1];
+            const uint32_t re1 = (x0 & 0xFFFFu) | (x1 << 16), im1 = (x0 >> 16) | (x1 & 0xFFFF0000u);
+            const uint32_t re2 = (y0 & 0xFFFFu) | (y1 << 16), im2 = (y0 >> 16) | (y1 & 0xFFFF0000u);
+            c0a[t] = im_row ? im1 : re1;
+            c1a[t] = im_row ? re1 : (im1 ^ 0x80008000u);
+            c0b[t] = im_row ? im2 : re2;
+            c1b[t] = im_row ? re2 : (im2 ^ 0x80008000u);
+        }
+        const bf16x8 A1c0 = as_bf8(make_uint4(c0a[0], c0a[1], c0a[2], c0a[3]));
+        const bf16x8 A1c1 = as_bf8(make_uint4(c1a[0], c1a[1], c1a[2], c1a[3]));
+        const bf16x8 A2c0 = as_bf8(make_uint4(c0b[0], c0b[1], c0b[2], c0b[3]));
+        const bf16x8 A2c1 = as_bf8(make_uint4(c1b[0], c1b[1], c1b[2], c1b[3]));
+#pragma unroll
+        for (int nb = 0; nb < NBLK; nb++) {
+            const int b = b0 + nb * 32 + i;
+            // X operand: [q][term][re/im][k-half][b][8]
+            const uint16_t *xq = xb + q * 8 * (int64_t)Bp * 8;
+            const bf16x8 xh0 = as_bf8(*(const uint4 *)(xq + ((0 * 2 + 0) * 2 + h) * (int64_t)Bp * 8 + b * 8));
+            const bf16x8 xh1 = as_bf8(*(const uint4 *)(xq + ((0 * 2 + 1) * 2 + h) * (int64_t)Bp * 8 + b * 8));
+            const bf16x8 xl0 = as_bf8(*(const uint4 *)(xq + ((1 * 2 + 0) * 2 + h) * (int64_t)Bp * 8 + b * 8));
+            const bf16x8 xl1 = as_bf8(*(const uint4 *)(xq + ((1 * 2 + 1) * 2 + h) * (int64_t)Bp * 8 + b * 8));
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c0, xh0, f1[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c1, xh1, f1[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c0, xl0, f1[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c1, xl1, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c0, xh0, g2[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xh1, g2[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c0, xl0, g2[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xl1, g2[nb], 0, 0, 0);
+        }
+    }
+    // D layout: element t of lane l is row 8 (t / 4) + 4 h + t % 4, column (trajectory) l % 32
+#pragma unroll
+    for (int nb = 0; nb < NBLK; nb++) {
+        const int b = b0 + nb * 32 + i;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int row = 8 * (t >> 2) + 4 * h + (t & 3);
+            unsafeAtomicAdd(F + (k * 64 + row) * Bp + b, f1[nb][t]);
+            unsafeAtomicAdd(F + (k * 64 + 32 + row) * Bp + b, g2[nb][t]);
+        }
+    }
+}
+
+// ddm of every trajectory: C = (1-rho)(F1_eq + F1) - (F2_eq - G) rho, ddm =
+// prefac/2 (C + C^+), rotated back, minus ddmdt_eq.  Clears F.  One thread
+// per (b, k, i, j); rows of F are read by the 16 threads of (b, k).
+__global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, float *__restrict__ F,
+                                                    const double *__restrict__ feq, const double2 *__restrict__ rhoS,
+                                                    const double *__restrict__ e, const DClock *__restrict__ clk,
+                                                    double a, int expt, int schrodinger, double prefac,
+                                                    const double2 *__restrict__ eq, double2 *__restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t bk = idx >> 4;
+    const int b = (int)(bk / nk);
+    const int64_t k = bk - (int64_t)b * nk;
+    const int l = (int)(idx & 15), i = l >> 2, j = l & 3;
+    const bool ok = b < B;
+    double2 d = make_double2(0, 0);
+    if (ok) {
+        double2 rk[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) rk[q] = rhoS[bk * 16 + q];
+        auto f1 = [&](int m) {  // F1[m] = F1_eq[m] + F[k][m] (re), F[k][16 + m] (im)
+            return make_double2(feq[k * 64 + 2 * m] + F[(k * 64 + m) * Bp + b],
+                                feq[k * 64 + 2 * m + 1] + F[(k * 64 + 16 + m) * Bp + b]);
+        };
+        auto f2 = [&](int m) {  // F2[m] = F2_eq[m] - G[m]
+            return make_double2(feq[k * 64 + 32 + 2 * m] - F[(k * 64 + 32 + m) * Bp + b],
+                                feq[k * 64 + 32 + 2 * m + 1] - F[(k * 64 + 48 + m) * Bp + b]);
+        };
+        auto cel = [&](int ii, int jj) {
+            double2 x = make_double2(0, 0);
+            for (int m = 0; m < 4; m++) {
+                double2 r1;  // (1 - rho) upper-triangle Hermitian, as zhemm reads it
+                if (ii < m) r1 = make_double2(-rk[ii * 4 + m].x, -rk[ii * 4 + m].y);
+                else if (ii > m) r1 = make_double2(-rk[m * 4 + ii].x, rk[m * 4 + ii].y);
+                else r1 = make_double2(1.0 - rk[ii * 5].x, 0.0);
+                cfma(x, r1, f1(4 * m + jj));
+            }
+            for (int m = 0; m < 4; m++) {
+                double2 rr;
+                if (m < jj) rr = rk[m * 4 + jj];
+                else if (m > jj) rr = cconj(rk[jj * 4 + m]);
+                else rr = make_double2(rk[m * 5].x, 0.0);
+                const double2 g = f2(4 * ii + m);
+                cfma(x, make_double2(-g.x, -g.y), rr);
+            }
+            return x;
+        };
+        d = cadd(cel(i, j), cconj(cel(j, i)));
+    }
+    __syncthreads();  // all rows of F read before they are cleared
+    if (!ok) return;
+    if (b == 0 || true) {
+        // thread l of (b, k) clears rows 4l .. 4l+3 of F for trajectory b
+#pragma unroll
+        for (int q = 0; q < 4; q++) F[(k * 64 + 4 * l + q) * Bp + b] = 0.0f;
+    }
+    d = make_double2(d.x * (prefac * 0.5), d.y * (prefac * 0.5));
+    const double t = clock_t_at(clk, a);
+    const double arg = (e[k * 4 + i] - e[k * 4 + j]) * t;
+    if (expt && i != j) d = cmul(d, ccis(arg));
+    if (eq) {
+        double2 q = eq[k * 16 + l];
+        if (i != j && !schrodinger) q = cmul(q, ccis(arg));
+        d = csub(d, q);
+    }
+    out[bk * 16 + l] = d;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define DMD_SEG 128
+
+// visit records, segments (built once per P; rebuilt when P changes)
+static int build_visits(dmd_scatt *h)
+{
+    if (h->bvis_ready && !h->bvis_dirty) return DMD_OK;
+    const int64_t nk = h->nk;
+    std::vector<int64_t> rowptr(nk + 1), revptr(nk + 1);
+    DMD_HIP(hipMemcpy(rowptr.data(), h->d_rowptr, (nk + 1) * 8, hipMemcpyDeviceToHost));
+    DMD_HIP(hipMemcpy(revptr.data(), h->d_revptr, (nk + 1) * 8, hipMemcpyDeviceToHost));
+    std::vector<int64_t> fwdpos(h->np), revpos(h->nrev);
+    std::vector<int32_t> col(h->np), revcol(h->nrev);
+    if (h->np) {
+        DMD_HIP(hipMemcpy(fwdpos.data(), h->d_fwdpos, h->np * 8, hipMemcpyDeviceToHost));
+        DMD_HIP(hipMemcpy(col.data(), h->d_col, h->np * 4, hipMemcpyDeviceToHost));
+    }
+    if (h->nrev) {
+        DMD_HIP(hipMemcpy(revpos.data(), h->d_revpos, h->nrev * 8, hipMemcpyDeviceToHost));
+        DMD_HIP(hipMemcpy(revcol.data(), h->d_revcol, h->nrev * 4, hipMemcpyDeviceToHost));
+    }
+    const int64_t nvis = h->np + h->nrev;
+    std::vector<int64_t> vslot(nvis);
+    std::vector<uint8_t> vdir(nvis);
+    std::vector<int32_t> vq(nvis);
+    std::vector<int4> seg;
+    int64_t v = 0;
+    for (int64_t k = 0; k < nk; k++) {
+        const int64_t start = v;
+        for (int64_t p = rowptr[k]; p < rowptr[k + 1]; p++, v++) { vslot[v] = fwdpos[p]; vdir[v] = 0; vq[v] = col[p]; }
+        for (int64_t p = revptr[k]; p < revptr[k + 1]; p++, v++) { vslot[v] = revpos[p]; vdir[v] = 1; vq[v] = revcol[p]; }
+        for (int64_t s = start; s < v; s += DMD_SEG) {
+            const int n = (int)std::min<int64_t>(DMD_SEG, v - s);
+            seg.push_back(make_int4((int)k, (int)(uint32_t)(s & 0xFFFFFFFF), (int)(s >> 32), n));
+        }
+    }
+    // longest segments first (only row tails are shorter)
+    std::stable_sort(seg.begin(), seg.end(), [](const int4 &a, const int4 &b) { return a.w > b.w; });
+    int rc;
+    dfree(h->d_vrec); dfree(h->d_vq); dfree(h->d_seg);
+    h->d_vrec = nullptr; h->d_vq = nullptr; h->d_seg = nullptr;
+    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * 512, "visit records"))) return rc;
+    if ((rc = dalloc_t(&h->d_vq, (size_t)nvis, "visit partners"))) return rc;
+    if ((rc = dalloc_t(&h->d_seg, seg.size(), "segments"))) return rc;
+    int64_t *d_vslot = nullptr;
+    uint8_t *d_vdir = nullptr;
+    if ((rc = dalloc_t(&d_vslot, (size_t)nvis, "visit slots"))) return rc;
+    if ((rc = dalloc_t(&d_vdir, (size_t)nvis, "visit directions"))) { dfree(d_vslot); return rc; }
+    hipError_t e = hipSuccess;
+    if (nvis) {
+        e = hipMemcpy(d_vslot, vslot.data(), nvis * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_vdir, vdir.data(), nvis, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && !seg.empty()) e = hipMemcpy(h->d_seg, seg.data(), seg.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nvis) {
+        hipLaunchKernelGGL(k_build_visits, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
+                           (const uint32_t *)h->d_P, h->d_pscale, h->d_vrec);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    }
+    dfree(d_vslot); dfree(d_vdir);
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("building visit records: ") + hipGetErrorString(e));
+    h->nseg = (int64_t)seg.size();
+    h->bvis_ready = true;
+    h->bvis_dirty = false;
+    return DMD_OK;
+}
+
+int batched_prepare(dmd_scatt *h, int B)
+{
+    int rc;
+    if ((rc = tiles4_prepare(h))) return rc;
+    if ((rc = build_visits(h))) return rc;
+    const int Bp = (B + 31) / 32 * 32;
+    if (Bp > h->b_cap) {
+        dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB);
+        h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr;
+        h->b_cap = 0;
+        if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 64 * Bp, "batched X"))) return rc;
+        if ((rc = dalloc_t(&h->d_Fb, (size_t)h->nk * 64 * Bp, "batched row sums"))) return rc;
+        if ((rc = dalloc_t(&h->d_rhoSB, (size_t)h->nk * 16 * Bp, "batched rho_S"))) return rc;
+        DMD_HIP(hipMemset(h->d_Fb, 0, (size_t)h->nk * 64 * Bp * sizeof(float)));
+        DMD_HIP(hipMemset(h->d_xb, 0, (size_t)h->nk * 64 * Bp * sizeof(uint16_t)));
+        h->b_cap = Bp;
+    }
+    return DMD_OK;
+}
+
+// ddm of B trajectories (rho: B x nk x 16 device, interaction picture at the
+// clock time a) into out (B x nk x 16).  The X and row-sum buffers are sized
+// for the padded B the last batched_prepare saw.
+int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *out)
+{
+    const int Bp = (B + 31) / 32 * 32;
+    if (Bp > h->b_cap) return fail(DMD_ERR_STATE, "batched buffers too small");
+    const int64_t nx = (int64_t)Bp * h->nk * 16;
+    hipLaunchKernelGGL(k_prepare_x, dim3(nblk(nx, 256)), dim3(256), 0, h->st, h->nk, B, Bp, rho, h->d_e, h->d_clk, a,
+                       h->expt, h->d_fsplit, h->d_rhoSB, (uint16_t *)h->d_xb);
+    if (h->nseg > 0) {
+        const int nblk2 = (Bp / 32) % 2 == 0 && h->batch_nblk == 2 ? 2 : 1;
+        dim3 g(nblk(h->nseg, 4), Bp / (32 * nblk2));
+        if (nblk2 == 2)
+            hipLaunchKernelGGL(k_batched<2>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                               (const uint16_t *)h->d_xb, h->d_Fb);
+        else
+            hipLaunchKernelGGL(k_batched<1>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                               (const uint16_t *)h->d_xb, h->d_Fb);
+    }
+    const int64_t nout = (int64_t)B * h->nk * 16;
+    hipLaunchKernelGGL(k_assemble_b, dim3(nblk(nout, 256)), dim3(256), 0, h->st, h->nk, B, Bp, h->d_Fb, h->d_feq,
+                       h->d_rhoSB, h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac,
+                       h->has_eq ? h->d_eq : nullptr, out);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+}  // namespace dmd

@@ -69,6 +69,7 @@ __device__ double2 coherent_elem(const double2 *__restrict__ Hk, const double *_
 
 struct StageArgs {
     int64_t nk;
+    int64_t nkb;        // states: nk, or B nk for B trajectories stored one after another
     int nbd, nb, b0;
     const double2 *H;   // nk x nbd^2 or null
     int coh;            // coherent term active
@@ -80,14 +81,15 @@ struct StageArgs {
     double dt;
 };
 
-__device__ __forceinline__ double2 rhs_elem(const StageArgs &A, int64_t k, int i, int j, const double2 *yk,
-                                            const double2 *__restrict__ w, double t)
+// right-hand side element (i, j) of state kk (row k = kk mod nk)
+__device__ __forceinline__ double2 rhs_elem(const StageArgs &A, int64_t k, int64_t kk, int i, int j,
+                                            const double2 *yk, const double2 *__restrict__ w, double t)
 {
     double2 kv = make_double2(0, 0);
     const int n = A.nbd;
     if (A.coh) kv = coherent_elem(A.H ? A.H + k * n * n : nullptr, A.edm + k * n, yk, n, i, j, t, A.schrod);
     int wi = i - A.b0, wj = j - A.b0;
-    if (wi >= 0 && wi < A.nb && wj >= 0 && wj < A.nb) kv = cadd(kv, w[k * A.nb * A.nb + wi * A.nb + wj]);
+    if (wi >= 0 && wi < A.nb && wj >= 0 && wj < A.nb) kv = cadd(kv, w[kk * A.nb * A.nb + wi * A.nb + wj]);
     return kv;
 }
 
@@ -95,7 +97,7 @@ __device__ __forceinline__ void rotate_out(const StageArgs &A, int64_t k, int i,
                                            double2 *__restrict__ rhoS)
 {
     int wi = i - A.b0, wj = j - A.b0;
-    if (wi < 0 || wi >= A.nb || wj < 0 || wj >= A.nb) return;
+    if (!rhoS || wi < 0 || wi >= A.nb || wj < 0 || wj >= A.nb) return;
     if (A.expt && wi != wj) v = cmul(v, ccis(-(A.e[k * A.nb + wi] - A.e[k * A.nb + wj]) * t));
     rhoS[k * A.nb * A.nb + wi * A.nb + wj] = v;
 }
@@ -109,10 +111,10 @@ __global__ void k_rk_stage(StageArgs A, int s, const double2 *__restrict__ yin, 
     const double a_s[4] = { 0.0, 0.5, 0.5, 1.0 }, w_s[4] = { 1.0, 2.0, 2.0, 1.0 };
     const int n = A.nbd, n2 = n * n;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= A.nk * n2) return;
-    int64_t k = idx / n2;
-    int ij = (int)(idx - k * n2), i = ij / n, j = ij - i * n;
-    double2 kv = rhs_elem(A, k, i, j, yin + k * n2, w, clock_t_at(A.clk, a_s[s]));
+    if (idx >= A.nkb * n2) return;
+    int64_t kk = idx / n2, k = kk % A.nk;
+    int ij = (int)(idx - kk * n2), i = ij / n, j = ij - i * n;
+    double2 kv = rhs_elem(A, k, kk, i, j, yin + kk * n2, w, clock_t_at(A.clk, a_s[s]));
     acc[idx] = s == 0 ? kv : cadd(acc[idx], make_double2(w_s[s] * kv.x, w_s[s] * kv.y));
     const double c = a_s[s + 1] * A.dt;
     double2 y = make_double2(y0[idx].x + c * kv.x, y0[idx].y + c * kv.y);
@@ -127,18 +129,18 @@ __global__ void k_rk_final(StageArgs A, const double2 *__restrict__ yin, const d
 {
     const int n = A.nbd, n2 = n * n;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= A.nk * n2) return;
-    int64_t k = idx / n2;
-    int ij = (int)(idx - k * n2), i = ij / n, j = ij - i * n;
+    if (idx >= A.nkb * n2) return;
+    int64_t kk = idx / n2, k = kk % A.nk;
+    int ij = (int)(idx - kk * n2), i = ij / n, j = ij - i * n;
     const double t = clock_t_at(A.clk, 1.0);
     const double c = A.dt / 6.0;
-    const double2 *yk = yin + k * n2;
-    double2 kij = rhs_elem(A, k, i, j, yk, w, t);
+    const double2 *yk = yin + kk * n2;
+    double2 kij = rhs_elem(A, k, kk, i, j, yk, w, t);
     double2 aij = cadd(acc[idx], kij);
     double2 yij = make_double2(y0[idx].x + c * aij.x, y0[idx].y + c * aij.y);
     if (i == j) { ynew[idx] = make_double2(yij.x, 0.0); return; }
-    int64_t jdx = k * n2 + j * n + i;
-    double2 kji = rhs_elem(A, k, j, i, yk, w, t);
+    int64_t jdx = kk * n2 + j * n + i;
+    double2 kji = rhs_elem(A, k, kk, j, i, yk, w, t);
     double2 aji = cadd(acc[jdx], kji);
     double2 yji = make_double2(y0[jdx].x + c * aji.x, y0[jdx].y + c * aji.y);
     double2 u = i < j ? yij : yji, l = i < j ? yji : yij;
@@ -152,9 +154,9 @@ __global__ void k_rk_finish(StageArgs A, const double2 *__restrict__ ynew, doubl
 {
     const int n = A.nbd, n2 = n * n;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= A.nk * n2) return;
-    int64_t k = idx / n2;
-    int ij = (int)(idx - k * n2), i = ij / n, j = ij - i * n;
+    if (idx >= A.nkb * n2) return;
+    int64_t kk = idx / n2, k = kk % A.nk;
+    int ij = (int)(idx - kk * n2), i = ij / n, j = ij - i * n;
     double2 v = ynew[idx];
     y0[idx] = v;
     const double tn = (A.clk->t0 + (double)(A.clk->step + 1) * A.clk->dt) + 0.0 * A.clk->dt;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(256) void k_obs_final(int nparts, const double *__r
 static StageArgs stage_args(dmd_scatt *h, double dt)
 {
     StageArgs A;
-    A.nk = h->nk; A.nbd = h->nbd; A.nb = h->nb; A.b0 = h->b0;
+    A.nk = h->nk; A.nkb = h->nk; A.nbd = h->nbd; A.nb = h->nb; A.b0 = h->b0;
     A.H = h->has_H ? h->d_H : nullptr;
     A.coh = (h->has_H || h->schrodinger) ? 1 : 0;
     A.schrod = h->schrodinger;
@@ -436,14 +438,174 @@ extern "C" int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64
     return DMD_OK;
 }
 
-extern "C" int dmd_scatt_eval_batched(dmd_scatt *h, double, int, const double *, double *)
+namespace dmd {
+// B x nk x nb^2 staging of batched inputs and outputs
+static int ensure_bio(dmd_scatt *h, int B)
 {
-    if (!h) return fail(DMD_ERR_ARG, "null handle");
-    return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_eval_batched: batched contraction not built yet");
+    const int64_t n = (int64_t)B * h->nk * h->nb2;
+    if (h->bio_cap >= n) return DMD_OK;
+    dfree(h->d_bin); dfree(h->d_bout);
+    h->d_bin = h->d_bout = nullptr;
+    h->bio_cap = 0;
+    int rc;
+    if ((rc = dalloc_t(&h->d_bin, n, "batched input")) || (rc = dalloc_t(&h->d_bout, n, "batched output"))) return rc;
+    h->bio_cap = n;
+    return DMD_OK;
+}
+static int ensure_rkb(dmd_scatt *h, int B)
+{
+    const int64_t n = (int64_t)B * h->nk * h->nb2;
+    if (h->rkb_cap >= n) return DMD_OK;
+    for (double2 **p : {&h->d_yB0, &h->d_yBa, &h->d_yBb, &h->d_accB, &h->d_wB}) { dfree(*p); *p = nullptr; }
+    h->rkb_cap = 0;
+    int rc;
+    for (double2 **p : {&h->d_yB0, &h->d_yBa, &h->d_yBb, &h->d_accB, &h->d_wB})
+        if ((rc = dalloc_t(p, n, "batched rk state"))) return rc;
+    if (!h->d_part && (rc = dalloc_t(&h->d_part, nblk(h->nk, 256), "obs partials"))) return rc;
+    h->rkb_cap = n;
+    return DMD_OK;
+}
+static bool batched_native(const dmd_scatt *h) { return h->precision == DMD_PREC_BF16 && h->nb == 4; }
+
+// one RK4 step of B trajectories (stage kernels over B nk states)
+static int launch_step_b(dmd_scatt *h, int B, const StageArgs &A)
+{
+    const int64_t n = A.nkb * (int64_t)h->nb2;
+    dim3 g(nblk(n, 256)), b(256);
+    const double a_s[4] = { 0.0, 0.5, 0.5, 1.0 };
+    double2 *yin[4] = { h->d_yB0, h->d_yBa, h->d_yBb, h->d_yBa };
+    double2 *yout[3] = { h->d_yBa, h->d_yBb, h->d_yBa };
+    int rc;
+    for (int s = 0; s < 4; s++) {
+        if ((rc = launch_batched(h, B, yin[s], a_s[s], h->d_wB))) return rc;
+        if (s < 3)
+            hipLaunchKernelGGL(k_rk_stage, g, b, 0, h->st, A, s, yin[s], h->d_wB, h->d_yB0, h->d_accB, yout[s],
+                               (double2 *)nullptr);
+        else
+            hipLaunchKernelGGL(k_rk_final, g, b, 0, h->st, A, yin[3], h->d_wB, h->d_yB0, h->d_accB, h->d_yBb);
+    }
+    hipLaunchKernelGGL(k_rk_finish, g, b, 0, h->st, A, h->d_yBb, h->d_yB0, (double2 *)nullptr);
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->st, h->d_clk);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+}  // namespace dmd
+
+extern "C" int dmd_scatt_eval_batched(dmd_scatt *h, double t, int B, const double *rho_B, double *ddmdt_B)
+{
+    if (!h || B < 1 || !rho_B || !ddmdt_B) return fail(DMD_ERR_ARG, "dmd_scatt_eval_batched: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    if (h->nbd != h->nb) return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_eval_batched: states are nk x nb^2 (no band window)");
+    const size_t per = (size_t)h->nk * h->nb2 * 2;  // doubles per trajectory
+    int rc;
+    if (!batched_native(h)) {  // one evaluation per trajectory
+        for (int b = 0; b < B; b++)
+            if ((rc = dmd_scatt_eval(h, t, rho_B + b * per, nullptr, ddmdt_B + b * per))) return rc;
+        return DMD_OK;
+    }
+    if ((rc = batched_prepare(h, B)) || (rc = ensure_bio(h, B))) return rc;
+    const size_t bytes = per * 8 * B;
+    // the state always lands in d_bin (dmd_scatt_time_eval_batched replays it)
+    if ((rc = to_device(h, h->d_bin, rho_B, bytes))) return rc;
+    double2 *out = is_device_ptr(ddmdt_B) ? (double2 *)ddmdt_B : h->d_bout;
+    if ((rc = set_clock(h, t, 0.0, 0))) return rc;
+    if ((rc = launch_batched(h, B, h->d_bin, 0.0, out))) return rc;
+    if (out == h->d_bout && (rc = from_device(h, ddmdt_B, h->d_bout, bytes))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    h->last_B = B;
+    return DMD_OK;
 }
 
-extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int, double, double, int64_t, int, double *, double *)
+extern "C" int dmd_scatt_time_eval_batched(dmd_scatt *h, double t, int n, double *kernel_ms)
 {
-    if (!h) return fail(DMD_ERR_ARG, "null handle");
-    return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_integrate_rk4_batched: batched contraction not built yet");
+    if (!h || n < 1) return fail(DMD_ERR_ARG, "dmd_scatt_time_eval_batched: bad arguments");
+    if (!batched_native(h) || h->last_B < 1)
+        return fail(DMD_ERR_STATE, "dmd_scatt_time_eval_batched: needs a BF16 nb = 4 handle after dmd_scatt_eval_batched");
+    int rc;
+    const int B = h->last_B;
+    if ((rc = set_clock(h, t, 0.0, 0))) return rc;
+    if ((rc = launch_batched(h, B, h->d_bin, 0.0, h->d_bout))) return rc;  // warm
+    DMD_HIP(hipStreamSynchronize(h->st));
+    DMD_HIP(hipEventRecord(h->ev0, h->st));
+    for (int i = 0; i < n; i++)
+        if ((rc = launch_batched(h, B, h->d_bin, 0.0, h->d_bout))) return rc;
+    DMD_HIP(hipEventRecord(h->ev1, h->st));
+    DMD_HIP(hipEventSynchronize(h->ev1));
+    float ms = 0;
+    DMD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    if (kernel_ms) *kernel_ms = ms / n;
+    h->last_ms = ms / n;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
+                                               int report_every, double *rho_B, double *obs_B)
+{
+    if (!h || B < 1 || !rho_B || n_steps < 0) return fail(DMD_ERR_ARG, "dmd_scatt_integrate_rk4_batched: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    const bool want_obs = report_every > 0 && h->has_obs;
+    if (want_obs && !obs_B) return fail(DMD_ERR_ARG, "obs_B is null");
+    const int64_t nobs = want_obs ? n_steps / report_every + 1 : 0;
+    const size_t per = (size_t)h->nk * h->nbd * h->nbd * 2;
+    int rc;
+    if (!batched_native(h)) {  // one trajectory after another
+        for (int b = 0; b < B; b++)
+            if ((rc = dmd_scatt_integrate_rk4(h, t0, dt, n_steps, report_every, rho_B + b * per,
+                                              want_obs ? obs_B + b * nobs : nullptr)))
+                return rc;
+        return DMD_OK;
+    }
+    if (h->nbd != h->nb) return fail(DMD_ERR_UNSUPPORTED, "batched RK4: states are nk x nb^2 (no band window)");
+    if ((rc = batched_prepare(h, B)) || (rc = ensure_rkb(h, B))) return rc;
+    double *d_obs = nullptr;
+    if (nobs && (rc = dalloc_t(&d_obs, (size_t)nobs * B, "obs series"))) return rc;
+    struct Guard { double *p; ~Guard() { dfree(p); } } guard{ d_obs };
+    const size_t bytes = per * 8 * B;
+    if ((rc = to_device(h, h->d_yB0, rho_B, bytes))) return rc;
+    if ((rc = set_clock(h, t0, dt, 0))) return rc;
+    StageArgs A = stage_args(h, dt);
+    A.nkb = h->nk * (int64_t)B;
+    const size_t n1 = (size_t)h->nk * h->nb2;
+    auto report = [&](int64_t iobs) -> int {
+        for (int b = 0; b < B; b++) {
+            int r = launch_obs(h, h->d_yB0 + b * n1, d_obs, (int)(b * nobs + iobs));
+            if (r) return r;
+        }
+        return DMD_OK;
+    };
+    int64_t iobs = 0;
+    if (want_obs && (rc = report(iobs++))) return rc;
+    const char *env = std::getenv("DMD_GRAPH");
+    const bool use_graph = !(env && env[0] == '0') && n_steps > 1;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph) {
+        hipGraph_t graph = nullptr;
+        DMD_HIP(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        rc = launch_step_b(h, B, A);
+        hipError_t e = hipStreamEndCapture(h->st, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    for (int64_t it = 0; it < n_steps; it++) {
+        if (exec) {
+            hipError_t e = hipGraphLaunch(exec, h->st);
+            if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+        } else if ((rc = launch_step_b(h, B, A))) {
+            return rc;
+        }
+        if (want_obs && (it + 1) % report_every == 0 && (rc = report(iobs++))) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            return rc;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(h->st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if ((rc = from_device(h, rho_B, h->d_yB0, bytes))) return rc;
+    if (nobs) DMD_HIP(hipMemcpyAsync(obs_B, d_obs, (size_t)nobs * B * 8, hipMemcpyDeviceToHost, h->st));
+    DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
 }

@@ -139,6 +139,22 @@ struct dmd_scatt {
     float2 *d_x32 = nullptr;             // nk x 16: drho_S in FP32 for the reduced forms
     double *d_hacc16 = nullptr;          // nk x 16 packed Hermitian row accumulator (nb = 4)
     bool feq_dirty = true;
+    // batched trajectories (BF16, nb = 4): visit records, segments, operands
+    uint32_t *d_vrec = nullptr;          // visits x 512 words: A1 | A2^T (BF16 re | im << 16)
+    int32_t *d_vq = nullptr;             // partner row of each visit
+    int4 *d_seg = nullptr;               // segments: (row, first visit lo, hi, count)
+    int64_t nseg = 0;
+    bool bvis_ready = false, bvis_dirty = true;
+    int b_cap = 0;                       // padded trajectory count the buffers hold
+    int batch_nblk = 2;                  // 32-trajectory blocks per wavefront
+    uint16_t *d_xb = nullptr;            // nk x 64 x Bp BF16 hi/lo drho operand
+    float *d_Fb = nullptr;               // nk x 64 x Bp row sums
+    double2 *d_rhoSB = nullptr;          // Bp x nk x 16 rho_S
+    double2 *d_bin = nullptr, *d_bout = nullptr;  // B x nk x 16 staging
+    int64_t bio_cap = 0;
+    double2 *d_yB0 = nullptr, *d_yBa = nullptr, *d_yBb = nullptr, *d_accB = nullptr, *d_wB = nullptr;
+    int64_t rkb_cap = 0;
+    int last_B = 0;                      // trajectories of the last dmd_scatt_eval_batched
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
@@ -192,4 +208,7 @@ int tiles4_generate(dmd_scatt *h, const dmd_synth_params &p);
 int tiles4_relayout(dmd_scatt *h, int to_blocked);
 int tiles4_prepare(dmd_scatt *h);  // F_eq of the reduced forms when P or f changed (not capturable)
 int launch_tiles4(dmd_scatt *h, bool dm1, double a, double2 *out);
+// batched path (dmd_batched.hip)
+int batched_prepare(dmd_scatt *h, int B);  // not capturable
+int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *out);
 }  // namespace dmd

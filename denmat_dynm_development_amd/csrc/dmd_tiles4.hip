@@ -587,6 +587,7 @@ int tiles4_place(dmd_scatt *h, const double2 *P1, const double2 *P2, int64_t n, 
                            add, (bf16c *)h->d_P, h->d_pscale);
     DMD_HIP(hipGetLastError());
     h->feq_dirty = true;
+    h->bvis_dirty = true;
     return DMD_OK;
 }
 
@@ -626,6 +627,7 @@ int tiles4_generate(dmd_scatt *h, const dmd_synth_params &p)
     dfree(scratch);
     if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
     h->feq_dirty = true;
+    h->bvis_dirty = true;
     return DMD_OK;
 }
 

@@ -224,6 +224,12 @@ class ScatteringOperator:
         call("dmd_scatt_time_eval", self._h, float(t), int(n), ctypes.byref(k), ctypes.byref(e))
         return k.value, e.value
 
+    def time_eval_batched(self, t, n):
+        """ms per batched evaluation of the states of the last eval_batched."""
+        k = ctypes.c_double()
+        call("dmd_scatt_time_eval_batched", self._h, float(t), int(n), ctypes.byref(k))
+        return k.value
+
 
 class ElectronPhonon(ScatteringOperator):
     """Drop-in shaped like the reference's electronphonon for the driver.

@@ -128,8 +128,11 @@ int dmd_scatt_set_split(dmd_scatt *h, const double *f);
  * overwritten) from rho (nk x nb^2) at time t.  one_minus_rho may be NULL,
  * then 1-rho is formed as set_oneminusdm does (DenMat.cpp:249-261). */
 int dmd_scatt_eval(dmd_scatt *h, double t, const double *rho, const double *one_minus_rho, double *ddmdt);
-/* B independent states sharing one pass over P: rho_B, ddmdt_B are
- * B x nk x nb^2 (one_minus_rho formed internally). */
+/* evolve (evolve_eph.cpp:34-166) for B independent states: rho_B, ddmdt_B
+ * are B x nk x nb^2 (one_minus_rho formed internally; no band window).  On
+ * a DMD_PREC_BF16 nb = 4 handle the B states share one pass over P on the
+ * matrix cores (drho as two BF16 terms, FP32 sums, FP64 assembly); on other
+ * handles it evaluates the states one after another. */
 int dmd_scatt_eval_batched(dmd_scatt *h, double t, int B, const double *rho_B, double *ddmdt_B);
 /* The initializer's one-step form over all stored pairs
  * (lindbladInit.../ElectronPhonon.h:774-862 without the file reads): ddm +=
@@ -162,8 +165,10 @@ int dmd_scatt_measure(dmd_scatt *h, double t, const double *rho, double *value);
  * (n_steps/report_every + 1 values). */
 int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps, int report_every,
                             double *rho_inout, double *obs_out);
-/* B trajectories in one pass over P per stage (rho_inout: B x nk x nb_dm^2,
- * obs_out: B x (n_steps/report_every + 1)). */
+/* B trajectories (rho_inout: B x nk x nb_dm^2, obs_out: B x (n_steps /
+ * report_every + 1)).  BF16 nb = 4 handles: one pass over P per RK stage for
+ * all B (a captured HIP graph per step; no band window); other handles:
+ * dmd_scatt_integrate_rk4 per trajectory. */
 int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
                                     int report_every, double *rho_inout, double *obs_out);
 
@@ -175,6 +180,10 @@ int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, double *las
  * (rho from the last eval); returns the mean ms per eval of the dominant
  * kernel and of the whole eval, measured with HIP events on the handle's stream. */
 int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel_ms, double *eval_ms);
+/* Times n batched evaluations (X operand, matrix-core pass, assembly) of the
+ * states of the last dmd_scatt_eval_batched (BF16 nb = 4 handles); no
+ * reference counterpart. */
+int dmd_scatt_time_eval_batched(dmd_scatt *h, double t, int n, double *kernel_ms);
 
 /* ---- synthetic ldbd_data ------------------------------------------------------- */
 /* Presets: 1..5 = the five benchmark workloads (see the package docs); 0 = a

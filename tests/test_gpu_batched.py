@@ -1,0 +1,125 @@
+"""Batched trajectories (B states sharing one pass over P on the matrix cores,
+BF16 P, nb = 4) through the C-ABI, against the FP64 oracle on the stored P
+and against the unbatched reduced-precision path, one trajectory at a time.
+
+The batched form carries drho = rho - diag(f) as two BF16 terms (a 16-bit
+significand) with FP32 sums; everything else matches the unbatched BF16 form.
+Tolerances are stated against the part of ddm that drho drives.
+"""
+import numpy as np
+import pytest
+
+from helpers import FS, random_P, random_energies, random_pairs, random_rho, rel_err
+
+pytestmark = pytest.mark.gpu
+BF16 = 2
+
+
+def _mods():
+    from denmat_dynm_development_amd import scatt
+    from oracle import oracle
+    return scatt, oracle
+
+
+def _hub_pairs(nk, npart, rng):
+    """random pairs plus a hub row 0 linked to every k (a row of nk visits, several segments)"""
+    k1, k2 = random_pairs(nk, npart, rng, diag=True)
+    s = set(zip(k1.tolist(), k2.tolist())) | {(0, k) for k in range(nk)} | {(k, nk - 1) for k in range(nk)}
+    pr = sorted(s)
+    return np.array([a for a, _ in pr], np.uint64), np.array([b for _, b in pr], np.uint64)
+
+
+def _states(B, nk, nb, f, rng, amp=1e-3):
+    rho = np.zeros((B, nk, nb, nb), np.complex128)
+    rho[:, :, np.arange(nb), np.arange(nb)] = f
+    rho = rho.reshape(B, nk, nb * nb)
+    for b in range(B):
+        rho[b] += random_rho(nk, nb, rng, scale=amp * (1 + b % 5), diag=0.0)
+    return rho
+
+
+@pytest.mark.parametrize("B", [3, 40, 64])
+def test_eval_batched_matches_oracle_and_unbatched(B):
+    scatt, oracle = _mods()
+    print(f"case batched eval B={B}", flush=True)
+    rng = np.random.default_rng(100 + B)
+    nk, nb = 300, 4
+    k1, k2 = _hub_pairs(nk, 3, rng)
+    perm = rng.permutation(len(k1))
+    k1, k2 = k1[perm], k2[perm]
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(B, nk, nb, f, rng)
+    eq = random_rho(nk, nb, rng, scale=1e-6, diag=1e-7)
+    t = 77 * FS
+    prefac = 2 * np.pi / nk
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=BF16) as op:
+        op.upload_P(P1, P2)
+        op.set_split(f)
+        op.set_ddmdt_eq(eq)
+        Q1, Q2 = op.download_P()
+        got = op.eval_batched(t, rho)
+        one = np.stack([op.eval(t, rho[b]) for b in range(B)])
+        rho_f = np.zeros((nk, nb * nb), np.complex128)
+        rho_f[:, ::nb + 1] = f
+        base = op.eval(t, rho_f)
+    worst = 0.0
+    for b in sorted({0, B // 2, B - 1}):
+        ref = oracle.evolve(k1, k2, Q1, Q2, e, rho[b], t=t, prefac=prefac, ddmdt_eq=eq)
+        ref_f = oracle.evolve(k1, k2, Q1, Q2, e, rho_f, t=t, prefac=prefac, ddmdt_eq=eq)
+        worst = max(worst, rel_err(got[b] - base, ref - ref_f))
+        assert rel_err(got[b], ref) < 1e-7
+    d_one = rel_err(got - base[None], one - base[None])
+    print(f"B={B}: drho part vs oracle {worst:.2e}, vs unbatched BF16 {d_one:.2e}")
+    assert worst < 5e-6   # 16-bit drho operand, FP32 sums
+    assert d_one < 5e-6
+
+
+def test_eval_batched_fp64_handle_is_per_trajectory():
+    scatt, oracle = _mods()
+    rng = np.random.default_rng(7)
+    nk, nb, B = 50, 4, 3
+    k1, k2 = random_pairs(nk, 4, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    rho = np.stack([random_rho(nk, nb, rng) for _ in range(B)])
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        got = op.eval_batched(11 * FS, rho)
+        for b in range(B):
+            assert rel_err(got[b], op.eval(11 * FS, rho[b])) < 1e-13  # FP64 atomics: rounding order only
+
+
+def test_rk4_batched_follows_unbatched(monkeypatch):
+    scatt, oracle = _mods()
+    print("case batched rk4", flush=True)
+    rng = np.random.default_rng(9)
+    nk, nb, B = 160, 4, 5
+    k1, k2 = _hub_pairs(nk, 4, rng)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    P1, P2 = 0.2 * P1, 0.2 * P2
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho0 = _states(B, nk, nb, f, rng)
+    s = random_rho(nk, nb, rng, scale=0.5, diag=0.0)
+    dt, n, rep = 2 * FS, 24, 6
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=BF16) as op:
+        op.upload_P(P1, P2)
+        op.set_split(f)
+        op.set_observable(s, f, 0, nb, 1.0)
+        rb, ob = op.integrate_rk4_batched(rho0, 0.0, dt, n, report_every=rep)
+        monkeypatch.setenv("DMD_GRAPH", "0")
+        rb2, ob2 = op.integrate_rk4_batched(rho0, 0.0, dt, n, report_every=rep)
+        monkeypatch.delenv("DMD_GRAPH")
+        singles = [op.integrate_rk4(rho0[b], 0.0, dt, n, report_every=rep) for b in range(B)]
+    r1 = np.stack([x[0] for x in singles])
+    o1 = np.stack([x[1] for x in singles])
+    sig = np.abs(o1 - o1[:, :1]).max()
+    d_state = rel_err(rb - rho0, r1 - rho0)
+    d_obs = np.abs(ob - o1).max() / sig
+    print(f"batched rk4: state change {d_state:.2e}, S(t) {d_obs:.2e} of the signal change {sig:.2e}")
+    assert ob.shape == (B, n // rep + 1)
+    assert d_state < 1e-4 and d_obs < 1e-4
+    # graph replay and eager stepping agree (FP32 atomics: same to rounding)
+    assert rel_err(rb2 - rho0, rb - rho0) < 1e-5
