@@ -57,6 +57,7 @@ class SynthParams(ctypes.Structure):
         ("temperature", ctypes.c_double),
         ("sigma_imp", ctypes.c_double),
         ("imp_fraction", ctypes.c_double),
+        ("branch_mask", ctypes.c_uint32),
     ]
 
     def as_dict(self):
@@ -95,6 +96,9 @@ PROTOTYPES = {
     "dmd_scatt_measure": [_vp, _d, _vp, _pd],
     "dmd_scatt_integrate_rk4": [_vp, _d, _d, _i64, _i, _vp, _vp],
     "dmd_scatt_integrate_rk4_batched": [_vp, _i, _d, _d, _i64, _i, _vp, _vp],
+    "dmd_scatt_vjp": [_vp, _d, _vp, _vp, _vp],
+    "dmd_scatt_combine_P": [_vp, _i, _vp, _vp],
+    "dmd_scatt_adjoint_rk4": [_vp, _d, _d, _i64, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
     "dmd_scatt_info": [_vp, _pi64, _pi64, _pd],
     "dmd_scatt_time_eval": [_vp, _d, _i, _pd, _pd],
     "dmd_scatt_time_eval_batched": [_vp, _d, _i, _pd],

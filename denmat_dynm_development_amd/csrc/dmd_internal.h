@@ -155,6 +155,12 @@ struct dmd_scatt {
     double2 *d_yB0 = nullptr, *d_yBa = nullptr, *d_yBb = nullptr, *d_accB = nullptr, *d_wB = nullptr;
     int64_t rkb_cap = 0;
     int last_B = 0;                      // trajectories of the last dmd_scatt_eval_batched
+    // adjoint scratch
+    int64_t ndiag = 0;                   // stored pairs with k = k' (no reverse direction)
+    int64_t *d_dpos = nullptr;           // their P positions
+    int32_t *d_dk = nullptr;             // and rows
+    double2 *d_adj = nullptr;            // 3 x nk x nb^2
+    double *d_dotpart = nullptr;         // dot-product partials
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
@@ -208,6 +214,10 @@ int tiles4_generate(dmd_scatt *h, const dmd_synth_params &p);
 int tiles4_relayout(dmd_scatt *h, int to_blocked);
 int tiles4_prepare(dmd_scatt *h);  // F_eq of the reduced forms when P or f changed (not capturable)
 int launch_tiles4(dmd_scatt *h, bool dm1, double a, double2 *out);
+// raw row sums [Pdm | dm1P] of the single-read contraction into d_acc_rows with
+// X in place of rho_S and Y in place of 1 - rho_S (Y null: 1 - X); P must be
+// pair-contiguous (the single-read layout)
+int launch_raw_rows(dmd_scatt *h, const double2 *X, const double2 *Y);
 // batched path (dmd_batched.hip)
 int batched_prepare(dmd_scatt *h, int B);  // not capturable
 int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *out);

@@ -783,7 +783,8 @@ dmd_scatt::~dmd_scatt()
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
     dfree(d_pscale); dfree(d_fsplit); dfree(d_feq); dfree(d_x32); dfree(d_hacc16);
     dfree(d_vrec); dfree(d_vq); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
-    dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB);
+    dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB); dfree(d_adj); dfree(d_dotpart);
+    dfree(d_dpos); dfree(d_dk);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -966,6 +967,12 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
             }
         }
     }
+    std::vector<int64_t> dpos;  // pairs with k = k' (the adjoint treats them apart)
+    std::vector<int32_t> dk;
+    for (int64_t i = 0; i < n_pairs; i++) {
+        int64_t f = KK(i);
+        if (k1st[f] == k2nd[f]) { dpos.push_back(fwdpos[i]); dk.push_back((int32_t)k1st[f]); }
+    }
     std::vector<int32_t> gbeg;  // workgroup -> first chunk (groups of chunks of one row block)
     const int group = lane_perm ? DMD_TILE_GROUP : DMD_TILE4_GROUP;
     for (int64_t c = 0; c < (int64_t)cbeg.size(); c++)
@@ -1001,6 +1008,13 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
               (rc = dalloc_t(&h->d_clen, h->nchunks, "chunks")) || (rc = dalloc_t(&h->d_cij, h->nchunks, "chunks")) ||
               (rc = dalloc_t(&h->d_gbeg, h->ngroups + 1, "chunk groups"))))
         return rc;
+    h->ndiag = (int64_t)dpos.size();
+    if (h->ndiag) {
+        if ((rc = dalloc_t(&h->d_dpos, h->ndiag, "diagonal pairs")) || (rc = dalloc_t(&h->d_dk, h->ndiag, "diagonal rows")))
+            return rc;
+        DMD_HIP(hipMemcpy(h->d_dpos, dpos.data(), h->ndiag * 8, hipMemcpyHostToDevice));
+        DMD_HIP(hipMemcpy(h->d_dk, dk.data(), h->ndiag * 4, hipMemcpyHostToDevice));
+    }
     if (precision == DMD_PREC_BF16 && (rc = dalloc_t(&h->d_pscale, nslots, "P scales"))) return rc;
     if (nb == 4 && (rc = dalloc_t(&h->d_hacc16, (size_t)nk * 16, "Hermitian accumulator"))) return rc;
     if (!identity) {
@@ -1272,6 +1286,21 @@ static void launch_single_nb(dmd_scatt *h, bool dm1, double a, double2 *out)
         hipLaunchKernelGGL((k_assemble<NB, false>), ga, b, 0, h->st, h->nk, h->d_acc_rows, h->d_rhoS, h->d_rho1S,
                            h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
     }
+}
+
+int launch_raw_rows(dmd_scatt *h, const double2 *X, const double2 *Y)
+{
+    if (h->nb > 2 || h->soa != 0) return fail(DMD_ERR_STATE, "raw row sums need nb <= 2 and the single-read layout");
+    dim3 g(nblk(h->nk, 4)), b(256);
+    if (h->nb == 1) {
+        if (Y) hipLaunchKernelGGL((k_fwd_atomic<1, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, X, Y, h->d_acc_rows);
+        else hipLaunchKernelGGL((k_fwd_atomic<1, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, X, X, h->d_acc_rows);
+    } else {
+        if (Y) hipLaunchKernelGGL((k_fwd_atomic<2, true>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, X, Y, h->d_acc_rows);
+        else hipLaunchKernelGGL((k_fwd_atomic<2, false>), g, b, 0, h->st, h->nk, h->d_P, h->d_rowptr, h->d_col, h->d_fwdpos, X, X, h->d_acc_rows);
+    }
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
 }
 
 static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)

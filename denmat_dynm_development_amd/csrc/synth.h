@@ -93,7 +93,9 @@ DMD_HD inline double synth_bose(double x) { return 1.0 / (exp(x) - 1.0); }
 // P += App (x) conj(A2pp) + Amm (x) conj(A2mm) (ElectronPhonon.h:566-570).
 // The e-i term (when imp_fraction != 0) is added after the branches with
 // weight imp_fraction, the fold-in of add_scatt_contrib
-// (src_v4.5.7/ElectronPhonon.cpp:87-97).  Needs nb <= DMD_SYNTH_MAX_NB.
+// (src_v4.5.7/ElectronPhonon.cpp:87-97).  A non-zero branch_mask keeps only
+// the branches (bits 0..) and the e-i term (bit 31) it names, so P is the sum
+// of its components.  Needs nb <= DMD_SYNTH_MAX_NB.
 #define DMD_SYNTH_MAX_NB 4
 template <int NB>
 DMD_HD inline void synth_pair_P_nb(const dmd_synth_params &p, const double *Ek, const double *Ekp,
@@ -107,6 +109,7 @@ DMD_HD inline void synth_pair_P_nb(const dmd_synth_params &p, const double *Ek, 
     double Ar[NB * NB][4], Ai[NB * NB][4];
     // Ar/Ai[.][0]=App=A2pp, [1]=Amm=A2mm, [2]=Amp=A2mp, [3]=Apm=A2pm
     for (int a = 0; a < p.n_modes; a++) {
+        if (p.branch_mask && !((p.branch_mask >> a) & 1u)) continue;
         const double w = synth_omega(p, a);
         const double nq = synth_bose(fmax(1e-3, w / p.temperature));
         const double sq1 = sqrt(nq + 1.0), sq0 = sqrt(nq);
@@ -145,7 +148,7 @@ DMD_HD inline void synth_pair_P_nb(const dmd_synth_params &p, const double *Ek, 
                         P2[o] += t2r; P2[o + 1] += t2i;
                     }
     }
-    if (p.imp_fraction != 0.0) {
+    if (p.imp_fraction != 0.0 && (!p.branch_mask || ((p.branch_mask >> 31) & 1u))) {
         // A = g_imp sqrt(delta(E_k - E_k')), P1 = A(x)conj(A), P2 = A^T(x)conj(A^T)
         for (int b1 = 0; b1 < nb; b1++)
             for (int b2 = 0; b2 < nb; b2++) {

@@ -159,7 +159,10 @@ extern "C" int dmd_synth_preset(int which, dmd_synth_params *p)
         p->seed = 4; p->nb = 4; p->nk = 32768; p->partners_min = 10; p->partners_max = 5000;
         p->powerlaw_alpha = 1.15;
         break;
-    case 5: p->seed = 5; p->nb = 2; p->nk = 16384; p->partners = 488; break;
+    case 5:  // six branches plus an e-i term: the seven parameters of the adjoint workload
+        p->seed = 5; p->nb = 2; p->nk = 16384; p->partners = 488;
+        p->sigma_imp = 5e-4; p->imp_fraction = 1e18 * bohr2cm * bohr2cm * bohr2cm * vcell;
+        break;
     default: return fail(DMD_ERR_ARG, "dmd_synth_preset: unknown preset " + std::to_string(which));
     }
     return DMD_OK;

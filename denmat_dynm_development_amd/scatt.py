@@ -224,6 +224,36 @@ class ScatteringOperator:
         call("dmd_scatt_time_eval", self._h, float(t), int(n), ctypes.byref(k), ctypes.byref(e))
         return k.value, e.value
 
+    # -- adjoint ---------------------------------------------------------------
+    def vjp(self, t, rho, v, out=None):
+        """(d ddmdt / d rho)^T v at the Hermitian state rho (FP64, nb <= 2)."""
+        if out is None:
+            out = _empty_like_state(rho, (self.nk, self.nb * self.nb))
+        pr, kr = _ptr(rho, np.complex128)
+        pv, kv = _ptr(v, np.complex128)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_vjp", self._h, float(t), pr, pv, po)
+        return out
+
+    def combine_P(self, comps, weights):
+        """P = sum_c weights[c] P of comps[c] (FP64 handles of the same pairs)."""
+        arr = (ctypes.c_void_p * len(comps))(*[c._h.value for c in comps])
+        w = np.ascontiguousarray(weights, np.float64)
+        call("dmd_scatt_combine_P", self._h, len(comps), ctypes.cast(arr, ctypes.c_void_p), w.ctypes.data)
+
+    def adjoint_rk4(self, rho0, t0, dt, n_steps, report_every, checkpoint_every, dL_dobs, comps=()):
+        """(dL/dtheta over comps, dL/drho0) of a loss with gradient dL_dobs on
+        the observable reports of integrate_rk4(rho0, t0, dt, n_steps, report_every)."""
+        r0 = np.ascontiguousarray(rho0, np.complex128).reshape(self.nk, self.nb * self.nb)
+        dobs = None if dL_dobs is None else np.ascontiguousarray(dL_dobs, np.float64)
+        arr = (ctypes.c_void_p * max(1, len(comps)))(*[c._h.value for c in comps])
+        dth = np.zeros(max(1, len(comps)))
+        drho = np.zeros_like(r0)
+        call("dmd_scatt_adjoint_rk4", self._h, float(t0), float(dt), int(n_steps), int(report_every),
+             int(checkpoint_every), r0.ctypes.data, None if dobs is None else dobs.ctypes.data, len(comps),
+             ctypes.cast(arr, ctypes.c_void_p), dth.ctypes.data, drho.ctypes.data)
+        return dth[:len(comps)], drho
+
     def time_eval_batched(self, t, n):
         """ms per batched evaluation of the states of the last eval_batched."""
         k = ctypes.c_double()

@@ -75,6 +75,8 @@ typedef struct {
     double temperature;   /* k_B T                                             */
     double sigma_imp;     /* e-i coupling std dev per component                */
     double imp_fraction;  /* |n_i V_cell| * scale_ei; 0 disables the e-i term   */
+    uint32_t branch_mask; /* 0: every term; else bit a keeps phonon branch a and
+                             bit 31 the e-i term (per-component P for gradients) */
 } dmd_synth_params;
 
 /* ---- errors --------------------------------------------------------------- */
@@ -171,6 +173,27 @@ int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps,
  * dmd_scatt_integrate_rk4 per trajectory. */
 int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
                                     int report_every, double *rho_inout, double *obs_out);
+
+/* ---- adjoint ------------------------------------------------------------------ */
+/* Vector-Jacobian product of evolve (evolve_eph.cpp:34-166) at the state rho
+ * (Hermitian, nk x nb^2) and time t: g = (d ddmdt / d rho)^T v under the real
+ * pairing Re sum conj(a) b, projected on Hermitian matrices.  FP64, nb <= 2,
+ * scattering term only; the handle switches to the single-read kernel.  No
+ * reference counterpart (the reference has no derivatives). */
+int dmd_scatt_vjp(dmd_scatt *h, double t, const double *rho, const double *v, double *g);
+/* P = sum_c weights[c] P_c over component handles holding FP64 P of the same
+ * pairs (e.g. per-branch and impurity parts; scale_eph generalised,
+ * ElectronPhonon.cpp:225,232 and :87-97).  No reference counterpart. */
+int dmd_scatt_combine_P(dmd_scatt *h, int ncomp, dmd_scatt *const *comps, const double *weights);
+/* Checkpointed discrete adjoint of dmd_scatt_integrate_rk4 (scattering term
+ * alone, FP64, nb <= 2): given dL/dS at the n_steps/report_every + 1 reports
+ * of the observable, returns dL/dtheta_c = sum over stages of Re <Kbar,
+ * f_c(Y)> for the component handles (the derivative with respect to the
+ * weight of P_c) and dL/drho0.  States are recomputed from checkpoints every
+ * checkpoint_every steps.  No reference counterpart. */
+int dmd_scatt_adjoint_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps, int report_every,
+                          int checkpoint_every, const double *rho0, const double *dL_dobs, int ncomp,
+                          dmd_scatt *const *comps, double *dL_dtheta, double *dL_drho0);
 
 /* ---- introspection ---------------------------------------------------------- */
 /* Bytes of P streamed per eval and stored pairs; the last kernel time in ms
