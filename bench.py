@@ -1,16 +1,22 @@
 """Benchmark of the Lindblad scattering hot path on one MI355X.
 
-A step is one fixed-step RK4 step (4 evaluations of dRho/dt|scatt over every
-stored k-pair, plus the stage combinations) of the on-device integrator, at
-the one-GPU workload: nb = 2, nk = 65536, ~900 partners drawn per k
-(~59 M stored pairs, 30.7 GB of complex128 P resident in HBM), T = 300 K,
-dt = 1 fs, FP64 throughout, synthetic ldbd_data generated from seed 2 on the
-device.  value = stored k-pair contractions per second (pairs x 4 x steps /
-time), whole job.  Also reported: HBM GB/s of the contraction kernel against
-the MI355X peak (roofline), trajectory-steps/s, and the CPU oracle timed on
-this host's cores on a bounded sample of the same workload.
+Default mode (the contract line): a step is one fixed-step RK4 step (4
+evaluations of dRho/dt|scatt over every stored k-pair, plus the stage
+combinations) of the on-device integrator at the one-GPU FP64 workload:
+nb = 2, nk = 65536, ~900 partners drawn per k (58.2 M stored pairs, 29.8 GB
+of complex128 P resident in HBM), T = 300 K, dt = 1 fs, synthetic ldbd_data
+generated from seed 2 on the device.  value = stored k-pair contractions per
+second (pairs x 4 x steps / time), whole job.  Also reported: the
+contraction kernel's HBM GB/s against the MI355X peak (roofline),
+trajectory-steps/s, the CPU oracle timed on this host's cores on a bounded
+sample of the same workload, and (mixed_batched) trajectory-steps/s of the
+batched reduced-precision forms against the FP64 kernel at the nb = 4
+power-law workload (nk = 32768, 10 .. 5000 partners per k), with the fitted
+spin lifetime of the same runs compared.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--preset 2] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
+  python bench.py --mode accuracy   # FP64 / complex64 / BF16 S_z(t) and tau_s, nb = 4 with e-i
+  python bench.py --mode adjoint    # forward + adjoint RK4 with 7 parameters, nb = 2, nk = 16384
 """
 import argparse
 import json
@@ -31,30 +37,44 @@ HBM_PEAK_GBPS = 8000.0
 MEASURED_TRAFFIC = {
     (2, "tiles"): (31.697e9, "profiles/k_tiles_soa_nb2_nk65536_58Mpairs.txt"),
 }
+PREC_NAME = {0: "fp64", 1: "c64", 2: "bf16"}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def initial_state(p, e, s3):
-    """rho0 = f + (df/de) b S_z (first order, electron.cpp:62-98) with b set so
-    that Tr(S_z drho) / sum Tr f = 1e-3; mu = CBM - 0.1 eV."""
+def initial_state(p, e, s3, axis=2, amp=1e-3):
+    """rho0 = f + (df/de) b S_axis (first order, electron.cpp:62-98) with b set
+    so that Tr(S_axis drho) / sum Tr f = amp; mu = CBM - 0.1 eV."""
     nk, nb = e.shape
     T = p.temperature
     mu = -0.1 / 27.21138505
     f = 1.0 / (np.exp((e - mu) / T) + 1.0)
-    sz = s3[:, 2].reshape(nk, nb, nb)
+    sa = s3[:, axis].reshape(nk, nb, nb)
     de = e[:, :, None] - e[:, None, :]
     fd = f[:, :, None] - f[:, None, :]
     deg = np.abs(de) < 1e-8
     favg = 0.5 * (f[:, :, None] + f[:, None, :])
     dfde = np.where(deg, favg * (favg - 1) / T, fd / np.where(deg, 1.0, de))
-    d1 = dfde * sz  # drho for b = 1
-    s_per_b = np.real(np.einsum("kij,kji->", sz, d1))
-    b = 1e-3 * f.sum() / s_per_b
+    d1 = dfde * sa  # drho for b = 1
+    s_per_b = np.real(np.einsum("kij,kji->", sa, d1))
+    b = amp * f.sum() / s_per_b
     rho = (np.einsum("ki,ij->kij", f, np.eye(nb)) + b * d1).reshape(nk, nb * nb)
     return rho.astype(np.complex128), f
+
+
+def batch_states(p, e, s3, B):
+    """B initial states: spin along z, x, y in turn, imbalance amplitudes
+    log-uniform in [1e-4, 1e-2] (trajectory 0: z, 1e-3)."""
+    states = []
+    f = None
+    for b in range(B):
+        axis = (2, 0, 1)[b % 3]
+        amp = 1e-3 if b == 0 else 10.0 ** (-4 + 2 * ((b * 0.6180339887) % 1.0))
+        rho, f = initial_state(p, e, s3, axis, amp)
+        states.append(rho)
+    return np.stack(states), f
 
 
 def cpu_baseline(p, e, k1, k2, rho, budget_s=12.0):
@@ -81,27 +101,13 @@ def cpu_baseline(p, e, k1, k2, rho, budget_s=12.0):
                       f"(oracle evolve, per-element phases, OpenMP over pairs), {el:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--preset", type=int, default=2)
-    ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="also run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the scattering operator runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
-        sys.exit(2)
+def alg_bytes(n_pairs, nk, nb, esize=16, extra=0):
+    """algorithmic bytes of one evaluation: P once + 4-byte index per stored
+    pair (+ extra per pair), rho in and ddmdt out"""
+    return n_pairs * (2 * nb ** 4 * esize + 4 + extra) + 2 * nk * nb ** 2 * 16
 
-    import torch
-    from denmat_dynm_development_amd import scatt
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    torch.cuda.set_device(0)
+def bench_plain(args, torch, scatt):
     p = scatt.synth_params(args.preset)
     t0 = time.perf_counter()
     e = scatt.synth_energies(p)
@@ -113,15 +119,13 @@ def main():
     op.set_flags(expt=True, kernel=args.kernel)
     t1 = time.perf_counter()
     op.generate_P(p)
-    log(f"P generated on device in {time.perf_counter() - t1:.1f}s; create+generate {time.perf_counter() - t0:.1f}s")
+    log(f"P generated on device in {time.perf_counter() - t1:.1f}s")
     rho_d = torch.from_numpy(rho0).cuda()
     dt = 1.0 * FS
     n_pairs = len(k1)
-    # warmup (also exercises graph capture)
     if args.warmup > 0:
-        op.integrate_rk4(rho_d, 0.0, dt, args.warmup)
+        op.integrate_rk4(rho_d, 0.0, dt, args.warmup)  # also exercises graph capture
     torch.cuda.synchronize()
-    log("warmup done")
     ts = time.perf_counter()
     rho_out, _ = op.integrate_rk4(rho_d, 0.0, dt, args.steps)
     torch.cuda.synchronize()
@@ -133,14 +137,21 @@ def main():
     # dominant kernel: the contraction, timed with HIP events on the handle's stream
     op.eval(0.0, rho_d)
     kernel_ms, eval_ms = op.time_eval(17 * FS, 10)
-    alg_bytes = n_pairs * (2 * p.nb ** 4 * 16 + 4) + 2 * p.nk * p.nb ** 2 * 16
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    ab = alg_bytes(n_pairs, p.nk, p.nb)
+    achieved = ab / (kernel_ms * 1e-3) / 1e9
+    # the same evaluation with host buffers (PCIe in and out each call)
+    th = time.perf_counter()
+    for _ in range(5):
+        op.eval(0.0, rho0)
+    pcie_ms = (time.perf_counter() - th) / 5 * 1e3
     log(f"contraction kernel {kernel_ms:.3f} ms/launch ({eval_ms:.3f} ms with the rotation), "
-        f"{achieved:.0f} GB/s algorithmic")
-    if args.kernel == 1 or p.nb > 2:
+        f"{achieved:.0f} GB/s algorithmic; host-buffer eval {pcie_ms:.3f} ms")
+    if args.kernel == 1 or p.nb == 3:
         kind, kname = "rows", f"k_rows<{p.nb}>"
     elif args.kernel == 2 or p.nb == 1:
         kind, kname = "single_read", f"k_fwd_atomic<{p.nb}> (+k_assemble)"
+    elif p.nb == 4:
+        kind, kname = "tiles", "k_tiles4<fp64> (+k_assemble_h4)"
     else:
         kind, kname = "tiles", "k_tiles_soa<false> (+k_assemble_h)"
     traffic = MEASURED_TRAFFIC.get((args.preset, kind))
@@ -163,12 +174,13 @@ def main():
                    "kernel": kind},
         "hbm_GBps": achieved,
         "traj_steps_per_s": args.steps / el,
+        "eval_ms_host_buffers": pcie_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic[0] if traffic else None,
                      "traffic_source": traffic[1] if traffic else None,
                      "kernel": kname,
-                     "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
+                     "kernel_ms": kernel_ms, "alg_bytes_per_launch": ab},
     }
     if args.copy_calib:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
@@ -177,11 +189,206 @@ def main():
         torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
+        del src, dst
+    op.close()
+    return out, (p, e, k1, k2, rho0)
+
+
+def run_traj(op, torch, rho, dt, steps, batched):
+    """timed RK4 steps with the observable every step; (seconds, S series)"""
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    if batched:
+        _, obs = op.integrate_rk4_batched(rho, 0.0, dt, steps, report_every=1)
+    else:
+        _, obs = op.integrate_rk4(rho, 0.0, dt, steps, report_every=1)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, obs
+
+
+def bench_mixed_batched(args, torch, scatt):
+    """FP64 tile kernel (one trajectory) against the batched reduced-precision
+    forms (B trajectories per pass over P) at the nb = 4 power-law workload."""
+    from denmat_dynm_development_amd import taufit
+    p = scatt.synth_params(args.batched_preset)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    Bs = [int(x) for x in args.B.split(",")]
+    states, f = batch_states(p, e, s3, max(Bs))
+    n_pairs = len(k1)
+    dt, K = 1.0 * FS, args.batched_steps
+    times = np.arange(K + 1) * dt
+    res = {"workload": f"preset {args.batched_preset}: nb={p.nb} nk={p.nk} stored_pairs={n_pairs} "
+                       f"(partners per k power law {p.partners_min}..{p.partners_max}), RK4 dt=1fs, {K} steps",
+           "stored_pairs": n_pairs}
+    log(f"batched workload: nb={p.nb} nk={p.nk} pairs={n_pairs}")
+    # FP64, one trajectory
+    op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+    op.generate_P(p)
+    op.set_observable(s3[:, 2], f, 0, p.nb, 1.0 / p.nk)
+    run_traj(op, torch, states[0], dt, 1, False)
+    el, obs = run_traj(op, torch, states[0], dt, K, False)
+    tau64 = taufit.fit_tau(times, obs)
+    op.eval(0.0, states[0])
+    kms, _ = op.time_eval(0.0, 3)
+    ab = alg_bytes(n_pairs, p.nk, p.nb)
+    res["fp64"] = {"traj_steps_per_s": K / el, "ms_per_step": el / K * 1e3, "tau_s_ps": tau64,
+                   "kernel": "k_tiles4<fp64>", "kernel_ms": kms, "hbm_GBps": ab / kms / 1e6,
+                   "roofline_frac": ab / kms / 1e6 / HBM_PEAK_GBPS}
+    log(f"fp64: {el / K * 1e3:.1f} ms/step, kernel {kms:.2f} ms ({ab / kms / 1e6:.0f} GB/s), tau {tau64:.6g} ps")
+    op.close()
+    base = K / el
+    for prec in (1, 2):
+        name = PREC_NAME[prec]
+        try:
+            op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk, precision=prec)
+            op.generate_P(p)
+            op.set_split(f)
+            op.set_observable(s3[:, 2], f, 0, p.nb, 1.0 / p.nk)
+            per = {}
+            for B in Bs:
+                run_traj(op, torch, states[:B], dt, 1, True)  # builds visit records, tunes, captures
+                el, obs = run_traj(op, torch, states[:B], dt, K, True)
+                tau = taufit.fit_tau(times, obs[0])
+                op.eval_batched(0.0, states[:B])
+                kms = op.time_eval_batched(0.0, 3)
+                per[str(B)] = {"traj_steps_per_s": B * K / el, "ms_per_step": el / K * 1e3,
+                               "speedup_vs_fp64": B * K / el / base, "eval_ms": kms,
+                               "pair_traj_per_s": n_pairs * B / (kms * 1e-3),
+                               "tau_s_ps": tau, "tau_rel_err_vs_fp64": abs(tau / tau64 - 1)}
+                log(f"{name} B={B}: {el / K * 1e3:.1f} ms/step, {B * K / el:.1f} traj-steps/s "
+                    f"({B * K / el / base:.1f}x fp64), eval {kms:.2f} ms, |dtau|/tau {abs(tau / tau64 - 1):.2e}")
+            res[name] = per
+            op.close()
+        except Exception as ex:  # e.g. not enough HBM for both stores of a form
+            res[name] = {"error": str(ex)}
+            log(f"{name}: {ex}")
+    return res
+
+
+def mode_accuracy(args, torch, scatt):
+    """S_z(t) and tau_s of FP64, complex64 and BF16 (unbatched and batched) on
+    the nb = 4 workload with the e-i term."""
+    from denmat_dynm_development_amd import taufit
+    p = scatt.synth_params(3)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    states, f = batch_states(p, e, s3, 3)
+    dt, K = 1.0 * FS, args.steps
+    times = np.arange(K + 1) * dt
+    out = {"mode": "accuracy", "workload": f"preset 3: nb=4 nk={p.nk} stored_pairs={len(k1)} with e-i, "
+                                           f"RK4 dt=1fs {K} steps", "results": {}}
+    ref = None
+    for prec in (0, 1, 2):
+        op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk, precision=prec)
+        op.generate_P(p)
+        op.set_split(f)
+        op.set_observable(s3[:, 2], f, 0, p.nb, 1.0 / p.nk)
+        el, obs = run_traj(op, torch, states[0], dt, K, False)
+        r = {"tau_s_ps": taufit.fit_tau(times, obs), "ms_per_step": el / K * 1e3}
+        if prec == 0:
+            ref = obs, r["tau_s_ps"]
+        else:
+            r["tau_rel_err"] = abs(r["tau_s_ps"] / ref[1] - 1)
+            r["S_max_abs_err_rel"] = float(np.abs(obs - ref[0]).max() / np.abs(ref[0]).max())
+            elb, ob = run_traj(op, torch, states, dt, K, True)
+            r["batched_B3_tau_rel_err"] = abs(taufit.fit_tau(times, ob[0]) / ref[1] - 1)
+            r["batched_B3_S_max_abs_err_rel"] = float(np.abs(ob[0] - ref[0]).max() / np.abs(ref[0]).max())
+        out["results"][PREC_NAME[prec]] = r
+        log(f"{PREC_NAME[prec]}: {r}")
+        op.close()
+    return out
+
+
+def mode_adjoint(args, torch, scatt):
+    """Forward RK4 and the checkpointed adjoint with the six branch scales and
+    the impurity density as parameters, nb = 2, nk = 16384; the gradient of
+    tau_s checked against central differences (h = 1e-6) of the forward."""
+    from denmat_dynm_development_amd.adjoint import SpinLifetimeProblem
+    p = scatt.synth_params(5)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    rho0, f = initial_state(p, e, s3)
+    masks = [1 << a for a in range(p.n_modes)] + [1 << 31]
+    params = []
+    for m in masks:
+        q = scatt.synth_params(5)
+        q.branch_mask = m
+        params.append(q)
+    K = args.steps
+    pr = SpinLifetimeProblem.from_device_generator(
+        params, p.nk, p.nb, k1, k2, e, rho0=rho0, s_obs=s3[:, 2], f=f, i0=0, i1=p.nb, obs_prefac=1.0 / p.nk,
+        t0=0.0, dt=1.0 * FS, n_steps=K, report_every=1, checkpoint_every=64, nk_full=p.nk)
+    theta = np.ones(len(masks))
+    try:
+        pr.tau(theta)  # warm (graph capture)
+        t = time.perf_counter()
+        tau = pr.tau(theta)
+        t_fwd = time.perf_counter() - t
+        t = time.perf_counter()
+        tau2, grad = pr.tau_and_grad(theta)
+        t_adj = time.perf_counter() - t - t_fwd
+        fd = []
+        for c in range(len(theta) if args.fd_all else 2):
+            tp, tm = theta.copy(), theta.copy()
+            tp[c] += 1e-6
+            tm[c] -= 1e-6
+            fd.append((pr.tau(tp) - pr.tau(tm)) / 2e-6)
+        rel = [abs(grad[c] / fd[c] - 1) for c in range(len(fd))]
+    finally:
+        pr.close()
+    out = {"mode": "adjoint", "workload": f"preset 5: nb=2 nk={p.nk} stored_pairs={len(k1)}, 6 branches + e-i, "
+                                          f"RK4 dt=1fs {K} steps, checkpoints every 64",
+           "tau_s_ps": tau, "dtau_dtheta": list(map(float, grad)), "fd_central_h1e-6": fd,
+           "adjoint_rel_err_vs_fd": rel, "forward_s": t_fwd, "adjoint_s": t_adj,
+           "adjoint_ms_per_step": t_adj / K * 1e3}
+    log(json.dumps(out))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--preset", type=int, default=2)
+    ap.add_argument("--kernel", type=int, default=0)
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-preset", type=int, default=4)
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--B", default="3,16,64")
+    ap.add_argument("--fd-all", action="store_true", help="adjoint mode: check every parameter")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the scattering operator runs on a single MI355X here "
+              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from denmat_dynm_development_amd import scatt
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    if args.mode == "accuracy":
+        print(json.dumps(mode_accuracy(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "adjoint":
+        print(json.dumps(mode_adjoint(args, torch, scatt)), flush=True)
+        return
+    out, (p, e, k1, k2, rho0) = bench_plain(args, torch, scatt)
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(p, e, k1, k2, rho0)
         log(f"cpu baseline {out['cpu_baseline']['value']:.3e} pairs/s on {out['cpu_baseline']['cores']} threads")
-    op.close()
+    if not args.no_batched:
+        out["mixed_batched"] = bench_mixed_batched(args, torch, scatt)
     print(json.dumps(out), flush=True)
 
 

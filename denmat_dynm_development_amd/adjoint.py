@@ -38,6 +38,17 @@ class SpinLifetimeProblem:
         self.op.set_observable(s_obs, f, i0, i1, obs_prefac)
 
     @classmethod
+    def from_device_generator(cls, params_by_component, nk, nb, k1, k2, e, **kw):
+        """Components written by the on-device generator (no host copy of P)."""
+        self = cls(nk, nb, k1, k2, e, [], **kw)
+        for q in params_by_component:
+            c = scatt.ScatteringOperator(nk, nb, k1, k2, e, nk_full=kw.get("nk_full"))
+            c.generate_P(q)
+            c.set_flags(expt=kw.get("expt", True), kernel=_lib.DMD_KERNEL_SINGLE_READ)
+            self.comps.append(c)
+        return self
+
+    @classmethod
     def from_components_generator(cls, params_by_component, nk, nb, k1, k2, e, **kw):
         """Components from the synthetic generator, one dmd_synth_params each
         (e.g. branch_mask = 1 << a for branch a, 1 << 31 for the e-i term)."""

@@ -12,25 +12,30 @@
 //
 // Matrix-core mapping (v_mfma_f32_32x32x16_bf16, FP32 accumulation): a
 // complex 16 x 16 matrix times 16 x 32 complex columns is the real 32 x 32
-// product [[Ar, -Ai], [Ai, Ar]] [Xr; Xi].  A is exact in BF16 (the stored
-// P); X is carried as two BF16 terms (hi + lo, a 16-bit significand), so
+// product [[Ar, -Ai], [Ai, Ar]] [Xr; Xi].  X is carried as two BF16 terms
+// (hi + lo, a 16-bit significand).  With a BF16 store A is exact in BF16 and
 // each visit issues 2 K-halves x 2 terms x 2 products = 8 MFMAs per 32
-// trajectories.
+// trajectories; with a complex64 store A is split the same way (hi + lo) and
+// the three leading cross terms make 12.
 //
 // Layouts in HBM.
 //  * Visit records, CSR by destination row: 2 x 256 words (BF16 re | im << 16)
 //    per visit, A1 row-major then A2^T row-major, conj and the per-pair
-//    power-of-two scale applied (2 KiB per visit, 4 KiB per stored pair).
+//    power-of-two scale applied (2 KiB per visit, 4 KiB per stored pair);
+//    a complex64 store adds the lo terms (4 KiB per visit).
 //  * X: per partner row q, [term][re/im][k-half][trajectory][8] BF16, so a
 //    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
 //    bytes.
 //  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
-// Uneven rows: every row's visits are cut into segments of at most
-// DMD_SEG visits, one wavefront each, so a 5000-partner row is as many
-// independent work items; segments of one row meet in FP32 atomics.
+// Uneven rows: every row's visits are cut into segments of at most seg_len
+// visits, one wavefront each, so a 5000-partner row is as many independent
+// work items; segments of one row meet in FP32 atomics.  seg_len and the
+// trajectory blocks per wavefront are picked by timing at the first batched
+// call for a padded B (autotune).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -70,6 +75,32 @@ __global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const int64_t
         const uint32_t w = rec[blk_off(src_mat, row, col, 4)];
         const float re = bflo(w) * sc, im = bfhi(w) * sc;
         out[t] = f2bf(re) | (f2bf(rev ? -im : im) << 16);
+    }
+}
+
+// Complex64 store: every element split into BF16 hi + lo (a 16-bit
+// significand); records [A1 hi | A2^T hi | A1 lo | A2^T lo] (4 KiB per visit).
+__global__ __launch_bounds__(64) void k_build_visits_split(int64_t nvis, const int64_t *__restrict__ vslot,
+                                                           const uint8_t *__restrict__ vdir,
+                                                           const float2 *__restrict__ Pc,
+                                                           uint32_t *__restrict__ vrec)
+{
+    const int64_t v = blockIdx.x;
+    if (v >= nvis) return;
+    const int64_t s = vslot[v];
+    const bool rev = vdir[v] != 0;
+    const float2 *rec = Pc + s * 512;
+    uint32_t *out = vrec + v * 1024;
+    for (int t = threadIdx.x; t < 512; t += 64) {
+        const int mat = t >> 8, r = (t >> 4) & 15, c = t & 15;
+        const int src_mat = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
+        const int row = mat == 0 ? r : c, col = mat == 0 ? c : r;
+        const float2 w = rec[blk_off(src_mat, row, col, 2)];
+        const float re = w.x, im = rev ? -w.y : w.y;
+        const uint32_t rh = f2bf(re), ih = f2bf(im);
+        const uint32_t rl = f2bf(re - __uint_as_float(rh << 16)), il = f2bf(im - __uint_as_float(ih << 16));
+        out[t] = rh | (ih << 16);
+        out[512 + t] = rl | (il << 16);
     }
 }
 
@@ -114,13 +145,33 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 u)
     return r;
 }
 
+// The lane's A operands (K chunk 0 multiplies Xr, chunk 1 Xi; re rows [Ar | -Ai],
+// im rows [Ai | Ar]) from 8 words (BF16 re | im << 16) of one row of A.
+__device__ __forceinline__ void a_operands(const uint4 a, const uint4 b, bool im_row, bf16x8 &c0, bf16x8 &c1)
+{
+    const uint32_t w[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+    uint32_t o0[4], o1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const uint32_t x0 = w[2 * t], x1 = w[2 * t + 1];
+        const uint32_t re = (x0 & 0xFFFFu) | (x1 << 16), im = (x0 >> 16) | (x1 & 0xFFFF0000u);
+        o0[t] = im_row ? im : re;
+        o1[t] = im_row ? re : (im ^ 0x80008000u);
+    }
+    c0 = as_bf8(make_uint4(o0[0], o0[1], o0[2], o0[3]));
+    c1 = as_bf8(make_uint4(o1[0], o1[1], o1[2], o1[3]));
+}
+
 // One wavefront per segment; NBLK blocks of 32 trajectories per pass (the
-// grid's y index selects the pass).
-template <int NBLK>
+// grid's y index selects the pass).  PSPLIT: A = A_hi + A_lo (complex64
+// store); the products A_hi X_hi + A_hi X_lo + A_lo X_hi (the A_lo X_lo term
+// is below the 16-bit significands).
+template <int NBLK, bool PSPLIT>
 __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
                                                  const uint32_t *__restrict__ vrec, const int32_t *__restrict__ vq,
                                                  const uint16_t *__restrict__ xb, float *__restrict__ F)
 {
+    constexpr int RW = PSPLIT ? 1024 : 512;  // words per visit record
     const int lane = threadIdx.x & 63;
     const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= nseg) return;
@@ -138,31 +189,17 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
         for (int t = 0; t < 16; t++) { f1[nb][t] = 0.0f; g2[nb][t] = 0.0f; }
     for (int u = 0; u < nv; u++) {
         const int64_t v = v0 + u;
-        const uint32_t *rec = vrec + v * 512;
+        const uint32_t *rec = vrec + v * RW;
         // the lane's 8 elements of row r of A1 and of A2^T (K range 8h .. 8h+7)
-        const uint4 a1a = *(const uint4 *)(rec + r * 16 + 8 * h), a1b = *(const uint4 *)(rec + r * 16 + 8 * h + 4);
-        const uint4 a2a = *(const uint4 *)(rec + 256 + r * 16 + 8 * h),
-                    a2b = *(const uint4 *)(rec + 256 + r * 16 + 8 * h + 4);
-        const int64_t q = vq[v];
-        // split words into re / im BF16 vectors; K chunk 0 multiplies Xr, chunk 1 Xi:
-        // re rows: [Ar | -Ai], im rows: [Ai | Ar]
-        const uint32_t w1[8] = { a1a.x, a1a.y, a1a.z, a1a.w, a1b.x, a1b.y, a1b.z, a1b.w };
-        const uint32_t w2[8] = { a2a.x, a2a.y, a2a.z, a2a.w, a2b.x, a2b.y, a2b.z, a2b.w };
-        uint32_t c0a[4], c1a[4], c0b[4], c1b[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const uint32_t x0 = w1[2 * t], x1 = w1[2 * t + 1], y0 = w2[2 * t], y1 = w2[2 * t + 1];
-            const uint32_t re1 = (x0 & 0xFFFFu) | (x1 << 16), im1 = (x0 >> 16) | (x1 & 0xFFFF0000u);
-            const uint32_t re2 = (y0 & 0xFFFFu) | (y1 << 16), im2 = (y0 >> 16) | (y1 & 0xFFFF0000u);
-            c0a[t] = im_row ? im1 : re1;
-            c1a[t] = im_row ? re1 : (im1 ^ 0x80008000u);
-            c0b[t] = im_row ? im2 : re2;
-            c1b[t] = im_row ? re2 : (im2 ^ 0x80008000u);
+        const int o1 = r * 16 + 8 * h, o2 = 256 + r * 16 + 8 * h;
+        bf16x8 A1c0, A1c1, A2c0, A2c1, L1c0, L1c1, L2c0, L2c1;
+        a_operands(*(const uint4 *)(rec + o1), *(const uint4 *)(rec + o1 + 4), im_row, A1c0, A1c1);
+        a_operands(*(const uint4 *)(rec + o2), *(const uint4 *)(rec + o2 + 4), im_row, A2c0, A2c1);
+        if (PSPLIT) {
+            a_operands(*(const uint4 *)(rec + 512 + o1), *(const uint4 *)(rec + 512 + o1 + 4), im_row, L1c0, L1c1);
+            a_operands(*(const uint4 *)(rec + 512 + o2), *(const uint4 *)(rec + 512 + o2 + 4), im_row, L2c0, L2c1);
         }
-        const bf16x8 A1c0 = as_bf8(make_uint4(c0a[0], c0a[1], c0a[2], c0a[3]));
-        const bf16x8 A1c1 = as_bf8(make_uint4(c1a[0], c1a[1], c1a[2], c1a[3]));
-        const bf16x8 A2c0 = as_bf8(make_uint4(c0b[0], c0b[1], c0b[2], c0b[3]));
-        const bf16x8 A2c1 = as_bf8(make_uint4(c1b[0], c1b[1], c1b[2], c1b[3]));
+        const int64_t q = vq[v];
 #pragma unroll
         for (int nb = 0; nb < NBLK; nb++) {
             const int b = b0 + nb * 32 + i;
@@ -180,6 +217,12 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
             g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xh1, g2[nb], 0, 0, 0);
             g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c0, xl0, g2[nb], 0, 0, 0);
             g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xl1, g2[nb], 0, 0, 0);
+            if (PSPLIT) {
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L1c0, xh0, f1[nb], 0, 0, 0);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L1c1, xh1, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L2c0, xh0, g2[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L2c1, xh1, g2[nb], 0, 0, 0);
+            }
         }
     }
     // D layout: element t of lane l is row 8 (t / 4) + 4 h + t % 4, column (trajectory) l % 32
@@ -266,7 +309,6 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-#define DMD_SEG 128
 
 // visit records, segments (built once per P; rebuilt when P changes)
 static int build_visits(dmd_scatt *h)
@@ -292,23 +334,19 @@ static int build_visits(dmd_scatt *h)
     std::vector<int32_t> vq(nvis);
     std::vector<int4> seg;
     int64_t v = 0;
+    h->h_vptr.assign(nk + 1, 0);
     for (int64_t k = 0; k < nk; k++) {
-        const int64_t start = v;
+        h->h_vptr[k] = v;
         for (int64_t p = rowptr[k]; p < rowptr[k + 1]; p++, v++) { vslot[v] = fwdpos[p]; vdir[v] = 0; vq[v] = col[p]; }
         for (int64_t p = revptr[k]; p < revptr[k + 1]; p++, v++) { vslot[v] = revpos[p]; vdir[v] = 1; vq[v] = revcol[p]; }
-        for (int64_t s = start; s < v; s += DMD_SEG) {
-            const int n = (int)std::min<int64_t>(DMD_SEG, v - s);
-            seg.push_back(make_int4((int)k, (int)(uint32_t)(s & 0xFFFFFFFF), (int)(s >> 32), n));
-        }
     }
-    // longest segments first (only row tails are shorter)
-    std::stable_sort(seg.begin(), seg.end(), [](const int4 &a, const int4 &b) { return a.w > b.w; });
+    h->h_vptr[nk] = v;
     int rc;
-    dfree(h->d_vrec); dfree(h->d_vq); dfree(h->d_seg);
-    h->d_vrec = nullptr; h->d_vq = nullptr; h->d_seg = nullptr;
-    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * 512, "visit records"))) return rc;
+    dfree(h->d_vrec); dfree(h->d_vq);
+    h->d_vrec = nullptr; h->d_vq = nullptr;
+    const bool split = h->precision == DMD_PREC_C64;
+    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * (split ? 1024 : 512), "visit records"))) return rc;
     if ((rc = dalloc_t(&h->d_vq, (size_t)nvis, "visit partners"))) return rc;
-    if ((rc = dalloc_t(&h->d_seg, seg.size(), "segments"))) return rc;
     int64_t *d_vslot = nullptr;
     uint8_t *d_vdir = nullptr;
     if ((rc = dalloc_t(&d_vslot, (size_t)nvis, "visit slots"))) return rc;
@@ -319,18 +357,80 @@ static int build_visits(dmd_scatt *h)
         if (e == hipSuccess) e = hipMemcpy(d_vdir, vdir.data(), nvis, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess && !seg.empty()) e = hipMemcpy(h->d_seg, seg.data(), seg.size() * sizeof(int4), hipMemcpyHostToDevice);
+
     if (e == hipSuccess && nvis) {
-        hipLaunchKernelGGL(k_build_visits, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
-                           (const uint32_t *)h->d_P, h->d_pscale, h->d_vrec);
+        if (split)
+            hipLaunchKernelGGL(k_build_visits_split, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
+                               (const float2 *)h->d_P, h->d_vrec);
+        else
+            hipLaunchKernelGGL(k_build_visits, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
+                               (const uint32_t *)h->d_P, h->d_pscale, h->d_vrec);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     }
     dfree(d_vslot); dfree(d_vdir);
     if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("building visit records: ") + hipGetErrorString(e));
-    h->nseg = (int64_t)seg.size();
     h->bvis_ready = true;
     h->bvis_dirty = false;
+    h->seg_len = 0;
+    h->tuned_Bp = 0;
+    return DMD_OK;
+}
+
+// segments of at most seg_len visits per row, longest first
+static int build_segments(dmd_scatt *h, int seg_len)
+{
+    if (h->seg_len == seg_len && h->d_seg) return DMD_OK;
+    std::vector<int4> seg;
+    for (int64_t k = 0; k < h->nk; k++)
+        for (int64_t s = h->h_vptr[k]; s < h->h_vptr[k + 1]; s += seg_len) {
+            const int n = (int)std::min<int64_t>(seg_len, h->h_vptr[k + 1] - s);
+            seg.push_back(make_int4((int)k, (int)(uint32_t)(s & 0xFFFFFFFF), (int)(s >> 32), n));
+        }
+    std::stable_sort(seg.begin(), seg.end(), [](const int4 &a, const int4 &b) { return a.w > b.w; });
+    dfree(h->d_seg);
+    h->d_seg = nullptr;
+    int rc;
+    if ((rc = dalloc_t(&h->d_seg, seg.size(), "segments"))) return rc;
+    if (!seg.empty()) DMD_HIP(hipMemcpy(h->d_seg, seg.data(), seg.size() * sizeof(int4), hipMemcpyHostToDevice));
+    h->nseg = (int64_t)seg.size();
+    h->seg_len = seg_len;
+    return DMD_OK;
+}
+
+static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req);
+
+// Autotune: segment length x trajectory blocks per wavefront, by timing the
+// matrix-core pass on the current operands; the row sums are cleared after.
+static int autotune(dmd_scatt *h, int Bp)
+{
+    const char *env = std::getenv("DMD_BATCH_TUNE");
+    const bool tune = !(env && env[0] == '0');
+    int best_seg = 128, best_nb = (Bp % 64 == 0) ? 2 : 1;
+    if (tune && h->np > 0) {
+        float best = 1e30f;
+        for (int sl : { 64, 128, 256 }) {
+            int rc = build_segments(h, sl);
+            if (rc) return rc;
+            for (int nb : { 1, 2 }) {
+                if (nb == 2 && Bp % 64) continue;
+                launch_batched_core(h, Bp, nb);  // warm
+                DMD_HIP(hipEventRecord(h->ev0, h->st));
+                for (int r = 0; r < 2; r++) launch_batched_core(h, Bp, nb);
+                DMD_HIP(hipEventRecord(h->ev1, h->st));
+                DMD_HIP(hipEventSynchronize(h->ev1));
+                float ms = 0;
+                DMD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                if (ms < best) { best = ms; best_seg = sl; best_nb = nb; }
+            }
+        }
+    }
+    int rc = build_segments(h, best_seg);
+    if (rc) return rc;
+    h->batch_nblk = best_nb;
+    h->tuned_Bp = Bp;
+    DMD_HIP(hipMemsetAsync(h->d_Fb, 0, (size_t)h->nk * 64 * h->b_cap * sizeof(float), h->st));
+    DMD_HIP(hipStreamSynchronize(h->st));
     return DMD_OK;
 }
 
@@ -351,7 +451,22 @@ int batched_prepare(dmd_scatt *h, int B)
         DMD_HIP(hipMemset(h->d_xb, 0, (size_t)h->nk * 64 * Bp * sizeof(uint16_t)));
         h->b_cap = Bp;
     }
+    if (h->tuned_Bp != Bp && (rc = autotune(h, Bp))) return rc;
     return DMD_OK;
+}
+
+// the matrix-core pass alone (X prepared, F accumulated)
+static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
+{
+    const int nb2 = (Bp / 32) % 2 == 0 && nblk_req == 2 ? 2 : 1;
+    dim3 g(nblk(h->nseg, 4), Bp / (32 * nb2));
+    const bool split = h->precision == DMD_PREC_C64;
+#define DMD_KB(N, S)                                                                                             \
+    hipLaunchKernelGGL((k_batched<N, S>), g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq, \
+                       (const uint16_t *)h->d_xb, h->d_Fb)
+    if (nb2 == 2) { if (split) DMD_KB(2, true); else DMD_KB(2, false); }
+    else { if (split) DMD_KB(1, true); else DMD_KB(1, false); }
+#undef DMD_KB
 }
 
 // ddm of B trajectories (rho: B x nk x 16 device, interaction picture at the
@@ -364,16 +479,7 @@ int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *o
     const int64_t nx = (int64_t)Bp * h->nk * 16;
     hipLaunchKernelGGL(k_prepare_x, dim3(nblk(nx, 256)), dim3(256), 0, h->st, h->nk, B, Bp, rho, h->d_e, h->d_clk, a,
                        h->expt, h->d_fsplit, h->d_rhoSB, (uint16_t *)h->d_xb);
-    if (h->nseg > 0) {
-        const int nblk2 = (Bp / 32) % 2 == 0 && h->batch_nblk == 2 ? 2 : 1;
-        dim3 g(nblk(h->nseg, 4), Bp / (32 * nblk2));
-        if (nblk2 == 2)
-            hipLaunchKernelGGL(k_batched<2>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
-                               (const uint16_t *)h->d_xb, h->d_Fb);
-        else
-            hipLaunchKernelGGL(k_batched<1>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
-                               (const uint16_t *)h->d_xb, h->d_Fb);
-    }
+    if (h->nseg > 0) launch_batched_core(h, Bp, h->batch_nblk);
     const int64_t nout = (int64_t)B * h->nk * 16;
     hipLaunchKernelGGL(k_assemble_b, dim3(nblk(nout, 256)), dim3(256), 0, h->st, h->nk, B, Bp, h->d_Fb, h->d_feq,
                        h->d_rhoSB, h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac,

@@ -465,7 +465,10 @@ static int ensure_rkb(dmd_scatt *h, int B)
     h->rkb_cap = n;
     return DMD_OK;
 }
-static bool batched_native(const dmd_scatt *h) { return h->precision == DMD_PREC_BF16 && h->nb == 4; }
+static bool batched_native(const dmd_scatt *h)
+{
+    return (h->precision == DMD_PREC_BF16 || h->precision == DMD_PREC_C64) && h->nb == 4;
+}
 
 // one RK4 step of B trajectories (stage kernels over B nk states)
 static int launch_step_b(dmd_scatt *h, int B, const StageArgs &A)
@@ -520,7 +523,7 @@ extern "C" int dmd_scatt_time_eval_batched(dmd_scatt *h, double t, int n, double
 {
     if (!h || n < 1) return fail(DMD_ERR_ARG, "dmd_scatt_time_eval_batched: bad arguments");
     if (!batched_native(h) || h->last_B < 1)
-        return fail(DMD_ERR_STATE, "dmd_scatt_time_eval_batched: needs a BF16 nb = 4 handle after dmd_scatt_eval_batched");
+        return fail(DMD_ERR_STATE, "dmd_scatt_time_eval_batched: needs a reduced-precision nb = 4 handle after dmd_scatt_eval_batched");
     int rc;
     const int B = h->last_B;
     if ((rc = set_clock(h, t, 0.0, 0))) return rc;

@@ -146,7 +146,10 @@ struct dmd_scatt {
     int64_t nseg = 0;
     bool bvis_ready = false, bvis_dirty = true;
     int b_cap = 0;                       // padded trajectory count the buffers hold
-    int batch_nblk = 2;                  // 32-trajectory blocks per wavefront
+    int batch_nblk = 2;                  // 32-trajectory blocks per wavefront (autotuned)
+    int seg_len = 0;                     // visits per segment (autotuned)
+    int tuned_Bp = 0;                    // padded B the tuning was done for
+    std::vector<int64_t> h_vptr;         // first visit of every row (host)
     uint16_t *d_xb = nullptr;            // nk x 64 x Bp BF16 hi/lo drho operand
     float *d_Fb = nullptr;               // nk x 64 x Bp row sums
     double2 *d_rhoSB = nullptr;          // Bp x nk x 16 rho_S

@@ -1,10 +1,12 @@
 """Batched trajectories (B states sharing one pass over P on the matrix cores,
-BF16 P, nb = 4) through the C-ABI, against the FP64 oracle on the stored P
-and against the unbatched reduced-precision path, one trajectory at a time.
+nb = 4, BF16 or complex64 P) through the C-ABI, against the FP64 oracle on
+the stored P and against the unbatched reduced-precision path, one
+trajectory at a time.
 
 The batched form carries drho = rho - diag(f) as two BF16 terms (a 16-bit
-significand) with FP32 sums; everything else matches the unbatched BF16 form.
-Tolerances are stated against the part of ddm that drho drives.
+significand) with FP32 sums, and complex64 P as BF16 hi + lo; everything
+else matches the unbatched forms.  Tolerances are stated against the part of
+ddm that drho drives.
 """
 import numpy as np
 import pytest
@@ -38,10 +40,10 @@ def _states(B, nk, nb, f, rng, amp=1e-3):
     return rho
 
 
-@pytest.mark.parametrize("B", [3, 40, 64])
-def test_eval_batched_matches_oracle_and_unbatched(B):
+@pytest.mark.parametrize("prec,B", [(2, 3), (2, 40), (2, 64), (1, 40), (1, 64)])
+def test_eval_batched_matches_oracle_and_unbatched(prec, B):
     scatt, oracle = _mods()
-    print(f"case batched eval B={B}", flush=True)
+    print(f"case batched eval prec={prec} B={B}", flush=True)
     rng = np.random.default_rng(100 + B)
     nk, nb = 300, 4
     k1, k2 = _hub_pairs(nk, 3, rng)
@@ -54,7 +56,7 @@ def test_eval_batched_matches_oracle_and_unbatched(B):
     eq = random_rho(nk, nb, rng, scale=1e-6, diag=1e-7)
     t = 77 * FS
     prefac = 2 * np.pi / nk
-    with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=BF16) as op:
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=prec) as op:
         op.upload_P(P1, P2)
         op.set_split(f)
         op.set_ddmdt_eq(eq)
@@ -71,9 +73,9 @@ def test_eval_batched_matches_oracle_and_unbatched(B):
         worst = max(worst, rel_err(got[b] - base, ref - ref_f))
         assert rel_err(got[b], ref) < 1e-7
     d_one = rel_err(got - base[None], one - base[None])
-    print(f"B={B}: drho part vs oracle {worst:.2e}, vs unbatched BF16 {d_one:.2e}")
-    assert worst < 5e-6   # 16-bit drho operand, FP32 sums
-    assert d_one < 5e-6
+    print(f"prec={prec} B={B}: drho part vs oracle {worst:.2e}, vs unbatched {d_one:.2e}")
+    # 16-bit drho operand (and, for complex64, 16-bit hi + lo P), FP32 sums
+    assert worst < 5e-6 and d_one < 5e-6
 
 
 def test_eval_batched_fp64_handle_is_per_trajectory():
