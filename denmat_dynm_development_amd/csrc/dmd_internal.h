@@ -31,10 +31,10 @@ inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 #define DMD_TILE_R 512
 // chunks of one row block a tile-kernel workgroup takes in sequence
 #define DMD_TILE_GROUP 1
-// nb = 4 tile kernel: rows per tile, chunks per workgroup, pairs per chunk at most
+// nb = 4 panel kernel: rows per panel, pairs per chunk (one workgroup) at most
 #define DMD_TILE4_R 32
-#define DMD_TILE4_GROUP 8
-#define DMD_TILE4_CMAX 4096
+#define DMD_TILE4_GROUP 1
+#define DMD_TILE4_CMAX 2048
 
 // nb = 4 blocked pair record (512 complex elements, P1 then P2): offset of
 // P_mat[row][col] for U elements per 16-byte unit.  Lane l = 4 (row/4) +

@@ -893,32 +893,34 @@ extern "C" int dmd_scatt_create(int64_t nk, int nb, int64_t n_pairs, const uint6
         const int64_t cmax = lane_perm ? std::min<int64_t>(16384, std::max<int64_t>(512, n_pairs / 4096)) / 64 * 64
                                        : DMD_TILE4_CMAX;
         const int64_t pad = lane_perm ? 64 : 4;
-        std::vector<int64_t> tc(nbk * nbk + 1, 0);
-        for (int64_t i = 0; i < n_pairs; i++) {
-            int64_t f = KK(i);
-            tc[(int64_t)(k1st[f] / R) * nbk + (int64_t)(k2nd[f] / R) + 1]++;
-        }
-        for (int64_t b = 0; b < nbk * nbk; b++) tc[b + 1] += tc[b];
+        // nb <= 2: tiles (K/R, K'/R); nb = 4: row panels K/R (all k' of the rows)
+        const int64_t nbc = lane_perm ? nbk : 1, ntile = nbk * nbc;
+        auto tile_of = [&](int64_t f) {
+            return (int64_t)(k1st[f] / R) * nbc + (lane_perm ? (int64_t)(k2nd[f] / R) : 0);
+        };
+        std::vector<int64_t> tc(ntile + 1, 0);
+        for (int64_t i = 0; i < n_pairs; i++) tc[tile_of(KK(i)) + 1]++;
+        for (int64_t b = 0; b < ntile; b++) tc[b + 1] += tc[b];
         std::vector<int64_t> byt(n_pairs);  // file indices grouped by tile, (k, k') order inside
         {
             std::vector<int64_t> cur(tc.begin(), tc.end() - 1);
             for (int64_t i = 0; i < n_pairs; i++) {
                 int64_t f = KK(i);
-                byt[cur[(int64_t)(k1st[f] / R) * nbk + (int64_t)(k2nd[f] / R)]++] = f;
+                byt[cur[tile_of(f)]++] = f;
             }
         }
         nslots = 0;
-        for (int64_t t = 0; t < nbk * nbk; t++)
+        for (int64_t t = 0; t < ntile; t++)
             for (int64_t a = tc[t]; a < tc[t + 1]; a += cmax) {
                 int64_t n = std::min<int64_t>(cmax, tc[t + 1] - a);
                 cbeg.push_back(nslots);
                 clen.push_back((int32_t)n);
-                cij.push_back(make_int2((int)(t / nbk), (int)(t % nbk)));
+                cij.push_back(make_int2((int)(t / nbc), (int)(t % nbc)));
                 nslots += (n + pad - 1) / pad * pad;
             }
         phys.assign(nslots, -1);
         int64_t c = 0;
-        for (int64_t t = 0; t < nbk * nbk; t++)
+        for (int64_t t = 0; t < ntile; t++)
             for (int64_t a = tc[t]; a < tc[t + 1]; a += cmax, c++) {
                 const int64_t n = clen[c], m = (n + 63) / 64;
                 for (int64_t L = 0; L < n; L++)

@@ -170,6 +170,7 @@ __device__ __forceinline__ double2 up4(const double2 *A, int i, int j, bool one_
     return v;
 }
 
+__device__ __forceinline__ double2 c_elem(const double2 *F, const double2 *rk, const double2 *r1k, int i, int j);
 // X_ij of C = (1-rho) F1 - F2 rho for one row; F as [F1 re/im | F2 re/im];
 // 1-rho from r1k when given (explicit dm1), else formed from rho.
 __device__ __forceinline__ double2 c_elem(const double *F, const double2 *rk, const double2 *r1k, int i, int j)
@@ -186,6 +187,12 @@ __device__ __forceinline__ double2 c_elem(const double *F, const double2 *rk, co
         cfma(x, make_double2(-f2.x, -f2.y), up4(rk, m, j, false));
     }
     return x;
+}
+
+// the same with F as 32 complex [F1 | F2]
+__device__ __forceinline__ double2 c_elem(const double2 *F, const double2 *rk, const double2 *r1k, int i, int j)
+{
+    return c_elem((const double *)F, rk, r1k, i, j);
 }
 
 // Packed Hermitian increment of element l = 4i + j: H_ii (i == j), Re H_ij
@@ -238,115 +245,125 @@ __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t 
     }
 }
 
-// The tile kernel.  X: nk x 16 rho_S (FP64 plain, !YNEG: y = 1 - x) or drho_S
-// (reduced forms, YNEG: y = -x); DM1: y = Y, the caller's explicit 1 - rho
-// (FP64 only).  RAW writes the row sums F1|F2 (F_eq).
+// The panel kernel.  A workgroup streams one chunk of the pairs of one row
+// block I (k in [I R, I R + R)); X: nk x 16 rho_S (FP64 plain, !YNEG:
+// y = 1 - x) or drho_S (reduced forms, YNEG: y = -x); DM1: y = Y, the
+// caller's explicit 1 - rho (FP64 only).  Forward row sums F1|F2 of block I
+// stay in LDS and are assembled once at the end; each pair's reverse
+// contribution to row k' is assembled at once into the 16 reals of its
+// Hermitian increment and added with one FP64 atomic line.  RAW writes the
+// raw row sums F1|F2 instead (F_eq).
 template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1, int R>
-__global__ __launch_bounds__(256) void k_tiles4(int64_t nk, const PT *__restrict__ P,
-                                                const float *__restrict__ pscale, const uint32_t *__restrict__ pk,
-                                                const int64_t *__restrict__ cbeg, const int32_t *__restrict__ clen,
-                                                const int2 *__restrict__ cij, const int32_t *__restrict__ gbeg,
-                                                const typename C2<CT>::T *__restrict__ X,
-                                                const typename C2<CT>::T *__restrict__ Y,
-                                                const double2 *__restrict__ rhoS, const double2 *__restrict__ rho1S,
-                                                double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restrict__ P,
+                                                 const float *__restrict__ pscale, const int32_t *__restrict__ k1s,
+                                                 const int32_t *__restrict__ k2s, const int64_t *__restrict__ cbeg,
+                                                 const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
+                                                 const typename C2<CT>::T *__restrict__ X,
+                                                 const typename C2<CT>::T *__restrict__ Y,
+                                                 const double2 *__restrict__ rhoS, const double2 *__restrict__ rho1S,
+                                                 double *__restrict__ out)
 {
     using V = typename C2<CT>::T;
     constexpr int RY = DM1 ? R : 1;
-    __shared__ double accI[R * RS4], accJ[R * RS4];
-    __shared__ V xI[R * 16], xJ[R * 16], yI[RY * 16], yJ[RY * 16];
-    const int c0 = gbeg[blockIdx.x], c1 = gbeg[blockIdx.x + 1];
-    const int I = cij[c0].x;
+    __shared__ double accI[R * RS4];
+    __shared__ V xI[R * 16], yI[RY * 16];
+    // per 16-lane group: the pair's reverse sums G1 | G2 and the rows rho_k', 1 - rho_k'
+    __shared__ double2 gsc[16][32];
+    __shared__ double2 rsc[16][16], r1sc[DM1 ? 16 : 1][16];
+    const int c = blockIdx.x;
+    const int I = cij[c].x;
     const int64_t I0 = (int64_t)I * R;
-    for (int t = threadIdx.x; t < R * RS4; t += blockDim.x) { accI[t] = 0.0; accJ[t] = 0.0; }
+    for (int t = threadIdx.x; t < R * RS4; t += blockDim.x) accI[t] = 0.0;
     for (int t = threadIdx.x; t < R * 16; t += blockDim.x) {
         xI[t] = I0 * 16 + t < nk * 16 ? X[I0 * 16 + t] : cz<V>();
         if (DM1) yI[t] = I0 * 16 + t < nk * 16 ? Y[I0 * 16 + t] : cz<V>();
     }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, grp = lane >> 4, l = lane & 15;
-    const int a = l >> 2, b = l & 3;
-    for (int c = c0; c < c1; c++) {
-        const int J = cij[c].y;
-        const int64_t J0 = (int64_t)J * R;
-        double *aJ = (I == J) ? accI : accJ;
-        const V *xj = (I == J) ? xI : xJ;
-        const V *yj = (I == J) ? yI : yJ;
-        if (I != J)
-            for (int t = threadIdx.x; t < R * 16; t += blockDim.x) {
-                xJ[t] = J0 * 16 + t < nk * 16 ? X[J0 * 16 + t] : cz<V>();
-                if (DM1) yJ[t] = J0 * 16 + t < nk * 16 ? Y[J0 * 16 + t] : cz<V>();
-            }
-        __syncthreads();
-        const int64_t sb = cbeg[c];
-        const int Q = clen[c] >> 2;  // the chunk length is a multiple of 4
-        for (int it = wv; it < Q; it += 4) {
-            const int64_t s = sb + (int64_t)grp * Q + it;
-            const uint32_t code = pk[s];
-            const bool ok = code != 0xFFFFFFFFu;
-            const int lo1 = ok ? (int)(code % R) : 0, lo2 = ok ? (int)(code / R) : 0;
-            V p1[16], p2[16];
-            Loader<PT, V>::load(P + s * 512, l, p1, p2);
-            // x, y slices: columns 4b.. of x (rows b of the matrices), rows 4a.. of y
-            V xkp[4], ykp[4], xk[4], yk[4];
+    const int a = l >> 2, b = l & 3, gi = wv * 4 + grp;
+    const int64_t sb = cbeg[c];
+    const int Q = clen[c] >> 2;  // the chunk length is a multiple of 4
+    for (int it = wv; it < Q; it += 4) {
+        const int64_t s = sb + (int64_t)grp * Q + it;
+        const int32_t kk = k1s[s];
+        const bool ok = kk >= 0;
+        const int64_t kp = ok ? k2s[s] : 0;
+        const int lo1 = ok ? (int)(kk - I0) : 0;
+        V p1[16], p2[16];
+        Loader<PT, V>::load(P + s * 512, l, p1, p2);
+        // x, y slices: columns 4b.. of x (rows b of the matrices), rows 4a.. of y
+        V xkp[4], ykp[4], xk[4], yk[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                xkp[q] = xj[lo2 * 16 + 4 * b + q];
-                xk[q] = xI[lo1 * 16 + 4 * b + q];
-                const V u = xj[lo2 * 16 + 4 * a + q], w = xI[lo1 * 16 + 4 * a + q];
-                if (DM1) {
-                    ykp[q] = yj[lo2 * 16 + 4 * a + q];
-                    yk[q] = yI[lo1 * 16 + 4 * a + q];
-                } else if (YNEG) {
-                    ykp[q].x = -u.x; ykp[q].y = -u.y;
-                    yk[q].x = -w.x; yk[q].y = -w.y;
-                } else {
-                    ykp[q].x = (a == q ? 1 : 0) - u.x; ykp[q].y = -u.y;
-                    yk[q].x = (a == q ? 1 : 0) - w.x; yk[q].y = -w.y;
-                }
-            }
-            V f1[4], f2[4], g1[4], g2[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) { f1[q] = cz<V>(); f2[q] = cz<V>(); g1[q] = cz<V>(); g2[q] = cz<V>(); }
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    cfma_t(f1[i], p1[4 * i + j], xkp[j]);   // Pdm_k[4a+i]   += P1[4a+i][4b+j] rho_k'[4b+j]
-                    cfma_t(f2[j], ykp[i], p2[4 * i + j]);   // dm1P_k[4b+j]  += (1-rho_k')[4a+i] P2[4a+i][4b+j]
-                    cfmac_t(g1[i], xk[j], p2[4 * i + j]);   // Pdm_k'[4a+i]  += rho_k[4b+j] conj(P2[4a+i][4b+j])
-                    cfmac_t(g2[j], yk[i], p1[4 * i + j]);   // dm1P_k'[4b+j] += (1-rho_k)[4a+i] conj(P1[4a+i][4b+j])
-                }
-            if (pscale) {
-                const CT sc = (CT)pscale[s];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    f1[q].x *= sc; f1[q].y *= sc; f2[q].x *= sc; f2[q].y *= sc;
-                    g1[q].x *= sc; g1[q].y *= sc; g2[q].x *= sc; g2[q].y *= sc;
-                }
-            }
-            const V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
-            const V F2 = reduce4(f2, lane, 8, 4);  // output 4b + a
-            const V G1 = reduce4(g1, lane, 2, 1);
-            const V G2 = reduce4(g2, lane, 8, 4);
-            if (ok) {
-                double *r1 = accI + lo1 * RS4;
-                atomicAdd(r1 + 2 * (4 * a + b), (double)F1.x);
-                atomicAdd(r1 + 2 * (4 * a + b) + 1, (double)F1.y);
-                atomicAdd(r1 + 32 + 2 * (4 * b + a), (double)F2.x);
-                atomicAdd(r1 + 32 + 2 * (4 * b + a) + 1, (double)F2.y);
-                if (I0 + lo1 < J0 + lo2) {  // k < k': the reverse direction
-                    double *r2 = aJ + lo2 * RS4;
-                    atomicAdd(r2 + 2 * (4 * a + b), (double)G1.x);
-                    atomicAdd(r2 + 2 * (4 * a + b) + 1, (double)G1.y);
-                    atomicAdd(r2 + 32 + 2 * (4 * b + a), (double)G2.x);
-                    atomicAdd(r2 + 32 + 2 * (4 * b + a) + 1, (double)G2.y);
-                }
+        for (int q = 0; q < 4; q++) {
+            xkp[q] = X[kp * 16 + 4 * b + q];
+            xk[q] = xI[lo1 * 16 + 4 * b + q];
+            const V u = X[kp * 16 + 4 * a + q], w = xI[lo1 * 16 + 4 * a + q];
+            if (DM1) {
+                ykp[q] = Y[kp * 16 + 4 * a + q];
+                yk[q] = yI[lo1 * 16 + 4 * a + q];
+            } else if (YNEG) {
+                ykp[q].x = -u.x; ykp[q].y = -u.y;
+                yk[q].x = -w.x; yk[q].y = -w.y;
+            } else {
+                ykp[q].x = (a == q ? 1 : 0) - u.x; ykp[q].y = -u.y;
+                yk[q].x = (a == q ? 1 : 0) - w.x; yk[q].y = -w.y;
             }
         }
-        __syncthreads();
-        if (I != J) flush_block4<RAW, DM1, R>(accJ, J0, nk, rhoS, rho1S, out);
-        __syncthreads();
+        V f1[4], f2[4], g1[4], g2[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) { f1[q] = cz<V>(); f2[q] = cz<V>(); g1[q] = cz<V>(); g2[q] = cz<V>(); }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                cfma_t(f1[i], p1[4 * i + j], xkp[j]);   // Pdm_k[4a+i]   += P1[4a+i][4b+j] rho_k'[4b+j]
+                cfma_t(f2[j], ykp[i], p2[4 * i + j]);   // dm1P_k[4b+j]  += (1-rho_k')[4a+i] P2[4a+i][4b+j]
+                cfmac_t(g1[i], xk[j], p2[4 * i + j]);   // Pdm_k'[4a+i]  += rho_k[4b+j] conj(P2[4a+i][4b+j])
+                cfmac_t(g2[j], yk[i], p1[4 * i + j]);   // dm1P_k'[4b+j] += (1-rho_k)[4a+i] conj(P1[4a+i][4b+j])
+            }
+        if (pscale) {
+            const CT sc = (CT)pscale[s];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                f1[q].x *= sc; f1[q].y *= sc; f2[q].x *= sc; f2[q].y *= sc;
+                g1[q].x *= sc; g1[q].y *= sc; g2[q].x *= sc; g2[q].y *= sc;
+            }
+        }
+        const V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
+        const V F2 = reduce4(f2, lane, 8, 4);  // output 4b + a
+        const V G1 = reduce4(g1, lane, 2, 1);
+        const V G2 = reduce4(g2, lane, 8, 4);
+        if (ok) {
+            double *r1 = accI + lo1 * RS4;
+            atomicAdd(r1 + 2 * (4 * a + b), (double)F1.x);
+            atomicAdd(r1 + 2 * (4 * a + b) + 1, (double)F1.y);
+            atomicAdd(r1 + 32 + 2 * (4 * b + a), (double)F2.x);
+            atomicAdd(r1 + 32 + 2 * (4 * b + a) + 1, (double)F2.y);
+        }
+        const bool rev = ok && kk < kp;
+        if (RAW) {
+            if (rev) {
+                double *o = out + kp * 64;
+                unsafeAtomicAdd(o + 2 * (4 * a + b), (double)G1.x);
+                unsafeAtomicAdd(o + 2 * (4 * a + b) + 1, (double)G1.y);
+                unsafeAtomicAdd(o + 32 + 2 * (4 * b + a), (double)G2.x);
+                unsafeAtomicAdd(o + 32 + 2 * (4 * b + a) + 1, (double)G2.y);
+            }
+        } else {
+            // the pair's reverse term, assembled: C' = (1-rho_k') G1 - G2 rho_k' (zhemm
+            // upper), H' = C' + C'^+, lane (i, j) = (a, b) owns packed element l
+            gsc[gi][4 * a + b] = make_double2((double)G1.x, (double)G1.y);
+            gsc[gi][16 + 4 * b + a] = make_double2((double)G2.x, (double)G2.y);
+            rsc[gi][l] = rhoS[kp * 16 + l];
+            if (DM1) r1sc[gi][l] = rho1S[kp * 16 + l];
+            __builtin_amdgcn_wave_barrier();
+            const double2 *r1p = DM1 ? r1sc[gi] : nullptr;
+            const double v = herm_pack(c_elem(gsc[gi], rsc[gi], r1p, a, b), c_elem(gsc[gi], rsc[gi], r1p, b, a), a, b);
+            __builtin_amdgcn_wave_barrier();  // the group's LDS rows are read before the next pair writes them
+            if (rev && v != 0.0) unsafeAtomicAdd(out + kp * 16 + l, v);
+        }
     }
+    __syncthreads();
     flush_block4<RAW, DM1, R>(accI, I0, nk, rhoS, rho1S, out);
 }
 
@@ -643,9 +660,9 @@ int tiles4_relayout(dmd_scatt *h, int to_blocked)
 template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1 = false>
 static void launch_k4(dmd_scatt *h, const void *X, const void *Y, double *out)
 {
-    if (h->ngroups <= 0) return;
-    hipLaunchKernelGGL((k_tiles4<PT, CT, RAW, YNEG, DM1, DMD_TILE4_R>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st,
-                       h->nk, (const PT *)h->d_P, h->d_pscale, h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg,
+    if (h->nchunks <= 0) return;
+    hipLaunchKernelGGL((k_panels4<PT, CT, RAW, YNEG, DM1, DMD_TILE4_R>), dim3((unsigned)h->nchunks), dim3(256), 0,
+                       h->st, h->nk, (const PT *)h->d_P, h->d_pscale, h->d_k1, h->d_k2, h->d_cbeg, h->d_clen, h->d_cij,
                        (const typename C2<CT>::T *)X, (const typename C2<CT>::T *)Y, h->d_rhoS, h->d_rho1S, out);
 }
 
