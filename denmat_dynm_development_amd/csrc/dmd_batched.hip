@@ -10,22 +10,23 @@
 // equilibrium part F(diag f, 1 - diag f) is the FP64 F_eq shared with the
 // unbatched reduced forms, and dm1P = F2_eq - G (1 - rho = (1 - f) - drho).
 //
-// Matrix-core mapping (v_mfma_f32_32x32x16_bf16, FP32 accumulation): a
-// complex 16 x 16 matrix times 16 x 32 complex columns is the real 32 x 32
-// product [[Ar, -Ai], [Ai, Ar]] [Xr; Xi].  X is carried as two BF16 terms
-// (hi + lo, a 16-bit significand).  With a BF16 store A is exact in BF16 and
-// each visit issues 2 K-halves x 2 terms x 2 products = 8 MFMAs per 32
-// trajectories; with a complex64 store A is split the same way (hi + lo) and
-// the three leading cross terms make 12.
+// Matrix-core mapping (v_mfma_f32_32x32x16_bf16, FP32 accumulation).  X is
+// Hermitian (rho_S is, f is real and diagonal), so its 16 complex entries are
+// 16 reals x = (Re X_ii; Re X_ij, Im X_ij for i < j).  A vec(X) is then the
+// real 32 x 16 product M x with M = [A_ii | A_ij + A_ji | i (A_ij - A_ji)]
+// (rows: Re and Im of the 16 outputs), one K = 16 MFMA per 32 trajectories.
+// M and x are carried as BF16 hi + lo (16-bit significands); the three
+// leading cross terms make 3 MFMAs per product, 6 per visit per 32
+// trajectories.
 //
 // Layouts in HBM.
-//  * Visit records, CSR by destination row: 2 x 256 words (BF16 re | im << 16)
-//    per visit, A1 row-major then A2^T row-major, conj and the per-pair
-//    power-of-two scale applied (2 KiB per visit, 4 KiB per stored pair);
-//    a complex64 store adds the lo terms (4 KiB per visit).
-//  * X: per partner row q, [term][re/im][k-half][trajectory][8] BF16, so a
+//  * Visit records, CSR by destination row: M1 (from A1) and M2 (from A2^T),
+//    32 x 16 BF16 row-major each, hi then lo: [M1 hi | M2 hi | M1 lo | M2 lo],
+//    4 KiB per visit (8 KiB per stored pair), conj and the per-pair scale of
+//    the BF16 store applied.
+//  * X: per partner row q, [term][k-half][trajectory][8] BF16, so a
 //    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
-//    bytes.
+//    bytes (Bp x 64 bytes per row q).
 //  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
 // Uneven rows: every row's visits are cut into segments of at most seg_len
 // visits, one wavefront each, so a 5000-partner row is as many independent
@@ -55,58 +56,69 @@ __device__ __forceinline__ uint32_t f2bf(float f)  // round to nearest even (fin
 __device__ __forceinline__ float bflo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bfhi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
-// Builds visit records from the blocked BF16 store.  One 64-lane group per visit.
+// Complex element (row, col) of A1 = P1 | conj(P2) or A2^T = P2^T | conj(P1)^T
+// of a stored pair, from the blocked store (BF16 with scale, or complex64).
+template <bool C64>
+__device__ __forceinline__ float2 a_elem(const void *rec, float sc, bool rev, int mat, int r, int c)
+{
+    const int src_mat = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
+    const int row = mat == 0 ? r : c, col = mat == 0 ? c : r;
+    float2 w;
+    if (C64) {
+        w = ((const float2 *)rec)[blk_off(src_mat, row, col, 2)];
+    } else {
+        const uint32_t u = ((const uint32_t *)rec)[blk_off(src_mat, row, col, 4)];
+        w = make_float2(bflo(u) * sc, bfhi(u) * sc);
+    }
+    if (rev) w.y = -w.y;
+    return w;
+}
+
+// Visit records: M = [A_ii | A_ij + A_ji | i (A_ij - A_ji)] real-ified (32 x 16)
+// for A1 and A2^T, BF16 hi and lo.  One 64-lane group per visit; lane t
+// covers rows (t / 16) of the 4 row blocks and K column t % 16.
+template <bool C64>
 __global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const int64_t *__restrict__ vslot,
-                                                     const uint8_t *__restrict__ vdir, const uint32_t *__restrict__ Pq,
-                                                     const float *__restrict__ pscale, uint32_t *__restrict__ vrec)
+                                                     const uint8_t *__restrict__ vdir, const void *__restrict__ Pq,
+                                                     const float *__restrict__ pscale, uint16_t *__restrict__ vrec)
 {
     const int64_t v = blockIdx.x;
     if (v >= nvis) return;
     const int64_t s = vslot[v];
     const bool rev = vdir[v] != 0;
-    const uint32_t *rec = Pq + s * 512;
-    const float sc = pscale[s];
-    uint32_t *out = vrec + v * 512;
-    for (int t = threadIdx.x; t < 512; t += 64) {
-        const int mat = t >> 8, r = (t >> 4) & 15, c = t & 15;
-        // A1 = P1 | conj(P2); A2^T[r][c] = A2[c][r], A2 = P2 | conj(P1)
-        const int src_mat = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
-        const int row = mat == 0 ? r : c, col = mat == 0 ? c : r;
-        const uint32_t w = rec[blk_off(src_mat, row, col, 4)];
-        const float re = bflo(w) * sc, im = bfhi(w) * sc;
-        out[t] = f2bf(re) | (f2bf(rev ? -im : im) << 16);
+    const void *rec = C64 ? (const void *)((const float2 *)Pq + s * 512) : (const void *)((const uint32_t *)Pq + s * 512);
+    const float sc = C64 ? 1.0f : pscale[s];
+    uint16_t *out = vrec + v * 2048;
+    // K column p -> (i, j, kind): 0..3 diagonal, 4..9 Re upper, 10..15 Im upper
+    const int pi[16] = { 0, 1, 2, 3, 0, 0, 0, 1, 1, 2, 0, 0, 0, 1, 1, 2 };
+    const int pj[16] = { 0, 1, 2, 3, 1, 2, 3, 2, 3, 3, 1, 2, 3, 2, 3, 3 };
+    for (int t = threadIdx.x; t < 2 * 16 * 16; t += 64) {
+        const int mat = t >> 8, r = (t >> 4) & 15, p = t & 15;
+        const int i = pi[p], j = pj[p];
+        const float2 aij = a_elem<C64>(rec, sc, rev, mat, r, 4 * i + j);
+        float2 m;
+        if (p < 4) {
+            m = aij;
+        } else {
+            const float2 aji = a_elem<C64>(rec, sc, rev, mat, r, 4 * j + i);
+            m = p < 10 ? make_float2(aij.x + aji.x, aij.y + aji.y)      // A_ij + A_ji
+                       : make_float2(-(aij.y - aji.y), aij.x - aji.x);  // i (A_ij - A_ji)
+        }
+        // real-ified rows: r (Re of output r) and 16 + r (Im of output r)
+        const float vals[2] = { m.x, m.y };
+        for (int part = 0; part < 2; part++) {
+            const float x = vals[part];
+            const uint32_t hi = f2bf(x), lo = f2bf(x - __uint_as_float(hi << 16));
+            const int o = mat * 512 + (part * 16 + r) * 16 + p;
+            out[o] = (uint16_t)hi;
+            out[1024 + o] = (uint16_t)lo;
+        }
     }
 }
 
-// Complex64 store: every element split into BF16 hi + lo (a 16-bit
-// significand); records [A1 hi | A2^T hi | A1 lo | A2^T lo] (4 KiB per visit).
-__global__ __launch_bounds__(64) void k_build_visits_split(int64_t nvis, const int64_t *__restrict__ vslot,
-                                                           const uint8_t *__restrict__ vdir,
-                                                           const float2 *__restrict__ Pc,
-                                                           uint32_t *__restrict__ vrec)
-{
-    const int64_t v = blockIdx.x;
-    if (v >= nvis) return;
-    const int64_t s = vslot[v];
-    const bool rev = vdir[v] != 0;
-    const float2 *rec = Pc + s * 512;
-    uint32_t *out = vrec + v * 1024;
-    for (int t = threadIdx.x; t < 512; t += 64) {
-        const int mat = t >> 8, r = (t >> 4) & 15, c = t & 15;
-        const int src_mat = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
-        const int row = mat == 0 ? r : c, col = mat == 0 ? c : r;
-        const float2 w = rec[blk_off(src_mat, row, col, 2)];
-        const float re = w.x, im = rev ? -w.y : w.y;
-        const uint32_t rh = f2bf(re), ih = f2bf(im);
-        const uint32_t rl = f2bf(re - __uint_as_float(rh << 16)), il = f2bf(im - __uint_as_float(ih << 16));
-        out[t] = rh | (ih << 16);
-        out[512 + t] = rl | (il << 16);
-    }
-}
-
-// drho_S of every trajectory as the hi/lo BF16 operand, and rho_S in FP64
-// for the assembly.  rho: B x nk x 16 (interaction picture).  One thread per
-// (b, k, element).
+// x = (Re drho_ii; Re drho_ij, Im drho_ij, i < j) of every trajectory as the
+// hi/lo BF16 operand, and rho_S in FP64 for the assembly.  rho: B x nk x 16
+// (interaction picture).  One thread per (b, k, K column p).
 __global__ void k_prepare_x(int64_t nk, int B, int Bp, const double2 *__restrict__ rho, const double *__restrict__ e,
                             const DClock *__restrict__ clk, double a, int expt, const double *__restrict__ f,
                             double2 *__restrict__ rhoS, uint16_t *__restrict__ xb)
@@ -117,25 +129,27 @@ __global__ void k_prepare_x(int64_t nk, int B, int Bp, const double2 *__restrict
     const int b = (int)(bk / nk);
     const int64_t k = bk - (int64_t)b * nk;
     const int l = (int)(idx & 15), i = l >> 2, j = l & 3;
-    double2 v = make_double2(0, 0);
+    double x = 0.0;
     if (b < B) {
-        v = rho[idx];
-        if (expt && i != j) v = cmul(v, ccis(-(e[k * 4 + i] - e[k * 4 + j]) * clock_t_at(clk, a)));
+        const double t = clock_t_at(clk, a);
+        double2 v = rho[idx];
+        if (expt && i != j) v = cmul(v, ccis(-(e[k * 4 + i] - e[k * 4 + j]) * t));
         rhoS[idx] = v;
-        if (f && i == j) v.x -= f[k * 4 + i];
+        // this thread's K column p = l: its (i, j, kind)
+        const int pi[16] = { 0, 1, 2, 3, 0, 0, 0, 1, 1, 2, 0, 0, 0, 1, 1, 2 };
+        const int pj[16] = { 0, 1, 2, 3, 1, 2, 3, 2, 3, 3, 1, 2, 3, 2, 3, 3 };
+        const int ii = pi[l], jj = pj[l];
+        double2 w = rho[bk * 16 + 4 * ii + jj];
+        if (expt && ii != jj) w = cmul(w, ccis(-(e[k * 4 + ii] - e[k * 4 + jj]) * t));
+        if (l < 4) x = w.x - (f ? f[k * 4 + ii] : 0.0);
+        else x = l < 10 ? w.x : w.y;
     }
-    // [q][term][re/im][k-half][b][8]
+    const uint32_t hi = f2bf((float)x);
+    const uint32_t lo = f2bf((float)(x - (double)__uint_as_float(hi << 16)));
     const int half = l >> 3, el = l & 7;
-    const float re = (float)v.x, im = (float)v.y;
-    const uint32_t rh = f2bf(re), ih = f2bf(im);
-    const uint32_t rl = f2bf((float)(v.x - (double)__uint_as_float(rh << 16)));
-    const uint32_t il = f2bf((float)(v.y - (double)__uint_as_float(ih << 16)));
-    const int64_t base = k * 8;  // 2 terms x 2 parts x 2 halves
-    auto at = [&](int term, int part) { return ((((base + term * 4 + part * 2 + half) * Bp) + b) * 8 + el); };
-    xb[at(0, 0)] = (uint16_t)rh;
-    xb[at(0, 1)] = (uint16_t)ih;
-    xb[at(1, 0)] = (uint16_t)rl;
-    xb[at(1, 1)] = (uint16_t)il;
+    // [q][term][k-half][b][8]
+    xb[(((k * 2 + 0) * 2 + half) * Bp + b) * 8 + el] = (uint16_t)hi;
+    xb[(((k * 2 + 1) * 2 + half) * Bp + b) * 8 + el] = (uint16_t)lo;
 }
 
 __device__ __forceinline__ bf16x8 as_bf8(uint4 u)
@@ -145,33 +159,14 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 u)
     return r;
 }
 
-// The lane's A operands (K chunk 0 multiplies Xr, chunk 1 Xi; re rows [Ar | -Ai],
-// im rows [Ai | Ar]) from 8 words (BF16 re | im << 16) of one row of A.
-__device__ __forceinline__ void a_operands(const uint4 a, const uint4 b, bool im_row, bf16x8 &c0, bf16x8 &c1)
-{
-    const uint32_t w[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
-    uint32_t o0[4], o1[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const uint32_t x0 = w[2 * t], x1 = w[2 * t + 1];
-        const uint32_t re = (x0 & 0xFFFFu) | (x1 << 16), im = (x0 >> 16) | (x1 & 0xFFFF0000u);
-        o0[t] = im_row ? im : re;
-        o1[t] = im_row ? re : (im ^ 0x80008000u);
-    }
-    c0 = as_bf8(make_uint4(o0[0], o0[1], o0[2], o0[3]));
-    c1 = as_bf8(make_uint4(o1[0], o1[1], o1[2], o1[3]));
-}
-
 // One wavefront per segment; NBLK blocks of 32 trajectories per pass (the
-// grid's y index selects the pass).  PSPLIT: A = A_hi + A_lo (complex64
-// store); the products A_hi X_hi + A_hi X_lo + A_lo X_hi (the A_lo X_lo term
-// is below the 16-bit significands).
-template <int NBLK, bool PSPLIT>
+// grid's y index selects the pass).  Products M_hi x_hi + M_hi x_lo + M_lo
+// x_hi (the M_lo x_lo term is below the 16-bit significands).
+template <int NBLK>
 __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
-                                                 const uint32_t *__restrict__ vrec, const int32_t *__restrict__ vq,
+                                                 const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
                                                  const uint16_t *__restrict__ xb, float *__restrict__ F)
 {
-    constexpr int RW = PSPLIT ? 1024 : 512;  // words per visit record
     const int lane = threadIdx.x & 63;
     const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= nseg) return;
@@ -180,49 +175,30 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
     const int64_t v0 = (int64_t)(uint32_t)S.y | ((int64_t)S.z << 32);
     const int nv = S.w;
     const int i = lane & 31, h = lane >> 5;  // A row (real-ified), K half
-    const int r = i & 15, im_row = i >> 4;
     const int b0 = blockIdx.y * NBLK * 32;
     floatx16 f1[NBLK], g2[NBLK];
 #pragma unroll
     for (int nb = 0; nb < NBLK; nb++)
 #pragma unroll
         for (int t = 0; t < 16; t++) { f1[nb][t] = 0.0f; g2[nb][t] = 0.0f; }
+    const int oa = i * 16 + 8 * h;  // the lane's 8 K columns of row i
     for (int u = 0; u < nv; u++) {
         const int64_t v = v0 + u;
-        const uint32_t *rec = vrec + v * RW;
-        // the lane's 8 elements of row r of A1 and of A2^T (K range 8h .. 8h+7)
-        const int o1 = r * 16 + 8 * h, o2 = 256 + r * 16 + 8 * h;
-        bf16x8 A1c0, A1c1, A2c0, A2c1, L1c0, L1c1, L2c0, L2c1;
-        a_operands(*(const uint4 *)(rec + o1), *(const uint4 *)(rec + o1 + 4), im_row, A1c0, A1c1);
-        a_operands(*(const uint4 *)(rec + o2), *(const uint4 *)(rec + o2 + 4), im_row, A2c0, A2c1);
-        if (PSPLIT) {
-            a_operands(*(const uint4 *)(rec + 512 + o1), *(const uint4 *)(rec + 512 + o1 + 4), im_row, L1c0, L1c1);
-            a_operands(*(const uint4 *)(rec + 512 + o2), *(const uint4 *)(rec + 512 + o2 + 4), im_row, L2c0, L2c1);
-        }
-        const int64_t q = vq[v];
+        const uint16_t *rec = vrec + v * 2048;
+        const bf16x8 M1h = as_bf8(*(const uint4 *)(rec + oa)), M2h = as_bf8(*(const uint4 *)(rec + 512 + oa));
+        const bf16x8 M1l = as_bf8(*(const uint4 *)(rec + 1024 + oa)), M2l = as_bf8(*(const uint4 *)(rec + 1536 + oa));
+        const uint16_t *xq = xb + (int64_t)vq[v] * 4 * (int64_t)Bp * 8;
 #pragma unroll
         for (int nb = 0; nb < NBLK; nb++) {
             const int b = b0 + nb * 32 + i;
-            // X operand: [q][term][re/im][k-half][b][8]
-            const uint16_t *xq = xb + q * 8 * (int64_t)Bp * 8;
-            const bf16x8 xh0 = as_bf8(*(const uint4 *)(xq + ((0 * 2 + 0) * 2 + h) * (int64_t)Bp * 8 + b * 8));
-            const bf16x8 xh1 = as_bf8(*(const uint4 *)(xq + ((0 * 2 + 1) * 2 + h) * (int64_t)Bp * 8 + b * 8));
-            const bf16x8 xl0 = as_bf8(*(const uint4 *)(xq + ((1 * 2 + 0) * 2 + h) * (int64_t)Bp * 8 + b * 8));
-            const bf16x8 xl1 = as_bf8(*(const uint4 *)(xq + ((1 * 2 + 1) * 2 + h) * (int64_t)Bp * 8 + b * 8));
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c0, xh0, f1[nb], 0, 0, 0);
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c1, xh1, f1[nb], 0, 0, 0);
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c0, xl0, f1[nb], 0, 0, 0);
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1c1, xl1, f1[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c0, xh0, g2[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xh1, g2[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c0, xl0, g2[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2c1, xl1, g2[nb], 0, 0, 0);
-            if (PSPLIT) {
-                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L1c0, xh0, f1[nb], 0, 0, 0);
-                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L1c1, xh1, f1[nb], 0, 0, 0);
-                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L2c0, xh0, g2[nb], 0, 0, 0);
-                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(L2c1, xh1, g2[nb], 0, 0, 0);
-            }
+            const bf16x8 xh = as_bf8(*(const uint4 *)(xq + ((0 * 2 + h) * (int64_t)Bp + b) * 8));
+            const bf16x8 xl = as_bf8(*(const uint4 *)(xq + ((1 * 2 + h) * (int64_t)Bp + b) * 8));
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xh, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xh, g2[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xl, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xl, g2[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1l, xh, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2l, xh, g2[nb], 0, 0, 0);
         }
     }
     // D layout: element t of lane l is row 8 (t / 4) + 4 h + t % 4, column (trajectory) l % 32
@@ -344,8 +320,8 @@ static int build_visits(dmd_scatt *h)
     int rc;
     dfree(h->d_vrec); dfree(h->d_vq);
     h->d_vrec = nullptr; h->d_vq = nullptr;
-    const bool split = h->precision == DMD_PREC_C64;
-    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * (split ? 1024 : 512), "visit records"))) return rc;
+    const bool c64 = h->precision == DMD_PREC_C64;
+    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * 2048, "visit records"))) return rc;
     if ((rc = dalloc_t(&h->d_vq, (size_t)nvis, "visit partners"))) return rc;
     int64_t *d_vslot = nullptr;
     uint8_t *d_vdir = nullptr;
@@ -359,12 +335,12 @@ static int build_visits(dmd_scatt *h)
     }
 
     if (e == hipSuccess && nvis) {
-        if (split)
-            hipLaunchKernelGGL(k_build_visits_split, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
-                               (const float2 *)h->d_P, h->d_vrec);
+        if (c64)
+            hipLaunchKernelGGL(k_build_visits<true>, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
+                               (const void *)h->d_P, h->d_pscale, h->d_vrec);
         else
-            hipLaunchKernelGGL(k_build_visits, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
-                               (const uint32_t *)h->d_P, h->d_pscale, h->d_vrec);
+            hipLaunchKernelGGL(k_build_visits<false>, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
+                               (const void *)h->d_P, h->d_pscale, h->d_vrec);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     }
@@ -444,11 +420,11 @@ int batched_prepare(dmd_scatt *h, int B)
         dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB);
         h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr;
         h->b_cap = 0;
-        if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 64 * Bp, "batched X"))) return rc;
+        if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 32 * Bp, "batched X"))) return rc;
         if ((rc = dalloc_t(&h->d_Fb, (size_t)h->nk * 64 * Bp, "batched row sums"))) return rc;
         if ((rc = dalloc_t(&h->d_rhoSB, (size_t)h->nk * 16 * Bp, "batched rho_S"))) return rc;
         DMD_HIP(hipMemset(h->d_Fb, 0, (size_t)h->nk * 64 * Bp * sizeof(float)));
-        DMD_HIP(hipMemset(h->d_xb, 0, (size_t)h->nk * 64 * Bp * sizeof(uint16_t)));
+        DMD_HIP(hipMemset(h->d_xb, 0, (size_t)h->nk * 32 * Bp * sizeof(uint16_t)));
         h->b_cap = Bp;
     }
     if (h->tuned_Bp != Bp && (rc = autotune(h, Bp))) return rc;
@@ -460,13 +436,12 @@ static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
 {
     const int nb2 = (Bp / 32) % 2 == 0 && nblk_req == 2 ? 2 : 1;
     dim3 g(nblk(h->nseg, 4), Bp / (32 * nb2));
-    const bool split = h->precision == DMD_PREC_C64;
-#define DMD_KB(N, S)                                                                                             \
-    hipLaunchKernelGGL((k_batched<N, S>), g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq, \
-                       (const uint16_t *)h->d_xb, h->d_Fb)
-    if (nb2 == 2) { if (split) DMD_KB(2, true); else DMD_KB(2, false); }
-    else { if (split) DMD_KB(1, true); else DMD_KB(1, false); }
-#undef DMD_KB
+    if (nb2 == 2)
+        hipLaunchKernelGGL(k_batched<2>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                           (const uint16_t *)h->d_xb, h->d_Fb);
+    else
+        hipLaunchKernelGGL(k_batched<1>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                           (const uint16_t *)h->d_xb, h->d_Fb);
 }
 
 // ddm of B trajectories (rho: B x nk x 16 device, interaction picture at the

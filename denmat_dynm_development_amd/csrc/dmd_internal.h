@@ -140,7 +140,7 @@ struct dmd_scatt {
     double *d_hacc16 = nullptr;          // nk x 16 packed Hermitian row accumulator (nb = 4)
     bool feq_dirty = true;
     // batched trajectories (BF16, nb = 4): visit records, segments, operands
-    uint32_t *d_vrec = nullptr;          // visits x 512 words: A1 | A2^T (BF16 re | im << 16)
+    uint16_t *d_vrec = nullptr;          // visits x 2048 BF16: M1 hi | M2 hi | M1 lo | M2 lo
     int32_t *d_vq = nullptr;             // partner row of each visit
     int4 *d_seg = nullptr;               // segments: (row, first visit lo, hi, count)
     int64_t nseg = 0;
@@ -150,7 +150,7 @@ struct dmd_scatt {
     int seg_len = 0;                     // visits per segment (autotuned)
     int tuned_Bp = 0;                    // padded B the tuning was done for
     std::vector<int64_t> h_vptr;         // first visit of every row (host)
-    uint16_t *d_xb = nullptr;            // nk x 64 x Bp BF16 hi/lo drho operand
+    uint16_t *d_xb = nullptr;            // nk x 32 x Bp BF16 hi/lo drho operand
     float *d_Fb = nullptr;               // nk x 64 x Bp row sums
     double2 *d_rhoSB = nullptr;          // Bp x nk x 16 rho_S
     double2 *d_bin = nullptr, *d_bout = nullptr;  // B x nk x 16 staging
