@@ -64,6 +64,54 @@ class SynthParams(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LdbdInfo(ctypes.Structure):
+    """dmd_ldbd_info (include/dmd_scatt.h): the contents of ldbd_size.dat."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("n_line2", ctypes.c_int),
+        ("nb", ctypes.c_int),
+        ("nv", ctypes.c_int),
+        ("b_bot_dm", ctypes.c_int),
+        ("b_top_dm", ctypes.c_int),
+        ("b_bot_eph", ctypes.c_int),
+        ("b_top_eph", ctypes.c_int),
+        ("nb_wannier", ctypes.c_int),
+        ("nk_full", ctypes.c_double),
+        ("nk", ctypes.c_int64),
+        ("kmesh", ctypes.c_int * 3),
+        ("nkpair", ctypes.c_int64),
+        ("mode_start", ctypes.c_int),
+        ("mode_stop", ctypes.c_int),
+        ("n_modes", ctypes.c_int),
+        ("temperature", ctypes.c_double),
+        ("mu_min", ctypes.c_double),
+        ("mu_max", ctypes.c_double),
+        ("mu", ctypes.c_double),
+        ("degauss", ctypes.c_double),
+        ("ndegauss", ctypes.c_double),
+        ("e_bot_probe", ctypes.c_double),
+        ("e_top_probe", ctypes.c_double),
+        ("e_bot_dm", ctypes.c_double),
+        ("e_top_dm", ctypes.c_double),
+        ("e_bot_eph", ctypes.c_double),
+        ("e_top_eph", ctypes.c_double),
+        ("omega_max", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["kmesh"] = tuple(self.kmesh)
+        return d
+
+    @property
+    def nb_dm(self):
+        return self.b_top_dm - self.b_bot_dm
+
+    @property
+    def nb_eph(self):
+        return self.b_top_eph - self.b_bot_eph
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -71,6 +119,8 @@ _d = ctypes.c_double
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pd = ctypes.POINTER(ctypes.c_double)
 _pSP = ctypes.POINTER(SynthParams)
+_pLI = ctypes.POINTER(LdbdInfo)
+_cs = ctypes.c_char_p
 
 # name -> argtypes (all return int unless listed in _RESTYPE)
 PROTOTYPES = {
@@ -109,6 +159,19 @@ PROTOTYPES = {
     "dmd_synth_couplings": [_pSP, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp],
     "dmd_synth_spin": [_pSP, _vp],
     "dmd_ldbd_write_synth": [ctypes.c_char_p, _pSP, _i],
+    "dmd_ldbd_read_info": [_cs, _pLI],
+    "dmd_ldbd_read_ek": [_cs, _pLI, _d, _vp],
+    "dmd_ldbd_read_kvec": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_smat": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_imsig": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_R": [_cs, ctypes.POINTER(_i), _vp],
+    "dmd_ldbd_read_kpairs": [_cs, _pLI, _i64, _i64, _vp, _vp],
+    "dmd_ldbd_read_P": [_cs, _pLI, _cs, _i, _i64, _i64, _vp, _vp],
+    "dmd_scatt_load_P": [_vp, _cs, _pLI, _cs, _i, _d, _i],
+    "dmd_scatt_create_from_ldbd": [_cs, _cs, _d, _d, _i, ctypes.POINTER(_vp)],
+    "dmd_scatt_upload_P_range": [_vp, _i64, _i64, _vp, _vp, _d, _i],
+    "dmd_restart_read": [_cs, _i64, _i, _vp, _pd],
+    "dmd_restart_write": [_cs, _i64, _i, _vp, _d],
 }
 _RESTYPE = {"dmd_scatt_last_error": ctypes.c_char_p}
 

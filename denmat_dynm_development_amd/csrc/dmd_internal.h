@@ -212,6 +212,11 @@ int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
 int contract_kind(const dmd_scatt *h);
 // nb = 4 tile path (dmd_tiles4.hip)
 int tiles4_place(dmd_scatt *h, const double2 *P1, const double2 *P2, int64_t n, int64_t first, double scale, int add);
+int place_P_dev(dmd_scatt *h, const double2 *a, const double2 *b, int64_t n, int64_t first, double scale, int add);
+std::string ldbd_path(const char *dir, const std::string &name);
+std::string ldbd_P_name(int which, const char *scatt, int impurity, int hole);
+int check_size(const std::string &path, int64_t expect);
+int pread_all(const std::string &path, int64_t off, int64_t n, void *buf, int nthreads);
 int tiles4_fetch(dmd_scatt *h, int64_t first, int64_t n, double2 *P1, double2 *P2);
 int tiles4_generate(dmd_scatt *h, const dmd_synth_params &p);
 int tiles4_relayout(dmd_scatt *h, int to_blocked);

@@ -1065,62 +1065,80 @@ extern "C" int dmd_scatt_destroy(dmd_scatt *h)
     return DMD_OK;
 }
 
-static int put_P(dmd_scatt *h, const double *P1, const double *P2, double scale, int add)
+// Places stored pairs [first, first+n) (file order) from device buffers a, b
+// (n x nb^4 complex128 each), scaled; add = 0 sets, 1 adds.  Stream-ordered.
+int dmd::place_P_dev(dmd_scatt *h, const double2 *a, const double2 *b, int64_t n, int64_t first, double scale, int add)
+{
+    if (n <= 0) return DMD_OK;
+    if (h->precision != DMD_PREC_FP64) return tiles4_place(h, a, b, n, first, scale, add);
+    const int64_t tot = n * 2 * (int64_t)h->nb4;
+    hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, first, h->nb4, a, b, h->d_inv, scale,
+                       add, h->soa, h->d_P);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+// P of stored pairs [first, first+count) from P1, P2 (count x nb^4 complex,
+// host or device memory).
+static int put_P(dmd_scatt *h, int64_t first, int64_t count, const double *P1, const double *P2, double scale,
+                 int add)
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
-    if (h->np == 0) { h->has_P = true; return DMD_OK; }
+    if (first < 0 || count < 0 || first + count > h->np) return fail(DMD_ERR_ARG, "pair range outside the stored pairs");
+    if (count == 0) {
+        if (h->np == 0) h->has_P = true;
+        return DMD_OK;
+    }
     if (!P1 || !P2) return fail(DMD_ERR_ARG, "P1/P2 null");
     const size_t per = (size_t)h->nb4;  // complex per P array per pair
     const bool dev1 = is_device_ptr(P1), dev2 = is_device_ptr(P2);
     // staging chunk: 64 Mi complex per array at most
     const int64_t chunk = std::max<int64_t>(1, (int64_t)((64ll << 20) / per));
     double2 *s1 = nullptr, *s2 = nullptr;
-    int rc;
+    int rc = DMD_OK;
     if (!dev1 || !dev2) {
-        int64_t c = std::min<int64_t>(chunk, h->np);
+        int64_t c = std::min<int64_t>(chunk, count);
         if ((rc = dalloc_t(&s1, (size_t)c * per, "P staging"))) return rc;
         if ((rc = dalloc_t(&s2, (size_t)c * per, "P staging"))) { dfree(s1); return rc; }
     }
-    for (int64_t f0 = 0; f0 < h->np; f0 += chunk) {
-        int64_t n = std::min<int64_t>(chunk, h->np - f0);
+    for (int64_t f0 = 0; f0 < count && rc == DMD_OK; f0 += chunk) {
+        int64_t n = std::min<int64_t>(chunk, count - f0);
         const double2 *a = (const double2 *)P1 + f0 * per, *b = (const double2 *)P2 + f0 * per;
+        hipError_t e = hipSuccess;
         if (!dev1) {
-            hipError_t e = hipMemcpyAsync(s1, a, n * per * 16, hipMemcpyHostToDevice, h->st);
-            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+            e = hipMemcpyAsync(s1, a, n * per * 16, hipMemcpyHostToDevice, h->st);
             a = s1;
         }
-        if (!dev2) {
-            hipError_t e = hipMemcpyAsync(s2, b, n * per * 16, hipMemcpyHostToDevice, h->st);
-            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
+        if (e == hipSuccess && !dev2) {
+            e = hipMemcpyAsync(s2, b, n * per * 16, hipMemcpyHostToDevice, h->st);
             b = s2;
         }
-        int64_t tot = n * 2 * (int64_t)per;
-        if (h->precision != DMD_PREC_FP64) {
-            if ((rc = tiles4_place(h, a, b, n, f0, scale, add))) { dfree(s1); dfree(s2); return rc; }
-        } else {
-            hipLaunchKernelGGL(k_place_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, f0, h->nb4, a, b, h->d_inv,
-                               scale, add, h->soa, h->d_P);
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
-        if (!dev1 || !dev2) {
-            e = hipStreamSynchronize(h->st);
-            if (e != hipSuccess) { dfree(s1); dfree(s2); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
-        }
+        if (e != hipSuccess) { rc = fail(DMD_ERR_HIP, hipGetErrorString(e)); break; }
+        rc = place_P_dev(h, a, b, n, first + f0, scale, add);
+        // pageable sources: the staging buffers are reused by the next chunk
+        if (rc == DMD_OK && (!dev1 || !dev2) && (e = hipStreamSynchronize(h->st)) != hipSuccess)
+            rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
     }
-    DMD_HIP(hipStreamSynchronize(h->st));
+    hipError_t e = hipStreamSynchronize(h->st);
     dfree(s1); dfree(s2);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
     h->has_P = true;
     return DMD_OK;
 }
 
 extern "C" int dmd_scatt_upload_P(dmd_scatt *h, const double *P1, const double *P2, double scale)
 {
-    return put_P(h, P1, P2, scale, 0);
+    return put_P(h, 0, h ? h->np : 0, P1, P2, scale, 0);
 }
 extern "C" int dmd_scatt_add_P(dmd_scatt *h, const double *P1, const double *P2, double scale)
 {
-    return put_P(h, P1, P2, scale, 1);
+    return put_P(h, 0, h ? h->np : 0, P1, P2, scale, 1);
+}
+extern "C" int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const double *P1,
+                                        const double *P2, double scale, int add)
+{
+    return put_P(h, first, count, P1, P2, scale, add ? 1 : 0);
 }
 
 extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2)

@@ -64,6 +64,25 @@ class ScatteringOperator:
              float(nk_full if nk_full is not None else nk), int(precision), ctypes.byref(h))
         self._h = h
 
+    @classmethod
+    def from_ldbd(cls, dirname, scatt="lindblad", scale_eph=1.0, scissor=0.0, precision=_lib.DMD_PREC_FP64):
+        """The handle of an ldbd_data folder, set up as the reference's
+        electronphonon sets itself up (ElectronPhonon.h:53-93): e-ph window of
+        e_dm, stored pairs, P x scale_eph streamed from the files, and the band
+        window when the density matrix has more bands."""
+        from . import ldbd
+        info = ldbd.read_info(dirname)
+        self = cls.__new__(cls)
+        self.nk, self.nb = int(info.nk), int(info.nb_eph)
+        self.nb_dm, self.b0 = int(info.nb_dm), int(info.b_bot_eph - info.b_bot_dm)
+        self.n_pairs = int(info.nkpair)
+        self.info = info
+        h = ctypes.c_void_p()
+        call("dmd_scatt_create_from_ldbd", str(dirname).encode(), scatt.encode(), float(scale_eph), float(scissor),
+             int(precision), ctypes.byref(h))
+        self._h = h
+        return self
+
     # -- lifetime ---------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
@@ -92,6 +111,22 @@ class ScatteringOperator:
         p1, k1 = _ptr(P1, np.complex128)
         p2, k2 = _ptr(P2, np.complex128)
         call("dmd_scatt_add_P", self._h, p1, p2, float(scale))
+
+    def upload_P_range(self, first, P1, P2, scale=1.0, add=False):
+        """P of stored pairs [first, first + len(P1)) in file order."""
+        p1, k1 = _ptr(P1, np.complex128)
+        p2, k2 = _ptr(P2, np.complex128)
+        count = (P1.numel() if _is_tensor(P1) else np.asarray(P1).size) // self.nb ** 4
+        call("dmd_scatt_upload_P_range", self._h, int(first), int(count), p1, p2, float(scale), int(bool(add)))
+
+    def load_P(self, dirname, info=None, scatt="lindblad", impurity=0, scale=1.0, add=False):
+        """P (or P += with add) from the ldbd_P files, streamed into HBM
+        (read_ldbd_eph, ElectronPhonon.cpp:209-233; impurity = iD + 1 for the
+        e-i files of read_ldbd_imp_P, ElectronImpurity.cpp:3-35)."""
+        from . import ldbd
+        info = ldbd.read_info(dirname) if info is None else info
+        call("dmd_scatt_load_P", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), int(impurity),
+             float(scale), int(bool(add)))
 
     def download_P(self, first=0, count=None):
         count = self.n_pairs - first if count is None else count

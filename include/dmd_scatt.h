@@ -132,9 +132,10 @@ int dmd_scatt_set_split(dmd_scatt *h, const double *f);
 int dmd_scatt_eval(dmd_scatt *h, double t, const double *rho, const double *one_minus_rho, double *ddmdt);
 /* evolve (evolve_eph.cpp:34-166) for B independent states: rho_B, ddmdt_B
  * are B x nk x nb^2 (one_minus_rho formed internally; no band window).  On
- * a DMD_PREC_BF16 nb = 4 handle the B states share one pass over P on the
- * matrix cores (drho as two BF16 terms, FP32 sums, FP64 assembly); on other
- * handles it evaluates the states one after another. */
+ * a DMD_PREC_C64 or DMD_PREC_BF16 nb = 4 handle the B states share one pass
+ * over P on the matrix cores (P and drho as two BF16 terms each, FP32 sums,
+ * FP64 equilibrium part and assembly); on other handles it evaluates the
+ * states one after another.  No reference counterpart for B > 1. */
 int dmd_scatt_eval_batched(dmd_scatt *h, double t, int B, const double *rho_B, double *ddmdt_B);
 /* The initializer's one-step form over all stored pairs
  * (lindbladInit.../ElectronPhonon.h:774-862 without the file reads): ddm +=
@@ -168,9 +169,9 @@ int dmd_scatt_measure(dmd_scatt *h, double t, const double *rho, double *value);
 int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps, int report_every,
                             double *rho_inout, double *obs_out);
 /* B trajectories (rho_inout: B x nk x nb_dm^2, obs_out: B x (n_steps /
- * report_every + 1)).  BF16 nb = 4 handles: one pass over P per RK stage for
- * all B (a captured HIP graph per step; no band window); other handles:
- * dmd_scatt_integrate_rk4 per trajectory. */
+ * report_every + 1)).  C64 / BF16 nb = 4 handles: one pass over P per RK
+ * stage for all B (a captured HIP graph per step; no band window); other
+ * handles: dmd_scatt_integrate_rk4 per trajectory. */
 int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
                                     int report_every, double *rho_inout, double *obs_out);
 
@@ -204,7 +205,7 @@ int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, double *las
  * kernel and of the whole eval, measured with HIP events on the handle's stream. */
 int dmd_scatt_time_eval(dmd_scatt *h, double t, int n, double *kernel_ms, double *eval_ms);
 /* Times n batched evaluations (X operand, matrix-core pass, assembly) of the
- * states of the last dmd_scatt_eval_batched (BF16 nb = 4 handles); no
+ * states of the last dmd_scatt_eval_batched (C64 / BF16 nb = 4 handles); no
  * reference counterpart. */
 int dmd_scatt_time_eval_batched(dmd_scatt *h, double t, int n, double *kernel_ms);
 
@@ -229,6 +230,80 @@ int dmd_synth_spin(const dmd_synth_params *p, double *s3);
  * ldbd_kpair_k1st/k2nd.bin and ldbd_P1/P2_lindblad.bin into dir
  * (layouts of lindbladInit.../electron_save.cpp:16-65 and ElectronPhonon.h:252-284). */
 int dmd_ldbd_write_synth(const char *dir, const dmd_synth_params *p, int nthreads);
+
+/* ---- ldbd_data reader ------------------------------------------------------------ */
+/* ldbd_size.dat as the reference reads it: electron::read_ldbd_size
+ * (src_v4.5.7/electron.cpp:100-132), electronphonon::get_brange / get_nkpair
+ * (ElectronPhonon.cpp:132-176), phonon::read_jdftx (phonon.h:49-79) and
+ * parameters::read_jdftx (parameters.cpp:7-23); the layouts are those of
+ * electron::saveSize (lindbladInit.../electron_save.cpp:16-50).  Band
+ * indices count from the first band of ldbd_ek.bin.  A valence-only file set
+ * (kind 1) names its e-ph window, pair count and pair/P files "_hole". */
+typedef struct {
+    int kind;                 /* first line: 0 conduction only, 1 valence only, 2 both */
+    int n_line2;              /* integers on line 2 (8 or 9; 11 for kind 1)              */
+    int nb, nv;               /* bands of ldbd_ek.bin; valence bands among them          */
+    int b_bot_dm, b_top_dm;   /* density-matrix bands [b_bot_dm, b_top_dm)               */
+    int b_bot_eph, b_top_eph; /* e-ph bands [b_bot_eph, b_top_eph) of this file set      */
+    int nb_wannier;
+    double nk_full;           /* k points of the full mesh (prefac = 2 pi / nk_full)     */
+    int64_t nk;               /* k points kept                                           */
+    int kmesh[3];
+    int64_t nkpair;           /* stored pairs (nkpair_hole for kind 1)                    */
+    int mode_start, mode_stop, n_modes;
+    double temperature;       /* Hartree                                                 */
+    double mu_min, mu_max, mu;
+    double degauss, ndegauss;
+    double e_bot_probe, e_top_probe, e_bot_dm, e_top_dm, e_bot_eph, e_top_eph;
+    double omega_max;         /* 0 when the file has no tenth line                        */
+} dmd_ldbd_info;
+int dmd_ldbd_read_info(const char *dir, dmd_ldbd_info *info);
+/* ldbd_ek.bin, nk x nb float64; scissor is added to bands nv.. (electron.cpp:172-180). */
+int dmd_ldbd_read_ek(const char *dir, const dmd_ldbd_info *info, double scissor, double *e);
+/* ldbd_kvec.bin, nk x 3 float64 (electron.cpp:134-147). */
+int dmd_ldbd_read_kvec(const char *dir, const dmd_ldbd_info *info, double *kvec);
+/* ldbd_smat.bin, nk x 3 x nb_dm^2 complex128 (electron.cpp:284-296). */
+int dmd_ldbd_read_smat(const char *dir, const dmd_ldbd_info *info, double *s);
+/* ldbd_imsig.bin, nk x nb_eph float64 (electron.cpp:181-198). */
+int dmd_ldbd_read_imsig(const char *dir, const dmd_ldbd_info *info, double *imsig);
+/* ldbd_R.dat: dim and the lattice vectors as rows of R[9] (lattice.h:48-56). */
+int dmd_ldbd_read_R(const char *dir, int *dim, double *R);
+/* Stored pairs [first, first+count) of ldbd_kpair_k1st/k2nd[_hole].bin
+ * (ElectronPhonon.cpp:182-204). */
+int dmd_ldbd_read_kpairs(const char *dir, const dmd_ldbd_info *info, int64_t first, int64_t count, uint64_t *k1st,
+                         uint64_t *k2nd);
+/* P1, P2 of pairs [first, first+count) from ldbd_P{1,2}_<scatt>[_D<impurity>][_hole].bin
+ * (scatt "lindblad" or "conventional"; impurity 0: e-ph, i > 0: defect i),
+ * unscaled, on the host (ElectronPhonon.cpp:209-233, ElectronImpurity.cpp:3-35). */
+int dmd_ldbd_read_P(const char *dir, const dmd_ldbd_info *info, const char *scatt, int impurity, int64_t first,
+                    int64_t count, double *P1, double *P2);
+/* read_ldbd_eph (ElectronPhonon.cpp:209-233; scale = scale_eph) and the e-i
+ * fold-in of read_ldbd_imp_P (ElectronImpurity.cpp:3-35, ElectronPhonon.cpp:
+ * 87-97; impurity = iD+1, scale = fraction * scale_ei, add = 1) straight into
+ * the handle: the files are streamed in 64 MiB chunks through pinned buffers,
+ * so no host copy of P is made. */
+int dmd_scatt_load_P(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, const char *scatt, int impurity,
+                     double scale, int add);
+/* electronphonon's set-up from an ldbd_data folder (ElectronPhonon.h:53-93):
+ * sizes, energies (e-ph window of e_dm, ElectronPhonon.h:79; scissor as
+ * electron.cpp:177-179), stored pairs and P x scale_eph, and the band window
+ * of evolve_driver when the e-ph bands are fewer than the density matrix's. */
+int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, double scale_eph, double scissor, int precision,
+                               dmd_scatt **out);
+/* P of stored pairs [first, first+count) (file order), set (add = 0) or added
+ * (add = 1), scaled; the range form of dmd_scatt_upload_P / dmd_scatt_add_P for
+ * callers that read P in pieces, as the reference's ranks read their share
+ * (ElectronPhonon.cpp:222-223).  Pairs never given keep P = 0. */
+int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const double *P1, const double *P2,
+                             double scale, int add);
+
+/* ---- restart files ---------------------------------------------------------------- */
+/* denmat_restart.bin (nk x nb^2 complex128) and time_restart.dat ("%14.7le")
+ * in the folder dir, as singdenmat_k::read_dm_restart / write_dm_tofile
+ * (DenMat.cpp:301-334) and parameters.cpp:299-310 use them; on read, dm or t
+ * may be NULL to skip that file. */
+int dmd_restart_read(const char *dir, int64_t nk, int nb, double *dm, double *t);
+int dmd_restart_write(const char *dir, int64_t nk, int nb, const double *dm, double t);
 
 #ifdef __cplusplus
 }

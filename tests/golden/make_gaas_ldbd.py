@@ -1,0 +1,30 @@
+"""Copies the small ldbd_data files of the reference's GaAs example run
+(Examples/RealTime_Example_GaAs_20230105/ld-dmd/t300/mult8_gauss0.005/
+dBz/only-eph/n2e16) into tests/golden/gaas_ldbd/, unchanged, as inputs of the
+reader tests: ldbd_size.dat, ldbd_R.dat, ldbd_ek.bin, ldbd_kvec.bin,
+ldbd_imsig.bin and restart/time_restart.dat (about 115 KB).  The run's P and
+k-pair files were never shipped with the example; ldbd_smat.bin (1.6 MB) and
+denmat_restart.bin stay out, their contents enter gaas_spin_restart.npz.
+
+Run in a container that holds the reference tree:
+  python tests/golden/make_gaas_ldbd.py /root/reference
+"""
+import os
+import shutil
+import sys
+
+RUN = "Examples/RealTime_Example_GaAs_20230105/ld-dmd/t300/mult8_gauss0.005/dBz/only-eph/n2e16"
+FILES = ["ldbd_data/ldbd_size.dat", "ldbd_data/ldbd_R.dat", "ldbd_data/ldbd_ek.bin", "ldbd_data/ldbd_kvec.bin",
+         "ldbd_data/ldbd_imsig.bin", "restart/time_restart.dat"]
+
+
+def main(ref_root):
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gaas_ldbd")
+    os.makedirs(out, exist_ok=True)
+    for f in FILES:
+        shutil.copyfile(os.path.join(ref_root, RUN, f), os.path.join(out, os.path.basename(f)))
+        print(f, os.path.getsize(os.path.join(out, os.path.basename(f))))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

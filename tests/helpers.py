@@ -96,3 +96,48 @@ def quantise_P(P1, P2, precision):
     q = bf16(both.real) + 1j * bf16(both.imag)
     n = P1.shape[1]
     return q[:, :n], q[:, n:]
+
+
+def write_ldbd_folder(d, kind, nb, nv, dm, eph, e, k1, k2, P1=None, P2=None, nk_full=None, kmesh=(0, 0, 0),
+                      modes=(0, 6, 0, 0), T=9.5004e-4, mu=(0.05, 0.05, 0.05), degauss=(1.8e-4, 1000),
+                      ewin=(-0.1, 0.1, -0.05, 0.08, 0.0, 0.06), omega_max=1.9e-3, scatt="lindblad", extra_P=()):
+    """An ldbd_data folder in the layouts of the initializer's saveSize / saveEk
+    / P writers (lindbladInit.../electron_save.cpp:16-65): kind 0 conduction
+    only, 1 valence only (hole files and fields), 2 both.  dm and eph are
+    [bottom, top) band ranges of the ldbd_ek.bin bands; extra_P holds
+    (impurity index, P1, P2) written as the _D<i> files."""
+    import os
+    os.makedirs(d, exist_ok=True)
+    nk = e.shape[0]
+    nk_full = float(nk if nk_full is None else nk_full)
+    np_ = len(k1)
+    first = ["There are scatterings only for conduction electrons", "There are scatterings only for valence electrons",
+             "There are scatterings for both valence and conduction electrons"][kind]
+    if kind == 1:
+        line2 = "%d %d %d %d %d %d %d %d %d %d %d # nb nv bBot_dm bTop_dm bBot_eph_elec bTop_eph_elec bBot_eph_hole " \
+                "bTop_eph_hole nb_wannier bBot_probe band_dft_skipped" % (nb, nv, dm[0], dm[1], 0, 0, eph[0], eph[1],
+                                                                           nb + 4, 0, 0)
+        line4 = "%d %d # nkpair_elec nkpair_hole" % (0, np_)
+    else:
+        line2 = "%d %d %d %d %d %d %d %d %d # nb nv bBot_dm bTop_dm bBot_eph bTop_eph nb_wannier bBot_probe " \
+                "band_dft_skipped" % (nb, nv, dm[0], dm[1], eph[0], eph[1], nb + 4, 0, 0)
+        line4 = "%d # nkpair" % np_
+    lines = [first, line2, "%21.14le %d %d %d %d # nk_full nk kmesh" % (nk_full, nk, *kmesh), line4,
+             "%d %d %d %d# modeStart modeStp modeSkipStart modeSkipStop" % tuple(modes),
+             "%21.14le # T" % T, "%21.14le %21.14le %21.14le # muMin, muMax mu (given carrier density)" % tuple(mu),
+             "%g %g # degauss" % tuple(degauss),
+             "%14.7le %14.7le %14.7le %14.7le %14.7le %14.7le # EBot_probe, ETop_probe, EBot_dm, ETop_dm, EBot_eph, "
+             "ETop_eph" % tuple(ewin)]
+    text = "\n".join(lines) + "\n" + "%14.7le # omegaMax" % omega_max
+    with open(os.path.join(d, "ldbd_size.dat"), "w") as f:
+        f.write(text)
+    np.ascontiguousarray(e, np.float64).tofile(os.path.join(d, "ldbd_ek.bin"))
+    sfx = "_hole" if kind == 1 else ""
+    np.ascontiguousarray(k1, np.uint64).tofile(os.path.join(d, f"ldbd_kpair_k1st{sfx}.bin"))
+    np.ascontiguousarray(k2, np.uint64).tofile(os.path.join(d, f"ldbd_kpair_k2nd{sfx}.bin"))
+    if P1 is not None:
+        np.ascontiguousarray(P1, np.complex128).tofile(os.path.join(d, f"ldbd_P1_{scatt}{sfx}.bin"))
+        np.ascontiguousarray(P2, np.complex128).tofile(os.path.join(d, f"ldbd_P2_{scatt}{sfx}.bin"))
+    for i, Q1, Q2 in extra_P:
+        np.ascontiguousarray(Q1, np.complex128).tofile(os.path.join(d, f"ldbd_P1_{scatt}_D{i}{sfx}.bin"))
+        np.ascontiguousarray(Q2, np.complex128).tofile(os.path.join(d, f"ldbd_P2_{scatt}_D{i}{sfx}.bin"))
