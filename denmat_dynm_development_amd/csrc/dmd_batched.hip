@@ -216,7 +216,9 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
 
 // ddm of every trajectory: C = (1-rho)(F1_eq + F1) - (F2_eq - G) rho, ddm =
 // prefac/2 (C + C^+), rotated back, minus ddmdt_eq.  Clears F.  One thread
-// per (b, k, i, j); rows of F are read by the 16 threads of (b, k).
+// per (k, b) with b fastest, so each of the 64 row sums of k is one coalesced
+// row across the trajectories; the thread owns its 64 sums, reads and clears
+// them, and writes its 16 outputs.
 __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, float *__restrict__ F,
                                                     const double *__restrict__ feq, const double2 *__restrict__ rhoS,
                                                     const double *__restrict__ e, const DClock *__restrict__ clk,
@@ -224,62 +226,74 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
                                                     const double2 *__restrict__ eq, double2 *__restrict__ out)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t bk = idx >> 4;
-    const int b = (int)(bk / nk);
-    const int64_t k = bk - (int64_t)b * nk;
-    const int l = (int)(idx & 15), i = l >> 2, j = l & 3;
-    const bool ok = b < B;
-    double2 d = make_double2(0, 0);
-    if (ok) {
+    const int64_t k = idx / Bp;
+    const int b = (int)(idx - k * Bp);
+    if (k >= nk) return;
+    float *Fk = F + k * 64 * Bp + b;
+    if (b < B) {
+        const int64_t bk = (int64_t)b * nk + k;
+        const double *fe = feq + k * 64;
         double2 rk[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) rk[q] = rhoS[bk * 16 + q];
-        auto f1 = [&](int m) {  // F1[m] = F1_eq[m] + F[k][m] (re), F[k][16 + m] (im)
-            return make_double2(feq[k * 64 + 2 * m] + F[(k * 64 + m) * Bp + b],
-                                feq[k * 64 + 2 * m + 1] + F[(k * 64 + 16 + m) * Bp + b]);
-        };
-        auto f2 = [&](int m) {  // F2[m] = F2_eq[m] - G[m]
-            return make_double2(feq[k * 64 + 32 + 2 * m] - F[(k * 64 + 32 + m) * Bp + b],
-                                feq[k * 64 + 32 + 2 * m + 1] - F[(k * 64 + 48 + m) * Bp + b]);
-        };
-        auto cel = [&](int ii, int jj) {
-            double2 x = make_double2(0, 0);
-            for (int m = 0; m < 4; m++) {
-                double2 r1;  // (1 - rho) upper-triangle Hermitian, as zhemm reads it
-                if (ii < m) r1 = make_double2(-rk[ii * 4 + m].x, -rk[ii * 4 + m].y);
-                else if (ii > m) r1 = make_double2(-rk[m * 4 + ii].x, rk[m * 4 + ii].y);
-                else r1 = make_double2(1.0 - rk[ii * 5].x, 0.0);
-                cfma(x, r1, f1(4 * m + jj));
-            }
-            for (int m = 0; m < 4; m++) {
-                double2 rr;
-                if (m < jj) rr = rk[m * 4 + jj];
-                else if (m > jj) rr = cconj(rk[jj * 4 + m]);
-                else rr = make_double2(rk[m * 5].x, 0.0);
-                const double2 g = f2(4 * ii + m);
-                cfma(x, make_double2(-g.x, -g.y), rr);
-            }
-            return x;
-        };
-        d = cadd(cel(i, j), cconj(cel(j, i)));
-    }
-    __syncthreads();  // all rows of F read before they are cleared
-    if (!ok) return;
-    if (b == 0 || true) {
-        // thread l of (b, k) clears rows 4l .. 4l+3 of F for trajectory b
+        // C = (1 - rho) F1 - F2 rho; 1 - rho and rho read as Hermitian from
+        // their upper triangles, as zhemm does (evolve_eph.cpp:142-143)
+        double2 C[16];
 #pragma unroll
-        for (int q = 0; q < 4; q++) F[(k * 64 + 4 * l + q) * Bp + b] = 0.0f;
+        for (int q = 0; q < 16; q++) C[q] = make_double2(0, 0);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const int r = 4 * m + jj;  // F1[m][jj] = F1_eq + F
+                const double2 f1 = make_double2(fe[2 * r] + (double)Fk[r * Bp], fe[2 * r + 1] + (double)Fk[(16 + r) * Bp]);
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++) {
+                    double2 r1;
+                    if (ii < m) r1 = make_double2(-rk[ii * 4 + m].x, -rk[ii * 4 + m].y);
+                    else if (ii > m) r1 = make_double2(-rk[m * 4 + ii].x, rk[m * 4 + ii].y);
+                    else r1 = make_double2(1.0 - rk[ii * 5].x, 0.0);
+                    cfma(C[ii * 4 + jj], r1, f1);
+                }
+            }
+        }
+#pragma unroll
+        for (int ii = 0; ii < 4; ii++) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int r = 4 * ii + m;  // F2[ii][m] = F2_eq - G
+                const double2 g = make_double2(fe[32 + 2 * r] - (double)Fk[(32 + r) * Bp],
+                                               fe[32 + 2 * r + 1] - (double)Fk[(48 + r) * Bp]);
+                const double2 ng = make_double2(-g.x, -g.y);
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    double2 rr;
+                    if (m < jj) rr = rk[m * 4 + jj];
+                    else if (m > jj) rr = cconj(rk[jj * 4 + m]);
+                    else rr = make_double2(rk[m * 5].x, 0.0);
+                    cfma(C[ii * 4 + jj], ng, rr);
+                }
+            }
+        }
+        const double t = clock_t_at(clk, a);
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+            const int i = l >> 2, j = l & 3;
+            double2 d = cadd(C[l], cconj(C[j * 4 + i]));
+            d = make_double2(d.x * (prefac * 0.5), d.y * (prefac * 0.5));
+            const double arg = (e[k * 4 + i] - e[k * 4 + j]) * t;
+            if (expt && i != j) d = cmul(d, ccis(arg));
+            if (eq) {
+                double2 q = eq[k * 16 + l];
+                if (i != j && !schrodinger) q = cmul(q, ccis(arg));
+                d = csub(d, q);
+            }
+            out[bk * 16 + l] = d;
+        }
     }
-    d = make_double2(d.x * (prefac * 0.5), d.y * (prefac * 0.5));
-    const double t = clock_t_at(clk, a);
-    const double arg = (e[k * 4 + i] - e[k * 4 + j]) * t;
-    if (expt && i != j) d = cmul(d, ccis(arg));
-    if (eq) {
-        double2 q = eq[k * 16 + l];
-        if (i != j && !schrodinger) q = cmul(q, ccis(arg));
-        d = csub(d, q);
-    }
-    out[bk * 16 + l] = d;
+    // this thread's 64 row sums are read; clear them for the next pass
+#pragma unroll
+    for (int r = 0; r < 64; r++) Fk[r * Bp] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -455,7 +469,7 @@ int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *o
     hipLaunchKernelGGL(k_prepare_x, dim3(nblk(nx, 256)), dim3(256), 0, h->st, h->nk, B, Bp, rho, h->d_e, h->d_clk, a,
                        h->expt, h->d_fsplit, h->d_rhoSB, (uint16_t *)h->d_xb);
     if (h->nseg > 0) launch_batched_core(h, Bp, h->batch_nblk);
-    const int64_t nout = (int64_t)B * h->nk * 16;
+    const int64_t nout = (int64_t)Bp * h->nk;
     hipLaunchKernelGGL(k_assemble_b, dim3(nblk(nout, 256)), dim3(256), 0, h->st, h->nk, B, Bp, h->d_Fb, h->d_feq,
                        h->d_rhoSB, h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac,
                        h->has_eq ? h->d_eq : nullptr, out);
