@@ -300,6 +300,13 @@ extern "C" int dmd_ldbd_write_synth(const char *dir, const dmd_synth_params *p, 
                  emin, emax, emin, emax, emin, emax);
     std::fprintf(fp, "%14.7le # omegaMax", wmax);
     std::fclose(fp);
+    // ldbd_R.dat (lattice.h:48-56): a GaAs-like fcc cell, a = 10.6829 bohr
+    fp = std::fopen((d + "ldbd_R.dat").c_str(), "w");
+    if (!fp) return fail(DMD_ERR_IO, "cannot open " + d + "ldbd_R.dat");
+    const double h = 5.34145;
+    std::fprintf(fp, "3\n%15.7le%15.7le%15.7le\n%15.7le%15.7le%15.7le\n%15.7le%15.7le%15.7le\n", 0.0, h, h, h, 0.0, h, h,
+                 h, 0.0);
+    std::fclose(fp);
     if ((rc = write_bin(d + "ldbd_ek.bin", e.data(), e.size() * 8))) return rc;
     if ((rc = write_bin(d + "ldbd_kvec.bin", kv.data(), kv.size() * 8))) return rc;
     if ((rc = write_bin(d + "ldbd_smat.bin", s3.data(), s3.size() * 8))) return rc;

@@ -1,0 +1,414 @@
+// dmd_run.cpp -- the param.in-driven real-time run of the reference
+// (src_v4.5.7/main.cpp:39-97 with dm_dynamics, DensityMatrix_Dynamics.h:31-77,
+// 177-345) for the Lindblad scattering operator of libdmd_scatt:
+//
+//   ldbd_data/ + param.in  ->  s{x,y,z}_{elec,hole}_tot.out, restart/
+//
+// run in the folder that holds param.in and ldbd_data/, as the reference is.
+// What it reads: lattice (ldbd_R.dat), band energies and window, T and mu
+// (ldbd_size.dat, or mu from carrier_density as electron::set_mu_and_n),
+// spin matrices, stored pairs and P (x scale_eph), ab initio e-i P of every
+// impurity_density<i> with impMode ab_neutral (x |n_i cell_size| x scale_ei);
+// the initial state is the Bpert first-order state (electron::
+// compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
+// equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
+// B (Bx, By, Bz) adds the coherent H_BS = B . s.  The evolution is the
+// on-device fixed-step RK4 with step min(tstep, ode_hmax), reports every
+// freq_measure steps of tstep.  Features whose physics this operator does not
+// carry (e-e, model e-i, linearized or sparse P, pump pulses, phenomenological
+// relaxation, orbital or g-factor terms) stop the run with a message rather
+// than being ignored.
+//
+//   dmd_run [--dry-run] [folder]
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <sys/stat.h>
+
+#include "dmd_scatt.h"
+#include "electronphonon_mi355x.h"
+#include "param_in.h"
+#include "units.h"
+
+namespace {
+
+using namespace dmdrun;
+
+struct cplx {
+    double x, y;
+};
+
+struct Settings {
+    bool restart = false;
+    std::string scatt = "lindblad", picture = "interaction";
+    bool ddmdteq = false, expt = true;
+    double scale_eph = 1, scale_ei = 1;
+    double t0 = 0, tend = 0, tstep = 0, hmax = 0;
+    int freq_measure = 1;
+    double mu = 0, carrier_density = 0, scissor = 0, temperature = 0;
+    double Bpert[3] = {0, 0, 0}, B[3] = {0, 0, 0};
+    std::vector<double> ni;  // impurity densities in bohr^-dim, ab_neutral
+};
+
+// electron.h:119-124
+double fermi(double T, double mu, double e)
+{
+    const double x = (e - mu) / T;
+    if (x < -46) return 1;
+    if (x > 46) return 0;
+    return 1. / (std::exp(x) + 1);
+}
+
+void refuse(const ParamIn &p, const std::string &key, double off_value, const std::string &why)
+{
+    if (p.has(key) && p.get(key, off_value) != off_value)
+        throw std::runtime_error(key + " = " + p.str(key, "") + ": " + why + " is not carried by this operator");
+}
+
+Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
+{
+    using namespace units;
+    Settings s;
+    const std::string code = p.str("code", "jdftx");
+    if (code != "jdftx") throw std::runtime_error("code = " + code + ": only ldbd_data input (code = jdftx) is read");
+    if (p.str("material_model", "none") != "none") throw std::runtime_error("material_model: model materials are not read");
+    s.restart = p.get("restart", 0) != 0;
+    if (p.get("alg_scatt_enable", 1) == 0 || p.get("alg_eph_enable", 1) == 0)
+        throw std::runtime_error("alg_scatt_enable / alg_eph_enable = 0: this runner drives the scattering operator");
+    const double scale_scatt = p.get("scale_scatt", 1.);
+    s.scale_eph = p.get("scale_eph", scale_scatt);
+    s.scale_ei = p.get("scale_ei", scale_scatt);
+    s.ddmdteq = p.get("alg_ddmdteq", 0) != 0;
+    s.expt = p.get("alg_expt", 1) != 0;
+    s.scatt = p.str("alg_scatt", "lindblad");
+    s.picture = p.str("alg_picture", "interaction");
+    if (s.picture != "interaction" && s.picture != "schrodinger")
+        throw std::runtime_error("alg_picture must be interaction or schrodinger");
+    refuse(p, "alg_summode", 1, "the A-matrix (non-summode) form");
+    refuse(p, "alg_semiclassical", 0, "the semiclassical (diagonal) mode");
+    refuse(p, "alg_linearize", 0, "the linearized operator");
+    refuse(p, "alg_linearize_dPee", 0, "the linearized e-e operator");
+    refuse(p, "alg_Pin_is_sparse", 0, "sparse P input");
+    refuse(p, "alg_sparseP", 0, "sparse P");
+    refuse(p, "alg_phenom_relax", 0, "phenomenological relaxation");
+    refuse(p, "alg_only_eimp", 0, "alg_only_eimp");
+    refuse(p, "alg_only_ee", 0, "alg_only_ee");
+    refuse(p, "alg_only_intravalley", 0, "valley filtering");
+    refuse(p, "alg_only_intervalley", 0, "valley filtering");
+    refuse(p, "alg_read_Bso", 0, "the internal spin-orbit field");
+    refuse(p, "needL", 0, "the orbital angular momentum term");
+    refuse(p, "gfac_normal_dist", 0, "k-resolved g factors");
+    refuse(p, "scale_Ez", 0, "the electric-field Hamiltonian");
+    refuse(p, "compute_tau_only", 0, "compute_tau_only");
+    refuse(p, "carrier_density_means_excess_density", 0, "excess-density mu");
+    if (p.str("eeMode", "none") != "none") throw std::runtime_error("eeMode: e-e scattering is not carried by this operator");
+    const double laserA = p.get("laserA", p.get("pumpA0", 0.));
+    if (laserA != 0) throw std::runtime_error("laserA / pumpA0: pump pulses are not carried by this operator");
+    // alg_eph_sepr_eh / need_elec / need_hole choose which file set the
+    // initializer wrote (ElectronPhonon.cpp:132-158); the files say it too
+    const bool need_hole = p.get("alg_eph_need_hole", 0) != 0, need_elec = p.get("alg_eph_need_elec", 1) != 0;
+    p.get("alg_eph_sepr_eh", 1);
+    if (need_hole && need_elec) throw std::runtime_error("alg_eph_need_elec and alg_eph_need_hole: one carrier set per run");
+    if (need_hole != (info.kind == 1))
+        throw std::runtime_error("alg_eph_need_hole does not match the ldbd_data file set (first line of ldbd_size.dat)");
+    const std::string ode = p.str("alg_ode_method", "rkf45");
+    if (ode != "rkf45" && ode != "euler") throw std::runtime_error("alg_ode_method must be rkf45 or euler");
+    s.temperature = info.temperature;  // code = jdftx: T of ldbd_size.dat (parameters.cpp:7-23)
+    s.scissor = p.get("scissor", 0, eV);
+    s.mu = p.get("mu", info.mu / eV, eV);
+    s.carrier_density = p.get("carrier_density", 0, std::pow(bohr2cm, dim));
+    for (int i = 0; i < 3; i++) {
+        static const char *bp[3] = {"Bxpert", "Bypert", "Bzpert"}, *bb[3] = {"Bx", "By", "Bz"};
+        s.Bpert[i] = p.get(bp[i], 0., Tesla2au);
+        s.B[i] = p.get(bb[i], 0., Tesla2au);
+    }
+    for (int iD = 0;; iD++) {  // parameters.cpp:210-224
+        const std::string sfx = iD == 0 ? "" : std::to_string(iD + 1);
+        const double n = p.get("impurity_density" + sfx, 0, std::pow(bohr2cm, dim));
+        if (n == 0) break;
+        const std::string mode = p.str("impMode" + sfx, "model_ionized");
+        if (mode != "ab_neutral")
+            throw std::runtime_error("impMode" + sfx + " = " + mode + ": only ab initio e-i P (ab_neutral) is read");
+        s.ni.push_back(n);
+    }
+    s.t0 = p.get("t0", 0., fs());
+    s.tend = p.get("tend", 0., fs());
+    s.tstep = p.get("tstep", 1., fs());
+    s.hmax = p.get("ode_hmax", s.tstep / fs(), fs());
+    s.freq_measure = (int)p.get("freq_measure", 1);
+    if (s.tstep <= 0 || s.hmax <= 0 || s.freq_measure < 1) throw std::runtime_error("tstep, ode_hmax > 0, freq_measure >= 1");
+    return s;
+}
+
+// electron::find_mu by bisection (electron.cpp:537-576) on the carriers of
+// the conduction (or, for a negative density, valence) bands: carrier_bvk =
+// density * nk_full * cell_size, compute_nfree (electron.cpp:582-600).
+double find_mu_bisection(double carrier_bvk, double T, double emin, double emax, const std::vector<double> &e,
+                         long nk, int nb, int nv)
+{
+    const bool hole = carrier_bvk < 0;
+    double lo = emin, hi = emax, mu = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; it++) {
+        mu = 0.5 * (lo + hi);
+        double n = 0;
+        for (long k = 0; k < nk; k++)
+            for (int i = hole ? 0 : nv; i < (hole ? nv : nb); i++) {
+                const double f = fermi(T, mu, e[(size_t)k * nb + i]);
+                n += hole ? f - 1. : f;
+            }
+        const double err = n - carrier_bvk;
+        if (std::fabs(err) < 1e-12) return mu;
+        if (n > carrier_bvk) hi = mu;
+        else lo = mu;
+    }
+    return mu;
+}
+
+// carrier-conserving chemical potential of bands [b0, b1) of the density
+// matrix for singdenmat_k::set_dm_eq (DenMat.cpp:97-149): the root of
+// sum f(e; mu) = sum Re dm_ii (holes: sum (f - 1) = sum (Re dm_ii - 1)).
+double mu_conserving(const std::vector<cplx> &dm, const std::vector<double> &edm, long nk, int nbd, int b0, int b1,
+                     bool hole, double T)
+{
+    double target = 0, emin = 1e300, emax = -1e300;
+    for (long k = 0; k < nk; k++)
+        for (int i = b0; i < b1; i++) {
+            target += hole ? dm[(size_t)k * nbd * nbd + i * nbd + i].x - 1. : dm[(size_t)k * nbd * nbd + i * nbd + i].x;
+            emin = std::min(emin, edm[(size_t)k * nbd + i]);
+            emax = std::max(emax, edm[(size_t)k * nbd + i]);
+        }
+    double lo = emin - 60 * T, hi = emax + 60 * T;
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); it++) {
+        const double mu = 0.5 * (lo + hi);
+        double n = 0;
+        for (long k = 0; k < nk; k++)
+            for (int i = b0; i < b1; i++) {
+                const double f = fermi(T, mu, edm[(size_t)k * nbd + i]);
+                n += hole ? f - 1. : f;
+            }
+        if (n > target) hi = mu;
+        else lo = mu;
+    }
+    return 0.5 * (lo + hi);
+}
+
+struct Report {
+    std::string name;  // "sx_elec" ...
+    int comp, i0, i1;
+    FILE *fp = nullptr;
+};
+
+void mkdir_p(const std::string &d)
+{
+    if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) throw std::runtime_error("cannot create " + d);
+}
+
+int run(const std::string &folder, bool dry)
+{
+    using namespace units;
+    const std::string base = folder.empty() ? std::string(".") : folder;
+    const std::string ldbd = base + "/ldbd_data";
+    const ParamIn p = ParamIn::read(base + "/param.in");
+    dmd_ldbd_info info;
+    check(dmd_ldbd_read_info(ldbd.c_str(), &info), "ldbd_size.dat");
+    int dim;
+    double R[9];
+    check(dmd_ldbd_read_R(ldbd.c_str(), &dim, R), "ldbd_R.dat");
+    dim = (int)p.get("dim", dim);
+    // lattice.h:24-37
+    const double volume = std::fabs(R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                                    R[2] * (R[3] * R[7] - R[4] * R[6]));
+    double cell = volume;
+    if (dim == 2) cell = volume / R[8];
+    if (dim == 1) cell = R[8];
+    if (dim == 0) cell = 1;
+    const Settings s = read_settings(p, info, dim);
+    const long nk = (long)info.nk;
+    const int nbf = info.nb, nbd = info.b_top_dm - info.b_bot_dm, nvd = std::max(info.nv - info.b_bot_dm, 0);
+    const double T = s.temperature;
+    std::vector<double> e((size_t)nk * nbf);
+    check(dmd_ldbd_read_ek(ldbd.c_str(), &info, s.scissor, e.data()), "ldbd_ek.bin");
+    std::vector<double> edm((size_t)nk * nbd);
+    for (long k = 0; k < nk; k++)
+        for (int b = 0; b < nbd; b++) edm[(size_t)k * nbd + b] = e[(size_t)k * nbf + info.b_bot_dm + b];
+    // electron::set_mu_and_n (electron.h:141-160) with emin/emax of electron.h:94-105
+    double mu = s.mu;
+    if (s.carrier_density != 0) {
+        double evmax = -1e300, emin = 1e300, emax = -1e300, ecmin = 1e300;
+        for (long k = 0; k < nk; k++)
+            for (int b = 0; b < nbd; b++) {
+                const double x = edm[(size_t)k * nbd + b];
+                if (b >= nvd) { emax = std::max(emax, x); ecmin = std::min(ecmin, x); }
+                else { evmax = std::max(evmax, x); emin = std::min(emin, x); }
+            }
+        if (nbd == nvd) emax = evmax + 1;
+        if (nvd == 0) emin = ecmin - 1;
+        mu = find_mu_bisection(s.carrier_density * info.nk_full * cell, T, emin, emax, e, nk, nbf, info.nv);
+    }
+    std::vector<double> fdm((size_t)nk * nbd);
+    for (size_t i = 0; i < fdm.size(); i++) fdm[i] = fermi(T, mu, edm[i]);
+    const int n_sub = (int)std::ceil(s.tstep / s.hmax - 1e-9);
+    const double h_rk = s.tstep / n_sub;
+    const double prefac_obs = 1. / (info.nk_full * cell * std::pow(bohr2cm, dim));  // observable.h:53-55
+    std::printf("dmd_run: nk = %ld nb_dm = %d (nv_dm = %d) e-ph bands [%d, %d) of the density matrix, %ld stored pairs\n",
+                nk, nbd, nvd, info.b_bot_eph - info.b_bot_dm, info.b_top_eph - info.b_bot_dm, (long)info.nkpair);
+    std::printf("T = %.10le mu = %.10le cell_size = %.10le dim = %d\n", T, mu, cell, dim);
+    std::printf("t0 = %.10le tend = %.10le tstep = %.10le (RK4 step %.10le x %d) freq_measure = %d\n", s.t0, s.tend,
+                s.tstep, h_rk, n_sub, s.freq_measure);
+    std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_expt = %d alg_picture = %s alg_scatt = %s\n",
+                s.scale_eph, s.scale_ei, (int)s.ddmdteq, (int)s.expt, s.picture.c_str(), s.scatt.c_str());
+    std::printf("Bpert = %.10le %.10le %.10le B = %.10le %.10le %.10le impurities = %zu restart = %d\n", s.Bpert[0],
+                s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
+    for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
+    if (dry) return 0;
+
+    // spin matrices (ldbd_smat.bin) and the initial state
+    std::vector<cplx> smat((size_t)nk * 3 * nbd * nbd);
+    check(dmd_ldbd_read_smat(ldbd.c_str(), &info, (double *)smat.data()), "ldbd_smat.bin");
+    const size_t nbd2 = (size_t)nbd * nbd;
+    std::vector<cplx> dm((size_t)nk * nbd2);
+    double t = s.t0;
+    const bool interaction = s.picture == "interaction";
+    if (s.restart) {
+        check(dmd_restart_read((base + "/restart").c_str(), nk, nbd, (double *)dm.data(), &t), "restart files");
+    } else {
+        // electron::compute_dm_Bpert_1st (electron.cpp:62-98), then init_dm's
+        // Hermitian average (DenMat.cpp:64-79)
+        std::vector<cplx> d0((size_t)nk * nbd2);
+        for (long k = 0; k < nk; k++) {
+            const cplx *sk = &smat[(size_t)k * 3 * nbd2];
+            for (int i = 0; i < nbd; i++)
+                for (int j = i; j < nbd; j++) {
+                    cplx H1 = {0, 0};
+                    for (int c = 0; c < 3; c++) {
+                        H1.x += s.Bpert[c] * sk[c * nbd2 + i * nbd + j].x;
+                        H1.y += s.Bpert[c] * sk[c * nbd2 + i * nbd + j].y;
+                    }
+                    const double ei = edm[(size_t)k * nbd + i], ej = edm[(size_t)k * nbd + j];
+                    const double fi = fdm[(size_t)k * nbd + i], fj = fdm[(size_t)k * nbd + j];
+                    double dfde;
+                    if (std::fabs(ei - ej) < 1e-8) {
+                        const double favg = 0.5 * (fi + fj);
+                        dfde = favg * (favg - 1.) / T;
+                    } else {
+                        dfde = (fi - fj) / (ei - ej);
+                    }
+                    cplx v = {dfde * H1.x, dfde * H1.y};
+                    if (i != j && interaction) {
+                        const double a = (ei - ej) * s.t0, c = std::cos(a), sn = std::sin(a);
+                        v = {v.x * c - v.y * sn, v.x * sn + v.y * c};
+                    }
+                    d0[(size_t)k * nbd2 + i * nbd + j] = v;
+                    if (i < j) d0[(size_t)k * nbd2 + j * nbd + i] = {v.x, -v.y};
+                }
+            for (int i = 0; i < nbd; i++) d0[(size_t)k * nbd2 + i * nbd + i].x += fdm[(size_t)k * nbd + i];
+        }
+        for (long k = 0; k < nk; k++)
+            for (int i = 0; i < nbd; i++)
+                for (int j = 0; j < nbd; j++) {
+                    const cplx a = d0[(size_t)k * nbd2 + i * nbd + j], b = d0[(size_t)k * nbd2 + j * nbd + i];
+                    dm[(size_t)k * nbd2 + i * nbd + j] = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+                }
+    }
+
+    // the scattering operator: e-ph P, ab initio e-i P, flags, ddmdt_eq, H_BS
+    electronphonon_mi355x<cplx> eph(ldbd, s.scatt, s.scale_eph, s.scissor);
+    for (size_t iD = 0; iD < s.ni.size(); iD++) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+    eph.set_algorithm(s.expt, !interaction);
+    if (s.ddmdteq) {
+        std::vector<double> feq((size_t)nk * nbd);
+        const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
+        const double muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;
+        for (long k = 0; k < nk; k++)
+            for (int i = 0; i < nbd; i++) feq[(size_t)k * nbd + i] = fermi(T, i >= nvd ? mue : muh, edm[(size_t)k * nbd + i]);
+        double *rows[1] = {feq.data()};
+        eph.compute_ddmdt_eq(rows);
+        std::printf("ddmdt_eq at mu_e = %.12le mu_h = %.12le\n", mue, muh);
+    }
+    dmd_scatt *h = eph.handle();
+    if (std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10) {  // electron.cpp:19-42
+        std::vector<cplx> H((size_t)nk * nbd2);
+        for (long k = 0; k < nk; k++)
+            for (size_t q = 0; q < nbd2; q++) {
+                cplx v = {0, 0};
+                for (int c = 0; c < 3; c++) {
+                    v.x += s.B[c] * smat[((size_t)k * 3 + c) * nbd2 + q].x;
+                    v.y += s.B[c] * smat[((size_t)k * 3 + c) * nbd2 + q].y;
+                }
+                H[(size_t)k * nbd2 + q] = v;
+            }
+        check(dmd_scatt_set_coherent(h, (const double *)H.data()), "coherent H_BS");
+    }
+
+    // reports (DensityMatrix_Dynamics.h:322-345, observable.h:131-403)
+    std::vector<Report> reps;
+    const char *cn[3] = {"sx", "sy", "sz"};
+    for (int c = 0; c < 3; c++) {
+        if (nbd > nvd) reps.push_back({std::string(cn[c]) + "_elec", c, nvd, nbd});
+        if (nvd > 0) reps.push_back({std::string(cn[c]) + "_hole", c, 0, nvd});
+    }
+    std::vector<cplx> sc((size_t)nk * nbd2);
+    for (auto &r : reps) {
+        const std::string fn = base + "/" + r.name + "_tot.out";
+        struct stat st;
+        const bool exists = stat(fn.c_str(), &st) == 0;
+        r.fp = std::fopen(fn.c_str(), "a");
+        if (!r.fp) throw std::runtime_error("cannot open " + fn);
+        if (!exists) std::fprintf(r.fp, "#time(au), s(t), s(t) without exp(iwt)\n");
+    }
+    mkdir_p(base + "/restart");
+    auto report = [&](int it) {
+        if (it % s.freq_measure != 0) return;
+        if (it > 0) check(dmd_restart_write((base + "/restart").c_str(), nk, nbd, (const double *)dm.data(), t), "restart");
+        for (auto &r : reps) {
+            for (long k = 0; k < nk; k++)
+                std::memcpy(&sc[(size_t)k * nbd2], &smat[((size_t)k * 3 + r.comp) * nbd2], nbd2 * sizeof(cplx));
+            check(dmd_scatt_set_observable(h, (const double *)sc.data(), fdm.data(), r.i0, r.i1, prefac_obs), "observable");
+            double v = 0, amp = 0;
+            check(dmd_scatt_measure(h, t, (const double *)dm.data(), &v), "measure");
+            check(dmd_scatt_measure(h, 0.0, (const double *)dm.data(), &amp), "measure");
+            if (std::fabs(t) < 1e-30) std::fprintf(r.fp, " 0.00000000000000e+01 %21.14le %21.14le\n", v, amp);
+            else std::fprintf(r.fp, "%21.14le %21.14le %21.14le\n", t, v, amp);
+            std::fflush(r.fp);
+        }
+    };
+    if (!s.restart) report(0);
+    double ti = t;
+    for (int it = 1; t < s.tend; it++) {
+        ti += s.tstep;
+        check(dmd_scatt_integrate_rk4(h, t, h_rk, n_sub, 0, (double *)dm.data(), nullptr), "RK4");
+        t = ti;
+        report(it);
+    }
+    for (auto &r : reps) std::fclose(r.fp);
+    std::printf("dmd_run: reached t = %.10le\n", t);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    bool dry = false;
+    std::string folder;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "--dry-run") dry = true;
+        else if (a == "-h" || a == "--help") {
+            std::printf("usage: dmd_run [--dry-run] [folder with param.in and ldbd_data/]\n");
+            return 0;
+        } else folder = a;
+    }
+    try {
+        return run(folder, dry);
+    } catch (const std::exception &ex) {
+        std::fprintf(stderr, "dmd_run: %s\n", ex.what());
+        return 1;
+    }
+}

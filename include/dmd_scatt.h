@@ -226,7 +226,7 @@ int dmd_synth_couplings(const dmd_synth_params *p, uint64_t k, uint64_t kp, doub
 int dmd_synth_spin(const dmd_synth_params *p, double *s3);
 
 /* ---- ldbd_data files (reference formats) ---------------------------------------- */
-/* Writes ldbd_size.dat, ldbd_ek.bin, ldbd_kvec.bin, ldbd_smat.bin,
+/* Writes ldbd_size.dat, ldbd_R.dat, ldbd_ek.bin, ldbd_kvec.bin, ldbd_smat.bin,
  * ldbd_kpair_k1st/k2nd.bin and ldbd_P1/P2_lindblad.bin into dir
  * (layouts of lindbladInit.../electron_save.cpp:16-65 and ElectronPhonon.h:252-284). */
 int dmd_ldbd_write_synth(const char *dir, const dmd_synth_params *p, int nthreads);

@@ -2,7 +2,7 @@
 (Examples/RealTime_Example_GaAs_20230105/ld-dmd/t300/mult8_gauss0.005/
 dBz/only-eph/n2e16) into tests/golden/gaas_ldbd/, unchanged, as inputs of the
 reader tests: ldbd_size.dat, ldbd_R.dat, ldbd_ek.bin, ldbd_kvec.bin,
-ldbd_imsig.bin and restart/time_restart.dat (about 115 KB).  The run's P and
+ldbd_imsig.bin, restart/time_restart.dat and the run's param.in (about 115 KB).  The run's P and
 k-pair files were never shipped with the example; ldbd_smat.bin (1.6 MB) and
 denmat_restart.bin stay out, their contents enter gaas_spin_restart.npz.
 
@@ -15,7 +15,7 @@ import sys
 
 RUN = "Examples/RealTime_Example_GaAs_20230105/ld-dmd/t300/mult8_gauss0.005/dBz/only-eph/n2e16"
 FILES = ["ldbd_data/ldbd_size.dat", "ldbd_data/ldbd_R.dat", "ldbd_data/ldbd_ek.bin", "ldbd_data/ldbd_kvec.bin",
-         "ldbd_data/ldbd_imsig.bin", "restart/time_restart.dat"]
+         "ldbd_data/ldbd_imsig.bin", "restart/time_restart.dat", "param.in"]
 
 
 def main(ref_root):

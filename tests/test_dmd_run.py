@@ -1,0 +1,98 @@
+"""The param.in front end of dmd_run (host code only: --dry-run stops before
+any GPU call): the settings it resolves from the reference GaAs example's own
+param.in and ldbd_data (tests/golden/gaas_ldbd), checked against what that
+run printed, and its refusals of features the operator does not carry."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "gaas_ldbd")
+RUN = os.path.join(ROOT, "denmat_dynm_development_amd", "dmd_run")
+
+
+def _dry(folder):
+    if not os.path.exists(RUN):
+        pytest.fail(f"{RUN} is missing: build it with __graft_entry__.build()")
+    r = subprocess.run([RUN, "--dry-run", str(folder)], capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout, r.stderr
+
+
+def _num(text, key):
+    m = re.search(r"\b" + re.escape(key) + r" = (\S+)", text)
+    assert m, (key, text)
+    return float(m.group(1))
+
+
+def _gaas_folder(tmp_path, param_text=None):
+    d = tmp_path / "run"
+    (d / "ldbd_data").mkdir(parents=True)
+    for f in ("ldbd_size.dat", "ldbd_R.dat", "ldbd_ek.bin", "ldbd_kvec.bin"):
+        shutil.copy(os.path.join(GOLD, f), d / "ldbd_data" / f)
+    if param_text is None:
+        shutil.copy(os.path.join(GOLD, "param.in"), d / "param.in")
+    else:
+        (d / "param.in").write_text(param_text)
+    return d
+
+
+def test_dry_run_resolves_the_reference_gaas_settings(tmp_path):
+    """The example's param.in (carrier_density = 2e16, Bzpert = 0.1, tstep =
+    100, tend = 1e5, alg_ddmdteq = 1) with its ldbd_data: mu from the carrier
+    density, the cell volume, the times and field in atomic units agree with
+    what the reference run printed to `out` (lines 50-168, 236: T, volume,
+    carrier_density, Bzpert, tstep, tend, mu)."""
+    rc, out, err = _dry(_gaas_folder(tmp_path))
+    assert rc == 0, err
+    assert abs(_num(out, "mu") / 4.9841627e-02 - 1) < 2e-8  # "mu =  4.9841627e-02" (8 digits)
+    assert abs(_num(out, "T") / 9.5004341e-04 - 1) < 1e-7
+    assert abs(_num(out, "cell_size") / 304.795 - 1) < 2e-6  # "volume = 304.795"
+    assert abs(_num(out, "tstep") / 4134.14 - 1) < 2e-6 and abs(_num(out, "tend") / 4.13414e+06 - 1) < 2e-6
+    assert abs(_num(out, "tstep") / (100 * 41.34137417094296) - 1) < 1e-10  # fs of comm/Units.h
+    m = re.search(r"Bpert = (\S+) (\S+) (\S+)", out)
+    assert float(m.group(3)) == pytest.approx(4.25438e-07, rel=2e-6) and float(m.group(1)) == 0
+    assert "alg_ddmdteq = 1" in out and "alg_picture = interaction" in out
+    assert "e-ph bands [4, 6)" in out and "nb_dm = 6 (nv_dm = 4)" in out
+    assert 'key "pumpE" is not read' in out  # said, not silently dropped
+
+
+def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
+    text = "\n".join([
+        "# a comment = not a key",
+        "alg_ddmdteq = 0",
+        "tstep = 3 \\",
+        "",
+        "ode_hmax = 1",
+        "tend = 30",
+        "freq_measure = 2",
+        "Bxpert = 0.5",
+        "mu = 0.05",
+        "scale_scatt = 0.7",
+        "scale_eph = 0.9",
+        "alg_picture = schrodinger",
+    ]) + "\n"
+    rc, out, err = _dry(_gaas_folder(tmp_path, text))
+    assert rc == 0, err
+    fs = 41.34137417094296
+    assert abs(_num(out, "tstep") / (3 * fs) - 1) < 1e-10 and "x 3)" in out  # RK4 step 1 fs x 3
+    assert _num(out, "freq_measure") == 2
+    assert abs(_num(out, "mu") / (0.05 / 27.21138505) - 1) < 1e-10
+    assert _num(out, "scale_eph") == 0.9 and _num(out, "scale_ei") == 0.7
+    assert abs(float(re.search(r"Bpert = (\S+)", out).group(1)) / (0.5 / 2.3505175675871e5) - 1) < 1e-10
+    assert "alg_picture = schrodinger" in out
+
+
+@pytest.mark.parametrize("line,msg", [
+    ("eeMode = Pee_update", "e-e scattering"),
+    ("alg_linearize = 1", "linearized operator"),
+    ("impurity_density = 1e17\nimpMode = model_ionized", "ab_neutral"),
+    ("alg_eph_need_hole = 1\nalg_eph_need_elec = 0", "file set"),
+    ("pumpA0 = 0.002", "pump"),
+    ("alg_ode_method = rk2", "rkf45 or euler"),
+])
+def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
+    assert rc != 0 and msg in err, (rc, err)

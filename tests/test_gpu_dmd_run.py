@@ -1,0 +1,94 @@
+"""dmd_run end to end on the GPU: a synthetic ldbd_data folder and a param.in
+(Bzpert, a coherent Bx, alg_ddmdteq, RK4 substeps, reports every few steps)
+against the oracle's own run of the same physics: first-order initial state,
+ddmdt_eq at the carrier-conserving f_eq, RK4 with the coherent term, S_x,
+S_y, S_z per report, and the restart files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "denmat_dynm_development_amd", "dmd_run")
+FS = 41.34137417094296  # fs of the reference's comm/Units.h
+TESLA = 1. / 2.3505175675871e5
+BOHR2CM = 5.291772109038e-9
+
+pytestmark = pytest.mark.gpu
+
+
+def _fermi(T, mu, e):
+    x = (e - mu) / T
+    return np.where(x < -46, 1.0, np.where(x > 46, 0.0, 1.0 / (np.exp(np.clip(x, -46, 46)) + 1.0)))
+
+
+def _mu_conserving(target, e, T):
+    lo, hi = e.min() - 60 * T, e.max() + 60 * T
+    for _ in range(200):
+        mu = 0.5 * (lo + hi)
+        if _fermi(T, mu, e).sum() > target:
+            hi = mu
+        else:
+            lo = mu
+    return 0.5 * (lo + hi)
+
+
+def test_dmd_run_follows_the_oracle(tmp_path):
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print("case dmd_run preset 0", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    (d / "ldbd_data").mkdir(parents=True)
+    scatt.write_ldbd(str(d / "ldbd_data"), p)
+    tstep_fs, hmax_fs, tend_fs, freq = 2.0, 1.0, 30.0, 3
+    (d / "param.in").write_text("\n".join([
+        "alg_ddmdteq = 1", "Bzpert = 2", "Bx = 0.5", f"tstep = {tstep_fs}", f"ode_hmax = {hmax_fs}",
+        f"tend = {tend_fs}", f"freq_measure = {freq}", "scale_eph = 0.9"]) + "\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    # the same run through the oracle
+    L = str(d / "ldbd_data")
+    info = ldbd.read_info(L)
+    e = ldbd.read_ek(L, info)
+    k1, k2 = ldbd.read_kpairs(L, info)
+    P1, P2 = ldbd.read_P(L, info)
+    P1, P2 = 0.9 * P1, 0.9 * P2
+    s3 = ldbd.read_smat(L, info)
+    _, R = ldbd.read_R(L)
+    cell = abs(np.linalg.det(R))
+    nk, nb = e.shape
+    T, mu = info.temperature, info.mu
+    f = _fermi(T, mu, e)
+    rho0 = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True)
+    rho0 = rho0.reshape(nk, nb, nb)
+    rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho0.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    dm_eq = np.zeros((nk, nb, nb), np.complex128)
+    dm_eq[:, np.arange(nb), np.arange(nb)] = feq
+    prefac = 2 * np.pi / info.nk_full
+    eq = oracle.evolve(k1, k2, P1, P2, e, dm_eq.reshape(nk, -1), t=0.0, prefac=prefac)
+    H = 0.5 * TESLA * s3[:, 0]
+    n_sub = int(np.ceil(tstep_fs / hmax_fs - 1e-9))
+    n_rep = int(np.ceil(tend_fs / tstep_fs - 1e-9)) // freq
+    obs_prefac = 1.0 / (info.nk_full * cell * BOHR2CM ** 3)
+    for c, name in enumerate(("sx", "sy", "sz")):
+        rows = np.loadtxt(d / f"{name}_elec_tot.out")
+        assert rows.shape == (n_rep + 1, 3)
+        rho, obs = oracle.rk4(k1, k2, P1, P2, e, rho0, tstep_fs / n_sub * FS, n_rep * freq * n_sub, prefac=prefac,
+                              H=H, ddmdt_eq=eq, report_every=freq * n_sub, s_obs=s3[:, c], f_dm=f, i0=0, i1=nb,
+                              obs_prefac=obs_prefac)
+        t_ref = np.arange(n_rep + 1) * freq * tstep_fs * FS
+        assert np.abs(rows[:, 0] - t_ref).max() < 1e-9 * t_ref.max()
+        scale = np.abs(obs).max()
+        print(f"{name}: max |S| {scale:.6e}, max |runner - oracle| {np.abs(rows[:, 1] - obs).max():.3e}")
+        assert np.abs(rows[:, 1] - obs).max() < 1e-10 * scale
+        # the phase-free column at t = 0 equals the phased one
+        assert rows[0, 2] == pytest.approx(rows[0, 1], rel=1e-14, abs=1e-300)
+    # restart files of the last report
+    dm_r, t_r = ldbd.read_restart(str(d / "restart"), nk, nb)
+    assert t_r == float("%14.7le" % (n_rep * freq * tstep_fs * FS))
+    assert np.abs(dm_r - rho).max() < 1e-10 * np.abs(rho).max()
