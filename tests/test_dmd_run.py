@@ -96,3 +96,29 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
     assert rc != 0 and msg in err, (rc, err)
+
+
+def test_teph_class_compiles_for_the_reference_complex_type(tmp_path):
+    """Every member of electronphonon_mi355x, instantiated for a complex type
+    laid out as the reference's (struct of two doubles, comm/scalar.h:67-69),
+    and the members dm_dynamics uses (DensityMatrix_Dynamics.h:59-63,70,
+    266-319) called with the reference's argument types."""
+    src = tmp_path / "teph_check.cpp"
+    src.write_text(r'''
+#include "electronphonon_mi355x.h"
+struct complex { double x, y; };
+template class dmdrun::electronphonon_mi355x<complex>;
+void driver_calls(dmdrun::electronphonon_mi355x<complex> *eph, complex **dm, complex **dm1, complex **ddmdt,
+                  double **f_eq, double t) {
+    eph->evolve_driver(t, dm, dm1, ddmdt);
+    eph->compute_ddmdt_eq(f_eq);
+    eph->reset_scatt(true, true, dm, nullptr, t, f_eq);
+    int b = eph->bStart + eph->bEnd;
+    bool no_models = eph->ee_model == nullptr && (eph->eimp.empty() || eph->eimp[0]->eimp_model == nullptr);
+    (void)b; (void)no_models;
+}
+''')
+    inc = [os.path.join(ROOT, "include"), os.path.join(ROOT, "denmat_dynm_development_amd", "host")]
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc[0], "-I", inc[1], str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
