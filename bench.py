@@ -253,8 +253,14 @@ def bench_mixed_batched(args, torch, scatt):
                 tau = taufit.fit_tau(times, obs[0])
                 op.eval_batched(0.0, states[:B])
                 kms = op.time_eval_batched(0.0, 3)
+                # matrix-core work of one evaluation: every stored pair is two visits (k < k')
+                # or one (k = k'); per visit and 32 trajectories 6 v_mfma_f32_32x32x16_bf16
+                n_visits = 2 * n_pairs - int(np.count_nonzero(k1 == k2))
+                mfma_flop = n_visits * ((B + 31) // 32) * 6 * 2 * 32 * 32 * 16
                 per[str(B)] = {"traj_steps_per_s": B * K / el, "ms_per_step": el / K * 1e3,
                                "speedup_vs_fp64": B * K / el / base, "eval_ms": kms,
+                               "mfma_tflops": mfma_flop / (kms * 1e-3) / 1e12,
+                               "mfma_frac_of_bf16_dense_peak": mfma_flop / (kms * 1e-3) / 2.5e15,
                                "pair_traj_per_s": n_pairs * B / (kms * 1e-3),
                                "tau_s_ps": tau, "tau_rel_err_vs_fp64": abs(tau / tau64 - 1)}
                 log(f"{name} B={B}: {el / K * 1e3:.1f} ms/step, {B * K / el:.1f} traj-steps/s "
@@ -286,13 +292,23 @@ def mode_accuracy(args, torch, scatt):
         op.generate_P(p)
         op.set_split(f)
         op.set_observable(s3[:, 2], f, 0, p.nb, 1.0 / p.nk)
+        # ddm at t = 37 fs on the three initial states, against the FP64 kernel
+        # (itself the oracle's to 1e-12 in the parity tests)
+        t_ddm = 37 * FS
+        ddm = [op.eval(t_ddm, states[b]) for b in range(3)]
         el, obs = run_traj(op, torch, states[0], dt, K, False)
         r = {"tau_s_ps": taufit.fit_tau(times, obs), "ms_per_step": el / K * 1e3}
         if prec == 0:
-            ref = obs, r["tau_s_ps"]
+            ref = obs, r["tau_s_ps"], ddm
         else:
+            dd = np.stack(ddm) - np.stack(ref[2])
+            r["ddm_max_rel"] = float(np.abs(dd).max() / np.abs(np.stack(ref[2])).max())
+            r["ddm_rms_rel"] = float(np.linalg.norm(dd) / np.linalg.norm(np.stack(ref[2])))
             r["tau_rel_err"] = abs(r["tau_s_ps"] / ref[1] - 1)
             r["S_max_abs_err_rel"] = float(np.abs(obs - ref[0]).max() / np.abs(ref[0]).max())
+            db = op.eval_batched(t_ddm, states) - np.stack(ref[2])
+            r["batched_B3_ddm_max_rel"] = float(np.abs(db).max() / np.abs(np.stack(ref[2])).max())
+            r["batched_B3_ddm_rms_rel"] = float(np.linalg.norm(db) / np.linalg.norm(np.stack(ref[2])))
             elb, ob = run_traj(op, torch, states, dt, K, True)
             r["batched_B3_tau_rel_err"] = abs(taufit.fit_tau(times, ob[0]) / ref[1] - 1)
             r["batched_B3_S_max_abs_err_rel"] = float(np.abs(ob[0] - ref[0]).max() / np.abs(ref[0]).max())
