@@ -136,6 +136,7 @@ PROTOTYPES = {
     "dmd_scatt_set_ddmdt_eq": [_vp, _vp],
     "dmd_scatt_compute_ddmdt_eq": [_vp, _vp],
     "dmd_scatt_set_split": [_vp, _vp],
+    "dmd_scatt_set_Lsc": [_vp, _vp],
     "dmd_scatt_eval": [_vp, _d, _vp, _vp, _vp],
     "dmd_scatt_eval_batched": [_vp, _d, _i, _vp, _vp],
     "dmd_scatt_compute_ddm": [_vp, _vp, _vp, _vp],

@@ -223,6 +223,7 @@ __global__ void k_obs_adj(int64_t nk, int n, int i0, int i1, const double2 *__re
 static int vjp_ready(dmd_scatt *h)
 {
     if (h->nb > 2) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is built for nb <= 2");
+    if (h->linear) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is built for the P form, not the linearized operator");
     if (h->nbd != h->nb || h->has_H || h->has_eq || h->schrodinger)
         return fail(DMD_ERR_UNSUPPORTED, "the adjoint covers the scattering term alone (no window, coherent term or ddmdt_eq)");
     if (h->precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is FP64");
@@ -303,7 +304,7 @@ extern "C" int dmd_scatt_combine_P(dmd_scatt *h, int ncomp, dmd_scatt *const *co
     for (int c = 0; c < ncomp; c++) {
         const dmd_scatt *o = comps[c];
         if (!o || o->nk != h->nk || o->nb != h->nb || o->np != h->np || o->nslots != h->nslots ||
-            o->precision != DMD_PREC_FP64 || !o->has_P)
+            o->precision != DMD_PREC_FP64 || !o->has_P || o->linear)
             return fail(DMD_ERR_ARG, "dmd_scatt_combine_P: components must hold FP64 P of the same pairs");
     }
     if (h->precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "dmd_scatt_combine_P: FP64 handles only");
@@ -325,6 +326,7 @@ extern "C" int dmd_scatt_combine_P(dmd_scatt *h, int ncomp, dmd_scatt *const *co
     DMD_HIP(hipGetLastError());
     DMD_HIP(hipStreamSynchronize(h->st));
     h->has_P = true;
+    h->linear = false;
     return DMD_OK;
 }
 

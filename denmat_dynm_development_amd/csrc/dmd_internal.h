@@ -27,6 +27,8 @@ int hw_threads();
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+
+
 // rows per tile of the tile kernel (a multiple of 256)
 #define DMD_TILE_R 512
 // chunks of one row block a tile-kernel workgroup takes in sequence
@@ -44,6 +46,14 @@ __host__ __device__ inline int blk_off(int mat, int row, int col, int U)
 {
     const int l = 4 * (row >> 2) + (col >> 2), e = 4 * (row & 3) + (col & 3);
     return mat * 256 + ((e / U) * 16 + l) * U + e % U;
+}
+
+// Address of element el of the pair in slot s: pair-contiguous, or the
+// structure-of-arrays granules of 8 pairs the tile kernel streams.
+__host__ __device__ __forceinline__ int64_t paddr(int64_t s, int el, int ep, int soa)
+{
+    if (soa == 2) return s * 512 + blk_off(el >> 8, (el >> 4) & 15, el & 15, 1);  // nb = 4 blocked records
+    return soa ? (s >> 3) * (8 * (int64_t)ep) + (int64_t)el * 8 + (s & 7) : s * ep + el;
 }
 
 // Device clock: t(a) = (t0 + step*dt) + a*dt, the same expression as the RK4
@@ -164,6 +174,9 @@ struct dmd_scatt {
     int32_t *d_dk = nullptr;             // and rows
     double2 *d_adj = nullptr;            // 3 x nk x nb^2
     double *d_dotpart = nullptr;         // dot-product partials
+    // linearized operator (alg_linearize): the P store holds Lscij | Lscji
+    bool linear = false;
+    double *d_flin = nullptr;            // f_eq (nk x nb) the operator was linearized at
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
@@ -226,6 +239,7 @@ int launch_tiles4(dmd_scatt *h, bool dm1, double a, double2 *out);
 // X in place of rho_S and Y in place of 1 - rho_S (Y null: 1 - X); P must be
 // pair-contiguous (the single-read layout)
 int launch_raw_rows(dmd_scatt *h, const double2 *X, const double2 *Y);
+int launch_linear(dmd_scatt *h, double a, double2 *out);
 // batched path (dmd_batched.hip)
 int batched_prepare(dmd_scatt *h, int B);  // not capturable
 int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *out);

@@ -50,6 +50,7 @@ extern "C" int dmd_scatt_load_P(dmd_scatt *h, const char *dir, const dmd_ldbd_in
         return fail(DMD_ERR_ARG, "dmd_scatt_load_P: the files hold " + std::to_string(info->nkpair) + " pairs of " +
                                      std::to_string(nbe) + " bands, the handle " + std::to_string(h->np) +
                                      " pairs of " + std::to_string(h->nb));
+    if (h->linear && add) return fail(DMD_ERR_STATE, "the handle holds the linearized operator: load P first");
     const int hole = info->kind == 1;
     const std::string f[2] = {ldbd_path(dir, ldbd_P_name(1, scatt, impurity, hole)),
                               ldbd_path(dir, ldbd_P_name(2, scatt, impurity, hole))};
@@ -87,6 +88,7 @@ extern "C" int dmd_scatt_load_P(dmd_scatt *h, const char *dir, const dmd_ldbd_in
     }
     DMD_HIP(hipStreamSynchronize(h->st));
     h->has_P = true;
+    h->linear = false;
     return DMD_OK;
 }
 

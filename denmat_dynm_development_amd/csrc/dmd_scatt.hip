@@ -701,14 +701,6 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nk, double *__restrict
 // ---------------------------------------------------------------------------
 // P placement: file order -> sorted layout, scaled; mode 0 set, 1 add.
 // ---------------------------------------------------------------------------
-// Address of element el of the pair in slot s: pair-contiguous, or the
-// structure-of-arrays granules of 8 pairs the tile kernel streams.
-__device__ __forceinline__ int64_t paddr(int64_t s, int el, int ep, int soa)
-{
-    if (soa == 2) return s * 512 + blk_off(el >> 8, (el >> 4) & 15, el & 15, 1);  // nb = 4 blocked records
-    return soa ? (s >> 3) * (8 * (int64_t)ep) + (int64_t)el * 8 + (s & 7) : s * ep + el;
-}
-
 __global__ void k_place_P(int64_t n, int64_t first, int nb4, const double2 *__restrict__ P1,
                           const double2 *__restrict__ P2, const int64_t *__restrict__ inv, double scale, int add,
                           int soa, double2 *__restrict__ Pd)
@@ -784,7 +776,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_pscale); dfree(d_fsplit); dfree(d_feq); dfree(d_x32); dfree(d_hacc16);
     dfree(d_vrec); dfree(d_vq); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
     dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB); dfree(d_adj); dfree(d_dotpart);
-    dfree(d_dpos); dfree(d_dk);
+    dfree(d_dpos); dfree(d_dk); dfree(d_flin);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1085,6 +1077,8 @@ static int put_P(dmd_scatt *h, int64_t first, int64_t count, const double *P1, c
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
     if (first < 0 || count < 0 || first + count > h->np) return fail(DMD_ERR_ARG, "pair range outside the stored pairs");
+    if (h->linear && (add || first != 0 || count != h->np))
+        return fail(DMD_ERR_STATE, "the handle holds the linearized operator: upload the whole P first");
     if (count == 0) {
         if (h->np == 0) h->has_P = true;
         return DMD_OK;
@@ -1124,6 +1118,7 @@ static int put_P(dmd_scatt *h, int64_t first, int64_t count, const double *P1, c
     if (rc) return rc;
     if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
     h->has_P = true;
+    h->linear = false;
     return DMD_OK;
 }
 
@@ -1188,6 +1183,7 @@ extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
     DMD_HIP(hipGetLastError());
     DMD_HIP(hipStreamSynchronize(h->st));
     h->has_P = true;
+    h->linear = false;
     return DMD_OK;
 }
 
@@ -1352,6 +1348,7 @@ int contract_kind(const dmd_scatt *h)
 // the contraction + epilogue on the current rhoS (and rho1S when dm1)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
+    if (h->linear) return launch_linear(h, a, out);
     const int kind = contract_kind(h);
     if (kind == DMD_KERNEL_TILES && h->nb == 4) {
         return launch_tiles4(h, dm1, a, out);

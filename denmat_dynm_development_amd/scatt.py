@@ -152,6 +152,12 @@ class ScatteringOperator:
         p, keep = _ptr(f, np.float64)
         call("dmd_scatt_set_split", self._h, p)
 
+    def set_Lsc(self, f_eq):
+        """Linearize at f_eq (nk x nb): set_Lsc, ElectronPhonon_linear.cpp:3-36.
+        The stored P becomes Lscij | Lscji; evaluations are evolve_linear."""
+        p, keep = _ptr(f_eq, np.float64)
+        call("dmd_scatt_set_Lsc", self._h, p)
+
     def set_ddmdt_eq(self, ddmdt_eq):
         p, keep = _ptr(ddmdt_eq, np.complex128)
         call("dmd_scatt_set_ddmdt_eq", self._h, p)

@@ -125,6 +125,18 @@ int dmd_scatt_compute_ddmdt_eq(dmd_scatt *h, const double *f);
  * part (evolve_eph.cpp:3-19,154-162).  NULL: f = 0.  No effect for FP64. */
 int dmd_scatt_set_split(dmd_scatt *h, const double *f);
 
+/* Linearized operator (alg_linearize): electronphonon::set_Lsc
+ * (ElectronPhonon_linear.cpp:3-36, e-ph part :147-207) at the occupations
+ * f_eq (nk x nb, the e-ph window).  The stored P (e-i P already folded in)
+ * is replaced, in place, by Lscij | Lscji per stored pair; Lscii of row k is
+ * folded into its diagonal pair (k, k), and rows without one drop it, as the
+ * reference does.  From then on eval, evolve_driver, integrate_rk4 and
+ * time_eval compute evolve_linear (evolve_eph_linear.cpp:3-53): ddmdt_k =
+ * prefac/2 sum (X + X^+), X = Lsc (rho_k' - f_eq_k'), with compute_Pt's phase
+ * when expt is set and no ddmdt_eq.  Uploading, generating or loading the
+ * whole P again returns the handle to the P form.  FP64 only. */
+int dmd_scatt_set_Lsc(dmd_scatt *h, const double *f_eq);
+
 /* ---- the hot path --------------------------------------------------------- */
 /* electronphonon::evolve (evolve_eph.cpp:34-166): ddmdt (nk x nb^2, fully
  * overwritten) from rho (nk x nb^2) at time t.  one_minus_rho may be NULL,

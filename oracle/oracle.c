@@ -29,6 +29,10 @@
  *                          reference's own GaAs restart/sz_elec_tot.out pair.
  *   oracle_dm_Bpert_1st    closed form of the first-order perturbation,
  *                          degenerate limit, Hermiticity.
+ *   oracle_set_Lsc +       the Jacobian of oracle_evolve at diag(f_eq) (exact
+ *   oracle_evolve_linear   central differences of a quadratic map) when every
+ *                          k has its diagonal pair; Lscii dropped for rows
+ *                          without one, as the reference does; linearity.
  * The bit-level rounding of the reference's CBLAS zgemm/zhemm (GSL 2.6 or MKL)
  * and libm sincos is not reproduced bit for bit ("parity unpinned" at the last
  * ulp); summation orders follow the reference's loops so results agree to a
@@ -239,6 +243,115 @@ int oracle_evolve(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const 
                 }
     }
     free(Pdm); free(dm1P);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Linearized operator (alg_linearize), e-ph part.                           */
+/* electronphonon::set_Lsc, src_v4.5.7/ElectronPhonon_linear.cpp:3-36, with  */
+/* add_Lscii_from_P1_P2 (:147-181) and add_Lscij_from_P1_P2 (:183-207, fac2  */
+/* off) over the stored pairs in file order, then Lscii[k] added to both     */
+/* matrices of every diagonal pair (k, k) (:25-34).  f_eq: nk x nb.          */
+/* ------------------------------------------------------------------------ */
+int oracle_set_Lsc(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                   const double *P1, const double *P2, const double *f_eq, double *Lij, double *Lji)
+{
+    const int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 8 || nk < 0 || npairs < 0) return -1;
+    for (int64_t ip = 0; ip < npairs; ip++)
+        if (k1[ip] > k2[ip] || (int64_t)k2[ip] >= nk) return -2;
+    zc *Lii = (zc *)calloc((size_t)nk * nb4 + 1, sizeof(zc));
+    if (!Lii) return -3;
+    memset(Lij, 0, sizeof(zc) * (size_t)npairs * nb4);
+    memset(Lji, 0, sizeof(zc) * (size_t)npairs * nb4);
+    for (int64_t ip = 0; ip < npairs; ip++) {
+        const int64_t ik = (int64_t)k1[ip], jk = (int64_t)k2[ip];
+        const zc *p1 = (const zc *)P1 + (size_t)ip * nb4, *p2 = (const zc *)P2 + (size_t)ip * nb4;
+        const double *fi = f_eq + ik * nb, *fj = f_eq + jk * nb;
+        zc *li = Lii + (size_t)ik * nb4, *lj = Lii + (size_t)jk * nb4;
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                const int n12 = (i1 * nb + i2) * nb2;
+                /* -delta_1a P1_{b2,33} f^jk_3 (and the conj(P2) mirror on jk) */
+                for (int ib = 0; ib < nb; ib++) {
+                    const int nb2b = (ib * nb + i2) * nb2, i12ab = n12 + i1 * nb + ib;
+                    for (int i3 = 0; i3 < nb; i3++) {
+                        const int ib233 = nb2b + i3 * nb + i3;
+                        li[i12ab] = zsub(li[i12ab], zscale(p1[ib233], fj[i3]));
+                        if (jk > ik) lj[i12ab] = zsub(lj[i12ab], zscale(zconj(p2[ib233]), fi[i3]));
+                    }
+                }
+                /* -(1 - f^jk_3) P2_{33,1a} delta_2b (and the conj(P1) mirror) */
+                for (int ia = 0; ia < nb; ia++) {
+                    const int i1a = i1 * nb + ia, i12ab = n12 + ia * nb + i2;
+                    for (int i3 = 0; i3 < nb; i3++) {
+                        const int i331a = (i3 * nb + i3) * nb2 + i1a;
+                        li[i12ab] = zsub(li[i12ab], zscale(p2[i331a], 1 - fj[i3]));
+                        if (jk > ik) lj[i12ab] = zsub(lj[i12ab], zscale(zconj(p1[i331a]), 1 - fi[i3]));
+                    }
+                }
+            }
+        zc *a = (zc *)Lij + (size_t)ip * nb4, *b = (zc *)Lji + (size_t)ip * nb4;
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                const int i12 = i1 * nb + i2, n12 = i12 * nb2;
+                for (int ia = 0; ia < nb; ia++)
+                    for (int ib = 0; ib < nb; ib++) {
+                        const int iab = ia * nb + ib, i12ab = n12 + iab, iab12 = iab * nb2 + i12;
+                        a[i12ab] = zadd(a[i12ab], zadd(zscale(p1[i12ab], 1 - fi[i1]), zscale(p2[iab12], fi[i2])));
+                        b[i12ab] = zadd(b[i12ab], zadd(zscale(zconj(p2[i12ab]), 1 - fj[i1]),
+                                                       zscale(zconj(p1[iab12]), fj[i2])));
+                    }
+            }
+    }
+    for (int64_t ip = 0; ip < npairs; ip++)
+        if (k1[ip] == k2[ip]) {
+            const zc *l = Lii + (size_t)k1[ip] * nb4;
+            zc *a = (zc *)Lij + (size_t)ip * nb4, *b = (zc *)Lji + (size_t)ip * nb4;
+            for (int i = 0; i < nb4; i++) { a[i] = zadd(a[i], l[i]); b[i] = zadd(b[i], l[i]); }
+        }
+    free(Lii);
+    return 0;
+}
+
+/* electronphonon::evolve_linear, src_v4.5.7/evolve_eph_linear.cpp:3-45, with
+ * compute_ddmdt (:47-53): drho = dm - diag(f_eq); per stored pair
+ * X = Lsct_ij drho_k' added to row k as prefac/2 (X + X^+), and for k < k'
+ * Y = Lsct_ji drho_k added to row k'.  Lsct carries the P1-type phase of
+ * compute_Pt (:55-73) when expt is set. */
+int oracle_evolve_linear(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                         const double *Lij, const double *Lji, const double *e, const double *dm,
+                         const double *f_eq, double t, double prefac, int expt, double *ddmdt)
+{
+    const int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 8 || nk < 0 || npairs < 0) return -1;
+    for (int64_t ip = 0; ip < npairs; ip++)
+        if (k1[ip] > k2[ip] || (int64_t)k2[ip] >= nk) return -2;
+    zc *d = (zc *)malloc(sizeof(zc) * ((size_t)nk * nb2 + 1)), *Lt = (zc *)malloc(sizeof(zc) * nb4);
+    if (!d || !Lt) { free(d); free(Lt); return -3; }
+    memcpy(d, dm, sizeof(zc) * (size_t)nk * nb2);
+    for (int64_t ik = 0; ik < nk; ik++)
+        for (int i = 0; i < nb; i++) d[ik * nb2 + i * nb + i].x -= f_eq[ik * nb + i];
+    memset(ddmdt, 0, sizeof(zc) * (size_t)nk * nb2);
+    zc contrib[64];
+    for (int64_t ip = 0; ip < npairs; ip++) {
+        const int64_t ik = (int64_t)k1[ip], ikp = (int64_t)k2[ip];
+        for (int dir = 0; dir < 2; dir++) {
+            if (dir == 1 && !(ik < ikp)) break;
+            const int64_t row = dir ? ikp : ik, col = dir ? ik : ikp;
+            const double *L = (dir ? Lji : Lij) + (size_t)ip * nb4 * 2;
+            if (expt) oracle_compute_Pt(nb, t, e + row * nb, e + col * nb, L, (double *)Lt, 0);
+            else memcpy(Lt, L, sizeof(zc) * nb4);
+            for (int i = 0; i < nb2; i++) contrib[i] = zmk(0, 0);
+            gemv_Ab(contrib, Lt, d + col * nb2, nb2);
+            zc *out = (zc *)ddmdt + row * nb2;
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++)
+                    out[i * nb + j] = zadd(out[i * nb + j],
+                                           zscale(zadd(contrib[i * nb + j], zconj(contrib[j * nb + i])), prefac * 0.5));
+        }
+    }
+    free(d); free(Lt);
     return 0;
 }
 

@@ -51,6 +51,9 @@ def lib():
                                  _d, _d, _d, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_double, _i64, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_int, _d, _d]
+        L.oracle_set_Lsc.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d]
+        L.oracle_evolve_linear.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, _d]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -101,6 +104,38 @@ def evolve(k1, k2, P1, P2, e, dm, dm1=None, t=0.0, prefac=1.0, expt=True, ddmdt_
                              int(nthreads), _p(out))
     if rc != 0:
         raise ValueError(f"oracle_evolve failed: {rc}")
+    return out
+
+
+def set_Lsc(k1, k2, P1, P2, f_eq):
+    """electronphonon::set_Lsc, e-ph part (src_v4.5.7/ElectronPhonon_linear.cpp:3-36,
+    147-207): (Lscij, Lscji) per stored pair, Lscii folded into diagonal pairs."""
+    f_eq = _f64(f_eq)
+    nk, nb = f_eq.shape
+    P1 = _c128(P1)
+    P2 = _c128(P2)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    Lij = np.zeros_like(P1)
+    Lji = np.zeros_like(P2)
+    rc = lib().oracle_set_Lsc(nk, nb, len(k1), _p(k1), _p(k2), _p(P1), _p(P2), _p(f_eq), _p(Lij), _p(Lji))
+    if rc != 0:
+        raise ValueError(f"oracle_set_Lsc failed: {rc}")
+    return Lij, Lji
+
+
+def evolve_linear(k1, k2, Lij, Lji, e, dm, f_eq, t=0.0, prefac=1.0, expt=True):
+    """electronphonon::evolve_linear (src_v4.5.7/evolve_eph_linear.cpp:3-53)."""
+    e = _f64(e)
+    nk, nb = e.shape
+    dm = _c128(dm).reshape(nk, nb * nb)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    out = np.zeros((nk, nb * nb), np.complex128)
+    rc = lib().oracle_evolve_linear(nk, nb, len(k1), _p(k1), _p(k2), _p(_c128(Lij)), _p(_c128(Lji)), _p(e), _p(dm),
+                                    _p(_f64(f_eq)), float(t), float(prefac), int(bool(expt)), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_evolve_linear failed: {rc}")
     return out
 
 
