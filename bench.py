@@ -17,6 +17,7 @@ spin lifetime of the same runs compared.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched]
   python bench.py --mode accuracy   # FP64 / complex64 / BF16 S_z(t) and tau_s, nb = 4 with e-i
   python bench.py --mode adjoint    # forward + adjoint RK4 with 7 parameters, nb = 2, nk = 16384
+  python bench.py --mode linear     # the linearized operator (set_Lsc + evolve_linear), nb = 2 and nb = 4
 """
 import argparse
 import json
@@ -368,6 +369,42 @@ def mode_adjoint(args, torch, scatt):
     return out
 
 
+def mode_linear(args, torch, scatt):
+    """The linearized operator (alg_linearize): set_Lsc at the Fermi occupations,
+    then evolve_linear timed per evaluation and per RK4 step, FP64, at the
+    default nb = 2 workload and the nb = 4 power-law workload."""
+    out = {"mode": "linear", "results": {}}
+    for preset in (2, 4):
+        p = scatt.synth_params(preset)
+        e = scatt.synth_energies(p)
+        k1, k2 = scatt.synth_kpairs(p)
+        s3 = scatt.synth_spin(p)
+        rho0, f = initial_state(p, e, s3)
+        op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+        op.generate_P(p)
+        t0 = time.perf_counter()
+        op.set_Lsc(f)
+        t_lsc = time.perf_counter() - t0
+        rho_d = torch.from_numpy(rho0).cuda()
+        op.eval(0.0, rho_d)
+        kms, ems = op.time_eval(17 * FS, 5)
+        ab = alg_bytes(len(k1), p.nk, p.nb)
+        op.integrate_rk4(rho_d, 0.0, FS, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.integrate_rk4(rho_d, 0.0, FS, args.steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        kname = {2: "k_tiles_soa<LIN>", 4: "k_panels4<LIN>"}.get(p.nb, "k_lin_rows")
+        r = {"nb": p.nb, "nk": p.nk, "stored_pairs": len(k1), "set_Lsc_s": t_lsc, "kernel": kname,
+             "kernel_ms": kms, "hbm_GBps": ab / kms / 1e6, "roofline_frac": ab / kms / 1e6 / HBM_PEAK_GBPS,
+             "ms_per_step": el / args.steps * 1e3, "pairs_per_s": len(k1) * 4 * args.steps / el}
+        out["results"][f"preset{preset}"] = r
+        log(f"linear preset {preset}: {r}")
+        op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,7 +412,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint"])
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
@@ -400,6 +437,9 @@ def main():
         return
     if args.mode == "adjoint":
         print(json.dumps(mode_adjoint(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "linear":
+        print(json.dumps(mode_linear(args, torch, scatt)), flush=True)
         return
     out, (p, e, k1, k2, rho0) = bench_plain(args, torch, scatt)
     if not args.no_cpu:

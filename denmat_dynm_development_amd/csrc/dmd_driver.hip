@@ -79,6 +79,7 @@ struct StageArgs {
     const double *e;    // nk x nb (window)
     const DClock *clk;
     double dt;
+    const double *fsub; // linearized operator: rhoS = rho_S - diag(f_eq) (nk x nb), else null
 };
 
 // right-hand side element (i, j) of state kk (row k = kk mod nk)
@@ -99,6 +100,7 @@ __device__ __forceinline__ void rotate_out(const StageArgs &A, int64_t k, int i,
     int wi = i - A.b0, wj = j - A.b0;
     if (!rhoS || wi < 0 || wi >= A.nb || wj < 0 || wj >= A.nb) return;
     if (A.expt && wi != wj) v = cmul(v, ccis(-(A.e[k * A.nb + wi] - A.e[k * A.nb + wj]) * t));
+    if (A.fsub && wi == wj) v.x -= A.fsub[k * A.nb + wi];
     rhoS[k * A.nb * A.nb + wi * A.nb + wj] = v;
 }
 
@@ -224,6 +226,7 @@ static StageArgs stage_args(dmd_scatt *h, double dt)
     A.schrod = h->schrodinger;
     A.expt = h->expt;
     A.edm = h->d_edm; A.e = h->d_e; A.clk = h->d_clk; A.dt = dt;
+    A.fsub = h->linear ? h->d_flin : nullptr;
     return A;
 }
 

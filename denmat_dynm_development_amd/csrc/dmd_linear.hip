@@ -148,11 +148,8 @@ __global__ __launch_bounds__(256) void k_lin_rows(int64_t nk, const double2 *__r
     double2 acc[J];
 #pragma unroll
     for (int j = 0; j < J; j++) acc[j] = make_double2(0, 0);
-    auto drho = [&](int64_t q, int ab) {
-        double2 v = rhoS[q * NB2 + ab];
-        if (ab % (NB + 1) == 0) v.x -= f[q * NB + ab / (NB + 1)];
-        return v;
-    };
+    (void)f;
+    auto drho = [&](int64_t q, int ab) { return rhoS[q * NB2 + ab]; };  // drho_S (rotation subtracted f_eq)
     if (active) {
         for (int pass = 0; pass < 2; pass++) {
             const int64_t p0 = pass ? revptr[k] : rowptr[k], p1 = pass ? revptr[k + 1] : rowptr[k + 1];

@@ -73,11 +73,12 @@ __device__ __forceinline__ double2 herm_up(const double2 *A, int i, int j)
 // (window rows/cols [b0, b0+NB) of an nbd x nbd state).  With a separate
 // 1-rho array (the reference's dm1 argument) it is rotated alike.
 // ---------------------------------------------------------------------------
+// fsub (the linearized operator's f_eq, nk x NB): rhoS = rho_S - diag(fsub).
 template <int NB>
 __global__ void k_rotate_in(int64_t nk, int nbd, int b0, const double2 *__restrict__ rho,
                             const double2 *__restrict__ rho1, const double *__restrict__ e,
                             const DClock *__restrict__ clk, double a, int expt,
-                            double2 *__restrict__ rhoS, double2 *__restrict__ rho1S)
+                            double2 *__restrict__ rhoS, double2 *__restrict__ rho1S, const double *__restrict__ fsub)
 {
     constexpr int NB2 = NB * NB;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,7 +90,9 @@ __global__ void k_rotate_in(int64_t nk, int nbd, int b0, const double2 *__restri
     double2 ph = make_double2(1.0, 0.0);
     bool rot = expt && i != j;
     if (rot) ph = ccis(-(e[k * NB + i] - e[k * NB + j]) * clock_t_at(clk, a));
-    rhoS[idx] = rot ? cmul(v, ph) : v;
+    v = rot ? cmul(v, ph) : v;
+    if (fsub && i == j) v.x -= fsub[k * NB + i];
+    rhoS[idx] = v;
     if (rho1) {
         double2 w = rho1[src];
         rho1S[idx] = rot ? cmul(w, ph) : w;
@@ -432,6 +435,16 @@ __device__ __forceinline__ void herm_inc(const double2 *pdm, const double2 *d1p,
     hv[3] = c[1].y - c[2].y;
 }
 
+// The packed Hermitian part H = X + X^+ of a 2 x 2 matrix X (the linearized
+// operator's per-pair increment, evolve_eph_linear.cpp:47-53).
+__device__ __forceinline__ void herm_of(const double2 *c, double hv[4])
+{
+    hv[0] = 2.0 * c[0].x;
+    hv[1] = 2.0 * c[3].x;
+    hv[2] = c[1].x + c[2].x;
+    hv[3] = c[1].y - c[2].y;
+}
+
 // Rows [base_row, base_row + 64) of one wave: lane l holds the 4 doubles of
 // row base_row + l (zeros past nk); the wave regroups them so every atomic
 // instruction covers 16 consecutive rows' 32-byte records (4 full lines).
@@ -484,7 +497,10 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
     for (int ij = 0; ij < NB2; ij++) out[k * NB2 + ij] = d[ij];
 }
 
-template <bool DM1>
+// LIN: the slot holds the linearized operator Lscij | Lscji and rhoS holds
+// drho_S; forward X = Lscij drho_k', reverse Y = Lscji drho_k, increments
+// X + X^+ and Y + Y^+ (evolve_eph_linear.cpp:26-37).
+template <bool DM1, bool LIN>
 __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint32_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
@@ -561,6 +577,31 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             for (int el = 0; el < NB4; el++) p1[el] = g[el * 8];
 #pragma unroll
             for (int el = 0; el < NB4; el++) p2[el] = g[(NB4 + el) * 8];
+            if (LIN) {
+                double2 x[NB2], y[NB2];
+#pragma unroll
+                for (int i = 0; i < NB2; i++) {
+                    x[i] = make_double2(0, 0);
+                    y[i] = make_double2(0, 0);
+#pragma unroll
+                    for (int q = 0; q < NB2; q++) {
+                        cfma(x[i], p1[i * NB2 + q], rkp[q]);  // X_k[i]  += Lscij[i,q] drho_k'[q]
+                        cfma(y[i], p2[i * NB2 + q], rk[q]);   // Y_k'[i] += Lscji[i,q] drho_k[q]
+                    }
+                }
+                double hf[4], hr[4];
+                herm_of(x, hf);
+                herm_of(y, hr);
+                if (ok) {
+#pragma unroll
+                    for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
+                }
+                if (rev) {
+#pragma unroll
+                    for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
+                }
+                continue;
+            }
             // forward (row k): Pdm_k = P1 rho_k', dm1P_k = (1-rho_k') P2
             {
                 double2 f1[NB2], f2[NB2];
@@ -1260,12 +1301,13 @@ namespace dmd {
 int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a)
 {
     int64_t n = h->nk * h->nb2;
+    const double *fs = h->linear ? h->d_flin : nullptr;  // the linearized operator acts on drho
     dim3 g(nblk(n, 256)), b(256);
     switch (h->nb) {
-    case 1: hipLaunchKernelGGL(k_rotate_in<1>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
-    case 2: hipLaunchKernelGGL(k_rotate_in<2>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
-    case 3: hipLaunchKernelGGL(k_rotate_in<3>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
-    case 4: hipLaunchKernelGGL(k_rotate_in<4>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S); break;
+    case 1: hipLaunchKernelGGL(k_rotate_in<1>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
+    case 2: hipLaunchKernelGGL(k_rotate_in<2>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
+    case 3: hipLaunchKernelGGL(k_rotate_in<3>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
+    case 4: hipLaunchKernelGGL(k_rotate_in<4>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
     }
     DMD_HIP(hipGetLastError());
     return DMD_OK;
@@ -1323,12 +1365,15 @@ static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const double2 *eq = h->has_eq ? h->d_eq : nullptr;
     if (h->ngroups > 0) {
-        if (dm1)
-            hipLaunchKernelGGL(k_tiles_soa<true>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
+        if (h->linear)
+            hipLaunchKernelGGL((k_tiles_soa<false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
+                               h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
+        else if (dm1)
+            hipLaunchKernelGGL((k_tiles_soa<true, false>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
+                               h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
         else
-            hipLaunchKernelGGL(k_tiles_soa<false>, dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P, h->d_pk,
-                               h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
+            hipLaunchKernelGGL((k_tiles_soa<false, false>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
+                               h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
     }
     hipLaunchKernelGGL(k_assemble_h, dim3(nblk(h->nk, 256)), dim3(256), 0, h->st, h->nk, h->d_acc_herm, h->d_e,
                        h->d_clk, a, h->expt, h->schrodinger, h->prefac, eq, out);
@@ -1348,8 +1393,8 @@ int contract_kind(const dmd_scatt *h)
 // the contraction + epilogue on the current rhoS (and rho1S when dm1)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
-    if (h->linear) return launch_linear(h, a, out);
     const int kind = contract_kind(h);
+    if (h->linear && kind != DMD_KERNEL_TILES) return launch_linear(h, a, out);
     if (kind == DMD_KERNEL_TILES && h->nb == 4) {
         return launch_tiles4(h, dm1, a, out);
     } else if (kind == DMD_KERNEL_TILES) {

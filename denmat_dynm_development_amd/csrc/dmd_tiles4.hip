@@ -209,7 +209,7 @@ constexpr int RS4 = 66;  // LDS doubles per row accumulator: F1 (32) | F2 (32) +
 // Flush of the rows [0, R) of one LDS block (rows base + r).  RAW: add F1|F2
 // (64 doubles) to out[row * 64 ..]; else assemble and add the 16 packed
 // Hermitian reals to out[row * 16 ..].  Clears the block.
-template <bool RAW, bool DM1, int R>
+template <bool RAW, bool DM1, int R, bool LIN = false>
 __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t nk, const double2 *__restrict__ rhoS,
                                              const double2 *__restrict__ rho1S, double *__restrict__ out)
 {
@@ -227,7 +227,11 @@ __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t 
         const int r = r0 + (threadIdx.x >> 4);
         const int64_t row = base + r;
         double v = 0.0;
-        if (row < nk) {
+        if (LIN && row < nk) {  // H = X + X^+ of the row sums X (evolve_eph_linear.cpp:47-53)
+            const double *F = acc + r * RS4;
+            v = herm_pack(make_double2(F[2 * (4 * i + j)], F[2 * (4 * i + j) + 1]),
+                          make_double2(F[2 * (4 * j + i)], F[2 * (4 * j + i) + 1]), i, j);
+        } else if (row < nk) {
             double2 rk[16], r1k[16];
 #pragma unroll
             for (int q = 0; q < 16; q++) {
@@ -252,8 +256,10 @@ __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t 
 // stay in LDS and are assembled once at the end; each pair's reverse
 // contribution to row k' is assembled at once into the 16 reals of its
 // Hermitian increment and added with one FP64 atomic line.  RAW writes the
-// raw row sums F1|F2 instead (F_eq).
-template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1, int R>
+// raw row sums F1|F2 instead (F_eq).  LIN: the record holds the linearized
+// operator Lscij | Lscji and X is drho_S: forward X_k += Lscij drho_k',
+// reverse Y_k' += Lscji drho_k, increments X + X^+ (evolve_eph_linear.cpp:26-53).
+template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1, int R, bool LIN = false>
 __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restrict__ P,
                                                  const float *__restrict__ pscale, const int32_t *__restrict__ k1s,
                                                  const int32_t *__restrict__ k2s, const int64_t *__restrict__ cbeg,
@@ -297,6 +303,7 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
         for (int q = 0; q < 4; q++) {
             xkp[q] = X[kp * 16 + 4 * b + q];
             xk[q] = xI[lo1 * 16 + 4 * b + q];
+            if (LIN) continue;
             const V u = X[kp * 16 + 4 * a + q], w = xI[lo1 * 16 + 4 * a + q];
             if (DM1) {
                 ykp[q] = Y[kp * 16 + 4 * a + q];
@@ -312,6 +319,28 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
         V f1[4], f2[4], g1[4], g2[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) { f1[q] = cz<V>(); f2[q] = cz<V>(); g1[q] = cz<V>(); g2[q] = cz<V>(); }
+        if (LIN) {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    cfma_t(f1[i], p1[4 * i + j], xkp[j]);  // X_k[4a+i]  += Lscij[4a+i][4b+j] drho_k'[4b+j]
+                    cfma_t(g1[i], p2[4 * i + j], xk[j]);   // Y_k'[4a+i] += Lscji[4a+i][4b+j] drho_k[4b+j]
+                }
+            const V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
+            const V G1 = reduce4(g1, lane, 2, 1);
+            if (ok) {
+                double *r1 = accI + lo1 * RS4;
+                atomicAdd(r1 + 2 * (4 * a + b), (double)F1.x);
+                atomicAdd(r1 + 2 * (4 * a + b) + 1, (double)F1.y);
+            }
+            gsc[gi][4 * a + b] = make_double2((double)G1.x, (double)G1.y);
+            __builtin_amdgcn_wave_barrier();
+            const double v = herm_pack(gsc[gi][4 * a + b], gsc[gi][4 * b + a], a, b);
+            __builtin_amdgcn_wave_barrier();  // read before the next pair writes the group's slots
+            if (ok && kk < kp && v != 0.0) unsafeAtomicAdd(out + kp * 16 + l, v);
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -364,7 +393,7 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
         }
     }
     __syncthreads();
-    flush_block4<RAW, DM1, R>(accI, I0, nk, rhoS, rho1S, out);
+    flush_block4<RAW, DM1, R, LIN>(accI, I0, nk, rhoS, rho1S, out);
 }
 
 // Epilogue: ddm = prefac/2 (H + H_eq), rotated back, minus ddmdt_eq.  H from
@@ -657,11 +686,11 @@ int tiles4_relayout(dmd_scatt *h, int to_blocked)
     return DMD_OK;
 }
 
-template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1 = false>
+template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1 = false, bool LIN = false>
 static void launch_k4(dmd_scatt *h, const void *X, const void *Y, double *out)
 {
     if (h->nchunks <= 0) return;
-    hipLaunchKernelGGL((k_panels4<PT, CT, RAW, YNEG, DM1, DMD_TILE4_R>), dim3((unsigned)h->nchunks), dim3(256), 0,
+    hipLaunchKernelGGL((k_panels4<PT, CT, RAW, YNEG, DM1, DMD_TILE4_R, LIN>), dim3((unsigned)h->nchunks), dim3(256), 0,
                        h->st, h->nk, (const PT *)h->d_P, h->d_pscale, h->d_k1, h->d_k2, h->d_cbeg, h->d_clen, h->d_cij,
                        (const typename C2<CT>::T *)X, (const typename C2<CT>::T *)Y, h->d_rhoS, h->d_rho1S, out);
 }
@@ -696,7 +725,9 @@ int launch_tiles4(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const double2 *eq = h->has_eq ? h->d_eq : nullptr;
     const int64_t n16 = h->nk * 16;
-    if (h->precision == DMD_PREC_FP64) {
+    if (h->linear) {
+        launch_k4<double2, double, false, false, false, true>(h, h->d_rhoS, nullptr, h->d_hacc16);
+    } else if (h->precision == DMD_PREC_FP64) {
         if (dm1) launch_k4<double2, double, false, false, true>(h, h->d_rhoS, h->d_rho1S, h->d_hacc16);
         else launch_k4<double2, double, false, false>(h, h->d_rhoS, nullptr, h->d_hacc16);
     } else {
