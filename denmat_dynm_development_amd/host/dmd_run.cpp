@@ -12,10 +12,11 @@
 // the initial state is the Bpert first-order state (electron::
 // compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
-// B (Bx, By, Bz) adds the coherent H_BS = B . s.  The evolution is the
+// B (Bx, By, Bz) adds the coherent H_BS = B . s; alg_linearize replaces the
+// operator by its linearization at that f_eq (set_Lsc).  The evolution is the
 // on-device fixed-step RK4 with step min(tstep, ode_hmax), reports every
 // freq_measure steps of tstep.  Features whose physics this operator does not
-// carry (e-e, model e-i, linearized or sparse P, pump pulses, phenomenological
+// carry (e-e, model e-i, sparse P, pump pulses, phenomenological
 // relaxation, orbital or g-factor terms) stop the run with a message rather
 // than being ignored.
 //
@@ -47,7 +48,7 @@ struct cplx {
 struct Settings {
     bool restart = false;
     std::string scatt = "lindblad", picture = "interaction";
-    bool ddmdteq = false, expt = true;
+    bool ddmdteq = false, expt = true, linearize = false;
     double scale_eph = 1, scale_ei = 1;
     double t0 = 0, tend = 0, tstep = 0, hmax = 0;
     int freq_measure = 1;
@@ -92,7 +93,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
         throw std::runtime_error("alg_picture must be interaction or schrodinger");
     refuse(p, "alg_summode", 1, "the A-matrix (non-summode) form");
     refuse(p, "alg_semiclassical", 0, "the semiclassical (diagonal) mode");
-    refuse(p, "alg_linearize", 0, "the linearized operator");
+    s.linearize = p.get("alg_linearize", 0) != 0;
     refuse(p, "alg_linearize_dPee", 0, "the linearized e-e operator");
     refuse(p, "alg_Pin_is_sparse", 0, "sparse P input");
     refuse(p, "alg_sparseP", 0, "sparse P");
@@ -261,8 +262,9 @@ int run(const std::string &folder, bool dry)
     std::printf("T = %.10le mu = %.10le cell_size = %.10le dim = %d\n", T, mu, cell, dim);
     std::printf("t0 = %.10le tend = %.10le tstep = %.10le (RK4 step %.10le x %d) freq_measure = %d\n", s.t0, s.tend,
                 s.tstep, h_rk, n_sub, s.freq_measure);
-    std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_expt = %d alg_picture = %s alg_scatt = %s\n",
-                s.scale_eph, s.scale_ei, (int)s.ddmdteq, (int)s.expt, s.picture.c_str(), s.scatt.c_str());
+    std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_linearize = %d alg_expt = %d alg_picture = %s "
+                "alg_scatt = %s\n", s.scale_eph, s.scale_ei, (int)s.ddmdteq, (int)s.linearize, (int)s.expt,
+                s.picture.c_str(), s.scatt.c_str());
     std::printf("Bpert = %.10le %.10le %.10le B = %.10le %.10le %.10le impurities = %zu restart = %d\n", s.Bpert[0],
                 s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
@@ -321,15 +323,20 @@ int run(const std::string &folder, bool dry)
     electronphonon_mi355x<cplx> eph(ldbd, s.scatt, s.scale_eph, s.scissor);
     for (size_t iD = 0; iD < s.ni.size(); iD++) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
     eph.set_algorithm(s.expt, !interaction);
-    if (s.ddmdteq) {
+    if (s.ddmdteq || s.linearize) {  // DensityMatrix_Dynamics.h:57-65
         std::vector<double> feq((size_t)nk * nbd);
         const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
         const double muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;
         for (long k = 0; k < nk; k++)
             for (int i = 0; i < nbd; i++) feq[(size_t)k * nbd + i] = fermi(T, i >= nvd ? mue : muh, edm[(size_t)k * nbd + i]);
         double *rows[1] = {feq.data()};
-        eph.compute_ddmdt_eq(rows);
-        std::printf("ddmdt_eq at mu_e = %.12le mu_h = %.12le\n", mue, muh);
+        if (s.linearize) {
+            eph.set_Lsc(rows);  // ddmdt_eq of the linearized operator is zero: evolve_linear at f_eq
+            std::printf("linearized at mu_e = %.12le mu_h = %.12le\n", mue, muh);
+        } else {
+            eph.compute_ddmdt_eq(rows);
+            std::printf("ddmdt_eq at mu_e = %.12le mu_h = %.12le\n", mue, muh);
+        }
     }
     dmd_scatt *h = eph.handle();
     if (std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10) {  // electron.cpp:19-42

@@ -112,6 +112,16 @@ public:
         check(dmd_scatt_compute_ddmdt_eq(h_, f.data()), "compute_ddmdt_eq");
     }
 
+    // set_Lsc (ElectronPhonon_linear.cpp:3-36): alg_linearize at the nk x
+    // nb_expand occupations f_eq_expand (cut to the e-ph window as :4 does)
+    void set_Lsc(double **f_eq_expand)
+    {
+        std::vector<double> f((size_t)nk_glob * nb);
+        for (long k = 0; k < nk_glob; k++)
+            for (int i = 0; i < nb; i++) f[(size_t)k * nb + i] = f_eq_expand[0][(size_t)k * nb_expand + bStart + i];
+        check(dmd_scatt_set_Lsc(h_, f.data()), "set_Lsc");
+    }
+
     // ElectronPhonon.cpp:15-42: P is rebuilt only for model e-i and e-e
     // scattering, neither of which this operator carries
     void reset_scatt(bool, bool, complex_t **, complex_t **, double, double **) {}

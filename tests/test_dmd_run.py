@@ -87,7 +87,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
 
 @pytest.mark.parametrize("line,msg", [
     ("eeMode = Pee_update", "e-e scattering"),
-    ("alg_linearize = 1", "linearized operator"),
+    ("alg_linearize_dPee = 1", "linearized e-e operator"),
     ("impurity_density = 1e17\nimpMode = model_ionized", "ab_neutral"),
     ("alg_eph_need_hole = 1\nalg_eph_need_elec = 0", "file set"),
     ("pumpA0 = 0.002", "pump"),
