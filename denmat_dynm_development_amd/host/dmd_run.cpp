@@ -185,8 +185,9 @@ double mu_conserving(const std::vector<cplx> &dm, const std::vector<double> &edm
             emax = std::max(emax, edm[(size_t)k * nbd + i]);
         }
     double lo = emin - 60 * T, hi = emax + 60 * T;
-    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(lo)); it++) {
+    for (int it = 0; it < 200; it++) {  // to the resolution of double: drho = rho - f_eq needs every digit
         const double mu = 0.5 * (lo + hi);
+        if (mu <= lo || mu >= hi) break;
         double n = 0;
         for (long k = 0; k < nk; k++)
             for (int i = b0; i < b1; i++) {

@@ -105,7 +105,9 @@ def test_dmd_run_linearized_follows_the_oracle(tmp_path):
     d = tmp_path / "run"
     (d / "ldbd_data").mkdir(parents=True)
     scatt.write_ldbd(str(d / "ldbd_data"), p)
-    (d / "param.in").write_text("alg_linearize = 1\nBzpert = 2\ntstep = 2\ntend = 20\nfreq_measure = 2\n")
+    # the synthetic P has no detailed balance, so its linearization at f_eq has
+    # growing modes: a short run keeps the rounding differences unamplified
+    (d / "param.in").write_text("alg_linearize = 1\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n")
     r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-1500:], r.stderr[-1500:])
     assert r.returncode == 0 and "linearized at" in r.stdout
@@ -126,11 +128,11 @@ def test_dmd_run_linearized_follows_the_oracle(tmp_path):
     Lij, Lji = oracle.set_Lsc(k1, k2, P1, P2, feq)
     prefac = 2 * np.pi / info.nk_full
     obs_prefac = 1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3)
-    dt = 2 * FS
+    dt = 1 * FS
     rows = np.loadtxt(d / "sz_elec_tot.out")
     ref = [oracle.spin(s3[:, 2], e, f, rho, 0.0, 0, nb, obs_prefac)]
     y = rho.copy()
-    for i in range(10):
+    for i in range(4):
         t = i * dt
         g = lambda tt, yy: oracle.evolve_linear(k1, k2, Lij, Lji, e, yy, feq, t=tt, prefac=prefac)
         s1 = g(t, y)
@@ -140,9 +142,8 @@ def test_dmd_run_linearized_follows_the_oracle(tmp_path):
         y = y + dt / 6 * (s1 + 2 * s2 + 2 * s4a + s4)
         yy = y.reshape(nk, nb, nb)
         y = (0.5 * (yy + yy.conj().transpose(0, 2, 1))).reshape(nk, -1)
-        if (i + 1) % 2 == 0:
-            ref.append(oracle.spin(s3[:, 2], e, f, y, (i + 1) * dt, 0, nb, obs_prefac))
+        ref.append(oracle.spin(s3[:, 2], e, f, y, (i + 1) * dt, 0, nb, obs_prefac))
     ref = np.array(ref)
-    assert rows.shape == (6, 3)
+    assert rows.shape == (5, 3)
     print("S_z runner", rows[:, 1], "oracle", ref)
     assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
