@@ -112,6 +112,15 @@ class LdbdInfo(ctypes.Structure):
         return self.b_top_eph - self.b_bot_eph
 
 
+class OdeStats(ctypes.Structure):
+    """dmd_ode_stats (include/dmd_scatt.h)."""
+    _fields_ = [("steps", ctypes.c_int64), ("rejected", ctypes.c_int64), ("rhs_evals", ctypes.c_int64),
+                ("h_next", ctypes.c_double), ("last_err_ratio", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -147,6 +156,8 @@ PROTOTYPES = {
     "dmd_scatt_measure": [_vp, _d, _vp, _pd],
     "dmd_scatt_integrate_rk4": [_vp, _d, _d, _i64, _i, _vp, _vp],
     "dmd_scatt_integrate_rk4_batched": [_vp, _i, _d, _d, _i64, _i, _vp, _vp],
+    "dmd_scatt_rkf45_apply": [_vp, _pd, _d, _pd, _d, _d, _d, _d, _vp, ctypes.POINTER(OdeStats)],
+    "dmd_scatt_integrate_euler": [_vp, _d, _d, _i64, _vp],
     "dmd_scatt_vjp": [_vp, _d, _vp, _vp, _vp],
     "dmd_scatt_combine_P": [_vp, _i, _vp, _vp],
     "dmd_scatt_adjoint_rk4": [_vp, _d, _d, _i64, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],

@@ -615,3 +615,262 @@ extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, d
     DMD_HIP(hipStreamSynchronize(h->st));
     return DMD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Adaptive and first-order integrators of the reference's dm_dynamics:
+// evolve_gsl (DensityMatrix_Dynamics.h:189-225) drives GSL's rkf45 stepper
+// with the standard step-size control and hmax = tstep; evolve_euler (:177-187,
+// 227-234) takes rho += dt (ddmdt + ddmdt^+)/2 (DenMat.cpp:236-248).  Both
+// evaluate the same right-hand side as the RK4 path (coherent + scattering
+// window, compute() at :243-278) on the device.
+//
+// The RKF45 stepper is Fehlberg's embedded 4(5) pair (the published tableau);
+// the fourth-order solution advances the state and the difference to the
+// fifth-order one estimates the local error.  Step-size control follows the
+// standard rule documented for GSL's odeiv2 control: D_i = eps_abs + eps_rel
+// |y_i| per real component, r = max_i |err_i| / D_i; r > 1.1 rejects the step
+// and retries with h S r^(-1/q), r < 0.5 grows the next step to h S
+// r^(-1/(q+1)), S = 0.9, q = 4, each change limited to [1/5, 5]; the step
+// never exceeds hmax and the last one ends exactly on t1.
+// ---------------------------------------------------------------------------
+namespace dmd {
+
+struct Coef6 {
+    double c[6];
+};
+struct Ptr6 {
+    const double2 *p[6];
+};
+
+// kout = f(t + a h, y): the full right-hand side from the window ddm w
+__global__ void k_rhs_full(StageArgs A, const double2 *__restrict__ y, const double2 *__restrict__ w, double a,
+                           double2 *__restrict__ kout)
+{
+    const int n = A.nbd, n2 = n * n;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= A.nk * n2) return;
+    int64_t k = idx / n2;
+    int ij = (int)(idx - k * n2), i = ij / n, j = ij - i * n;
+    kout[idx] = rhs_elem(A, k, k, i, j, y + k * n2, w, clock_t_at(A.clk, a));
+}
+
+// out = y0 + h sum_j c_j k_j (m terms)
+__global__ void k_comb(int64_t n, const double2 *__restrict__ y0, double h, int m, Coef6 c, Ptr6 k,
+                       double2 *__restrict__ out)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    double2 s = make_double2(0, 0);
+    for (int j = 0; j < m; j++) {
+        const double2 v = k.p[j][idx];
+        s.x += c.c[j] * v.x;
+        s.y += c.c[j] * v.y;
+    }
+    out[idx] = make_double2(y0[idx].x + h * s.x, y0[idx].y + h * s.y);
+}
+
+// ynew = y0 + h sum b_j k_j; err = h sum e_j k_j; per-block max of |err| / D
+__global__ __launch_bounds__(256) void k_rkf_final(int64_t n, const double2 *__restrict__ y0, double h, Coef6 b,
+                                                   Coef6 e, Ptr6 k, double eps_abs, double eps_rel,
+                                                   double2 *__restrict__ ynew, double *__restrict__ part)
+{
+    __shared__ double red[256];
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double r = 0.0;
+    if (idx < n) {
+        double2 s = make_double2(0, 0), q = make_double2(0, 0);
+        for (int j = 0; j < 6; j++) {
+            const double2 v = k.p[j][idx];
+            s.x += b.c[j] * v.x; s.y += b.c[j] * v.y;
+            q.x += e.c[j] * v.x; q.y += e.c[j] * v.y;
+        }
+        const double2 y = make_double2(y0[idx].x + h * s.x, y0[idx].y + h * s.y);
+        ynew[idx] = y;
+        r = fmax(fabs(h * q.x) / (eps_abs + eps_rel * fabs(y.x)), fabs(h * q.y) / (eps_abs + eps_rel * fabs(y.y)));
+    }
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_max_final(int nparts, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double red[256];
+    double r = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) r = fmax(r, part[i]);
+    red[threadIdx.x] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// update_dm_euler (DenMat.cpp:236-248): y_ij += dt (k_ij + conj(k_ji)) / 2
+__global__ void k_euler(int64_t nk, int nbd, double dt, const double2 *__restrict__ kv, double2 *__restrict__ y)
+{
+    const int n2 = nbd * nbd;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nk * n2) return;
+    int64_t k = idx / n2;
+    int ij = (int)(idx - k * n2), i = ij / nbd, j = ij - i * nbd;
+    const double2 a = kv[idx], b = kv[k * n2 + j * nbd + i];
+    y[idx].x += dt * 0.5 * (a.x + b.x);
+    y[idx].y += dt * 0.5 * (a.y - b.y);
+}
+
+// adaptive scratch: y0, ytmp, ynew and six stage derivatives (9 states), the
+// error partials and the reduced ratio
+static int ensure_adaptive(dmd_scatt *h)
+{
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    int rc;
+    if ((rc = ensure_rk(h))) return rc;
+    if (h->ad_cap >= n) return DMD_OK;
+    dfree(h->d_ad); dfree(h->d_adpart);
+    h->d_ad = nullptr; h->d_adpart = nullptr;
+    h->ad_cap = 0;
+    if ((rc = dalloc_t(&h->d_ad, (size_t)9 * n, "adaptive integrator states"))) return rc;
+    if ((rc = dalloc_t(&h->d_adpart, (size_t)nblk(n, 256) + 1, "error partials"))) return rc;
+    h->ad_cap = n;
+    return DMD_OK;
+}
+
+// k = f(t + a hstep, y) with the clock at (t, hstep)
+static int full_rhs(dmd_scatt *h, const StageArgs &A, const double2 *y, double a, double2 *kout)
+{
+    int rc;
+    if ((rc = launch_rotate(h, y, nullptr, h->nbd, h->b0, a))) return rc;
+    if ((rc = launch_contract(h, false, a, h->d_w))) return rc;
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    hipLaunchKernelGGL(k_rhs_full, dim3(nblk(n, 256)), dim3(256), 0, h->st, A, y, h->d_w, a, kout);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+// one RKF45 attempt of size hs from (t, y0): ynew and the error ratio
+static int rkf45_attempt(dmd_scatt *h, double t, double hs, double eps_abs, double eps_rel, double *ratio)
+{
+    static const double c[6] = { 0.0, 1.0 / 4, 3.0 / 8, 12.0 / 13, 1.0, 1.0 / 2 };
+    static const double a[6][5] = {
+        { 0, 0, 0, 0, 0 },
+        { 1.0 / 4, 0, 0, 0, 0 },
+        { 3.0 / 32, 9.0 / 32, 0, 0, 0 },
+        { 1932.0 / 2197, -7200.0 / 2197, 7296.0 / 2197, 0, 0 },
+        { 439.0 / 216, -8.0, 3680.0 / 513, -845.0 / 4104, 0 },
+        { -8.0 / 27, 2.0, -3544.0 / 2565, 1859.0 / 4104, -11.0 / 40 },
+    };
+    static const double b4[6] = { 25.0 / 216, 0, 1408.0 / 2565, 2197.0 / 4104, -1.0 / 5, 0 };
+    static const double b5[6] = { 16.0 / 135, 0, 6656.0 / 12825, 28561.0 / 56430, -9.0 / 50, 2.0 / 55 };
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    double2 *y0 = h->d_ad, *ytmp = h->d_ad + n, *ynew = h->d_ad + 2 * n, *K[6];
+    for (int j = 0; j < 6; j++) K[j] = h->d_ad + (3 + j) * n;
+    int rc;
+    if ((rc = set_clock(h, t, hs, 0))) return rc;
+    StageArgs A = stage_args(h, hs);
+    const dim3 g(nblk(n, 256)), bl(256);
+    Ptr6 kp;
+    for (int j = 0; j < 6; j++) kp.p[j] = K[j];
+    for (int s = 0; s < 6; s++) {
+        const double2 *ys = y0;
+        if (s > 0) {
+            Coef6 cf{};
+            for (int j = 0; j < s; j++) cf.c[j] = a[s][j];
+            hipLaunchKernelGGL(k_comb, g, bl, 0, h->st, n, y0, hs, s, cf, kp, ytmp);
+            ys = ytmp;
+        }
+        if ((rc = full_rhs(h, A, ys, c[s], K[s]))) return rc;
+    }
+    Coef6 cb{}, ce{};
+    for (int j = 0; j < 6; j++) { cb.c[j] = b4[j]; ce.c[j] = b5[j] - b4[j]; }
+    hipLaunchKernelGGL(k_rkf_final, g, bl, 0, h->st, n, y0, hs, cb, ce, kp, eps_abs, eps_rel, ynew, h->d_adpart);
+    const int nparts = (int)nblk(n, 256);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, h->st, nparts, h->d_adpart, h->d_adpart + nparts);
+    DMD_HIP(hipGetLastError());
+    DMD_HIP(hipMemcpyAsync(ratio, h->d_adpart + nparts, 8, hipMemcpyDeviceToHost, h->st));
+    DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
+}
+
+}  // namespace dmd
+
+extern "C" int dmd_scatt_rkf45_apply(dmd_scatt *h, double *t, double t1, double *h_step, double hmin, double hmax,
+                                     double eps_abs, double eps_rel, double *rho, dmd_ode_stats *stats)
+{
+    if (!h || !t || !h_step || !rho || !(hmax > 0) || hmin < 0 || hmin > hmax || !(*h_step > 0) ||
+        !(eps_abs >= 0) || !(eps_rel >= 0) || (eps_abs == 0 && eps_rel == 0))
+        return fail(DMD_ERR_ARG, "dmd_scatt_rkf45_apply: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    int rc;
+    if ((rc = ensure_adaptive(h))) return rc;
+    if ((rc = tiles4_prepare(h))) return rc;
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    if ((rc = to_device(h, h->d_ad, rho, (size_t)n * 16))) return rc;
+    dmd_ode_stats st{};
+    double tc = *t, hs = std::min(*h_step, hmax);
+    constexpr double S = 0.9, Q = 4.0;
+    while (tc < t1) {
+        double htry = std::min(hs, t1 - tc);
+        bool final_step = htry >= t1 - tc;
+        double ratio = 0;
+        for (;;) {
+            if ((rc = rkf45_attempt(h, tc, htry, eps_abs, eps_rel, &ratio))) return rc;
+            st.rhs_evals += 6;
+            if (!(ratio == ratio)) return fail(DMD_ERR_STATE, "rkf45: the error estimate is not finite");
+            if (ratio > 1.1 && htry > hmin) {
+                double f = S * std::pow(ratio, -1.0 / Q);
+                htry = std::max(hmin, htry * std::max(0.2, f));
+                final_step = false;
+                st.rejected++;
+                continue;
+            }
+            break;
+        }
+        // accept: y0 <- ynew
+        DMD_HIP(hipMemcpyAsync(h->d_ad, h->d_ad + 2 * n, (size_t)n * 16, hipMemcpyDeviceToDevice, h->st));
+        tc = final_step ? t1 : tc + htry;
+        st.steps++;
+        st.last_err_ratio = ratio;
+        double hnext = htry;
+        if (ratio < 0.5) hnext = htry * std::min(5.0, std::max(1.0, S * std::pow(std::max(ratio, 1e-300), -1.0 / (Q + 1))));
+        // a step clamped to land on t1 does not shrink the suggestion
+        hs = std::min(hmax, final_step ? std::max(hs, hnext) : hnext);
+    }
+    DMD_HIP(hipStreamSynchronize(h->st));
+    if ((rc = from_device(h, rho, h->d_ad, (size_t)n * 16))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    *t = tc;
+    *h_step = hs;
+    st.h_next = hs;
+    if (stats) *stats = st;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int64_t n_steps, double *rho)
+{
+    if (!h || !rho || n_steps < 0) return fail(DMD_ERR_ARG, "dmd_scatt_integrate_euler: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    int rc;
+    if ((rc = ensure_adaptive(h))) return rc;
+    if ((rc = tiles4_prepare(h))) return rc;
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    double2 *y = h->d_ad, *kv = h->d_ad + 3 * n;
+    if ((rc = to_device(h, y, rho, (size_t)n * 16))) return rc;
+    for (int64_t it = 0; it < n_steps; it++) {
+        const double t = t0 + (double)it * dt;
+        if ((rc = set_clock(h, t, dt, 0))) return rc;
+        StageArgs A = stage_args(h, dt);
+        if ((rc = full_rhs(h, A, y, 0.0, kv))) return rc;
+        hipLaunchKernelGGL(k_euler, dim3(nblk(n, 256)), dim3(256), 0, h->st, h->nk, h->nbd, dt, kv, y);
+        DMD_HIP(hipGetLastError());
+    }
+    DMD_HIP(hipStreamSynchronize(h->st));
+    if ((rc = from_device(h, rho, y, (size_t)n * 16))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
+}

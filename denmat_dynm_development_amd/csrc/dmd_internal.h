@@ -177,6 +177,10 @@ struct dmd_scatt {
     // linearized operator (alg_linearize): the P store holds Lscij | Lscji
     bool linear = false;
     double *d_flin = nullptr;            // f_eq (nk x nb) the operator was linearized at
+    // adaptive / Euler integrators: 9 states (y0, ytmp, ynew, k1..k6) and the error partials
+    double2 *d_ad = nullptr;
+    double *d_adpart = nullptr;
+    int64_t ad_cap = 0;
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;

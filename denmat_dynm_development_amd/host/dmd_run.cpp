@@ -14,8 +14,10 @@
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
 // B (Bx, By, Bz) adds the coherent H_BS = B . s; alg_linearize replaces the
 // operator by its linearization at that f_eq (set_Lsc).  The evolution is the
-// on-device fixed-step RK4 with step min(tstep, ode_hmax), reports every
-// freq_measure steps of tstep.  Features whose physics this operator does not
+// on-device integrator alg_ode_method picks: rkf45 (the reference's default,
+// adaptive with ode_hstart / ode_hmin / ode_hmax / ode_epsabs), euler, or
+// fixed-step rk4 with step min(tstep, ode_hmax); reports every freq_measure
+// steps of tstep.  Features whose physics this operator does not
 // carry (e-e, model e-i, sparse P, pump pulses, phenomenological
 // relaxation, orbital or g-factor terms) stop the run with a message rather
 // than being ignored.
@@ -50,7 +52,8 @@ struct Settings {
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
     double scale_eph = 1, scale_ei = 1;
-    double t0 = 0, tend = 0, tstep = 0, hmax = 0;
+    double t0 = 0, tend = 0, tstep = 0, hmax = 0, hstart = 0, hmin = 0, epsabs = 1e-8;
+    std::string ode = "rkf45";
     int freq_measure = 1;
     double mu = 0, carrier_density = 0, scissor = 0, temperature = 0;
     double Bpert[3] = {0, 0, 0}, B[3] = {0, 0, 0};
@@ -118,8 +121,10 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     if (need_hole && need_elec) throw std::runtime_error("alg_eph_need_elec and alg_eph_need_hole: one carrier set per run");
     if (need_hole != (info.kind == 1))
         throw std::runtime_error("alg_eph_need_hole does not match the ldbd_data file set (first line of ldbd_size.dat)");
-    const std::string ode = p.str("alg_ode_method", "rkf45");
-    if (ode != "rkf45" && ode != "euler") throw std::runtime_error("alg_ode_method must be rkf45 or euler");
+    // parameters.cpp:360-367; rk4 (fixed step min(tstep, ode_hmax)) is this program's addition
+    s.ode = p.str("alg_ode_method", "rkf45");
+    if (s.ode != "rkf45" && s.ode != "euler" && s.ode != "rk4")
+        throw std::runtime_error("alg_ode_method must be rkf45, euler or rk4");
     s.temperature = info.temperature;  // code = jdftx: T of ldbd_size.dat (parameters.cpp:7-23)
     s.scissor = p.get("scissor", 0, eV);
     s.mu = p.get("mu", info.mu / eV, eV);
@@ -142,6 +147,11 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     s.tend = p.get("tend", 0., fs());
     s.tstep = p.get("tstep", 1., fs());
     s.hmax = p.get("ode_hmax", s.tstep / fs(), fs());
+    s.hstart = p.get("ode_hstart", 1e-3, fs());
+    s.hmin = p.get("ode_hmin", 0, fs());
+    s.epsabs = p.get("ode_epsabs", 1e-8);
+    if (s.hstart <= 0 || s.hmin < 0 || s.hmin > s.hmax || s.epsabs <= 0)
+        throw std::runtime_error("ode_hstart > 0, 0 <= ode_hmin <= ode_hmax and ode_epsabs > 0 are required");
     s.freq_measure = (int)p.get("freq_measure", 1);
     if (s.tstep <= 0 || s.hmax <= 0 || s.freq_measure < 1) throw std::runtime_error("tstep, ode_hmax > 0, freq_measure >= 1");
     return s;
@@ -261,8 +271,12 @@ int run(const std::string &folder, bool dry)
     std::printf("dmd_run: nk = %ld nb_dm = %d (nv_dm = %d) e-ph bands [%d, %d) of the density matrix, %ld stored pairs\n",
                 nk, nbd, nvd, info.b_bot_eph - info.b_bot_dm, info.b_top_eph - info.b_bot_dm, (long)info.nkpair);
     std::printf("T = %.10le mu = %.10le cell_size = %.10le dim = %d\n", T, mu, cell, dim);
-    std::printf("t0 = %.10le tend = %.10le tstep = %.10le (RK4 step %.10le x %d) freq_measure = %d\n", s.t0, s.tend,
-                s.tstep, h_rk, n_sub, s.freq_measure);
+    std::printf("t0 = %.10le tend = %.10le tstep = %.10le freq_measure = %d alg_ode_method = %s\n", s.t0, s.tend, s.tstep,
+                s.freq_measure, s.ode.c_str());
+    if (s.ode == "rk4") std::printf("RK4 step %.10le x %d per tstep\n", h_rk, n_sub);
+    if (s.ode == "rkf45")
+        std::printf("rkf45: ode_hstart = %.10le ode_hmin = %.10le ode_hmax = %.10le ode_epsabs = %lg\n", s.hstart, s.hmin,
+                    s.hmax, s.epsabs);
     std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_linearize = %d alg_expt = %d alg_picture = %s "
                 "alg_scatt = %s\n", s.scale_eph, s.scale_ei, (int)s.ddmdteq, (int)s.linearize, (int)s.expt,
                 s.picture.c_str(), s.scatt.c_str());
@@ -387,13 +401,30 @@ int run(const std::string &folder, bool dry)
         }
     };
     if (!s.restart) report(0);
-    double ti = t;
+    // the time loop of evolve_gsl / evolve_euler (DensityMatrix_Dynamics.h:177-225)
+    double ti = t, hstep = s.hstart;
+    dmd_ode_stats st{}, tot{};
     for (int it = 1; t < s.tend; it++) {
         ti += s.tstep;
-        check(dmd_scatt_integrate_rk4(h, t, h_rk, n_sub, 0, (double *)dm.data(), nullptr), "RK4");
+        if (s.ode == "rkf45") {
+            check(dmd_scatt_rkf45_apply(h, &t, ti, &hstep, s.hmin, s.hmax, s.epsabs, 0.0, (double *)dm.data(), &st),
+                  "rkf45");
+            tot.steps += st.steps;
+            tot.rejected += st.rejected;
+            tot.rhs_evals += st.rhs_evals;
+        } else if (s.ode == "euler") {
+            check(dmd_scatt_integrate_euler(h, t, s.tstep, 1, (double *)dm.data()), "euler");
+            tot.rhs_evals += 1;
+        } else {
+            check(dmd_scatt_integrate_rk4(h, t, h_rk, n_sub, 0, (double *)dm.data(), nullptr), "RK4");
+            tot.rhs_evals += 4 * n_sub;
+        }
         t = ti;
         report(it);
     }
+    std::printf("dmd_run: %lld right-hand-side evaluations", (long long)tot.rhs_evals);
+    if (s.ode == "rkf45") std::printf(" (%lld steps, %lld rejected)", (long long)tot.steps, (long long)tot.rejected);
+    std::printf("\n");
     for (auto &r : reps) std::fclose(r.fp);
     std::printf("dmd_run: reached t = %.10le\n", t);
     return 0;

@@ -242,6 +242,27 @@ class ScatteringOperator:
              obs.ctypes.data)
         return rho, obs[:nobs]
 
+    def rkf45_apply(self, rho, t, t1, h_step, hmin=0.0, hmax=None, eps_abs=1e-8, eps_rel=0.0):
+        """One gsl_odeiv2_driver_apply of evolve_gsl (rkf45, standard control)
+        from t to t1; returns (rho(t1), t1, next step suggestion, stats)."""
+        y = np.array(rho, dtype=np.complex128, copy=True).reshape(self.nk, self.nb_dm * self.nb_dm) \
+            if not _is_tensor(rho) else rho.clone()
+        tt = ctypes.c_double(float(t))
+        hh = ctypes.c_double(float(h_step))
+        st = _lib.OdeStats()
+        py, ky = _ptr(y, np.complex128)
+        call("dmd_scatt_rkf45_apply", self._h, ctypes.byref(tt), float(t1), ctypes.byref(hh), float(hmin),
+             float(hmax if hmax is not None else t1 - t), float(eps_abs), float(eps_rel), py, ctypes.byref(st))
+        return y, tt.value, hh.value, st.as_dict()
+
+    def integrate_euler(self, rho, t0, dt, n_steps):
+        """evolve_euler: n_steps of rho += dt (ddmdt + ddmdt^+)/2 from t0."""
+        y = np.array(rho, dtype=np.complex128, copy=True).reshape(self.nk, self.nb_dm * self.nb_dm) \
+            if not _is_tensor(rho) else rho.clone()
+        py, ky = _ptr(y, np.complex128)
+        call("dmd_scatt_integrate_euler", self._h, float(t0), float(dt), int(n_steps), py)
+        return y
+
     def integrate_rk4_batched(self, rho0_B, t0, dt, n_steps, report_every=0):
         B = rho0_B.shape[0]
         rho = np.array(rho0_B, dtype=np.complex128, copy=True).reshape(B, self.nk, self.nb_dm * self.nb_dm)

@@ -187,6 +187,30 @@ int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps,
 int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
                                     int report_every, double *rho_inout, double *obs_out);
 
+/* Statistics of an adaptive integration call. */
+typedef struct {
+    int64_t steps;          /* accepted steps                                   */
+    int64_t rejected;       /* rejected attempts                                */
+    int64_t rhs_evals;      /* right-hand-side evaluations                      */
+    double h_next;          /* suggested next step (also returned in *h_step)   */
+    double last_err_ratio;  /* max |err_i| / D_i of the last accepted step      */
+} dmd_ode_stats;
+/* gsl_odeiv2_driver_apply of evolve_gsl (DensityMatrix_Dynamics.h:189-225:
+ * rkf45 stepper, standard control with a_y = 1, a_dydt = 0): integrates the
+ * same right-hand side as dmd_scatt_integrate_rk4 from *t to t1 with
+ * Fehlberg's embedded 4(5) pair on the device, D_i = eps_abs + eps_rel |y_i|
+ * per real component, steps in [hmin, hmax] (ode_hmin, ode_hmax = tstep,
+ * parameters.cpp:360-367), the last one ending exactly on t1.  *h_step carries
+ * the controller's suggestion between calls as the GSL driver's state does
+ * (start with ode_hstart); on return *t = t1.  rho_inout: nk x nb_dm^2, host
+ * or device.  GSL's own rkf45 trajectories are not reproduced bit for bit
+ * (its source is not part of the reference). */
+int dmd_scatt_rkf45_apply(dmd_scatt *h, double *t, double t1, double *h_step, double hmin, double hmax,
+                          double eps_abs, double eps_rel, double *rho_inout, dmd_ode_stats *stats);
+/* evolve_euler (DensityMatrix_Dynamics.h:177-187,227-234): n_steps of
+ * rho += dt (ddmdt + ddmdt^+)/2 (update_dm_euler, DenMat.cpp:236-248) from t0. */
+int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int64_t n_steps, double *rho_inout);
+
 /* ---- adjoint ------------------------------------------------------------------ */
 /* Vector-Jacobian product of evolve (evolve_eph.cpp:34-166) at the state rho
  * (Hermitian, nk x nb^2) and time t: g = (d ddmdt / d rho)^T v under the real

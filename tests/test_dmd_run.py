@@ -55,6 +55,7 @@ def test_dry_run_resolves_the_reference_gaas_settings(tmp_path):
     m = re.search(r"Bpert = (\S+) (\S+) (\S+)", out)
     assert float(m.group(3)) == pytest.approx(4.25438e-07, rel=2e-6) and float(m.group(1)) == 0
     assert "alg_ddmdteq = 1" in out and "alg_picture = interaction" in out
+    assert "alg_ode_method = rkf45" in out and "ode_epsabs = 1e-08" in out  # the reference's defaults
     assert "e-ph bands [4, 6)" in out and "nb_dm = 6 (nv_dm = 4)" in out
     assert 'key "pumpE" is not read' in out  # said, not silently dropped
 
@@ -73,11 +74,12 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
         "scale_scatt = 0.7",
         "scale_eph = 0.9",
         "alg_picture = schrodinger",
+        "alg_ode_method = rk4",
     ]) + "\n"
     rc, out, err = _dry(_gaas_folder(tmp_path, text))
     assert rc == 0, err
     fs = 41.34137417094296
-    assert abs(_num(out, "tstep") / (3 * fs) - 1) < 1e-10 and "x 3)" in out  # RK4 step 1 fs x 3
+    assert abs(_num(out, "tstep") / (3 * fs) - 1) < 1e-10 and "x 3 per tstep" in out  # RK4 step 1 fs x 3
     assert _num(out, "freq_measure") == 2
     assert abs(_num(out, "mu") / (0.05 / 27.21138505) - 1) < 1e-10
     assert _num(out, "scale_eph") == 0.9 and _num(out, "scale_ei") == 0.7
@@ -91,7 +93,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("impurity_density = 1e17\nimpMode = model_ionized", "ab_neutral"),
     ("alg_eph_need_hole = 1\nalg_eph_need_elec = 0", "file set"),
     ("pumpA0 = 0.002", "pump"),
-    ("alg_ode_method = rk2", "rkf45 or euler"),
+    ("alg_ode_method = rk2", "rkf45, euler or rk4"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))

@@ -1,0 +1,93 @@
+"""The adaptive RKF45 driver and the Euler stepper of the reference's
+dm_dynamics (evolve_gsl with rkf45, evolve_euler) on the device: the
+Fehlberg tableau pinned by the closed form of one step on -i[e, rho] (the
+fourth-order stability polynomial 1 + z + z^2/2 + z^3/6 + z^4/24 + z^5/104
+and the embedded error (R5 - R4)(z), R5 = ... + z^5/120 + z^6/2080), Euler's
+1 + z, the adaptive solution of a Lindblad system against a converged oracle
+RK4, and the step control's response to the tolerance."""
+import numpy as np
+import pytest
+
+from helpers import FS, random_P, random_energies, random_pairs, random_rho, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    from denmat_dynm_development_amd import _lib, scatt
+    from oracle import oracle
+    return _lib, scatt, oracle
+
+
+def _coherent_only(scatt, nk, nb, rng):
+    e = random_energies(nk, nb, rng, emax=0.02)
+    op = scatt.ScatteringOperator(nk, nb, np.zeros(0, np.uint64), np.zeros(0, np.uint64), e)
+    op.upload_P(np.zeros((0, nb ** 4), np.complex128), np.zeros((0, nb ** 4), np.complex128))
+    op.set_flags(expt=False, schrodinger=True)  # the right-hand side is -i[e, rho] alone
+    return op, e
+
+
+def test_rkf45_one_step_is_the_fehlberg_polynomial():
+    _lib, scatt, oracle = _mods()
+    print("case rkf45 closed form", flush=True)
+    rng = np.random.default_rng(81)
+    nk, nb = 6, 3
+    op, e = _coherent_only(scatt, nk, nb, rng)
+    with op:
+        rho = random_rho(nk, nb, rng, scale=0.3, diag=0.4)
+        h = 7.0 * FS
+        y, t, hn, st = op.rkf45_apply(rho, 0.0, h, h, 0.0, h, 1.0, 0.0)  # loose tolerance: one accepted step
+        assert st["steps"] == 1 and st["rejected"] == 0 and st["rhs_evals"] == 6 and t == h
+        z = (-1j * (e[:, :, None] - e[:, None, :]) * h).reshape(nk, nb * nb)
+        R4 = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24 + z ** 5 / 104
+        R5 = 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24 + z ** 5 / 120 + z ** 6 / 2080
+        assert np.abs(y - R4 * rho).max() < 1e-14
+        err = (R5 - R4) * rho
+        ratio = max(np.abs(err.real).max(), np.abs(err.imag).max()) / 1.0
+        assert st["last_err_ratio"] == pytest.approx(ratio, rel=1e-6)
+        # a planted mistake: the fifth-order weights would give R5, which this is not
+        assert np.abs(y - R5 * rho).max() > 1e3 * np.abs(y - R4 * rho).max()
+        # Euler: 1 + z per step
+        ye = op.integrate_euler(rho, 0.0, h, 1)
+        assert np.abs(ye - (1 + z) * rho).max() < 1e-15
+
+
+def test_rkf45_adaptive_follows_converged_oracle_and_tolerance():
+    _lib, scatt, oracle = _mods()
+    print("case rkf45 adaptive", flush=True)
+    rng = np.random.default_rng(82)
+    nk, nb = 24, 2
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    P1, P2 = 0.2 * P1, 0.2 * P2
+    prefac = 2 * np.pi / nk
+    f = np.sort(rng.uniform(0.1, 0.5, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho0 = np.zeros((nk, nb, nb), np.complex128)
+    rho0[:, np.arange(nb), np.arange(nb)] = f
+    rho0 = rho0.reshape(nk, -1) + random_rho(nk, nb, rng, scale=0.02, diag=0.0)
+    T = 200 * FS
+    ref, _ = oracle.rk4(k1, k2, P1, P2, e, rho0, T / 4000, 4000, prefac=prefac)
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        errs, steps = [], []
+        for eps in (1e-6, 1e-9, 1e-12):
+            y, t, hn, st = op.rkf45_apply(rho0, 0.0, T, 1e-3 * FS, 0.0, 50 * FS, eps, 0.0)
+            assert t == T and st["last_err_ratio"] <= 1.1
+            errs.append(np.abs(y - ref).max())
+            steps.append(st["steps"])
+            print(f"eps {eps:g}: {st}, max |y - ref| {errs[-1]:.3e}")
+        assert steps[0] < steps[1] < steps[2]
+        assert errs[2] < errs[1] < errs[0] and errs[2] < 1e-10
+        # the driver state carries over: continuing from T to 2T with the suggestion
+        y2, t2, hn2, st2 = op.rkf45_apply(y, T, 2 * T, hn, 0.0, 50 * FS, 1e-12, 0.0)
+        ref2, _ = oracle.rk4(k1, k2, P1, P2, e, ref, T / 4000, 4000, prefac=prefac, t0=T)
+        assert t2 == 2 * T and np.abs(y2 - ref2).max() < 1e-10
+        # Euler against a numpy Euler with the oracle's right-hand side
+        dt, n = 2 * FS, 15
+        ye = op.integrate_euler(rho0, 0.0, dt, n)
+        yr = rho0.copy()
+        for i in range(n):
+            k = oracle.evolve(k1, k2, P1, P2, e, yr, t=i * dt, prefac=prefac).reshape(nk, nb, nb)
+            yr = yr + 0.5 * dt * (k + k.conj().transpose(0, 2, 1)).reshape(nk, -1)
+        assert rel_err(ye, yr) < 1e-12
