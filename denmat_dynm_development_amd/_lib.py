@@ -158,6 +158,7 @@ PROTOTYPES = {
     "dmd_scatt_integrate_rk4_batched": [_vp, _i, _d, _d, _i64, _i, _vp, _vp],
     "dmd_scatt_rkf45_apply": [_vp, _pd, _d, _pd, _d, _d, _d, _d, _vp, ctypes.POINTER(OdeStats)],
     "dmd_scatt_integrate_euler": [_vp, _d, _d, _i64, _vp],
+    "dmd_scatt_compute_rhs": [_vp, _d, _vp, _vp],
     "dmd_scatt_vjp": [_vp, _d, _vp, _vp, _vp],
     "dmd_scatt_combine_P": [_vp, _i, _vp, _vp],
     "dmd_scatt_adjoint_rk4": [_vp, _d, _d, _i64, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],

@@ -874,3 +874,22 @@ extern "C" int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int
     DMD_HIP(hipStreamSynchronize(h->st));
     return DMD_OK;
 }
+
+// dm_dynamics::compute (DensityMatrix_Dynamics.h:243-278): the whole
+// right-hand side at t (coherent + scattering window), as the integrators use it
+extern "C" int dmd_scatt_compute_rhs(dmd_scatt *h, double t, const double *rho, double *ddmdt)
+{
+    if (!h || !rho || !ddmdt) return fail(DMD_ERR_ARG, "dmd_scatt_compute_rhs: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    int rc;
+    if ((rc = ensure_adaptive(h))) return rc;
+    if ((rc = tiles4_prepare(h))) return rc;
+    const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
+    if ((rc = to_device(h, h->d_ad, rho, (size_t)n * 16))) return rc;
+    if ((rc = set_clock(h, t, 0.0, 0))) return rc;
+    StageArgs A = stage_args(h, 0.0);
+    if ((rc = full_rhs(h, A, h->d_ad, 0.0, h->d_ad + 3 * n))) return rc;
+    if ((rc = from_device(h, ddmdt, h->d_ad + 3 * n, (size_t)n * 16))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
+}

@@ -2,7 +2,7 @@
 // (src_v4.5.7/main.cpp:39-97 with dm_dynamics, DensityMatrix_Dynamics.h:31-77,
 // 177-345) for the Lindblad scattering operator of libdmd_scatt:
 //
-//   ldbd_data/ + param.in  ->  s{x,y,z}_{elec,hole}_tot.out, restart/
+//   ldbd_data/ + param.in  ->  s{x,y,z}_{elec,hole}_tot.out, tau_s*_tot.out, restart/
 //
 // run in the folder that holds param.in and ldbd_data/, as the reference is.
 // What it reads: lattice (ldbd_R.dat), band energies and window, T and mu
@@ -48,13 +48,13 @@ struct cplx {
 };
 
 struct Settings {
-    bool restart = false;
+    bool restart = false, tau_only = false;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
     double scale_eph = 1, scale_ei = 1;
     double t0 = 0, tend = 0, tstep = 0, hmax = 0, hstart = 0, hmin = 0, epsabs = 1e-8;
     std::string ode = "rkf45";
-    int freq_measure = 1;
+    int freq_measure = 1, freq_compute_tau = 10;
     double mu = 0, carrier_density = 0, scissor = 0, temperature = 0;
     double Bpert[3] = {0, 0, 0}, B[3] = {0, 0, 0};
     std::vector<double> ni;  // impurity densities in bohr^-dim, ab_neutral
@@ -83,6 +83,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     if (code != "jdftx") throw std::runtime_error("code = " + code + ": only ldbd_data input (code = jdftx) is read");
     if (p.str("material_model", "none") != "none") throw std::runtime_error("material_model: model materials are not read");
     s.restart = p.get("restart", 0) != 0;
+    s.tau_only = p.get("compute_tau_only", 0) != 0;  // parameters.cpp:73
     if (p.get("alg_scatt_enable", 1) == 0 || p.get("alg_eph_enable", 1) == 0)
         throw std::runtime_error("alg_scatt_enable / alg_eph_enable = 0: this runner drives the scattering operator");
     const double scale_scatt = p.get("scale_scatt", 1.);
@@ -109,7 +110,6 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     refuse(p, "needL", 0, "the orbital angular momentum term");
     refuse(p, "gfac_normal_dist", 0, "k-resolved g factors");
     refuse(p, "scale_Ez", 0, "the electric-field Hamiltonian");
-    refuse(p, "compute_tau_only", 0, "compute_tau_only");
     refuse(p, "carrier_density_means_excess_density", 0, "excess-density mu");
     if (p.str("eeMode", "none") != "none") throw std::runtime_error("eeMode: e-e scattering is not carried by this operator");
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
@@ -153,7 +153,9 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     if (s.hstart <= 0 || s.hmin < 0 || s.hmin > s.hmax || s.epsabs <= 0)
         throw std::runtime_error("ode_hstart > 0, 0 <= ode_hmin <= ode_hmax and ode_epsabs > 0 are required");
     s.freq_measure = (int)p.get("freq_measure", 1);
-    if (s.tstep <= 0 || s.hmax <= 0 || s.freq_measure < 1) throw std::runtime_error("tstep, ode_hmax > 0, freq_measure >= 1");
+    s.freq_compute_tau = (int)p.get("freq_compute_tau", p.get("freq_measure_ene", 10));  // parameters.cpp:324-326
+    if (s.tstep <= 0 || s.hmax <= 0 || s.freq_measure < 1 || s.freq_compute_tau < 1)
+        throw std::runtime_error("tstep, ode_hmax > 0, freq_measure, freq_compute_tau >= 1");
     return s;
 }
 
@@ -273,6 +275,7 @@ int run(const std::string &folder, bool dry)
     std::printf("T = %.10le mu = %.10le cell_size = %.10le dim = %d\n", T, mu, cell, dim);
     std::printf("t0 = %.10le tend = %.10le tstep = %.10le freq_measure = %d alg_ode_method = %s\n", s.t0, s.tend, s.tstep,
                 s.freq_measure, s.ode.c_str());
+    std::printf("freq_compute_tau = %d compute_tau_only = %d\n", s.freq_compute_tau, s.tau_only ? 1 : 0);
     if (s.ode == "rk4") std::printf("RK4 step %.10le x %d per tstep\n", h_rk, n_sub);
     if (s.ode == "rkf45")
         std::printf("rkf45: ode_hstart = %.10le ode_hmin = %.10le ode_hmax = %.10le ode_epsabs = %lg\n", s.hstart, s.hmin,
@@ -400,11 +403,79 @@ int run(const std::string &folder, bool dry)
             std::fflush(r.fp);
         }
     };
+    // instantaneous relaxation times (report_tau, DensityMatrix_Dynamics.h:346-354;
+    // observable.h:254-274,392-402): tau = -S / dS/dt from the right-hand side
+    // at t, without and with the precession term of the interaction picture,
+    // and by a forward difference over dt_tiny = min(ode_hstart, tstep / 10)
+    std::vector<Report> taus;
+    const double Bp = std::sqrt(s.Bpert[0] * s.Bpert[0] + s.Bpert[1] * s.Bpert[1] + s.Bpert[2] * s.Bpert[2]);
+    for (auto &r : reps) {
+        if (Bp > 1e-10 && s.Bpert[r.comp] == 0) continue;
+        taus.push_back({"tau_" + r.name, r.comp, r.i0, r.i1});
+        const std::string fn = base + "/" + taus.back().name + "_tot.out";
+        struct stat st2;
+        const bool exists = stat(fn.c_str(), &st2) == 0;
+        taus.back().fp = std::fopen(fn.c_str(), "a");
+        if (!taus.back().fp) throw std::runtime_error("cannot open " + fn);
+        if (!exists) std::fprintf(taus.back().fp, "#time(au), tau by FD, tau without precession, tau\n");
+    }
+    std::vector<cplx> ddm((size_t)nk * nbd2), dmp((size_t)nk * nbd2), s2((size_t)nk * nbd2);
+    const std::vector<double> fzero((size_t)nk * nbd, 0.0);
+    const double dt_tiny = std::min(s.hstart, s.tstep / 10);
+    auto report_tau = [&](int it) {
+        if (it > 0 && (it - 1) % s.freq_compute_tau != 0) return;
+        if (taus.empty()) return;
+        check(dmd_scatt_compute_rhs(h, t, (const double *)dm.data(), (double *)ddm.data()), "compute");
+        for (size_t q = 0; q < dm.size(); q++) dmp[q] = {dm[q].x + ddm[q].x * dt_tiny, dm[q].y + ddm[q].y * dt_tiny};
+        for (auto &r : taus) {
+            for (long k = 0; k < nk; k++) {
+                std::memcpy(&sc[(size_t)k * nbd2], &smat[((size_t)k * 3 + r.comp) * nbd2], nbd2 * sizeof(cplx));
+                for (int i = 0; i < nbd; i++)
+                    for (int b2 = 0; b2 < nbd; b2++) {  // 2i e_i s_ib: Re(2i e_i c) = -2 Im(e_i c)
+                        const cplx v = sc[(size_t)k * nbd2 + i * nbd + b2];
+                        const double ei = edm[(size_t)k * nbd + i];
+                        s2[(size_t)k * nbd2 + i * nbd + b2] = {-2 * ei * v.y, 2 * ei * v.x};
+                    }
+            }
+            double tot = 0, dot = 0, term2 = 0, tplus = 0;
+            check(dmd_scatt_set_observable(h, (const double *)sc.data(), fdm.data(), r.i0, r.i1, prefac_obs), "observable");
+            check(dmd_scatt_measure(h, t, (const double *)dm.data(), &tot), "measure");
+            check(dmd_scatt_measure(h, t + dt_tiny, (const double *)dmp.data(), &tplus), "measure");
+            check(dmd_scatt_set_observable(h, (const double *)sc.data(), fzero.data(), r.i0, r.i1, prefac_obs), "observable");
+            check(dmd_scatt_measure(h, t, (const double *)ddm.data(), &dot), "measure");
+            if (interaction) {
+                check(dmd_scatt_set_observable(h, (const double *)s2.data(), fdm.data(), r.i0, r.i1, prefac_obs),
+                      "observable");
+                check(dmd_scatt_measure(h, t, (const double *)dm.data(), &term2), "measure");
+            }
+            const double tau_fd = -tot * dt_tiny / (tplus - tot);
+            const double psu = ps();
+            if (interaction) {
+                if (std::fabs(t) < 1e-30)
+                    std::fprintf(r.fp, " 0.0000000000e+01 fs: %14.7le %14.7le %14.7le ps\n", tau_fd / psu, -tot / dot / psu,
+                                 -tot / (dot + term2) / psu);
+                else
+                    std::fprintf(r.fp, "%17.10le fs: %14.7le %14.7le %14.7le ps\n", t / fs(), tau_fd / psu, -tot / dot / psu,
+                                 -tot / (dot + term2) / psu);
+            } else {
+                if (std::fabs(t) < 1e-30) std::fprintf(r.fp, " 0.0000000000e+01 fs: %14.7le ps\n", -tot / dot / psu);
+                else std::fprintf(r.fp, "%17.10le fs: %14.7le ps\n", t / fs(), -tot / dot / psu);
+            }
+            std::fflush(r.fp);
+        }
+    };
     if (!s.restart) report(0);
+    if (s.tau_only) {  // DensityMatrix_Dynamics.h:75 and main.cpp:79: taus at t0, no evolution
+        report_tau(0);
+        for (auto &r : reps) std::fclose(r.fp);
+        for (auto &r : taus) std::fclose(r.fp);
+        return 0;
+    }
     // the time loop of evolve_gsl / evolve_euler (DensityMatrix_Dynamics.h:177-225)
     double ti = t, hstep = s.hstart;
     dmd_ode_stats st{}, tot{};
     for (int it = 1; t < s.tend; it++) {
+        report_tau(it);
         ti += s.tstep;
         if (s.ode == "rkf45") {
             check(dmd_scatt_rkf45_apply(h, &t, ti, &hstep, s.hmin, s.hmax, s.epsabs, 0.0, (double *)dm.data(), &st),
@@ -426,6 +497,7 @@ int run(const std::string &folder, bool dry)
     if (s.ode == "rkf45") std::printf(" (%lld steps, %lld rejected)", (long long)tot.steps, (long long)tot.rejected);
     std::printf("\n");
     for (auto &r : reps) std::fclose(r.fp);
+    for (auto &r : taus) std::fclose(r.fp);
     std::printf("dmd_run: reached t = %.10le\n", t);
     return 0;
 }

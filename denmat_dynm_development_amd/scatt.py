@@ -255,6 +255,16 @@ class ScatteringOperator:
              float(hmax if hmax is not None else t1 - t), float(eps_abs), float(eps_rel), py, ctypes.byref(st))
         return y, tt.value, hh.value, st.as_dict()
 
+    def compute_rhs(self, t, rho, out=None):
+        """The whole right-hand side at t (dm_dynamics::compute)."""
+        shape = (self.nk, self.nb_dm * self.nb_dm)
+        if out is None:
+            out = _empty_like_state(rho, shape)
+        pr, kr = _ptr(rho, np.complex128)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_compute_rhs", self._h, float(t), pr, po)
+        return out
+
     def integrate_euler(self, rho, t0, dt, n_steps):
         """evolve_euler: n_steps of rho += dt (ddmdt + ddmdt^+)/2 from t0."""
         y = np.array(rho, dtype=np.complex128, copy=True).reshape(self.nk, self.nb_dm * self.nb_dm) \

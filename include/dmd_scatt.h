@@ -187,6 +187,10 @@ int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps,
 int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, double dt, int64_t n_steps,
                                     int report_every, double *rho_inout, double *obs_out);
 
+/* dm_dynamics::compute (DensityMatrix_Dynamics.h:243-278): the whole
+ * right-hand side d rho / dt at t (coherent term and scattering window, as
+ * the integrators evaluate it); rho, ddmdt: nk x nb_dm^2, host or device. */
+int dmd_scatt_compute_rhs(dmd_scatt *h, double t, const double *rho, double *ddmdt);
 /* Statistics of an adaptive integration call. */
 typedef struct {
     int64_t steps;          /* accepted steps                                   */

@@ -58,6 +58,7 @@ def test_dry_run_resolves_the_reference_gaas_settings(tmp_path):
     assert "alg_ode_method = rkf45" in out and "ode_epsabs = 1e-08" in out  # the reference's defaults
     assert "e-ph bands [4, 6)" in out and "nb_dm = 6 (nv_dm = 4)" in out
     assert 'key "pumpE" is not read' in out  # said, not silently dropped
+    assert _num(out, "freq_compute_tau") == 10 and _num(out, "compute_tau_only") == 0  # freq_measure_ene = 10
 
 
 def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
@@ -69,6 +70,8 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
         "ode_hmax = 1",
         "tend = 30",
         "freq_measure = 2",
+        "freq_measure_ene = 4",
+        "compute_tau_only = 1",
         "Bxpert = 0.5",
         "mu = 0.05",
         "scale_scatt = 0.7",
@@ -81,6 +84,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     fs = 41.34137417094296
     assert abs(_num(out, "tstep") / (3 * fs) - 1) < 1e-10 and "x 3 per tstep" in out  # RK4 step 1 fs x 3
     assert _num(out, "freq_measure") == 2
+    assert _num(out, "freq_compute_tau") == 4 and _num(out, "compute_tau_only") == 1  # parameters.cpp:326
     assert abs(_num(out, "mu") / (0.05 / 27.21138505) - 1) < 1e-10
     assert _num(out, "scale_eph") == 0.9 and _num(out, "scale_ei") == 0.7
     assert abs(float(re.search(r"Bpert = (\S+)", out).group(1)) / (0.5 / 2.3505175675871e5) - 1) < 1e-10

@@ -196,3 +196,85 @@ def test_dmd_run_rkf45_and_euler(tmp_path):
         if (i + 1) % 2 == 0:
             ref.append(oracle.spin(s3[:, 2], e, f, y, (i + 1) * FS, 0, nb, obs_prefac))
     assert np.abs(res["euler"][:, 1] - np.array(ref)).max() < 1e-11 * np.abs(np.array(ref)).max()
+
+
+def _tau_ref(oracle, rho, ddm, sz, e, f, t, dt, obs_prefac):
+    """The three relaxation times of one tau line (observable.h:289-304,
+    392-397) from the oracle's spin measure: S at t and at t + dt after one
+    Euler step, dS/dt from the right-hand side, and the precession term
+    sum -2 Im(e_i s_ib(t) drho_bi) as the measure of the matrix 2i e_i s_ib."""
+    nk, nb = e.shape
+    tot = oracle.spin(sz, e, f, rho, t, 0, nb, obs_prefac)
+    plus = oracle.spin(sz, e, f, rho + ddm * dt, t + dt, 0, nb, obs_prefac)
+    dot = oracle.spin(sz, e, np.zeros_like(f), ddm, t, 0, nb, obs_prefac)
+    s2 = (2j * e[:, :, None] * sz.reshape(nk, nb, nb)).reshape(nk, -1)
+    term2 = oracle.spin(s2, e, f, rho, t, 0, nb, obs_prefac)
+    return np.array([-tot * dt / (plus - tot), -tot / dot, -tot / (dot + term2)])
+
+
+def test_dmd_run_tau_reports(tmp_path):
+    """tau_s*_tot.out (report_tau, DensityMatrix_Dynamics.h:347-354): with a
+    Bpert along z only tau_sz is written; compute_tau_only gives one line at
+    t0, an evolving run one line every freq_compute_tau steps; each line
+    against the oracle's right-hand side and spin measure at that time, and
+    the finite-difference tau against the analytic one with precession."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print("case dmd_run tau", flush=True)
+    p = scatt.synth_params(0)
+    common = ["alg_ddmdteq = 1", "Bzpert = 2", "Bx = 0.5", "tstep = 1", "ode_hmax = 1", "alg_ode_method = rk4"]
+    d1, d2 = tmp_path / "only", tmp_path / "run"
+    for d, extra in ((d1, ["compute_tau_only = 1"]), (d2, ["tend = 6", "freq_compute_tau = 2", "freq_measure = 6"])):
+        (d / "ldbd_data").mkdir(parents=True)
+        scatt.write_ldbd(str(d / "ldbd_data"), p)
+        (d / "param.in").write_text("\n".join(common + extra) + "\n")
+        r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        assert not (d / "tau_sx_elec_tot.out").exists() and not (d / "tau_sy_elec_tot.out").exists()
+    assert (d1 / "tau_sz_elec_tot.out").read_text().startswith("#time(au), tau by FD, tau without precession, tau\n")
+    L = str(d1 / "ldbd_data")
+    info = ldbd.read_info(L)
+    e = ldbd.read_ek(L, info)
+    k1, k2 = ldbd.read_kpairs(L, info)
+    P1, P2 = ldbd.read_P(L, info)
+    s3 = ldbd.read_smat(L, info)
+    _, R = ldbd.read_R(L)
+    nk, nb = e.shape
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    rho0 = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho0.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    dm_eq = np.zeros((nk, nb, nb), np.complex128)
+    dm_eq[:, np.arange(nb), np.arange(nb)] = feq
+    prefac = 2 * np.pi / info.nk_full
+    eq = oracle.evolve(k1, k2, P1, P2, e, dm_eq.reshape(nk, -1), t=0.0, prefac=prefac)
+    H = 0.5 * TESLA * s3[:, 0]
+    obs_prefac = 1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3)
+    dt = min(1e-3, 1 / 10) * FS
+    PS = 1e3 * FS
+
+    def rhs(t, y):
+        return oracle.evolve(k1, k2, P1, P2, e, y, t=t, prefac=prefac, ddmdt_eq=eq) + oracle.coherent(H, e, y, t)
+
+    only = np.loadtxt(d1 / "tau_sz_elec_tot.out", usecols=(2, 3, 4))
+    assert only.shape == (3,)
+    ref0 = _tau_ref(oracle, rho0, rhs(0.0, rho0), s3[:, 2], e, f, 0.0, dt, obs_prefac) / PS
+    print("tau at t0 runner", only, "oracle", ref0)
+    assert np.abs(only[1:] - ref0[1:]).max() < 2e-7 * np.abs(ref0[1:]).max()
+    assert abs(only[0] - ref0[0]) < 1e-5 * abs(ref0[0])  # a difference quotient: fewer stable digits
+    # the finite difference follows the derivative with precession, not the one without
+    assert abs(only[0] - only[2]) < 1e-3 * abs(only[2])
+    rows = np.loadtxt(d2 / "tau_sz_elec_tot.out", usecols=(0, 2, 3, 4))
+    assert rows.shape == (3, 4)  # steps 1, 3, 5 of 6
+    assert np.abs(rows[:, 0] - np.array([0, 2, 4])).max() < 1e-9
+    y = rho0.copy()
+    for i, t_fs in enumerate((0, 2, 4)):
+        if t_fs:
+            y, _ = oracle.rk4(k1, k2, P1, P2, e, y, 1 * FS, 2, prefac=prefac, t0=(t_fs - 2) * FS, H=H, ddmdt_eq=eq)
+        ref = _tau_ref(oracle, y, rhs(t_fs * FS, y), s3[:, 2], e, f, t_fs * FS, dt, obs_prefac) / PS
+        print(f"tau at {t_fs} fs runner", rows[i, 1:], "oracle", ref)
+        assert np.abs(rows[i, 2:] - ref[1:]).max() < 2e-7 * np.abs(ref[1:]).max()
+        assert abs(rows[i, 1] - ref[0]) < 1e-5 * abs(ref[0])

@@ -91,3 +91,38 @@ def test_rkf45_adaptive_follows_converged_oracle_and_tolerance():
             k = oracle.evolve(k1, k2, P1, P2, e, yr, t=i * dt, prefac=prefac).reshape(nk, nb, nb)
             yr = yr + 0.5 * dt * (k + k.conj().transpose(0, 2, 1)).reshape(nk, -1)
         assert rel_err(ye, yr) < 1e-12
+
+
+def test_compute_rhs_is_the_whole_right_hand_side():
+    """dmd_scatt_compute_rhs: the right-hand side the integrators step, at t,
+    with the band window, the stored ddmdt_eq and the coherent term, against
+    the oracle's evolve on the cut and its coherent term on the whole matrix."""
+    _lib, scatt, oracle = _mods()
+    print("case compute_rhs window", flush=True)
+    rng = np.random.default_rng(83)
+    nk, nb, nbd, b0 = 30, 2, 4, 1
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    e_dm = np.sort(rng.uniform(0, 4e-3, (nk, nbd)), axis=1)
+    e_dm[:, b0:b0 + nb] = e
+    P1, P2 = random_P(len(k1), nb, rng)
+    prefac = 2 * np.pi / nk
+    H = random_rho(nk, nbd, rng, scale=1e-4, diag=1e-4)
+    eq = random_rho(nk, nb, rng, scale=1e-6, diag=1e-6)
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        op.set_window(nbd, b0, e_dm)
+        op.set_coherent(H)
+        op.set_ddmdt_eq(eq)
+        for t_fs in (0.0, 23.0):
+            t = t_fs * FS
+            rho = random_rho(nk, nbd, rng, scale=0.05, diag=0.3)
+            got = op.compute_rhs(t, rho)
+            cut = rho.reshape(nk, nbd, nbd)[:, b0:b0 + nb, b0:b0 + nb].reshape(nk, -1)
+            inner = oracle.evolve(k1, k2, P1, P2, e, cut, t=t, prefac=prefac, ddmdt_eq=eq)
+            ref = oracle.coherent(H, e_dm, rho, t).reshape(nk, nbd, nbd)
+            ref[:, b0:b0 + nb, b0:b0 + nb] += inner.reshape(nk, nb, nb)
+            assert rel_err(got, ref.reshape(nk, -1)) < 1e-12, (t_fs, rel_err(got, ref.reshape(nk, -1)))
+            # one Euler step is rho + dt * (this right-hand side), Hermitian already
+            dt = 0.5 * FS
+            assert np.abs(op.integrate_euler(rho, t, dt, 1) - (rho + dt * got)).max() < 1e-15 + 1e-13 * dt * np.abs(got).max()
