@@ -232,6 +232,8 @@ int tiles4_place(dmd_scatt *h, const double2 *P1, const double2 *P2, int64_t n, 
 int place_P_dev(dmd_scatt *h, const double2 *a, const double2 *b, int64_t n, int64_t first, double scale, int add);
 std::string ldbd_path(const char *dir, const std::string &name);
 std::string ldbd_P_name(int which, const char *scatt, int impurity, int hole);
+std::string ldbd_sP_name(int which, const char *scatt, int hole, const char *part);
+int file_bytes(const std::string &path, int64_t *bytes);
 int check_size(const std::string &path, int64_t expect);
 int pread_all(const std::string &path, int64_t off, int64_t n, void *buf, int nthreads);
 int tiles4_fetch(dmd_scatt *h, int64_t first, int64_t n, double2 *P1, double2 *P2);

@@ -509,7 +509,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                                                    const double2 *__restrict__ rhoS,
                                                    const double2 *__restrict__ rho1S, double *__restrict__ hacc)
 {
-    constexpr int NB = 2, NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R;
+    constexpr int NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R;  // nb = 2
     constexpr int W = 4, RPT = R / 256;  // rows per thread in the flush
     // LDS: Hermitian row increments of blocks I and J, structure of arrays
     // (double d of row r at d*R + r, conflict-free for the flush), and the
@@ -1199,6 +1199,51 @@ extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, 
     if (e == hipSuccess) e = hipMemcpyAsync(P2, s2, count * per * 16, hipMemcpyDefault, h->st);
     if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     dfree(s1); dfree(s2);
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    return DMD_OK;
+}
+
+// Zero the elements with |P| <= thr (|.| as comm/scalar.h:102-103, no
+// contraction); a and b hold n complex each.
+__global__ void k_threshold(int64_t n, double2 *__restrict__ a, double2 *__restrict__ b, double thr)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * n) return;
+    double2 *p = idx < n ? a + idx : b + (idx - n);
+    const double2 v = *p;
+    if (!(sqrt(__dadd_rn(__dmul_rn(v.x, v.x), __dmul_rn(v.y, v.y))) > thr)) *p = make_double2(0.0, 0.0);
+}
+
+// set_sparseP's sparsification (ElectronPhonon.cpp:115-129) on the stored P,
+// in chunks of 64 MiB per array: fetch into pair order, zero, place back.
+extern "C" int dmd_scatt_threshold_P(dmd_scatt *h, double thr)
+{
+    if (!h || !(thr >= 0)) return fail(DMD_ERR_ARG, "dmd_scatt_threshold_P: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator, not P");
+    if (h->np == 0) return DMD_OK;
+    const int64_t per = h->nb4;
+    const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / (per * 16)));
+    double2 *s1 = nullptr, *s2 = nullptr;
+    int rc;
+    if ((rc = dalloc_t(&s1, (size_t)(chunk * per), "threshold staging"))) return rc;
+    if ((rc = dalloc_t(&s2, (size_t)(chunk * per), "threshold staging"))) { dfree(s1); return rc; }
+    for (int64_t first = 0; first < h->np && rc == DMD_OK; first += chunk) {
+        const int64_t n = std::min(chunk, h->np - first), tot = n * 2 * per;
+        if (h->precision != DMD_PREC_FP64) {
+            if ((rc = tiles4_fetch(h, first, n, s1, s2))) break;
+        } else {
+            hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, first, h->nb4, h->d_P,
+                               h->d_inv, h->soa, s1, s2);
+        }
+        hipLaunchKernelGGL(k_threshold, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n * per, s1, s2, thr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { rc = fail(DMD_ERR_HIP, hipGetErrorString(e)); break; }
+        rc = place_P_dev(h, s1, s2, n, first, 1.0, 0);
+    }
+    hipError_t e = hipStreamSynchronize(h->st);
+    dfree(s1); dfree(s2);
+    if (rc) return rc;
     if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
     return DMD_OK;
 }

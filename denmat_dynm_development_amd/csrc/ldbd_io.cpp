@@ -36,6 +36,14 @@ std::string ldbd_P_name(int which, const char *scatt, int impurity, int hole)
     return s + ".bin";
 }
 
+// sP1_<scatt>[_hole]_{ns,s,i,j}.bin, the sparse P files (ElectronPhonon.cpp:240-242)
+std::string ldbd_sP_name(int which, const char *scatt, int hole, const char *part)
+{
+    std::string s = "sP" + std::to_string(which) + "_" + std::string(scatt && *scatt ? scatt : "lindblad");
+    if (hole) s += "_hole";
+    return s + "_" + part + ".bin";
+}
+
 int file_bytes(const std::string &path, int64_t *bytes)
 {
     struct stat st;

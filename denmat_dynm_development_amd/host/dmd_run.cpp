@@ -17,10 +17,12 @@
 // on-device integrator alg_ode_method picks: rkf45 (the reference's default,
 // adaptive with ode_hstart / ode_hmin / ode_hmax / ode_epsabs), euler, or
 // fixed-step rk4 with step min(tstep, ode_hmax); reports every freq_measure
-// steps of tstep.  Features whose physics this operator does not
-// carry (e-e, model e-i, sparse P, pump pulses, phenomenological
-// relaxation, orbital or g-factor terms) stop the run with a message rather
-// than being ignored.
+// steps of tstep, and tau_s*_tot.out every freq_compute_tau steps.
+// alg_sparseP zeroes |P| <= alg_thr_sparseP and writes the sparse sP files;
+// alg_Pin_is_sparse reads P from them.  Features whose physics this operator
+// does not carry (e-e, model e-i, pump pulses, phenomenological relaxation,
+// orbital or g-factor terms) stop the run with a message rather than being
+// ignored.
 //
 //   dmd_run [--dry-run] [folder]
 #include <algorithm>
@@ -48,7 +50,8 @@ struct cplx {
 };
 
 struct Settings {
-    bool restart = false, tau_only = false;
+    bool restart = false, tau_only = false, sparseP = false, Pin_is_sparse = false;
+    double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
     double scale_eph = 1, scale_ei = 1;
@@ -99,8 +102,13 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     refuse(p, "alg_semiclassical", 0, "the semiclassical (diagonal) mode");
     s.linearize = p.get("alg_linearize", 0) != 0;
     refuse(p, "alg_linearize_dPee", 0, "the linearized e-e operator");
-    refuse(p, "alg_Pin_is_sparse", 0, "sparse P input");
-    refuse(p, "alg_sparseP", 0, "sparse P");
+    // parameters.cpp:118-120; sparse P is kept dense in HBM with the small elements zeroed
+    s.Pin_is_sparse = p.get("alg_Pin_is_sparse", 0) != 0;
+    s.sparseP = p.get("alg_sparseP", 0) != 0;
+    s.thr_sparseP = p.get("alg_thr_sparseP", 1e-40);
+    if (!(s.thr_sparseP >= 0)) throw std::runtime_error("alg_thr_sparseP must be >= 0");
+    if (s.Pin_is_sparse && s.linearize)  // ElectronPhonon_linear.cpp:140-143
+        throw std::runtime_error("linearization for sparse P matrices is not implemented");
     refuse(p, "alg_phenom_relax", 0, "phenomenological relaxation");
     refuse(p, "alg_only_eimp", 0, "alg_only_eimp");
     refuse(p, "alg_only_ee", 0, "alg_only_ee");
@@ -285,6 +293,8 @@ int run(const std::string &folder, bool dry)
                 s.picture.c_str(), s.scatt.c_str());
     std::printf("Bpert = %.10le %.10le %.10le B = %.10le %.10le %.10le impurities = %zu restart = %d\n", s.Bpert[0],
                 s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
+    std::printf("alg_sparseP = %d alg_Pin_is_sparse = %d alg_thr_sparseP = %lg\n", (int)s.sparseP, (int)s.Pin_is_sparse,
+                s.thr_sparseP);
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
     if (dry) return 0;
 
@@ -338,8 +348,15 @@ int run(const std::string &folder, bool dry)
     }
 
     // the scattering operator: e-ph P, ab initio e-i P, flags, ddmdt_eq, H_BS
-    electronphonon_mi355x<cplx> eph(ldbd, s.scatt, s.scale_eph, s.scissor);
+    electronphonon_mi355x<cplx> eph(ldbd, s.scatt, s.scale_eph, s.scissor, DMD_PREC_FP64, s.Pin_is_sparse,
+                                    s.thr_sparseP);
     for (size_t iD = 0; iD < s.ni.size(); iD++) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+    // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
+    // alg_linearize set_eph returns before it (:6)
+    if (s.sparseP && !s.linearize && !s.Pin_is_sparse) {
+        eph.set_sparseP(true);
+        std::printf("sparse P: |P| <= %lg zeroed, sP files in %s\n", s.thr_sparseP, ldbd.c_str());
+    }
     eph.set_algorithm(s.expt, !interaction);
     if (s.ddmdteq || s.linearize) {  // DensityMatrix_Dynamics.h:57-65
         std::vector<double> feq((size_t)nk * nbd);

@@ -41,15 +41,23 @@ public:
     std::vector<impurity_slot> eimp_slots;
     std::vector<impurity_slot *> eimp;
     void *ee_model = nullptr;
+    // alg.Pin_is_sparse and alg.thr_sparseP (parameters.cpp:118-120)
+    bool Pin_is_sparse = false;
+    double thr_sparseP = 1e-40;
 
     // electronphonon's set-up from ldbd_data (ElectronPhonon.h:53-93 and
-    // set_eph, ElectronPhonon.cpp:3-13): P x scale_eph streamed into HBM.
+    // set_eph, ElectronPhonon.cpp:3-13): P x scale_eph streamed into HBM, from
+    // the sparse sP files when Pin_is_sparse (ElectronPhonon.cpp:235-246).
     explicit electronphonon_mi355x(const std::string &ldbd_dir, const std::string &scatt = "lindblad",
-                                   double scale_eph = 1.0, double scissor = 0.0, int precision = DMD_PREC_FP64)
-        : dir_(ldbd_dir), scatt_(scatt)
+                                   double scale_eph = 1.0, double scissor = 0.0, int precision = DMD_PREC_FP64,
+                                   bool Pin_is_sparse_ = false, double thr_sparseP_ = 1e-40)
+        : Pin_is_sparse(Pin_is_sparse_), thr_sparseP(thr_sparseP_), dir_(ldbd_dir), scatt_(scatt)
     {
         check(dmd_ldbd_read_info(dir_.c_str(), &info_), "ldbd_size.dat");
-        check(dmd_scatt_create_from_ldbd(dir_.c_str(), scatt_.c_str(), scale_eph, scissor, precision, &h_),
+        check(Pin_is_sparse ? dmd_scatt_create_from_ldbd_sparse(dir_.c_str(), scatt_.c_str(), scale_eph, scissor,
+                                                                precision, &h_)
+                            : dmd_scatt_create_from_ldbd(dir_.c_str(), scatt_.c_str(), scale_eph, scissor, precision,
+                                                         &h_),
               "scattering operator from " + dir_);
         bStart = info_.b_bot_eph - info_.b_bot_dm;
         bEnd = info_.b_top_eph - info_.b_bot_dm;
@@ -68,14 +76,27 @@ public:
 
     // add_scatt_contrib("eimp", iD) of an ab initio defect (ElectronPhonon.cpp:
     // 44-113 with read_ldbd_imp_P, ElectronImpurity.cpp:3-48): P += fraction *
-    // scale_ei * P_D<iD+1>, fraction = |n_i cell_size| (ElectronImpurity.h:17-18).
+    // scale_ei * P_D<iD+1>, fraction = |n_i cell_size| (ElectronImpurity.h:17-18);
+    // under Pin_is_sparse the sum is re-sparsified at thr_sparseP (:91-94).
     void add_eimp_ab_neutral(int iD, double fraction, double scale_ei)
     {
         check(dmd_scatt_load_P(h_, dir_.c_str(), &info_, scatt_.c_str(), iD + 1, fraction * scale_ei, 1),
               "e-i P of defect " + std::to_string(iD + 1));
+        if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
         if ((int)eimp_slots.size() <= iD) eimp_slots.resize(iD + 1);
         eimp.clear();
         for (auto &s : eimp_slots) eimp.push_back(&s);
+    }
+
+    // set_sparseP(true) at the end of set_eph (ElectronPhonon.cpp:12,115-129)
+    // under alg_sparseP: P with |P| <= thr_sparseP zeroed and, with
+    // write_files, the sparse sP files written next to ldbd_data's P files
+    // (an existing _s file is kept).
+    void set_sparseP(bool write_files = true)
+    {
+        if (Pin_is_sparse) return;
+        check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
+        if (write_files) check(dmd_ldbd_write_sP(h_, dir_.c_str(), &info_, scatt_.c_str(), thr_sparseP), "sP files");
     }
 
     // alg.expt and alg.picture (parameters.cpp:104,107)

@@ -7,6 +7,7 @@ check every file's size as the reference's check_file_size does and raise
 DMDError (code DMD_ERR_IO) where the reference would exit.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -89,6 +90,30 @@ def read_P(dirname, info=None, scatt="lindblad", impurity=0, first=0, count=None
     return P1, P2
 
 
+def sP_path(dirname, which, part, scatt="lindblad", hole=False):
+    """ldbd_data/sP<which>_<scatt>[_hole]_<part>.bin (ElectronPhonon.cpp:240-242)."""
+    return os.path.join(str(dirname), f"sP{which}_{scatt}{'_hole' if hole else ''}_{part}.bin")
+
+
+def read_sP(dirname, which, info=None, scatt="lindblad"):
+    """(ns, s, i, j) of the sparse P<which> files: int32 counts per stored
+    pair, complex128 values, int32 row indices i1*nb+i2 and column indices
+    i3*nb+i4 (sparse_mat::write_to_files, comm/sparse_matrix.h:37-42), with
+    the sizes checked against each other and the stored pairs."""
+    info = _info(dirname, info)
+    hole = info.kind == 1
+    ns = np.fromfile(sP_path(dirname, which, "ns", scatt, hole), dtype=np.int32)
+    if len(ns) != info.nkpair:
+        raise ValueError(f"sP{which} _ns: {len(ns)} counts for {info.nkpair} stored pairs")
+    tot = int(ns.astype(np.int64).sum())
+    sv = np.fromfile(sP_path(dirname, which, "s", scatt, hole), dtype=np.complex128)
+    iv = np.fromfile(sP_path(dirname, which, "i", scatt, hole), dtype=np.int32)
+    jv = np.fromfile(sP_path(dirname, which, "j", scatt, hole), dtype=np.int32)
+    if not (len(sv) == len(iv) == len(jv) == tot):
+        raise ValueError(f"sP{which}: {tot} elements counted, files hold {len(sv)}, {len(iv)}, {len(jv)}")
+    return ns, sv, iv, jv
+
+
 def read_restart(dirname, nk, nb):
     """(dm as (nk, nb * nb) complex128, t) from denmat_restart.bin and time_restart.dat."""
     dm = np.empty((nk, nb * nb), np.complex128)
@@ -112,4 +137,4 @@ def write_restart(dirname, dm, t):
 
 
 __all__ = ["LdbdInfo", "read_info", "read_ek", "read_kvec", "read_smat", "read_imsig", "read_R", "read_kpairs",
-           "read_P", "read_restart", "read_restart_time", "write_restart", "_lib"]
+           "read_P", "sP_path", "read_sP", "read_restart", "read_restart_time", "write_restart", "_lib"]

@@ -65,10 +65,12 @@ class ScatteringOperator:
         self._h = h
 
     @classmethod
-    def from_ldbd(cls, dirname, scatt="lindblad", scale_eph=1.0, scissor=0.0, precision=_lib.DMD_PREC_FP64):
+    def from_ldbd(cls, dirname, scatt="lindblad", scale_eph=1.0, scissor=0.0, precision=_lib.DMD_PREC_FP64,
+                  Pin_is_sparse=False):
         """The handle of an ldbd_data folder, set up as the reference's
         electronphonon sets itself up (ElectronPhonon.h:53-93): e-ph window of
-        e_dm, stored pairs, P x scale_eph streamed from the files, and the band
+        e_dm, stored pairs, P x scale_eph streamed from the files (the sparse
+        sP files with Pin_is_sparse, ElectronPhonon.cpp:235-246), and the band
         window when the density matrix has more bands."""
         from . import ldbd
         info = ldbd.read_info(dirname)
@@ -78,8 +80,8 @@ class ScatteringOperator:
         self.n_pairs = int(info.nkpair)
         self.info = info
         h = ctypes.c_void_p()
-        call("dmd_scatt_create_from_ldbd", str(dirname).encode(), scatt.encode(), float(scale_eph), float(scissor),
-             int(precision), ctypes.byref(h))
+        call("dmd_scatt_create_from_ldbd_sparse" if Pin_is_sparse else "dmd_scatt_create_from_ldbd",
+             str(dirname).encode(), scatt.encode(), float(scale_eph), float(scissor), int(precision), ctypes.byref(h))
         self._h = h
         return self
 
@@ -127,6 +129,27 @@ class ScatteringOperator:
         info = ldbd.read_info(dirname) if info is None else info
         call("dmd_scatt_load_P", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), int(impurity),
              float(scale), int(bool(add)))
+
+    def load_sP(self, dirname, info=None, scatt="lindblad", scale=1.0, thr=1e-40):
+        """P from the sparse files sP{1,2}_<scatt>[_hole]_{ns,s,i,j}.bin, each
+        value x scale kept when |value x scale| > thr (ElectronPhonon.cpp:
+        235-246), placed dense in the handle's layout."""
+        from . import ldbd
+        info = ldbd.read_info(dirname) if info is None else info
+        call("dmd_scatt_load_sP", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), float(scale),
+             float(thr))
+
+    def threshold_P(self, thr):
+        """alg_sparseP's sparsification (set_sparseP, ElectronPhonon.cpp:115-129):
+        every stored element with |P| <= thr becomes 0."""
+        call("dmd_scatt_threshold_P", self._h, float(thr))
+
+    def write_sP(self, dirname, info=None, scatt="lindblad", thr=1e-40):
+        """The stored P as sparse files (write_smat, ElectronPhonon.cpp:126-128);
+        a matrix whose _s file exists is left alone."""
+        from . import ldbd
+        info = ldbd.read_info(dirname) if info is None else info
+        call("dmd_ldbd_write_sP", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), float(thr))
 
     def download_P(self, first=0, count=None):
         count = self.n_pairs - first if count is None else count

@@ -337,6 +337,37 @@ int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, double scale_
 int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const double *P1, const double *P2,
                              double scale, int add);
 
+/* ---- sparse P (alg_sparseP, alg_Pin_is_sparse) ------------------------------------ */
+/* Sparse files sP{1,2}_<scatt>[_hole]_{ns,s,i,j}.bin (sparse2D::read_smat,
+ * comm/sparse2D.h:98-119; sparse_mat(FILE*, ...), comm/sparse_matrix.h:30-36):
+ * per stored pair an int count ns in _ns, and that many complex128 values in
+ * _s, int row indices i1*nb+i2 in _i and int column indices i3*nb+i4 in _j.
+ * The device keeps P dense in its usual layout (nb <= 4: a pair's record is
+ * 256 B to 8 KiB), so every kernel serves the sparse modes unchanged. */
+/* read_ldbd_eph's sparse branch (ElectronPhonon.cpp:235-246): P from the
+ * sparse files, each value times scale and kept when |scale*s| > thr (the
+ * re-sparsification at :245-246 keeps |.| > 1e-40); an index pair given twice
+ * keeps the last value (sparse_mat::todense, comm/sparse_matrix.h:76-80).
+ * Sizes and indices are checked against the handle. */
+int dmd_scatt_load_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, const char *scatt, double scale,
+                      double thr);
+/* The sparsification of alg_sparseP and of the e-i fold-in under
+ * alg_Pin_is_sparse (set_sparseP, ElectronPhonon.cpp:115-129;
+ * sparse_plus_dense, comm/sparse_matrix.cpp:42-47; sparse_mat::sparse keeps
+ * |A| > thr, comm/sparse_matrix.h:57-70): every stored element of P1 and P2
+ * with |P| <= thr becomes 0.  Not on a linearized handle. */
+int dmd_scatt_threshold_P(dmd_scatt *h, double thr);
+/* write_smat of set_sparseP (ElectronPhonon.cpp:126-128; comm/sparse2D.h:
+ * 136-151, sparse_mat::write_to_files, comm/sparse_matrix.h:37-42): the
+ * handle's P1 and P2 as sparse files, elements with |P| > thr in row-major
+ * order; a matrix whose _s file exists is not written again.  All four files
+ * carry the <scatt>[_hole] suffix the reader looks for (ElectronPhonon.cpp:241). */
+int dmd_ldbd_write_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, const char *scatt, double thr);
+/* dmd_scatt_create_from_ldbd with P from the sparse files (alg_Pin_is_sparse),
+ * scaled by scale_eph and kept when |scale_eph*s| > 1e-40. */
+int dmd_scatt_create_from_ldbd_sparse(const char *dir, const char *scatt, double scale_eph, double scissor,
+                                      int precision, dmd_scatt **out);
+
 /* ---- restart files ---------------------------------------------------------------- */
 /* denmat_restart.bin (nk x nb^2 complex128) and time_restart.dat ("%14.7le")
  * in the folder dir, as singdenmat_k::read_dm_restart / write_dm_tofile

@@ -20,6 +20,9 @@
  *                          interaction vs Schroedinger picture, per-pair
  *                          initializer form (oracle_compute_ddm_pairs).
  *   oracle_compute_ddm_pairs  equality with oracle_evolve at t=0 (linearity).
+ *   oracle_evolve_sparse   equality with oracle_evolve on the densified P
+ *                          (zeros where no element is listed) at t = 0 and
+ *                          with phases, for any element order.
  *   oracle_compute_P       Gram-form positivity, the P1(k,k')/P2(k',k)
  *                          conjugate identity, ImSigma from the P diagonal
  *                          against a direct golden-rule sum.
@@ -180,6 +183,9 @@ static void evolve_pairs(int64_t p0, int64_t p1, int nb, const uint64_t *k1, con
 /* Pdm/dm1P and the partials are summed in thread order (the allreduce at    */
 /* :137 without its per-row collectives).                                    */
 /* ------------------------------------------------------------------------ */
+static void assemble(int64_t nk, int nb, const double *e, const double *dm, const double *dm1, const zc *Pdm,
+                     const zc *dm1P, double t, double prefac, const double *ddmdt_eq, int schrodinger, double *ddmdt);
+
 int oracle_evolve(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
                   const double *P1, const double *P2, const double *e,
                   const double *dm, const double *dm1, double t, double prefac, int expt,
@@ -219,7 +225,16 @@ int oracle_evolve(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const 
         free(pPdm); free(pdm1P);
         if (fail) { free(Pdm); free(dm1P); return -3; }
     }
-    /* final assembly, evolve_eph.cpp:140-152 */
+    assemble(nk, nb, e, dm, dm1, Pdm, dm1P, t, prefac, ddmdt_eq, schrodinger, ddmdt);
+    free(Pdm); free(dm1P);
+    return 0;
+}
+
+/* final assembly, evolve_eph.cpp:140-152, and the ddmdteq subtraction, :154-162 */
+static void assemble(int64_t nk, int nb, const double *e, const double *dm, const double *dm1, const zc *Pdm,
+                     const zc *dm1P, double t, double prefac, const double *ddmdt_eq, int schrodinger, double *ddmdt)
+{
+    const int nb2 = nb * nb;
     const zc c1 = zmk(1, 0), cm1 = zmk(-1, 0), c0 = zmk(0, 0);
     for (int64_t ik = 0; ik < nk; ik++) {
         zc contrib[64];
@@ -242,7 +257,74 @@ int oracle_evolve(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const 
                         ZW(ddmdt, o) = zsub(Z(ddmdt, o), zmul(Z(ddmdt_eq, o), zcis((e[ik * nb + i] - e[ik * nb + j]) * t)));
                 }
     }
-    free(Pdm); free(dm1P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* The sparse branch of electronphonon::evolve (alg_sparseP /               */
+/* alg_Pin_is_sparse), src_v4.5.7/evolve_eph.cpp:69-86: per stored pair the */
+/* COO lists of P1 and P2 (ns values, row indices i1*nb+i2, column indices  */
+/* i3*nb+i4, in the order of the files sP{1,2}_*_{ns,s,i,j}.bin), phases by */
+/* compute_sPt (:194-203), products by sparse_zgemm (src_v4.5.7/comm/       */
+/* sparse_matrix.cpp:6-40): left, n = 1: Pdm[i] += s * dm[j]; right, m = 1: */
+/* dm1P[j] += dm1[i] * s; the reverse direction with the conjugated lists   */
+/* of the other matrix (:81-85).  Then the same assembly as oracle_evolve.  */
+/* ------------------------------------------------------------------------ */
+int oracle_evolve_sparse(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                         const int32_t *ns1, const double *s1, const int32_t *i1, const int32_t *j1,
+                         const int32_t *ns2, const double *s2, const int32_t *i2, const int32_t *j2,
+                         const double *e, const double *dm, const double *dm1, double t, double prefac, int expt,
+                         double *ddmdt)
+{
+    const int nb2 = nb * nb;
+    if (nb < 1 || nb > 8 || nk < 0 || npairs < 0) return -1;
+    for (int64_t ip = 0; ip < npairs; ip++)
+        if (k1[ip] > k2[ip] || (int64_t)k2[ip] >= nk || ns1[ip] < 0 || ns2[ip] < 0) return -2;
+    size_t nrow = (size_t)nk * nb2;
+    zc *Pdm = (zc *)calloc(nrow, sizeof(zc)), *dm1P = (zc *)calloc(nrow, sizeof(zc));
+    zc *t1 = (zc *)malloc(sizeof(zc) * (size_t)nb2 * nb2), *t2 = (zc *)malloc(sizeof(zc) * (size_t)nb2 * nb2);
+    if (!Pdm || !dm1P || !t1 || !t2) { free(Pdm); free(dm1P); free(t1); free(t2); return -3; }
+    const zc *S1 = (const zc *)s1, *S2 = (const zc *)s2, *D = (const zc *)dm, *D1 = (const zc *)dm1;
+    int64_t o1 = 0, o2 = 0;
+    for (int64_t ip = 0; ip < npairs; ip++) {
+        const int64_t ik = (int64_t)k1[ip], ikp = (int64_t)k2[ip];
+        const double *ek = e + ik * nb, *ekp = e + ikp * nb;
+        const int n1 = ns1[ip], n2 = ns2[ip];
+        if (n1 > nb2 * nb2 || n2 > nb2 * nb2) { free(Pdm); free(dm1P); free(t1); free(t2); return -2; }
+        for (int q = 0; q < n1; q++) {   /* compute_sPt(..., sP1, smat1_time, false) */
+            const int r = i1[o1 + q], c = j1[o1 + q];
+            if (r < 0 || r >= nb2 || c < 0 || c >= nb2) { free(Pdm); free(dm1P); free(t1); free(t2); return -2; }
+            const int a = r / nb, b = r % nb, k = c / nb, l = c % nb;
+            t1[q] = expt ? zmul(S1[o1 + q], zcis((ek[a] - ekp[k] - ek[b] + ekp[l]) * t)) : S1[o1 + q];
+        }
+        for (int q = 0; q < n2; q++) {   /* compute_sPt(..., sP2, smat2_time, true) */
+            const int r = i2[o2 + q], c = j2[o2 + q];
+            if (r < 0 || r >= nb2 || c < 0 || c >= nb2) { free(Pdm); free(dm1P); free(t1); free(t2); return -2; }
+            const int a = r / nb, b = r % nb, k = c / nb, l = c % nb;
+            t2[q] = expt ? zmul(S2[o2 + q], zcis((ek[k] - ekp[a] - ek[l] + ekp[b]) * t)) : S2[o2 + q];
+        }
+        for (int q = 0; q < n1; q++) {   /* :78 */
+            const size_t o = ik * nb2 + i1[o1 + q];
+            Pdm[o] = zadd(Pdm[o], zmul(t1[q], D[ikp * nb2 + j1[o1 + q]]));
+        }
+        for (int q = 0; q < n2; q++) {   /* :79 */
+            const size_t o = ik * nb2 + j2[o2 + q];
+            dm1P[o] = zadd(dm1P[o], zmul(D1[ikp * nb2 + i2[o2 + q]], t2[q]));
+        }
+        if (ik < ikp) {                  /* :80-85: sm1_next = conj(smat2_time), sm2_next = conj(smat1_time) */
+            for (int q = 0; q < n2; q++) {
+                const size_t o = ikp * nb2 + i2[o2 + q];
+                Pdm[o] = zadd(Pdm[o], zmul(zconj(t2[q]), D[ik * nb2 + j2[o2 + q]]));
+            }
+            for (int q = 0; q < n1; q++) {
+                const size_t o = ikp * nb2 + j1[o1 + q];
+                dm1P[o] = zadd(dm1P[o], zmul(D1[ik * nb2 + i1[o1 + q]], zconj(t1[q])));
+            }
+        }
+        o1 += n1;
+        o2 += n2;
+    }
+    assemble(nk, nb, e, dm, dm1, Pdm, dm1P, t, prefac, NULL, 0, ddmdt);
+    free(Pdm); free(dm1P); free(t1); free(t2);
     return 0;
 }
 

@@ -54,6 +54,10 @@ def lib():
         L.oracle_set_Lsc.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d]
         L.oracle_evolve_linear.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, _d]
+        _i32p = ctypes.POINTER(ctypes.c_int32)
+        L.oracle_evolve_sparse.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _i32p, _d, _i32p, _i32p,
+                                           _i32p, _d, _i32p, _i32p, _d, _d, _d, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, _d]
         L.oracle_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -104,6 +108,40 @@ def evolve(k1, k2, P1, P2, e, dm, dm1=None, t=0.0, prefac=1.0, expt=True, ddmdt_
                              int(nthreads), _p(out))
     if rc != 0:
         raise ValueError(f"oracle_evolve failed: {rc}")
+    return out
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def evolve_sparse(k1, k2, sP1, sP2, e, dm, dm1=None, t=0.0, prefac=1.0, expt=True):
+    """The sparse branch of electronphonon::evolve (src_v4.5.7/evolve_eph.cpp:69-86,
+    194-203; comm/sparse_matrix.cpp:6-40).  sP1, sP2: (ns, s, i, j) as the
+    files sP{1,2}_*_{ns,s,i,j}.bin hold them: per-pair counts, then complex
+    values, row indices i1*nb+i2 and column indices i3*nb+i4 in file order."""
+    e = _f64(e)
+    nk, nb = e.shape
+    dm = _c128(dm).reshape(nk, nb * nb)
+    dm1 = one_minus(dm, nb) if dm1 is None else _c128(dm1).reshape(nk, nb * nb)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    keep, args = [], []
+    for ns, sv, iv, jv in (sP1, sP2):
+        a_ns, p_ns = _i32(ns)
+        a_s = _c128(sv)
+        a_i, p_i = _i32(iv)
+        a_j, p_j = _i32(jv)
+        if len(a_ns) != len(k1) or int(a_ns.sum()) != len(a_s) or len(a_i) != len(a_s) or len(a_j) != len(a_s):
+            raise ValueError("sparse lists do not match the stored pairs")
+        keep += [a_ns, a_s, a_i, a_j]
+        args += [p_ns, _p(a_s), p_i, p_j]
+    out = np.zeros((nk, nb * nb), np.complex128)
+    rc = lib().oracle_evolve_sparse(nk, nb, len(k1), _p(k1), _p(k2), *args, _p(e), _p(dm), _p(dm1), float(t),
+                                    float(prefac), int(bool(expt)), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_evolve_sparse failed: {rc}")
     return out
 
 

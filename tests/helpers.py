@@ -141,3 +141,48 @@ def write_ldbd_folder(d, kind, nb, nv, dm, eph, e, k1, k2, P1=None, P2=None, nk_
     for i, Q1, Q2 in extra_P:
         np.ascontiguousarray(Q1, np.complex128).tofile(os.path.join(d, f"ldbd_P1_{scatt}_D{i}{sfx}.bin"))
         np.ascontiguousarray(Q2, np.complex128).tofile(os.path.join(d, f"ldbd_P2_{scatt}_D{i}{sfx}.bin"))
+
+
+def sparse_lists(P, rng=None, dup_frac=0.0):
+    """COO lists (ns, s, i, j) of the nonzero elements of P (npairs x nb^4),
+    row index i1*nb+i2 and column index i3*nb+i4, in row-major order or, with
+    rng, shuffled within each pair; dup_frac of the pairs also list one element
+    twice, first with a wrong value (a reader keeps the last)."""
+    npairs, n4 = P.shape
+    nb2 = int(round(np.sqrt(n4)))
+    ns, s, ii, jj = [], [], [], []
+    for p in range(npairs):
+        idx = np.flatnonzero(P[p] != 0)
+        if rng is not None:
+            idx = rng.permutation(idx)
+        vals = list(P[p, idx])
+        rows, cols = list(idx // nb2), list(idx % nb2)
+        if rng is not None and len(idx) and rng.uniform() < dup_frac:
+            q = int(rng.integers(len(idx)))
+            vals.insert(0, vals[q] * 3 + 1)
+            rows.insert(0, rows[q])
+            cols.insert(0, cols[q])
+        ns.append(len(vals))
+        s += vals
+        ii += rows
+        jj += cols
+    return (np.array(ns, np.int32), np.array(s, np.complex128).reshape(-1), np.array(ii, np.int32),
+            np.array(jj, np.int32))
+
+
+def sparsify(P, rng, zero_frac=0.5):
+    """P with about zero_frac of its elements set to exactly zero."""
+    return np.where(rng.uniform(size=P.shape) < zero_frac, 0, P)
+
+
+def write_sP_files(d, which, lists, scatt="lindblad", hole=False):
+    """sP<which>_<scatt>[_hole]_{ns,s,i,j}.bin as sparse_mat::write_to_files
+    lays them out (comm/sparse_matrix.h:37-42): int counts, complex128 values,
+    int row and int column indices, pair after pair."""
+    import os
+    ns, s, i, j = lists
+    base = os.path.join(str(d), f"sP{which}_{scatt}{'_hole' if hole else ''}_")
+    np.asarray(ns, np.int32).tofile(base + "ns.bin")
+    np.asarray(s, np.complex128).tofile(base + "s.bin")
+    np.asarray(i, np.int32).tofile(base + "i.bin")
+    np.asarray(j, np.int32).tofile(base + "j.bin")

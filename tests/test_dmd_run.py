@@ -72,6 +72,8 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
         "freq_measure = 2",
         "freq_measure_ene = 4",
         "compute_tau_only = 1",
+        "alg_sparseP = 1",
+        "alg_thr_sparseP = 1e-5",
         "Bxpert = 0.5",
         "mu = 0.05",
         "scale_scatt = 0.7",
@@ -85,6 +87,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     assert abs(_num(out, "tstep") / (3 * fs) - 1) < 1e-10 and "x 3 per tstep" in out  # RK4 step 1 fs x 3
     assert _num(out, "freq_measure") == 2
     assert _num(out, "freq_compute_tau") == 4 and _num(out, "compute_tau_only") == 1  # parameters.cpp:326
+    assert _num(out, "alg_sparseP") == 1 and _num(out, "alg_Pin_is_sparse") == 0 and _num(out, "alg_thr_sparseP") == 1e-5
     assert abs(_num(out, "mu") / (0.05 / 27.21138505) - 1) < 1e-10
     assert _num(out, "scale_eph") == 0.9 and _num(out, "scale_ei") == 0.7
     assert abs(float(re.search(r"Bpert = (\S+)", out).group(1)) / (0.5 / 2.3505175675871e5) - 1) < 1e-10
@@ -98,6 +101,8 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("alg_eph_need_hole = 1\nalg_eph_need_elec = 0", "file set"),
     ("pumpA0 = 0.002", "pump"),
     ("alg_ode_method = rk2", "rkf45, euler or rk4"),
+    ("alg_Pin_is_sparse = 1\nalg_linearize = 1", "linearization for sparse P matrices is not implemented"),
+    ("alg_thr_sparseP = -1", "alg_thr_sparseP"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
