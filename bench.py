@@ -18,6 +18,7 @@ spin lifetime of the same runs compared.
   python bench.py --mode accuracy   # FP64 / complex64 / BF16 S_z(t) and tau_s, nb = 4 with e-i
   python bench.py --mode adjoint    # forward + adjoint RK4 with 7 parameters, nb = 2, nk = 16384
   python bench.py --mode linear     # the linearized operator (set_Lsc + evolve_linear), nb = 2 and nb = 4
+  python bench.py --mode ode        # the adaptive rkf45 driver and compute_rhs against fixed-step RK4, nb = 2
 """
 import argparse
 import json
@@ -405,6 +406,59 @@ def mode_linear(args, torch, scatt):
     return out
 
 
+def mode_ode(args, torch, scatt):
+    """The reference's default integrator on the device: rkf45 (evolve_gsl,
+    GSL's standard control) at the default nb = 2 workload with the
+    reference's defaults (tstep 1 fs, ode_hstart 1e-3 fs, ode_hmax = tstep,
+    ode_epsabs 1e-8) over --steps tsteps after a first tstep that ramps the step
+    up; steps, rejections and right-hand sides per tstep, ms per right-hand side
+    against the fixed-step RK4's, and one compute_rhs (what report_tau calls)."""
+    p = scatt.synth_params(2)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    rho0, f = initial_state(p, e, s3)
+    op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+    op.generate_P(p)
+    rho = torch.from_numpy(rho0).cuda()
+    t, h = 0.0, 1e-3 * FS
+    t0 = time.perf_counter()
+    y, t, h, st = op.rkf45_apply(rho, t, FS, h, 0.0, FS, 1e-8)
+    torch.cuda.synchronize()
+    first = {"s": time.perf_counter() - t0, **st}
+    log(f"ode: first tstep {first}")
+    n = {"steps": 0, "rejected": 0, "rhs_evals": 0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y, t, h, st = op.rkf45_apply(y, t, t + FS, h, 0.0, FS, 1e-8)
+        for k in n:
+            n[k] += st[k]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    op.integrate_rk4(rho.clone(), 0.0, FS, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.integrate_rk4(rho.clone(), 0.0, FS, args.steps)
+    torch.cuda.synchronize()
+    el4 = time.perf_counter() - t0
+    out = torch.empty_like(y)
+    op.compute_rhs(t, y, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        op.compute_rhs(t, y, out)
+    torch.cuda.synchronize()
+    rhs_ms = (time.perf_counter() - t0) / 5 * 1e3
+    r = {"mode": "ode", "workload": "nb=2 nk=65536 58.2M stored pairs FP64", "tsteps": args.steps,
+         "first_tstep": first, "rkf45": {**n, "ms_per_tstep": el / args.steps * 1e3,
+                                          "ms_per_rhs": el / max(n["rhs_evals"], 1) * 1e3},
+         "rk4": {"ms_per_step": el4 / args.steps * 1e3, "ms_per_rhs": el4 / (4 * args.steps) * 1e3},
+         "compute_rhs_ms": rhs_ms, "t_end_fs": t / FS}
+    log(f"ode: {r}")
+    op.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,7 +466,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear"])
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
@@ -440,6 +494,9 @@ def main():
         return
     if args.mode == "linear":
         print(json.dumps(mode_linear(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "ode":
+        print(json.dumps(mode_ode(args, torch, scatt)), flush=True)
         return
     out, (p, e, k1, k2, rho0) = bench_plain(args, torch, scatt)
     if not args.no_cpu:

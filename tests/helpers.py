@@ -1,7 +1,8 @@
 """Shared builders for test inputs (seeded)."""
 import numpy as np
 
-FS = 41.341373335182  # 1 fs in Hartree atomic time (src_v4.5.7/comm/Units.h:62)
+FS = 41.341373335182  # 1 fs in Hartree atomic time (CODATA); the reference's comm/Units.h:59-61 derives
+# 41.34137417 from its base constants, which dmd_run uses for param.in times
 
 
 def random_pairs(nk, npart, rng, diag=False):
