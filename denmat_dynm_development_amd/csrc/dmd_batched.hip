@@ -185,8 +185,8 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
     for (int u = 0; u < nv; u++) {
         const int64_t v = v0 + u;
         const uint16_t *rec = vrec + v * 2048;
-        const bf16x8 M1h = as_bf8(*(const uint4 *)(rec + oa)), M2h = as_bf8(*(const uint4 *)(rec + 512 + oa));
-        const bf16x8 M1l = as_bf8(*(const uint4 *)(rec + 1024 + oa)), M2l = as_bf8(*(const uint4 *)(rec + 1536 + oa));
+        const bf16x8 M1h = as_bf8(ld_stream_u4(rec + oa)), M2h = as_bf8(ld_stream_u4(rec + 512 + oa));
+        const bf16x8 M1l = as_bf8(ld_stream_u4(rec + 1024 + oa)), M2l = as_bf8(ld_stream_u4(rec + 1536 + oa));
         const uint16_t *xq = xb + (int64_t)vq[v] * 4 * (int64_t)Bp * 8;
 #pragma unroll
         for (int nb = 0; nb < NBLK; nb++) {

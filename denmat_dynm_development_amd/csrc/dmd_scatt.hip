@@ -535,7 +535,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
         __syncthreads();
         const int mq = (m + W - 1) / W;
         const int i0 = wv * mq, i1 = min(m, i0 + mq);
-        uint32_t code_next = (i0 < i1 && lane * m + i0 < n) ? pk[b + (int64_t)i0 * 64 + lane] : 0u;
+        uint32_t code_next = (i0 < i1 && lane * m + i0 < n) ? ld_stream(pk + b + (int64_t)i0 * 64 + lane) : 0u;
         // rho of the lane's current row k: a lane walks a run of one k, so it is
         // reloaded only where k changes
         int lo1_cur = -1;
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             // idle lanes read the chunk's first slot (always valid) and add nothing
             const int64_t s = b + (ok ? (int64_t)it * 64 + lane : 0);
             const uint32_t code = code_next;
-            code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? pk[b + (int64_t)(it + 1) * 64 + lane] : 0u;
+            code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? ld_stream(pk + b + (int64_t)(it + 1) * 64 + lane) : 0u;
             const int lo1 = code % R, lo2 = code / R;
             const int64_t k = I0 + lo1, kp = J0 + lo2;
             const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
@@ -574,9 +574,9 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             const bool rev = ok && k < kp;
             double2 p1[NB4], p2[NB4];
 #pragma unroll
-            for (int el = 0; el < NB4; el++) p1[el] = g[el * 8];
+            for (int el = 0; el < NB4; el++) p1[el] = ld_stream(g + el * 8);
 #pragma unroll
-            for (int el = 0; el < NB4; el++) p2[el] = g[(NB4 + el) * 8];
+            for (int el = 0; el < NB4; el++) p2[el] = ld_stream(g + (NB4 + el) * 8);
             if (LIN) {
                 double2 x[NB2], y[NB2];
 #pragma unroll

@@ -123,9 +123,9 @@ template <> struct Loader<double2, double2> {
     static __device__ __forceinline__ void load(const double2 *rec, int l, double2 p1[16], double2 p2[16])
     {
 #pragma unroll
-        for (int e = 0; e < 16; e++) p1[e] = rec[e * 16 + l];
+        for (int e = 0; e < 16; e++) p1[e] = ld_stream(rec + e * 16 + l);
 #pragma unroll
-        for (int e = 0; e < 16; e++) p2[e] = rec[256 + e * 16 + l];
+        for (int e = 0; e < 16; e++) p2[e] = ld_stream(rec + 256 + e * 16 + l);
     }
 };
 template <typename V> struct Loader<float2, V> {
@@ -135,10 +135,10 @@ template <typename V> struct Loader<float2, V> {
         for (int mat = 0; mat < 2; mat++)
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const float4 q = *(const float4 *)(rec + mat * 256 + (u * 16 + l) * 2);
+                const uint4 q = ld_stream_u4(rec + mat * 256 + (u * 16 + l) * 2);
                 V *p = mat ? p2 : p1;
-                p[2 * u].x = q.x; p[2 * u].y = q.y;
-                p[2 * u + 1].x = q.z; p[2 * u + 1].y = q.w;
+                p[2 * u].x = __uint_as_float(q.x); p[2 * u].y = __uint_as_float(q.y);
+                p[2 * u + 1].x = __uint_as_float(q.z); p[2 * u + 1].y = __uint_as_float(q.w);
             }
     }
 };
@@ -149,7 +149,7 @@ template <typename V> struct Loader<bf16c, V> {
         for (int mat = 0; mat < 2; mat++)
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const uint4 q = *(const uint4 *)(rec + mat * 256 + (u * 16 + l) * 4);
+                const uint4 q = ld_stream_u4(rec + mat * 256 + (u * 16 + l) * 4);
                 V *p = mat ? p2 : p1;
                 const uint32_t w[4] = { q.x, q.y, q.z, q.w };
 #pragma unroll
