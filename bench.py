@@ -37,7 +37,7 @@ HBM_PEAK_GBPS = 8000.0
 # WRITE_SIZE runs of this script (calibrated against a 1 GiB elementwise kernel:
 # FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); key (preset, kernel kind) -> (bytes, profile file)
 MEASURED_TRAFFIC = {
-    (2, "tiles"): (31.697e9, "profiles/k_tiles_soa_nb2_nk65536_58Mpairs.txt"),
+    (2, "tiles"): (31.402e9, "profiles/k_tiles_soa_nb2_nk65536_58Mpairs.txt"),
 }
 PREC_NAME = {0: "fp64", 1: "c64", 2: "bf16"}
 
