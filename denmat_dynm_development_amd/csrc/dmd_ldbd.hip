@@ -115,6 +115,7 @@ extern "C" int dmd_scatt_load_P(dmd_scatt *h, const char *dir, const dmd_ldbd_in
         DMD_HIP(hipEventCreateWithFlags(&S.ev[s], hipEventDisableTiming));
     }
     const int64_t nchunks = (h->np + chunk - 1) / chunk;
+    h->has_P = false;  // until every chunk is placed: a failure leaves no half-loaded P in use
     for (int64_t c = 0; c < nchunks; c++) {
         const int s = (int)(c & 1);
         const int64_t first = c * chunk, n = std::min(chunk, h->np - first);
@@ -172,6 +173,7 @@ extern "C" int dmd_scatt_load_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_i
     std::vector<double2> sv;
     std::vector<int32_t> iv, jv;
     const int64_t nchunks = (h->np + chunk - 1) / chunk;
+    h->has_P = false;  // until every chunk is placed: a failure leaves no half-loaded P in use
     for (int64_t c = 0; c < nchunks; c++) {
         const int s = (int)(c & 1);
         const int64_t first = c * chunk, n = std::min(chunk, h->np - first);

@@ -1156,8 +1156,11 @@ static int put_P(dmd_scatt *h, int64_t first, int64_t count, const double *P1, c
     }
     hipError_t e = hipStreamSynchronize(h->st);
     dfree(s1); dfree(s2);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc) {
+        h->has_P = false;  // a failure after the first chunk leaves P partly written
+        return rc;
+    }
     h->has_P = true;
     h->linear = false;
     return DMD_OK;
@@ -1243,9 +1246,9 @@ extern "C" int dmd_scatt_threshold_P(dmd_scatt *h, double thr)
     }
     hipError_t e = hipStreamSynchronize(h->st);
     dfree(s1); dfree(s2);
-    if (rc) return rc;
-    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
-    return DMD_OK;
+    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc) h->has_P = false;  // partly thresholded: not to be used
+    return rc;
 }
 
 extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)

@@ -491,7 +491,16 @@ int run(const std::string &folder, bool dry)
     // the time loop of evolve_gsl / evolve_euler (DensityMatrix_Dynamics.h:177-225)
     double ti = t, hstep = s.hstart;
     dmd_ode_stats st{}, tot{};
+    const std::string exit_file = base + "/EXIT_DMD";
     for (int it = 1; t < s.tend; it++) {
+        struct stat xs;
+        if (stat(exit_file.c_str(), &xs) == 0) {  // "touch EXIT_DMD" stops the run (DensityMatrix_Dynamics.h:244)
+            std::remove(exit_file.c_str());
+            std::printf("dmd_run: EXIT_DMD found at t = %.10le, clean exit\n", t);
+            for (auto &r : reps) std::fclose(r.fp);
+            for (auto &r : taus) std::fclose(r.fp);
+            return 0;
+        }
         report_tau(it);
         ti += s.tstep;
         if (s.ode == "rkf45") {

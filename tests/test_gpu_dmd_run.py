@@ -345,3 +345,20 @@ def test_dmd_run_sparse_modes(tmp_path):
     (d2 / "param.in").write_text("\n".join(common + ["alg_Pin_is_sparse = 1", "alg_linearize = 1"]) + "\n")
     r = subprocess.run([RUN, str(d2)], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "linearization for sparse P matrices is not implemented" in r.stderr
+
+
+def test_dmd_run_exit_file(tmp_path):
+    """A file EXIT_DMD in the run folder stops the evolution before the next
+    step with exit status 0 and is removed (DensityMatrix_Dynamics.h:244)."""
+    from denmat_dynm_development_amd import scatt
+    print("case dmd_run EXIT_DMD", flush=True)
+    d = tmp_path / "run"
+    (d / "ldbd_data").mkdir(parents=True)
+    scatt.write_ldbd(str(d / "ldbd_data"), scatt.synth_params(0))
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 50\nfreq_measure = 1\n")
+    (d / "EXIT_DMD").write_text("")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-800:], r.stderr[-800:])
+    assert r.returncode == 0 and "clean exit" in r.stdout
+    assert not (d / "EXIT_DMD").exists()
+    assert np.loadtxt(d / "sz_elec_tot.out").shape == (3,)  # the t = 0 report only

@@ -79,13 +79,18 @@ def test_load_sP_rejects_bad_files(tmp_path):
     write_ldbd_folder(d, 0, 2, 0, (0, 2), (0, 2), e, k1, k2)
     s1, s2 = sparse_lists(P1), sparse_lists(P2)
     with scatt.ScatteringOperator(nk, 2, k1, k2, e) as op:
+        op.upload_P(P1, P2)
         write_sP_files(d, 2, s2)
         for bad, what in (((s1[0], s1[1][:-1], s1[2], s1[3]), "size"),
-                          ((s1[0], s1[1], s1[2] + 3, s1[3]), "index"),
-                          ((s1[0][:-1], s1[1], s1[2], s1[3]), "size")):
+                          ((s1[0][:-1], s1[1], s1[2], s1[3]), "size"),
+                          ((s1[0], s1[1], s1[2] + 3, s1[3]), "index")):
             write_sP_files(d, 1, bad)
             with pytest.raises(_lib.DMDError, match=what):
                 op.load_sP(d)
+        # the size checks leave the uploaded P alone; the index failure comes
+        # inside the chunk loop, after which no half-loaded P is used
+        with pytest.raises(_lib.DMDError, match="P has not been"):
+            op.eval(0.0, random_rho(nk, 2, rng))
         write_sP_files(d, 1, s1)
         op.load_sP(d)
         A1, _ = op.download_P()
