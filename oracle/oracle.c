@@ -25,7 +25,10 @@
  *                          with phases, for any element order.
  *   oracle_compute_P       Gram-form positivity, the P1(k,k')/P2(k',k)
  *                          conjugate identity, ImSigma from the P diagonal
- *                          against a direct golden-rule sum.
+ *                          against a direct golden-rule sum; with
+ *                          oracle_evolve, the initializer's golden-rule T1
+ *                          (T1_rate_dRho) equals the one-step Lindblad T1
+ *                          for any drho (tests/test_oracle_t1_rate.py).
  *   oracle_coherent        two-level precession closed form, tracelessness.
  *   oracle_rk4             1+z+z^2/2+z^3/6+z^4/24 amplification, O(h^4).
  *   oracle_spin            brute force, zero at rho=f, linearity, and the
