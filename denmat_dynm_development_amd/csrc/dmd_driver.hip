@@ -819,6 +819,9 @@ extern "C" int dmd_scatt_rkf45_apply(dmd_scatt *h, double *t, double t1, double 
         bool final_step = htry >= t1 - tc;
         double ratio = 0;
         for (;;) {
+            // an error estimate that never passes would shrink h to nothing: stop instead
+            if (!(htry > 0) || tc + htry == tc)
+                return fail(DMD_ERR_STATE, "rkf45: the step size underflowed at t = " + std::to_string(tc));
             if ((rc = rkf45_attempt(h, tc, htry, eps_abs, eps_rel, &ratio))) return rc;
             st.rhs_evals += 6;
             if (!(ratio == ratio)) return fail(DMD_ERR_STATE, "rkf45: the error estimate is not finite");

@@ -126,3 +126,19 @@ def test_compute_rhs_is_the_whole_right_hand_side():
             # one Euler step is rho + dt * (this right-hand side), Hermitian already
             dt = 0.5 * FS
             assert np.abs(op.integrate_euler(rho, t, dt, 1) - (rho + dt * got)).max() < 1e-15 + 1e-13 * dt * np.abs(got).max()
+
+
+def test_rkf45_stops_when_the_step_underflows():
+    """An error bound no step can meet (eps_abs = 1e-300) shrinks h by 0.2 per
+    rejection until t + h == t: the driver returns an error there instead of
+    looping (GSL's driver also gives up once no progress is possible)."""
+    _lib, scatt, oracle = _mods()
+    print("case rkf45 underflow", flush=True)
+    rng = np.random.default_rng(84)
+    nk, nb = 6, 2
+    op, e = _coherent_only(scatt, nk, nb, rng)
+    with op:
+        rho = random_rho(nk, nb, rng, scale=0.3, diag=0.4)
+        t0 = 100 * FS
+        with pytest.raises(_lib.DMDError, match="underflowed"):
+            op.rkf45_apply(rho, t0, t0 + FS, FS, 0.0, FS, 1e-300, 0.0)
