@@ -325,10 +325,21 @@ static int build_visits(dmd_scatt *h)
     std::vector<int4> seg;
     int64_t v = 0;
     h->h_vptr.assign(nk + 1, 0);
+    std::vector<int64_t> perm;
     for (int64_t k = 0; k < nk; k++) {
         h->h_vptr[k] = v;
+        const int64_t v0 = v;
         for (int64_t p = rowptr[k]; p < rowptr[k + 1]; p++, v++) { vslot[v] = fwdpos[p]; vdir[v] = 0; vq[v] = col[p]; }
         for (int64_t p = revptr[k]; p < revptr[k + 1]; p++, v++) { vslot[v] = revpos[p]; vdir[v] = 1; vq[v] = revcol[p]; }
+        // a row's visits by partner, so that a partner-block segment order is contiguous
+        perm.resize(v - v0);
+        for (int64_t q = 0; q < v - v0; q++) perm[q] = v0 + q;
+        std::stable_sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) { return vq[x] < vq[y]; });
+        std::vector<int64_t> s2(perm.size());
+        std::vector<uint8_t> d2(perm.size());
+        std::vector<int32_t> q2(perm.size());
+        for (size_t q = 0; q < perm.size(); q++) { s2[q] = vslot[perm[q]]; d2[q] = vdir[perm[q]]; q2[q] = vq[perm[q]]; }
+        for (size_t q = 0; q < perm.size(); q++) { vslot[v0 + q] = s2[q]; vdir[v0 + q] = d2[q]; vq[v0 + q] = q2[q]; }
     }
     h->h_vptr[nk] = v;
     int rc;
@@ -360,24 +371,45 @@ static int build_visits(dmd_scatt *h)
     }
     dfree(d_vslot); dfree(d_vdir);
     if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("building visit records: ") + hipGetErrorString(e));
+    h->h_vq = std::move(vq);
     h->bvis_ready = true;
     h->bvis_dirty = false;
     h->seg_len = 0;
+    h->seg_pblock = 0;
     h->tuned_Bp = 0;
     return DMD_OK;
 }
 
-// segments of at most seg_len visits per row, longest first
-static int build_segments(dmd_scatt *h, int seg_len)
+// Segments of at most seg_len visits of one row, longest first.  With
+// pblock > 0 a segment also keeps to one block of pblock partner rows, and
+// the segments run block after block, so the partner operands of the running
+// wavefronts share the L2 (a block of X at B trajectories is 64 B x B per row).
+static int build_segments(dmd_scatt *h, int seg_len, int pblock)
 {
-    if (h->seg_len == seg_len && h->d_seg) return DMD_OK;
+    if (h->seg_len == seg_len && h->seg_pblock == pblock && h->d_seg) return DMD_OK;
     std::vector<int4> seg;
-    for (int64_t k = 0; k < h->nk; k++)
-        for (int64_t s = h->h_vptr[k]; s < h->h_vptr[k + 1]; s += seg_len) {
-            const int n = (int)std::min<int64_t>(seg_len, h->h_vptr[k + 1] - s);
-            seg.push_back(make_int4((int)k, (int)(uint32_t)(s & 0xFFFFFFFF), (int)(s >> 32), n));
+    std::vector<int32_t> blk;
+    for (int64_t k = 0; k < h->nk; k++) {
+        const int64_t e = h->h_vptr[k + 1];
+        for (int64_t s = h->h_vptr[k]; s < e;) {
+            int64_t t = s + 1;
+            const int32_t b = pblock > 0 ? h->h_vq[s] / pblock : 0;
+            while (t < e && t - s < seg_len && (pblock <= 0 || h->h_vq[t] / pblock == b)) t++;
+            seg.push_back(make_int4((int)k, (int)(uint32_t)(s & 0xFFFFFFFF), (int)(s >> 32), (int)(t - s)));
+            blk.push_back(b);
+            s = t;
         }
-    std::stable_sort(seg.begin(), seg.end(), [](const int4 &a, const int4 &b) { return a.w > b.w; });
+    }
+    std::vector<int64_t> order(seg.size());
+    for (size_t q = 0; q < order.size(); q++) order[q] = (int64_t)q;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return blk[a] != blk[b] ? blk[a] < blk[b] : seg[a].w > seg[b].w;
+    });
+    {
+        std::vector<int4> sorted(seg.size());
+        for (size_t q = 0; q < order.size(); q++) sorted[q] = seg[order[q]];
+        seg.swap(sorted);
+    }
     dfree(h->d_seg);
     h->d_seg = nullptr;
     int rc;
@@ -385,6 +417,7 @@ static int build_segments(dmd_scatt *h, int seg_len)
     if (!seg.empty()) DMD_HIP(hipMemcpy(h->d_seg, seg.data(), seg.size() * sizeof(int4), hipMemcpyHostToDevice));
     h->nseg = (int64_t)seg.size();
     h->seg_len = seg_len;
+    h->seg_pblock = pblock;
     return DMD_OK;
 }
 
@@ -396,26 +429,41 @@ static int autotune(dmd_scatt *h, int Bp)
 {
     const char *env = std::getenv("DMD_BATCH_TUNE");
     const bool tune = !(env && env[0] == '0');
-    int best_seg = 128, best_nb = (Bp % 64 == 0) ? 2 : 1;
+    // DMD_BATCH_PBLOCK=n fixes the partner-block size of the segment order (0: row order)
+    const char *penv = std::getenv("DMD_BATCH_PBLOCK");
+    const int forced_pb = penv && *penv ? std::max(0, std::atoi(penv)) : -1;
+    int best_seg = 128, best_nb = (Bp % 64 == 0) ? 2 : 1, best_pb = forced_pb > 0 ? forced_pb : 0;
+    auto trial = [&](int nb, float *ms) -> int {
+        launch_batched_core(h, Bp, nb);  // warm
+        DMD_HIP(hipEventRecord(h->ev0, h->st));
+        for (int r = 0; r < 2; r++) launch_batched_core(h, Bp, nb);
+        DMD_HIP(hipEventRecord(h->ev1, h->st));
+        DMD_HIP(hipEventSynchronize(h->ev1));
+        DMD_HIP(hipEventElapsedTime(ms, h->ev0, h->ev1));
+        return DMD_OK;
+    };
     if (tune && h->np > 0) {
-        float best = 1e30f;
+        float best = 1e30f, ms = 0;
+        int rc;
         for (int sl : { 64, 128, 256 }) {
-            int rc = build_segments(h, sl);
-            if (rc) return rc;
+            if ((rc = build_segments(h, sl, best_pb))) return rc;
             for (int nb : { 1, 2 }) {
                 if (nb == 2 && Bp % 64) continue;
-                launch_batched_core(h, Bp, nb);  // warm
-                DMD_HIP(hipEventRecord(h->ev0, h->st));
-                for (int r = 0; r < 2; r++) launch_batched_core(h, Bp, nb);
-                DMD_HIP(hipEventRecord(h->ev1, h->st));
-                DMD_HIP(hipEventSynchronize(h->ev1));
-                float ms = 0;
-                DMD_HIP(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                if ((rc = trial(nb, &ms))) return rc;
                 if (ms < best) { best = ms; best_seg = sl; best_nb = nb; }
             }
         }
+        // partner blocks that fit the L2 of one XCD (4 MiB) and four times that
+        if (forced_pb < 0) {
+            const int p0 = std::max(256, 65536 / Bp);
+            for (int pb : { p0, 4 * p0 }) {
+                if ((rc = build_segments(h, best_seg, pb))) return rc;
+                if ((rc = trial(best_nb, &ms))) return rc;
+                if (ms < best) { best = ms; best_pb = pb; }
+            }
+        }
     }
-    int rc = build_segments(h, best_seg);
+    int rc = build_segments(h, best_seg, best_pb);
     if (rc) return rc;
     h->batch_nblk = best_nb;
     h->tuned_Bp = Bp;

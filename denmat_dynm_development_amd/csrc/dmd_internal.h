@@ -178,6 +178,8 @@ struct dmd_scatt {
     int seg_len = 0;                     // visits per segment (autotuned)
     int tuned_Bp = 0;                    // padded B the tuning was done for
     std::vector<int64_t> h_vptr;         // first visit of every row (host)
+    std::vector<int32_t> h_vq;           // partner row of every visit (host; sorted within each row)
+    int seg_pblock = 0;                  // partner-block size of the segment order (0: row order; autotuned)
     uint16_t *d_xb = nullptr;            // nk x 32 x Bp BF16 hi/lo drho operand
     float *d_Fb = nullptr;               // nk x 64 x Bp row sums
     double2 *d_rhoSB = nullptr;          // Bp x nk x 16 rho_S

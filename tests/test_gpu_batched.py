@@ -125,3 +125,41 @@ def test_rk4_batched_follows_unbatched(monkeypatch):
     assert d_state < 1e-4 and d_obs < 1e-4
     # graph replay and eager stepping agree (FP32 atomics: same to rounding)
     assert rel_err(rb2 - rho0, rb - rho0) < 1e-5
+
+
+def test_partner_block_segment_orders_agree(monkeypatch):
+    """The segment order the autotuner may pick (row order, or visits grouped
+    in blocks of partner rows so the partner operands share the L2) changes
+    only the FP32 summation order: every forced order matches the oracle on
+    the stored P, and the orders match each other."""
+    scatt, oracle = _mods()
+    print("case batched partner blocks", flush=True)
+    rng = np.random.default_rng(140)
+    nk, nb, B = 200, 4, 32
+    k1, k2 = _hub_pairs(nk, 4, rng)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(B, nk, nb, f, rng)
+    t = 31 * FS
+    prefac = 2 * np.pi / nk
+    rho_f = np.zeros((nk, nb * nb), np.complex128)
+    rho_f[:, ::nb + 1] = f
+    results = []
+    for pblock in (0, 7, 64):
+        monkeypatch.setenv("DMD_BATCH_PBLOCK", str(pblock))
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=1) as op:
+            op.upload_P(P1, P2)
+            op.set_split(f)
+            Q1, Q2 = op.download_P()
+            got = op.eval_batched(t, rho)
+            base = op.eval(t, rho_f)
+        results.append(got - base[None])
+    monkeypatch.delenv("DMD_BATCH_PBLOCK")
+    ref_f = oracle.evolve(k1, k2, Q1, Q2, e, rho_f, t=t, prefac=prefac)
+    for b in (0, B - 1):
+        ref = oracle.evolve(k1, k2, Q1, Q2, e, rho[b], t=t, prefac=prefac)
+        for r in results:
+            assert rel_err(r[b], ref - ref_f) < 5e-6
+    for r in results[1:]:
+        assert rel_err(r, results[0]) < 2e-6
