@@ -453,10 +453,10 @@ static int autotune(dmd_scatt *h, int Bp)
                 if (ms < best) { best = ms; best_seg = sl; best_nb = nb; }
             }
         }
-        // partner blocks that fit the L2 of one XCD (4 MiB) and four times that
+        // partner blocks that fit the L2 of one XCD (4 MiB), and 4x and 16x that
         if (forced_pb < 0) {
             const int p0 = std::max(256, 65536 / Bp);
-            for (int pb : { p0, 4 * p0 }) {
+            for (int pb : { p0, 4 * p0, 16 * p0 }) {
                 if ((rc = build_segments(h, best_seg, pb))) return rc;
                 if ((rc = trial(best_nb, &ms))) return rc;
                 if (ms < best) { best = ms; best_pb = pb; }
