@@ -305,8 +305,12 @@ extern "C" int dmd_ldbd_write_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_i
 // conduction bands, electron.cpp:177-179), the stored pairs and P (scaled by
 // scale_eph, ElectronPhonon.cpp:225,232), and sets the band window of
 // evolve_driver when the e-ph window is narrower than the density matrix's.
+// P source of create_from: the dense files, the sparse files, none (P unset),
+// or no scattering (no stored pairs, empty P)
+enum { SRC_DENSE = 0, SRC_SPARSE = 1, SRC_NONE = 2, SRC_NO_PAIRS = 3 };
+
 static int create_from(const char *dir, const char *scatt, double scale_eph, double scissor, int precision,
-                       bool sparse, dmd_scatt **out)
+                       int source, dmd_scatt **out)
 {
     if (!out) return fail(DMD_ERR_ARG, "dmd_scatt_create_from_ldbd: null out");
     *out = nullptr;
@@ -323,17 +327,19 @@ static int create_from(const char *dir, const char *scatt, double scale_eph, dou
         for (int b = 0; b < nbd; b++) edm[k * nbd + b] = e[k * info.nb + info.b_bot_dm + b];
         for (int b = 0; b < nbe; b++) eeph[k * nbe + b] = edm[k * nbd + b0 + b];
     }
-    std::vector<uint64_t> k1(info.nkpair), k2(info.nkpair);
-    if ((rc = dmd_ldbd_read_kpairs(dir, &info, 0, info.nkpair, k1.data(), k2.data()))) return rc;
+    const int64_t np = source == SRC_NO_PAIRS ? 0 : info.nkpair;
+    std::vector<uint64_t> k1(np), k2(np);
+    if (np && (rc = dmd_ldbd_read_kpairs(dir, &info, 0, np, k1.data(), k2.data()))) return rc;
     dmd_scatt *h = nullptr;
-    if ((rc = dmd_scatt_create(nk, nbe, info.nkpair, k1.data(), k2.data(), eeph.data(), info.nk_full, precision, &h)))
+    if ((rc = dmd_scatt_create(nk, nbe, np, k1.data(), k2.data(), eeph.data(), info.nk_full, precision, &h)))
         return rc;
     if (nbd != nbe && (rc = dmd_scatt_set_window(h, nbd, b0, edm.data()))) {
         dmd_scatt_destroy(h);
         return rc;
     }
-    rc = sparse ? dmd_scatt_load_sP(h, dir, &info, scatt, scale_eph, 1e-40)
-                : dmd_scatt_load_P(h, dir, &info, scatt, 0, scale_eph, 0);
+    if (source == SRC_SPARSE) rc = dmd_scatt_load_sP(h, dir, &info, scatt, scale_eph, 1e-40);
+    else if (source == SRC_DENSE) rc = dmd_scatt_load_P(h, dir, &info, scatt, 0, scale_eph, 0);
+    else if (source == SRC_NO_PAIRS) h->has_P = true;  // nothing to scatter
     if (rc) {
         dmd_scatt_destroy(h);
         return rc;
@@ -345,7 +351,15 @@ static int create_from(const char *dir, const char *scatt, double scale_eph, dou
 extern "C" int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, double scale_eph, double scissor,
                                           int precision, dmd_scatt **out)
 {
-    return create_from(dir, scatt, scale_eph, scissor, precision, false, out);
+    return create_from(dir, scatt, scale_eph, scissor, precision, SRC_DENSE, out);
+}
+
+// alg_eph_enable = 0 (P for the e-i fold-in to fill) or alg_scatt_enable = 0
+// (no stored pairs)
+extern "C" int dmd_scatt_create_from_ldbd_noP(const char *dir, double scissor, int precision, int no_pairs,
+                                              dmd_scatt **out)
+{
+    return create_from(dir, "lindblad", 0.0, scissor, precision, no_pairs ? SRC_NO_PAIRS : SRC_NONE, out);
 }
 
 // alg_Pin_is_sparse: the sparse files' P, scaled and re-sparsified at 1e-40
@@ -353,5 +367,5 @@ extern "C" int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, do
 extern "C" int dmd_scatt_create_from_ldbd_sparse(const char *dir, const char *scatt, double scale_eph, double scissor,
                                                  int precision, dmd_scatt **out)
 {
-    return create_from(dir, scatt, scale_eph, scissor, precision, true, out);
+    return create_from(dir, scatt, scale_eph, scissor, precision, SRC_SPARSE, out);
 }

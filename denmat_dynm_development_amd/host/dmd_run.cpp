@@ -51,6 +51,7 @@ struct cplx {
 
 struct Settings {
     bool restart = false, tau_only = false, sparseP = false, Pin_is_sparse = false;
+    bool scatt_enable = true, eph_enable = true, only_eimp = false;
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
@@ -87,8 +88,9 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     if (p.str("material_model", "none") != "none") throw std::runtime_error("material_model: model materials are not read");
     s.restart = p.get("restart", 0) != 0;
     s.tau_only = p.get("compute_tau_only", 0) != 0;  // parameters.cpp:73
-    if (p.get("alg_scatt_enable", 1) == 0 || p.get("alg_eph_enable", 1) == 0)
-        throw std::runtime_error("alg_scatt_enable / alg_eph_enable = 0: this runner drives the scattering operator");
+    s.scatt_enable = p.get("alg_scatt_enable", 1) != 0;  // parameters.cpp:78-79,86
+    s.eph_enable = p.get("alg_eph_enable", 1) != 0;
+    s.only_eimp = p.get("alg_only_eimp", 0) != 0;
     const double scale_scatt = p.get("scale_scatt", 1.);
     s.scale_eph = p.get("scale_eph", scale_scatt);
     s.scale_ei = p.get("scale_ei", scale_scatt);
@@ -110,7 +112,6 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
     if (s.Pin_is_sparse && s.linearize)  // ElectronPhonon_linear.cpp:140-143
         throw std::runtime_error("linearization for sparse P matrices is not implemented");
     refuse(p, "alg_phenom_relax", 0, "phenomenological relaxation");
-    refuse(p, "alg_only_eimp", 0, "alg_only_eimp");
     refuse(p, "alg_only_ee", 0, "alg_only_ee");
     refuse(p, "alg_only_intravalley", 0, "valley filtering");
     refuse(p, "alg_only_intervalley", 0, "valley filtering");
@@ -151,6 +152,17 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
             throw std::runtime_error("impMode" + sfx + " = " + mode + ": only ab initio e-i P (ab_neutral) is read");
         s.ni.push_back(n);
     }
+    // parameters.cpp:459-464,485-486,493-497
+    if (s.only_eimp && s.ni.empty()) throw std::runtime_error("alg_only_eimp is only possible if impurity_density is non-zero");
+    if ((!s.ni.empty() || s.eph_enable) && !s.scatt_enable)
+        throw std::runtime_error("alg_scatt_enable must be true if any scattering mechanism is considered");
+    if (!s.eph_enable && s.ni.empty() && s.scatt_enable)
+        throw std::runtime_error("alg_scatt_enable must be false if no scattering mechanism is considered");
+    if (s.eph_enable && s.only_eimp)
+        throw std::runtime_error("if only e-imp or only e-e, e-ph should not be enabled. Set alg_eph_enable = 0");
+    if (s.sparseP && !s.ni.empty()) throw std::runtime_error("ab initio neutral e-imp does not support sparse matrices");
+    if (s.Pin_is_sparse && !s.eph_enable && s.ni.empty())
+        throw std::runtime_error("alg_Pin_is_sparse implies ab initio e-ph or e-i is enabled");
     s.t0 = p.get("t0", 0., fs());
     s.tend = p.get("tend", 0., fs());
     s.tstep = p.get("tstep", 1., fs());
@@ -295,6 +307,8 @@ int run(const std::string &folder, bool dry)
                 s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
     std::printf("alg_sparseP = %d alg_Pin_is_sparse = %d alg_thr_sparseP = %lg\n", (int)s.sparseP, (int)s.Pin_is_sparse,
                 s.thr_sparseP);
+    std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d\n", (int)s.scatt_enable, (int)s.eph_enable,
+                (int)s.only_eimp);
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
     if (dry) return 0;
 
@@ -348,8 +362,12 @@ int run(const std::string &folder, bool dry)
     }
 
     // the scattering operator: e-ph P, ab initio e-i P, flags, ddmdt_eq, H_BS
-    electronphonon_mi355x<cplx> eph(ldbd, s.scatt, s.scale_eph, s.scissor, DMD_PREC_FP64, s.Pin_is_sparse,
-                                    s.thr_sparseP);
+    using Teph = electronphonon_mi355x<cplx>;
+    const Teph::p_source src = !s.scatt_enable ? Teph::P_NO_SCATTERING
+                               : !s.eph_enable ? Teph::P_NONE
+                               : s.Pin_is_sparse ? Teph::P_SPARSE : Teph::P_DENSE;
+    Teph eph(ldbd, s.scatt, s.scale_eph, s.scissor, DMD_PREC_FP64, src, s.thr_sparseP);
+    eph.Pin_is_sparse = s.Pin_is_sparse;  // e-i folded into sparse input is re-sparsified
     for (size_t iD = 0; iD < s.ni.size(); iD++) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
     // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
     // alg_linearize set_eph returns before it (:6)
@@ -358,7 +376,7 @@ int run(const std::string &folder, bool dry)
         std::printf("sparse P: |P| <= %lg zeroed, sP files in %s\n", s.thr_sparseP, ldbd.c_str());
     }
     eph.set_algorithm(s.expt, !interaction);
-    if (s.ddmdteq || s.linearize) {  // DensityMatrix_Dynamics.h:57-65
+    if ((s.ddmdteq || s.linearize) && s.scatt_enable) {  // DensityMatrix_Dynamics.h:57-65
         std::vector<double> feq((size_t)nk * nbd);
         const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
         const double muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;

@@ -41,24 +41,32 @@ public:
     std::vector<impurity_slot> eimp_slots;
     std::vector<impurity_slot *> eimp;
     void *ee_model = nullptr;
+    // where the e-ph P comes from: the ldbd_P files, the sparse sP files
+    // (alg_Pin_is_sparse, ElectronPhonon.cpp:235-246), none (alg_eph_enable =
+    // 0: the e-i fold-in fills P, :211), or no scattering at all
+    // (alg_scatt_enable = 0, ElectronPhonon.h:72)
+    enum p_source { P_DENSE = 0, P_SPARSE = 1, P_NONE = 2, P_NO_SCATTERING = 3 };
     // alg.Pin_is_sparse and alg.thr_sparseP (parameters.cpp:118-120)
     bool Pin_is_sparse = false;
     double thr_sparseP = 1e-40;
 
     // electronphonon's set-up from ldbd_data (ElectronPhonon.h:53-93 and
-    // set_eph, ElectronPhonon.cpp:3-13): P x scale_eph streamed into HBM, from
-    // the sparse sP files when Pin_is_sparse (ElectronPhonon.cpp:235-246).
+    // set_eph, ElectronPhonon.cpp:3-13): P x scale_eph streamed into HBM.
     explicit electronphonon_mi355x(const std::string &ldbd_dir, const std::string &scatt = "lindblad",
                                    double scale_eph = 1.0, double scissor = 0.0, int precision = DMD_PREC_FP64,
-                                   bool Pin_is_sparse_ = false, double thr_sparseP_ = 1e-40)
-        : Pin_is_sparse(Pin_is_sparse_), thr_sparseP(thr_sparseP_), dir_(ldbd_dir), scatt_(scatt)
+                                   p_source source = P_DENSE, double thr_sparseP_ = 1e-40)
+        : Pin_is_sparse(source == P_SPARSE), thr_sparseP(thr_sparseP_), dir_(ldbd_dir), scatt_(scatt),
+          P_set_(source != P_NONE)
     {
         check(dmd_ldbd_read_info(dir_.c_str(), &info_), "ldbd_size.dat");
-        check(Pin_is_sparse ? dmd_scatt_create_from_ldbd_sparse(dir_.c_str(), scatt_.c_str(), scale_eph, scissor,
-                                                                precision, &h_)
-                            : dmd_scatt_create_from_ldbd(dir_.c_str(), scatt_.c_str(), scale_eph, scissor, precision,
-                                                         &h_),
-              "scattering operator from " + dir_);
+        int rc;
+        if (source == P_SPARSE)
+            rc = dmd_scatt_create_from_ldbd_sparse(dir_.c_str(), scatt_.c_str(), scale_eph, scissor, precision, &h_);
+        else if (source == P_DENSE)
+            rc = dmd_scatt_create_from_ldbd(dir_.c_str(), scatt_.c_str(), scale_eph, scissor, precision, &h_);
+        else
+            rc = dmd_scatt_create_from_ldbd_noP(dir_.c_str(), scissor, precision, source == P_NO_SCATTERING, &h_);
+        check(rc, "scattering operator from " + dir_);
         bStart = info_.b_bot_eph - info_.b_bot_dm;
         bEnd = info_.b_top_eph - info_.b_bot_dm;
         nb_expand = info_.b_top_dm - info_.b_bot_dm;
@@ -80,8 +88,10 @@ public:
     // under Pin_is_sparse the sum is re-sparsified at thr_sparseP (:91-94).
     void add_eimp_ab_neutral(int iD, double fraction, double scale_ei)
     {
-        check(dmd_scatt_load_P(h_, dir_.c_str(), &info_, scatt_.c_str(), iD + 1, fraction * scale_ei, 1),
+        // without e-ph P the first defect sets P (P = 0 + P_ei, as alg_eph_enable = 0 leaves it)
+        check(dmd_scatt_load_P(h_, dir_.c_str(), &info_, scatt_.c_str(), iD + 1, fraction * scale_ei, P_set_ ? 1 : 0),
               "e-i P of defect " + std::to_string(iD + 1));
+        P_set_ = true;
         if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
         if ((int)eimp_slots.size() <= iD) eimp_slots.resize(iD + 1);
         eimp.clear();
@@ -152,6 +162,7 @@ public:
 
 private:
     std::string dir_, scatt_;
+    bool P_set_ = true;
     dmd_ldbd_info info_{};
     dmd_scatt *h_ = nullptr;
 };

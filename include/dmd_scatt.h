@@ -363,6 +363,12 @@ int dmd_scatt_threshold_P(dmd_scatt *h, double thr);
  * order; a matrix whose _s file exists is not written again.  All four files
  * carry the <scatt>[_hole] suffix the reader looks for (ElectronPhonon.cpp:241). */
 int dmd_ldbd_write_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, const char *scatt, double thr);
+/* The handle of an ldbd_data folder without the e-ph P: alg_eph_enable = 0
+ * (ElectronPhonon.cpp:211, P left zero for the e-i fold-in to fill; P counts
+ * as unset until dmd_scatt_load_P / _load_sP / _upload_P gives it), or, with
+ * no_pairs = 1, alg_scatt_enable = 0 (ElectronPhonon.h:72: no scattering; the
+ * handle has no stored pairs and its right-hand side is the coherent term). */
+int dmd_scatt_create_from_ldbd_noP(const char *dir, double scissor, int precision, int no_pairs, dmd_scatt **out);
 /* dmd_scatt_create_from_ldbd with P from the sparse files (alg_Pin_is_sparse),
  * scaled by scale_eph and kept when |scale_eph*s| > 1e-40. */
 int dmd_scatt_create_from_ldbd_sparse(const char *dir, const char *scatt, double scale_eph, double scissor,

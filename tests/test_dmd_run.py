@@ -103,6 +103,11 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("alg_ode_method = rk2", "rkf45, euler or rk4"),
     ("alg_Pin_is_sparse = 1\nalg_linearize = 1", "linearization for sparse P matrices is not implemented"),
     ("alg_thr_sparseP = -1", "alg_thr_sparseP"),
+    ("alg_eph_enable = 0", "alg_scatt_enable must be false"),
+    ("alg_scatt_enable = 0", "alg_scatt_enable must be true"),
+    ("alg_eph_enable = 0\nalg_only_eimp = 1", "only possible if impurity_density"),
+    ("alg_only_eimp = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "Set alg_eph_enable = 0"),
+    ("alg_sparseP = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "does not support sparse"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
@@ -133,3 +138,15 @@ void driver_calls(dmdrun::electronphonon_mi355x<complex> *eph, complex **dm, com
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc[0], "-I", inc[1], str(src)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def test_dry_run_scattering_switches(tmp_path):
+    """alg_scatt_enable = 0 with alg_eph_enable = 0 (coherent dynamics only) and
+    alg_eph_enable = 0 with an ab initio impurity (e-i only) are accepted."""
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\nalg_scatt_enable = 0\nalg_eph_enable = 0\n"))
+    assert rc == 0, err
+    assert _num(out, "alg_scatt_enable") == 0 and _num(out, "alg_eph_enable") == 0
+    rc, out, err = _dry(_gaas_folder(tmp_path / "b", "tstep = 1\ntend = 10\nalg_eph_enable = 0\nalg_only_eimp = 1\n"
+                                     "impurity_density = 1e17\nimpMode = ab_neutral\n"))
+    assert rc == 0, err
+    assert _num(out, "alg_only_eimp") == 1 and _num(out, "impurities") == 1

@@ -362,3 +362,94 @@ def test_dmd_run_exit_file(tmp_path):
     assert r.returncode == 0 and "clean exit" in r.stdout
     assert not (d / "EXIT_DMD").exists()
     assert np.loadtxt(d / "sz_elec_tot.out").shape == (3,)  # the t = 0 report only
+
+
+@pytest.mark.parametrize("eph", [1, 0])
+def test_dmd_run_ab_initio_impurity(tmp_path, eph):
+    """impurity_density with impMode = ab_neutral: P = scale_eph P_eph +
+    |n_i cell_size| scale_ei P_D1 (ElectronPhonon.cpp:44-113); with
+    alg_eph_enable = 0 (and alg_only_eimp) the e-ph files are not read and P is
+    the e-i term alone.  S_z against the oracle's RK4 with that P."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    from helpers import random_P
+    print(f"case dmd_run e-i eph={eph}", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    info = ldbd.read_info(str(L))
+    P1, P2 = ldbd.read_P(str(L), info)
+    Q1, Q2 = random_P(info.nkpair, info.nb_eph, np.random.default_rng(7))
+    Q1.astype(np.complex128).tofile(L / "ldbd_P1_lindblad_D1.bin")
+    Q2.astype(np.complex128).tofile(L / "ldbd_P2_lindblad_D1.bin")
+    if not eph:
+        for f in ("ldbd_P1_lindblad.bin", "ldbd_P2_lindblad.bin"):
+            (L / f).unlink()
+    n_cm3, se, sei = 3e17, 0.8, 1.3
+    lines = ["alg_ode_method = rk4", "Bzpert = 2", "tstep = 1", "tend = 4", "freq_measure = 1",
+             f"impurity_density = {n_cm3}", "impMode = ab_neutral", f"scale_eph = {se}", f"scale_ei = {sei}"]
+    if not eph:
+        lines += ["alg_eph_enable = 0", "alg_only_eimp = 1"]
+    (d / "param.in").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-800:], r.stderr[-800:])
+    assert r.returncode == 0
+    e = ldbd.read_ek(str(L), info)
+    k1, k2 = ldbd.read_kpairs(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    cell = abs(np.linalg.det(R))
+    frac = abs(n_cm3 * BOHR2CM ** 3 * cell)
+    A1 = frac * sei * Q1 + (se * P1 if eph else 0)
+    A2 = frac * sei * Q2 + (se * P2 if eph else 0)
+    nk, nb = e.shape
+    f = _fermi(info.temperature, info.mu, e)
+    rho0 = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), info.temperature, 0.0, True).reshape(nk, nb, nb)
+    rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    _, obs = oracle.rk4(k1, k2, A1, A2, e, rho0, 1 * FS, 4, prefac=2 * np.pi / info.nk_full, report_every=1,
+                        s_obs=s3[:, 2], f_dm=f, i0=0, i1=nb, obs_prefac=1.0 / (info.nk_full * cell * BOHR2CM ** 3))
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", obs)
+    assert np.abs(rows[:, 1] - obs).max() < 1e-10 * np.abs(obs).max()
+
+
+def test_dmd_run_without_scattering(tmp_path):
+    """alg_scatt_enable = 0 (with alg_eph_enable = 0): no P or k-pair files are
+    read and the dynamics is the coherent term alone; S_z under Bx against the
+    oracle's RK4 with no stored pairs."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print("case dmd_run no scattering", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    for f in L.glob("ldbd_P*"):
+        f.unlink()
+    for f in L.glob("ldbd_kpair*"):
+        f.unlink()
+    (d / "param.in").write_text("alg_ode_method = rk4\nalg_scatt_enable = 0\nalg_eph_enable = 0\nBzpert = 2\nBx = 3\n"
+                                "tstep = 2\node_hmax = 1\ntend = 20\nfreq_measure = 2\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-800:], r.stderr[-800:])
+    assert r.returncode == 0
+    info = ldbd.read_info(str(L))
+    e = ldbd.read_ek(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    nk, nb = e.shape
+    f = _fermi(info.temperature, info.mu, e)
+    rho0 = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), info.temperature, 0.0, True).reshape(nk, nb, nb)
+    rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    none = np.zeros(0, np.uint64)
+    Z = np.zeros((0, nb ** 4), np.complex128)
+    _, obs = oracle.rk4(none, none, Z, Z, e, rho0, 1 * FS, 20, prefac=1.0, H=3 * TESLA * s3[:, 0], report_every=4,
+                        s_obs=s3[:, 2], f_dm=f, i0=0, i1=nb, obs_prefac=1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3))
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", obs)
+    assert rows.shape == (6, 3)
+    assert np.abs(rows[:, 1] - obs).max() < 1e-10 * np.abs(obs).max()
+    assert np.abs(rows[:, 1] - rows[0, 1]).max() > 1e-3 * abs(rows[0, 1])  # it precesses
