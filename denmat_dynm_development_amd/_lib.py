@@ -186,6 +186,7 @@ PROTOTYPES = {
     "dmd_scatt_threshold_P": [_vp, _d],
     "dmd_ldbd_write_sP": [_vp, _cs, _pLI, _cs, _d],
     "dmd_scatt_create_from_ldbd_sparse": [_cs, _cs, _d, _d, _i, ctypes.POINTER(_vp)],
+    "dmd_scatt_create_from_ldbd_noP": [_cs, _d, _i, _i, ctypes.POINTER(_vp)],
     "dmd_scatt_upload_P_range": [_vp, _i64, _i64, _vp, _vp, _d, _i],
     "dmd_restart_read": [_cs, _i64, _i, _vp, _pd],
     "dmd_restart_write": [_cs, _i64, _i, _vp, _d],

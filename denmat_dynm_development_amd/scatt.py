@@ -66,12 +66,14 @@ class ScatteringOperator:
 
     @classmethod
     def from_ldbd(cls, dirname, scatt="lindblad", scale_eph=1.0, scissor=0.0, precision=_lib.DMD_PREC_FP64,
-                  Pin_is_sparse=False):
+                  Pin_is_sparse=False, eph_enable=True, scatt_enable=True):
         """The handle of an ldbd_data folder, set up as the reference's
         electronphonon sets itself up (ElectronPhonon.h:53-93): e-ph window of
         e_dm, stored pairs, P x scale_eph streamed from the files (the sparse
         sP files with Pin_is_sparse, ElectronPhonon.cpp:235-246), and the band
-        window when the density matrix has more bands."""
+        window when the density matrix has more bands.  eph_enable=False
+        reads no P (the e-i fold-in, load_P(impurity=1, add=False), sets it);
+        scatt_enable=False keeps no stored pairs (coherent dynamics only)."""
         from . import ldbd
         info = ldbd.read_info(dirname)
         self = cls.__new__(cls)
@@ -80,8 +82,15 @@ class ScatteringOperator:
         self.n_pairs = int(info.nkpair)
         self.info = info
         h = ctypes.c_void_p()
-        call("dmd_scatt_create_from_ldbd_sparse" if Pin_is_sparse else "dmd_scatt_create_from_ldbd",
-             str(dirname).encode(), scatt.encode(), float(scale_eph), float(scissor), int(precision), ctypes.byref(h))
+        if not scatt_enable or not eph_enable:
+            call("dmd_scatt_create_from_ldbd_noP", str(dirname).encode(), float(scissor), int(precision),
+                 int(not scatt_enable), ctypes.byref(h))
+            if not scatt_enable:
+                self.n_pairs = 0
+        else:
+            call("dmd_scatt_create_from_ldbd_sparse" if Pin_is_sparse else "dmd_scatt_create_from_ldbd",
+                 str(dirname).encode(), scatt.encode(), float(scale_eph), float(scissor), int(precision),
+                 ctypes.byref(h))
         self._h = h
         return self
 

@@ -139,3 +139,26 @@ def test_reduced_precision_handles_from_files(tmp_path, prec):
         B1, B2 = b.download_P()
         assert np.array_equal(A1, B1) and np.array_equal(A2, B2)
         assert rel_err(a.eval(9 * FS, rho), b.eval(9 * FS, rho)) < 1e-13
+
+
+def test_handles_without_eph_P(tmp_path):
+    """from_ldbd(eph_enable=False): no P is read and P counts as unset until the
+    e-i term sets it; from_ldbd(scatt_enable=False): no stored pairs, ddm = 0."""
+    _lib, ldbd, scatt, oracle = _mods()
+    print("case handles without e-ph P", flush=True)
+    rng = np.random.default_rng(30)
+    nk, nbf = 40, 2
+    e = np.sort(rng.uniform(0.0, 0.01, (nk, nbf)), axis=1)
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    Q1, Q2 = random_P(len(k1), nbf, rng)
+    write_ldbd_folder(str(tmp_path), 0, nbf, 0, (0, nbf), (0, nbf), e, k1, k2, nk_full=1000.0, extra_P=[(1, Q1, Q2)])
+    rho = random_rho(nk, nbf, rng)
+    with scatt.ScatteringOperator.from_ldbd(tmp_path, eph_enable=False) as op:
+        with pytest.raises(_lib.DMDError, match="P has not been"):
+            op.eval(0.0, rho)
+        op.load_P(tmp_path, impurity=1, scale=0.4, add=False)
+        A1, A2 = op.download_P()
+        assert np.array_equal(A1, 0.4 * Q1) and np.array_equal(A2, 0.4 * Q2)
+    with scatt.ScatteringOperator.from_ldbd(tmp_path, scatt_enable=False) as op:
+        assert op.info()["n_pairs"] == 0
+        assert np.abs(op.eval(0.0, rho)).max() == 0.0
