@@ -1251,6 +1251,58 @@ extern "C" int dmd_scatt_threshold_P(dmd_scatt *h, double thr)
     return rc;
 }
 
+// P of the pairs with keep = 0 zeroed; a and b hold n pairs of per complex each
+__global__ void k_zero_pairs(int64_t n, int per, const uint8_t *__restrict__ keep, double2 *__restrict__ a,
+                             double2 *__restrict__ b)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, m = n * per;
+    if (idx >= 2 * m) return;
+    const int64_t e = idx < m ? idx : idx - m;
+    if (!keep[e / per]) (idx < m ? a : b)[e] = make_double2(0.0, 0.0);
+}
+
+// the pair filter of evolve (evolve_eph.cpp:52-58) applied to the stored P
+extern "C" int dmd_scatt_mask_pairs(dmd_scatt *h, const uint8_t *keep)
+{
+    if (!h || (!keep && h->np > 0)) return fail(DMD_ERR_ARG, "dmd_scatt_mask_pairs: bad arguments");
+    if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator, not P");
+    if (h->np == 0) return DMD_OK;
+    const int64_t per = h->nb4;
+    const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / (per * 16)));
+    double2 *s1 = nullptr, *s2 = nullptr;
+    uint8_t *dk = nullptr;
+    int rc;
+    if ((rc = dalloc_t(&s1, (size_t)(chunk * per), "mask staging"))) return rc;
+    if ((rc = dalloc_t(&s2, (size_t)(chunk * per), "mask staging"))) { dfree(s1); return rc; }
+    if ((rc = dalloc_t(&dk, (size_t)chunk, "mask staging"))) { dfree(s1); dfree(s2); return rc; }
+    for (int64_t first = 0; first < h->np && rc == DMD_OK; first += chunk) {
+        const int64_t n = std::min(chunk, h->np - first), tot = n * 2 * per;
+        bool any = false;
+        for (int64_t q = 0; q < n && !any; q++) any = keep[first + q] == 0;
+        if (!any) continue;  // nothing to drop in this chunk
+        hipError_t e = hipMemcpyAsync(dk, keep + first, (size_t)n, hipMemcpyHostToDevice, h->st);
+        if (e != hipSuccess) { rc = fail(DMD_ERR_HIP, hipGetErrorString(e)); break; }
+        if (h->precision != DMD_PREC_FP64) {
+            if ((rc = tiles4_fetch(h, first, n, s1, s2))) break;
+        } else {
+            hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, first, h->nb4, h->d_P,
+                               h->d_inv, h->soa, s1, s2);
+        }
+        hipLaunchKernelGGL(k_zero_pairs, dim3(nblk(tot, 256)), dim3(256), 0, h->st, n, (int)per, dk, s1, s2);
+        e = hipGetLastError();
+        if (e != hipSuccess) { rc = fail(DMD_ERR_HIP, hipGetErrorString(e)); break; }
+        if ((rc = place_P_dev(h, s1, s2, n, first, 1.0, 0))) break;
+        // keep + first is host memory the next chunk's copy may not overwrite before this one ran
+        if ((e = hipStreamSynchronize(h->st)) != hipSuccess) { rc = fail(DMD_ERR_HIP, hipGetErrorString(e)); break; }
+    }
+    hipError_t e = hipStreamSynchronize(h->st);
+    dfree(s1); dfree(s2); dfree(dk);
+    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc) h->has_P = false;
+    return rc;
+}
+
 extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
 {
     if (!h || !p) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: null argument");

@@ -26,10 +26,13 @@
 //
 //   dmd_run [--dry-run] [folder]
 #include <algorithm>
+#include <array>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -62,6 +65,10 @@ struct Settings {
     double mu = 0, carrier_density = 0, scissor = 0, temperature = 0;
     double Bpert[3] = {0, 0, 0}, B[3] = {0, 0, 0};
     std::vector<double> ni;  // impurity densities in bohr^-dim, ab_neutral
+    // k-pair filter of evolve (evolve_eph.cpp:52-58)
+    bool only_intra = false, only_inter = false;
+    std::vector<std::array<double, 3>> vpos;  // valley positions (crystal coordinates)
+    std::vector<std::pair<int, int>> vforbid;  // forbidden valley transitions
 };
 
 // electron.h:119-124
@@ -79,7 +86,7 @@ void refuse(const ParamIn &p, const std::string &key, double off_value, const st
         throw std::runtime_error(key + " = " + p.str(key, "") + ": " + why + " is not carried by this operator");
 }
 
-Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
+Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, const std::string &base_dir)
 {
     using namespace units;
     Settings s;
@@ -113,8 +120,35 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim)
         throw std::runtime_error("linearization for sparse P matrices is not implemented");
     refuse(p, "alg_phenom_relax", 0, "phenomenological relaxation");
     refuse(p, "alg_only_ee", 0, "alg_only_ee");
-    refuse(p, "alg_only_intravalley", 0, "valley filtering");
-    refuse(p, "alg_only_intervalley", 0, "valley filtering");
+    s.only_intra = p.get("alg_only_intravalley", 0) != 0;  // parameters.cpp:88-89
+    s.only_inter = p.get("alg_only_intervalley", 0) != 0;
+    if (s.only_intra && s.only_inter)
+        throw std::runtime_error("only_intravalley and only_intervalley cannot be true at the same time");
+    for (int iv = 1;; iv++) {  // valley1, valley2, ... (parameters.cpp:330-350)
+        std::istringstream in(p.str("valley" + std::to_string(iv), "1 1 1"));
+        std::array<double, 3> v{1, 1, 1};
+        in >> v[0] >> v[1] >> v[2];
+        if (std::fabs(v[0]) >= 1 || std::fabs(v[1]) >= 1 || std::fabs(v[2]) >= 1) break;
+        s.vpos.push_back(v);
+    }
+    if (s.vpos.size() == 1)
+        throw std::runtime_error("if you want to analyse valley dynamics, at least you need provide positions of two valleys");
+    if (s.vpos.size() >= 2) {  // parameters.cpp:25-44: one "i j" per line
+        const std::string f = p.str("file_forbid_vtrans", "NONE");
+        if (f != "NONE") {
+            std::ifstream in(f[0] == '/' ? f : base_dir + "/" + f);
+            if (!in) throw std::runtime_error("cannot open file_forbid_vtrans " + f);
+            std::string line;
+            while (std::getline(in, line)) {
+                std::istringstream ls(line);
+                int i, j;
+                if (!(ls >> i >> j)) continue;
+                if (i < 0 || j < 0 || i >= (int)s.vpos.size() || j >= (int)s.vpos.size())
+                    throw std::runtime_error("file_forbid_vtrans: valley index outside 0.." + std::to_string(s.vpos.size() - 1));
+                s.vforbid.push_back({i, j});
+            }
+        }
+    }
     refuse(p, "alg_read_Bso", 0, "the internal spin-orbit field");
     refuse(p, "needL", 0, "the orbital angular momentum term");
     refuse(p, "gfac_normal_dist", 0, "k-resolved g factors");
@@ -232,6 +266,68 @@ double mu_conserving(const std::vector<cplx> &dm, const std::vector<double> &edm
     return 0.5 * (lo + hi);
 }
 
+// The k-pair filter of evolve (evolve_eph.cpp:52-58, lattice.h:82-139):
+// lengths under the reciprocal metric GGT = G G^T, G = 2 pi R^-1
+// (lattice.h:40-41; matrix3.h:119-122), of differences wrapped into
+// [-0.5, 0.5) (comm/kmap.h:6-15); K = (1/3, 1/3, 0) and K' = -K (lattice.h:43-44).
+struct KpairFilter {
+    double M[3][3];
+    const Settings &s;
+    KpairFilter(const double R[9], const Settings &set) : s(set)
+    {
+        const double a = R[0], b = R[1], c = R[2], d = R[3], e = R[4], f = R[5], g = R[6], h = R[7], i = R[8];
+        const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+        const double inv[3][3] = {{(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det},
+                                  {(f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det},
+                                  {(d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det}};
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                double acc = 0;
+                for (int m = 0; m < 3; m++) acc += inv[r][m] * inv[q][m];
+                M[r][q] = 4 * M_PI * M_PI * acc;
+            }
+    }
+    double lsq(const double *k, const double *c) const
+    {
+        double v[3];
+        for (int d = 0; d < 3; d++) {
+            v[d] = k[d] - c[d];
+            v[d] -= std::floor(0.5 + v[d]);
+            if (std::fabs(v[d] - 0.5) < 1e-6) v[d] = -0.5;
+        }
+        return v[0] * v[0] * M[0][0] + v[1] * v[1] * M[1][1] + v[2] * v[2] * M[2][2] +
+               2 * (v[0] * v[1] * M[0][1] + v[0] * v[2] * M[0][2] + v[1] * v[2] * M[1][2]);
+    }
+    bool isK(const double *k) const
+    {
+        static const double K[3] = {1. / 3, 1. / 3, 0}, Kp[3] = {-1. / 3, -1. / 3, 0};
+        return lsq(k, K) < lsq(k, Kp);
+    }
+    int valley(const double *k) const
+    {
+        if (s.vpos.size() < 2) return -1;
+        int iv = 0;
+        double best = lsq(k, s.vpos[0].data());
+        for (size_t q = 1; q < s.vpos.size(); q++) {
+            const double l = lsq(k, s.vpos[q].data());
+            if (l < best) { best = l; iv = (int)q; }
+        }
+        return iv;
+    }
+    bool active() const { return s.only_intra || s.only_inter || (s.vpos.size() >= 2 && !s.vforbid.empty()); }
+    bool allowed(const double *k1, const double *k2) const
+    {
+        const bool intra = isK(k1) == isK(k2);
+        if (intra && s.only_inter) return false;
+        if (!intra && s.only_intra) return false;
+        const int v1 = valley(k1), v2 = valley(k2);
+        if (v1 >= 0 && v2 >= 0)
+            for (const auto &fb : s.vforbid)
+                if ((fb.first == v1 && fb.second == v2) || (fb.first == v2 && fb.second == v1)) return false;
+        return true;
+    }
+};
+
 struct Report {
     std::string name;  // "sx_elec" ...
     int comp, i0, i1;
@@ -262,7 +358,7 @@ int run(const std::string &folder, bool dry)
     if (dim == 2) cell = volume / R[8];
     if (dim == 1) cell = R[8];
     if (dim == 0) cell = 1;
-    const Settings s = read_settings(p, info, dim);
+    const Settings s = read_settings(p, info, dim, base);
     const long nk = (long)info.nk;
     const int nbf = info.nb, nbd = info.b_top_dm - info.b_bot_dm, nvd = std::max(info.nv - info.b_bot_dm, 0);
     const double T = s.temperature;
@@ -309,6 +405,8 @@ int run(const std::string &folder, bool dry)
                 s.thr_sparseP);
     std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d\n", (int)s.scatt_enable, (int)s.eph_enable,
                 (int)s.only_eimp);
+    std::printf("alg_only_intravalley = %d alg_only_intervalley = %d valleys = %zu forbidden_transitions = %zu\n",
+                (int)s.only_intra, (int)s.only_inter, s.vpos.size(), s.vforbid.size());
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
     if (dry) return 0;
 
@@ -374,6 +472,23 @@ int run(const std::string &folder, bool dry)
     if (s.sparseP && !s.linearize && !s.Pin_is_sparse) {
         eph.set_sparseP(true);
         std::printf("sparse P: |P| <= %lg zeroed, sP files in %s\n", s.thr_sparseP, ldbd.c_str());
+    }
+    // the k-pair filter of evolve, after the sP files took the unfiltered P;
+    // evolve_linear has none (evolve_eph_linear.cpp)
+    const KpairFilter filt(R, s);
+    if (filt.active() && s.scatt_enable && !s.linearize) {
+        std::vector<double> kv((size_t)nk * 3);
+        check(dmd_ldbd_read_kvec(ldbd.c_str(), &info, kv.data()), "ldbd_kvec.bin");
+        std::vector<uint64_t> k1(info.nkpair), k2(info.nkpair);
+        check(dmd_ldbd_read_kpairs(ldbd.c_str(), &info, 0, info.nkpair, k1.data(), k2.data()), "k pairs");
+        std::vector<uint8_t> keep(info.nkpair);
+        int64_t dropped = 0;
+        for (int64_t q = 0; q < info.nkpair; q++) {
+            keep[q] = filt.allowed(&kv[k1[q] * 3], &kv[k2[q] * 3]) ? 1 : 0;
+            dropped += !keep[q];
+        }
+        check(dmd_scatt_mask_pairs(eph.handle(), keep.data()), "k-pair filter");
+        std::printf("k-pair filter: %lld of %lld stored pairs dropped\n", (long long)dropped, (long long)info.nkpair);
     }
     eph.set_algorithm(s.expt, !interaction);
     if ((s.ddmdteq || s.linearize) && s.scatt_enable) {  // DensityMatrix_Dynamics.h:57-65

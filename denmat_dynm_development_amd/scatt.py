@@ -80,7 +80,7 @@ class ScatteringOperator:
         self.nk, self.nb = int(info.nk), int(info.nb_eph)
         self.nb_dm, self.b0 = int(info.nb_dm), int(info.b_bot_eph - info.b_bot_dm)
         self.n_pairs = int(info.nkpair)
-        self.info = info
+        self.ldbd_info = info
         h = ctypes.c_void_p()
         if not scatt_enable or not eph_enable:
             call("dmd_scatt_create_from_ldbd_noP", str(dirname).encode(), float(scissor), int(precision),
@@ -152,6 +152,14 @@ class ScatteringOperator:
         """alg_sparseP's sparsification (set_sparseP, ElectronPhonon.cpp:115-129):
         every stored element with |P| <= thr becomes 0."""
         call("dmd_scatt_threshold_P", self._h, float(thr))
+
+    def mask_pairs(self, keep):
+        """The k-pair filter of evolve (evolve_eph.cpp:52-58): P of the stored
+        pairs with keep == 0 (file order) becomes 0."""
+        k = np.ascontiguousarray(keep, dtype=np.uint8)
+        if k.size != self.n_pairs:
+            raise ValueError("keep needs one entry per stored pair")
+        call("dmd_scatt_mask_pairs", self._h, k.ctypes.data)
 
     def write_sP(self, dirname, info=None, scatt="lindblad", thr=1e-40):
         """The stored P as sparse files (write_smat, ElectronPhonon.cpp:126-128);

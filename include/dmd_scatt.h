@@ -337,6 +337,13 @@ int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, double scale_
 int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const double *P1, const double *P2,
                              double scale, int add);
 
+/* The k-pair filter of evolve (evolve_eph.cpp:52-58: alg_only_intravalley,
+ * alg_only_intervalley and forbidden valley transitions, lattice.h:82-139):
+ * the P of every stored pair with keep[p] = 0 (file order, n_pairs bytes)
+ * becomes 0, so neither direction of that pair scatters.  The pairs keep
+ * their slots.  Not on a linearized handle (evolve_linear has no filter). */
+int dmd_scatt_mask_pairs(dmd_scatt *h, const uint8_t *keep);
+
 /* ---- sparse P (alg_sparseP, alg_Pin_is_sparse) ------------------------------------ */
 /* Sparse files sP{1,2}_<scatt>[_hole]_{ns,s,i,j}.bin (sparse2D::read_smat,
  * comm/sparse2D.h:98-119; sparse_mat(FILE*, ...), comm/sparse_matrix.h:30-36):

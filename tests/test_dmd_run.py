@@ -103,6 +103,8 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("alg_ode_method = rk2", "rkf45, euler or rk4"),
     ("alg_Pin_is_sparse = 1\nalg_linearize = 1", "linearization for sparse P matrices is not implemented"),
     ("alg_thr_sparseP = -1", "alg_thr_sparseP"),
+    ("alg_only_intravalley = 1\nalg_only_intervalley = 1", "cannot be true at the same time"),
+    ("valley1 = 0.3 0.3 0", "at least you need provide positions of two valleys"),
     ("alg_eph_enable = 0", "alg_scatt_enable must be false"),
     ("alg_scatt_enable = 0", "alg_scatt_enable must be true"),
     ("alg_eph_enable = 0\nalg_only_eimp = 1", "only possible if impurity_density"),
@@ -150,3 +152,18 @@ def test_dry_run_scattering_switches(tmp_path):
                                      "impurity_density = 1e17\nimpMode = ab_neutral\n"))
     assert rc == 0, err
     assert _num(out, "alg_only_eimp") == 1 and _num(out, "impurities") == 1
+
+
+def test_dry_run_valleys_and_forbidden_transitions(tmp_path):
+    """valley1, valley2, ... (crystal coordinates, parameters.cpp:330-350) and
+    file_forbid_vtrans ("i j" per line, parameters.cpp:25-44)."""
+    d = _gaas_folder(tmp_path, "tstep = 1\ntend = 10\nvalley1 = 0.3333333 0.3333333 0\n"
+                               "valley2 = -0.3333333 -0.3333333 0\nfile_forbid_vtrans = forbid.txt\n"
+                               "alg_only_intervalley = 1\n")
+    (d / "forbid.txt").write_text("0 1\n")
+    rc, out, err = _dry(d)
+    assert rc == 0, err
+    assert _num(out, "valleys") == 2 and _num(out, "forbidden_transitions") == 1 and _num(out, "alg_only_intervalley") == 1
+    (d / "forbid.txt").write_text("0 2\n")
+    rc, out, err = _dry(d)
+    assert rc != 0 and "valley index" in err

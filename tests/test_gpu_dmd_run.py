@@ -453,3 +453,38 @@ def test_dmd_run_without_scattering(tmp_path):
     assert rows.shape == (6, 3)
     assert np.abs(rows[:, 1] - obs).max() < 1e-10 * np.abs(obs).max()
     assert np.abs(rows[:, 1] - rows[0, 1]).max() > 1e-3 * abs(rows[0, 1])  # it precesses
+
+
+def test_dmd_run_valley_filters(tmp_path):
+    """The k-pair filter of evolve (evolve_eph.cpp:52-58).  The instantaneous
+    rate dS/dt is linear in P, so the rates of the intravalley-only and
+    intervalley-only runs add up to the unfiltered rate (1/tau of the "without
+    precession" column), the two filters drop complementary pairs, and
+    forbidding the transition between valleys placed at K and K' is the
+    intravalley filter."""
+    from denmat_dynm_development_amd import scatt
+    print("case dmd_run valley filters", flush=True)
+    p = scatt.synth_params(0)
+    common = ["alg_ode_method = rk4", "Bzpert = 2", "tstep = 1", "compute_tau_only = 1"]
+    cases = {"all": [], "intra": ["alg_only_intravalley = 1"], "inter": ["alg_only_intervalley = 1"],
+             "forbid": ["valley1 = 0.3333333333333333 0.3333333333333333 0",
+                        "valley2 = -0.3333333333333333 -0.3333333333333333 0", "file_forbid_vtrans = forbid.txt"]}
+    tau, dropped = {}, {}
+    for name, extra in cases.items():
+        d = tmp_path / name
+        (d / "ldbd_data").mkdir(parents=True)
+        scatt.write_ldbd(str(d / "ldbd_data"), p)
+        (d / "forbid.txt").write_text("0 1\n")
+        (d / "param.in").write_text("\n".join(common + extra) + "\n")
+        r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+        print(name, r.stdout[-400:], r.stderr[-400:])
+        assert r.returncode == 0
+        m = [l for l in r.stdout.splitlines() if l.startswith("k-pair filter:")]
+        dropped[name] = int(m[0].split()[2]) if m else 0
+        tau[name] = np.loadtxt(d / "tau_sz_elec_tot.out", usecols=(2, 3, 4))
+    total = int([l for l in r.stdout.splitlines() if "stored pairs" in l][0].split(",")[-1].split()[0])
+    assert dropped["all"] == 0 and 0 < dropped["intra"] < total and dropped["intra"] + dropped["inter"] == total
+    rate = {k: 1 / v[1] for k, v in tau.items()}  # tau without precession, ps
+    print("rates", rate)
+    assert abs(rate["intra"] + rate["inter"] - rate["all"]) < 2e-7 * abs(rate["all"])
+    assert np.allclose(tau["forbid"], tau["intra"], rtol=2e-7, atol=0) and dropped["forbid"] == dropped["intra"]
