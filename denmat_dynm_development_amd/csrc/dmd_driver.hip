@@ -80,6 +80,7 @@ struct StageArgs {
     const DClock *clk;
     double dt;
     const double *fsub; // linearized operator: rhoS = rho_S - diag(f_eq) (nk x nb), else null
+    int semic;          // alg_semiclassical: off-diagonals of the state and of the result are zero
 };
 
 // right-hand side element (i, j) of state kk (row k = kk mod nk)
@@ -88,7 +89,10 @@ __device__ __forceinline__ double2 rhs_elem(const StageArgs &A, int64_t k, int64
 {
     double2 kv = make_double2(0, 0);
     const int n = A.nbd;
-    if (A.coh) kv = coherent_elem(A.H ? A.H + k * n * n : nullptr, A.edm + k * n, yk, n, i, j, t, A.schrod);
+    // alg_semiclassical (DensityMatrix_Dynamics.h:249,277): the state's off-diagonals are zeroed before
+    // compute and the result's after, so the coherent term -i[H, diag y] leaves nothing
+    if (A.semic && i != j) return kv;
+    if (A.coh && !A.semic) kv = coherent_elem(A.H ? A.H + k * n * n : nullptr, A.edm + k * n, yk, n, i, j, t, A.schrod);
     int wi = i - A.b0, wj = j - A.b0;
     if (wi >= 0 && wi < A.nb && wj >= 0 && wj < A.nb) kv = cadd(kv, w[kk * A.nb * A.nb + wi * A.nb + wj]);
     return kv;
@@ -101,6 +105,7 @@ __device__ __forceinline__ void rotate_out(const StageArgs &A, int64_t k, int i,
     if (!rhoS || wi < 0 || wi >= A.nb || wj < 0 || wj >= A.nb) return;
     if (A.expt && wi != wj) v = cmul(v, ccis(-(A.e[k * A.nb + wi] - A.e[k * A.nb + wj]) * t));
     if (A.fsub && wi == wj) v.x -= A.fsub[k * A.nb + wi];
+    if (A.semic && wi != wj) v = make_double2(0.0, 0.0);
     rhoS[k * A.nb * A.nb + wi * A.nb + wj] = v;
 }
 
@@ -227,6 +232,7 @@ static StageArgs stage_args(dmd_scatt *h, double dt)
     A.expt = h->expt;
     A.edm = h->d_edm; A.e = h->d_e; A.clk = h->d_clk; A.dt = dt;
     A.fsub = h->linear ? h->d_flin : nullptr;
+    A.semic = h->semiclassical;
     return A;
 }
 
@@ -403,7 +409,7 @@ extern "C" int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64
     if ((rc = to_device(h, h->d_y0, rho, n * 16))) return rc;
     if ((rc = set_clock(h, t0, dt, 0))) return rc;
     StageArgs A = stage_args(h, dt);
-    if ((rc = launch_rotate(h, h->d_y0, nullptr, h->nbd, h->b0, 0.0))) return rc;
+    if ((rc = launch_rotate(h, h->d_y0, nullptr, h->nbd, h->b0, 0.0, h->semiclassical))) return rc;
     int iobs = 0;
     if (want_obs && (rc = launch_obs(h, h->d_y0, d_obs, iobs++))) return rc;
     const char *env = std::getenv("DMD_GRAPH");
@@ -549,6 +555,7 @@ extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, d
 {
     if (!h || B < 1 || !rho_B || n_steps < 0) return fail(DMD_ERR_ARG, "dmd_scatt_integrate_rk4_batched: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
+    if (h->semiclassical) return fail(DMD_ERR_UNSUPPORTED, "the batched integrator has no semiclassical mode");
     const bool want_obs = report_every > 0 && h->has_obs;
     if (want_obs && !obs_B) return fail(DMD_ERR_ARG, "obs_B is null");
     const int64_t nobs = want_obs ? n_steps / report_every + 1 : 0;
@@ -712,13 +719,20 @@ __global__ __launch_bounds__(256) void k_max_final(int nparts, const double *__r
 }
 
 // update_dm_euler (DenMat.cpp:236-248): y_ij += dt (k_ij + conj(k_ji)) / 2
-__global__ void k_euler(int64_t nk, int nbd, double dt, const double2 *__restrict__ kv, double2 *__restrict__ y)
+// semic: evolve_euler's compute() zeroes the state's own off-diagonals
+// (DensityMatrix_Dynamics.h:229,249), so they stay zero after the step
+__global__ void k_euler(int64_t nk, int nbd, double dt, const double2 *__restrict__ kv, double2 *__restrict__ y,
+                        int semic)
 {
     const int n2 = nbd * nbd;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nk * n2) return;
     int64_t k = idx / n2;
     int ij = (int)(idx - k * n2), i = ij / nbd, j = ij - i * nbd;
+    if (semic && i != j) {
+        y[idx] = make_double2(0.0, 0.0);
+        return;
+    }
     const double2 a = kv[idx], b = kv[k * n2 + j * nbd + i];
     y[idx].x += dt * 0.5 * (a.x + b.x);
     y[idx].y += dt * 0.5 * (a.y - b.y);
@@ -745,7 +759,7 @@ static int ensure_adaptive(dmd_scatt *h)
 static int full_rhs(dmd_scatt *h, const StageArgs &A, const double2 *y, double a, double2 *kout)
 {
     int rc;
-    if ((rc = launch_rotate(h, y, nullptr, h->nbd, h->b0, a))) return rc;
+    if ((rc = launch_rotate(h, y, nullptr, h->nbd, h->b0, a, h->semiclassical))) return rc;
     if ((rc = launch_contract(h, false, a, h->d_w))) return rc;
     const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
     hipLaunchKernelGGL(k_rhs_full, dim3(nblk(n, 256)), dim3(256), 0, h->st, A, y, h->d_w, a, kout);
@@ -869,7 +883,8 @@ extern "C" int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int
         if ((rc = set_clock(h, t, dt, 0))) return rc;
         StageArgs A = stage_args(h, dt);
         if ((rc = full_rhs(h, A, y, 0.0, kv))) return rc;
-        hipLaunchKernelGGL(k_euler, dim3(nblk(n, 256)), dim3(256), 0, h->st, h->nk, h->nbd, dt, kv, y);
+        hipLaunchKernelGGL(k_euler, dim3(nblk(n, 256)), dim3(256), 0, h->st, h->nk, h->nbd, dt, kv, y,
+                           h->semiclassical);
         DMD_HIP(hipGetLastError());
     }
     DMD_HIP(hipStreamSynchronize(h->st));
@@ -894,5 +909,13 @@ extern "C" int dmd_scatt_compute_rhs(dmd_scatt *h, double t, const double *rho, 
     if ((rc = full_rhs(h, A, h->d_ad, 0.0, h->d_ad + 3 * n))) return rc;
     if ((rc = from_device(h, ddmdt, h->d_ad + 3 * n, (size_t)n * 16))) return rc;
     DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
+}
+
+// alg_semiclassical (DensityMatrix_Dynamics.h:249,277)
+extern "C" int dmd_scatt_set_semiclassical(dmd_scatt *h, int on)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    h->semiclassical = on ? 1 : 0;
     return DMD_OK;
 }

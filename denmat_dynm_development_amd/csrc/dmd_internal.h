@@ -196,6 +196,7 @@ struct dmd_scatt {
     double *d_dotpart = nullptr;         // dot-product partials
     // linearized operator (alg_linearize): the P store holds Lscij | Lscji
     bool linear = false;
+    int semiclassical = 0;  // alg_semiclassical: the integrators keep rho diagonal
     double *d_flin = nullptr;            // f_eq (nk x nb) the operator was linearized at
     // adaptive / Euler integrators: 9 states (y0, ytmp, ynew, k1..k6) and the error partials
     double2 *d_ad = nullptr;
@@ -242,7 +243,7 @@ int to_device(dmd_scatt *h, void *dst, const void *src, size_t bytes);
 int from_device(dmd_scatt *h, void *dst, const void *src, size_t bytes);
 int set_clock(dmd_scatt *h, double t0, double dt, long long step);
 // rotate the window [b0, b0+nb) of an nbd x nbd state to the Schroedinger picture (h->d_rhoS)
-int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a);
+int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a, int semic = 0);
 // contraction + Hermitian epilogue on h->d_rhoS (and h->d_rho1S when dm1) into out (nk x nb^2)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
 // DMD_KERNEL_ROWS, _SINGLE_READ or _TILES: the kernel launch_contract uses

@@ -78,7 +78,8 @@ template <int NB>
 __global__ void k_rotate_in(int64_t nk, int nbd, int b0, const double2 *__restrict__ rho,
                             const double2 *__restrict__ rho1, const double *__restrict__ e,
                             const DClock *__restrict__ clk, double a, int expt,
-                            double2 *__restrict__ rhoS, double2 *__restrict__ rho1S, const double *__restrict__ fsub)
+                            double2 *__restrict__ rhoS, double2 *__restrict__ rho1S, const double *__restrict__ fsub,
+                            int semic)
 {
     constexpr int NB2 = NB * NB;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -92,6 +93,7 @@ __global__ void k_rotate_in(int64_t nk, int nbd, int b0, const double2 *__restri
     if (rot) ph = ccis(-(e[k * NB + i] - e[k * NB + j]) * clock_t_at(clk, a));
     v = rot ? cmul(v, ph) : v;
     if (fsub && i == j) v.x -= fsub[k * NB + i];
+    if (semic && i != j) v = make_double2(0.0, 0.0);  // alg_semiclassical (DensityMatrix_Dynamics.h:249)
     rhoS[idx] = v;
     if (rho1) {
         double2 w = rho1[src];
@@ -1398,16 +1400,16 @@ namespace dmd {
 // launch helpers
 // ---------------------------------------------------------------------------
 // rotate the window of state `in` (nbd x nbd per k, offset b0) to rhoS
-int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a)
+int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a, int semic)
 {
     int64_t n = h->nk * h->nb2;
     const double *fs = h->linear ? h->d_flin : nullptr;  // the linearized operator acts on drho
     dim3 g(nblk(n, 256)), b(256);
     switch (h->nb) {
-    case 1: hipLaunchKernelGGL(k_rotate_in<1>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
-    case 2: hipLaunchKernelGGL(k_rotate_in<2>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
-    case 3: hipLaunchKernelGGL(k_rotate_in<3>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
-    case 4: hipLaunchKernelGGL(k_rotate_in<4>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs); break;
+    case 1: hipLaunchKernelGGL(k_rotate_in<1>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs, semic); break;
+    case 2: hipLaunchKernelGGL(k_rotate_in<2>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs, semic); break;
+    case 3: hipLaunchKernelGGL(k_rotate_in<3>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs, semic); break;
+    case 4: hipLaunchKernelGGL(k_rotate_in<4>, g, b, 0, h->st, h->nk, nbd, b0, in, in1, h->d_e, h->d_clk, a, h->expt, h->d_rhoS, h->d_rho1S, fs, semic); break;
     }
     DMD_HIP(hipGetLastError());
     return DMD_OK;

@@ -54,7 +54,7 @@ struct cplx {
 
 struct Settings {
     bool restart = false, tau_only = false, sparseP = false, Pin_is_sparse = false;
-    bool scatt_enable = true, eph_enable = true, only_eimp = false;
+    bool scatt_enable = true, eph_enable = true, only_eimp = false, semiclassical = false;
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
@@ -108,7 +108,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     if (s.picture != "interaction" && s.picture != "schrodinger")
         throw std::runtime_error("alg_picture must be interaction or schrodinger");
     refuse(p, "alg_summode", 1, "the A-matrix (non-summode) form");
-    refuse(p, "alg_semiclassical", 0, "the semiclassical (diagonal) mode");
+    s.semiclassical = p.get("alg_semiclassical", 0) != 0;  // parameters.cpp:101
     s.linearize = p.get("alg_linearize", 0) != 0;
     refuse(p, "alg_linearize_dPee", 0, "the linearized e-e operator");
     // parameters.cpp:118-120; sparse P is kept dense in HBM with the small elements zeroed
@@ -403,8 +403,8 @@ int run(const std::string &folder, bool dry)
                 s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
     std::printf("alg_sparseP = %d alg_Pin_is_sparse = %d alg_thr_sparseP = %lg\n", (int)s.sparseP, (int)s.Pin_is_sparse,
                 s.thr_sparseP);
-    std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d\n", (int)s.scatt_enable, (int)s.eph_enable,
-                (int)s.only_eimp);
+    std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d alg_semiclassical = %d\n",
+                (int)s.scatt_enable, (int)s.eph_enable, (int)s.only_eimp, (int)s.semiclassical);
     std::printf("alg_only_intravalley = %d alg_only_intervalley = %d valleys = %zu forbidden_transitions = %zu\n",
                 (int)s.only_intra, (int)s.only_inter, s.vpos.size(), s.vforbid.size());
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
@@ -491,6 +491,7 @@ int run(const std::string &folder, bool dry)
         std::printf("k-pair filter: %lld of %lld stored pairs dropped\n", (long long)dropped, (long long)info.nkpair);
     }
     eph.set_algorithm(s.expt, !interaction);
+    eph.set_semiclassical(s.semiclassical);
     if ((s.ddmdteq || s.linearize) && s.scatt_enable) {  // DensityMatrix_Dynamics.h:57-65
         std::vector<double> feq((size_t)nk * nbd);
         const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;

@@ -109,6 +109,10 @@ public:
         if (write_files) check(dmd_ldbd_write_sP(h_, dir_.c_str(), &info_, scatt_.c_str(), thr_sparseP), "sP files");
     }
 
+    // alg.semiclassical (parameters.cpp:101): the driver-side right-hand side
+    // keeps rho diagonal (DensityMatrix_Dynamics.h:249,277)
+    void set_semiclassical(bool on) { check(dmd_scatt_set_semiclassical(h_, on ? 1 : 0), "semiclassical"); }
+
     // alg.expt and alg.picture (parameters.cpp:104,107)
     void set_algorithm(bool expt, bool schrodinger)
     {

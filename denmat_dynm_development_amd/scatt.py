@@ -153,6 +153,11 @@ class ScatteringOperator:
         every stored element with |P| <= thr becomes 0."""
         call("dmd_scatt_threshold_P", self._h, float(thr))
 
+    def set_semiclassical(self, on=True):
+        """alg_semiclassical (DensityMatrix_Dynamics.h:249,277): the integrators
+        and compute_rhs keep only the populations."""
+        call("dmd_scatt_set_semiclassical", self._h, int(bool(on)))
+
     def mask_pairs(self, keep):
         """The k-pair filter of evolve (evolve_eph.cpp:52-58): P of the stored
         pairs with keep == 0 (file order) becomes 0."""

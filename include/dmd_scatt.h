@@ -337,6 +337,15 @@ int dmd_scatt_create_from_ldbd(const char *dir, const char *scatt, double scale_
 int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const double *P1, const double *P2,
                              double scale, int add);
 
+/* alg_semiclassical (dm_dynamics::compute, DensityMatrix_Dynamics.h:249,277):
+ * the integrators' right-hand side (integrate_rk4, rkf45_apply,
+ * integrate_euler, compute_rhs) zeroes the state's off-diagonal elements
+ * before the scattering term and the result's after, so only the
+ * populations evolve and the coherent term drops out.  Not for the batched
+ * integrator or the adjoint.  dmd_scatt_eval (electronphonon::evolve) is
+ * unaffected, as in the reference. */
+int dmd_scatt_set_semiclassical(dmd_scatt *h, int on);
+
 /* The k-pair filter of evolve (evolve_eph.cpp:52-58: alg_only_intravalley,
  * alg_only_intervalley and forbidden valley transitions, lattice.h:82-139):
  * the P of every stored pair with keep[p] = 0 (file order, n_pairs bytes)

@@ -74,6 +74,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
         "compute_tau_only = 1",
         "alg_sparseP = 1",
         "alg_thr_sparseP = 1e-5",
+        "alg_semiclassical = 1",
         "Bxpert = 0.5",
         "mu = 0.05",
         "scale_scatt = 0.7",
@@ -88,6 +89,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     assert _num(out, "freq_measure") == 2
     assert _num(out, "freq_compute_tau") == 4 and _num(out, "compute_tau_only") == 1  # parameters.cpp:326
     assert _num(out, "alg_sparseP") == 1 and _num(out, "alg_Pin_is_sparse") == 0 and _num(out, "alg_thr_sparseP") == 1e-5
+    assert _num(out, "alg_semiclassical") == 1
     assert abs(_num(out, "mu") / (0.05 / 27.21138505) - 1) < 1e-10
     assert _num(out, "scale_eph") == 0.9 and _num(out, "scale_ei") == 0.7
     assert abs(float(re.search(r"Bpert = (\S+)", out).group(1)) / (0.5 / 2.3505175675871e5) - 1) < 1e-10

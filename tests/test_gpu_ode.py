@@ -142,3 +142,55 @@ def test_rkf45_stops_when_the_step_underflows():
         t0 = 100 * FS
         with pytest.raises(_lib.DMDError, match="underflowed"):
             op.rkf45_apply(rho, t0, t0 + FS, FS, 0.0, FS, 1e-300, 0.0)
+
+
+def test_semiclassical_right_hand_side_and_steppers():
+    """alg_semiclassical (DensityMatrix_Dynamics.h:249,277): the right-hand
+    side is zero_offdiag(F(zero_offdiag(y))) with the coherent term gone; RK4
+    and RKF45 keep the state's initial off-diagonals, Euler zeroes them as the
+    reference's in-place compute() does; eval (evolve) is unaffected."""
+    _lib, scatt, oracle = _mods()
+    print("case semiclassical", flush=True)
+    rng = np.random.default_rng(85)
+    nk, nb = 20, 2
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    P1, P2 = 0.2 * P1, 0.2 * P2
+    prefac = 2 * np.pi / nk
+    H = random_rho(nk, nb, rng, scale=1e-4, diag=1e-4)
+    rho0 = random_rho(nk, nb, rng, scale=0.02, diag=0.3)
+
+    def zoff(y):
+        z = y.reshape(nk, nb, nb).copy()
+        z[:, ~np.eye(nb, dtype=bool)] = 0
+        return z.reshape(nk, -1)
+
+    def rhs(t, y):
+        return zoff(oracle.evolve(k1, k2, P1, P2, e, zoff(y), t=t, prefac=prefac))
+
+    dt, n = 2 * FS, 6
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        op.set_coherent(H)
+        plain = op.eval(3 * FS, rho0)
+        op.set_semiclassical(True)
+        assert rel_err(op.eval(3 * FS, rho0), plain) < 1e-14  # evolve itself is unchanged
+        got = op.compute_rhs(3 * FS, rho0)
+        assert rel_err(got, rhs(3 * FS, rho0)) < 1e-12
+        y4, _ = op.integrate_rk4(rho0, 0.0, dt, n)
+        y = rho0.copy()
+        for i in range(n):
+            t = i * dt
+            s1 = rhs(t, y)
+            s2 = rhs(t + dt / 2, y + dt / 2 * s1)
+            s3 = rhs(t + dt / 2, y + dt / 2 * s2)
+            s4 = rhs(t + dt, y + dt * s3)
+            y = y + dt / 6 * (s1 + 2 * s2 + 2 * s3 + s4)
+        assert np.abs(y4 - y).max() < 1e-12 * np.abs(y).max()
+        assert np.array_equal(zoff(y4) - y4, zoff(rho0) - rho0)  # off-diagonals untouched
+        ye = op.integrate_euler(rho0, 0.0, dt, 1)
+        ref = zoff(rho0) + dt * rhs(0.0, rho0)
+        assert np.abs(ye - ref).max() < 1e-14 * np.abs(ref).max()
+        with pytest.raises(_lib.DMDError):
+            op.integrate_rk4_batched(np.stack([rho0, rho0]), 0.0, dt, 1)
