@@ -336,7 +336,8 @@ extern "C" int dmd_scatt_adjoint_rk4(dmd_scatt *h, double t0, double dt, int64_t
 {
     if (!h || !rho0 || n_steps < 1 || checkpoint_every < 1 || (ncomp > 0 && (!comps || !dL_dtheta)))
         return fail(DMD_ERR_ARG, "dmd_scatt_adjoint_rk4: bad arguments");
-    if (h->semiclassical) return fail(DMD_ERR_UNSUPPORTED, "the adjoint has no semiclassical mode");
+    if (h->semiclassical || h->ph_tau_inv != 0.0)
+        return fail(DMD_ERR_UNSUPPORTED, "the adjoint has no semiclassical mode or phenomenological relaxation");
     int rc;
     if ((rc = vjp_ready(h))) return rc;
     const bool obs = report_every > 0 && dL_dobs;

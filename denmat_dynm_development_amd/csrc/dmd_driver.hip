@@ -81,6 +81,9 @@ struct StageArgs {
     double dt;
     const double *fsub; // linearized operator: rhoS = rho_S - diag(f_eq) (nk x nb), else null
     int semic;          // alg_semiclassical: off-diagonals of the state and of the result are zero
+    double ph_tau_inv;  // alg_phenom_relax: 1/tau (0: off), bands [ph_b0, ph_b1), f_eq nk x nbd
+    int ph_b0, ph_b1;
+    const double *ph_feq;
 };
 
 // right-hand side element (i, j) of state kk (row k = kk mod nk)
@@ -95,6 +98,12 @@ __device__ __forceinline__ double2 rhs_elem(const StageArgs &A, int64_t k, int64
     if (A.coh && !A.semic) kv = coherent_elem(A.H ? A.H + k * n * n : nullptr, A.edm + k * n, yk, n, i, j, t, A.schrod);
     int wi = i - A.b0, wj = j - A.b0;
     if (wi >= 0 && wi < A.nb && wj >= 0 && wj < A.nb) kv = cadd(kv, w[kk * A.nb * A.nb + wi * A.nb + wj]);
+    // phenom_relax::evolve (scattering_model/phenomenon_relax.h:42-51): -(rho - rho_eq) / tau
+    if (A.ph_tau_inv != 0.0 && i >= A.ph_b0 && i < A.ph_b1 && j >= A.ph_b0 && j < A.ph_b1) {
+        double2 d = yk[i * n + j];
+        if (i == j) d.x -= A.ph_feq[k * n + i];
+        kv = csub(kv, make_double2(d.x * A.ph_tau_inv, d.y * A.ph_tau_inv));
+    }
     return kv;
 }
 
@@ -233,6 +242,10 @@ static StageArgs stage_args(dmd_scatt *h, double dt)
     A.edm = h->d_edm; A.e = h->d_e; A.clk = h->d_clk; A.dt = dt;
     A.fsub = h->linear ? h->d_flin : nullptr;
     A.semic = h->semiclassical;
+    A.ph_tau_inv = h->ph_tau_inv;
+    A.ph_b0 = h->ph_b0;
+    A.ph_b1 = h->ph_b1;
+    A.ph_feq = h->d_phfeq;
     return A;
 }
 
@@ -555,7 +568,8 @@ extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, d
 {
     if (!h || B < 1 || !rho_B || n_steps < 0) return fail(DMD_ERR_ARG, "dmd_scatt_integrate_rk4_batched: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
-    if (h->semiclassical) return fail(DMD_ERR_UNSUPPORTED, "the batched integrator has no semiclassical mode");
+    if (h->semiclassical || h->ph_tau_inv != 0.0)
+        return fail(DMD_ERR_UNSUPPORTED, "the batched integrator has no semiclassical mode or phenomenological relaxation");
     const bool want_obs = report_every > 0 && h->has_obs;
     if (want_obs && !obs_B) return fail(DMD_ERR_ARG, "obs_B is null");
     const int64_t nobs = want_obs ? n_steps / report_every + 1 : 0;
@@ -917,5 +931,29 @@ extern "C" int dmd_scatt_set_semiclassical(dmd_scatt *h, int on)
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
     h->semiclassical = on ? 1 : 0;
+    return DMD_OK;
+}
+
+// alg_phenom_relax (phenom_relax, scattering_model/phenomenon_relax.h; added in
+// compute, DensityMatrix_Dynamics.h:272-275): -(rho - diag f_eq)/tau on the
+// density-matrix bands [b_start, b_end); tau <= 0 switches it off.
+extern "C" int dmd_scatt_set_phenom_relax(dmd_scatt *h, double tau, int b_start, int b_end, const double *f_eq)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (!(tau > 0)) {
+        h->ph_tau_inv = 0.0;
+        return DMD_OK;
+    }
+    if (!f_eq || b_start < 0 || b_end > h->nbd || b_start >= b_end)
+        return fail(DMD_ERR_ARG, "dmd_scatt_set_phenom_relax: bands must satisfy 0 <= b_start < b_end <= " +
+                                     std::to_string(h->nbd) + " and f_eq must be given");
+    int rc;
+    const size_t n = (size_t)h->nk * h->nbd;
+    if (!h->d_phfeq && (rc = dalloc_t(&h->d_phfeq, n, "phenomenological f_eq"))) return rc;
+    if ((rc = to_device(h, h->d_phfeq, f_eq, n * 8))) return rc;
+    DMD_HIP(hipStreamSynchronize(h->st));
+    h->ph_tau_inv = 1.0 / tau;
+    h->ph_b0 = b_start;
+    h->ph_b1 = b_end;
     return DMD_OK;
 }

@@ -197,6 +197,10 @@ struct dmd_scatt {
     // linearized operator (alg_linearize): the P store holds Lscij | Lscji
     bool linear = false;
     int semiclassical = 0;  // alg_semiclassical: the integrators keep rho diagonal
+    // alg_phenom_relax: -(rho - diag f_eq) / tau on density-matrix bands [ph_b0, ph_b1)
+    double ph_tau_inv = 0.0;
+    int ph_b0 = 0, ph_b1 = 0;
+    double *d_phfeq = nullptr;  // nk x nbd
     double *d_flin = nullptr;            // f_eq (nk x nb) the operator was linearized at
     // adaptive / Euler integrators: 9 states (y0, ytmp, ynew, k1..k6) and the error partials
     double2 *d_ad = nullptr;

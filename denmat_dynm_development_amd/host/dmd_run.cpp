@@ -54,7 +54,10 @@ struct cplx {
 
 struct Settings {
     bool restart = false, tau_only = false, sparseP = false, Pin_is_sparse = false;
-    bool scatt_enable = true, eph_enable = true, only_eimp = false, semiclassical = false;
+    bool scatt_enable = true, eph_enable = true, only_eimp = false, semiclassical = false, phenom = false;
+    double tau_phenom = 0;
+    int bStart_tau = 0, bEnd_tau = 0;
+    bool excess = false;  // carrier_density_means_excess_density
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
@@ -118,7 +121,14 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     if (!(s.thr_sparseP >= 0)) throw std::runtime_error("alg_thr_sparseP must be >= 0");
     if (s.Pin_is_sparse && s.linearize)  // ElectronPhonon_linear.cpp:140-143
         throw std::runtime_error("linearization for sparse P matrices is not implemented");
-    refuse(p, "alg_phenom_relax", 0, "phenomenological relaxation");
+    s.phenom = p.get("alg_phenom_relax", 0) != 0;  // parameters.cpp:122-128
+    if (s.phenom) {
+        s.tau_phenom = p.get("tau_phenom", 1e15, 1000 * fs());
+        s.bStart_tau = (int)p.get("bStart_tau", 0);
+        s.bEnd_tau = (int)p.get("bEnd_tau", 0);
+        if (s.bEnd_tau > info.b_top_dm - info.b_bot_dm) throw std::runtime_error("bEnd_tau > nb_dm");
+        if (s.bStart_tau < 0) throw std::runtime_error("bStart_tau < 0");
+    }
     refuse(p, "alg_only_ee", 0, "alg_only_ee");
     s.only_intra = p.get("alg_only_intravalley", 0) != 0;  // parameters.cpp:88-89
     s.only_inter = p.get("alg_only_intervalley", 0) != 0;
@@ -153,7 +163,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     refuse(p, "needL", 0, "the orbital angular momentum term");
     refuse(p, "gfac_normal_dist", 0, "k-resolved g factors");
     refuse(p, "scale_Ez", 0, "the electric-field Hamiltonian");
-    refuse(p, "carrier_density_means_excess_density", 0, "excess-density mu");
+    s.excess = p.get("carrier_density_means_excess_density", 0) != 0;  // parameters.cpp:170
     if (p.str("eeMode", "none") != "none") throw std::runtime_error("eeMode: e-e scattering is not carried by this operator");
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
     if (laserA != 0) throw std::runtime_error("laserA / pumpA0: pump pulses are not carried by this operator");
@@ -217,17 +227,19 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
 // the conduction (or, for a negative density, valence) bands: carrier_bvk =
 // density * nk_full * cell_size, compute_nfree (electron.cpp:582-600).
 double find_mu_bisection(double carrier_bvk, double T, double emin, double emax, const std::vector<double> &e,
-                         long nk, int nb, int nv)
+                         long nk, int nb, int nv, bool excess = false)
 {
+    // excess: electrons minus holes over every band (electron.cpp:547-549)
     const bool hole = carrier_bvk < 0;
+    const int i0 = excess ? 0 : (hole ? 0 : nv), i1 = excess ? nb : (hole ? nv : nb);
     double lo = emin, hi = emax, mu = 0.5 * (lo + hi);
     for (int it = 0; it < 200; it++) {
         mu = 0.5 * (lo + hi);
         double n = 0;
         for (long k = 0; k < nk; k++)
-            for (int i = hole ? 0 : nv; i < (hole ? nv : nb); i++) {
+            for (int i = i0; i < i1; i++) {
                 const double f = fermi(T, mu, e[(size_t)k * nb + i]);
-                n += hole ? f - 1. : f;
+                n += i < nv ? f - 1. : f;
             }
         const double err = n - carrier_bvk;
         if (std::fabs(err) < 1e-12) return mu;
@@ -369,7 +381,7 @@ int run(const std::string &folder, bool dry)
         for (int b = 0; b < nbd; b++) edm[(size_t)k * nbd + b] = e[(size_t)k * nbf + info.b_bot_dm + b];
     // electron::set_mu_and_n (electron.h:141-160) with emin/emax of electron.h:94-105
     double mu = s.mu;
-    if (s.carrier_density != 0) {
+    if (s.carrier_density != 0 || s.excess) {
         double evmax = -1e300, emin = 1e300, emax = -1e300, ecmin = 1e300;
         for (long k = 0; k < nk; k++)
             for (int b = 0; b < nbd; b++) {
@@ -379,7 +391,7 @@ int run(const std::string &folder, bool dry)
             }
         if (nbd == nvd) emax = evmax + 1;
         if (nvd == 0) emin = ecmin - 1;
-        mu = find_mu_bisection(s.carrier_density * info.nk_full * cell, T, emin, emax, e, nk, nbf, info.nv);
+        mu = find_mu_bisection(s.carrier_density * info.nk_full * cell, T, emin, emax, e, nk, nbf, info.nv, s.excess);
     }
     std::vector<double> fdm((size_t)nk * nbd);
     for (size_t i = 0; i < fdm.size(); i++) fdm[i] = fermi(T, mu, edm[i]);
@@ -405,6 +417,9 @@ int run(const std::string &folder, bool dry)
                 s.thr_sparseP);
     std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d alg_semiclassical = %d\n",
                 (int)s.scatt_enable, (int)s.eph_enable, (int)s.only_eimp, (int)s.semiclassical);
+    std::printf("alg_phenom_relax = %d tau_phenom = %.10le bStart_tau = %d bEnd_tau = %d "
+                "carrier_density_means_excess_density = %d\n", (int)s.phenom, s.tau_phenom, s.bStart_tau, s.bEnd_tau,
+                (int)s.excess);
     std::printf("alg_only_intravalley = %d alg_only_intervalley = %d valleys = %zu forbidden_transitions = %zu\n",
                 (int)s.only_intra, (int)s.only_inter, s.vpos.size(), s.vforbid.size());
     for (const auto &k : p.unread()) std::printf("note: param.in key \"%s\" is not read by this runner\n", k.c_str());
@@ -492,8 +507,10 @@ int run(const std::string &folder, bool dry)
     }
     eph.set_algorithm(s.expt, !interaction);
     eph.set_semiclassical(s.semiclassical);
-    if ((s.ddmdteq || s.linearize) && s.scatt_enable) {  // DensityMatrix_Dynamics.h:57-65
-        std::vector<double> feq((size_t)nk * nbd);
+    // dm_eq (DensityMatrix_Dynamics.h:57-67): the carrier-conserving f_eq,
+    // set only when scattering is on (dm_eq stays 0 otherwise)
+    std::vector<double> feq((size_t)nk * nbd, 0.0);
+    if ((s.ddmdteq || s.linearize || s.phenom) && s.scatt_enable) {
         const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
         const double muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;
         for (long k = 0; k < nk; k++)
@@ -502,10 +519,15 @@ int run(const std::string &folder, bool dry)
         if (s.linearize) {
             eph.set_Lsc(rows);  // ddmdt_eq of the linearized operator is zero: evolve_linear at f_eq
             std::printf("linearized at mu_e = %.12le mu_h = %.12le\n", mue, muh);
-        } else {
+        } else if (s.ddmdteq) {
             eph.compute_ddmdt_eq(rows);
             std::printf("ddmdt_eq at mu_e = %.12le mu_h = %.12le\n", mue, muh);
         }
+    }
+    if (s.phenom && s.bEnd_tau > s.bStart_tau) {  // phenom_relax over [bStart_tau, bEnd_tau); an empty range does nothing
+        double *rows[1] = {feq.data()};
+        eph.set_phenom_relax(s.tau_phenom, s.bStart_tau, s.bEnd_tau, rows);
+        std::printf("phenomenological relaxation: tau = %.10le on bands [%d, %d)\n", s.tau_phenom, s.bStart_tau, s.bEnd_tau);
     }
     dmd_scatt *h = eph.handle();
     if (std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10) {  // electron.cpp:19-42

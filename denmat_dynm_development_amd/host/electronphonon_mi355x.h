@@ -113,6 +113,14 @@ public:
     // keeps rho diagonal (DensityMatrix_Dynamics.h:249,277)
     void set_semiclassical(bool on) { check(dmd_scatt_set_semiclassical(h_, on ? 1 : 0), "semiclassical"); }
 
+    // phenom_relax (scattering_model/phenomenon_relax.h:14-51): -(rho - dm_eq)/tau
+    // on density-matrix bands [bStart_tau, bEnd_tau); f_eq_expand is nk x nb_expand
+    void set_phenom_relax(double tau, int bStart_tau, int bEnd_tau, double **f_eq_expand)
+    {
+        check(dmd_scatt_set_phenom_relax(h_, tau, bStart_tau, bEnd_tau, f_eq_expand ? f_eq_expand[0] : nullptr),
+              "phenomenological relaxation");
+    }
+
     // alg.expt and alg.picture (parameters.cpp:104,107)
     void set_algorithm(bool expt, bool schrodinger)
     {

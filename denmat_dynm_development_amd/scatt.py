@@ -158,6 +158,12 @@ class ScatteringOperator:
         and compute_rhs keep only the populations."""
         call("dmd_scatt_set_semiclassical", self._h, int(bool(on)))
 
+    def set_phenom_relax(self, tau, b_start, b_end, f_eq):
+        """alg_phenom_relax: -(rho - diag f_eq) / tau on density-matrix bands
+        [b_start, b_end) in the integrators (tau <= 0: off)."""
+        p, keep = _ptr(f_eq, np.float64)
+        call("dmd_scatt_set_phenom_relax", self._h, float(tau), int(b_start), int(b_end), p)
+
     def mask_pairs(self, keep):
         """The k-pair filter of evolve (evolve_eph.cpp:52-58): P of the stored
         pairs with keep == 0 (file order) becomes 0."""

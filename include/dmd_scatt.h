@@ -346,6 +346,14 @@ int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t count, const d
  * unaffected, as in the reference. */
 int dmd_scatt_set_semiclassical(dmd_scatt *h, int on);
 
+/* alg_phenom_relax (phenom_relax, scattering_model/phenomenon_relax.h:5-51,
+ * added by compute, DensityMatrix_Dynamics.h:272-275): the integrators'
+ * right-hand side gains -(rho - diag f_eq) / tau on the density-matrix bands
+ * [b_start, b_end) (bStart_tau, bEnd_tau); f_eq is nk x nb_dm (dm_eq of
+ * set_dm_eq).  tau <= 0 switches it off.  Not for the batched integrator or
+ * the adjoint; dmd_scatt_eval is unaffected. */
+int dmd_scatt_set_phenom_relax(dmd_scatt *h, double tau, int b_start, int b_end, const double *f_eq);
+
 /* The k-pair filter of evolve (evolve_eph.cpp:52-58: alg_only_intravalley,
  * alg_only_intervalley and forbidden valley transitions, lattice.h:82-139):
  * the P of every stored pair with keep[p] = 0 (file order, n_pairs bytes)

@@ -7,6 +7,7 @@ import re
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -169,3 +170,26 @@ def test_dry_run_valleys_and_forbidden_transitions(tmp_path):
     (d / "forbid.txt").write_text("0 2\n")
     rc, out, err = _dry(d)
     assert rc != 0 and "valley index" in err
+
+
+def test_dry_run_excess_density_and_phenomenological_relaxation(tmp_path):
+    """carrier_density_means_excess_density (electron.cpp:537-552: electrons
+    minus holes over every band) gives the reference run's mu when the holes
+    are negligible (GaAs gap at 300 K) and a mid-gap mu for zero excess;
+    alg_phenom_relax reads tau_phenom in ps and the band range."""
+    text = open(os.path.join(GOLD, "param.in")).read()
+    rc, out, err = _dry(_gaas_folder(tmp_path, text + "\ncarrier_density_means_excess_density = 1\n"))
+    assert rc == 0, err
+    assert abs(_num(out, "mu") / 4.9841627e-02 - 1) < 2e-8
+    rc, out, err = _dry(_gaas_folder(tmp_path / "z", "tstep = 1\ntend = 10\ncarrier_density = 0\n"
+                                                   "carrier_density_means_excess_density = 1\n"))
+    assert rc == 0, err
+    e = np.fromfile(os.path.join(GOLD, "ldbd_ek.bin")).reshape(-1, 10)
+    mu = _num(out, "mu")
+    assert e[:, 5].max() < mu < e[:, 6].min()  # between the valence and conduction bands (nv = 6)
+    rc, out, err = _dry(_gaas_folder(tmp_path / "p", "tstep = 1\ntend = 10\nalg_phenom_relax = 1\ntau_phenom = 2\n"
+                                                   "bStart_tau = 1\nbEnd_tau = 5\n"))
+    assert rc == 0, err
+    assert abs(_num(out, "tau_phenom") / (2000 * 41.34137417094296) - 1) < 1e-10 and _num(out, "bEnd_tau") == 5
+    rc, out, err = _dry(_gaas_folder(tmp_path / "q", "tstep = 1\ntend = 10\nalg_phenom_relax = 1\nbEnd_tau = 9\n"))
+    assert rc != 0 and "bEnd_tau > nb_dm" in err

@@ -488,3 +488,62 @@ def test_dmd_run_valley_filters(tmp_path):
     print("rates", rate)
     assert abs(rate["intra"] + rate["inter"] - rate["all"]) < 2e-7 * abs(rate["all"])
     assert np.allclose(tau["forbid"], tau["intra"], rtol=2e-7, atol=0) and dropped["forbid"] == dropped["intra"]
+
+
+def test_dmd_run_phenomenological_relaxation(tmp_path):
+    """alg_phenom_relax with tau_phenom on all density-matrix bands and no
+    scattering switched off: the runner's S_z against the oracle's RK4 plus
+    -(rho - diag f_eq)/tau, f_eq at the carrier-conserving mu."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print("case dmd_run phenom relax", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    tau_ps = 0.05
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n"
+                                f"alg_phenom_relax = 1\ntau_phenom = {tau_ps}\nbStart_tau = 0\nbEnd_tau = 2\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0 and "phenomenological relaxation" in r.stdout
+    info = ldbd.read_info(str(L))
+    e = ldbd.read_ek(str(L), info)
+    k1, k2 = ldbd.read_kpairs(str(L), info)
+    P1, P2 = ldbd.read_P(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    nk, nb = e.shape
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    tau = tau_ps * 1000 * FS
+    prefac = 2 * np.pi / info.nk_full
+
+    def g(t, y):
+        out = oracle.evolve(k1, k2, P1, P2, e, y, t=t, prefac=prefac).reshape(nk, nb, nb)
+        dd = y.reshape(nk, nb, nb).copy()
+        dd[:, np.arange(nb), np.arange(nb)] -= feq
+        return (out - dd / tau).reshape(nk, -1)
+
+    obs_prefac = 1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3)
+    ref = [oracle.spin(s3[:, 2], e, f, rho, 0.0, 0, nb, obs_prefac)]
+    y, dt = rho.copy(), 1 * FS
+    for i in range(4):
+        t = i * dt
+        a1 = g(t, y)
+        a2 = g(t + dt / 2, y + dt / 2 * a1)
+        a3 = g(t + dt / 2, y + dt / 2 * a2)
+        a4 = g(t + dt, y + dt * a3)
+        y = y + dt / 6 * (a1 + 2 * a2 + 2 * a3 + a4)
+        yy = y.reshape(nk, nb, nb)
+        y = (0.5 * (yy + yy.conj().transpose(0, 2, 1))).reshape(nk, -1)
+        ref.append(oracle.spin(s3[:, 2], e, f, y, (i + 1) * dt, 0, nb, obs_prefac))
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    ref = np.array(ref)
+    print("S_z runner", rows[:, 1], "oracle", ref)
+    assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()

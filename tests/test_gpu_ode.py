@@ -194,3 +194,57 @@ def test_semiclassical_right_hand_side_and_steppers():
         assert np.abs(ye - ref).max() < 1e-14 * np.abs(ref).max()
         with pytest.raises(_lib.DMDError):
             op.integrate_rk4_batched(np.stack([rho0, rho0]), 0.0, dt, 1)
+
+
+def test_phenomenological_relaxation_term():
+    """alg_phenom_relax (phenom_relax, scattering_model/phenomenon_relax.h:42-51):
+    the integrators' right-hand side gains -(rho - diag f_eq)/tau on the
+    density-matrix bands [b_start, b_end), here overlapping the e-ph window
+    partly; compute_rhs and RK4 against the oracle plus that term in numpy."""
+    _lib, scatt, oracle = _mods()
+    print("case phenom relax", flush=True)
+    rng = np.random.default_rng(86)
+    nk, nb, nbd, b0 = 24, 2, 4, 1
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    e_dm = np.sort(rng.uniform(0, 4e-3, (nk, nbd)), axis=1)
+    e_dm[:, b0:b0 + nb] = e
+    P1, P2 = random_P(len(k1), nb, rng)
+    prefac = 2 * np.pi / nk
+    feq = rng.uniform(0.1, 0.9, (nk, nbd))
+    tau, bs, be = 300 * FS, 0, 3
+    rho0 = random_rho(nk, nbd, rng, scale=0.03, diag=0.3)
+
+    def rhs(t, y):
+        yy = y.reshape(nk, nbd, nbd)
+        inner = oracle.evolve(k1, k2, P1, P2, e, yy[:, b0:b0 + nb, b0:b0 + nb].reshape(nk, -1), t=t, prefac=prefac)
+        out = np.zeros_like(yy)
+        out[:, b0:b0 + nb, b0:b0 + nb] = inner.reshape(nk, nb, nb)
+        d = yy[:, bs:be, bs:be].copy()
+        d[:, np.arange(be - bs), np.arange(be - bs)] -= feq[:, bs:be]
+        out[:, bs:be, bs:be] -= d / tau
+        return out.reshape(nk, -1)
+
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        op.set_window(nbd, b0, e_dm)
+        op.set_phenom_relax(tau, bs, be, feq)
+        assert rel_err(op.compute_rhs(9 * FS, rho0), rhs(9 * FS, rho0)) < 1e-12
+        dt, n = 2 * FS, 5
+        got, _ = op.integrate_rk4(rho0, 0.0, dt, n)
+        y = rho0.copy()
+        for i in range(n):
+            t = i * dt
+            s1 = rhs(t, y)
+            s2 = rhs(t + dt / 2, y + dt / 2 * s1)
+            s3 = rhs(t + dt / 2, y + dt / 2 * s2)
+            s4 = rhs(t + dt, y + dt * s3)
+            y = y + dt / 6 * (s1 + 2 * s2 + 2 * s3 + s4)
+            yy = y.reshape(nk, nbd, nbd)
+            y = (0.5 * (yy + yy.conj().transpose(0, 2, 1))).reshape(nk, -1)
+        assert np.abs(got - y).max() < 1e-11 * np.abs(y - rho0).max()
+        with pytest.raises(_lib.DMDError):
+            op.set_phenom_relax(tau, 2, 5, feq)  # beyond nb_dm
+        op.set_phenom_relax(0.0, 0, 0, None)  # off
+        inner_only = op.compute_rhs(9 * FS, rho0)
+        assert rel_err(inner_only, rhs(9 * FS, rho0) + 0) > 1e-6
