@@ -19,10 +19,11 @@
 // fixed-step rk4 with step min(tstep, ode_hmax); reports every freq_measure
 // steps of tstep, and tau_s*_tot.out every freq_compute_tau steps.
 // alg_sparseP zeroes |P| <= alg_thr_sparseP and writes the sparse sP files;
-// alg_Pin_is_sparse reads P from them.  Features whose physics this operator
-// does not carry (e-e, model e-i, pump pulses, phenomenological relaxation,
-// orbital or g-factor terms) stop the run with a message rather than being
-// ignored.
+// alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
+// the valley k-pair filters and the scattering switches act as in the
+// reference.  Features whose physics this operator does not carry (e-e,
+// model e-i, pump pulses, orbital or g-factor terms) stop the run with a
+// message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
 #include <algorithm>
