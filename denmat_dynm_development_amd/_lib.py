@@ -176,6 +176,9 @@ PROTOTYPES = {
     "dmd_ldbd_read_ek": [_cs, _pLI, _d, _vp],
     "dmd_ldbd_read_kvec": [_cs, _pLI, _vp],
     "dmd_ldbd_read_smat": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_lmat": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_Bso": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_HEzmat": [_cs, _pLI, _vp],
     "dmd_ldbd_read_imsig": [_cs, _pLI, _vp],
     "dmd_ldbd_read_R": [_cs, ctypes.POINTER(_i), _vp],
     "dmd_ldbd_read_kpairs": [_cs, _pLI, _i64, _i64, _vp, _vp],

@@ -237,6 +237,32 @@ extern "C" int dmd_ldbd_read_smat(const char *dir, const dmd_ldbd_info *info, do
     return read_exact(ldbd_path(dir, "ldbd_smat.bin"), info->nk * 3 * nbd * nbd * 16, s);
 }
 
+// ldbd_lmat.bin: nk x 3 x nb_dm^2 complex128, the orbital angular momentum
+// matrices of needL (electron.cpp:315-335).
+extern "C" int dmd_ldbd_read_lmat(const char *dir, const dmd_ldbd_info *info, double *l)
+{
+    if (!info || !l) return fail(DMD_ERR_ARG, "dmd_ldbd_read_lmat: bad arguments");
+    const int64_t nbd = info->b_top_dm - info->b_bot_dm;
+    return read_exact(ldbd_path(dir, "ldbd_lmat.bin"), info->nk * 3 * nbd * nbd * 16, l);
+}
+
+// ldbd_Bso.bin: nk x 3 float64, the internal spin-orbit field of
+// alg_read_Bso (electron.cpp:159-170).
+extern "C" int dmd_ldbd_read_Bso(const char *dir, const dmd_ldbd_info *info, double *bso)
+{
+    if (!info || !bso) return fail(DMD_ERR_ARG, "dmd_ldbd_read_Bso: bad arguments");
+    return read_exact(ldbd_path(dir, "ldbd_Bso.bin"), info->nk * 3 * 8, bso);
+}
+
+// ldbd_HEzmat.bin: nk x nb_dm^2 complex128, the electric-field Hamiltonian
+// of scale_Ez (electron.cpp:44-61).
+extern "C" int dmd_ldbd_read_HEzmat(const char *dir, const dmd_ldbd_info *info, double *hez)
+{
+    if (!info || !hez) return fail(DMD_ERR_ARG, "dmd_ldbd_read_HEzmat: bad arguments");
+    const int64_t nbd = info->b_top_dm - info->b_bot_dm;
+    return read_exact(ldbd_path(dir, "ldbd_HEzmat.bin"), info->nk * nbd * nbd * 16, hez);
+}
+
 // ldbd_imsig.bin: nk x nb_eph float64 (electron.cpp:181-198).
 extern "C" int dmd_ldbd_read_imsig(const char *dir, const dmd_ldbd_info *info, double *imsig)
 {

@@ -12,7 +12,9 @@
 // the initial state is the Bpert first-order state (electron::
 // compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
-// B (Bx, By, Bz) adds the coherent H_BS = B . s; alg_linearize replaces the
+// B (Bx, By, Bz) adds the coherent H_BS = B . s (with needL, alg_read_Bso and
+// scale_Ez: (gs/2 B + Bso_k) . s + B/2 . l + scale_Ez H_Ez, electron.cpp:19-61,
+// and alg_set_scv_zero); alg_linearize replaces the
 // operator by its linearization at that f_eq (set_Lsc).  The evolution is the
 // on-device integrator alg_ode_method picks: rkf45 (the reference's default,
 // adaptive with ode_hstart / ode_hmin / ode_hmax / ode_epsabs), euler, or
@@ -22,7 +24,7 @@
 // alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
 // the valley k-pair filters and the scattering switches act as in the
 // reference.  Features whose physics this operator does not carry (e-e,
-// model e-i, pump pulses, orbital or g-factor terms) stop the run with a
+// model e-i, pump pulses, k-resolved random g factors) stop the run with a
 // message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
@@ -59,6 +61,8 @@ struct Settings {
     double tau_phenom = 0;
     int bStart_tau = 0, bEnd_tau = 0;
     bool excess = false;  // carrier_density_means_excess_density
+    bool read_Bso = false, needL = false, set_scv_zero = false;
+    double scale_Ez = 0;
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false;
@@ -160,10 +164,12 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
             }
         }
     }
-    refuse(p, "alg_read_Bso", 0, "the internal spin-orbit field");
-    refuse(p, "needL", 0, "the orbital angular momentum term");
-    refuse(p, "gfac_normal_dist", 0, "k-resolved g factors");
-    refuse(p, "scale_Ez", 0, "the electric-field Hamiltonian");
+    // coherent-Hamiltonian inputs (electron.cpp:19-61, parameters.cpp:115,186,295)
+    s.read_Bso = p.get("alg_read_Bso", 0) != 0;
+    s.needL = p.get("needL", 0) != 0;
+    s.scale_Ez = p.get("scale_Ez", 0.);
+    s.set_scv_zero = p.get("alg_set_scv_zero", 0) != 0;
+    refuse(p, "gfac_normal_dist", 0, "k-resolved random g factors");
     s.excess = p.get("carrier_density_means_excess_density", 0) != 0;  // parameters.cpp:170
     if (p.str("eeMode", "none") != "none") throw std::runtime_error("eeMode: e-e scattering is not carried by this operator");
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
@@ -345,6 +351,7 @@ struct Report {
     std::string name;  // "sx_elec" ...
     int comp, i0, i1;
     FILE *fp = nullptr;
+    bool orbital = false;  // lx/ly/lz: ldbd_lmat.bin in place of the spin matrices
 };
 
 void mkdir_p(const std::string &d)
@@ -418,6 +425,8 @@ int run(const std::string &folder, bool dry)
                 s.thr_sparseP);
     std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d alg_semiclassical = %d\n",
                 (int)s.scatt_enable, (int)s.eph_enable, (int)s.only_eimp, (int)s.semiclassical);
+    std::printf("alg_read_Bso = %d needL = %d scale_Ez = %lg alg_set_scv_zero = %d\n", (int)s.read_Bso, (int)s.needL,
+                s.scale_Ez, (int)s.set_scv_zero);
     std::printf("alg_phenom_relax = %d tau_phenom = %.10le bStart_tau = %d bEnd_tau = %d "
                 "carrier_density_means_excess_density = %d\n", (int)s.phenom, s.tau_phenom, s.bStart_tau, s.bEnd_tau,
                 (int)s.excess);
@@ -427,9 +436,21 @@ int run(const std::string &folder, bool dry)
     if (dry) return 0;
 
     // spin matrices (ldbd_smat.bin) and the initial state
-    std::vector<cplx> smat((size_t)nk * 3 * nbd * nbd);
+    std::vector<cplx> smat((size_t)nk * 3 * nbd * nbd), lmat;
     check(dmd_ldbd_read_smat(ldbd.c_str(), &info, (double *)smat.data()), "ldbd_smat.bin");
+    if (s.needL) {  // electron.h:87
+        lmat.resize(smat.size());
+        check(dmd_ldbd_read_lmat(ldbd.c_str(), &info, (double *)lmat.data()), "ldbd_lmat.bin");
+    }
     const size_t nbd2 = (size_t)nbd * nbd;
+    if (s.set_scv_zero)  // alg_set_scv_zero: no valence-conduction elements (electron.cpp:290-297,321-328)
+        for (std::vector<cplx> *m : {&smat, &lmat})
+            for (size_t kc = 0; kc < m->size() / nbd2; kc++)
+                for (int i = 0; i < nvd; i++)
+                    for (int j = nvd; j < nbd; j++) {
+                        (*m)[kc * nbd2 + i * nbd + j] = {0, 0};
+                        (*m)[kc * nbd2 + j * nbd + i] = {0, 0};
+                    }
     std::vector<cplx> dm((size_t)nk * nbd2);
     double t = s.t0;
     const bool interaction = s.picture == "interaction";
@@ -447,6 +468,15 @@ int run(const std::string &folder, bool dry)
                     for (int c = 0; c < 3; c++) {
                         H1.x += s.Bpert[c] * sk[c * nbd2 + i * nbd + j].x;
                         H1.y += s.Bpert[c] * sk[c * nbd2 + i * nbd + j].y;
+                    }
+                    if (s.needL) {  // H1 = (gs Bpert.s + Bpert.l) / 2 (electron.cpp:74-76)
+                        const cplx *lk = &lmat[(size_t)k * 3 * nbd2];
+                        cplx L1 = {0, 0};
+                        for (int c = 0; c < 3; c++) {
+                            L1.x += s.Bpert[c] * lk[c * nbd2 + i * nbd + j].x;
+                            L1.y += s.Bpert[c] * lk[c * nbd2 + i * nbd + j].y;
+                        }
+                        H1 = {0.5 * (2.0023193043625635 * H1.x + L1.x), 0.5 * (2.0023193043625635 * H1.y + L1.y)};
                     }
                     const double ei = edm[(size_t)k * nbd + i], ej = edm[(size_t)k * nbd + j];
                     const double fi = fdm[(size_t)k * nbd + i], fj = fdm[(size_t)k * nbd + j];
@@ -531,18 +561,48 @@ int run(const std::string &folder, bool dry)
         std::printf("phenomenological relaxation: tau = %.10le on bands [%d, %d)\n", s.tau_phenom, s.bStart_tau, s.bEnd_tau);
     }
     dmd_scatt *h = eph.handle();
-    if (std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10) {  // electron.cpp:19-42
-        std::vector<cplx> H((size_t)nk * nbd2);
-        for (long k = 0; k < nk; k++)
+    // H_coh = H_BS + H_Ez (main.cpp:51-52, DenMat.cpp:7-14): H_BS = Btot . s with
+    // Btot = B, or gs/2 B + Bso_k under alg_read_Bso, plus B/2 . l under needL
+    // (electron.cpp:19-42); H_Ez = scale_Ez x ldbd_HEzmat.bin when the file exists (:44-61)
+    const bool has_BS = std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10 || s.read_Bso;
+    std::vector<double> bso;
+    if (s.read_Bso) {
+        bso.resize((size_t)nk * 3);
+        check(dmd_ldbd_read_Bso(ldbd.c_str(), &info, bso.data()), "ldbd_Bso.bin");
+    }
+    std::vector<cplx> hez;
+    if (std::fabs(s.scale_Ez) > 1e-10) {
+        struct stat hs;
+        if (stat((ldbd + "/ldbd_HEzmat.bin").c_str(), &hs) == 0) {
+            hez.resize((size_t)nk * nbd2);
+            check(dmd_ldbd_read_HEzmat(ldbd.c_str(), &info, (double *)hez.data()), "ldbd_HEzmat.bin");
+        }
+    }
+    if (has_BS || !hez.empty()) {
+        std::vector<cplx> H((size_t)nk * nbd2, cplx{0, 0});
+        for (long k = 0; k < nk; k++) {
+            double Bt[3];
+            for (int c = 0; c < 3; c++)
+                Bt[c] = s.read_Bso ? 0.5 * 2.0023193043625635 * s.B[c] + bso[(size_t)k * 3 + c] : s.B[c];
             for (size_t q = 0; q < nbd2; q++) {
                 cplx v = {0, 0};
-                for (int c = 0; c < 3; c++) {
-                    v.x += s.B[c] * smat[((size_t)k * 3 + c) * nbd2 + q].x;
-                    v.y += s.B[c] * smat[((size_t)k * 3 + c) * nbd2 + q].y;
+                if (has_BS)
+                    for (int c = 0; c < 3; c++) {
+                        v.x += Bt[c] * smat[((size_t)k * 3 + c) * nbd2 + q].x;
+                        v.y += Bt[c] * smat[((size_t)k * 3 + c) * nbd2 + q].y;
+                        if (s.needL) {
+                            v.x += 0.5 * s.B[c] * lmat[((size_t)k * 3 + c) * nbd2 + q].x;
+                            v.y += 0.5 * s.B[c] * lmat[((size_t)k * 3 + c) * nbd2 + q].y;
+                        }
+                    }
+                if (!hez.empty()) {
+                    v.x += s.scale_Ez * hez[(size_t)k * nbd2 + q].x;
+                    v.y += s.scale_Ez * hez[(size_t)k * nbd2 + q].y;
                 }
                 H[(size_t)k * nbd2 + q] = v;
             }
-        check(dmd_scatt_set_coherent(h, (const double *)H.data()), "coherent H_BS");
+        }
+        check(dmd_scatt_set_coherent(h, (const double *)H.data()), "coherent H");
     }
 
     // reports (DensityMatrix_Dynamics.h:322-345, observable.h:131-403)
@@ -552,6 +612,11 @@ int run(const std::string &folder, bool dry)
         if (nbd > nvd) reps.push_back({std::string(cn[c]) + "_elec", c, nvd, nbd});
         if (nvd > 0) reps.push_back({std::string(cn[c]) + "_hole", c, 0, nvd});
     }
+    const char *ln[3] = {"lx", "ly", "lz"};
+    for (int c = 0; s.needL && c < 3; c++) {  // DensityMatrix_Dynamics.h:333-335
+        if (nbd > nvd) reps.push_back({std::string(ln[c]) + "_elec", c, nvd, nbd, nullptr, true});
+        if (nvd > 0) reps.push_back({std::string(ln[c]) + "_hole", c, 0, nvd, nullptr, true});
+    }
     std::vector<cplx> sc((size_t)nk * nbd2);
     for (auto &r : reps) {
         const std::string fn = base + "/" + r.name + "_tot.out";
@@ -559,15 +624,17 @@ int run(const std::string &folder, bool dry)
         const bool exists = stat(fn.c_str(), &st) == 0;
         r.fp = std::fopen(fn.c_str(), "a");
         if (!r.fp) throw std::runtime_error("cannot open " + fn);
-        if (!exists) std::fprintf(r.fp, "#time(au), s(t), s(t) without exp(iwt)\n");
+        if (!exists) std::fprintf(r.fp, r.orbital ? "#time(au), l(t), l(t) without exp(iwt)\n"
+                                                  : "#time(au), s(t), s(t) without exp(iwt)\n");
     }
     mkdir_p(base + "/restart");
     auto report = [&](int it) {
         if (it % s.freq_measure != 0) return;
         if (it > 0) check(dmd_restart_write((base + "/restart").c_str(), nk, nbd, (const double *)dm.data(), t), "restart");
         for (auto &r : reps) {
+            const std::vector<cplx> &om = r.orbital ? lmat : smat;
             for (long k = 0; k < nk; k++)
-                std::memcpy(&sc[(size_t)k * nbd2], &smat[((size_t)k * 3 + r.comp) * nbd2], nbd2 * sizeof(cplx));
+                std::memcpy(&sc[(size_t)k * nbd2], &om[((size_t)k * 3 + r.comp) * nbd2], nbd2 * sizeof(cplx));
             check(dmd_scatt_set_observable(h, (const double *)sc.data(), fdm.data(), r.i0, r.i1, prefac_obs), "observable");
             double v = 0, amp = 0;
             check(dmd_scatt_measure(h, t, (const double *)dm.data(), &v), "measure");
@@ -584,7 +651,7 @@ int run(const std::string &folder, bool dry)
     std::vector<Report> taus;
     const double Bp = std::sqrt(s.Bpert[0] * s.Bpert[0] + s.Bpert[1] * s.Bpert[1] + s.Bpert[2] * s.Bpert[2]);
     for (auto &r : reps) {
-        if (Bp > 1e-10 && s.Bpert[r.comp] == 0) continue;
+        if (r.orbital || (Bp > 1e-10 && s.Bpert[r.comp] == 0)) continue;  // spin only
         taus.push_back({"tau_" + r.name, r.comp, r.i0, r.i1});
         const std::string fn = base + "/" + taus.back().name + "_tot.out";
         struct stat st2;

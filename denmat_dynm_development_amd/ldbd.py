@@ -53,6 +53,30 @@ def read_smat(dirname, info=None):
     return s
 
 
+def read_lmat(dirname, info=None):
+    """ldbd_lmat.bin (needL) as (nk, 3, nb_dm * nb_dm) complex128."""
+    info = _info(dirname, info)
+    m = np.empty((info.nk, 3, info.nb_dm * info.nb_dm), np.complex128)
+    call("dmd_ldbd_read_lmat", _d(dirname), ctypes.byref(info), m.ctypes.data)
+    return m
+
+
+def read_Bso(dirname, info=None):
+    """ldbd_Bso.bin (alg_read_Bso) as (nk, 3) float64."""
+    info = _info(dirname, info)
+    b = np.empty((info.nk, 3), np.float64)
+    call("dmd_ldbd_read_Bso", _d(dirname), ctypes.byref(info), b.ctypes.data)
+    return b
+
+
+def read_HEzmat(dirname, info=None):
+    """ldbd_HEzmat.bin (scale_Ez) as (nk, nb_dm * nb_dm) complex128."""
+    info = _info(dirname, info)
+    m = np.empty((info.nk, info.nb_dm * info.nb_dm), np.complex128)
+    call("dmd_ldbd_read_HEzmat", _d(dirname), ctypes.byref(info), m.ctypes.data)
+    return m
+
+
 def read_imsig(dirname, info=None):
     info = _info(dirname, info)
     x = np.empty((info.nk, info.nb_eph), np.float64)
@@ -136,5 +160,6 @@ def write_restart(dirname, dm, t):
     call("dmd_restart_write", _d(dirname), int(nk), int(nb), dm.ctypes.data, float(t))
 
 
-__all__ = ["LdbdInfo", "read_info", "read_ek", "read_kvec", "read_smat", "read_imsig", "read_R", "read_kpairs",
-           "read_P", "sP_path", "read_sP", "read_restart", "read_restart_time", "write_restart", "_lib"]
+__all__ = ["LdbdInfo", "read_info", "read_ek", "read_kvec", "read_smat", "read_lmat", "read_Bso", "read_HEzmat",
+           "read_imsig", "read_R", "read_kpairs", "read_P", "sP_path", "read_sP", "read_restart", "read_restart_time",
+           "write_restart", "_lib"]

@@ -304,6 +304,12 @@ int dmd_ldbd_read_ek(const char *dir, const dmd_ldbd_info *info, double scissor,
 int dmd_ldbd_read_kvec(const char *dir, const dmd_ldbd_info *info, double *kvec);
 /* ldbd_smat.bin, nk x 3 x nb_dm^2 complex128 (electron.cpp:284-296). */
 int dmd_ldbd_read_smat(const char *dir, const dmd_ldbd_info *info, double *s);
+/* ldbd_lmat.bin (needL, electron.cpp:315-335): nk x 3 x nb_dm^2 complex. */
+int dmd_ldbd_read_lmat(const char *dir, const dmd_ldbd_info *info, double *l);
+/* ldbd_Bso.bin (alg_read_Bso, electron.cpp:159-170): nk x 3 float64. */
+int dmd_ldbd_read_Bso(const char *dir, const dmd_ldbd_info *info, double *bso);
+/* ldbd_HEzmat.bin (scale_Ez, electron.cpp:44-61): nk x nb_dm^2 complex. */
+int dmd_ldbd_read_HEzmat(const char *dir, const dmd_ldbd_info *info, double *hez);
 /* ldbd_imsig.bin, nk x nb_eph float64 (electron.cpp:181-198). */
 int dmd_ldbd_read_imsig(const char *dir, const dmd_ldbd_info *info, double *imsig);
 /* ldbd_R.dat: dim and the lattice vectors as rows of R[9] (lattice.h:48-56). */

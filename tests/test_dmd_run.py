@@ -113,6 +113,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("alg_eph_enable = 0\nalg_only_eimp = 1", "only possible if impurity_density"),
     ("alg_only_eimp = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "Set alg_eph_enable = 0"),
     ("alg_sparseP = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "does not support sparse"),
+    ("gfac_normal_dist = 1", "random g factors"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
@@ -193,3 +194,16 @@ def test_dry_run_excess_density_and_phenomenological_relaxation(tmp_path):
     assert abs(_num(out, "tau_phenom") / (2000 * 41.34137417094296) - 1) < 1e-10 and _num(out, "bEnd_tau") == 5
     rc, out, err = _dry(_gaas_folder(tmp_path / "q", "tstep = 1\ntend = 10\nalg_phenom_relax = 1\nbEnd_tau = 9\n"))
     assert rc != 0 and "bEnd_tau > nb_dm" in err
+
+
+def test_dry_run_coherent_hamiltonian_inputs(tmp_path):
+    """needL, alg_read_Bso, scale_Ez and alg_set_scv_zero (parameters.cpp:115,
+    186, 295, 323) are read, not refused, and default to off."""
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n"))
+    assert rc == 0, err
+    assert "alg_read_Bso = 0 needL = 0 scale_Ez = 0 alg_set_scv_zero = 0" in out
+    rc, out, err = _dry(_gaas_folder(tmp_path / "b", "tstep = 1\ntend = 10\nneedL = 1\nalg_read_Bso = 1\n"
+                                                   "scale_Ez = 0.25\nalg_set_scv_zero = 1\n"))
+    assert rc == 0, err
+    assert "alg_read_Bso = 1 needL = 1 scale_Ez = 0.25 alg_set_scv_zero = 1" in out
+    assert "not read by this runner" not in out

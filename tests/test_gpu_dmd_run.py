@@ -547,3 +547,79 @@ def test_dmd_run_phenomenological_relaxation(tmp_path):
     ref = np.array(ref)
     print("S_z runner", rows[:, 1], "oracle", ref)
     assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def _herm_fields(rng, nk, nb, ncomp, scale):
+    """ncomp Hermitian nb x nb matrices per k, flattened (nk, ncomp, nb*nb)."""
+    a = rng.normal(size=(nk, ncomp, nb, nb)) + 1j * rng.normal(size=(nk, ncomp, nb, nb))
+    return (scale * 0.5 * (a + a.conj().transpose(0, 1, 3, 2))).reshape(nk, ncomp, nb * nb)
+
+
+@pytest.mark.parametrize("scv_zero", [0, 1])
+def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
+    """needL, alg_read_Bso and scale_Ez together (electron.cpp:19-61, 62-98,
+    DenMat.cpp:7-14): the first-order state from (gs Bpert.s + Bpert.l)/2, the
+    coherent H = (gs/2 B + Bso_k).s + B/2.l + scale_Ez H_Ez, and S_z, L_z per
+    report against the oracle's RK4 with that H; with alg_set_scv_zero (and
+    nv = 1) the valence-conduction elements of s and l are zeroed first."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print(f"case dmd_run needL/read_Bso/scale_Ez scv_zero={scv_zero}", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    if scv_zero:  # one valence band inside the density-matrix window
+        lines = (L / "ldbd_size.dat").read_text().split("\n")
+        f2 = lines[1].split()
+        f2[1] = "1"
+        lines[1] = " ".join(f2)
+        (L / "ldbd_size.dat").write_text("\n".join(lines))
+    info = ldbd.read_info(str(L))
+    nk, nb = info.nk, info.nb_dm
+    rng = np.random.default_rng(11)
+    l3 = _herm_fields(rng, nk, nb, 3, 0.7)
+    bso = rng.normal(scale=1e-5, size=(nk, 3))
+    hez = _herm_fields(rng, nk, nb, 1, 4e-6)[:, 0]
+    l3.tofile(L / "ldbd_lmat.bin")
+    bso.tofile(L / "ldbd_Bso.bin")
+    hez.tofile(L / "ldbd_HEzmat.bin")
+    scale_ez, B = 0.7, np.array([0.5, 0.0, 1.5])
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\nBxpert = 0.5\nBx = 0.5\nBz = 1.5\nneedL = 1\n"
+                                f"alg_read_Bso = 1\nscale_Ez = {scale_ez}\nalg_set_scv_zero = {scv_zero}\n"
+                                "tstep = 1\ntend = 4\nfreq_measure = 1\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    assert ldbd.read_lmat(str(L), info).shape == (nk, 3, nb * nb)
+    assert np.array_equal(ldbd.read_Bso(str(L), info), bso) and np.array_equal(ldbd.read_HEzmat(str(L), info), hez)
+    e = ldbd.read_ek(str(L), info)
+    k1, k2 = ldbd.read_kpairs(str(L), info)
+    P1, P2 = ldbd.read_P(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    nv = 1 if scv_zero else 0
+    if scv_zero:
+        for m in (s3, l3):
+            m4 = m.reshape(nk, 3, nb, nb)
+            m4[:, :, :nv, nv:] = 0
+            m4[:, :, nv:, :nv] = 0
+    gs = 2.0023193043625635
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    rho0 = oracle.dm_Bpert_1st(e, f, 0.5 * (gs * s3 + l3), np.array([0.5 * TESLA, 0, 2 * TESLA]), T, 0.0, True)
+    rho0 = rho0.reshape(nk, nb, nb)
+    rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    Bt = 0.5 * gs * B * TESLA + bso
+    H = np.einsum("kc,kcq->kq", Bt, s3) + 0.5 * TESLA * np.einsum("c,kcq->kq", B, l3) + scale_ez * hez
+    obs_prefac = 1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3)
+    for name, m in (("sz", s3), ("lz", l3)):
+        _, obs = oracle.rk4(k1, k2, P1, P2, e, rho0, 1 * FS, 4, prefac=2 * np.pi / info.nk_full, H=H, report_every=1,
+                            s_obs=m[:, 2], f_dm=f, i0=nv, i1=nb, obs_prefac=obs_prefac)
+        rows = np.loadtxt(d / f"{name}_elec_tot.out")
+        print(name, "runner", rows[:, 1], "oracle", obs)
+        assert np.abs(rows[:, 1] - obs).max() < 1e-10 * np.abs(obs).max()
+    head = (d / "lz_elec_tot.out").read_text().splitlines()[0]
+    assert head == "#time(au), l(t), l(t) without exp(iwt)"
+    assert not (d / "tau_lz_elec_tot.out").exists()
