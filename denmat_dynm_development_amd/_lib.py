@@ -22,6 +22,9 @@ DMD_ERR_NOMEM = -6
 DMD_PREC_FP64 = 0
 DMD_PREC_C64 = 1
 DMD_PREC_BF16 = 2
+DMD_SCR_UNSCREENED = 0
+DMD_SCR_DEBYE = 1
+DMD_SCR_BECHSTEDT = 2
 
 DMD_KERNEL_AUTO = 0
 DMD_KERNEL_ROWS = 1
@@ -96,6 +99,7 @@ class LdbdInfo(ctypes.Structure):
         ("e_bot_eph", ctypes.c_double),
         ("e_top_eph", ctypes.c_double),
         ("omega_max", ctypes.c_double),
+        ("b_skip_wannier", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -110,6 +114,22 @@ class LdbdInfo(ctypes.Structure):
     @property
     def nb_eph(self):
         return self.b_top_eph - self.b_bot_eph
+
+
+class EimpModel(ctypes.Structure):
+    """dmd_eimp_model (include/dmd_scatt.h): the model ionized-impurity matrix
+    element and its static screening."""
+    _fields_ = [("screening", ctypes.c_int), ("lindblad", ctypes.c_int), ("detail_balance", ctypes.c_int),
+                ("nb_wannier", ctypes.c_int), ("prefac_A", ctypes.c_double), ("prefac_vq", ctypes.c_double),
+                ("prefac_vq_bare", ctypes.c_double), ("qscr2_debye", ctypes.c_double), ("fac0", ctypes.c_double),
+                ("fac2", ctypes.c_double), ("fac4", ctypes.c_double), ("degauss", ctypes.c_double),
+                ("ethr", ctypes.c_double), ("omegaL", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("GGT", ctypes.c_double * 9)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["GGT"] = list(self.GGT)
+        return d
 
 
 class OdeStats(ctypes.Structure):
@@ -140,6 +160,7 @@ PROTOTYPES = {
     "dmd_scatt_upload_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_add_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_download_P": [_vp, _i64, _i64, _vp, _vp],
+    "dmd_scatt_add_eimp_model": [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
     "dmd_scatt_generate_P": [_vp, _pSP],
     "dmd_scatt_set_flags": [_vp, _i, _i, _i],
     "dmd_scatt_set_ddmdt_eq": [_vp, _vp],
@@ -179,6 +200,7 @@ PROTOTYPES = {
     "dmd_ldbd_read_lmat": [_cs, _pLI, _vp],
     "dmd_ldbd_read_Bso": [_cs, _pLI, _vp],
     "dmd_ldbd_read_HEzmat": [_cs, _pLI, _vp],
+    "dmd_ldbd_read_Umat": [_cs, _pLI, _vp],
     "dmd_ldbd_read_imsig": [_cs, _pLI, _vp],
     "dmd_ldbd_read_R": [_cs, ctypes.POINTER(_i), _vp],
     "dmd_ldbd_read_kpairs": [_cs, _pLI, _i64, _i64, _vp, _vp],

@@ -1,0 +1,204 @@
+// dmd_eimp.hip -- model ionized-impurity scattering (impMode = model_ionized)
+// generated on the device: elecimp_model::calc_P (src_v4.5.7/scattering_model/
+// ElecImp_Model.h:185-213) for every stored pair, with the matrix element of
+// calc_eimp / calc_A (:67-137) and the static screened Coulomb potential of
+// coulomb_model::vq (Coulomb_Model.h:632-662), then folded into P as
+// add_scatt_contrib("eimp") does (ElectronPhonon.cpp:44-113).
+//
+// Per stored pair (k, k'), bands b1 of k and b2 of k':
+//   <k|k'>_{b1 b2} = sum_w conj(U_k[w, b1]) U_k'[w, b2]          (calc_ovlp)
+//   M_{b1 b2}      = prefac_A v(q) <k|k'>_{b1 b2},  q = wrap(k - k')
+//   A2 = M sqrt-or-full Gaussian(e_k,b1 - e_k',b2) inside |de| < ethr,
+//   A1 = A2 (lindblad) or M (conventional)
+//   P1[(i1 i2),(i3 i4)] = A1[i1,i3] conj(A2[i2,i4])
+//   P2[(i1 i2),(i3 i4)] = A1[i3,i1] conj(A2[i4,i2])
+// with the emission/absorption pair of A at de -+ omegaL and the detailed-
+// balance factors under detailBalance.  One thread per P element: the A
+// entries it needs are four overlaps of nb_wannier terms each, far cheaper
+// than the element's two 16-byte writes, so nothing is staged per pair and
+// every write is coalesced.  The chunk goes into the handle's P layout
+// through place_P_dev (all precisions).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "dmd_internal.h"
+
+namespace dmd {
+
+struct EimpConst {
+    dmd_eimp_model m;
+    double prefac_delta, prefac_exp;
+};
+
+// |wrap(ka - kb)|^2 under GGT (comm/kmap.h:6-15, matrix3.h:119-122), then
+// the static v(q) of the chosen formula
+__device__ double eimp_vq(const dmd_eimp_model &m, const double *ka, const double *kb)
+{
+    double v[3];
+    for (int d = 0; d < 3; d++) {
+        v[d] = ka[d] - kb[d];
+        v[d] -= floor(0.5 + v[d]);
+        if (fabs(v[d] - 0.5) < 1e-6) v[d] = -0.5;
+    }
+    const double *G = m.GGT;
+    const double q2 = v[0] * v[0] * G[0] + v[1] * v[1] * G[4] + v[2] * v[2] * G[8] +
+                      2 * (v[0] * v[1] * G[1] + v[0] * v[2] * G[2] + v[1] * v[2] * G[5]);
+    if (m.screening == DMD_SCR_DEBYE) return m.prefac_vq / (m.qscr2_debye + q2);
+    if (q2 < 1e-20) return 0.0;  // Gamma is skipped by the unscreened and Bechstedt forms
+    if (m.screening == DMD_SCR_BECHSTEDT) {
+        const double epsq = 1 + 1. / (m.fac0 + m.fac2 * q2 + m.fac4 * q2 * q2);
+        return m.prefac_vq_bare / epsq / q2;
+    }
+    return m.prefac_vq / q2;
+}
+
+// The A entries of one (b1, b2): without detailed balance a[0] = A1, a[1] = A2;
+// with it a[0..7] = A1pp, A1pm, A2pp, A2pm, A1mp, A1mm, A2mp, A2mm.
+__device__ void eimp_A(const EimpConst &c, double pv, const double2 *__restrict__ Uk, const double2 *__restrict__ Ukp,
+                       int nb, int b1, int b2, double de, double2 *a)
+{
+    double2 o = make_double2(0.0, 0.0);
+    for (int w = 0; w < c.m.nb_wannier; w++) o = cadd(o, cmulc(Ukp[w * nb + b2], Uk[w * nb + b1]));
+    const double2 M = make_double2(pv * o.x, pv * o.y);
+    const double ethr = c.m.ethr;
+    if (!c.m.detail_balance) {
+        double2 A2 = make_double2(0.0, 0.0);
+        if (fabs(de) < ethr) {
+            const double dl = c.prefac_delta * exp(c.prefac_exp * (de * de));
+            A2 = make_double2(M.x * dl, M.y * dl);
+        }
+        a[0] = c.m.lindblad ? A2 : M;
+        a[1] = A2;
+        return;
+    }
+    const double wL = c.m.omegaL;
+    double2 G2p = make_double2(0.0, 0.0), G2m = make_double2(0.0, 0.0);
+    if (fabs(de + wL) < ethr) {
+        const double dl = c.prefac_delta * exp(c.prefac_exp * ((de + wL) * (de + wL)));
+        G2p = make_double2(M.x * dl, M.y * dl);
+    }
+    if (fabs(de - wL) < ethr) {
+        const double dl = c.prefac_delta * exp(c.prefac_exp * ((de - wL) * (de - wL)));
+        G2m = make_double2(M.x * dl, M.y * dl);
+    }
+    const double2 G1p = c.m.lindblad ? G2p : M, G1m = c.m.lindblad ? G2m : M;
+    const double dEbyT = de / c.m.temperature;
+    const double fp = (-dEbyT < 46) ? exp(-dEbyT / 2) : 1.0, fm = (dEbyT < 46) ? exp(dEbyT / 2) : 1.0;
+    a[0] = make_double2(G1p.x * fp, G1p.y * fp);
+    a[1] = G1p;
+    a[2] = make_double2(G2p.x * fp, G2p.y * fp);
+    a[3] = G2p;
+    a[4] = make_double2(G1m.x * fm, G1m.y * fm);
+    a[5] = G1m;
+    a[6] = make_double2(G2m.x * fm, G2m.y * fm);
+    a[7] = G2m;
+}
+
+// P1, P2 of n stored pairs (k1[f], k2[f]) in the ldbd_P element order
+template <int NB>
+__global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *__restrict__ k1,
+                                                    const uint64_t *__restrict__ k2, const double2 *__restrict__ U,
+                                                    const double *__restrict__ kvec, const double *__restrict__ e,
+                                                    EimpConst c, double2 *__restrict__ P1, double2 *__restrict__ P2)
+{
+    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * NB4) return;
+    const int64_t f = idx / NB4;
+    const int el = (int)(idx - f * NB4);
+    const int i1 = el / (NB * NB2), i2 = (el / NB2) % NB, i3 = (el / NB) % NB, i4 = el % NB;
+    const int64_t k = (int64_t)k1[f], kp = (int64_t)k2[f];
+    const double pv = c.m.prefac_A * eimp_vq(c.m, kvec + 3 * k, kvec + 3 * kp);
+    const int nw = c.m.nb_wannier;
+    const double2 *Uk = U + k * nw * NB, *Ukp = U + kp * nw * NB;
+    const double *ek = e + k * NB, *ekp = e + kp * NB;
+    double2 a13[8], a24[8], a31[8], a42[8];
+    eimp_A(c, pv, Uk, Ukp, NB, i1, i3, ek[i1] - ekp[i3], a13);
+    eimp_A(c, pv, Uk, Ukp, NB, i2, i4, ek[i2] - ekp[i4], a24);
+    eimp_A(c, pv, Uk, Ukp, NB, i3, i1, ek[i3] - ekp[i1], a31);
+    eimp_A(c, pv, Uk, Ukp, NB, i4, i2, ek[i4] - ekp[i2], a42);
+    double2 p1, p2;
+    if (!c.m.detail_balance) {
+        p1 = cmulc(a13[0], a24[1]);
+        p2 = cmulc(a31[0], a42[1]);
+    } else {
+        p1 = cadd(cmulc(a13[0], a24[2]), cmulc(a13[5], a24[7]));  // A1pp conj(A2pp) + A1mm conj(A2mm)
+        p2 = cadd(cmulc(a31[4], a42[6]), cmulc(a31[1], a42[3]));  // A1mp conj(A2mp) + A1pm conj(A2pm)
+    }
+    P1[idx] = p1;
+    P2[idx] = p2;
+}
+
+}  // namespace dmd
+
+using namespace dmd;
+
+extern "C" int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                                        const double *kvec, const dmd_eimp_model *m, double scale, int add)
+{
+    if (!h || !m || (h->np > 0 && (!k1st || !k2nd || !U || !kvec)))
+        return fail(DMD_ERR_ARG, "dmd_scatt_add_eimp_model: null argument");
+    if (m->nb_wannier <= 0 || !(m->degauss > 0) || !(m->temperature > 0) || m->screening < DMD_SCR_UNSCREENED ||
+        m->screening > DMD_SCR_BECHSTEDT)
+        return fail(DMD_ERR_ARG, "dmd_scatt_add_eimp_model: bad model (nb_wannier, degauss, temperature, screening)");
+    if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator: add the e-i P before set_Lsc");
+    if (add && !h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated (add = 0 sets it)");
+    if (h->nb < 1 || h->nb > 4) return fail(DMD_ERR_UNSUPPORTED, "the model e-i P is built for nb = 1..4");
+    for (int64_t q = 0; q < h->np; q++)
+        if (k1st[q] >= (uint64_t)h->nk || k2nd[q] >= (uint64_t)h->nk)
+            return fail(DMD_ERR_ARG, "dmd_scatt_add_eimp_model: k index outside [0, nk)");
+    if (h->np == 0) { h->has_P = true; return DMD_OK; }
+    EimpConst c;
+    c.m = *m;
+    const double pg = 1. / (m->degauss * std::sqrt(2. * M_PI));  // ElecImp_Model.h:36-39
+    c.prefac_delta = m->lindblad ? std::sqrt(pg) : pg;
+    if (m->detail_balance) c.prefac_delta *= 0.5;
+    c.prefac_exp = m->lindblad ? -0.25 / std::pow(m->degauss, 2) : -0.5 / std::pow(m->degauss, 2);
+
+    const int64_t per = h->nb4, nw = m->nb_wannier;
+    const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / per));
+    double2 *dU = nullptr, *s1 = nullptr, *s2 = nullptr;
+    double *dk = nullptr;
+    uint64_t *d1 = nullptr, *d2 = nullptr;
+    int rc;
+    auto release = [&]() { dfree(dU); dfree(dk); dfree(s1); dfree(s2); dfree(d1); dfree(d2); };
+    if ((rc = dalloc_t(&dU, (size_t)(h->nk * nw * h->nb), "U")) || (rc = dalloc_t(&dk, (size_t)h->nk * 3, "kvec")) ||
+        (rc = dalloc_t(&s1, (size_t)(chunk * per), "e-i P staging")) ||
+        (rc = dalloc_t(&s2, (size_t)(chunk * per), "e-i P staging")) || (rc = dalloc_t(&d1, (size_t)chunk, "k1")) ||
+        (rc = dalloc_t(&d2, (size_t)chunk, "k2"))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dU, U, (size_t)(h->nk * nw * h->nb) * 16, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dk, kvec, (size_t)h->nk * 3 * 8, hipMemcpyDefault, h->st);
+    for (int64_t first = 0; first < h->np && e == hipSuccess && rc == DMD_OK; first += chunk) {
+        const int64_t n = std::min(chunk, h->np - first);
+        e = hipMemcpyAsync(d1, k1st + first, (size_t)n * 8, hipMemcpyDefault, h->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d2, k2nd + first, (size_t)n * 8, hipMemcpyDefault, h->st);
+        if (e != hipSuccess) break;
+        const unsigned g = nblk(n * per, 256);
+        switch (h->nb) {
+        case 1: hipLaunchKernelGGL(k_eimp_model<1>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, c, s1, s2); break;
+        case 2: hipLaunchKernelGGL(k_eimp_model<2>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, c, s1, s2); break;
+        case 3: hipLaunchKernelGGL(k_eimp_model<3>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, c, s1, s2); break;
+        default: hipLaunchKernelGGL(k_eimp_model<4>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, c, s1, s2); break;
+        }
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        rc = place_P_dev(h, s1, s2, n, first, scale, add ? 1 : 0);
+        // the next chunk's pair copies come from pageable host memory
+        if (rc == DMD_OK) e = hipStreamSynchronize(h->st);
+    }
+    hipError_t e2 = hipStreamSynchronize(h->st);
+    release();
+    if (rc == DMD_OK && e == hipSuccess) e = e2;
+    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc) {
+        h->has_P = false;  // P partly updated
+        return rc;
+    }
+    h->has_P = true;
+    return DMD_OK;
+}

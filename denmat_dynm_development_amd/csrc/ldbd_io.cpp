@@ -163,6 +163,7 @@ extern "C" int dmd_ldbd_read_info(const char *dir, dmd_ldbd_info *info)
     info->b_bot_eph = (int)v[e0];
     info->b_top_eph = (int)v[e0 + 1];
     info->nb_wannier = n > e0 + 2 ? (int)v[e0 + 2] : 0;
+    info->b_skip_wannier = n > e0 + 3 ? (int)v[e0 + 3] : 0;  // bskipped_wannier (electron.cpp:108-111)
     if (info->nb <= 0 || info->nv < 0 || info->nv > info->nb)
         return fail(DMD_ERR_IO, path + ": need 0 <= nv <= nb and nb > 0");
     if (info->b_bot_dm < 0 || info->b_bot_dm > info->b_top_dm || info->b_top_dm > info->nb)
@@ -261,6 +262,37 @@ extern "C" int dmd_ldbd_read_HEzmat(const char *dir, const dmd_ldbd_info *info, 
     if (!info || !hez) return fail(DMD_ERR_ARG, "dmd_ldbd_read_HEzmat: bad arguments");
     const int64_t nbd = info->b_top_dm - info->b_bot_dm;
     return read_exact(ldbd_path(dir, "ldbd_HEzmat.bin"), info->nk * nbd * nbd * 16, hez);
+}
+
+// ldbd_Umat.bin (electron.cpp:388-410): the full nk x nb_wannier^2 matrices,
+// of which the e-ph columns b_bot_eph + b_skip_wannier .. + nb_eph are kept,
+// or nk x nb_wannier x nb_eph directly.
+extern "C" int dmd_ldbd_read_Umat(const char *dir, const dmd_ldbd_info *info, double *U)
+{
+    if (!info || !U) return fail(DMD_ERR_ARG, "dmd_ldbd_read_Umat: bad arguments");
+    const int64_t nw = info->nb_wannier, nbe = info->b_top_eph - info->b_bot_eph, nk = info->nk;
+    if (nw <= 0) return fail(DMD_ERR_IO, "ldbd_size.dat gives no nb_wannier for ldbd_Umat.bin");
+    const std::string path = ldbd_path(dir, "ldbd_Umat.bin");
+    int64_t bytes = -1;
+    int rc = file_bytes(path, &bytes);
+    if (rc) return rc;
+    if (bytes == nk * nw * nw * 16) {
+        const int64_t c0 = info->b_bot_eph + info->b_skip_wannier;
+        if (c0 < 0 || c0 + nbe > nw)
+            return fail(DMD_ERR_IO, path + ": e-ph columns [" + std::to_string(c0) + ", " + std::to_string(c0 + nbe) +
+                                        ") outside nb_wannier = " + std::to_string(nw));
+        std::vector<double> full((size_t)(nk * nw * nw * 2));
+        if ((rc = read_exact(path, bytes, full.data()))) return rc;
+        for (int64_t k = 0; k < nk; k++)
+            for (int64_t w = 0; w < nw; w++)
+                for (int64_t b = 0; b < nbe; b++) {
+                    const size_t src = (size_t)(((k * nw + w) * nw + c0 + b) * 2), dst = (size_t)(((k * nw + w) * nbe + b) * 2);
+                    U[dst] = full[src];
+                    U[dst + 1] = full[src + 1];
+                }
+        return DMD_OK;
+    }
+    return read_exact(path, nk * nw * nbe * 16, U);
 }
 
 // ldbd_imsig.bin: nk x nb_eph float64 (electron.cpp:181-198).

@@ -77,6 +77,15 @@ def read_HEzmat(dirname, info=None):
     return m
 
 
+def read_Umat(dirname, info=None):
+    """ldbd_Umat.bin (electron.cpp:388-410) as (nk, nb_wannier, nb_eph) complex128:
+    the e-ph columns of the full matrices, or the reduced file as it is."""
+    info = _info(dirname, info)
+    U = np.empty((info.nk, info.nb_wannier, info.nb_eph), np.complex128)
+    call("dmd_ldbd_read_Umat", _d(dirname), ctypes.byref(info), U.ctypes.data)
+    return U
+
+
 def read_imsig(dirname, info=None):
     info = _info(dirname, info)
     x = np.empty((info.nk, info.nb_eph), np.float64)
@@ -161,5 +170,5 @@ def write_restart(dirname, dm, t):
 
 
 __all__ = ["LdbdInfo", "read_info", "read_ek", "read_kvec", "read_smat", "read_lmat", "read_Bso", "read_HEzmat",
-           "read_imsig", "read_R", "read_kpairs", "read_P", "sP_path", "read_sP", "read_restart", "read_restart_time",
+           "read_Umat", "read_imsig", "read_R", "read_kpairs", "read_P", "sP_path", "read_sP", "read_restart", "read_restart_time",
            "write_restart", "_lib"]

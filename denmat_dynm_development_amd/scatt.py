@@ -190,6 +190,24 @@ class ScatteringOperator:
     def generate_P(self, params):
         call("dmd_scatt_generate_P", self._h, ctypes.byref(params))
 
+    def add_eimp_model(self, k1st, k2nd, U, kvec, model, scale=1.0, add=True):
+        """Model ionized-impurity P of every stored pair generated on the device
+        and added (add) or set: elecimp_model::calc_P through
+        add_scatt_contrib("eimp") (ElecImp_Model.h:185-213, ElectronPhonon.cpp:
+        44-113), scale = scale_ei.  U: (nk, nb_wannier, nb) complex (the e-ph
+        columns of ldbd_Umat.bin), kvec: (nk, 3), model: an EimpModel
+        (eimp_model())."""
+        k1 = np.ascontiguousarray(k1st, dtype=np.uint64)
+        k2 = np.ascontiguousarray(k2nd, dtype=np.uint64)
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        kv = np.ascontiguousarray(kvec, dtype=np.float64)
+        if len(k1) != self.n_pairs or len(k2) != self.n_pairs:
+            raise ValueError("k1st/k2nd need one entry per stored pair")
+        if U.shape != (self.nk, model.nb_wannier, self.nb) or kv.shape != (self.nk, 3):
+            raise ValueError(f"U must be (nk, nb_wannier, nb) = {(self.nk, model.nb_wannier, self.nb)} and kvec (nk, 3)")
+        call("dmd_scatt_add_eimp_model", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             ctypes.byref(model), float(scale), int(bool(add)))
+
     # -- flags / eq ----------------------------------------------------------------
     def set_flags(self, expt=True, schrodinger=False, kernel=_lib.DMD_KERNEL_AUTO):
         call("dmd_scatt_set_flags", self._h, int(bool(expt)), int(bool(schrodinger)), int(kernel))
@@ -409,6 +427,63 @@ class ElectronPhonon(ScatteringOperator):
 
     def evolve(self, t, dm, dm1=None):
         return self.eval(t, dm, dm1)
+
+
+SCREENING = {"unscreened": _lib.DMD_SCR_UNSCREENED, "debye": _lib.DMD_SCR_DEBYE,
+             "Bechstedt": _lib.DMD_SCR_BECHSTEDT}
+
+
+def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0, n_ionized=None,
+               epsilon_background=1.0, meff=1.0, degauss=None, ethr=None, scatt="lindblad",
+               detail_balance=False):
+    """The dmd_eimp_model of one ionized impurity (atomic units throughout):
+    elecimp_model's prefactors (ElecImp_Model.h:22-40: prefac_A = Z sqrt(n_i
+    V_cell), omegaL = 1e-4 T) and coulomb_model's static screening
+    (Coulomb_Model.h:419-425,554-575: prefac_vq = 4 pi / (eps V_cell); Debye
+    q^2 = 4 pi n_free / (eps T); Thomas-Fermi q^2 = 6 pi n_free / (eps E_F)
+    with k_F = (3 pi^2 n_free)^(1/3), E_F = k_F^2 / 2 meff for Bechstedt).
+    n_free and n_ionized are in bohr^-3; degauss is the delta width of the e-i
+    term, ethr the |de| cut-off (degauss x ndegauss of the e-ph set)."""
+    if screening not in SCREENING:
+        raise ValueError(f"screening {screening!r}: one of {sorted(SCREENING)} (the |q|-only static forms)")
+    if screening != "unscreened" and not n_free > 0:
+        raise ValueError("nfreetot must be postive")
+    n_ionized = n_free if n_ionized is None else n_ionized
+    if not n_ionized > 0:
+        raise ValueError("eip.ni_ionized[iD] must be postive")
+    if epsilon_background < 1:
+        raise ValueError("epsilon_background < 1 is not right")
+    m = _lib.EimpModel()
+    m.screening = SCREENING[screening]
+    m.lindblad = 1 if scatt == "lindblad" else 0
+    m.detail_balance = int(bool(detail_balance))
+    m.nb_wannier = int(nb_wannier)
+    m.prefac_A = Z * np.sqrt(n_ionized * cell_size)
+    m.prefac_vq = 4 * np.pi / epsilon_background / cell_size
+    m.prefac_vq_bare = 4 * np.pi / cell_size
+    n = max(n_free, 0.0)
+    kF = (3 * np.pi * np.pi * n) ** (1. / 3.)
+    EF = kF * kF / 2. / meff
+    qTF2 = 6 * np.pi * n / epsilon_background / EF if n > 0 else 0.0
+    m.qscr2_debye = 4 * np.pi * n / epsilon_background / temperature
+    if screening == "Bechstedt":
+        m.fac0 = 1. / (epsilon_background - 1) if epsilon_background > 1 else np.inf
+        m.fac2 = 1. / qTF2
+        m.fac4 = 3. / 4. / (kF * kF) / qTF2
+    m.degauss = float(degauss)
+    m.ethr = float(ethr)
+    m.omegaL = 1e-4 * temperature
+    m.temperature = float(temperature)
+    for i, v in enumerate(np.asarray(GGT, dtype=np.float64).reshape(9)):
+        m.GGT[i] = v
+    return m
+
+
+def reciprocal_metric(R):
+    """GGT = G G^T with G = 2 pi R^-1 (the rows of R are the lattice vectors;
+    lattice.h:40-41)."""
+    Ri = np.linalg.inv(np.asarray(R, dtype=np.float64))
+    return 4 * np.pi * np.pi * Ri @ Ri.T
 
 
 def synth_params(preset):

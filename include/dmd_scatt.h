@@ -100,6 +100,39 @@ int dmd_scatt_upload_P(dmd_scatt *h, const double *P1, const double *P2, double 
  * (ElectronPhonon.cpp:87-97; ab_neutral scale = |n_i V_cell| * scale_ei,
  * ElectronImpurity.cpp:38-39 and ElectronImpurity.h:17-18). */
 int dmd_scatt_add_P(dmd_scatt *h, const double *P1, const double *P2, double scale);
+/* Model ionized-impurity scattering (impMode = model_ionized): the screened
+ * Coulomb matrix element of elecimp_model (scattering_model/ElecImp_Model.h:
+ * 67-137) with the static screening formulas of coulomb_model::vq
+ * (Coulomb_Model.h:632-662) that depend on |q| alone. */
+typedef struct {
+    int screening;        /* DMD_SCR_UNSCREENED, DMD_SCR_DEBYE or DMD_SCR_BECHSTEDT    */
+    int lindblad;         /* 1: alg_scatt = lindblad (A1 = A2), 0: conventional          */
+    int detail_balance;   /* detailBalance: emission/absorption at +-omegaL              */
+    int nb_wannier;       /* rows of each U_k                                            */
+    double prefac_A;      /* Z sqrt(n_ionized V_cell)                                   */
+    double prefac_vq;     /* 4 pi / (epsilon_background V_cell)                         */
+    double prefac_vq_bare;/* 4 pi / V_cell                                               */
+    double qscr2_debye;   /* 4 pi n_free / (epsilon_background T)                       */
+    double fac0, fac2, fac4; /* Bechstedt: 1/(eps-1), 1/q_TF^2, 3/(4 k_F^2 q_TF^2)        */
+    double degauss;       /* Gaussian width of the delta (degauss_eimp)                  */
+    double ethr;          /* |de| cut-off, degauss * ndegauss of the e-ph set           */
+    double omegaL;        /* 1e-4 T (detail_balance only)                                */
+    double temperature;
+    double GGT[9];        /* reciprocal metric (2 pi)^2 R^-1 R^-T                        */
+} dmd_eimp_model;
+#define DMD_SCR_UNSCREENED 0
+#define DMD_SCR_DEBYE 1
+#define DMD_SCR_BECHSTEDT 2
+/* P += scale * P_imp (add = 1) or P = scale * P_imp (add = 0) with P_imp of
+ * every stored pair generated on the device: elecimp_model::calc_P
+ * (ElecImp_Model.h:185-213) called from add_scatt_contrib("eimp")
+ * (ElectronPhonon.cpp:44-113; scale = scale_ei).  k1st/k2nd are the stored
+ * pairs in file order (host); U is nk x nb_wannier x nb complex128, the
+ * e-ph band columns of ldbd_Umat.bin (dmd_ldbd_read_Umat); kvec nk x 3.
+ * The overlap <k|k'> = U_k^H U_k', the screened v(q), the Gaussian delta
+ * and the products A1 x conj(A2) are formed per element in one pass. */
+int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                             const double *kvec, const dmd_eimp_model *m, double scale, int add);
 /* P of stored pairs [first, first+count) in file order, as complex128. */
 int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2);
 /* Fills P on the device from the synthetic description (the handle's e must
@@ -296,6 +329,7 @@ typedef struct {
     double degauss, ndegauss;
     double e_bot_probe, e_top_probe, e_bot_dm, e_top_dm, e_bot_eph, e_top_eph;
     double omega_max;         /* 0 when the file has no tenth line                        */
+    int b_skip_wannier;       /* line 2 after nb_wannier: bands below the Wannier set     */
 } dmd_ldbd_info;
 int dmd_ldbd_read_info(const char *dir, dmd_ldbd_info *info);
 /* ldbd_ek.bin, nk x nb float64; scissor is added to bands nv.. (electron.cpp:172-180). */
@@ -310,6 +344,10 @@ int dmd_ldbd_read_lmat(const char *dir, const dmd_ldbd_info *info, double *l);
 int dmd_ldbd_read_Bso(const char *dir, const dmd_ldbd_info *info, double *bso);
 /* ldbd_HEzmat.bin (scale_Ez, electron.cpp:44-61): nk x nb_dm^2 complex. */
 int dmd_ldbd_read_HEzmat(const char *dir, const dmd_ldbd_info *info, double *hez);
+/* ldbd_Umat.bin (electron.cpp:388-410) as nk x nb_wannier x nb_eph complex:
+ * the full nk x nb_wannier^2 file gives columns b_bot_eph + b_skip_wannier
+ * onward, a file of that reduced size is read as it is. */
+int dmd_ldbd_read_Umat(const char *dir, const dmd_ldbd_info *info, double *U);
 /* ldbd_imsig.bin, nk x nb_eph float64 (electron.cpp:181-198). */
 int dmd_ldbd_read_imsig(const char *dir, const dmd_ldbd_info *info, double *imsig);
 /* ldbd_R.dat: dim and the lattice vectors as rows of R[9] (lattice.h:48-56). */

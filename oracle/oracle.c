@@ -29,6 +29,20 @@
  *                          oracle_evolve, the initializer's golden-rule T1
  *                          (T1_rate_dRho) equals the one-step Lindblad T1
  *                          for any drho (tests/test_oracle_t1_rate.py).
+ *   oracle_eimp_vq /       the reference's own GaAs ldbd_vq.out: every |q|^2
+ *   oracle_eimp_q2         it lists is a wrapped k - k' length of that k mesh
+ *                          (smallest nonzero 2.815e-05), and v = prefac_vq /
+ *                          (q^2 + q_scr^2) with its q_scr^2 column reproduces
+ *                          its v column; the Debye length and the Thomas-Fermi
+ *                          inputs against its header; Bechstedt limits (bare
+ *                          4 pi / V q^2 at large q, background-screened at
+ *                          small q).
+ *   oracle_eimp_model_P    the P diagonal is the golden-rule |M|^2 times a
+ *                          normalised Gaussian delta for both scattering forms
+ *                          (and the emission/absorption pair with detailed
+ *                          balance), P2(k,k') = conj(P1(k',k)), the reshaped
+ *                          P1[(i1 i3),(i2 i4)] is Hermitian PSD of rank 1
+ *                          (2 with detailed balance).
  *   oracle_coherent        two-level precession closed form, tracelessness.
  *   oracle_rk4             1+z+z^2/2+z^3/6+z^4/24 amplification, O(h^4).
  *   oracle_spin            brute force, zero at rho=f, linearity, and the
@@ -577,6 +591,114 @@ int oracle_compute_P_eimp(int nb, const double *A1, const double *A2, double *P1
                 }
             }
         }
+    return 0;
+}
+
+/* Static screened Coulomb potential, coulomb_model::vq                      */
+/* (src_v4.5.7/scattering_model/Coulomb_Model.h:632-662) for the formulas     */
+/* that depend on |q| alone, q wrapped into [-0.5, 0.5) (comm/kmap.h:6-15)    */
+/* and measured with GGT (comm/matrix3.h:119-122).                            */
+/* screening: 0 unscreened, 1 debye, 2 Bechstedt.                            */
+double oracle_eimp_q2(const double *GGT, const double *ka, const double *kb)
+{
+    double v[3];
+    for (int d = 0; d < 3; d++) {
+        v[d] = ka[d] - kb[d];
+        v[d] -= floor(0.5 + v[d]);
+        if (fabs(v[d] - 0.5) < 1e-6) v[d] = -0.5;
+    }
+    return v[0] * v[0] * GGT[0] + v[1] * v[1] * GGT[4] + v[2] * v[2] * GGT[8] +
+           2 * (v[0] * v[1] * GGT[1] + v[0] * v[2] * GGT[2] + v[1] * v[2] * GGT[5]);
+}
+double oracle_eimp_vq(int screening, double q2, double prefac_vq, double prefac_vq_bare, double qscr2_debye,
+                      double fac0, double fac2, double fac4)
+{
+    if (screening == 1) return prefac_vq / (qscr2_debye + q2);
+    if (q2 < 1e-20) return 0.0;
+    if (screening == 2) {
+        double epsq = 1 + 1. / (fac0 + fac2 * q2 + fac4 * q2 * q2);
+        return prefac_vq_bare / epsq / q2;
+    }
+    return prefac_vq / q2;
+}
+
+/* Model ionized-impurity P of stored pairs, elecimp_model::calc_P with       */
+/* calc_ovlp, calc_eimp and calc_A (ElecImp_Model.h:67-137,185-213):          */
+/* ovlp = U_k^H U_k' (U: nk x nw x nb), eimp = prefac_A v(q) ovlp,            */
+/* A2 = eimp sqrt-Gaussian (lindblad) or Gaussian (conventional) of            */
+/* de = e_k,b1 - e_k',b2 inside |de| < ethr, A1 = A2 or eimp; with detailed   */
+/* balance the +-omegaL pair and the exp(-+de/2T) factors.  P1, P2 are        */
+/* npairs x nb^4, overwritten.                                                 */
+int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                        const double *U, const double *kvec, const double *e, const double *GGT, int screening,
+                        int lindblad, int detail_balance, double prefac_A, double prefac_vq, double prefac_vq_bare,
+                        double qscr2_debye, double fac0, double fac2, double fac4, double degauss, double ethr,
+                        double omegaL, double T, double *P1, double *P2)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    double pgauss = 1. / (degauss * sqrt(2. * M_PI));
+    double pdelta = lindblad ? sqrt(pgauss) : pgauss;
+    if (detail_balance) pdelta *= 0.5;
+    double pexp = lindblad ? -0.25 / (degauss * degauss) : -0.5 / (degauss * degauss);
+    for (int64_t p = 0; p < npairs; p++)
+        if (k1[p] >= (uint64_t)nk || k2[p] >= (uint64_t)nk) return -1;
+    zc *ovlp = (zc *)malloc(sizeof(zc) * nb2), *eimp = (zc *)malloc(sizeof(zc) * nb2);
+    zc *A[8];
+    for (int a = 0; a < 8; a++) A[a] = (zc *)calloc(nb2, sizeof(zc));
+    for (int64_t p = 0; p < npairs; p++) {
+        int64_t ik = (int64_t)k1[p], jk = (int64_t)k2[p];
+        const zc *Ui = (const zc *)U + ik * nw * nb, *Uj = (const zc *)U + jk * nw * nb;
+        for (int b1 = 0; b1 < nb; b1++)
+            for (int b2 = 0; b2 < nb; b2++) {
+                zc o = zmk(0, 0);
+                for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(Ui[w * nb + b1]), Uj[w * nb + b2]));
+                ovlp[b1 * nb + b2] = o;
+            }
+        double vq = oracle_eimp_vq(screening, oracle_eimp_q2(GGT, kvec + 3 * ik, kvec + 3 * jk), prefac_vq,
+                                   prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
+        for (int q = 0; q < nb2; q++) eimp[q] = zmul(zmk(prefac_A * vq, 0), ovlp[q]);
+        for (int b1 = 0; b1 < nb; b1++)
+            for (int b2 = 0; b2 < nb; b2++) {
+                int q = b1 * nb + b2;
+                double de = e[ik * nb + b1] - e[jk * nb + b2];
+                if (!detail_balance) {
+                    A[1][q] = zmk(0, 0);
+                    if (fabs(de) < ethr) A[1][q] = zscale(eimp[q], pdelta * exp(pexp * pow(de, 2)));
+                    A[0][q] = lindblad ? A[1][q] : eimp[q];
+                } else {
+                    zc G2p = zmk(0, 0), G2m = zmk(0, 0);
+                    if (fabs(de + omegaL) < ethr) G2p = zscale(eimp[q], pdelta * exp(pexp * pow(de + omegaL, 2)));
+                    zc G1p = lindblad ? G2p : eimp[q];
+                    double dEbyT = de / T;
+                    double fDB = (-dEbyT < 46) ? exp(-dEbyT / 2) : 1;
+                    A[0][q] = zscale(G1p, fDB); A[1][q] = G1p; A[2][q] = zscale(G2p, fDB); A[3][q] = G2p;
+                    if (fabs(de - omegaL) < ethr) G2m = zscale(eimp[q], pdelta * exp(pexp * pow(de - omegaL, 2)));
+                    zc G1m = lindblad ? G2m : eimp[q];
+                    fDB = (dEbyT < 46) ? exp(dEbyT / 2) : 1;
+                    A[4][q] = zscale(G1m, fDB); A[5][q] = G1m; A[6][q] = zscale(G2m, fDB); A[7][q] = G2m;
+                }
+            }
+        zc *Q1 = (zc *)P1 + p * nb4, *Q2 = (zc *)P2 + p * nb4;
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                int n12 = (i1 * nb + i2) * nb2;
+                for (int i3 = 0; i3 < nb; i3++) {
+                    int i13 = i1 * nb + i3, i31 = i3 * nb + i1;
+                    for (int i4 = 0; i4 < nb; i4++) {
+                        int i24 = i2 * nb + i4, i42 = i4 * nb + i2;
+                        if (!detail_balance) {
+                            Q1[n12 + i3 * nb + i4] = zmul(A[0][i13], zconj(A[1][i24]));
+                            Q2[n12 + i3 * nb + i4] = zmul(A[0][i31], zconj(A[1][i42]));
+                        } else {  /* pp + mm, mp + pm */
+                            Q1[n12 + i3 * nb + i4] = zadd(zmul(A[0][i13], zconj(A[2][i24])), zmul(A[5][i13], zconj(A[7][i24])));
+                            Q2[n12 + i3 * nb + i4] = zadd(zmul(A[4][i31], zconj(A[6][i42])), zmul(A[1][i31], zconj(A[3][i42])));
+                        }
+                    }
+                }
+            }
+    }
+    free(ovlp); free(eimp);
+    for (int a = 0; a < 8; a++) free(A[a]);
     return 0;
 }
 

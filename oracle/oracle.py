@@ -42,6 +42,12 @@ def lib():
                                        ctypes.c_double, ctypes.c_int, _d, _d]
         L.oracle_compute_P_eimp.argtypes = [ctypes.c_int, _d, _d, _d, _d]
         L.oracle_coherent.argtypes = [_i64, ctypes.c_int, _d, _d, _d, ctypes.c_double, ctypes.c_int, _d]
+        L.oracle_eimp_q2.argtypes = [_d, _d, _d]
+        L.oracle_eimp_q2.restype = ctypes.c_double
+        L.oracle_eimp_vq.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7
+        L.oracle_eimp_vq.restype = ctypes.c_double
+        L.oracle_eimp_model_P.argtypes = ([_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d] +
+                                          [ctypes.c_int] * 3 + [ctypes.c_double] * 11 + [_d, _d])
         L.oracle_spin.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, _d]
         L.oracle_spin.restype = ctypes.c_double
@@ -201,6 +207,39 @@ def compute_P(Ek, Ekp, omega, gm, gp, T, sigma, detail_balance=False):
     P2 = np.zeros(nb ** 4, np.complex128)
     lib().oracle_compute_P(nb, _p(_f64(Ek)), _p(_f64(Ekp)), nm, _p(_f64(omega)), _p(_c128(gm)),
                            _p(_c128(gp)), float(T), float(sigma), int(detail_balance), _p(P1), _p(P2))
+    return P1, P2
+
+
+def eimp_q2(GGT, ka, kb):
+    """|wrap(ka - kb)|^2 under GGT (comm/kmap.h:6-15, matrix3.h:119-122)."""
+    return lib().oracle_eimp_q2(_p(_f64(GGT)), _p(_f64(ka)), _p(_f64(kb)))
+
+
+def eimp_vq(m, q2):
+    """coulomb_model::vq (Coulomb_Model.h:632-662) for a model dict m."""
+    return lib().oracle_eimp_vq(int(m["screening"]), float(q2), m["prefac_vq"], m["prefac_vq_bare"],
+                                m["qscr2_debye"], m["fac0"], m["fac2"], m["fac4"])
+
+
+def eimp_model_P(k1, k2, U, kvec, e, m):
+    """elecimp_model::calc_P of every stored pair (ElecImp_Model.h:67-137,185-213).
+    U: (nk, nb_wannier, nb) complex; kvec (nk, 3); e (nk, nb); m a dict with the
+    fields of dmd_eimp_model."""
+    e = _f64(e)
+    nk, nb = e.shape
+    U = _c128(U)
+    nw = U.shape[1]
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    P1 = np.zeros((len(k1), nb ** 4), np.complex128)
+    P2 = np.zeros_like(P1)
+    rc = lib().oracle_eimp_model_P(nk, nb, nw, len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), _p(e),
+                                   _p(_f64(m["GGT"])), int(m["screening"]), int(m["lindblad"]),
+                                   int(m["detail_balance"]), m["prefac_A"], m["prefac_vq"], m["prefac_vq_bare"],
+                                   m["qscr2_debye"], m["fac0"], m["fac2"], m["fac4"], m["degauss"], m["ethr"],
+                                   m["omegaL"], m["temperature"], _p(P1), _p(P2))
+    if rc:
+        raise ValueError("oracle_eimp_model_P: k index out of range")
     return P1, P2
 
 
