@@ -8,7 +8,9 @@
 // What it reads: lattice (ldbd_R.dat), band energies and window, T and mu
 // (ldbd_size.dat, or mu from carrier_density as electron::set_mu_and_n),
 // spin matrices, stored pairs and P (x scale_eph), ab initio e-i P of every
-// impurity_density<i> with impMode ab_neutral (x |n_i cell_size| x scale_ei);
+// impurity_density<i> with impMode ab_neutral (x |n_i cell_size| x scale_ei)
+// or the model ionized-impurity P (impMode model_ionized, scrMode medium,
+// static unscreened / debye / Bechstedt screening) generated on the GPU;
 // the initial state is the Bpert first-order state (electron::
 // compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
@@ -24,7 +26,8 @@
 // alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
 // the valley k-pair filters and the scattering switches act as in the
 // reference.  Features whose physics this operator does not carry (e-e,
-// model e-i, pump pulses, k-resolved random g factors) stop the run with a
+// k-summed or electron-gas screening, pump pulses, k-resolved random g
+// factors) stop the run with a
 // message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
@@ -72,7 +75,15 @@ struct Settings {
     int freq_measure = 1, freq_compute_tau = 10;
     double mu = 0, carrier_density = 0, scissor = 0, temperature = 0;
     double Bpert[3] = {0, 0, 0}, B[3] = {0, 0, 0};
-    std::vector<double> ni;  // impurity densities in bohr^-dim, ab_neutral
+    std::vector<double> ni;  // impurity densities in bohr^-dim
+    std::vector<bool> ni_model;  // impMode model_ionized (else ab_neutral)
+    std::vector<double> ni_Z;    // Z_impurity
+    bool eimp_db = false;        // detailBalance of the model e-i term
+    double degauss = 0, ndegauss = 0, degauss_eimp = 0;
+    // screening (coulombParam, parameters.cpp:192-206)
+    std::string scrMode = "none", scrFormula = "RPA", scr_dynamic = "static";
+    double eps_background = 1, meff = 1;
+    bool update_screening = true;
     // k-pair filter of evolve (evolve_eph.cpp:52-58)
     bool only_intra = false, only_inter = false;
     std::vector<std::array<double, 3>> vpos;  // valley positions (crystal coordinates)
@@ -194,15 +205,55 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         s.Bpert[i] = p.get(bp[i], 0., Tesla2au);
         s.B[i] = p.get(bb[i], 0., Tesla2au);
     }
-    for (int iD = 0;; iD++) {  // parameters.cpp:210-224
+    s.degauss = p.get("degauss", info.degauss / eV, eV);  // parameters.cpp:131-140
+    s.ndegauss = p.get("ndegauss", info.ndegauss);
+    s.degauss_eimp = p.get("degauss_eimp", s.degauss / eV, eV);
+    s.eimp_db = p.get("detailBalance", 0) != 0;
+    s.scrMode = p.str("scrMode", "none");
+    s.scrFormula = p.str("scrFormula", "RPA");
+    s.scr_dynamic = p.str("dynamic_screening", "static");
+    s.eps_background = p.get("epsilon_background", 1);
+    s.meff = p.get("meff_screening", 1);
+    s.update_screening = p.get("update_screening", 1) != 0;
+    // coulombParam::check_params (Scatt_Param.h:21-49)
+    if (s.scrMode != "none" && s.scrMode != "medium") throw std::runtime_error("scrMode must be none or medium now");
+    if (s.scrFormula != "unscreened" && s.scrFormula != "debye" && s.scrFormula != "Bechstedt" &&
+        s.scrFormula != "heg" && s.scrFormula != "lindhard" && s.scrFormula != "RPA")
+        throw std::runtime_error("scrFormula must be unscreened, debye, Bechstedt, heg, lindhard or RPA now");
+    if (s.scr_dynamic != "static" &&
+        (s.scrFormula == "unscreened" || s.scrFormula == "debye" || s.scrFormula == "Bechstedt"))
+        throw std::runtime_error("debye and Bechstedt models are inn static limit");
+    if (s.scr_dynamic != "static" && s.scr_dynamic != "ppa" && s.scr_dynamic != "real-axis" && s.scr_dynamic != "model")
+        throw std::runtime_error("dynamic must be static or ppa or real-axis or model now");
+    if (s.meff <= 0) throw std::runtime_error("meff must > 0");
+    if (s.eps_background < 1) throw std::runtime_error("epsilon_background < 1 is not right");
+    for (int iD = 0;; iD++) {  // parameters.cpp:210-224,466-478
         const std::string sfx = iD == 0 ? "" : std::to_string(iD + 1);
         const double n = p.get("impurity_density" + sfx, 0, std::pow(bohr2cm, dim));
         if (n == 0) break;
         const std::string mode = p.str("impMode" + sfx, "model_ionized");
-        if (mode != "ab_neutral")
-            throw std::runtime_error("impMode" + sfx + " = " + mode + ": only ab initio e-i P (ab_neutral) is read");
+        if (mode != "model_ionized" && mode != "ab_neutral")
+            throw std::runtime_error("impMode must be model_ionized or ab_neutral now");
+        const double Z = p.get("Z_impurity" + sfx, 1);
+        if (mode == "model_ionized") {
+            if (s.scrMode == "none")
+                throw std::runtime_error("scrMode should not be none if considering model electron-ionized-impurity scattering");
+            if (s.scrFormula != "unscreened" && s.scrFormula != "debye" && s.scrFormula != "Bechstedt")
+                throw std::runtime_error("scrFormula = " + s.scrFormula +
+                                         ": the model e-i term is built for the static unscreened, debye and Bechstedt "
+                                         "screening (the k-summed RPA/lindhard and the electron-gas forms are not)");
+            // occ_of_impurity_level reads lng, which parameters.cpp never fills (Scatt_Param.h:88-93)
+            refuse(p, "partial_ionized" + sfx, 0, "partially ionized impurities");
+            if (n < 0) throw std::runtime_error("eip.ni_ionized[iD] must be postive");  // ElecImp_Model.h:33
+        }
+        if (Z <= 0) throw std::runtime_error("for ionized impurity, Z is defined as a positive value in this code");
+        (void)p.get("E_impurity" + sfx, 0.);  // read by the reference for partial ionization only
+        (void)p.get("g_impurity" + sfx, 2.);
         s.ni.push_back(n);
+        s.ni_model.push_back(mode == "model_ionized");
+        s.ni_Z.push_back(Z);
     }
+    (void)p.get("freq_update_eimp_model", 0);  // updates only during a pump pulse (DensityMatrix_Dynamics.h:301-318)
     // parameters.cpp:459-464,485-486,493-497
     if (s.only_eimp && s.ni.empty()) throw std::runtime_error("alg_only_eimp is only possible if impurity_density is non-zero");
     if ((!s.ni.empty() || s.eph_enable) && !s.scatt_enable)
@@ -211,8 +262,14 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         throw std::runtime_error("alg_scatt_enable must be false if no scattering mechanism is considered");
     if (s.eph_enable && s.only_eimp)
         throw std::runtime_error("if only e-imp or only e-e, e-ph should not be enabled. Set alg_eph_enable = 0");
-    if (s.sparseP && !s.ni.empty()) throw std::runtime_error("ab initio neutral e-imp does not support sparse matrices");
-    if (s.Pin_is_sparse && !s.eph_enable && s.ni.empty())
+    bool any_ab = false;
+    for (size_t iD = 0; iD < s.ni.size(); iD++) {
+        if (!s.ni_model[iD]) any_ab = true;
+        if (!s.ni_model[iD] && s.sparseP) throw std::runtime_error("ab initio neutral e-imp does not support sparse matrices");
+        if (s.carrier_density * s.ni[iD] < 0)
+            throw std::runtime_error("currently carrier_denstiy and impurity_density must have the same sign");
+    }
+    if (s.Pin_is_sparse && !s.eph_enable && !any_ab)
         throw std::runtime_error("alg_Pin_is_sparse implies ab initio e-ph or e-i is enabled");
     s.t0 = p.get("t0", 0., fs());
     s.tend = p.get("tend", 0., fs());
@@ -285,6 +342,22 @@ double mu_conserving(const std::vector<cplx> &dm, const std::vector<double> &edm
     return 0.5 * (lo + hi);
 }
 
+// GGT = G G^T with G = 2 pi R^-1, R's rows the lattice vectors (lattice.h:40-41)
+void reciprocal_metric(const double R[9], double GGT[9])
+{
+    const double a = R[0], b = R[1], c = R[2], d = R[3], e = R[4], f = R[5], g = R[6], h = R[7], i = R[8];
+    const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    const double inv[3][3] = {{(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det},
+                              {(f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det},
+                              {(d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det}};
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) {
+            double acc = 0;
+            for (int m = 0; m < 3; m++) acc += inv[r][m] * inv[q][m];
+            GGT[r * 3 + q] = 4 * M_PI * M_PI * acc;
+        }
+}
+
 // The k-pair filter of evolve (evolve_eph.cpp:52-58, lattice.h:82-139):
 // lengths under the reciprocal metric GGT = G G^T, G = 2 pi R^-1
 // (lattice.h:40-41; matrix3.h:119-122), of differences wrapped into
@@ -294,17 +367,10 @@ struct KpairFilter {
     const Settings &s;
     KpairFilter(const double R[9], const Settings &set) : s(set)
     {
-        const double a = R[0], b = R[1], c = R[2], d = R[3], e = R[4], f = R[5], g = R[6], h = R[7], i = R[8];
-        const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
-        const double inv[3][3] = {{(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det},
-                                  {(f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det},
-                                  {(d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det}};
+        double G[9];
+        reciprocal_metric(R, G);
         for (int r = 0; r < 3; r++)
-            for (int q = 0; q < 3; q++) {
-                double acc = 0;
-                for (int m = 0; m < 3; m++) acc += inv[r][m] * inv[q][m];
-                M[r][q] = 4 * M_PI * M_PI * acc;
-            }
+            for (int q = 0; q < 3; q++) M[r][q] = G[r * 3 + q];
     }
     double lsq(const double *k, const double *c) const
     {
@@ -425,6 +491,13 @@ int run(const std::string &folder, bool dry)
                 s.thr_sparseP);
     std::printf("alg_scatt_enable = %d alg_eph_enable = %d alg_only_eimp = %d alg_semiclassical = %d\n",
                 (int)s.scatt_enable, (int)s.eph_enable, (int)s.only_eimp, (int)s.semiclassical);
+    {
+        int n_model = 0;
+        for (bool mdl : s.ni_model) n_model += mdl;
+        std::printf("scrMode = %s scrFormula = %s epsilon_background = %lg model_impurities = %d detailBalance = %d "
+                    "degauss_eimp = %.10le\n", s.scrMode.c_str(), s.scrFormula.c_str(), s.eps_background, n_model,
+                    (int)s.eimp_db, s.degauss_eimp);
+    }
     std::printf("alg_read_Bso = %d needL = %d scale_Ez = %lg alg_set_scv_zero = %d\n", (int)s.read_Bso, (int)s.needL,
                 s.scale_Ez, (int)s.set_scv_zero);
     std::printf("alg_phenom_relax = %d tau_phenom = %.10le bStart_tau = %d bEnd_tau = %d "
@@ -505,6 +578,17 @@ int run(const std::string &folder, bool dry)
                 }
     }
 
+    // dm_eq (DensityMatrix_Dynamics.h:57-67): the carrier-conserving f_eq,
+    // set only when scattering is on (dm_eq stays 0 otherwise)
+    std::vector<double> feq((size_t)nk * nbd, 0.0);
+    double mue = mu, muh = mu;
+    if ((s.ddmdteq || s.linearize || s.phenom) && s.scatt_enable) {
+        mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
+        muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;
+        for (long k = 0; k < nk; k++)
+            for (int i = 0; i < nbd; i++) feq[(size_t)k * nbd + i] = fermi(T, i >= nvd ? mue : muh, edm[(size_t)k * nbd + i]);
+    }
+
     // the scattering operator: e-ph P, ab initio e-i P, flags, ddmdt_eq, H_BS
     using Teph = electronphonon_mi355x<cplx>;
     const Teph::p_source src = !s.scatt_enable ? Teph::P_NO_SCATTERING
@@ -512,7 +596,80 @@ int run(const std::string &folder, bool dry)
                                : s.Pin_is_sparse ? Teph::P_SPARSE : Teph::P_DENSE;
     Teph eph(ldbd, s.scatt, s.scale_eph, s.scissor, DMD_PREC_FP64, src, s.thr_sparseP);
     eph.Pin_is_sparse = s.Pin_is_sparse;  // e-i folded into sparse input is re-sparsified
-    for (size_t iD = 0; iD < s.ni.size(); iD++) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+    // coulomb_model (Coulomb_Model.h:412-425,533-575), built when scrMode = medium:
+    // 3D only; the free-carrier density n_free = |n_e| + |n_h| at mu over every
+    // band (electron.h:153-157, electron.cpp:578-596), or, when the linearized
+    // operator is built with update_screening, from f_eq over the e-ph bands
+    // (ElectronPhonon_linear.cpp:11, coulomb_model::init)
+    double n_free = 0;
+    if (s.scrMode == "medium" && s.scatt_enable) {
+        if (dim < 3) throw std::runtime_error("debye screening model for lowD is not implemented");
+        if (s.linearize && s.update_screening) {
+            for (long k = 0; k < nk; k++)
+                for (int b = info.b_bot_eph - info.b_bot_dm; b < info.b_top_eph - info.b_bot_dm; b++)
+                    n_free += b < nvd ? 1 - feq[(size_t)k * nbd + b] : feq[(size_t)k * nbd + b];
+            n_free /= info.nk_full * cell;
+        } else {
+            double ne = 0, nh = 0;
+            for (long k = 0; k < nk; k++)
+                for (int b = 0; b < nbf; b++) {
+                    const double f = fermi(T, mu, e[(size_t)k * nbf + b]);
+                    if (b >= info.nv) ne += f;
+                    else nh += f - 1.;
+                }
+            n_free = std::fabs(ne / (info.nk_full * cell)) + std::fabs(nh / (info.nk_full * cell));
+        }
+        if ((s.scrFormula == "debye" || s.scrFormula == "Bechstedt" || s.scrFormula == "heg") && n_free <= 0)
+            throw std::runtime_error("nfreetot must be postive");
+        std::printf("screening: %s, epsilon_background = %lg, nfree = %lg cm-3\n", s.scrFormula.c_str(), s.eps_background,
+                    n_free / std::pow(bohr2cm, 3));
+    }
+    std::vector<uint64_t> mk1, mk2;
+    std::vector<cplx> mU;
+    std::vector<double> mkv;
+    for (size_t iD = 0; iD < s.ni.size(); iD++) {
+        if (!s.ni_model[iD]) {
+            eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+            continue;
+        }
+        if (mU.empty()) {  // electron::read_ldbd_Umat (electron.cpp:388-410), kvec, stored pairs
+            struct stat us;
+            if (stat((ldbd + "/ldbd_Umat.bin").c_str(), &us) != 0)
+                throw std::runtime_error("impMode = model_ionized needs ldbd_data/ldbd_Umat.bin (the overlaps <k|k'>)");
+            const int nbe = info.b_top_eph - info.b_bot_eph;
+            mU.resize((size_t)nk * info.nb_wannier * nbe);
+            check(dmd_ldbd_read_Umat(ldbd.c_str(), &info, (double *)mU.data()), "ldbd_Umat.bin");
+            mkv.resize((size_t)nk * 3);
+            check(dmd_ldbd_read_kvec(ldbd.c_str(), &info, mkv.data()), "ldbd_kvec.bin");
+            mk1.resize(info.nkpair);
+            mk2.resize(info.nkpair);
+            check(dmd_ldbd_read_kpairs(ldbd.c_str(), &info, 0, info.nkpair, mk1.data(), mk2.data()), "k pairs");
+        }
+        // elecimp_model (ElecImp_Model.h:22-40) with coulomb_model::init_model (Coulomb_Model.h:554-575)
+        dmd_eimp_model m{};
+        m.screening = s.scrFormula == "debye" ? DMD_SCR_DEBYE : s.scrFormula == "Bechstedt" ? DMD_SCR_BECHSTEDT
+                                                                                               : DMD_SCR_UNSCREENED;
+        m.lindblad = s.scatt == "lindblad";
+        m.detail_balance = s.eimp_db;
+        m.nb_wannier = info.nb_wannier;
+        m.prefac_A = s.ni_Z[iD] * std::sqrt(s.ni[iD] * cell);
+        m.prefac_vq = 4 * M_PI / s.eps_background / cell;
+        m.prefac_vq_bare = 4 * M_PI / cell;
+        const double kF = std::pow(3 * M_PI * M_PI * n_free, 1. / 3.), EF = kF * kF / 2. / s.meff;
+        const double qTF2 = 6 * M_PI * n_free / s.eps_background / EF;
+        m.qscr2_debye = 4 * M_PI * n_free / s.eps_background / T;
+        m.fac0 = 1. / (s.eps_background - 1);
+        m.fac2 = 1. / qTF2;
+        m.fac4 = 3. / 4. / (kF * kF) / qTF2;
+        m.degauss = s.degauss_eimp;
+        m.ethr = s.degauss * s.ndegauss;
+        m.omegaL = 1e-4 * T;
+        m.temperature = T;
+        reciprocal_metric(R, m.GGT);
+        std::printf("model e-i %zu: Z = %lg n_i = %lg cm-3 qscr2_debye = %lg degauss = %lg detailBalance = %d\n", iD + 1,
+                    s.ni_Z[iD], s.ni[iD] / std::pow(bohr2cm, 3), m.qscr2_debye, m.degauss, (int)m.detail_balance);
+        eph.add_eimp_model((int)iD, mk1.data(), mk2.data(), mU.data(), mkv.data(), m, s.scale_ei);
+    }
     // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
     // alg_linearize set_eph returns before it (:6)
     if (s.sparseP && !s.linearize && !s.Pin_is_sparse) {
@@ -538,14 +695,7 @@ int run(const std::string &folder, bool dry)
     }
     eph.set_algorithm(s.expt, !interaction);
     eph.set_semiclassical(s.semiclassical);
-    // dm_eq (DensityMatrix_Dynamics.h:57-67): the carrier-conserving f_eq,
-    // set only when scattering is on (dm_eq stays 0 otherwise)
-    std::vector<double> feq((size_t)nk * nbd, 0.0);
     if ((s.ddmdteq || s.linearize || s.phenom) && s.scatt_enable) {
-        const double mue = nbd > nvd ? mu_conserving(dm, edm, nk, nbd, nvd, nbd, false, T) : mu;
-        const double muh = nvd > 0 ? mu_conserving(dm, edm, nk, nbd, 0, nvd, true, T) : mu;
-        for (long k = 0; k < nk; k++)
-            for (int i = 0; i < nbd; i++) feq[(size_t)k * nbd + i] = fermi(T, i >= nvd ? mue : muh, edm[(size_t)k * nbd + i]);
         double *rows[1] = {feq.data()};
         if (s.linearize) {
             eph.set_Lsc(rows);  // ddmdt_eq of the linearized operator is zero: evolve_linear at f_eq

@@ -98,6 +98,23 @@ public:
         for (auto &s : eimp_slots) eimp.push_back(&s);
     }
 
+    // add_scatt_contrib("eimp", iD) with impMode = model_ionized
+    // (ElectronPhonon.cpp:44-113, elecimp_model::calc_P, ElecImp_Model.h:185-213):
+    // P += scale_ei * P_imp, P_imp generated on the device from the overlaps
+    // U_k^H U_k' (U: nk x nb_wannier x nb of this window), the k points and the
+    // screened Coulomb model m; re-sparsified under Pin_is_sparse as above.
+    void add_eimp_model(int iD, const uint64_t *k1st, const uint64_t *k2nd, const complex_t *U, const double *kvec,
+                        const dmd_eimp_model &m, double scale_ei)
+    {
+        check(dmd_scatt_add_eimp_model(h_, k1st, k2nd, (const double *)U, kvec, &m, scale_ei, P_set_ ? 1 : 0),
+              "model e-i P of defect " + std::to_string(iD + 1));
+        P_set_ = true;
+        if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
+        if ((int)eimp_slots.size() <= iD) eimp_slots.resize(iD + 1);
+        eimp.clear();
+        for (auto &s : eimp_slots) eimp.push_back(&s);
+    }
+
     // set_sparseP(true) at the end of set_eph (ElectronPhonon.cpp:12,115-129)
     // under alg_sparseP: P with |P| <= thr_sparseP zeroed and, with
     // write_files, the sparse sP files written next to ldbd_data's P files

@@ -100,7 +100,14 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
 @pytest.mark.parametrize("line,msg", [
     ("eeMode = Pee_update", "e-e scattering"),
     ("alg_linearize_dPee = 1", "linearized e-e operator"),
-    ("impurity_density = 1e17\nimpMode = model_ionized", "ab_neutral"),
+    ("impurity_density = 1e17\nimpMode = model_ionized", "scrMode should not be none"),
+    ("impurity_density = 1e17\nscrMode = medium", "built for the static unscreened, debye and Bechstedt"),
+    ("impurity_density = 1e17\nscrMode = medium\nscrFormula = debye\npartial_ionized = 1", "partially ionized"),
+    ("impurity_density = 1e17\nscrMode = medium\nscrFormula = debye\nZ_impurity = 0", "Z is defined as a positive"),
+    ("impurity_density = 1e17\nimpMode = other", "impMode must be model_ionized or ab_neutral"),
+    ("scrMode = full", "scrMode must be none or medium"),
+    ("scrMode = medium\nscrFormula = debye\ndynamic_screening = ppa", "static limit"),
+    ("impurity_density = -1e17\nimpMode = ab_neutral\ncarrier_density = 1e17", "same sign"),
     ("alg_eph_need_hole = 1\nalg_eph_need_elec = 0", "file set"),
     ("pumpA0 = 0.002", "pump"),
     ("alg_ode_method = rk2", "rkf45, euler or rk4"),
@@ -206,4 +213,18 @@ def test_dry_run_coherent_hamiltonian_inputs(tmp_path):
                                                    "scale_Ez = 0.25\nalg_set_scv_zero = 1\n"))
     assert rc == 0, err
     assert "alg_read_Bso = 1 needL = 1 scale_Ez = 0.25 alg_set_scv_zero = 1" in out
+    assert "not read by this runner" not in out
+
+
+def test_dry_run_model_ionized_impurity(tmp_path):
+    """impMode = model_ionized (the default) with scrMode = medium and a
+    |q|-only screening formula is accepted, with the reference's defaults
+    (epsilon_background 1, detailBalance 0, degauss_eimp = degauss of
+    ldbd_size.dat); it combines with alg_sparseP, which ab_neutral does not."""
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\nimpurity_density = 1e17\nscrMode = medium\n"
+                                               "scrFormula = Bechstedt\nepsilon_background = 12.9\nalg_sparseP = 1\n"))
+    assert rc == 0, err
+    assert "scrMode = medium scrFormula = Bechstedt epsilon_background = 12.9 model_impurities = 1" in out
+    degauss = float(open(os.path.join(GOLD, "ldbd_size.dat")).read().splitlines()[7].split()[0])
+    assert _num(out, "degauss_eimp") == pytest.approx(degauss, rel=1e-12)
     assert "not read by this runner" not in out

@@ -623,3 +623,94 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
     head = (d / "lz_elec_tot.out").read_text().splitlines()[0]
     assert head == "#time(au), l(t), l(t) without exp(iwt)"
     assert not (d / "tau_lz_elec_tot.out").exists()
+
+
+@pytest.mark.parametrize("linear", [0, 1])
+def test_dmd_run_model_ionized_impurity(tmp_path, linear):
+    """impMode = model_ionized with Debye screening: P = P_eph + scale_ei P_imp,
+    P_imp from the full ldbd_Umat.bin (nb_wannier = 5, e-ph columns from
+    b_bot_eph + b_skip_wannier = 1), the k points and n_free = |n_e| + |n_h| at
+    mu over every band (from f_eq over the e-ph bands when the linearized
+    operator is built with update_screening), against the oracle's P_imp and
+    RK4 (of evolve, or of evolve_linear at f_eq), S_z per report."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print(f"case dmd_run model e-i linear={linear}", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    lines = (L / "ldbd_size.dat").read_text().split("\n")
+    f2 = lines[1].split("#")[0].split()
+    f2[6], f2[7] = "5", "1"  # nb_wannier, bands skipped below the Wannier set
+    lines[1] = " ".join(f2) + " # nb nv bBot_dm bTop_dm bBot_eph bTop_eph nb_wannier bskipped_wannier band_dft_skipped"
+    (L / "ldbd_size.dat").write_text("\n".join(lines))
+    info = ldbd.read_info(str(L))
+    assert info.nb_wannier == 5 and info.b_skip_wannier == 1
+    nk = info.nk
+    rng = np.random.default_rng(21)
+    Ufull = (rng.normal(size=(nk, 5, 5)) + 1j * rng.normal(size=(nk, 5, 5))) / np.sqrt(5)
+    Ufull.tofile(L / "ldbd_Umat.bin")
+    U = ldbd.read_Umat(str(L), info)
+    assert np.array_equal(U, Ufull[:, :, 1:3])
+    e = ldbd.read_ek(str(L), info)
+    kv = ldbd.read_kvec(str(L), info)
+    k1, k2 = ldbd.read_kpairs(str(L), info)
+    P1, P2 = ldbd.read_P(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    V = abs(np.linalg.det(R))
+    nb = e.shape[1]
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    n_free = (feq.sum() if linear else f.sum()) / (info.nk_full * V)  # nv = 0: electrons only
+    eps, Z = 4.0, 1.5
+
+    def model(n_i):
+        return scatt.eimp_model("debye", n_free, T, V, scatt.reciprocal_metric(R), 5, Z=Z, n_ionized=n_i,
+                                epsilon_background=eps, degauss=info.degauss, ethr=info.degauss * info.ndegauss)
+    R1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(1e-6).as_dict())
+    n_i = 1e-6 * np.abs(P1).max() / np.abs(R1).max()  # e-i as strong as e-ph
+    scale_ei = 0.8
+    M1, M2 = oracle.eimp_model_P(k1, k2, U, kv, e, model(n_i).as_dict())
+    S1, S2 = P1 + scale_ei * M1, P2 + scale_ei * M2
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n"
+                                f"alg_linearize = {linear}\nscale_ei = {scale_ei}\n"
+                                f"impurity_density = {n_i / BOHR2CM ** 3:.17g}\nimpMode = model_ionized\n"
+                                f"Z_impurity = {Z}\nscrMode = medium\nscrFormula = debye\nepsilon_background = {eps}\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1200:], r.stderr[-800:])
+    assert r.returncode == 0 and "model e-i 1:" in r.stdout
+    prefac = 2 * np.pi / info.nk_full
+    obs_prefac = 1.0 / (info.nk_full * V * BOHR2CM ** 3)
+    if linear:
+        Lij, Lji = oracle.set_Lsc(k1, k2, S1, S2, feq)
+        g = lambda tt, yy: oracle.evolve_linear(k1, k2, Lij, Lji, e, yy, feq, t=tt, prefac=prefac)
+    else:
+        g = lambda tt, yy: oracle.evolve(k1, k2, S1, S2, e, yy, t=tt, prefac=prefac)
+    ref = [oracle.spin(s3[:, 2], e, f, rho, 0.0, 0, nb, obs_prefac)]
+    y, dt = rho.copy(), 1 * FS
+    for i in range(4):
+        t = i * dt
+        a1 = g(t, y)
+        a2 = g(t + dt / 2, y + dt / 2 * a1)
+        a3 = g(t + dt / 2, y + dt / 2 * a2)
+        a4 = g(t + dt, y + dt * a3)
+        y = y + dt / 6 * (a1 + 2 * a2 + 2 * a3 + a4)
+        yy = y.reshape(nk, nb, nb)
+        y = (0.5 * (yy + yy.conj().transpose(0, 2, 1))).reshape(nk, -1)
+        ref.append(oracle.spin(s3[:, 2], e, f, y, (i + 1) * dt, 0, nb, obs_prefac))
+    ref = np.array(ref)
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", ref)
+    assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
+    # without the e-i term the run differs well beyond that
+    y0 = rho.copy()
+    a = oracle.evolve(k1, k2, P1, P2, e, y0, prefac=prefac)
+    b = oracle.evolve(k1, k2, S1, S2, e, y0, prefac=prefac)
+    assert np.abs(a - b).max() > 0.1 * np.abs(b).max()
