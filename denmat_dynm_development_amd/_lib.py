@@ -25,6 +25,8 @@ DMD_PREC_BF16 = 2
 DMD_SCR_UNSCREENED = 0
 DMD_SCR_DEBYE = 1
 DMD_SCR_BECHSTEDT = 2
+DMD_SCR_RPA = 3
+DMD_SCR_LINDHARD = 4
 
 DMD_KERNEL_AUTO = 0
 DMD_KERNEL_ROWS = 1
@@ -124,11 +126,12 @@ class EimpModel(ctypes.Structure):
                 ("prefac_vq_bare", ctypes.c_double), ("qscr2_debye", ctypes.c_double), ("fac0", ctypes.c_double),
                 ("fac2", ctypes.c_double), ("fac4", ctypes.c_double), ("degauss", ctypes.c_double),
                 ("ethr", ctypes.c_double), ("omegaL", ctypes.c_double), ("temperature", ctypes.c_double),
-                ("GGT", ctypes.c_double * 9)]
+                ("GGT", ctypes.c_double * 9), ("kmesh", ctypes.c_int * 3), ("nk_full", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}
         d["GGT"] = list(self.GGT)
+        d["kmesh"] = list(self.kmesh)
         return d
 
 
@@ -160,7 +163,7 @@ PROTOTYPES = {
     "dmd_scatt_upload_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_add_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_download_P": [_vp, _i64, _i64, _vp, _vp],
-    "dmd_scatt_add_eimp_model": [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
+    "dmd_scatt_add_eimp_model": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
     "dmd_scatt_generate_P": [_vp, _pSP],
     "dmd_scatt_set_flags": [_vp, _i, _i, _i],
     "dmd_scatt_set_ddmdt_eq": [_vp, _vp],

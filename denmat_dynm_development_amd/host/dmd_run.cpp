@@ -10,7 +10,7 @@
 // spin matrices, stored pairs and P (x scale_eph), ab initio e-i P of every
 // impurity_density<i> with impMode ab_neutral (x |n_i cell_size| x scale_ei)
 // or the model ionized-impurity P (impMode model_ionized, scrMode medium,
-// static unscreened / debye / Bechstedt screening) generated on the GPU;
+// static RPA / unscreened / debye / Bechstedt screening) generated on the GPU;
 // the initial state is the Bpert first-order state (electron::
 // compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
@@ -26,9 +26,8 @@
 // alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
 // the valley k-pair filters and the scattering switches act as in the
 // reference.  Features whose physics this operator does not carry (e-e,
-// k-summed or electron-gas screening, pump pulses, k-resolved random g
-// factors) stop the run with a
-// message rather than being ignored.
+// lindhard or electron-gas screening, pump pulses, k-resolved random g
+// factors) stop the run with a message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
 #include <algorithm>
@@ -238,10 +237,12 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         if (mode == "model_ionized") {
             if (s.scrMode == "none")
                 throw std::runtime_error("scrMode should not be none if considering model electron-ionized-impurity scattering");
-            if (s.scrFormula != "unscreened" && s.scrFormula != "debye" && s.scrFormula != "Bechstedt")
+            // lindhard: coulomb_model fills vq_RPA for RPA only (Coulomb_Model.h:531), so vq reads an
+            // empty table; heg: frequency integrals of the electron gas
+            if (s.scrFormula == "lindhard" || s.scrFormula == "heg")
                 throw std::runtime_error("scrFormula = " + s.scrFormula +
-                                         ": the model e-i term is built for the static unscreened, debye and Bechstedt "
-                                         "screening (the k-summed RPA/lindhard and the electron-gas forms are not)");
+                                         ": the model e-i term is built for the static RPA, unscreened, debye and "
+                                         "Bechstedt screening");
             // occ_of_impurity_level reads lng, which parameters.cpp never fills (Scatt_Param.h:88-93)
             refuse(p, "partial_ionized" + sfx, 0, "partially ionized impurities");
             if (n < 0) throw std::runtime_error("eip.ni_ionized[iD] must be postive");  // ElecImp_Model.h:33
@@ -648,7 +649,9 @@ int run(const std::string &folder, bool dry)
         // elecimp_model (ElecImp_Model.h:22-40) with coulomb_model::init_model (Coulomb_Model.h:554-575)
         dmd_eimp_model m{};
         m.screening = s.scrFormula == "debye" ? DMD_SCR_DEBYE : s.scrFormula == "Bechstedt" ? DMD_SCR_BECHSTEDT
-                                                                                               : DMD_SCR_UNSCREENED;
+                      : s.scrFormula == "RPA" ? DMD_SCR_RPA : DMD_SCR_UNSCREENED;
+        for (int d = 0; d < 3; d++) m.kmesh[d] = info.kmesh[d];
+        m.nk_full = info.nk_full;
         m.lindblad = s.scatt == "lindblad";
         m.detail_balance = s.eimp_db;
         m.nb_wannier = info.nb_wannier;
@@ -668,7 +671,17 @@ int run(const std::string &folder, bool dry)
         reciprocal_metric(R, m.GGT);
         std::printf("model e-i %zu: Z = %lg n_i = %lg cm-3 qscr2_debye = %lg degauss = %lg detailBalance = %d\n", iD + 1,
                     s.ni_Z[iD], s.ni[iD] / std::pow(bohr2cm, 3), m.qscr2_debye, m.degauss, (int)m.detail_balance);
-        eph.add_eimp_model((int)iD, mk1.data(), mk2.data(), mU.data(), mkv.data(), m, s.scale_ei);
+        // RPA: occupations of the e-ph bands at mu (coulomb_model's f, Coulomb_Model.h:428; init_RPA
+        // leaves them when the screening is updated, :587-590)
+        std::vector<double> feph;
+        if (m.screening == DMD_SCR_RPA) {
+            const int b0 = info.b_bot_eph - info.b_bot_dm, nbe = info.b_top_eph - info.b_bot_eph;
+            feph.resize((size_t)nk * nbe);
+            for (long k = 0; k < nk; k++)
+                for (int b = 0; b < nbe; b++) feph[(size_t)k * nbe + b] = fdm[(size_t)k * nbd + b0 + b];
+        }
+        eph.add_eimp_model((int)iD, mk1.data(), mk2.data(), mU.data(), mkv.data(), feph.empty() ? nullptr : feph.data(), m,
+                           s.scale_ei);
     }
     // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
     // alg_linearize set_eph returns before it (:6)

@@ -103,10 +103,11 @@ public:
     // P += scale_ei * P_imp, P_imp generated on the device from the overlaps
     // U_k^H U_k' (U: nk x nb_wannier x nb of this window), the k points and the
     // screened Coulomb model m; re-sparsified under Pin_is_sparse as above.
+    // f: nk x nb occupations of this window at mu, for the RPA / lindhard screening (else null)
     void add_eimp_model(int iD, const uint64_t *k1st, const uint64_t *k2nd, const complex_t *U, const double *kvec,
-                        const dmd_eimp_model &m, double scale_ei)
+                        const double *f, const dmd_eimp_model &m, double scale_ei)
     {
-        check(dmd_scatt_add_eimp_model(h_, k1st, k2nd, (const double *)U, kvec, &m, scale_ei, P_set_ ? 1 : 0),
+        check(dmd_scatt_add_eimp_model(h_, k1st, k2nd, (const double *)U, kvec, f, &m, scale_ei, P_set_ ? 1 : 0),
               "model e-i P of defect " + std::to_string(iD + 1));
         P_set_ = true;
         if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");

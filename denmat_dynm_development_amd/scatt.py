@@ -190,13 +190,13 @@ class ScatteringOperator:
     def generate_P(self, params):
         call("dmd_scatt_generate_P", self._h, ctypes.byref(params))
 
-    def add_eimp_model(self, k1st, k2nd, U, kvec, model, scale=1.0, add=True):
+    def add_eimp_model(self, k1st, k2nd, U, kvec, model, scale=1.0, add=True, f=None):
         """Model ionized-impurity P of every stored pair generated on the device
         and added (add) or set: elecimp_model::calc_P through
         add_scatt_contrib("eimp") (ElecImp_Model.h:185-213, ElectronPhonon.cpp:
         44-113), scale = scale_ei.  U: (nk, nb_wannier, nb) complex (the e-ph
         columns of ldbd_Umat.bin), kvec: (nk, 3), model: an EimpModel
-        (eimp_model())."""
+        (eimp_model()); f: (nk, nb) occupations at mu, for the RPA screening."""
         k1 = np.ascontiguousarray(k1st, dtype=np.uint64)
         k2 = np.ascontiguousarray(k2nd, dtype=np.uint64)
         U = np.ascontiguousarray(U, dtype=np.complex128)
@@ -205,7 +205,13 @@ class ScatteringOperator:
             raise ValueError("k1st/k2nd need one entry per stored pair")
         if U.shape != (self.nk, model.nb_wannier, self.nb) or kv.shape != (self.nk, 3):
             raise ValueError(f"U must be (nk, nb_wannier, nb) = {(self.nk, model.nb_wannier, self.nb)} and kvec (nk, 3)")
-        call("dmd_scatt_add_eimp_model", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+        fp = None
+        if f is not None:
+            f = np.ascontiguousarray(f, dtype=np.float64)
+            if f.shape != (self.nk, self.nb):
+                raise ValueError("f must be (nk, nb)")
+            fp = f.ctypes.data
+        call("dmd_scatt_add_eimp_model", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data, fp,
              ctypes.byref(model), float(scale), int(bool(add)))
 
     # -- flags / eq ----------------------------------------------------------------
@@ -430,12 +436,12 @@ class ElectronPhonon(ScatteringOperator):
 
 
 SCREENING = {"unscreened": _lib.DMD_SCR_UNSCREENED, "debye": _lib.DMD_SCR_DEBYE,
-             "Bechstedt": _lib.DMD_SCR_BECHSTEDT}
+             "Bechstedt": _lib.DMD_SCR_BECHSTEDT, "RPA": _lib.DMD_SCR_RPA}
 
 
 def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0, n_ionized=None,
                epsilon_background=1.0, meff=1.0, degauss=None, ethr=None, scatt="lindblad",
-               detail_balance=False):
+               detail_balance=False, kmesh=(0, 0, 0), nk_full=0.0):
     """The dmd_eimp_model of one ionized impurity (atomic units throughout):
     elecimp_model's prefactors (ElecImp_Model.h:22-40: prefac_A = Z sqrt(n_i
     V_cell), omegaL = 1e-4 T) and coulomb_model's static screening
@@ -443,10 +449,12 @@ def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0
     q^2 = 4 pi n_free / (eps T); Thomas-Fermi q^2 = 6 pi n_free / (eps E_F)
     with k_F = (3 pi^2 n_free)^(1/3), E_F = k_F^2 / 2 meff for Bechstedt).
     n_free and n_ionized are in bohr^-3; degauss is the delta width of the e-i
-    term, ethr the |de| cut-off (degauss x ndegauss of the e-ph set)."""
+    term, ethr the |de| cut-off (degauss x ndegauss of the e-ph set).  RPA sums
+    over the k mesh (kmesh, nk_full; Coulomb_Model.h:690-744) at the
+    occupations given to add_eimp_model."""
     if screening not in SCREENING:
-        raise ValueError(f"screening {screening!r}: one of {sorted(SCREENING)} (the |q|-only static forms)")
-    if screening != "unscreened" and not n_free > 0:
+        raise ValueError(f"screening {screening!r}: one of {sorted(SCREENING)} (the static forms)")
+    if screening in ("debye", "Bechstedt") and not n_free > 0:
         raise ValueError("nfreetot must be postive")
     n_ionized = n_free if n_ionized is None else n_ionized
     if not n_ionized > 0:
@@ -476,6 +484,9 @@ def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0
     m.temperature = float(temperature)
     for i, v in enumerate(np.asarray(GGT, dtype=np.float64).reshape(9)):
         m.GGT[i] = v
+    for i in range(3):
+        m.kmesh[i] = int(kmesh[i])
+    m.nk_full = float(nk_full)
     return m
 
 

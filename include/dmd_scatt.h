@@ -102,10 +102,12 @@ int dmd_scatt_upload_P(dmd_scatt *h, const double *P1, const double *P2, double 
 int dmd_scatt_add_P(dmd_scatt *h, const double *P1, const double *P2, double scale);
 /* Model ionized-impurity scattering (impMode = model_ionized): the screened
  * Coulomb matrix element of elecimp_model (scattering_model/ElecImp_Model.h:
- * 67-137) with the static screening formulas of coulomb_model::vq
- * (Coulomb_Model.h:632-662) that depend on |q| alone. */
+ * 67-137) with the static screening of coulomb_model::vq (Coulomb_Model.h:
+ * 632-662): the closed forms in |q| (unscreened, debye, Bechstedt) and the
+ * static RPA / lindhard sums over the k mesh (calc_qscr2_static_RPA,
+ * :690-744, and calc_vq_RPA, :820-828). */
 typedef struct {
-    int screening;        /* DMD_SCR_UNSCREENED, DMD_SCR_DEBYE or DMD_SCR_BECHSTEDT    */
+    int screening;        /* DMD_SCR_UNSCREENED, _DEBYE, _BECHSTEDT, _RPA or _LINDHARD   */
     int lindblad;         /* 1: alg_scatt = lindblad (A1 = A2), 0: conventional          */
     int detail_balance;   /* detailBalance: emission/absorption at +-omegaL              */
     int nb_wannier;       /* rows of each U_k                                            */
@@ -119,10 +121,14 @@ typedef struct {
     double omegaL;        /* 1e-4 T (detail_balance only)                                */
     double temperature;
     double GGT[9];        /* reciprocal metric (2 pi)^2 R^-1 R^-T                        */
+    int kmesh[3];         /* k mesh of the k points (RPA and lindhard)                   */
+    double nk_full;       /* k points of the full mesh (RPA and lindhard)                */
 } dmd_eimp_model;
 #define DMD_SCR_UNSCREENED 0
 #define DMD_SCR_DEBYE 1
 #define DMD_SCR_BECHSTEDT 2
+#define DMD_SCR_RPA 3
+#define DMD_SCR_LINDHARD 4
 /* P += scale * P_imp (add = 1) or P = scale * P_imp (add = 0) with P_imp of
  * every stored pair generated on the device: elecimp_model::calc_P
  * (ElecImp_Model.h:185-213) called from add_scatt_contrib("eimp")
@@ -130,9 +136,11 @@ typedef struct {
  * pairs in file order (host); U is nk x nb_wannier x nb complex128, the
  * e-ph band columns of ldbd_Umat.bin (dmd_ldbd_read_Umat); kvec nk x 3.
  * The overlap <k|k'> = U_k^H U_k', the screened v(q), the Gaussian delta
- * and the products A1 x conj(A2) are formed per element in one pass. */
+ * and the products A1 x conj(A2) are formed per element in one pass.  f
+ * (nk x nb occupations of the e-ph bands at mu) enters the RPA and lindhard
+ * screening only and may be null otherwise. */
 int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
-                             const double *kvec, const dmd_eimp_model *m, double scale, int add);
+                             const double *kvec, const double *f, const dmd_eimp_model *m, double scale, int add);
 /* P of stored pairs [first, first+count) in file order, as complex128. */
 int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2);
 /* Fills P on the device from the synthetic description (the handle's e must

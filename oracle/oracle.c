@@ -37,6 +37,12 @@
  *                          inputs against its header; Bechstedt limits (bare
  *                          4 pi / V q^2 at large q, background-screened at
  *                          small q).
+ *   oracle_eimp_rpa_vq     the reference's GaAs run's q_scr^2 at q = 0 from its own
+ *                          bands, k points and mu (there the overlaps are the
+ *                          identity); the nondegenerate limit at q = 0 is the
+ *                          Debye q^2 times sum f(1-f) / sum f; every q's sum
+ *                          against a brute-force double sum over (k, k') with
+ *                          k - k' in the q cell, on meshes with missing points.
  *   oracle_eimp_model_P    the P diagonal is the golden-rule |M|^2 times a
  *                          normalised Gaussian delta for both scattering forms
  *                          (and the emission/absorption pair with detailed
@@ -629,11 +635,95 @@ double oracle_eimp_vq(int screening, double q2, double prefac_vq, double prefac_
 /* de = e_k,b1 - e_k',b2 inside |de| < ethr, A1 = A2 or eimp; with detailed   */
 /* balance the +-omegaL pair and the exp(-+de/2T) factors.  P1, P2 are        */
 /* npairs x nb^4, overwritten.                                                 */
+/* kIndexMap::ikvec3 / qIndexMap::iqvec3 (src_v4.5.7/comm/kmap.cpp): mesh    */
+/* cell of a point in crystal coordinates, wrapped into [0, 1) first.         */
+static int64_t mesh_code(const double *k, const int *km)
+{
+    int v[3];
+    for (int d = 0; d < 3; d++) {
+        double x = k[d] - floor(k[d]);
+        v[d] = (int)round(km[d] * x);
+        if (v[d] == km[d]) v[d] = 0;
+    }
+    return ((int64_t)v[0] * km[1] + v[1]) * km[2] + v[2];
+}
+
+/* Static RPA screening and v(q) of every stored pair, coulomb_model:        */
+/* qIndexMap::build over all (ik, jk) (kmap.h:43-55), kIndexMap (first k of  */
+/* a cell kept), calc_qscr2_static_RPA (Coulomb_Model.h:690-744),            */
+/* calc_vq_RPA's static table (:820-828) and vq (:670-675).  f: nk x nb at mu. */
+int oracle_eimp_rpa_vq(int64_t nk, int nb, int nw, const double *kvec, const int *kmesh, const double *U,
+                       const double *e, const double *f, double T, const double *GGT, double prefac_vq,
+                       double nk_full, int64_t npairs, const uint64_t *k1, const uint64_t *k2, double *vq_pair,
+                       double *qscr2_out, int64_t *nq_out)
+{
+    int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
+    int64_t *kmap = (int64_t *)malloc(sizeof(int64_t) * cells), *qmap = (int64_t *)malloc(sizeof(int64_t) * cells);
+    double *qvec = (double *)malloc(sizeof(double) * 3 * cells);
+    for (int64_t c = 0; c < cells; c++) kmap[c] = qmap[c] = -1;
+    for (int64_t ik = 0; ik < nk; ik++) {
+        int64_t c = mesh_code(kvec + 3 * ik, kmesh);
+        if (kmap[c] < 0) kmap[c] = ik;
+    }
+    int64_t nq = 0;
+    for (int64_t ik = 0; ik < nk; ik++)
+        for (int64_t jk = 0; jk < nk; jk++) {
+            double q[3];
+            for (int d = 0; d < 3; d++) {
+                q[d] = kvec[3 * ik + d] - kvec[3 * jk + d];
+                q[d] -= floor(0.5 + q[d]);
+                if (fabs(q[d] - 0.5) < 1e-6) q[d] = -0.5;
+            }
+            int64_t c = mesh_code(q, kmesh);
+            if (qmap[c] < 0) {
+                qmap[c] = nq;
+                memcpy(qvec + 3 * nq, q, sizeof q);
+                nq++;
+            }
+        }
+    double *qscr2 = (double *)calloc(nq, sizeof(double)), *vq = (double *)malloc(sizeof(double) * nq);
+    for (int64_t iq = 0; iq < nq; iq++) {
+        double s = 0;
+        for (int64_t ik = 0; ik < nk; ik++) {
+            double kj[3];
+            for (int d = 0; d < 3; d++) kj[d] = kvec[3 * ik + d] - qvec[3 * iq + d];
+            int64_t jk = kmap[mesh_code(kj, kmesh)];
+            if (jk < 0) continue;
+            const zc *Ui = (const zc *)U + ik * nw * nb, *Uj = (const zc *)U + jk * nw * nb;
+            for (int b1 = 0; b1 < nb; b1++)
+                for (int b2 = 0; b2 < nb; b2++) {
+                    double de = e[ik * nb + b1] - e[jk * nb + b2], dfde;
+                    if (fabs(de) < 1e-8) {
+                        double favg = 0.5 * (f[ik * nb + b1] + f[jk * nb + b2]);
+                        dfde = (1 - favg) * favg / T;
+                    } else {
+                        dfde = (f[jk * nb + b2] - f[ik * nb + b1]) / de;
+                    }
+                    zc o = zmk(0, 0);
+                    for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(Ui[w * nb + b1]), Uj[w * nb + b2]));
+                    s += dfde * (o.x * o.x + o.y * o.y);
+                }
+        }
+        qscr2[iq] = prefac_vq / nk_full * s;
+        vq[iq] = prefac_vq / (oracle_eimp_q2(GGT, qvec + 3 * iq, (const double[3]){0, 0, 0}) + qscr2[iq]);
+    }
+    for (int64_t p = 0; p < npairs; p++) {
+        double q[3];
+        for (int d = 0; d < 3; d++) q[d] = kvec[3 * k1[p] + d] - kvec[3 * k2[p] + d];
+        double q2 = oracle_eimp_q2(GGT, kvec + 3 * k1[p], kvec + 3 * k2[p]);
+        vq_pair[p] = q2 < 1e-20 ? 0.0 : vq[qmap[mesh_code(q, kmesh)]];
+    }
+    if (qscr2_out) memcpy(qscr2_out, qscr2, sizeof(double) * nq);
+    if (nq_out) *nq_out = nq;
+    free(kmap); free(qmap); free(qvec); free(qscr2); free(vq);
+    return 0;
+}
+
 int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
                         const double *U, const double *kvec, const double *e, const double *GGT, int screening,
                         int lindblad, int detail_balance, double prefac_A, double prefac_vq, double prefac_vq_bare,
                         double qscr2_debye, double fac0, double fac2, double fac4, double degauss, double ethr,
-                        double omegaL, double T, double *P1, double *P2)
+                        double omegaL, double T, const double *vq_pair, double *P1, double *P2)
 {
     int nb2 = nb * nb, nb4 = nb2 * nb2;
     double pgauss = 1. / (degauss * sqrt(2. * M_PI));
@@ -654,8 +744,9 @@ int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64
                 for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(Ui[w * nb + b1]), Uj[w * nb + b2]));
                 ovlp[b1 * nb + b2] = o;
             }
-        double vq = oracle_eimp_vq(screening, oracle_eimp_q2(GGT, kvec + 3 * ik, kvec + 3 * jk), prefac_vq,
-                                   prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
+        double vq = vq_pair ? vq_pair[p]
+                            : oracle_eimp_vq(screening, oracle_eimp_q2(GGT, kvec + 3 * ik, kvec + 3 * jk), prefac_vq,
+                                             prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
         for (int q = 0; q < nb2; q++) eimp[q] = zmul(zmk(prefac_A * vq, 0), ovlp[q]);
         for (int b1 = 0; b1 < nb; b1++)
             for (int b2 = 0; b2 < nb; b2++) {

@@ -47,7 +47,10 @@ def lib():
         L.oracle_eimp_vq.argtypes = [ctypes.c_int] + [ctypes.c_double] * 7
         L.oracle_eimp_vq.restype = ctypes.c_double
         L.oracle_eimp_model_P.argtypes = ([_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d] +
-                                          [ctypes.c_int] * 3 + [ctypes.c_double] * 11 + [_d, _d])
+                                          [ctypes.c_int] * 3 + [ctypes.c_double] * 11 + [_d, _d, _d])
+        L.oracle_eimp_rpa_vq.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int), _d, _d,
+                                         _d, ctypes.c_double, _d, ctypes.c_double, ctypes.c_double, _i64, _u64, _u64,
+                                         _d, _d, ctypes.POINTER(ctypes.c_int64)]
         L.oracle_spin.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, _d]
         L.oracle_spin.restype = ctypes.c_double
@@ -221,10 +224,28 @@ def eimp_vq(m, q2):
                                 m["qscr2_debye"], m["fac0"], m["fac2"], m["fac4"])
 
 
-def eimp_model_P(k1, k2, U, kvec, e, m):
+def eimp_rpa_vq(k1, k2, U, kvec, e, f, m):
+    """(v(q) per stored pair, q_scr^2 per q of the mesh's q list) of the static RPA
+    screening (Coulomb_Model.h:690-744,820-828)."""
+    e = _f64(e)
+    nk, nb = e.shape
+    U = _c128(U)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    km = (ctypes.c_int * 3)(*[int(x) for x in m["kmesh"]])
+    vq = np.zeros(len(k1))
+    qs = np.zeros(int(np.prod(m["kmesh"])))
+    nq = ctypes.c_int64(0)
+    lib().oracle_eimp_rpa_vq(nk, nb, U.shape[1], _p(_f64(kvec)), km, _p(U), _p(e), _p(_f64(f)),
+                             float(m["temperature"]), _p(_f64(m["GGT"])), float(m["prefac_vq"]), float(m["nk_full"]),
+                             len(k1), _p(k1), _p(k2), _p(vq), _p(qs), ctypes.byref(nq))
+    return vq, qs[:nq.value]
+
+
+def eimp_model_P(k1, k2, U, kvec, e, m, f=None):
     """elecimp_model::calc_P of every stored pair (ElecImp_Model.h:67-137,185-213).
     U: (nk, nb_wannier, nb) complex; kvec (nk, 3); e (nk, nb); m a dict with the
-    fields of dmd_eimp_model."""
+    fields of dmd_eimp_model; f (nk, nb) at mu for the RPA screening."""
     e = _f64(e)
     nk, nb = e.shape
     U = _c128(U)
@@ -233,11 +254,15 @@ def eimp_model_P(k1, k2, U, kvec, e, m):
     k2 = np.ascontiguousarray(k2, dtype=np.uint64)
     P1 = np.zeros((len(k1), nb ** 4), np.complex128)
     P2 = np.zeros_like(P1)
+    vq = None
+    if int(m["screening"]) == 3:
+        vq, _ = eimp_rpa_vq(k1, k2, U, kvec, e, f, m)
     rc = lib().oracle_eimp_model_P(nk, nb, nw, len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), _p(e),
                                    _p(_f64(m["GGT"])), int(m["screening"]), int(m["lindblad"]),
                                    int(m["detail_balance"]), m["prefac_A"], m["prefac_vq"], m["prefac_vq_bare"],
                                    m["qscr2_debye"], m["fac0"], m["fac2"], m["fac4"], m["degauss"], m["ethr"],
-                                   m["omegaL"], m["temperature"], _p(P1), _p(P2))
+                                   m["omegaL"], m["temperature"], _p(vq) if vq is not None else None, _p(P1),
+                                   _p(P2))
     if rc:
         raise ValueError("oracle_eimp_model_P: k index out of range")
     return P1, P2

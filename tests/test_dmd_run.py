@@ -101,7 +101,8 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("eeMode = Pee_update", "e-e scattering"),
     ("alg_linearize_dPee = 1", "linearized e-e operator"),
     ("impurity_density = 1e17\nimpMode = model_ionized", "scrMode should not be none"),
-    ("impurity_density = 1e17\nscrMode = medium", "built for the static unscreened, debye and Bechstedt"),
+    ("impurity_density = 1e17\nscrMode = medium\nscrFormula = lindhard", "built for the static RPA"),
+    ("impurity_density = 1e17\nscrMode = medium\nscrFormula = heg\ndynamic_screening = model", "built for the static RPA"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = debye\npartial_ionized = 1", "partially ionized"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = debye\nZ_impurity = 0", "Z is defined as a positive"),
     ("impurity_density = 1e17\nimpMode = other", "impMode must be model_ionized or ab_neutral"),

@@ -625,8 +625,8 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
     assert not (d / "tau_lz_elec_tot.out").exists()
 
 
-@pytest.mark.parametrize("linear", [0, 1])
-def test_dmd_run_model_ionized_impurity(tmp_path, linear):
+@pytest.mark.parametrize("linear,scr", [(0, "debye"), (1, "debye"), (0, "RPA")])
+def test_dmd_run_model_ionized_impurity(tmp_path, linear, scr):
     """impMode = model_ionized with Debye screening: P = P_eph + scale_ei P_imp,
     P_imp from the full ldbd_Umat.bin (nb_wannier = 5, e-ph columns from
     b_bot_eph + b_skip_wannier = 1), the k points and n_free = |n_e| + |n_h| at
@@ -635,7 +635,7 @@ def test_dmd_run_model_ionized_impurity(tmp_path, linear):
     RK4 (of evolve, or of evolve_linear at f_eq), S_z per report."""
     from denmat_dynm_development_amd import ldbd, scatt
     from oracle import oracle
-    print(f"case dmd_run model e-i linear={linear}", flush=True)
+    print(f"case dmd_run model e-i linear={linear} {scr}", flush=True)
     p = scatt.synth_params(0)
     d = tmp_path / "run"
     L = d / "ldbd_data"
@@ -645,6 +645,10 @@ def test_dmd_run_model_ionized_impurity(tmp_path, linear):
     f2 = lines[1].split("#")[0].split()
     f2[6], f2[7] = "5", "1"  # nb_wannier, bands skipped below the Wannier set
     lines[1] = " ".join(f2) + " # nb nv bBot_dm bTop_dm bBot_eph bTop_eph nb_wannier bskipped_wannier band_dft_skipped"
+    if scr == "RPA":  # the 64 k points on a 4 x 4 x 4 mesh (the synthetic P does not depend on them)
+        lines[2] = " 6.40000000000000e+01 64 4 4 4 # nk_full nk kmesh"
+        g = np.array([(a, b, c) for a in range(4) for b in range(4) for c in range(4)], np.float64) / 4
+        g.tofile(L / "ldbd_kvec.bin")
     (L / "ldbd_size.dat").write_text("\n".join(lines))
     info = ldbd.read_info(str(L))
     assert info.nb_wannier == 5 and info.b_skip_wannier == 1
@@ -672,17 +676,18 @@ def test_dmd_run_model_ionized_impurity(tmp_path, linear):
     eps, Z = 4.0, 1.5
 
     def model(n_i):
-        return scatt.eimp_model("debye", n_free, T, V, scatt.reciprocal_metric(R), 5, Z=Z, n_ionized=n_i,
-                                epsilon_background=eps, degauss=info.degauss, ethr=info.degauss * info.ndegauss)
-    R1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(1e-6).as_dict())
+        return scatt.eimp_model(scr, n_free, T, V, scatt.reciprocal_metric(R), 5, Z=Z, n_ionized=n_i,
+                                epsilon_background=eps, degauss=info.degauss, ethr=info.degauss * info.ndegauss,
+                                kmesh=tuple(info.kmesh), nk_full=info.nk_full)
+    R1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(1e-6).as_dict(), f=f)
     n_i = 1e-6 * np.abs(P1).max() / np.abs(R1).max()  # e-i as strong as e-ph
     scale_ei = 0.8
-    M1, M2 = oracle.eimp_model_P(k1, k2, U, kv, e, model(n_i).as_dict())
+    M1, M2 = oracle.eimp_model_P(k1, k2, U, kv, e, model(n_i).as_dict(), f=f)
     S1, S2 = P1 + scale_ei * M1, P2 + scale_ei * M2
     (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n"
                                 f"alg_linearize = {linear}\nscale_ei = {scale_ei}\n"
                                 f"impurity_density = {n_i / BOHR2CM ** 3:.17g}\nimpMode = model_ionized\n"
-                                f"Z_impurity = {Z}\nscrMode = medium\nscrFormula = debye\nepsilon_background = {eps}\n")
+                                f"Z_impurity = {Z}\nscrMode = medium\nscrFormula = {scr}\nepsilon_background = {eps}\n")
     r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-1200:], r.stderr[-800:])
     assert r.returncode == 0 and "model e-i 1:" in r.stdout

@@ -109,3 +109,32 @@ def test_model_P_rejects_bad_input():
             op.add_eimp_model(k1, bad, U, kv, m, add=False)
         with pytest.raises(ValueError):
             op.add_eimp_model(k1, k2, U[:, :2], kv, m, add=False)
+
+
+def test_model_P_with_rpa_screening():
+    """Static RPA screening summed on the device over a k mesh with missing
+    points (mesh map, per-q sums, per-pair v(q)) against the oracle's
+    qIndexMap / kIndexMap restatement, for nb = 2 and 4."""
+    _lib, scatt, oracle = _mods()
+    rng = np.random.default_rng(12)
+    km = (5, 4, 3)
+    cells = np.array([(a, b, c) for a in range(5) for b in range(4) for c in range(3)])
+    for nb in (2, 4):
+        nw = nb + 2
+        keep = np.sort(rng.choice(len(cells), 45, replace=False))
+        kv = cells[keep] / np.array(km) + rng.integers(-1, 2, size=(45, 3))
+        nk = len(kv)
+        k1, k2 = random_pairs(nk, 10, rng, diag=True)
+        e = np.sort(rng.uniform(0, 2e-3, size=(nk, nb)), axis=1)
+        f = 1 / (np.exp((e - 1e-3) / 9.5e-4) + 1)
+        U = rng.normal(size=(nk, nw, nb)) + 1j * rng.normal(size=(nk, nw, nb))
+        R = np.diag([7.0, 8.0, 9.0])
+        m = scatt.eimp_model("RPA", 0.0, 9.5e-4, abs(np.linalg.det(R)), scatt.reciprocal_metric(R), nw, Z=1.1,
+                             n_ionized=2e-6, epsilon_background=3.0, degauss=3e-4, ethr=1.5e-3, kmesh=km, nk_full=60.0)
+        R1, R2 = oracle.eimp_model_P(k1, k2, U, kv, e, m.as_dict(), f=f)
+        print(f"case eimp model RPA nb={nb}", flush=True)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+            op.add_eimp_model(k1, k2, U, kv, m, add=False, f=f)
+            A1, A2 = op.download_P()
+        s = np.abs(R1).max()
+        assert s > 0 and np.abs(A1 - R1).max() <= 1e-12 * s and np.abs(A2 - R2).max() <= 1e-12 * s

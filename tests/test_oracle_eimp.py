@@ -147,3 +147,84 @@ def test_model_P_reverse_pair_and_gram_form(db):
             continue
         assert ev[0] >= -1e-12 * top
         assert np.sum(ev > 1e-9 * top) <= (2 if db else 1)
+
+
+def _fermi(T, mu, e):
+    return 1.0 / (np.exp((e - mu) / T) + 1.0)
+
+
+def test_rpa_screening_at_q0_against_the_reference_run():
+    """The static RPA q_scr^2 at q = 0 of the reference's GaAs e-ph + e-i run
+    (first row of its ldbd_vq.out, 2.9821264e-06) from its own k points, e-ph
+    conduction bands and mu (4.9841627e-02 Ha, printed to 8 digits), with
+    eps = 12.9; at q = 0 the overlaps of orthonormal Wannier rotations are the
+    identity.  Its q list has the table's 6807 entries."""
+    g = np.load(os.path.join(GOLD, "gaas_vq.npz"))
+    d = os.path.join(GOLD, "gaas_ldbd")
+    info = ldbd.read_info(d)
+    kv = ldbd.read_kvec(d, info)
+    _, R = ldbd.read_R(d)
+    e = ldbd.read_ek(d, info)[:, info.b_bot_eph:info.b_top_eph]
+    nb = e.shape[1]
+    f = _fermi(info.temperature, 4.9841627e-02, e)
+    U = np.tile(np.eye(nb, dtype=np.complex128), (info.nk, 1, 1))
+    V = abs(np.linalg.det(R))
+    m = {"kmesh": list(info.kmesh), "temperature": info.temperature, "GGT": scatt.reciprocal_metric(R),
+         "prefac_vq": 4 * np.pi / 12.9 / V, "nk_full": info.nk_full}
+    k0 = np.zeros(1, np.uint64)
+    vq, qs = oracle.eimp_rpa_vq(k0, k0, U, kv, e, f, m)
+    assert len(qs) == len(g["q2"]) == 6807
+    assert qs[0] == pytest.approx(float(g["qscr2"][0]), rel=1e-6)
+    assert vq[0] == 0.0  # Gamma skipped
+
+
+def test_rpa_screening_limits_and_brute_force():
+    """q = 0 with orthonormal rotations: prefac_vq / nk_full sum f(1-f)/T, the
+    Debye q^2 times sum f(1-f) / sum f; every q of a mesh with missing points
+    (random, non-unitary rotations) against a brute-force double sum over
+    (k, k') whose difference falls in the q cell; v(q) of each pair =
+    prefac_vq / (|q|^2 + q_scr^2(k - k'))."""
+    rng = np.random.default_rng(8)
+    km = (4, 3, 2)
+    cells = np.array([(a, b, c) for a in range(4) for b in range(3) for c in range(2)])
+    keep = np.sort(rng.choice(len(cells), 18, replace=False))
+    kv = cells[keep] / np.array(km) + rng.integers(-1, 2, size=(18, 3))  # off by whole reciprocal vectors
+    nk, nb, nw, T = 18, 2, 3, 9.5e-4
+    e = np.sort(rng.uniform(0, 3e-3, size=(nk, nb)), axis=1)
+    f = _fermi(T, 1e-3, e)
+    R = np.diag([7.0, 8.0, 9.0])
+    m = {"kmesh": list(km), "temperature": T, "GGT": scatt.reciprocal_metric(R), "prefac_vq": 0.37, "nk_full": 24.0}
+    Q, _ = np.linalg.qr(rng.normal(size=(nk, nw, nw)) + 1j * rng.normal(size=(nk, nw, nw)))
+    k1, k2 = np.triu_indices(nk)
+    k1, k2 = k1.astype(np.uint64), k2.astype(np.uint64)
+    _, qs = oracle.eimp_rpa_vq(k1, k2, Q[:, :, :nb], kv, e, f, m)
+    assert qs[0] == pytest.approx(0.37 / 24 * np.sum(f * (1 - f)) / T, rel=1e-13)
+    U = rng.normal(size=(nk, nw, nb)) + 1j * rng.normal(size=(nk, nw, nb))
+    vq, qs = oracle.eimp_rpa_vq(k1, k2, U, kv, e, f, m)
+
+    def code(x):
+        v = np.rint(np.array(km) * (x - np.floor(x))).astype(int) % np.array(km)
+        return (v[..., 0] * km[1] + v[..., 1]) * km[2] + v[..., 2]
+    diff = kv[:, None, :] - kv[None, :, :]
+    C = code(diff)
+    first = {}
+    for i in range(nk):
+        for j in range(nk):
+            first.setdefault(C[i, j], len(first))
+    assert len(first) == len(qs)
+    brute = np.zeros(len(qs))
+    for i in range(nk):
+        for j in range(nk):
+            o = U[i].conj().T @ U[j]
+            de = e[i][:, None] - e[j][None, :]
+            dd = np.where(np.abs(de) < 1e-8, 0.25 * (2 - f[i][:, None] - f[j][None, :]) * (f[i][:, None] + f[j][None, :]) / T,
+                          (f[j][None, :] - f[i][:, None]) / np.where(np.abs(de) < 1e-8, 1, de))
+            brute[first[C[i, j]]] += np.sum(dd * np.abs(o) ** 2)
+    brute *= 0.37 / 24
+    assert np.allclose(qs, brute, rtol=1e-12, atol=0)
+    wrapped = diff - np.floor(0.5 + diff)
+    for p in range(0, len(k1), 7):
+        i, j = int(k1[p]), int(k2[p])
+        q2 = wrapped[i, j] @ m["GGT"] @ wrapped[i, j]
+        want = 0.0 if q2 < 1e-20 else 0.37 / (q2 + brute[first[C[i, j]]])
+        assert vq[p] == pytest.approx(want, rel=1e-12, abs=0)
