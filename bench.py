@@ -19,6 +19,7 @@ spin lifetime of the same runs compared.
   python bench.py --mode adjoint    # forward + adjoint RK4 with 7 parameters, nb = 2, nk = 16384
   python bench.py --mode linear     # the linearized operator (set_Lsc + evolve_linear), nb = 2 and nb = 4
   python bench.py --mode ode        # the adaptive rkf45 driver and compute_rhs against fixed-step RK4, nb = 2
+  python bench.py --mode eimp       # model e-i P generated on the device (Debye and static RPA), nb = 2
 """
 import argparse
 import json
@@ -459,6 +460,49 @@ def mode_ode(args, torch, scatt):
     return r
 
 
+def mode_eimp(args, torch, scatt):
+    """The model ionized-impurity P (add_scatt_contrib("eimp") with
+    elecimp_model::calc_P) generated on the device for every stored pair of the
+    default workload (nb = 2, nk = 65536 on a 64 x 32 x 32 k mesh, 58 M pairs,
+    nb_wannier = 4): Debye screening (closed form per pair) and the static RPA
+    screening (a k-mesh sum for every distinct q, then v(q) per pair).  Wall
+    time of the whole call; the kernels' own times come from rocprofv3."""
+    p = scatt.synth_params(args.preset)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    km = (64, 32, 32)
+    assert p.nk == km[0] * km[1] * km[2]
+    g = np.stack(np.meshgrid(np.arange(km[0]), np.arange(km[1]), np.arange(km[2]), indexing="ij"), -1).reshape(-1, 3)
+    kv = g / np.array(km, dtype=np.float64)
+    rng = np.random.default_rng(1)
+    nw = 4
+    U = (rng.normal(size=(p.nk, nw, p.nb)) + 1j * rng.normal(size=(p.nk, nw, p.nb))) / 2
+    T = p.temperature
+    f = 1 / (np.exp((e - (e.min() + 0.02 * (e.max() - e.min()))) / T) + 1)
+    R = np.array([[0.0, 5.34145, 5.34145], [5.34145, 0.0, 5.34145], [5.34145, 5.34145, 0.0]])
+    V = abs(np.linalg.det(R))
+    n_free = f.sum() / (p.nk * V)
+    out = {"mode": "eimp", "workload": f"preset {args.preset}: nb={p.nb} nk={p.nk} stored_pairs={len(k1)}, "
+                                       f"k mesh {km}, nb_wannier={nw}", "results": {}}
+    op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+    for scr in ("debye", "RPA"):
+        m = scatt.eimp_model(scr, n_free, T, V, scatt.reciprocal_metric(R), nw, n_ionized=n_free,
+                             epsilon_background=12.9, degauss=p.degauss,
+                             ethr=1e-3, kmesh=km, nk_full=float(p.nk))
+        op.add_eimp_model(k1, k2, U, kv, m, add=False, f=f)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.add_eimp_model(k1, k2, U, kv, m, add=False, f=f)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        wb = len(k1) * 2 * p.nb ** 4 * 16
+        r = {"wall_s": el, "pairs_per_s": len(k1) / el, "P_written_GB": wb / 1e9, "P_GBps_wall": wb / el / 1e9}
+        out["results"][scr] = r
+        log(f"eimp {scr}: {r}")
+    op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,7 +510,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode"])
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
@@ -497,6 +541,9 @@ def main():
         return
     if args.mode == "ode":
         print(json.dumps(mode_ode(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "eimp":
+        print(json.dumps(mode_eimp(args, torch, scatt)), flush=True)
         return
     out, (p, e, k1, k2, rho0) = bench_plain(args, torch, scatt)
     if not args.no_cpu:

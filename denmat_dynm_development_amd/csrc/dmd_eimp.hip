@@ -13,11 +13,10 @@
 //   P1[(i1 i2),(i3 i4)] = A1[i1,i3] conj(A2[i2,i4])
 //   P2[(i1 i2),(i3 i4)] = A1[i3,i1] conj(A2[i4,i2])
 // with the emission/absorption pair of A at de -+ omegaL and the detailed-
-// balance factors under detailBalance.  One thread per P element: the A
-// entries it needs are four overlaps of nb_wannier terms each, far cheaper
-// than the element's two 16-byte writes, so nothing is staged per pair and
-// every write is coalesced.  The chunk goes into the handle's P layout
-// through place_P_dev (all precisions).
+// balance factors under detailBalance.  The nb^2 A entries of a pair are
+// formed once (one thread each, into LDS) and shared by its nb^4 elements,
+// one thread per element, so the writes coalesce.  The chunk goes into the
+// handle's P layout through place_P_dev (all precisions).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -165,7 +164,10 @@ __global__ void k_kgrid(int64_t nk, const int *__restrict__ ik3, int m1, int m2,
 }
 
 // P1, P2 of n stored pairs (k1[f], k2[f]) in the ldbd_P element order; vq,
-// when given, holds v(q) of every pair of the chunk (the k-summed screenings)
+// when given, holds v(q) of every pair of the chunk (the k-summed screenings).
+// A workgroup takes PP = 256 / NB^4 pairs (3 for NB = 3): first one thread per
+// A entry (b1, b2) of each pair forms that entry (overlap, v(q), delta) into
+// LDS, then one thread per P element multiplies two entries.
 template <int NB>
 __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *__restrict__ k1,
                                                     const uint64_t *__restrict__ k2, const double2 *__restrict__ U,
@@ -173,22 +175,29 @@ __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *_
                                                     const double *__restrict__ vq, EimpConst c,
                                                     double2 *__restrict__ P1, double2 *__restrict__ P2)
 {
-    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * NB4) return;
-    const int64_t f = idx / NB4;
-    const int el = (int)(idx - f * NB4);
+    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2, PP = 256 / NB4 > 0 ? 256 / NB4 : 1;
+    constexpr int NA = 8;  // A slots per entry (2 used without detailed balance)
+    __shared__ double2 A[PP][NB2][NA];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * PP;
+    if (t < PP * NB2) {
+        const int lp = t / NB2, q = t % NB2;
+        const int64_t f = p0 + lp;
+        if (f < n) {
+            const int64_t k = (int64_t)k1[f], kp = (int64_t)k2[f];
+            const double pv = c.m.prefac_A * (vq ? vq[f] : eimp_vq(c.m, kvec + 3 * k, kvec + 3 * kp));
+            const int nw = c.m.nb_wannier, b1 = q / NB, b2 = q % NB;
+            eimp_A(c, pv, U + k * nw * NB, U + kp * nw * NB, NB, b1, b2, e[k * NB + b1] - e[kp * NB + b2], A[lp][q]);
+        }
+    }
+    __syncthreads();
+    if (t >= PP * NB4) return;
+    const int lp = t / NB4, el = t % NB4;
+    const int64_t f = p0 + lp;
+    if (f >= n) return;
     const int i1 = el / (NB * NB2), i2 = (el / NB2) % NB, i3 = (el / NB) % NB, i4 = el % NB;
-    const int64_t k = (int64_t)k1[f], kp = (int64_t)k2[f];
-    const double pv = c.m.prefac_A * (vq ? vq[f] : eimp_vq(c.m, kvec + 3 * k, kvec + 3 * kp));
-    const int nw = c.m.nb_wannier;
-    const double2 *Uk = U + k * nw * NB, *Ukp = U + kp * nw * NB;
-    const double *ek = e + k * NB, *ekp = e + kp * NB;
-    double2 a13[8], a24[8], a31[8], a42[8];
-    eimp_A(c, pv, Uk, Ukp, NB, i1, i3, ek[i1] - ekp[i3], a13);
-    eimp_A(c, pv, Uk, Ukp, NB, i2, i4, ek[i2] - ekp[i4], a24);
-    eimp_A(c, pv, Uk, Ukp, NB, i3, i1, ek[i3] - ekp[i1], a31);
-    eimp_A(c, pv, Uk, Ukp, NB, i4, i2, ek[i4] - ekp[i2], a42);
+    const double2 *a13 = A[lp][i1 * NB + i3], *a24 = A[lp][i2 * NB + i4], *a31 = A[lp][i3 * NB + i1],
+                  *a42 = A[lp][i4 * NB + i2];
     double2 p1, p2;
     if (!c.m.detail_balance) {
         p1 = cmulc(a13[0], a24[1]);
@@ -197,8 +206,8 @@ __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *_
         p1 = cadd(cmulc(a13[0], a24[2]), cmulc(a13[5], a24[7]));  // A1pp conj(A2pp) + A1mm conj(A2mm)
         p2 = cadd(cmulc(a31[4], a42[6]), cmulc(a31[1], a42[3]));  // A1mp conj(A2mp) + A1pm conj(A2pm)
     }
-    P1[idx] = p1;
-    P2[idx] = p2;
+    P1[f * NB4 + el] = p1;
+    P2[f * NB4 + el] = p2;
 }
 
 // kIndexMap::ikvec3 / qIndexMap::iqvec3 (comm/kmap.cpp): the mesh cell of a
@@ -236,22 +245,38 @@ static int rpa_vq(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, cons
         mesh_cell(q, km, v);
         code[p] = ((int64_t)v[0] * km[1] + v[1]) * km[2] + v[2];
     }
-    std::vector<int64_t> order((size_t)np);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return code[a] < code[b] || (code[a] == code[b] && a < b); });
+    // distinct cells numbered in order of first appearance: a dense cell table
+    // (one int per mesh cell) for meshes up to 2^28 cells, else a sort
     std::vector<int32_t> uq((size_t)np);
     std::vector<int> q3;
     std::vector<int64_t> rep;  // first pair of each distinct q
-    for (int64_t i = 0; i < np; i++) {
-        const int64_t p = order[i];
-        if (i == 0 || code[p] != code[order[i - 1]]) {
-            const int64_t c = code[p];
-            q3.push_back((int)(c / ((int64_t)km[1] * km[2])));
-            q3.push_back((int)((c / km[2]) % km[1]));
-            q3.push_back((int)(c % km[2]));
-            rep.push_back(p);
+    auto add_q = [&](int64_t p) {
+        const int64_t c = code[p];
+        q3.push_back((int)(c / ((int64_t)km[1] * km[2])));
+        q3.push_back((int)((c / km[2]) % km[1]));
+        q3.push_back((int)(c % km[2]));
+        rep.push_back(p);
+    };
+    if (cells <= (1ll << 28)) {
+        std::vector<int32_t> cell_q((size_t)cells, -1);
+        for (int64_t p = 0; p < np; p++) {
+            int32_t &u = cell_q[(size_t)code[p]];
+            if (u < 0) {
+                u = (int32_t)rep.size();
+                add_q(p);
+            }
+            uq[p] = u;
         }
-        uq[p] = (int32_t)(rep.size() - 1);
+    } else {
+        std::vector<int64_t> order((size_t)np);
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(),
+                  [&](int64_t a, int64_t b) { return code[a] < code[b] || (code[a] == code[b] && a < b); });
+        for (int64_t i = 0; i < np; i++) {
+            const int64_t p = order[i];
+            if (i == 0 || code[p] != code[order[i - 1]]) add_q(p);
+            uq[p] = (int32_t)(rep.size() - 1);
+        }
     }
     const int64_t nq = (int64_t)rep.size();
     int *d_ik3 = nullptr, *d_q3 = nullptr, *d_grid = nullptr;
@@ -359,7 +384,8 @@ extern "C" int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, cons
         if (e == hipSuccess) e = hipMemcpyAsync(d2, k2nd + first, (size_t)n * 8, hipMemcpyDefault, h->st);
         if (e == hipSuccess && ksum) e = hipMemcpyAsync(dvq, vq.data() + first, (size_t)n * 8, hipMemcpyHostToDevice, h->st);
         if (e != hipSuccess) break;
-        const unsigned g = nblk(n * per, 256);
+        const int pp = std::max<int>(1, 256 / (int)per);
+        const unsigned g = nblk(n, pp);
         switch (h->nb) {
         case 1: hipLaunchKernelGGL(k_eimp_model<1>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, ksum ? dvq : nullptr, c, s1, s2); break;
         case 2: hipLaunchKernelGGL(k_eimp_model<2>, dim3(g), dim3(256), 0, h->st, n, d1, d2, dU, dk, h->d_e, ksum ? dvq : nullptr, c, s1, s2); break;
