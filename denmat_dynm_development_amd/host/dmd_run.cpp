@@ -186,11 +186,16 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     if (laserA != 0) throw std::runtime_error("laserA / pumpA0: pump pulses are not carried by this operator");
     // alg_eph_sepr_eh / need_elec / need_hole choose which file set the
     // initializer wrote (ElectronPhonon.cpp:132-158); the files say it too
-    const bool need_hole = p.get("alg_eph_need_hole", 0) != 0, need_elec = p.get("alg_eph_need_elec", 1) != 0;
-    p.get("alg_eph_sepr_eh", 1);
-    if (need_hole && need_elec) throw std::runtime_error("alg_eph_need_elec and alg_eph_need_hole: one carrier set per run");
-    if (need_hole != (info.kind == 1))
-        throw std::runtime_error("alg_eph_need_hole does not match the ldbd_data file set (first line of ldbd_size.dat)");
+    // (parameters.cpp:92-99,454-455); main.cpp:65 builds one electronphonon with
+    // isHole = !need_elec
+    const bool sepr_eh = p.get("alg_eph_sepr_eh", 1) != 0;
+    const bool need_elec = !sepr_eh || p.get("alg_eph_need_elec", 1) != 0;
+    const bool need_hole = !sepr_eh || p.get("alg_eph_need_hole", 0) != 0;
+    if (sepr_eh && !need_elec && !need_hole)
+        throw std::runtime_error("if alg_eph_sepr_eh, either alg_eph_need_elec or alg_eph_need_hole");
+    if (!need_elec != (info.kind == 1))
+        throw std::runtime_error("alg_eph_need_elec / alg_eph_need_hole do not match the ldbd_data file set "
+                                 "(first line of ldbd_size.dat)");
     // parameters.cpp:360-367; rk4 (fixed step min(tstep, ode_hmax)) is this program's addition
     s.ode = p.str("alg_ode_method", "rkf45");
     if (s.ode != "rkf45" && s.ode != "euler" && s.ode != "rk4")

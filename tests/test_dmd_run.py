@@ -122,6 +122,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     ("alg_only_eimp = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "Set alg_eph_enable = 0"),
     ("alg_sparseP = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "does not support sparse"),
     ("gfac_normal_dist = 1", "random g factors"),
+    ("alg_eph_need_elec = 0\nalg_eph_need_hole = 0", "either alg_eph_need_elec or alg_eph_need_hole"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
     rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\n" + line + "\n"))
@@ -229,3 +230,12 @@ def test_dry_run_model_ionized_impurity(tmp_path):
     degauss = float(open(os.path.join(GOLD, "ldbd_size.dat")).read().splitlines()[7].split()[0])
     assert _num(out, "degauss_eimp") == pytest.approx(degauss, rel=1e-12)
     assert "not read by this runner" not in out
+
+
+def test_dry_run_carrier_set_selection(tmp_path):
+    """One electronphonon per run with isHole = !need_elec (main.cpp:65,
+    parameters.cpp:92-99): alg_eph_sepr_eh = 0 (both flags implied) and
+    need_elec = need_hole = 1 take the conduction set of this ldbd_data."""
+    for i, extra in enumerate(["alg_eph_sepr_eh = 0", "alg_eph_need_elec = 1\nalg_eph_need_hole = 1"]):
+        rc, out, err = _dry(_gaas_folder(tmp_path / str(i), "tstep = 1\ntend = 10\n" + extra + "\n"))
+        assert rc == 0, err
