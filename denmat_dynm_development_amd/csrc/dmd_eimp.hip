@@ -210,6 +210,80 @@ __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *_
     P2[f * NB4 + el] = p2;
 }
 
+// Model e-e P at the equilibrium occupations (elecelec_model::calc_P with
+// dm = null, no exchange; ElecElec_Model.h:185-262): one thread per P element
+// of a stored pair (k1, k2) and a loop over k3 with k4 = k1 + k3 - k2 looked
+// up in the mesh map.  With mee = v(k1-k2) <1|2>_{pq} <3|4>_{xy}, dl =
+// exp(pexp de^2) inside ethr (de = e1p - e2q + e3x - e4y), A2 = dl mee, A1 = A2
+// (lindblad) or mee:
+//   P1[(i1 i2),(i3 i4)] += sum_xy (1-f3x) f4y A1(i1 i3 x y) conj(A2(i2 i4 x y))
+//   P2[(i1 i2),(i3 i4)] += sum_xy f3x (1-f4y) A1(i3 i1 x y) conj(A2(i4 i2 x y))
+// scaled by prefac_gauss / nk_full.  Pairs with k1 = k2 get 0.  Each thread
+// forms the few overlaps it needs in registers (the operator is built once).
+template <int NB>
+__device__ double2 ee_ovlp(const double2 *__restrict__ Ua, const double2 *__restrict__ Ub, int nw, int p, int q)
+{
+    double2 o = make_double2(0.0, 0.0);
+    for (int w = 0; w < nw; w++) o = cadd(o, cmulc(Ub[w * NB + q], Ua[w * NB + p]));
+    return o;
+}
+template <int NB>
+__global__ __launch_bounds__(256) void k_ee_model(int64_t n, const uint64_t *__restrict__ k1,
+                                                  const uint64_t *__restrict__ k2, const double2 *__restrict__ U,
+                                                  const int *__restrict__ ik3, const int *__restrict__ kgrid, int m0,
+                                                  int m1, int m2, int64_t nk, const double *__restrict__ e,
+                                                  const double *__restrict__ f, const double *__restrict__ vqp,
+                                                  const double *__restrict__ kvec, EimpConst c, double pfac,
+                                                  double2 *__restrict__ P1, double2 *__restrict__ P2)
+{
+    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * NB4) return;
+    const int64_t fp = idx / NB4;
+    const int el = (int)(idx - fp * NB4);
+    const int i1 = el / (NB * NB2), i2 = (el / NB2) % NB, i3 = (el / NB) % NB, i4 = el % NB;
+    const int64_t a = (int64_t)k1[fp], b = (int64_t)k2[fp];
+    const int nw = c.m.nb_wannier;
+    double2 acc1 = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
+    if (a != b) {
+        const double v = vqp ? vqp[fp] : eimp_vq(c.m, kvec + 3 * a, kvec + 3 * b);
+        const double2 *Ua = U + a * nw * NB, *Ub = U + b * nw * NB;
+        double2 o13 = ee_ovlp<NB>(Ua, Ub, nw, i1, i3), o24 = ee_ovlp<NB>(Ua, Ub, nw, i2, i4),
+                o31 = ee_ovlp<NB>(Ua, Ub, nw, i3, i1), o42 = ee_ovlp<NB>(Ua, Ub, nw, i4, i2);
+        o13 = make_double2(o13.x * v, o13.y * v);
+        o24 = make_double2(o24.x * v, o24.y * v);
+        o31 = make_double2(o31.x * v, o31.y * v);
+        o42 = make_double2(o42.x * v, o42.y * v);
+        const double *ea = e + a * NB, *eb = e + b * NB;
+        for (int64_t c3 = 0; c3 < nk; c3++) {
+            const int x = ((ik3[3 * a] + ik3[3 * c3] - ik3[3 * b]) % m0 + m0) % m0;
+            const int y = ((ik3[3 * a + 1] + ik3[3 * c3 + 1] - ik3[3 * b + 1]) % m1 + m1) % m1;
+            const int z = ((ik3[3 * a + 2] + ik3[3 * c3 + 2] - ik3[3 * b + 2]) % m2 + m2) % m2;
+            const int c4i = kgrid[((int64_t)x * m1 + y) * m2 + z];
+            if (c4i == INT_MAX) continue;
+            const int64_t c4 = c4i;
+            const double2 *U3 = U + c3 * nw * NB, *U4 = U + c4 * nw * NB;
+            const double *e3 = e + c3 * NB, *e4 = e + c4 * NB, *f3 = f + c3 * NB, *f4 = f + c4 * NB;
+            for (int xx = 0; xx < NB; xx++)
+                for (int yy = 0; yy < NB; yy++) {
+                    const double2 o34 = ee_ovlp<NB>(U3, U4, nw, xx, yy);
+                    const double d34 = e3[xx] - e4[yy];
+                    auto dl = [&](double de) { return fabs(de) < c.m.ethr ? exp(c.prefac_exp * (de * de)) : 0.0; };
+                    const double d13 = dl(ea[i1] - eb[i3] + d34), d24 = dl(ea[i2] - eb[i4] + d34),
+                                 d31 = dl(ea[i3] - eb[i1] + d34), d42 = dl(ea[i4] - eb[i2] + d34);
+                    const double2 m13 = cmul(o13, o34), m24 = cmul(o24, o34), m31 = cmul(o31, o34), m42 = cmul(o42, o34);
+                    const double s13 = c.m.lindblad ? d13 : 1.0, s31 = c.m.lindblad ? d31 : 1.0;
+                    const double w1 = (1 - f3[xx]) * f4[yy] * s13 * d24, w2 = f3[xx] * (1 - f4[yy]) * s31 * d42;
+                    const double2 t1 = cmulc(m13, m24), t2 = cmulc(m31, m42);
+                    acc1 = make_double2(acc1.x + w1 * t1.x, acc1.y + w1 * t1.y);
+                    acc2 = make_double2(acc2.x + w2 * t2.x, acc2.y + w2 * t2.y);
+                }
+        }
+    }
+    P1[idx] = make_double2(acc1.x * pfac, acc1.y * pfac);
+    P2[idx] = make_double2(acc2.x * pfac, acc2.y * pfac);
+}
+
 // kIndexMap::ikvec3 / qIndexMap::iqvec3 (comm/kmap.cpp): the mesh cell of a
 // k (or q) in crystal coordinates, wrapped into [0, 1) first
 static void mesh_cell(const double *k, const int *km, int *v)
@@ -403,6 +477,106 @@ extern "C" int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, cons
     if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
     if (rc) {
         h->has_P = false;  // P partly updated
+        return rc;
+    }
+    h->has_P = true;
+    return DMD_OK;
+}
+
+// The e-e fold-in of set_eph, add_scatt_contrib("ee") (ElectronPhonon.cpp:10-11,
+// 44-113) for eeMode = Pee_fixed_at_eq: P += scale * P_ee(f) with P_ee from
+// k_ee_model; m carries the screening (as for the e-i term; prefac_A unused),
+// degauss = degauss_ee, ethr, lindblad, nb_wannier, kmesh, nk_full.
+extern "C" int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                                      const double *kvec, const double *f, const dmd_eimp_model *m, double scale,
+                                      int add)
+{
+    if (!h || !m || (h->np > 0 && (!k1st || !k2nd || !U || !kvec || !f)))
+        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: null argument");
+    if (m->nb_wannier <= 0 || !(m->degauss > 0) || m->screening < DMD_SCR_UNSCREENED || m->screening > DMD_SCR_RPA ||
+        m->detail_balance)
+        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: bad model (nb_wannier, degauss, screening, detail_balance)");
+    const int *km = m->kmesh;
+    if (km[0] <= 0 || km[1] <= 0 || km[2] <= 0 || (int64_t)km[0] * km[1] * km[2] > (1ll << 31))
+        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: the e-e sum needs the k mesh");
+    if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator: add the e-e P before set_Lsc");
+    if (add && !h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated (add = 0 sets it)");
+    if (h->nb < 1 || h->nb > 4) return fail(DMD_ERR_UNSUPPORTED, "the model e-e P is built for nb = 1..4");
+    for (int64_t q = 0; q < h->np; q++)
+        if (k1st[q] >= (uint64_t)h->nk || k2nd[q] >= (uint64_t)h->nk)
+            return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: k index outside [0, nk)");
+    if (h->np == 0) { h->has_P = true; return DMD_OK; }
+    EimpConst c;
+    c.m = *m;
+    c.prefac_delta = 1.0;
+    c.prefac_exp = m->lindblad ? -0.25 / std::pow(m->degauss, 2) : -0.5 / std::pow(m->degauss, 2);
+    const double pfac = 1. / (m->degauss * std::sqrt(2. * M_PI)) / m->nk_full;  // prefac_gauss / nk_full
+    const int64_t per = h->nb4, nw = m->nb_wannier, nk = h->nk, cells = (int64_t)km[0] * km[1] * km[2];
+    const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / per));
+    std::vector<int> ik3((size_t)nk * 3);
+    for (int64_t k = 0; k < nk; k++) mesh_cell(kvec + 3 * k, km, &ik3[3 * k]);
+    double2 *dU = nullptr, *s1 = nullptr, *s2 = nullptr;
+    double *dk = nullptr, *dvq = nullptr, *df = nullptr;
+    uint64_t *d1 = nullptr, *d2 = nullptr;
+    int *d_ik3 = nullptr, *d_grid = nullptr;
+    int rc;
+    auto release = [&]() {
+        dfree(dU); dfree(dk); dfree(s1); dfree(s2); dfree(d1); dfree(d2); dfree(dvq); dfree(df); dfree(d_ik3); dfree(d_grid);
+    };
+    const bool ksum = m->screening == DMD_SCR_RPA;
+    std::vector<double> vq;
+    if ((rc = dalloc_t(&dU, (size_t)(nk * nw * h->nb), "U")) || (rc = dalloc_t(&dk, (size_t)nk * 3, "kvec")) ||
+        (rc = dalloc_t(&df, (size_t)nk * h->nb, "f")) || (rc = dalloc_t(&d_ik3, (size_t)nk * 3, "k cells")) ||
+        (rc = dalloc_t(&d_grid, (size_t)cells, "k mesh map")) ||
+        (rc = dalloc_t(&s1, (size_t)(chunk * per), "e-e P staging")) ||
+        (rc = dalloc_t(&s2, (size_t)(chunk * per), "e-e P staging")) || (rc = dalloc_t(&d1, (size_t)chunk, "k1")) ||
+        (rc = dalloc_t(&d2, (size_t)chunk, "k2"))) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(dU, U, (size_t)(nk * nw * h->nb) * 16, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dk, kvec, (size_t)nk * 3 * 8, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(df, f, (size_t)nk * h->nb * 8, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ik3, ik3.data(), ik3.size() * 4, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, h->st, cells, d_grid, INT_MAX);
+        hipLaunchKernelGGL(k_kgrid, dim3(nblk(nk, 256)), dim3(256), 0, h->st, nk, d_ik3, km[1], km[2], d_grid);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && ksum) {
+        if ((rc = rpa_vq(h, k1st, k2nd, dU, kvec, f, *m, vq)) || (rc = dalloc_t(&dvq, (size_t)chunk, "v(q)"))) {
+            (void)hipStreamSynchronize(h->st);
+            release();
+            return rc;
+        }
+    }
+    for (int64_t first = 0; first < h->np && e == hipSuccess && rc == DMD_OK; first += chunk) {
+        const int64_t n = std::min(chunk, h->np - first);
+        e = hipMemcpyAsync(d1, k1st + first, (size_t)n * 8, hipMemcpyDefault, h->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(d2, k2nd + first, (size_t)n * 8, hipMemcpyDefault, h->st);
+        if (e == hipSuccess && ksum) e = hipMemcpyAsync(dvq, vq.data() + first, (size_t)n * 8, hipMemcpyHostToDevice, h->st);
+        if (e != hipSuccess) break;
+        const double *vp = ksum ? dvq : nullptr;
+#define DMD_EE_LAUNCH(NB_)                                                                                           \
+    hipLaunchKernelGGL(k_ee_model<NB_>, dim3(nblk(n * per, 256)), dim3(256), 0, h->st, n, d1, d2, dU, d_ik3, d_grid, km[0], \
+                       km[1], km[2], nk, h->d_e, df, vp, dk, c, pfac, s1, s2)
+        switch (h->nb) {
+        case 1: DMD_EE_LAUNCH(1); break;
+        case 2: DMD_EE_LAUNCH(2); break;
+        case 3: DMD_EE_LAUNCH(3); break;
+        default: DMD_EE_LAUNCH(4); break;
+        }
+#undef DMD_EE_LAUNCH
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        rc = place_P_dev(h, s1, s2, n, first, scale, add ? 1 : 0);
+        if (rc == DMD_OK) e = hipStreamSynchronize(h->st);
+    }
+    hipError_t e2 = hipStreamSynchronize(h->st);
+    release();
+    if (rc == DMD_OK && e == hipSuccess) e = e2;
+    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rc) {
+        h->has_P = false;
         return rc;
     }
     h->has_P = true;

@@ -179,6 +179,16 @@ class ScatteringOperator:
         info = ldbd.read_info(dirname) if info is None else info
         call("dmd_ldbd_write_sP", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), float(thr))
 
+    def add_ee_model(self, k1st, k2nd, U, kvec, f, model, scale=1.0, add=True):
+        """The model e-e P at the equilibrium occupations f (nk, nb), generated
+        on the device and added (add) or set: add_scatt_contrib("ee") with
+        elecelec_model::calc_P at dm = null, no exchange (ElecElec_Model.h:
+        185-262), as eeMode = Pee_fixed_at_eq keeps it; scale = scale_ee.
+        model: eimp_model(...) with degauss = degauss_ee and the k mesh."""
+        k1, k2, U, kv, f = _ee_args(self, k1st, k2nd, U, kvec, f, model)
+        call("dmd_scatt_add_ee_model", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             f.ctypes.data, ctypes.byref(model), float(scale), int(bool(add)))
+
     def download_P(self, first=0, count=None):
         count = self.n_pairs - first if count is None else count
         n4 = self.nb ** 4
@@ -433,6 +443,19 @@ class ElectronPhonon(ScatteringOperator):
 
     def evolve(self, t, dm, dm1=None):
         return self.eval(t, dm, dm1)
+
+
+def _ee_args(op, k1st, k2nd, U, kvec, f, model):
+    k1 = np.ascontiguousarray(k1st, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2nd, dtype=np.uint64)
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    kv = np.ascontiguousarray(kvec, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if len(k1) != op.n_pairs or len(k2) != op.n_pairs:
+        raise ValueError("k1st/k2nd need one entry per stored pair")
+    if U.shape != (op.nk, model.nb_wannier, op.nb) or kv.shape != (op.nk, 3) or f.shape != (op.nk, op.nb):
+        raise ValueError("U must be (nk, nb_wannier, nb), kvec (nk, 3) and f (nk, nb)")
+    return k1, k2, U, kv, f
 
 
 SCREENING = {"unscreened": _lib.DMD_SCR_UNSCREENED, "debye": _lib.DMD_SCR_DEBYE,

@@ -141,6 +141,17 @@ typedef struct {
  * screening only and may be null otherwise. */
 int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
                              const double *kvec, const double *f, const dmd_eimp_model *m, double scale, int add);
+/* P += scale * P_ee (add = 1) or P = scale * P_ee (add = 0): the model e-e P
+ * at the equilibrium occupations f (nk x nb, e-ph bands at mu) that set_eph
+ * folds in through add_scatt_contrib("ee") (ElectronPhonon.cpp:10-11,44-113;
+ * elecelec_model::calc_P with dm = null, ElecElec_Model.h:185-262), kept fixed
+ * as eeMode = Pee_fixed_at_eq does; no exchange term (ee_antisymmetry = 0).
+ * m: the screening as for dmd_scatt_add_eimp_model (unscreened, debye,
+ * Bechstedt or RPA; prefac_A unused), degauss = degauss_ee, ethr, lindblad,
+ * nb_wannier, kmesh (required: k4 = k1 + k3 - k2 is looked up on it), nk_full;
+ * scale = scale_ee. */
+int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                           const double *kvec, const double *f, const dmd_eimp_model *m, double scale, int add);
 /* P of stored pairs [first, first+count) in file order, as complex128. */
 int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2);
 /* Fills P on the device from the synthetic description (the handle's e must

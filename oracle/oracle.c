@@ -49,6 +49,11 @@
  *                          balance), P2(k,k') = conj(P1(k',k)), the reshaped
  *                          P1[(i1 i3),(i2 i4)] is Hermitian PSD of rank 1
  *                          (2 with detailed balance).
+ *   oracle_ee_model_P      the P diagonal is the e-e golden rule
+ *                          (1/nk_full) sum_k3 |M|^2 (1-f3) f4 delta(de) with a
+ *                          normalised Gaussian, for both scattering forms;
+ *                          P2(k1,k2) = conj(P1(k2,k1)); the Lindblad P1
+ *                          reshaped to [(i1 i3),(i2 i4)] is Hermitian PSD.
  *   oracle_coherent        two-level precession closed form, tracelessness.
  *   oracle_rk4             1+z+z^2/2+z^3/6+z^4/24 amplification, O(h^4).
  *   oracle_spin            brute force, zero at rho=f, linearity, and the
@@ -790,6 +795,98 @@ int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64
     }
     free(ovlp); free(eimp);
     for (int a = 0; a < 8; a++) free(A[a]);
+    return 0;
+}
+
+/* Model e-e P at the equilibrium occupations, elecelec_model::calc_P with  */
+/* dm = null and no exchange (scattering_model/ElecElec_Model.h:69-112,      */
+/* 144-184,185-262,285-296): for each stored pair (k1, k2), k1 != k2, and    */
+/* every k3 whose k4 = k1 + k3 - k2 is a kept k point (first k of a mesh      */
+/* cell), mee = v(k1-k2) <1|2><3|4>, delta = exp(pexp de^2) inside ethr,      */
+/* A2 = delta mee, A1 = A2 (lindblad) or mee;                                 */
+/* P1[(i1 i2),(i3 i4)] += sum_xy (1-f3x) A1[(i1 i3),(x y)] f4y conj(A2[(i2 i4),(x y)]), */
+/* P2[(i1 i2),(i3 i4)] += sum_xy f3x A1[(i3 i1),(x y)] (1-f4y) conj(A2[(i4 i2),(x y)]), */
+/* times prefac_gauss / nk_full.  vq_pair: v(q) of each stored pair.          */
+int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                      const double *U, const double *kvec, const int *kmesh, const double *e, const double *f,
+                      const double *vq_pair, int lindblad, double degauss, double ethr, double nk_full, double *P1,
+                      double *P2)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
+    int64_t *kmap = (int64_t *)malloc(sizeof(int64_t) * cells);
+    for (int64_t c = 0; c < cells; c++) kmap[c] = -1;
+    for (int64_t ik = 0; ik < nk; ik++) {
+        int64_t c = mesh_code(kvec + 3 * ik, kmesh);
+        if (kmap[c] < 0) kmap[c] = ik;
+    }
+    double pgauss = 1. / (degauss * sqrt(2. * M_PI));
+    double pexp = lindblad ? -0.25 / (degauss * degauss) : -0.5 / (degauss * degauss);
+    zc *o12 = (zc *)malloc(sizeof(zc) * nb2), *o34 = (zc *)malloc(sizeof(zc) * nb2);
+    zc *A1 = (zc *)malloc(sizeof(zc) * nb4), *A2 = (zc *)malloc(sizeof(zc) * nb4);
+    zc *Ar = (zc *)malloc(sizeof(zc) * nb2), *Arb = (zc *)malloc(sizeof(zc) * nb2);
+    double *dl = (double *)malloc(sizeof(double) * nb4);
+    for (int64_t p = 0; p < npairs; p++) {
+        int64_t a = (int64_t)k1[p], b = (int64_t)k2[p];
+        zc *Q1 = (zc *)P1 + p * nb4, *Q2 = (zc *)P2 + p * nb4;
+        for (int q = 0; q < nb4; q++) Q1[q] = Q2[q] = zmk(0, 0);
+        if (a == b) continue;
+        const zc *Ua = (const zc *)U + a * nw * nb, *Ub = (const zc *)U + b * nw * nb;
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) {
+                zc o = zmk(0, 0);
+                for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(Ua[w * nb + i]), Ub[w * nb + j]));
+                o12[i * nb + j] = zmul(o, zmk(vq_pair[p], 0));
+            }
+        for (int64_t c3 = 0; c3 < nk; c3++) {
+            double k4v[3];
+            for (int d = 0; d < 3; d++) k4v[d] = kvec[3 * a + d] + kvec[3 * c3 + d] - kvec[3 * b + d];
+            int64_t c4 = kmap[mesh_code(k4v, kmesh)];
+            if (c4 < 0) continue;
+            int conserved = 0;
+            for (int i1 = 0; i1 < nb; i1++)
+                for (int i2 = 0; i2 < nb; i2++)
+                    for (int i3 = 0; i3 < nb; i3++)
+                        for (int i4 = 0; i4 < nb; i4++) {
+                            double de = e[a * nb + i1] - e[b * nb + i2] + e[c3 * nb + i3] - e[c4 * nb + i4];
+                            int q = ((i1 * nb + i2) * nb + i3) * nb + i4;
+                            dl[q] = 0;
+                            if (fabs(de) < ethr) { dl[q] = exp(pexp * pow(de, 2)); conserved = 1; }
+                        }
+            if (!conserved) continue;
+            const zc *U3 = (const zc *)U + c3 * nw * nb, *U4 = (const zc *)U + c4 * nw * nb;
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    zc o = zmk(0, 0);
+                    for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(U3[w * nb + i]), U4[w * nb + j]));
+                    o34[i * nb + j] = o;
+                }
+            for (int q = 0; q < nb4; q++) {
+                zc mee = zmul(o12[q / nb2], o34[q % nb2]);
+                A2[q] = zscale(mee, dl[q]);
+                A1[q] = lindblad ? A2[q] : mee;
+            }
+            const double *f3 = f + c3 * nb, *f4 = f + c4 * nb;
+            for (int i1 = 0; i1 < nb; i1++)
+                for (int i3 = 0; i3 < nb; i3++) {
+                    for (int x = 0; x < nb; x++)
+                        for (int y = 0; y < nb; y++) {
+                            Ar[x * nb + y] = zscale(zscale(A1[(i1 * nb + i3) * nb2 + x * nb + y], 1 - f3[x]), f4[y]);
+                            Arb[x * nb + y] = zscale(zscale(A1[(i3 * nb + i1) * nb2 + x * nb + y], f3[x]), 1 - f4[y]);
+                        }
+                    for (int i2 = 0; i2 < nb; i2++)
+                        for (int i4 = 0; i4 < nb; i4++) {
+                            int q = ((i1 * nb + i2) * nb + i3) * nb + i4, n24 = (i2 * nb + i4) * nb2, n42 = (i4 * nb + i2) * nb2;
+                            for (int xy = 0; xy < nb2; xy++) {
+                                Q1[q] = zadd(Q1[q], zmul(Ar[xy], zconj(A2[n24 + xy])));
+                                Q2[q] = zadd(Q2[q], zmul(Arb[xy], zconj(A2[n42 + xy])));
+                            }
+                        }
+                }
+        }
+        for (int q = 0; q < nb4; q++) { Q1[q] = zscale(Q1[q], pgauss / nk_full); Q2[q] = zscale(Q2[q], pgauss / nk_full); }
+    }
+    free(kmap); free(o12); free(o34); free(A1); free(A2); free(Ar); free(Arb); free(dl);
     return 0;
 }
 

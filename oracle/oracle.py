@@ -48,6 +48,9 @@ def lib():
         L.oracle_eimp_vq.restype = ctypes.c_double
         L.oracle_eimp_model_P.argtypes = ([_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d] +
                                           [ctypes.c_int] * 3 + [ctypes.c_double] * 11 + [_d, _d, _d])
+        L.oracle_ee_model_P.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d,
+                                        ctypes.POINTER(ctypes.c_int), _d, _d, _d, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, _d, _d]
         L.oracle_eimp_rpa_vq.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int), _d, _d,
                                          _d, ctypes.c_double, _d, ctypes.c_double, ctypes.c_double, _i64, _u64, _u64,
                                          _d, _d, ctypes.POINTER(ctypes.c_int64)]
@@ -265,6 +268,23 @@ def eimp_model_P(k1, k2, U, kvec, e, m, f=None):
                                    _p(P2))
     if rc:
         raise ValueError("oracle_eimp_model_P: k index out of range")
+    return P1, P2
+
+
+def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, lindblad=True):
+    """elecelec_model::calc_P at the equilibrium f, no exchange
+    (ElecElec_Model.h:185-262); vq_pair: v(k1 - k2) of each stored pair."""
+    e = _f64(e)
+    nk, nb = e.shape
+    U = _c128(U)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    km = (ctypes.c_int * 3)(*[int(x) for x in kmesh])
+    P1 = np.zeros((len(k1), nb ** 4), np.complex128)
+    P2 = np.zeros_like(P1)
+    lib().oracle_ee_model_P(nk, nb, U.shape[1], len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), km, _p(e),
+                            _p(_f64(f)), _p(_f64(vq_pair)), int(lindblad), float(degauss), float(ethr),
+                            float(nk_full), _p(P1), _p(P2))
     return P1, P2
 
 
