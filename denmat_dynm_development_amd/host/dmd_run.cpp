@@ -10,7 +10,8 @@
 // spin matrices, stored pairs and P (x scale_eph), ab initio e-i P of every
 // impurity_density<i> with impMode ab_neutral (x |n_i cell_size| x scale_ei)
 // or the model ionized-impurity P (impMode model_ionized, scrMode medium,
-// static RPA / unscreened / debye / Bechstedt screening) generated on the GPU;
+// static RPA / unscreened / debye / Bechstedt screening) generated on the GPU,
+// and the model e-e P at equilibrium (eeMode Pee_fixed_at_eq);
 // the initial state is the Bpert first-order state (electron::
 // compute_dm_Bpert_1st) or the restart files; alg_ddmdteq subtracts the
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
@@ -25,8 +26,8 @@
 // alg_sparseP zeroes |P| <= alg_thr_sparseP and writes the sparse sP files;
 // alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
 // the valley k-pair filters and the scattering switches act as in the
-// reference.  Features whose physics this operator does not carry (e-e,
-// lindhard or electron-gas screening, pump pulses, k-resolved random g
+// reference.  Features whose physics this operator does not carry (e-e
+// rebuilt from rho, lindhard or electron-gas screening, pump pulses, k-resolved random g
 // factors) stop the run with a message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
@@ -78,6 +79,9 @@ struct Settings {
     std::vector<bool> ni_model;  // impMode model_ionized (else ab_neutral)
     std::vector<double> ni_Z;    // Z_impurity
     bool eimp_db = false;        // detailBalance of the model e-i term
+    std::string eeMode = "none";
+    bool only_ee = false;
+    double scale_ee = 1, degauss_ee = 0;
     double degauss = 0, ndegauss = 0, degauss_eimp = 0;
     // screening (coulombParam, parameters.cpp:192-206)
     std::string scrMode = "none", scrFormula = "RPA", scr_dynamic = "static";
@@ -119,6 +123,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     const double scale_scatt = p.get("scale_scatt", 1.);
     s.scale_eph = p.get("scale_eph", scale_scatt);
     s.scale_ei = p.get("scale_ei", scale_scatt);
+    s.scale_ee = p.get("scale_ee", scale_scatt);
     s.ddmdteq = p.get("alg_ddmdteq", 0) != 0;
     s.expt = p.get("alg_expt", 1) != 0;
     s.scatt = p.str("alg_scatt", "lindblad");
@@ -144,7 +149,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         if (s.bEnd_tau > info.b_top_dm - info.b_bot_dm) throw std::runtime_error("bEnd_tau > nb_dm");
         if (s.bStart_tau < 0) throw std::runtime_error("bStart_tau < 0");
     }
-    refuse(p, "alg_only_ee", 0, "alg_only_ee");
+    s.only_ee = p.get("alg_only_ee", 0) != 0;
     s.only_intra = p.get("alg_only_intravalley", 0) != 0;  // parameters.cpp:88-89
     s.only_inter = p.get("alg_only_intervalley", 0) != 0;
     if (s.only_intra && s.only_inter)
@@ -181,7 +186,17 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     s.set_scv_zero = p.get("alg_set_scv_zero", 0) != 0;
     refuse(p, "gfac_normal_dist", 0, "k-resolved random g factors");
     s.excess = p.get("carrier_density_means_excess_density", 0) != 0;  // parameters.cpp:170
-    if (p.str("eeMode", "none") != "none") throw std::runtime_error("eeMode: e-e scattering is not carried by this operator");
+    // e-e (parameters.cpp:226-231,479-492): the P of set_eph at the equilibrium
+    // occupations, kept fixed (Pee_fixed_at_eq); Pee_update rebuilds it from rho
+    s.eeMode = p.str("eeMode", "none");
+    if (s.eeMode != "none" && s.eeMode != "Pee_fixed_at_eq" && s.eeMode != "Pee_update")
+        throw std::runtime_error("eeMode must be none or Pee_fixed_at_eq now");
+    if (s.eeMode == "Pee_update")
+        throw std::runtime_error("eeMode = Pee_update: the e-e P rebuilt from rho during the run is not carried by this "
+                                 "operator (Pee_fixed_at_eq is)");
+    refuse(p, "ee_antisymmetry", 0, "the exchange term of e-e scattering");
+    s.degauss_ee = p.get("degauss_ee", p.get("degauss", info.degauss / eV, eV) / eV, eV);
+    (void)p.get("freq_update_ee_model", 0);
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
     if (laserA != 0) throw std::runtime_error("laserA / pumpA0: pump pulses are not carried by this operator");
     // alg_eph_sepr_eh / need_elec / need_hole choose which file set the
@@ -262,12 +277,25 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     (void)p.get("freq_update_eimp_model", 0);  // updates only during a pump pulse (DensityMatrix_Dynamics.h:301-318)
     // parameters.cpp:459-464,485-486,493-497
     if (s.only_eimp && s.ni.empty()) throw std::runtime_error("alg_only_eimp is only possible if impurity_density is non-zero");
-    if ((!s.ni.empty() || s.eph_enable) && !s.scatt_enable)
+    const bool ee = s.eeMode != "none";
+    if ((!s.ni.empty() || ee || s.eph_enable) && !s.scatt_enable)
         throw std::runtime_error("alg_scatt_enable must be true if any scattering mechanism is considered");
-    if (!s.eph_enable && s.ni.empty() && s.scatt_enable)
+    if (!s.eph_enable && s.ni.empty() && !ee && s.scatt_enable)
         throw std::runtime_error("alg_scatt_enable must be false if no scattering mechanism is considered");
-    if (s.eph_enable && s.only_eimp)
+    if (s.scrMode == "none" && ee)
+        throw std::runtime_error("scrMode should not be none if considering electron-electron scattering");
+    if (s.eph_enable && (s.only_eimp || s.only_ee))
         throw std::runtime_error("if only e-imp or only e-e, e-ph should not be enabled. Set alg_eph_enable = 0");
+    if (s.only_ee && !ee) throw std::runtime_error("alg_only_ee is only possible if eeMode is not none");
+    if (s.only_eimp && ee) throw std::runtime_error("if you want only e-i scattering, please set eeMode to be none");
+    // under alg_linearize the e-e sum takes f_eq while RPA keeps summing f of mu
+    // (ElectronPhonon_linear.cpp:19, Coulomb_Model.h:587-590); one f goes to the device
+    if (ee && s.linearize && s.scrFormula == "RPA")
+        throw std::runtime_error("eeMode with alg_linearize and scrFormula = RPA: the e-e term here takes one set of "
+                                 "occupations for both the screening and the sum");
+    if (ee && (s.scrFormula == "lindhard" || s.scrFormula == "heg" || s.scr_dynamic != "static"))
+        throw std::runtime_error("eeMode: the model e-e term is built for the static RPA, unscreened, debye and "
+                                 "Bechstedt screening");
     bool any_ab = false;
     for (size_t iD = 0; iD < s.ni.size(); iD++) {
         if (!s.ni_model[iD]) any_ab = true;
@@ -503,6 +531,8 @@ int run(const std::string &folder, bool dry)
         std::printf("scrMode = %s scrFormula = %s epsilon_background = %lg model_impurities = %d detailBalance = %d "
                     "degauss_eimp = %.10le\n", s.scrMode.c_str(), s.scrFormula.c_str(), s.eps_background, n_model,
                     (int)s.eimp_db, s.degauss_eimp);
+        std::printf("eeMode = %s alg_only_ee = %d scale_ee = %lg degauss_ee = %.10le\n", s.eeMode.c_str(), (int)s.only_ee,
+                    s.scale_ee, s.degauss_ee);
     }
     std::printf("alg_read_Bso = %d needL = %d scale_Ez = %lg alg_set_scv_zero = %d\n", (int)s.read_Bso, (int)s.needL,
                 s.scale_Ez, (int)s.set_scv_zero);
@@ -633,34 +663,29 @@ int run(const std::string &folder, bool dry)
     std::vector<uint64_t> mk1, mk2;
     std::vector<cplx> mU;
     std::vector<double> mkv;
-    for (size_t iD = 0; iD < s.ni.size(); iD++) {
-        if (!s.ni_model[iD]) {
-            eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
-            continue;
-        }
-        if (mU.empty()) {  // electron::read_ldbd_Umat (electron.cpp:388-410), kvec, stored pairs
-            struct stat us;
-            if (stat((ldbd + "/ldbd_Umat.bin").c_str(), &us) != 0)
-                throw std::runtime_error("impMode = model_ionized needs ldbd_data/ldbd_Umat.bin (the overlaps <k|k'>)");
-            const int nbe = info.b_top_eph - info.b_bot_eph;
-            mU.resize((size_t)nk * info.nb_wannier * nbe);
-            check(dmd_ldbd_read_Umat(ldbd.c_str(), &info, (double *)mU.data()), "ldbd_Umat.bin");
-            mkv.resize((size_t)nk * 3);
-            check(dmd_ldbd_read_kvec(ldbd.c_str(), &info, mkv.data()), "ldbd_kvec.bin");
-            mk1.resize(info.nkpair);
-            mk2.resize(info.nkpair);
-            check(dmd_ldbd_read_kpairs(ldbd.c_str(), &info, 0, info.nkpair, mk1.data(), mk2.data()), "k pairs");
-        }
-        // elecimp_model (ElecImp_Model.h:22-40) with coulomb_model::init_model (Coulomb_Model.h:554-575)
+    const int b0e = info.b_bot_eph - info.b_bot_dm, nbe = info.b_top_eph - info.b_bot_eph;
+    auto model_inputs = [&](const char *who) {  // electron::read_ldbd_Umat (electron.cpp:388-410), kvec, stored pairs
+        if (!mU.empty()) return;
+        struct stat us;
+        if (stat((ldbd + "/ldbd_Umat.bin").c_str(), &us) != 0)
+            throw std::runtime_error(std::string(who) + " needs ldbd_data/ldbd_Umat.bin (the overlaps <k|k'>)");
+        mU.resize((size_t)nk * info.nb_wannier * nbe);
+        check(dmd_ldbd_read_Umat(ldbd.c_str(), &info, (double *)mU.data()), "ldbd_Umat.bin");
+        mkv.resize((size_t)nk * 3);
+        check(dmd_ldbd_read_kvec(ldbd.c_str(), &info, mkv.data()), "ldbd_kvec.bin");
+        mk1.resize(info.nkpair);
+        mk2.resize(info.nkpair);
+        check(dmd_ldbd_read_kpairs(ldbd.c_str(), &info, 0, info.nkpair, mk1.data(), mk2.data()), "k pairs");
+    };
+    // coulomb_model::init_model (Coulomb_Model.h:554-575) and vq's static forms
+    auto screened = [&]() {
         dmd_eimp_model m{};
         m.screening = s.scrFormula == "debye" ? DMD_SCR_DEBYE : s.scrFormula == "Bechstedt" ? DMD_SCR_BECHSTEDT
                       : s.scrFormula == "RPA" ? DMD_SCR_RPA : DMD_SCR_UNSCREENED;
         for (int d = 0; d < 3; d++) m.kmesh[d] = info.kmesh[d];
         m.nk_full = info.nk_full;
         m.lindblad = s.scatt == "lindblad";
-        m.detail_balance = s.eimp_db;
         m.nb_wannier = info.nb_wannier;
-        m.prefac_A = s.ni_Z[iD] * std::sqrt(s.ni[iD] * cell);
         m.prefac_vq = 4 * M_PI / s.eps_background / cell;
         m.prefac_vq_bare = 4 * M_PI / cell;
         const double kF = std::pow(3 * M_PI * M_PI * n_free, 1. / 3.), EF = kF * kF / 2. / s.meff;
@@ -669,24 +694,48 @@ int run(const std::string &folder, bool dry)
         m.fac0 = 1. / (s.eps_background - 1);
         m.fac2 = 1. / qTF2;
         m.fac4 = 3. / 4. / (kF * kF) / qTF2;
-        m.degauss = s.degauss_eimp;
         m.ethr = s.degauss * s.ndegauss;
         m.omegaL = 1e-4 * T;
         m.temperature = T;
         reciprocal_metric(R, m.GGT);
+        return m;
+    };
+    // occupations of the e-ph bands: at mu (coulomb_model's f, Coulomb_Model.h:428), or f_eq
+    // for the linearized operator's e-e term (ElectronPhonon_linear.cpp:19)
+    auto eph_cols = [&](const std::vector<double> &fsrc) {
+        std::vector<double> o((size_t)nk * nbe);
+        for (long k = 0; k < nk; k++)
+            for (int b = 0; b < nbe; b++) o[(size_t)k * nbe + b] = fsrc[(size_t)k * nbd + b0e + b];
+        return o;
+    };
+    for (size_t iD = 0; iD < s.ni.size() && !s.only_ee; iD++) {
+        if (!s.ni_model[iD]) {
+            eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+            continue;
+        }
+        model_inputs("impMode = model_ionized");
+        // elecimp_model (ElecImp_Model.h:22-40)
+        dmd_eimp_model m = screened();
+        m.detail_balance = s.eimp_db;
+        m.prefac_A = s.ni_Z[iD] * std::sqrt(s.ni[iD] * cell);
+        m.degauss = s.degauss_eimp;
         std::printf("model e-i %zu: Z = %lg n_i = %lg cm-3 qscr2_debye = %lg degauss = %lg detailBalance = %d\n", iD + 1,
                     s.ni_Z[iD], s.ni[iD] / std::pow(bohr2cm, 3), m.qscr2_debye, m.degauss, (int)m.detail_balance);
-        // RPA: occupations of the e-ph bands at mu (coulomb_model's f, Coulomb_Model.h:428; init_RPA
-        // leaves them when the screening is updated, :587-590)
+        // RPA sums at f of mu (init_RPA leaves f when the screening is updated, Coulomb_Model.h:587-590)
         std::vector<double> feph;
-        if (m.screening == DMD_SCR_RPA) {
-            const int b0 = info.b_bot_eph - info.b_bot_dm, nbe = info.b_top_eph - info.b_bot_eph;
-            feph.resize((size_t)nk * nbe);
-            for (long k = 0; k < nk; k++)
-                for (int b = 0; b < nbe; b++) feph[(size_t)k * nbe + b] = fdm[(size_t)k * nbd + b0 + b];
-        }
+        if (m.screening == DMD_SCR_RPA) feph = eph_cols(fdm);
         eph.add_eimp_model((int)iD, mk1.data(), mk2.data(), mU.data(), mkv.data(), feph.empty() ? nullptr : feph.data(), m,
                            s.scale_ei);
+    }
+    if (s.eeMode != "none" && !s.only_eimp && s.scatt_enable) {  // add_scatt_contrib("ee") (ElectronPhonon.cpp:11)
+        model_inputs("eeMode");
+        for (int d = 0; d < 3; d++)
+            if (info.kmesh[d] <= 0) throw std::runtime_error("eeMode needs the k mesh of ldbd_size.dat (line 3)");
+        dmd_eimp_model m = screened();
+        m.degauss = s.degauss_ee;
+        const std::vector<double> fee = eph_cols(s.linearize ? feq : fdm);
+        std::printf("model e-e (%s): degauss_ee = %lg scale_ee = %lg\n", s.eeMode.c_str(), m.degauss, s.scale_ee);
+        eph.add_ee_model(mk1.data(), mk2.data(), mU.data(), mkv.data(), fee.data(), m, s.scale_ee);
     }
     // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
     // alg_linearize set_eph returns before it (:6)

@@ -116,6 +116,19 @@ public:
         for (auto &s : eimp_slots) eimp.push_back(&s);
     }
 
+    // add_scatt_contrib("ee") (ElectronPhonon.cpp:11,44-113) with eeMode =
+    // Pee_fixed_at_eq: P += scale_ee * P_ee(f), the model e-e P at the
+    // occupations f (nk x nb of this window) generated on the device; alg_only_ee
+    // (no e-ph P set) makes it P itself
+    void add_ee_model(const uint64_t *k1st, const uint64_t *k2nd, const complex_t *U, const double *kvec,
+                      const double *f, const dmd_eimp_model &m, double scale_ee)
+    {
+        check(dmd_scatt_add_ee_model(h_, k1st, k2nd, (const double *)U, kvec, f, &m, scale_ee, P_set_ ? 1 : 0),
+              "model e-e P");
+        P_set_ = true;
+        if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
+    }
+
     // set_sparseP(true) at the end of set_eph (ElectronPhonon.cpp:12,115-129)
     // under alg_sparseP: P with |P| <= thr_sparseP zeroed and, with
     // write_files, the sparse sP files written next to ldbd_data's P files

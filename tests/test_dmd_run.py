@@ -98,7 +98,11 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
 
 
 @pytest.mark.parametrize("line,msg", [
-    ("eeMode = Pee_update", "e-e scattering"),
+    ("eeMode = Pee_update\nscrMode = medium", "rebuilt from rho"),
+    ("eeMode = Pee_fixed_at_eq", "scrMode should not be none if considering electron-electron"),
+    ("eeMode = Pee_fixed_at_eq\nscrMode = medium\nee_antisymmetry = 1", "exchange term"),
+    ("alg_only_ee = 1\nalg_eph_enable = 0\nimpurity_density = 1e17\nimpMode = ab_neutral", "alg_only_ee is only possible"),
+    ("eeMode = other", "eeMode must be none or Pee_fixed_at_eq"),
     ("alg_linearize_dPee = 1", "linearized e-e operator"),
     ("impurity_density = 1e17\nimpMode = model_ionized", "scrMode should not be none"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = lindhard", "built for the static RPA"),
@@ -239,3 +243,14 @@ def test_dry_run_carrier_set_selection(tmp_path):
     for i, extra in enumerate(["alg_eph_sepr_eh = 0", "alg_eph_need_elec = 1\nalg_eph_need_hole = 1"]):
         rc, out, err = _dry(_gaas_folder(tmp_path / str(i), "tstep = 1\ntend = 10\n" + extra + "\n"))
         assert rc == 0, err
+
+
+def test_dry_run_fixed_ee(tmp_path):
+    """eeMode = Pee_fixed_at_eq with scrMode = medium is read with the
+    reference's defaults (scale_ee = scale_scatt, degauss_ee = degauss)."""
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\neeMode = Pee_fixed_at_eq\nscrMode = medium\n"
+                                               "scrFormula = debye\nscale_scatt = 0.5\n"))
+    assert rc == 0, err
+    assert "eeMode = Pee_fixed_at_eq alg_only_ee = 0 scale_ee = 0.5" in out
+    degauss = float(open(os.path.join(GOLD, "ldbd_size.dat")).read().splitlines()[7].split()[0])
+    assert _num(out, "degauss_ee") == pytest.approx(degauss, rel=1e-12)

@@ -719,3 +719,61 @@ def test_dmd_run_model_ionized_impurity(tmp_path, linear, scr):
     a = oracle.evolve(k1, k2, P1, P2, e, y0, prefac=prefac)
     b = oracle.evolve(k1, k2, S1, S2, e, y0, prefac=prefac)
     assert np.abs(a - b).max() > 0.1 * np.abs(b).max()
+
+
+def test_dmd_run_fixed_ee(tmp_path):
+    """eeMode = Pee_fixed_at_eq with Debye screening: P = P_eph + scale_ee P_ee,
+    P_ee the model e-e P at the occupations of mu (set_eph's add_scatt_contrib
+    ("ee")) on a 4 x 4 x 4 k mesh, against the oracle's P_ee and RK4, S_z per
+    report."""
+    from denmat_dynm_development_amd import ldbd, scatt
+    from oracle import oracle
+    print("case dmd_run fixed e-e", flush=True)
+    p = scatt.synth_params(0)
+    d = tmp_path / "run"
+    L = d / "ldbd_data"
+    L.mkdir(parents=True)
+    scatt.write_ldbd(str(L), p)
+    lines = (L / "ldbd_size.dat").read_text().split("\n")
+    lines[2] = " 6.40000000000000e+01 64 4 4 4 # nk_full nk kmesh"
+    (L / "ldbd_size.dat").write_text("\n".join(lines))
+    g = np.array([(a, b, c) for a in range(4) for b in range(4) for c in range(4)], np.float64) / 4
+    g.tofile(L / "ldbd_kvec.bin")
+    info = ldbd.read_info(str(L))
+    nk, nb = info.nk, info.nb_eph
+    rng = np.random.default_rng(31)
+    U = (rng.normal(size=(nk, info.nb_wannier, nb)) + 1j * rng.normal(size=(nk, info.nb_wannier, nb))) / 2
+    U.tofile(L / "ldbd_Umat.bin")
+    e = ldbd.read_ek(str(L), info)
+    kv = ldbd.read_kvec(str(L), info)
+    k1, k2 = ldbd.read_kpairs(str(L), info)
+    P1, P2 = ldbd.read_P(str(L), info)
+    s3 = ldbd.read_smat(str(L), info)
+    _, R = ldbd.read_R(str(L))
+    V = abs(np.linalg.det(R))
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    m = scatt.eimp_model("debye", f.sum() / (info.nk_full * V), T, V, scatt.reciprocal_metric(R), info.nb_wannier,
+                         n_ionized=1.0, epsilon_background=4.0, degauss=info.degauss, ethr=info.degauss * info.ndegauss,
+                         kmesh=(4, 4, 4), nk_full=info.nk_full).as_dict()
+    vq = np.array([oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)])
+    E1, E2 = oracle.ee_model_P(k1, k2, U, kv, e, f, vq, (4, 4, 4), info.degauss, info.degauss * info.ndegauss,
+                               info.nk_full, True)
+    scale_ee = float(f"{np.abs(P1).max() / np.abs(E1).max():.6g}")
+    S1, S2 = P1 + scale_ee * E1, P2 + scale_ee * E2
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n"
+                                f"eeMode = Pee_fixed_at_eq\nscale_ee = {scale_ee}\nscrMode = medium\n"
+                                "scrFormula = debye\nepsilon_background = 4\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1000:], r.stderr[-800:])
+    assert r.returncode == 0 and "model e-e (Pee_fixed_at_eq)" in r.stdout
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    _, obs = oracle.rk4(k1, k2, S1, S2, e, rho, 1 * FS, 4, prefac=2 * np.pi / info.nk_full, report_every=1,
+                        s_obs=s3[:, 2], f_dm=f, i0=0, i1=nb, obs_prefac=1.0 / (info.nk_full * V * BOHR2CM ** 3))
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", obs)
+    assert np.abs(rows[:, 1] - obs).max() < 1e-10 * np.abs(obs).max()
+    a = oracle.evolve(k1, k2, P1, P2, e, rho, prefac=1.0)
+    b = oracle.evolve(k1, k2, S1, S2, e, rho, prefac=1.0)
+    assert np.abs(a - b).max() > 0.05 * np.abs(b).max()  # the e-e term matters
