@@ -126,7 +126,8 @@ class EimpModel(ctypes.Structure):
                 ("prefac_vq_bare", ctypes.c_double), ("qscr2_debye", ctypes.c_double), ("fac0", ctypes.c_double),
                 ("fac2", ctypes.c_double), ("fac4", ctypes.c_double), ("degauss", ctypes.c_double),
                 ("ethr", ctypes.c_double), ("omegaL", ctypes.c_double), ("temperature", ctypes.c_double),
-                ("GGT", ctypes.c_double * 9), ("kmesh", ctypes.c_int * 3), ("nk_full", ctypes.c_double)]
+                ("GGT", ctypes.c_double * 9), ("kmesh", ctypes.c_int * 3), ("nk_full", ctypes.c_double),
+                ("exchange", ctypes.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

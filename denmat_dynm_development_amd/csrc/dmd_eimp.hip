@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *_
 }
 
 // Model e-e P at the equilibrium occupations (elecelec_model::calc_P with
-// dm = null, no exchange; ElecElec_Model.h:185-262): one thread per P element
+// dm = null; ElecElec_Model.h:185-262): one thread per P element
 // of a stored pair (k1, k2) and a loop over k3 with k4 = k1 + k3 - k2 looked
 // up in the mesh map.  With mee = v(k1-k2) <1|2>_{pq} <3|4>_{xy}, dl =
 // exp(pexp de^2) inside ethr (de = e1p - e2q + e3x - e4y), A2 = dl mee, A1 = A2
@@ -245,7 +245,8 @@ __global__ __launch_bounds__(256) void k_ee_model(int64_t n, const uint64_t *__r
     const int64_t a = (int64_t)k1[fp], b = (int64_t)k2[fp];
     const int nw = c.m.nb_wannier;
     double2 acc1 = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
-    if (a != b) {
+    const bool ex = c.m.exchange != 0;
+    if (a != b || ex) {
         const double v = vqp ? vqp[fp] : eimp_vq(c.m, kvec + 3 * a, kvec + 3 * b);
         const double2 *Ua = U + a * nw * NB, *Ub = U + b * nw * NB;
         double2 o13 = ee_ovlp<NB>(Ua, Ub, nw, i1, i3), o24 = ee_ovlp<NB>(Ua, Ub, nw, i2, i4),
@@ -260,9 +261,20 @@ __global__ __launch_bounds__(256) void k_ee_model(int64_t n, const uint64_t *__r
             const int y = ((ik3[3 * a + 1] + ik3[3 * c3 + 1] - ik3[3 * b + 1]) % m1 + m1) % m1;
             const int z = ((ik3[3 * a + 2] + ik3[3 * c3 + 2] - ik3[3 * b + 2]) % m2 + m2) % m2;
             const int c4i = kgrid[((int64_t)x * m1 + y) * m2 + z];
-            if (c4i == INT_MAX) continue;
+            if (c4i == INT_MAX || (ex && a == b && b == c3)) continue;
             const int64_t c4 = c4i;
             const double2 *U3 = U + c3 * nw * NB, *U4 = U + c4 * nw * NB;
+            // exchange (ElecElec_Model.h:69-75,114-149): mee_ex(p q x y) = <1p|v|4y> <3x|2q>
+            double2 o14[NB2], o32[NB2];
+            if (ex) {
+                const double v14 = eimp_vq(c.m, kvec + 3 * a, kvec + 3 * c4);
+                for (int i = 0; i < NB; i++)
+                    for (int j = 0; j < NB; j++) {
+                        const double2 o = ee_ovlp<NB>(Ua, U4, nw, i, j);
+                        o14[i * NB + j] = make_double2(o.x * v14, o.y * v14);
+                        o32[i * NB + j] = ee_ovlp<NB>(U3, Ub, nw, i, j);
+                    }
+            }
             const double *e3 = e + c3 * NB, *e4 = e + c4 * NB, *f3 = f + c3 * NB, *f4 = f + c4 * NB;
             for (int xx = 0; xx < NB; xx++)
                 for (int yy = 0; yy < NB; yy++) {
@@ -271,7 +283,17 @@ __global__ __launch_bounds__(256) void k_ee_model(int64_t n, const uint64_t *__r
                     auto dl = [&](double de) { return fabs(de) < c.m.ethr ? exp(c.prefac_exp * (de * de)) : 0.0; };
                     const double d13 = dl(ea[i1] - eb[i3] + d34), d24 = dl(ea[i2] - eb[i4] + d34),
                                  d31 = dl(ea[i3] - eb[i1] + d34), d42 = dl(ea[i4] - eb[i2] + d34);
-                    const double2 m13 = cmul(o13, o34), m24 = cmul(o24, o34), m31 = cmul(o31, o34), m42 = cmul(o42, o34);
+                    double2 m13 = cmul(o13, o34), m24 = cmul(o24, o34), m31 = cmul(o31, o34), m42 = cmul(o42, o34);
+                    if (ex) {  // mee = (mee - mee_ex) / 2
+                        auto anti = [&](double2 m, int pp, int qq) {
+                            const double2 x_ = cmul(o14[pp * NB + yy], o32[xx * NB + qq]);
+                            return make_double2(0.5 * m.x - 0.5 * x_.x, 0.5 * m.y - 0.5 * x_.y);
+                        };
+                        m13 = anti(m13, i1, i3);
+                        m24 = anti(m24, i2, i4);
+                        m31 = anti(m31, i3, i1);
+                        m42 = anti(m42, i4, i2);
+                    }
                     const double s13 = c.m.lindblad ? d13 : 1.0, s31 = c.m.lindblad ? d31 : 1.0;
                     const double w1 = (1 - f3[xx]) * f4[yy] * s13 * d24, w2 = f3[xx] * (1 - f4[yy]) * s31 * d42;
                     const double2 t1 = cmulc(m13, m24), t2 = cmulc(m31, m42);
@@ -496,6 +518,8 @@ extern "C" int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const 
     if (m->nb_wannier <= 0 || !(m->degauss > 0) || m->screening < DMD_SCR_UNSCREENED || m->screening > DMD_SCR_RPA ||
         m->detail_balance)
         return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: bad model (nb_wannier, degauss, screening, detail_balance)");
+    if (m->exchange && m->screening == DMD_SCR_RPA)
+        return fail(DMD_ERR_UNSUPPORTED, "the e-e exchange term is built for the closed-form screenings (v(k1 - k4))");
     const int *km = m->kmesh;
     if (km[0] <= 0 || km[1] <= 0 || km[2] <= 0 || (int64_t)km[0] * km[1] * km[2] > (1ll << 31))
         return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: the e-e sum needs the k mesh");
@@ -510,7 +534,8 @@ extern "C" int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const 
     c.m = *m;
     c.prefac_delta = 1.0;
     c.prefac_exp = m->lindblad ? -0.25 / std::pow(m->degauss, 2) : -0.5 / std::pow(m->degauss, 2);
-    const double pfac = 1. / (m->degauss * std::sqrt(2. * M_PI)) / m->nk_full;  // prefac_gauss / nk_full
+    // prefac_gauss / nk_full, doubled with the exchange term (ElecElec_Model.h:251-252)
+    const double pfac = (m->exchange ? 2.0 : 1.0) / (m->degauss * std::sqrt(2. * M_PI)) / m->nk_full;
     const int64_t per = h->nb4, nw = m->nb_wannier, nk = h->nk, cells = (int64_t)km[0] * km[1] * km[2];
     const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / per));
     std::vector<int> ik3((size_t)nk * 3);

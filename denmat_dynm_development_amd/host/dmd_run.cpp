@@ -80,7 +80,7 @@ struct Settings {
     std::vector<double> ni_Z;    // Z_impurity
     bool eimp_db = false;        // detailBalance of the model e-i term
     std::string eeMode = "none";
-    bool only_ee = false;
+    bool only_ee = false, ee_exchange = false;
     double scale_ee = 1, degauss_ee = 0;
     double degauss = 0, ndegauss = 0, degauss_eimp = 0;
     // screening (coulombParam, parameters.cpp:192-206)
@@ -194,7 +194,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     if (s.eeMode == "Pee_update")
         throw std::runtime_error("eeMode = Pee_update: the e-e P rebuilt from rho during the run is not carried by this "
                                  "operator (Pee_fixed_at_eq is)");
-    refuse(p, "ee_antisymmetry", 0, "the exchange term of e-e scattering");
+    s.ee_exchange = p.get("ee_antisymmetry", 0) != 0;
     s.degauss_ee = p.get("degauss_ee", p.get("degauss", info.degauss / eV, eV) / eV, eV);
     (void)p.get("freq_update_ee_model", 0);
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
@@ -290,6 +290,9 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     if (s.only_eimp && ee) throw std::runtime_error("if you want only e-i scattering, please set eeMode to be none");
     // under alg_linearize the e-e sum takes f_eq while RPA keeps summing f of mu
     // (ElectronPhonon_linear.cpp:19, Coulomb_Model.h:587-590); one f goes to the device
+    if (ee && s.ee_exchange && s.scrFormula == "RPA")
+        throw std::runtime_error("ee_antisymmetry with scrFormula = RPA: the exchange term is built for the "
+                                 "closed-form screenings");
     if (ee && s.linearize && s.scrFormula == "RPA")
         throw std::runtime_error("eeMode with alg_linearize and scrFormula = RPA: the e-e term here takes one set of "
                                  "occupations for both the screening and the sum");
@@ -531,8 +534,8 @@ int run(const std::string &folder, bool dry)
         std::printf("scrMode = %s scrFormula = %s epsilon_background = %lg model_impurities = %d detailBalance = %d "
                     "degauss_eimp = %.10le\n", s.scrMode.c_str(), s.scrFormula.c_str(), s.eps_background, n_model,
                     (int)s.eimp_db, s.degauss_eimp);
-        std::printf("eeMode = %s alg_only_ee = %d scale_ee = %lg degauss_ee = %.10le\n", s.eeMode.c_str(), (int)s.only_ee,
-                    s.scale_ee, s.degauss_ee);
+        std::printf("eeMode = %s alg_only_ee = %d scale_ee = %lg degauss_ee = %.10le ee_antisymmetry = %d\n",
+                    s.eeMode.c_str(), (int)s.only_ee, s.scale_ee, s.degauss_ee, (int)s.ee_exchange);
     }
     std::printf("alg_read_Bso = %d needL = %d scale_Ez = %lg alg_set_scv_zero = %d\n", (int)s.read_Bso, (int)s.needL,
                 s.scale_Ez, (int)s.set_scv_zero);
@@ -733,6 +736,7 @@ int run(const std::string &folder, bool dry)
             if (info.kmesh[d] <= 0) throw std::runtime_error("eeMode needs the k mesh of ldbd_size.dat (line 3)");
         dmd_eimp_model m = screened();
         m.degauss = s.degauss_ee;
+        m.exchange = s.ee_exchange;
         const std::vector<double> fee = eph_cols(s.linearize ? feq : fdm);
         std::printf("model e-e (%s): degauss_ee = %lg scale_ee = %lg\n", s.eeMode.c_str(), m.degauss, s.scale_ee);
         eph.add_ee_model(mk1.data(), mk2.data(), mU.data(), mkv.data(), fee.data(), m, s.scale_ee);

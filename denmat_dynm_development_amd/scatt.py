@@ -464,7 +464,7 @@ SCREENING = {"unscreened": _lib.DMD_SCR_UNSCREENED, "debye": _lib.DMD_SCR_DEBYE,
 
 def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0, n_ionized=None,
                epsilon_background=1.0, meff=1.0, degauss=None, ethr=None, scatt="lindblad",
-               detail_balance=False, kmesh=(0, 0, 0), nk_full=0.0):
+               detail_balance=False, kmesh=(0, 0, 0), nk_full=0.0, exchange=False):
     """The dmd_eimp_model of one ionized impurity (atomic units throughout):
     elecimp_model's prefactors (ElecImp_Model.h:22-40: prefac_A = Z sqrt(n_i
     V_cell), omegaL = 1e-4 T) and coulomb_model's static screening
@@ -510,6 +510,7 @@ def eimp_model(screening, n_free, temperature, cell_size, GGT, nb_wannier, Z=1.0
     for i in range(3):
         m.kmesh[i] = int(kmesh[i])
     m.nk_full = float(nk_full)
+    m.exchange = int(bool(exchange))  # e-e only: ee_antisymmetry
     return m
 
 

@@ -123,6 +123,7 @@ typedef struct {
     double GGT[9];        /* reciprocal metric (2 pi)^2 R^-1 R^-T                        */
     int kmesh[3];         /* k mesh of the k points (RPA and lindhard)                   */
     double nk_full;       /* k points of the full mesh (RPA and lindhard)                */
+    int exchange;         /* e-e only: the exchange term (ee_antisymmetry)               */
 } dmd_eimp_model;
 #define DMD_SCR_UNSCREENED 0
 #define DMD_SCR_DEBYE 1
@@ -145,7 +146,8 @@ int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t 
  * at the equilibrium occupations f (nk x nb, e-ph bands at mu) that set_eph
  * folds in through add_scatt_contrib("ee") (ElectronPhonon.cpp:10-11,44-113;
  * elecelec_model::calc_P with dm = null, ElecElec_Model.h:185-262), kept fixed
- * as eeMode = Pee_fixed_at_eq does; no exchange term (ee_antisymmetry = 0).
+ * as eeMode = Pee_fixed_at_eq does; m->exchange adds the exchange term of
+ * ee_antisymmetry (closed-form screenings only).
  * m: the screening as for dmd_scatt_add_eimp_model (unscreened, debye,
  * Bechstedt or RPA; prefac_A unused), degauss = degauss_ee, ethr, lindblad,
  * nb_wannier, kmesh (required: k4 = k1 + k3 - k2 is looked up on it), nk_full;

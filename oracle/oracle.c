@@ -810,7 +810,8 @@ int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64
 int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
                       const double *U, const double *kvec, const int *kmesh, const double *e, const double *f,
                       const double *vq_pair, int lindblad, double degauss, double ethr, double nk_full, double *P1,
-                      double *P2)
+                      double *P2, int antisym, const double *GGT, int screening, double prefac_vq, double prefac_vq_bare,
+                      double qscr2_debye, double fac0, double fac2, double fac4)
 {
     int nb2 = nb * nb, nb4 = nb2 * nb2;
     int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
@@ -825,12 +826,13 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
     zc *o12 = (zc *)malloc(sizeof(zc) * nb2), *o34 = (zc *)malloc(sizeof(zc) * nb2);
     zc *A1 = (zc *)malloc(sizeof(zc) * nb4), *A2 = (zc *)malloc(sizeof(zc) * nb4);
     zc *Ar = (zc *)malloc(sizeof(zc) * nb2), *Arb = (zc *)malloc(sizeof(zc) * nb2);
+    zc *o14 = (zc *)malloc(sizeof(zc) * nb2), *o32 = (zc *)malloc(sizeof(zc) * nb2);
     double *dl = (double *)malloc(sizeof(double) * nb4);
     for (int64_t p = 0; p < npairs; p++) {
         int64_t a = (int64_t)k1[p], b = (int64_t)k2[p];
         zc *Q1 = (zc *)P1 + p * nb4, *Q2 = (zc *)P2 + p * nb4;
         for (int q = 0; q < nb4; q++) Q1[q] = Q2[q] = zmk(0, 0);
-        if (a == b) continue;
+        if (a == b && !antisym) continue;
         const zc *Ua = (const zc *)U + a * nw * nb, *Ub = (const zc *)U + b * nw * nb;
         for (int i = 0; i < nb; i++)
             for (int j = 0; j < nb; j++) {
@@ -839,6 +841,7 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                 o12[i * nb + j] = zmul(o, zmk(vq_pair[p], 0));
             }
         for (int64_t c3 = 0; c3 < nk; c3++) {
+            if (antisym && a == b && b == c3) continue;
             double k4v[3];
             for (int d = 0; d < 3; d++) k4v[d] = kvec[3 * a + d] + kvec[3 * c3 + d] - kvec[3 * b + d];
             int64_t c4 = kmap[mesh_code(k4v, kmesh)];
@@ -861,8 +864,28 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                     for (int w = 0; w < nw; w++) o = zadd(o, zmul(zconj(U3[w * nb + i]), U4[w * nb + j]));
                     o34[i * nb + j] = o;
                 }
+            if (antisym) {  /* calc_ovlps' <1|v|4> and <3|2> (ElecElec_Model.h:69-75) */
+                double v14 = oracle_eimp_vq(screening, oracle_eimp_q2(GGT, kvec + 3 * a, kvec + 3 * c4), prefac_vq,
+                                            prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
+                const zc *U2 = (const zc *)U + b * nw * nb;
+                for (int i = 0; i < nb; i++)
+                    for (int j = 0; j < nb; j++) {
+                        zc o = zmk(0, 0), r = zmk(0, 0);
+                        for (int w = 0; w < nw; w++) {
+                            o = zadd(o, zmul(zconj(Ua[w * nb + i]), U4[w * nb + j]));
+                            r = zadd(r, zmul(zconj(U3[w * nb + i]), U2[w * nb + j]));
+                        }
+                        o14[i * nb + j] = zmul(o, zmk(v14, 0));
+                        o32[i * nb + j] = r;
+                    }
+            }
             for (int q = 0; q < nb4; q++) {
                 zc mee = zmul(o12[q / nb2], o34[q % nb2]);
+                if (antisym) {  /* mee = (mee - mee_ex) / 2, mee_ex = <1|v|4><3|2> (:114-149) */
+                    int i1 = q / (nb * nb2), i2 = (q / nb2) % nb, i3 = (q / nb) % nb, i4 = q % nb;
+                    zc ex = zmul(o14[i1 * nb + i4], o32[i3 * nb + i2]);
+                    mee = zadd(zmul(zmk(-0.5, 0), ex), zmul(zmk(0.5, 0), mee));
+                }
                 A2[q] = zscale(mee, dl[q]);
                 A1[q] = lindblad ? A2[q] : mee;
             }
@@ -884,8 +907,10 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                         }
                 }
         }
-        for (int q = 0; q < nb4; q++) { Q1[q] = zscale(Q1[q], pgauss / nk_full); Q2[q] = zscale(Q2[q], pgauss / nk_full); }
+        double pf = (antisym ? 2 : 1) * pgauss / nk_full;
+        for (int q = 0; q < nb4; q++) { Q1[q] = zscale(Q1[q], pf); Q2[q] = zscale(Q2[q], pf); }
     }
+    free(o14); free(o32);
     free(kmap); free(o12); free(o34); free(A1); free(A2); free(Ar); free(Arb); free(dl);
     return 0;
 }

@@ -50,7 +50,8 @@ def lib():
                                           [ctypes.c_int] * 3 + [ctypes.c_double] * 11 + [_d, _d, _d])
         L.oracle_ee_model_P.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d,
                                         ctypes.POINTER(ctypes.c_int), _d, _d, _d, ctypes.c_int, ctypes.c_double,
-                                        ctypes.c_double, ctypes.c_double, _d, _d]
+                                        ctypes.c_double, ctypes.c_double, _d, _d, ctypes.c_int, _d, ctypes.c_int] + \
+            [ctypes.c_double] * 6
         L.oracle_eimp_rpa_vq.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int), _d, _d,
                                          _d, ctypes.c_double, _d, ctypes.c_double, ctypes.c_double, _i64, _u64, _u64,
                                          _d, _d, ctypes.POINTER(ctypes.c_int64)]
@@ -271,9 +272,13 @@ def eimp_model_P(k1, k2, U, kvec, e, m, f=None):
     return P1, P2
 
 
-def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, lindblad=True):
-    """elecelec_model::calc_P at the equilibrium f, no exchange
-    (ElecElec_Model.h:185-262); vq_pair: v(k1 - k2) of each stored pair."""
+def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, lindblad=True, antisym_model=None):
+    """elecelec_model::calc_P at the equilibrium f (ElecElec_Model.h:185-262);
+    vq_pair: v(k1 - k2) of each stored pair.  antisym_model: a screening dict
+    (eimp_model fields) to include the exchange term (ee_antisymmetry), whose
+    v(k1 - k4) is evaluated from it."""
+    m = antisym_model or {"GGT": np.eye(3), "screening": 0, "prefac_vq": 0.0, "prefac_vq_bare": 0.0,
+                          "qscr2_debye": 0.0, "fac0": 0.0, "fac2": 0.0, "fac4": 0.0}
     e = _f64(e)
     nk, nb = e.shape
     U = _c128(U)
@@ -284,7 +289,9 @@ def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, li
     P2 = np.zeros_like(P1)
     lib().oracle_ee_model_P(nk, nb, U.shape[1], len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), km, _p(e),
                             _p(_f64(f)), _p(_f64(vq_pair)), int(lindblad), float(degauss), float(ethr),
-                            float(nk_full), _p(P1), _p(P2))
+                            float(nk_full), _p(P1), _p(P2), int(antisym_model is not None), _p(_f64(m["GGT"])),
+                            int(m["screening"]), float(m["prefac_vq"]), float(m["prefac_vq_bare"]),
+                            float(m["qscr2_debye"]), float(m["fac0"]), float(m["fac2"]), float(m["fac4"]))
     return P1, P2
 
 

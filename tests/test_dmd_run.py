@@ -100,7 +100,7 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
 @pytest.mark.parametrize("line,msg", [
     ("eeMode = Pee_update\nscrMode = medium", "rebuilt from rho"),
     ("eeMode = Pee_fixed_at_eq", "scrMode should not be none if considering electron-electron"),
-    ("eeMode = Pee_fixed_at_eq\nscrMode = medium\nee_antisymmetry = 1", "exchange term"),
+    ("eeMode = Pee_fixed_at_eq\nscrMode = medium\nee_antisymmetry = 1", "closed-form screenings"),
     ("alg_only_ee = 1\nalg_eph_enable = 0\nimpurity_density = 1e17\nimpMode = ab_neutral", "alg_only_ee is only possible"),
     ("eeMode = other", "eeMode must be none or Pee_fixed_at_eq"),
     ("alg_linearize_dPee = 1", "linearized e-e operator"),

@@ -80,3 +80,23 @@ def test_ee_P_adds_to_P_and_drives_evolve():
             rho = random_rho(nk, nb, rng)
             ref = oracle.evolve(k1, k2, S1, S2, e, rho, t=t_fs * FS, prefac=2 * np.pi / nk)
             assert rel_err(op.eval(t_fs * FS, rho), ref) < 1e-12
+
+
+def test_ee_P_with_exchange():
+    """ee_antisymmetry on the device against the oracle (Debye, nb = 2 and 3)."""
+    scatt, oracle = _mods()
+    for nb in (2, 3):
+        rng, km, kv, e, f, U, k1, k2 = _case(nb, 70 + nb)
+        nk = len(kv)
+        R = np.diag([7.0, 8.0, 9.0])
+        m = scatt.eimp_model("debye", 3e-6, 9.5e-4, abs(np.linalg.det(R)), scatt.reciprocal_metric(R), U.shape[1],
+                             n_ionized=1.0, epsilon_background=3.0, degauss=4e-4, ethr=1.2e-3, kmesh=km, nk_full=36.0,
+                             exchange=True)
+        vq = _vq(oracle, k1, k2, U, kv, e, f, m)
+        R1, R2 = oracle.ee_model_P(k1, k2, U, kv, e, f, vq, km, 4e-4, 1.2e-3, 36.0, True, antisym_model=m.as_dict())
+        print(f"case ee model exchange nb={nb}", flush=True)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+            op.add_ee_model(k1, k2, U, kv, f, m, add=False)
+            A1, A2 = op.download_P()
+        s_ = np.abs(R1).max()
+        assert s_ > 0 and np.abs(A1 - R1).max() <= 1e-12 * s_ and np.abs(A2 - R2).max() <= 1e-12 * s_

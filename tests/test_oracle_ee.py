@@ -75,3 +75,47 @@ def test_ee_P_reverse_pair_and_gram_form():
         assert np.abs(G - G.conj().T).max() <= 1e-13 * np.abs(G).max()
         ev = np.linalg.eigvalsh(0.5 * (G + G.conj().T))
         assert ev[0] >= -1e-12 * ev[-1]
+
+
+def test_ee_exchange_diagonal_and_reverse_pair():
+    """ee_antisymmetry: mee -> (mee - mee_ex) / 2 with mee_ex = v(k1-k4) <1|4><3|2>
+    and the prefactor doubled, k1 = k2 pairs kept (k3 = k1 skipped): the P
+    diagonal is 2 (1/nk_full) sum |(mee - mee_ex)/2|^2 (1-f3) f4 delta(de), and
+    P2(k1, k2) = conj(P1(k2, k1)) still holds."""
+    rng = np.random.default_rng(9)
+    km, kv, e, f, U, k1, k2 = _mesh_system(rng, nb=2, nw=3)
+    nk, nb = e.shape
+    R = np.diag([7.0, 8.0, 9.0])
+    from denmat_dynm_development_amd import scatt
+    m = scatt.eimp_model("debye", 2e-6, 9.5e-4, abs(np.linalg.det(R)), scatt.reciprocal_metric(R), 3,
+                         n_ionized=1.0, epsilon_background=3.0, degauss=4e-4, ethr=1.2e-3).as_dict()
+    vq = np.array([oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)])
+    sig, ethr, nkf = 4e-4, 1.2e-3, 30.0
+    P1, P2 = oracle.ee_model_P(k1, k2, U, kv, e, f, vq, km, sig, ethr, nkf, True, antisym_model=m)
+    P1r, _ = oracle.ee_model_P(k2, k1, U, kv, e, f, vq, km, sig, ethr, nkf, True, antisym_model=m)
+    assert np.abs(P2 - P1r.conj()).max() <= 1e-13 * np.abs(P1).max()
+    code = lambda x: tuple(np.rint(np.array(km) * (x - np.floor(x))).astype(int) % np.array(km))
+    cell = {}
+    for i in range(nk):
+        cell.setdefault(code(kv[i]), i)
+    g = lambda x: np.where(np.abs(x) < ethr, norm.pdf(x, scale=sig), 0.0)
+    vfun = lambda a, b: oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b]))
+    for p in range(0, len(k1), 7):
+        a, b = int(k1[p]), int(k2[p])
+        want = np.zeros((nb, nb))
+        for c3 in range(nk):
+            c4 = cell.get(code(kv[a] + kv[c3] - kv[b]))
+            if c4 is None or (a == b == c3):
+                continue
+            o12 = vq[p] * (U[a].conj().T @ U[b])
+            o34 = U[c3].conj().T @ U[c4]
+            o14 = vfun(a, c4) * (U[a].conj().T @ U[c4])
+            o32 = U[c3].conj().T @ U[b]
+            for n in range(nb):
+                for mm in range(nb):
+                    mee = o12[n, mm] * o34 - o14[n][None, :] * o32[:, mm][:, None]
+                    de = e[a, n] - e[b, mm] + e[c3][:, None] - e[c4][None, :]
+                    want[n, mm] += 2 * np.sum(np.abs(mee / 2) ** 2 * (1 - f[c3])[:, None] * f[c4][None, :] * g(de))
+        want /= nkf
+        diag = P1[p].reshape(nb, nb, nb, nb)[np.arange(nb), np.arange(nb)][:, np.arange(nb), np.arange(nb)]
+        assert np.allclose(diag.real, want, rtol=1e-11, atol=1e-14 * max(want.max(), 1e-300))
