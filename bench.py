@@ -19,7 +19,7 @@ spin lifetime of the same runs compared.
   python bench.py --mode adjoint    # forward + adjoint RK4 with 7 parameters, nb = 2, nk = 16384
   python bench.py --mode linear     # the linearized operator (set_Lsc + evolve_linear), nb = 2 and nb = 4
   python bench.py --mode ode        # the adaptive rkf45 driver and compute_rhs against fixed-step RK4, nb = 2
-  python bench.py --mode eimp       # model e-i P generated on the device (Debye and static RPA), nb = 2
+  python bench.py --mode eimp       # model e-i P (Debye, static RPA) and e-e P generated on the device, nb = 2
 """
 import argparse
 import json
@@ -499,6 +499,31 @@ def mode_eimp(args, torch, scatt):
         r = {"wall_s": el, "pairs_per_s": len(k1) / el, "P_written_GB": wb / 1e9, "P_GBps_wall": wb / el / 1e9}
         out["results"][scr] = r
         log(f"eimp {scr}: {r}")
+    op.close()
+    # the model e-e P at equilibrium: a sum over k3 per pair (O(pairs x nk x nb^6)); 100,000 pairs of the
+    # nb = 2, nk = 4096 workload on a 16 x 16 x 16 mesh
+    p1 = scatt.synth_params(1)
+    e1 = scatt.synth_energies(p1)
+    a1, b1 = scatt.synth_kpairs(p1)
+    a1, b1 = a1[:100000], b1[:100000]
+    km1 = (16, 16, 16)
+    g1 = np.stack(np.meshgrid(*[np.arange(n) for n in km1], indexing="ij"), -1).reshape(-1, 3) / 16.0
+    U1 = (rng.normal(size=(p1.nk, nw, p1.nb)) + 1j * rng.normal(size=(p1.nk, nw, p1.nb))) / 2
+    f1 = 1 / (np.exp((e1 - (e1.min() + 0.02 * (e1.max() - e1.min()))) / p1.temperature) + 1)
+    m = scatt.eimp_model("debye", f1.sum() / (p1.nk * V), p1.temperature, V, scatt.reciprocal_metric(R), nw,
+                         n_ionized=1.0, epsilon_background=12.9, degauss=p1.degauss, ethr=1e-3, kmesh=km1,
+                         nk_full=float(p1.nk))
+    op = scatt.ScatteringOperator(p1.nk, p1.nb, a1, b1, e1, nk_full=p1.nk)
+    op.add_ee_model(a1, b1, U1, g1, f1, m, add=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op.add_ee_model(a1, b1, U1, g1, f1, m, add=False)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    terms = len(a1) * p1.nk * p1.nb ** 6
+    out["results"]["ee_debye"] = {"workload": f"nb={p1.nb} nk={p1.nk} (16^3 mesh), {len(a1)} pairs", "wall_s": el,
+                                  "pairs_per_s": len(a1) / el, "pair_k3_terms_per_s": terms / el}
+    log(f"ee debye: {out['results']['ee_debye']}")
     op.close()
     return out
 
