@@ -48,60 +48,65 @@ def log(msg):
 
 
 def initial_state(p, e, s3, axis=2, amp=1e-3):
-    """rho0 = f + (df/de) b S_axis (first order, electron.cpp:62-98) with b set
-    so that Tr(S_axis drho) / sum Tr f = amp; mu = CBM - 0.1 eV."""
-    nk, nb = e.shape
-    T = p.temperature
-    mu = -0.1 / 27.21138505
-    f = 1.0 / (np.exp((e - mu) / T) + 1.0)
-    sa = s3[:, axis].reshape(nk, nb, nb)
-    de = e[:, :, None] - e[:, None, :]
-    fd = f[:, :, None] - f[:, None, :]
-    deg = np.abs(de) < 1e-8
-    favg = 0.5 * (f[:, :, None] + f[:, None, :])
-    dfde = np.where(deg, favg * (favg - 1) / T, fd / np.where(deg, 1.0, de))
-    d1 = dfde * sa  # drho for b = 1
-    s_per_b = np.real(np.einsum("kij,kji->", sa, d1))
-    b = amp * f.sum() / s_per_b
-    rho = (np.einsum("ki,ij->kij", f, np.eye(nb)) + b * d1).reshape(nk, nb * nb)
-    return rho.astype(np.complex128), f
+    from denmat_dynm_development_amd import scatt
+    return scatt.synth_initial_state(p, e, s3, axis, amp)
 
 
 def batch_states(p, e, s3, B):
-    """B initial states: spin along z, x, y in turn, imbalance amplitudes
-    log-uniform in [1e-4, 1e-2] (trajectory 0: z, 1e-3)."""
-    states = []
-    f = None
-    for b in range(B):
-        axis = (2, 0, 1)[b % 3]
-        amp = 1e-3 if b == 0 else 10.0 ** (-4 + 2 * ((b * 0.6180339887) % 1.0))
-        rho, f = initial_state(p, e, s3, axis, amp)
-        states.append(rho)
-    return np.stack(states), f
+    from denmat_dynm_development_amd import scatt
+    return scatt.synth_batch_states(p, e, s3, B)
 
 
-def cpu_baseline(p, e, k1, k2, rho, budget_s=12.0):
-    """Oracle (oracle/liboracle.so, OpenMP over pairs) on the first rows of the
-    same workload; returns pairs/s, threads used and the sample description."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, e, k1, k2, rho, chunk=2_000_000):
+    """The oracle (oracle/liboracle.so: the reference's evolve loop with
+    per-element compute_Pt phases, OpenMP over pairs) timed on this host.
+
+    - All threads: ONE full right-hand-side evaluation of the workload, every
+      stored pair, streamed in chunks of `chunk` pairs (the operator is a sum
+      over pairs, so the chunks' times add up to one evaluation's); P of each
+      chunk is generated untimed.  The per-evaluation rate is extrapolated to
+      the run's 4 x steps evaluations, and labelled so.
+    - One core: 3 evaluations of the first 1 M stored pairs.
+    Threads: the OpenMP default of this process (OMP_NUM_THREADS, which is
+    this host's CPU share per GPU), next to nproc and the CPU model."""
     from denmat_dynm_development_amd import scatt
     from oracle import oracle
-    nthreads = max(1, min(os.cpu_count() or 1, 16))
-    n = min(len(k1), 2_000_000)
-    # the first n stored pairs close over rows [0, k1[n-1]]
-    P1, P2 = scatt.synth_P(p, e, k1[:n], k2[:n], nthreads)
+    nthreads = max(1, oracle.max_threads())
     prefac = 2 * np.pi / p.nk
+    t_eval = 0.0
+    n = len(k1)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        P1, P2 = scatt.synth_P(p, e, k1[a:b], k2[a:b], nthreads)
+        t0 = time.perf_counter()
+        oracle.evolve(k1[a:b], k2[a:b], P1, P2, e, rho, t=17 * FS, prefac=prefac, nthreads=nthreads)
+        t_eval += time.perf_counter() - t0
+        del P1, P2
+    n1 = min(n, 1_000_000)
+    P1, P2 = scatt.synth_P(p, e, k1[:n1], k2[:n1], nthreads)
     t0 = time.perf_counter()
-    evals = 0
-    while True:
-        oracle.evolve(k1[:n], k2[:n], P1, P2, e, rho, t=evals * FS, prefac=prefac, nthreads=nthreads)
-        evals += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or evals >= 50:
-            break
-    rate = n * evals / el
-    return {"value": rate, "unit": "pairs/s", "cores": nthreads, "kind": "port",
-            "sample": f"{evals} RHS evaluations over the first {n} stored pairs of the workload "
-                      f"(oracle evolve, per-element phases, OpenMP over pairs), {el:.1f} s"}
+    for r in range(3):
+        oracle.evolve(k1[:n1], k2[:n1], P1, P2, e, rho, t=(17 + r) * FS, prefac=prefac, nthreads=1)
+    t_one = time.perf_counter() - t0
+    return {"value": n / t_eval, "unit": "pairs/s", "cores": nthreads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "one_core": {"value": 3 * n1 / t_one, "unit": "pairs/s", "cores": 1,
+                         "sample": f"3 evaluations of the first {n1} stored pairs, {t_one:.1f} s"},
+            "eval_s": t_eval,
+            "sample": f"one full right-hand-side evaluation of the workload ({n} stored pairs, in chunks of "
+                      f"{chunk}) on {nthreads} threads, {t_eval:.2f} s; oracle evolve with per-element phases",
+            "extrapolated": "the rate of one full evaluation stands for the run's evaluations (not timed in full)"}
 
 
 def alg_bytes(n_pairs, nk, nb, esize=16, extra=0):
@@ -242,8 +247,10 @@ def bench_mixed_batched(args, torch, scatt):
     log(f"fp64: {el / K * 1e3:.1f} ms/step, kernel {kms:.2f} ms ({ab / kms / 1e6:.0f} GB/s), tau {tau64:.6g} ps")
     op.close()
     base = K / el
+    # the batched pass stores every visit's matrices as BF16 hi + lo words, so the
+    # complex64 form runs on P rounded to a 16-bit significand (exact for BF16 P)
     for prec in (1, 2):
-        name = PREC_NAME[prec]
+        name = {1: "c64_P_as_bf16_hi_lo", 2: "bf16"}[prec]
         try:
             op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk, precision=prec)
             op.generate_P(p)

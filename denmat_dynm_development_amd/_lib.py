@@ -164,8 +164,12 @@ PROTOTYPES = {
     "dmd_scatt_upload_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_add_P": [_vp, _vp, _vp, _d],
     "dmd_scatt_download_P": [_vp, _i64, _i64, _vp, _vp],
+    "dmd_scatt_copy_P": [_vp, _vp, _d, _i],
     "dmd_scatt_add_eimp_model": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
     "dmd_scatt_add_ee_model": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
+    "dmd_scatt_add_ee_model_state": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
+    "dmd_scatt_ee_vq_cells": [_vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _vp],
+    "dmd_scatt_add_Lsc_ee": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _d, _d, _i],
     "dmd_scatt_generate_P": [_vp, _pSP],
     "dmd_scatt_set_flags": [_vp, _i, _i, _i],
     "dmd_scatt_set_ddmdt_eq": [_vp, _vp],

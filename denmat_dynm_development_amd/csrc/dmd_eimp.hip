@@ -26,35 +26,9 @@
 #include <vector>
 
 #include "dmd_internal.h"
+#include "dmd_model.h"
 
 namespace dmd {
-
-struct EimpConst {
-    dmd_eimp_model m;
-    double prefac_delta, prefac_exp;
-};
-
-// |wrap(ka - kb)|^2 under GGT (comm/kmap.h:6-15, matrix3.h:119-122), then
-// the static v(q) of the chosen formula
-__device__ double eimp_vq(const dmd_eimp_model &m, const double *ka, const double *kb)
-{
-    double v[3];
-    for (int d = 0; d < 3; d++) {
-        v[d] = ka[d] - kb[d];
-        v[d] -= floor(0.5 + v[d]);
-        if (fabs(v[d] - 0.5) < 1e-6) v[d] = -0.5;
-    }
-    const double *G = m.GGT;
-    const double q2 = v[0] * v[0] * G[0] + v[1] * v[1] * G[4] + v[2] * v[2] * G[8] +
-                      2 * (v[0] * v[1] * G[1] + v[0] * v[2] * G[2] + v[1] * v[2] * G[5]);
-    if (m.screening == DMD_SCR_DEBYE) return m.prefac_vq / (m.qscr2_debye + q2);
-    if (q2 < 1e-20) return 0.0;  // Gamma is skipped by the unscreened and Bechstedt forms
-    if (m.screening == DMD_SCR_BECHSTEDT) {
-        const double epsq = 1 + 1. / (m.fac0 + m.fac2 * q2 + m.fac4 * q2 * q2);
-        return m.prefac_vq_bare / epsq / q2;
-    }
-    return m.prefac_vq / q2;
-}
 
 // The A entries of one (b1, b2): without detailed balance a[0] = A1, a[1] = A2;
 // with it a[0..7] = A1pp, A1pm, A2pp, A2pm, A1mp, A1mm, A2mp, A2mm.
@@ -152,17 +126,6 @@ __global__ __launch_bounds__(256) void k_rpa_qscr2(int64_t nq, const int *__rest
         __syncthreads();
     }
 }
-__global__ void k_fill_int(int64_t n, int *__restrict__ a, int v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
-}
-__global__ void k_kgrid(int64_t nk, const int *__restrict__ ik3, int m1, int m2, int *__restrict__ kgrid)
-{
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nk) atomicMin(kgrid + ((int64_t)ik3[3 * k] * m1 + ik3[3 * k + 1]) * m2 + ik3[3 * k + 2], (int)k);
-}
-
 // P1, P2 of n stored pairs (k1[f], k2[f]) in the ldbd_P element order; vq,
 // when given, holds v(q) of every pair of the chunk (the k-summed screenings).
 // A workgroup takes PP = 256 / NB^4 pairs (3 for NB = 3): first one thread per
@@ -210,118 +173,11 @@ __global__ __launch_bounds__(256) void k_eimp_model(int64_t n, const uint64_t *_
     P2[f * NB4 + el] = p2;
 }
 
-// Model e-e P at the equilibrium occupations (elecelec_model::calc_P with
-// dm = null; ElecElec_Model.h:185-262): one thread per P element
-// of a stored pair (k1, k2) and a loop over k3 with k4 = k1 + k3 - k2 looked
-// up in the mesh map.  With mee = v(k1-k2) <1|2>_{pq} <3|4>_{xy}, dl =
-// exp(pexp de^2) inside ethr (de = e1p - e2q + e3x - e4y), A2 = dl mee, A1 = A2
-// (lindblad) or mee:
-//   P1[(i1 i2),(i3 i4)] += sum_xy (1-f3x) f4y A1(i1 i3 x y) conj(A2(i2 i4 x y))
-//   P2[(i1 i2),(i3 i4)] += sum_xy f3x (1-f4y) A1(i3 i1 x y) conj(A2(i4 i2 x y))
-// scaled by prefac_gauss / nk_full.  Pairs with k1 = k2 get 0.  Each thread
-// forms the few overlaps it needs in registers (the operator is built once).
-template <int NB>
-__device__ double2 ee_ovlp(const double2 *__restrict__ Ua, const double2 *__restrict__ Ub, int nw, int p, int q)
-{
-    double2 o = make_double2(0.0, 0.0);
-    for (int w = 0; w < nw; w++) o = cadd(o, cmulc(Ub[w * NB + q], Ua[w * NB + p]));
-    return o;
-}
-template <int NB>
-__global__ __launch_bounds__(256) void k_ee_model(int64_t n, const uint64_t *__restrict__ k1,
-                                                  const uint64_t *__restrict__ k2, const double2 *__restrict__ U,
-                                                  const int *__restrict__ ik3, const int *__restrict__ kgrid, int m0,
-                                                  int m1, int m2, int64_t nk, const double *__restrict__ e,
-                                                  const double *__restrict__ f, const double *__restrict__ vqp,
-                                                  const double *__restrict__ kvec, EimpConst c, double pfac,
-                                                  double2 *__restrict__ P1, double2 *__restrict__ P2)
-{
-    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * NB4) return;
-    const int64_t fp = idx / NB4;
-    const int el = (int)(idx - fp * NB4);
-    const int i1 = el / (NB * NB2), i2 = (el / NB2) % NB, i3 = (el / NB) % NB, i4 = el % NB;
-    const int64_t a = (int64_t)k1[fp], b = (int64_t)k2[fp];
-    const int nw = c.m.nb_wannier;
-    double2 acc1 = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
-    const bool ex = c.m.exchange != 0;
-    if (a != b || ex) {
-        const double v = vqp ? vqp[fp] : eimp_vq(c.m, kvec + 3 * a, kvec + 3 * b);
-        const double2 *Ua = U + a * nw * NB, *Ub = U + b * nw * NB;
-        double2 o13 = ee_ovlp<NB>(Ua, Ub, nw, i1, i3), o24 = ee_ovlp<NB>(Ua, Ub, nw, i2, i4),
-                o31 = ee_ovlp<NB>(Ua, Ub, nw, i3, i1), o42 = ee_ovlp<NB>(Ua, Ub, nw, i4, i2);
-        o13 = make_double2(o13.x * v, o13.y * v);
-        o24 = make_double2(o24.x * v, o24.y * v);
-        o31 = make_double2(o31.x * v, o31.y * v);
-        o42 = make_double2(o42.x * v, o42.y * v);
-        const double *ea = e + a * NB, *eb = e + b * NB;
-        for (int64_t c3 = 0; c3 < nk; c3++) {
-            const int x = ((ik3[3 * a] + ik3[3 * c3] - ik3[3 * b]) % m0 + m0) % m0;
-            const int y = ((ik3[3 * a + 1] + ik3[3 * c3 + 1] - ik3[3 * b + 1]) % m1 + m1) % m1;
-            const int z = ((ik3[3 * a + 2] + ik3[3 * c3 + 2] - ik3[3 * b + 2]) % m2 + m2) % m2;
-            const int c4i = kgrid[((int64_t)x * m1 + y) * m2 + z];
-            if (c4i == INT_MAX || (ex && a == b && b == c3)) continue;
-            const int64_t c4 = c4i;
-            const double2 *U3 = U + c3 * nw * NB, *U4 = U + c4 * nw * NB;
-            // exchange (ElecElec_Model.h:69-75,114-149): mee_ex(p q x y) = <1p|v|4y> <3x|2q>
-            double2 o14[NB2], o32[NB2];
-            if (ex) {
-                const double v14 = eimp_vq(c.m, kvec + 3 * a, kvec + 3 * c4);
-                for (int i = 0; i < NB; i++)
-                    for (int j = 0; j < NB; j++) {
-                        const double2 o = ee_ovlp<NB>(Ua, U4, nw, i, j);
-                        o14[i * NB + j] = make_double2(o.x * v14, o.y * v14);
-                        o32[i * NB + j] = ee_ovlp<NB>(U3, Ub, nw, i, j);
-                    }
-            }
-            const double *e3 = e + c3 * NB, *e4 = e + c4 * NB, *f3 = f + c3 * NB, *f4 = f + c4 * NB;
-            for (int xx = 0; xx < NB; xx++)
-                for (int yy = 0; yy < NB; yy++) {
-                    const double2 o34 = ee_ovlp<NB>(U3, U4, nw, xx, yy);
-                    const double d34 = e3[xx] - e4[yy];
-                    auto dl = [&](double de) { return fabs(de) < c.m.ethr ? exp(c.prefac_exp * (de * de)) : 0.0; };
-                    const double d13 = dl(ea[i1] - eb[i3] + d34), d24 = dl(ea[i2] - eb[i4] + d34),
-                                 d31 = dl(ea[i3] - eb[i1] + d34), d42 = dl(ea[i4] - eb[i2] + d34);
-                    double2 m13 = cmul(o13, o34), m24 = cmul(o24, o34), m31 = cmul(o31, o34), m42 = cmul(o42, o34);
-                    if (ex) {  // mee = (mee - mee_ex) / 2
-                        auto anti = [&](double2 m, int pp, int qq) {
-                            const double2 x_ = cmul(o14[pp * NB + yy], o32[xx * NB + qq]);
-                            return make_double2(0.5 * m.x - 0.5 * x_.x, 0.5 * m.y - 0.5 * x_.y);
-                        };
-                        m13 = anti(m13, i1, i3);
-                        m24 = anti(m24, i2, i4);
-                        m31 = anti(m31, i3, i1);
-                        m42 = anti(m42, i4, i2);
-                    }
-                    const double s13 = c.m.lindblad ? d13 : 1.0, s31 = c.m.lindblad ? d31 : 1.0;
-                    const double w1 = (1 - f3[xx]) * f4[yy] * s13 * d24, w2 = f3[xx] * (1 - f4[yy]) * s31 * d42;
-                    const double2 t1 = cmulc(m13, m24), t2 = cmulc(m31, m42);
-                    acc1 = make_double2(acc1.x + w1 * t1.x, acc1.y + w1 * t1.y);
-                    acc2 = make_double2(acc2.x + w2 * t2.x, acc2.y + w2 * t2.y);
-                }
-        }
-    }
-    P1[idx] = make_double2(acc1.x * pfac, acc1.y * pfac);
-    P2[idx] = make_double2(acc2.x * pfac, acc2.y * pfac);
-}
-
-// kIndexMap::ikvec3 / qIndexMap::iqvec3 (comm/kmap.cpp): the mesh cell of a
-// k (or q) in crystal coordinates, wrapped into [0, 1) first
-static void mesh_cell(const double *k, const int *km, int *v)
-{
-    for (int d = 0; d < 3; d++) {
-        const double x = k[d] - std::floor(k[d]);
-        v[d] = (int)std::round(km[d] * x);
-        if (v[d] == km[d]) v[d] = 0;
-    }
-}
-
 // v(q) of every stored pair for the k-summed screenings: coulomb_model::vq
 // (Coulomb_Model.h:632-683) with vq_RPA = prefac_vq / (|q|^2 + q_scr^2(q))
 // (:820-828); q_scr^2 is summed on the device for each distinct q of the
 // stored pairs only.  Host vectors in, host vector out.
-static int rpa_vq(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double2 *dU, const double *kvec,
+int rpa_vq(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double2 *dU, const double *kvec,
                   const double *f, const dmd_eimp_model &m, std::vector<double> &vq)
 {
     const int *km = m.kmesh;
@@ -422,6 +278,99 @@ static int rpa_vq(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, cons
     return DMD_OK;
 }
 
+// First (ik, jk) in (ik, jk) order whose k difference falls in each mesh cell
+// (qIndexMap::build over all (ik, jk), comm/kmap.h:43-55): code ik * nk + jk.
+// Grid: x over jk, y over ik (strided).
+__global__ __launch_bounds__(256) void k_qcells_first(int64_t nk, const int *__restrict__ ik3, int m0, int m1, int m2,
+                                                      unsigned long long *__restrict__ first)
+{
+    const int64_t jk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jk >= nk) return;
+    for (int64_t ik = blockIdx.y; ik < nk; ik += gridDim.y) {
+        const int x = ((ik3[3 * ik] - ik3[3 * jk]) % m0 + m0) % m0, y = ((ik3[3 * ik + 1] - ik3[3 * jk + 1]) % m1 + m1) % m1,
+                  z = ((ik3[3 * ik + 2] - ik3[3 * jk + 2]) % m2 + m2) % m2;
+        atomicMin(first + ((int64_t)x * m1 + y) * m2 + z, (unsigned long long)(ik * nk + jk));
+    }
+}
+__global__ void k_fill_u64(int64_t n, unsigned long long *__restrict__ a, unsigned long long v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+int rpa_vq_cells(dmd_scatt *h, const double2 *dU, const double *kvec, const double *f, const dmd_eimp_model &m,
+                 std::vector<double> &vq_cell)
+{
+    const int *km = m.kmesh;
+    if (km[0] <= 0 || km[1] <= 0 || km[2] <= 0) return fail(DMD_ERR_ARG, "RPA screening needs the k mesh");
+    if (!f) return fail(DMD_ERR_ARG, "RPA screening needs the occupations f");
+    const int64_t cells = (int64_t)km[0] * km[1] * km[2], nk = h->nk;
+    if (cells > (1ll << 26)) return fail(DMD_ERR_UNSUPPORTED, "k mesh too large for the per-cell RPA table");
+    std::vector<int> ik3((size_t)nk * 3);
+    for (int64_t k = 0; k < nk; k++) mesh_cell(kvec + 3 * k, km, &ik3[3 * k]);
+    int *d_ik3 = nullptr, *d_q3 = nullptr, *d_grid = nullptr;
+    double *d_f = nullptr, *d_sum = nullptr;
+    unsigned long long *d_first = nullptr;
+    int rc;
+    auto release = [&]() { dfree(d_ik3); dfree(d_q3); dfree(d_grid); dfree(d_f); dfree(d_sum); dfree(d_first); };
+    if ((rc = dalloc_t(&d_ik3, (size_t)nk * 3, "k cells")) || (rc = dalloc_t(&d_grid, (size_t)cells, "k mesh map")) ||
+        (rc = dalloc_t(&d_f, (size_t)nk * h->nb, "f")) || (rc = dalloc_t(&d_first, (size_t)cells, "q cells"))) {
+        release();
+        return rc;
+    }
+    std::vector<unsigned long long> first((size_t)cells);
+    hipError_t e = hipMemcpyAsync(d_ik3, ik3.data(), ik3.size() * 4, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_f, f, (size_t)nk * h->nb * 8, hipMemcpyDefault, h->st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(nblk(cells, 256)), dim3(256), 0, h->st, cells, d_first, ~0ull);
+        const dim3 g(nblk(nk, 256), (unsigned)std::min<int64_t>(nk, 65535));
+        hipLaunchKernelGGL(k_qcells_first, g, dim3(256), 0, h->st, nk, d_ik3, km[0], km[1], km[2], d_first);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(first.data(), d_first, (size_t)cells * 8, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    if (e != hipSuccess) {
+        release();
+        return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    }
+    std::vector<int> q3;
+    std::vector<int64_t> used;
+    for (int64_t c = 0; c < cells; c++)
+        if (first[c] != ~0ull) {
+            used.push_back(c);
+            q3.push_back((int)(c / ((int64_t)km[1] * km[2])));
+            q3.push_back((int)((c / km[2]) % km[1]));
+            q3.push_back((int)(c % km[2]));
+        }
+    const int64_t nq = (int64_t)used.size();
+    std::vector<double> sum((size_t)nq, 0.0);
+    if ((rc = dalloc_t(&d_q3, (size_t)nq * 3, "q cells")) || (rc = dalloc_t(&d_sum, (size_t)nq, "q_scr^2"))) {
+        release();
+        return rc;
+    }
+    e = hipMemcpyAsync(d_q3, q3.data(), q3.size() * 4, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sum, sum.data(), (size_t)nq * 8, hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, h->st, cells, d_grid, INT_MAX);
+        hipLaunchKernelGGL(k_kgrid, dim3(nblk(nk, 256)), dim3(256), 0, h->st, nk, d_ik3, km[1], km[2], d_grid);
+        const dim3 g(nblk(nk, 256), (unsigned)std::min<int64_t>(nq, 65535));
+        hipLaunchKernelGGL(k_rpa_qscr2, g, dim3(256), 0, h->st, nq, d_q3, nk, d_ik3, d_grid, km[0], km[1], km[2], dU,
+                           m.nb_wannier, h->nb, h->d_e, d_f, m.temperature, 0, d_sum);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(sum.data(), d_sum, (size_t)nq * 8, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    release();
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    vq_cell.assign((size_t)cells, 0.0);
+    for (int64_t u = 0; u < nq; u++) {
+        const int64_t code = (int64_t)first[used[u]], ik = code / nk, jk = code % nk;
+        const double q2 = model_q2(m.GGT, kvec + 3 * ik, kvec + 3 * jk);
+        vq_cell[used[u]] = q2 < 1e-20 ? 0.0 : m.prefac_vq / (q2 + m.prefac_vq / m.nk_full * sum[u]);
+    }
+    return DMD_OK;
+}
+
 }  // namespace dmd
 
 using namespace dmd;
@@ -505,105 +454,3 @@ extern "C" int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, cons
     return DMD_OK;
 }
 
-// The e-e fold-in of set_eph, add_scatt_contrib("ee") (ElectronPhonon.cpp:10-11,
-// 44-113) for eeMode = Pee_fixed_at_eq: P += scale * P_ee(f) with P_ee from
-// k_ee_model; m carries the screening (as for the e-i term; prefac_A unused),
-// degauss = degauss_ee, ethr, lindblad, nb_wannier, kmesh, nk_full.
-extern "C" int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
-                                      const double *kvec, const double *f, const dmd_eimp_model *m, double scale,
-                                      int add)
-{
-    if (!h || !m || (h->np > 0 && (!k1st || !k2nd || !U || !kvec || !f)))
-        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: null argument");
-    if (m->nb_wannier <= 0 || !(m->degauss > 0) || m->screening < DMD_SCR_UNSCREENED || m->screening > DMD_SCR_RPA ||
-        m->detail_balance)
-        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: bad model (nb_wannier, degauss, screening, detail_balance)");
-    if (m->exchange && m->screening == DMD_SCR_RPA)
-        return fail(DMD_ERR_UNSUPPORTED, "the e-e exchange term is built for the closed-form screenings (v(k1 - k4))");
-    const int *km = m->kmesh;
-    if (km[0] <= 0 || km[1] <= 0 || km[2] <= 0 || (int64_t)km[0] * km[1] * km[2] > (1ll << 31))
-        return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: the e-e sum needs the k mesh");
-    if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator: add the e-e P before set_Lsc");
-    if (add && !h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated (add = 0 sets it)");
-    if (h->nb < 1 || h->nb > 4) return fail(DMD_ERR_UNSUPPORTED, "the model e-e P is built for nb = 1..4");
-    for (int64_t q = 0; q < h->np; q++)
-        if (k1st[q] >= (uint64_t)h->nk || k2nd[q] >= (uint64_t)h->nk)
-            return fail(DMD_ERR_ARG, "dmd_scatt_add_ee_model: k index outside [0, nk)");
-    if (h->np == 0) { h->has_P = true; return DMD_OK; }
-    EimpConst c;
-    c.m = *m;
-    c.prefac_delta = 1.0;
-    c.prefac_exp = m->lindblad ? -0.25 / std::pow(m->degauss, 2) : -0.5 / std::pow(m->degauss, 2);
-    // prefac_gauss / nk_full, doubled with the exchange term (ElecElec_Model.h:251-252)
-    const double pfac = (m->exchange ? 2.0 : 1.0) / (m->degauss * std::sqrt(2. * M_PI)) / m->nk_full;
-    const int64_t per = h->nb4, nw = m->nb_wannier, nk = h->nk, cells = (int64_t)km[0] * km[1] * km[2];
-    const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / per));
-    std::vector<int> ik3((size_t)nk * 3);
-    for (int64_t k = 0; k < nk; k++) mesh_cell(kvec + 3 * k, km, &ik3[3 * k]);
-    double2 *dU = nullptr, *s1 = nullptr, *s2 = nullptr;
-    double *dk = nullptr, *dvq = nullptr, *df = nullptr;
-    uint64_t *d1 = nullptr, *d2 = nullptr;
-    int *d_ik3 = nullptr, *d_grid = nullptr;
-    int rc;
-    auto release = [&]() {
-        dfree(dU); dfree(dk); dfree(s1); dfree(s2); dfree(d1); dfree(d2); dfree(dvq); dfree(df); dfree(d_ik3); dfree(d_grid);
-    };
-    const bool ksum = m->screening == DMD_SCR_RPA;
-    std::vector<double> vq;
-    if ((rc = dalloc_t(&dU, (size_t)(nk * nw * h->nb), "U")) || (rc = dalloc_t(&dk, (size_t)nk * 3, "kvec")) ||
-        (rc = dalloc_t(&df, (size_t)nk * h->nb, "f")) || (rc = dalloc_t(&d_ik3, (size_t)nk * 3, "k cells")) ||
-        (rc = dalloc_t(&d_grid, (size_t)cells, "k mesh map")) ||
-        (rc = dalloc_t(&s1, (size_t)(chunk * per), "e-e P staging")) ||
-        (rc = dalloc_t(&s2, (size_t)(chunk * per), "e-e P staging")) || (rc = dalloc_t(&d1, (size_t)chunk, "k1")) ||
-        (rc = dalloc_t(&d2, (size_t)chunk, "k2"))) {
-        release();
-        return rc;
-    }
-    hipError_t e = hipMemcpyAsync(dU, U, (size_t)(nk * nw * h->nb) * 16, hipMemcpyDefault, h->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(dk, kvec, (size_t)nk * 3 * 8, hipMemcpyDefault, h->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(df, f, (size_t)nk * h->nb * 8, hipMemcpyDefault, h->st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_ik3, ik3.data(), ik3.size() * 4, hipMemcpyHostToDevice, h->st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_fill_int, dim3(nblk(cells, 256)), dim3(256), 0, h->st, cells, d_grid, INT_MAX);
-        hipLaunchKernelGGL(k_kgrid, dim3(nblk(nk, 256)), dim3(256), 0, h->st, nk, d_ik3, km[1], km[2], d_grid);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess && ksum) {
-        if ((rc = rpa_vq(h, k1st, k2nd, dU, kvec, f, *m, vq)) || (rc = dalloc_t(&dvq, (size_t)chunk, "v(q)"))) {
-            (void)hipStreamSynchronize(h->st);
-            release();
-            return rc;
-        }
-    }
-    for (int64_t first = 0; first < h->np && e == hipSuccess && rc == DMD_OK; first += chunk) {
-        const int64_t n = std::min(chunk, h->np - first);
-        e = hipMemcpyAsync(d1, k1st + first, (size_t)n * 8, hipMemcpyDefault, h->st);
-        if (e == hipSuccess) e = hipMemcpyAsync(d2, k2nd + first, (size_t)n * 8, hipMemcpyDefault, h->st);
-        if (e == hipSuccess && ksum) e = hipMemcpyAsync(dvq, vq.data() + first, (size_t)n * 8, hipMemcpyHostToDevice, h->st);
-        if (e != hipSuccess) break;
-        const double *vp = ksum ? dvq : nullptr;
-#define DMD_EE_LAUNCH(NB_)                                                                                           \
-    hipLaunchKernelGGL(k_ee_model<NB_>, dim3(nblk(n * per, 256)), dim3(256), 0, h->st, n, d1, d2, dU, d_ik3, d_grid, km[0], \
-                       km[1], km[2], nk, h->d_e, df, vp, dk, c, pfac, s1, s2)
-        switch (h->nb) {
-        case 1: DMD_EE_LAUNCH(1); break;
-        case 2: DMD_EE_LAUNCH(2); break;
-        case 3: DMD_EE_LAUNCH(3); break;
-        default: DMD_EE_LAUNCH(4); break;
-        }
-#undef DMD_EE_LAUNCH
-        if ((e = hipGetLastError()) != hipSuccess) break;
-        rc = place_P_dev(h, s1, s2, n, first, scale, add ? 1 : 0);
-        if (rc == DMD_OK) e = hipStreamSynchronize(h->st);
-    }
-    hipError_t e2 = hipStreamSynchronize(h->st);
-    release();
-    if (rc == DMD_OK && e == hipSuccess) e = e2;
-    if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
-    if (rc) {
-        h->has_P = false;
-        return rc;
-    }
-    h->has_P = true;
-    return DMD_OK;
-}

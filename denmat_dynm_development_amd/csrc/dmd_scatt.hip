@@ -1208,6 +1208,58 @@ extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, 
     return DMD_OK;
 }
 
+// dst's P (set, or added to with add = 1) = scale x the P src stores, pair by
+// pair in file order, converted on the device between precisions and layouts
+// (a reduced store's values as complex128 into an FP64 handle, or an FP64 P
+// quantised into a reduced one).  Both handles hold the same stored pairs.
+extern "C" int dmd_scatt_copy_P(dmd_scatt *dst, dmd_scatt *src, double scale, int add)
+{
+    if (!dst || !src || dst == src) return fail(DMD_ERR_ARG, "dmd_scatt_copy_P: two distinct handles are needed");
+    if (dst->nk != src->nk || dst->nb != src->nb || dst->np != src->np || dst->dev != src->dev)
+        return fail(DMD_ERR_ARG, "dmd_scatt_copy_P: the handles differ in nk, nb, stored pairs or device");
+    if (!src->has_P) return fail(DMD_ERR_STATE, "dmd_scatt_copy_P: the source holds no P");
+    if (src->linear) return fail(DMD_ERR_STATE, "dmd_scatt_copy_P: the source holds the linearized operator");
+    if (dst->linear && add) return fail(DMD_ERR_STATE, "dmd_scatt_copy_P: the destination holds the linearized operator");
+    if (add && !dst->has_P) return fail(DMD_ERR_STATE, "dmd_scatt_copy_P: nothing to add to");
+    if (src->np == 0) {
+        dst->has_P = true;
+        dst->linear = false;
+        return DMD_OK;
+    }
+    const size_t per = (size_t)src->nb4;
+    const int64_t chunk = std::min<int64_t>(src->np, std::max<int64_t>(1, (int64_t)((64ll << 20) / per)));
+    double2 *s1 = nullptr, *s2 = nullptr;
+    int rc;
+    if ((rc = dalloc_t(&s1, (size_t)chunk * per, "P copy staging"))) return rc;
+    if ((rc = dalloc_t(&s2, (size_t)chunk * per, "P copy staging"))) { dfree(s1); return rc; }
+    for (int64_t f0 = 0; f0 < src->np && rc == DMD_OK; f0 += chunk) {
+        const int64_t n = std::min<int64_t>(chunk, src->np - f0);
+        if (src->precision != DMD_PREC_FP64) {
+            rc = tiles4_fetch(src, f0, n, s1, s2);
+        } else {
+            const int64_t tot = n * 2 * (int64_t)per;
+            hipLaunchKernelGGL(k_fetch_P, dim3(nblk(tot, 256)), dim3(256), 0, src->st, n, f0, src->nb4, src->d_P,
+                               src->d_inv, src->soa, s1, s2);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+        }
+        hipError_t e = hipStreamSynchronize(src->st);
+        if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+        if (rc == DMD_OK) rc = place_P_dev(dst, s1, s2, n, f0, scale, add);
+        e = hipStreamSynchronize(dst->st);  // the staging is reused by the next chunk
+        if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    }
+    dfree(s1);
+    dfree(s2);
+    if (rc) {
+        dst->has_P = false;
+        return rc;
+    }
+    dst->has_P = true;
+    dst->linear = false;
+    return DMD_OK;
+}
+
 // Zero the elements with |P| <= thr (|.| as comm/scalar.h:102-103, no
 // contraction); a and b hold n complex each.
 __global__ void k_threshold(int64_t n, double2 *__restrict__ a, double2 *__restrict__ b, double thr)

@@ -80,6 +80,7 @@ struct Settings {
     std::vector<double> ni_Z;    // Z_impurity
     bool eimp_db = false;        // detailBalance of the model e-i term
     std::string eeMode = "none";
+    int freq_update_ee = 0;
     bool only_ee = false, ee_exchange = false;
     double scale_ee = 1, degauss_ee = 0;
     double degauss = 0, ndegauss = 0, degauss_eimp = 0;
@@ -133,7 +134,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     refuse(p, "alg_summode", 1, "the A-matrix (non-summode) form");
     s.semiclassical = p.get("alg_semiclassical", 0) != 0;  // parameters.cpp:101
     s.linearize = p.get("alg_linearize", 0) != 0;
-    refuse(p, "alg_linearize_dPee", 0, "the linearized e-e operator");
+    refuse(p, "alg_linearize_dPee", 0, "the linearized change of the e-e P inside evolve (linearize_dPee)");
     // parameters.cpp:118-120; sparse P is kept dense in HBM with the small elements zeroed
     s.Pin_is_sparse = p.get("alg_Pin_is_sparse", 0) != 0;
     s.sparseP = p.get("alg_sparseP", 0) != 0;
@@ -187,16 +188,14 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     refuse(p, "gfac_normal_dist", 0, "k-resolved random g factors");
     s.excess = p.get("carrier_density_means_excess_density", 0) != 0;  // parameters.cpp:170
     // e-e (parameters.cpp:226-231,479-492): the P of set_eph at the equilibrium
-    // occupations, kept fixed (Pee_fixed_at_eq); Pee_update rebuilds it from rho
+    // occupations, kept fixed (Pee_fixed_at_eq), or rebuilt from the state every
+    // freq_update_ee_model steps (Pee_update; DensityMatrix_Dynamics.h:281-320)
     s.eeMode = p.str("eeMode", "none");
     if (s.eeMode != "none" && s.eeMode != "Pee_fixed_at_eq" && s.eeMode != "Pee_update")
         throw std::runtime_error("eeMode must be none or Pee_fixed_at_eq now");
-    if (s.eeMode == "Pee_update")
-        throw std::runtime_error("eeMode = Pee_update: the e-e P rebuilt from rho during the run is not carried by this "
-                                 "operator (Pee_fixed_at_eq is)");
     s.ee_exchange = p.get("ee_antisymmetry", 0) != 0;
     s.degauss_ee = p.get("degauss_ee", p.get("degauss", info.degauss / eV, eV) / eV, eV);
-    (void)p.get("freq_update_ee_model", 0);
+    s.freq_update_ee = p.get("freq_update_ee_model", 0);
     const double laserA = p.get("laserA", p.get("pumpA0", 0.));
     if (laserA != 0) throw std::runtime_error("laserA / pumpA0: pump pulses are not carried by this operator");
     // alg_eph_sepr_eh / need_elec / need_hole choose which file set the
@@ -288,14 +287,12 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         throw std::runtime_error("if only e-imp or only e-e, e-ph should not be enabled. Set alg_eph_enable = 0");
     if (s.only_ee && !ee) throw std::runtime_error("alg_only_ee is only possible if eeMode is not none");
     if (s.only_eimp && ee) throw std::runtime_error("if you want only e-i scattering, please set eeMode to be none");
-    // under alg_linearize the e-e sum takes f_eq while RPA keeps summing f of mu
-    // (ElectronPhonon_linear.cpp:19, Coulomb_Model.h:587-590); one f goes to the device
-    if (ee && s.ee_exchange && s.scrFormula == "RPA")
-        throw std::runtime_error("ee_antisymmetry with scrFormula = RPA: the exchange term is built for the "
-                                 "closed-form screenings");
-    if (ee && s.linearize && s.scrFormula == "RPA")
-        throw std::runtime_error("eeMode with alg_linearize and scrFormula = RPA: the e-e term here takes one set of "
-                                 "occupations for both the screening and the sum");
+    if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee == 0)  // parameters.cpp:483-484
+        throw std::runtime_error("want to update Pee but set freq_update_ee_model as 0");
+    if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee < 0)
+        throw std::runtime_error("freq_update_ee_model < 0 (the e-e P rebuilt inside every right-hand side) is not "
+                                 "carried: the integrators here run whole steps on the device; use a positive "
+                                 "freq_update_ee_model");
     if (ee && (s.scrFormula == "lindhard" || s.scrFormula == "heg" || s.scr_dynamic != "static"))
         throw std::runtime_error("eeMode: the model e-e term is built for the static RPA, unscreened, debye and "
                                  "Bechstedt screening");
@@ -711,11 +708,47 @@ int run(const std::string &folder, bool dry)
             for (int b = 0; b < nbe; b++) o[(size_t)k * nbe + b] = fsrc[(size_t)k * nbd + b0e + b];
         return o;
     };
-    for (size_t iD = 0; iD < s.ni.size() && !s.only_ee; iD++) {
-        if (!s.ni_model[iD]) {
-            eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
-            continue;
+    // Pee_update: reset_scatt rebuilds P from the e-ph (and ab initio e-i) part,
+    // the model e-i terms (screening from the state under update_screening) and
+    // the e-e term at the state; that base is kept before the model terms go in
+    const bool pee_update = s.eeMode == "Pee_update" && !s.linearize && !s.only_eimp && s.scatt_enable;
+    for (size_t iD = 0; iD < s.ni.size() && !s.only_ee; iD++)
+        if (!s.ni_model[iD]) eph.add_eimp_ab_neutral((int)iD, std::fabs(s.ni[iD] * cell), s.scale_ei);
+    // the e-e term's static RPA v(q) for every k-point difference, at f of mu
+    // (init_RPA keeps coulomb_model's f, Coulomb_Model.h:587-590)
+    std::vector<double> ee_vq_cell;
+    if (s.eeMode != "none" && !s.only_eimp && s.scatt_enable && s.scrFormula == "RPA") {
+        model_inputs("eeMode");
+        dmd_eimp_model m = screened();
+        const std::vector<double> fmu = eph_cols(fdm);
+        ee_vq_cell.resize((size_t)info.kmesh[0] * info.kmesh[1] * info.kmesh[2]);
+        check(dmd_scatt_ee_vq_cells(eph.handle(), (const double *)mU.data(), mkv.data(), fmu.data(), &m,
+                                    ee_vq_cell.data()),
+              "RPA v(q) table");
+    }
+    if (pee_update) {
+        eph.keep_base(true, ee_vq_cell);
+        if (s.update_screening && (s.scrFormula == "debye" || s.scrFormula == "Bechstedt")) {
+            // coulomb_model::init(dm): n_free from the window's occupations
+            // (bands below nv as holes), then init_model (Coulomb_Model.h:533-575)
+            const int nvw = nvd - b0e;
+            eph.rescreen = [&, nvw](const std::vector<double> &occ, dmd_eimp_model &m) {
+                double n = 0;
+                for (long k = 0; k < nk; k++)
+                    for (int b = 0; b < nbe; b++) n += b < nvw ? 1 - occ[(size_t)k * nbe + b] : occ[(size_t)k * nbe + b];
+                n /= info.nk_full * cell;
+                if (n <= 0) throw std::runtime_error("nfreetot must be postive");
+                const double kF = std::pow(3 * M_PI * M_PI * n, 1. / 3.), EF = kF * kF / 2. / s.meff;
+                const double qTF2 = 6 * M_PI * n / s.eps_background / EF;
+                m.qscr2_debye = 4 * M_PI * n / s.eps_background / T;
+                m.fac0 = 1. / (s.eps_background - 1);
+                m.fac2 = 1. / qTF2;
+                m.fac4 = 3. / 4. / (kF * kF) / qTF2;
+            };
         }
+    }
+    for (size_t iD = 0; iD < s.ni.size() && !s.only_ee; iD++) {
+        if (!s.ni_model[iD]) continue;
         model_inputs("impMode = model_ionized");
         // elecimp_model (ElecImp_Model.h:22-40)
         dmd_eimp_model m = screened();
@@ -739,7 +772,7 @@ int run(const std::string &folder, bool dry)
         m.exchange = s.ee_exchange;
         const std::vector<double> fee = eph_cols(s.linearize ? feq : fdm);
         std::printf("model e-e (%s): degauss_ee = %lg scale_ee = %lg\n", s.eeMode.c_str(), m.degauss, s.scale_ee);
-        eph.add_ee_model(mk1.data(), mk2.data(), mU.data(), mkv.data(), fee.data(), m, s.scale_ee);
+        eph.add_ee_model(mk1.data(), mk2.data(), mU.data(), mkv.data(), fee.data(), m, s.scale_ee, ee_vq_cell);
     }
     // set_eph ends with set_sparseP(true) (ElectronPhonon.cpp:12); under
     // alg_linearize set_eph returns before it (:6)
@@ -770,6 +803,12 @@ int run(const std::string &folder, bool dry)
         double *rows[1] = {feq.data()};
         if (s.linearize) {
             eph.set_Lsc(rows);  // ddmdt_eq of the linearized operator is zero: evolve_linear at f_eq
+            // add_Lsc_contrib("ee") beyond the P1/P2 form: fac2 under ee_antisymmetry,
+            // and P3..P6 unless eeMode = Pee_fixed_at_eq (ElectronPhonon_linear.cpp:61-95)
+            if (s.eeMode != "none" && !s.only_eimp) {
+                const std::vector<double> fw = eph_cols(feq);
+                eph.add_Lsc_ee(fw.data(), s.eeMode != "Pee_fixed_at_eq", 1.0, ee_vq_cell);
+            }
             std::printf("linearized at mu_e = %.12le mu_h = %.12le\n", mue, muh);
         } else if (s.ddmdteq) {
             eph.compute_ddmdt_eq(rows);
@@ -946,7 +985,17 @@ int run(const std::string &folder, bool dry)
             for (auto &r : taus) std::fclose(r.fp);
             return 0;
         }
+        // update_scatt_outside (DensityMatrix_Dynamics.h:282-292,316-318): the e-e P
+        // rebuilt from the state at it = 1, 1 + freq, ...; evolve_euler_one_step
+        // rebuilds before report_tau, evolve_gsl after it
+        auto update_ee = [&]() {
+            if (!pee_update || (it - 1) % s.freq_update_ee != 0) return;
+            cplx *rows_dm[1] = {dm.data()};
+            eph.reset_scatt(false, true, rows_dm, nullptr, t, nullptr);
+        };
+        if (s.ode == "euler") update_ee();
         report_tau(it);
+        if (s.ode != "euler") update_ee();
         ti += s.tstep;
         if (s.ode == "rkf45") {
             check(dmd_scatt_rkf45_apply(h, &t, ti, &hstep, s.hmin, s.hmax, s.epsabs, 0.0, (double *)dm.data(), &st),

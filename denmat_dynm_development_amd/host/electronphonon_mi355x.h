@@ -9,6 +9,7 @@
 // std::runtime_error with the library's message where the reference exits.
 #pragma once
 #include <cmath>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -56,7 +57,7 @@ public:
                                    double scale_eph = 1.0, double scissor = 0.0, int precision = DMD_PREC_FP64,
                                    p_source source = P_DENSE, double thr_sparseP_ = 1e-40)
         : Pin_is_sparse(source == P_SPARSE), thr_sparseP(thr_sparseP_), dir_(ldbd_dir), scatt_(scatt),
-          P_set_(source != P_NONE)
+          scissor_(scissor), precision_(precision), P_set_(source != P_NONE)
     {
         check(dmd_ldbd_read_info(dir_.c_str(), &info_), "ldbd_size.dat");
         int rc;
@@ -78,6 +79,7 @@ public:
     ~electronphonon_mi355x()
     {
         if (h_) dmd_scatt_destroy(h_);
+        if (base_) dmd_scatt_destroy(base_);
     }
     electronphonon_mi355x(const electronphonon_mi355x &) = delete;
     electronphonon_mi355x &operator=(const electronphonon_mi355x &) = delete;
@@ -110,6 +112,8 @@ public:
         check(dmd_scatt_add_eimp_model(h_, k1st, k2nd, (const double *)U, kvec, f, &m, scale_ei, P_set_ ? 1 : 0),
               "model e-i P of defect " + std::to_string(iD + 1));
         P_set_ = true;
+        keep_inputs(k1st, k2nd, U, kvec, f, m.nb_wannier);
+        eimp_models_.push_back({iD, m, scale_ei});
         if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
         if ((int)eimp_slots.size() <= iD) eimp_slots.resize(iD + 1);
         eimp.clear();
@@ -120,13 +124,68 @@ public:
     // Pee_fixed_at_eq: P += scale_ee * P_ee(f), the model e-e P at the
     // occupations f (nk x nb of this window) generated on the device; alg_only_ee
     // (no e-ph P set) makes it P itself
+    // vq_cell: the static RPA v(q) of every k-mesh cell (dmd_scatt_ee_vq_cells);
+    // empty: the closed-form screening, or the RPA summed for the stored pairs' q
     void add_ee_model(const uint64_t *k1st, const uint64_t *k2nd, const complex_t *U, const double *kvec,
-                      const double *f, const dmd_eimp_model &m, double scale_ee)
+                      const double *f, const dmd_eimp_model &m, double scale_ee, const std::vector<double> &vq_cell = {})
     {
-        check(dmd_scatt_add_ee_model(h_, k1st, k2nd, (const double *)U, kvec, f, &m, scale_ee, P_set_ ? 1 : 0),
-              "model e-e P");
+        if (vq_cell.empty())
+            check(dmd_scatt_add_ee_model(h_, k1st, k2nd, (const double *)U, kvec, f, &m, scale_ee, P_set_ ? 1 : 0),
+                  "model e-e P");
+        else
+            check(dmd_scatt_add_ee_model_state(h_, k1st, k2nd, (const double *)U, kvec, f, nullptr, nullptr,
+                                               vq_cell.data(), &m, scale_ee, P_set_ ? 1 : 0),
+                  "model e-e P");
         P_set_ = true;
         if (Pin_is_sparse) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
+        keep_inputs(k1st, k2nd, U, kvec, nullptr, m.nb_wannier);
+        ee_f_.assign(f, f + (size_t)nk_glob * nb);
+        has_ee_ = true;
+        ee_m_ = m;
+        scale_ee_ = scale_ee;
+    }
+
+    // The Pee_update set-up: the P of the parts reset_scatt does not rebuild
+    // (the e-ph P and ab initio e-i, set_ephmat and read_ldbd_imp_P,
+    // ElectronPhonon.cpp:34-38) is kept in a second handle before the model
+    // terms are added.  update_ee: rebuild the e-e P from the state on
+    // reset_scatt (eeMode = Pee_update); vq_cell: the static RPA v(q) per mesh
+    // cell for the e-e term (empty for the closed-form screenings).
+    void keep_base(bool update_ee, const std::vector<double> &vq_cell = {})
+    {
+        if (base_) return;
+        std::vector<uint64_t> k1(info_.nkpair), k2(info_.nkpair);
+        check(dmd_ldbd_read_kpairs(dir_.c_str(), &info_, 0, info_.nkpair, k1.data(), k2.data()), "k pairs");
+        std::vector<double> ek((size_t)info_.nk * info_.nb), e((size_t)info_.nk * nb);
+        check(dmd_ldbd_read_ek(dir_.c_str(), &info_, scissor_, ek.data()), "ldbd_ek.bin");
+        const int b0 = info_.b_bot_eph;
+        for (long k = 0; k < nk_glob; k++)
+            for (int b = 0; b < nb; b++) e[(size_t)k * nb + b] = ek[(size_t)k * info_.nb + b0 + b];
+        e_window_ = e;
+        check(dmd_scatt_create((int64_t)nk_glob, nb, (int64_t)info_.nkpair, k1.data(), k2.data(), e.data(), info_.nk_full,
+                               precision_, &base_),
+              "base P handle");
+        check(dmd_scatt_copy_P(base_, h_, 1.0, 0), "base P copy");
+        update_ee_ = update_ee;
+        ee_vq_cell_ = vq_cell;
+    }
+    // coulomb_model::init(dm) under update_screening (Coulomb_Model.h:533-552):
+    // given the occupations of this window (nk x nb, the state's diagonal) it
+    // re-derives a closed-form screening of model m; unset: the screening stays.
+    std::function<void(const std::vector<double> &occ, dmd_eimp_model &m)> rescreen;
+
+    // The linearized operator's e-e terms beyond the P1/P2 form
+    // (add_Lsc_contrib("ee"), ElectronPhonon_linear.cpp:38-104,183-253), after
+    // set_Lsc: the fac2 doubling under ee_antisymmetry and, with dP_terms (eeMode
+    // other than Pee_fixed_at_eq), P3/P4 and P5/P6 at f_eq (nk x nb of this window).
+    // coef34 = 1 keeps the reference's sign of the P3/P4 terms.
+    void add_Lsc_ee(const double *f_eq, bool dP_terms, double coef34 = 1.0, const std::vector<double> &vq_cell = {})
+    {
+        if (!has_ee_) return;
+        check(dmd_scatt_add_Lsc_ee(h_, mk1_.data(), mk2_.data(), (const double *)mU_.data(), mkv_.data(), f_eq,
+                                   vq_cell.empty() ? nullptr : vq_cell.data(), &ee_m_, scale_ee_,
+                                   ee_m_.exchange ? 1.0 : 0.0, coef34, dP_terms ? 1 : 0),
+              "linearized e-e terms");
     }
 
     // set_sparseP(true) at the end of set_eph (ElectronPhonon.cpp:12,115-129)
@@ -136,6 +195,7 @@ public:
     void set_sparseP(bool write_files = true)
     {
         if (Pin_is_sparse) return;
+        sparse_on_ = true;
         check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
         if (write_files) check(dmd_ldbd_write_sP(h_, dir_.c_str(), &info_, scatt_.c_str(), thr_sparseP), "sP files");
     }
@@ -155,6 +215,7 @@ public:
     // alg.expt and alg.picture (parameters.cpp:104,107)
     void set_algorithm(bool expt, bool schrodinger)
     {
+        interaction_ = !schrodinger;
         check(dmd_scatt_set_flags(h_, expt ? 1 : 0, schrodinger ? 1 : 0, DMD_KERNEL_AUTO), "flags");
     }
 
@@ -196,18 +257,104 @@ public:
         check(dmd_scatt_set_Lsc(h_, f.data()), "set_Lsc");
     }
 
-    // ElectronPhonon.cpp:15-42: P is rebuilt only for model e-i and e-e
-    // scattering, neither of which this operator carries
-    void reset_scatt(bool, bool, complex_t **, complex_t **, double, double **) {}
+    // reset_scatt (ElectronPhonon.cpp:15-42): with a base kept (keep_base) and
+    // either flag set, P = base + every model e-i term (screening re-derived
+    // from the state when rescreen is set) + the e-e term at the state
+    // (Pee_update) or at f, and the re-sparsification of alg_sparseP.  The
+    // e-e term takes the window of dm (and dm1; null: 1 - dm) turned from the
+    // interaction to the Schroedinger picture as add_scatt_contrib("ee") does
+    // (:56-65): the upper triangle times cis((e_j - e_i) t), the lower its
+    // conjugate, dm1 = -dm off the diagonal.
+    void reset_scatt(bool reset_eimp, bool reset_ee, complex_t **dm_expand, complex_t **dm1_expand, double t,
+                     double **f_eq_expand)
+    {
+        (void)f_eq_expand;
+        if ((!reset_eimp && !reset_ee) || !base_ || (eimp_models_.empty() && !has_ee_)) return;
+        const long nk = nk_glob;
+        const int nb2 = nb * nb, nbx = nb_expand;
+        std::vector<double> dm((size_t)nk * nb2 * 2), dm1((size_t)nk * nb2 * 2), occ((size_t)nk * nb);
+        const double *src = (const double *)dm_expand[0], *src1 = dm1_expand ? (const double *)dm1_expand[0] : nullptr;
+        for (long k = 0; k < nk; k++)
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) {
+                    const size_t o = ((size_t)k * nbx + bStart + i) * nbx + bStart + j, q = (size_t)k * nb2 + i * nb + j;
+                    dm[2 * q] = src[2 * o];
+                    dm[2 * q + 1] = src[2 * o + 1];
+                    dm1[2 * q] = src1 ? src1[2 * o] : (i == j ? 1.0 : 0.0) - src[2 * o];
+                    dm1[2 * q + 1] = src1 ? src1[2 * o + 1] : -src[2 * o + 1];
+                    if (i == j) occ[(size_t)k * nb + i] = src[2 * o];
+                }
+        if (interaction_)
+            for (long k = 0; k < nk; k++)
+                for (int i = 0; i < nb; i++)
+                    for (int j = i + 1; j < nb; j++) {
+                        const size_t q = (size_t)k * nb2 + i * nb + j, r = (size_t)k * nb2 + j * nb + i;
+                        const double a = (e_window_[(size_t)k * nb + j] - e_window_[(size_t)k * nb + i]) * t;
+                        const double c = std::cos(a), sn = std::sin(a), x = dm[2 * q], y = dm[2 * q + 1];
+                        dm[2 * q] = x * c - y * sn;
+                        dm[2 * q + 1] = x * sn + y * c;
+                        dm1[2 * q] = -dm[2 * q];
+                        dm1[2 * q + 1] = -dm[2 * q + 1];
+                        dm[2 * r] = dm[2 * q];
+                        dm[2 * r + 1] = -dm[2 * q + 1];
+                        dm1[2 * r] = dm1[2 * q];
+                        dm1[2 * r + 1] = -dm1[2 * q + 1];
+                    }
+        check(dmd_scatt_copy_P(h_, base_, 1.0, 0), "P reset");
+        for (auto &em : eimp_models_) {
+            dmd_eimp_model m = em.m;
+            if (rescreen) rescreen(occ, m);
+            check(dmd_scatt_add_eimp_model(h_, mk1_.data(), mk2_.data(), (const double *)mU_.data(), mkv_.data(),
+                                           mf_.empty() ? nullptr : mf_.data(), &m, em.scale, 1),
+                  "model e-i P of defect " + std::to_string(em.iD + 1));
+        }
+        if (has_ee_) {
+            dmd_eimp_model m = ee_m_;
+            if (rescreen) rescreen(occ, m);
+            const bool at_state = update_ee_ && reset_ee;
+            check(dmd_scatt_add_ee_model_state(h_, mk1_.data(), mk2_.data(), (const double *)mU_.data(), mkv_.data(),
+                                               ee_f_.data(), at_state ? dm.data() : nullptr, at_state ? dm1.data() : nullptr,
+                                               ee_vq_cell_.empty() ? nullptr : ee_vq_cell_.data(), &m, scale_ee_, 1),
+                  "model e-e P");
+        }
+        if (Pin_is_sparse || sparse_on_) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
+    }
 
     dmd_scatt *handle() const { return h_; }
     const dmd_ldbd_info &info() const { return info_; }
 
 private:
+    struct eimp_entry {
+        int iD;
+        dmd_eimp_model m;
+        double scale;
+    };
+    // the model terms' inputs, kept for reset_scatt (pairs, U, kvec, f at mu)
+    void keep_inputs(const uint64_t *k1st, const uint64_t *k2nd, const complex_t *U, const double *kvec,
+                     const double *f, int nw)
+    {
+        if (f && mf_.empty()) mf_.assign(f, f + (size_t)nk_glob * nb);
+        if (!mk1_.empty() || info_.nkpair == 0) return;
+        mk1_.assign(k1st, k1st + info_.nkpair);
+        mk2_.assign(k2nd, k2nd + info_.nkpair);
+        mU_.assign(U, U + (size_t)nk_glob * nw * nb);
+        mkv_.assign(kvec, kvec + (size_t)nk_glob * 3);
+    }
     std::string dir_, scatt_;
+    double scissor_ = 0.0;
+    int precision_ = DMD_PREC_FP64;
     bool P_set_ = true;
+    bool interaction_ = true, sparse_on_ = false;
     dmd_ldbd_info info_{};
     dmd_scatt *h_ = nullptr;
+    dmd_scatt *base_ = nullptr;
+    std::vector<uint64_t> mk1_, mk2_;
+    std::vector<complex_t> mU_;
+    std::vector<double> mkv_, mf_, ee_f_, e_window_, ee_vq_cell_;
+    std::vector<eimp_entry> eimp_models_;
+    bool has_ee_ = false, update_ee_ = false;
+    dmd_eimp_model ee_m_{};
+    double scale_ee_ = 1.0;
 };
 
 }  // namespace dmdrun

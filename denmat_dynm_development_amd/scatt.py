@@ -189,6 +189,51 @@ class ScatteringOperator:
         call("dmd_scatt_add_ee_model", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
              f.ctypes.data, ctypes.byref(model), float(scale), int(bool(add)))
 
+    def ee_vq_cells(self, U, kvec, f, model):
+        """The static RPA v(q) of every k-mesh cell that is a difference of two k
+        points (dmd_scatt_ee_vq_cells); f (nk, nb) at mu.  Returns an array of
+        kmesh[0] * kmesh[1] * kmesh[2] values."""
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        kv = np.ascontiguousarray(kvec, dtype=np.float64)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros(int(np.prod(list(model.kmesh))))
+        call("dmd_scatt_ee_vq_cells", self._h, U.ctypes.data, kv.ctypes.data, f.ctypes.data, ctypes.byref(model),
+             out.ctypes.data)
+        return out
+
+    def add_ee_model_state(self, k1st, k2nd, U, kvec, f, model, dm=None, dm1=None, vq_cell=None, scale=1.0,
+                           add=True):
+        """The e-e P at the density matrix dm (nk, nb^2, e-ph window,
+        Schroedinger picture) with dm1 = 1 - dm as set_oneminusdm forms it
+        (default), or at f when dm is None: the Pee_update rebuild
+        (dmd_scatt_add_ee_model_state).  vq_cell: ee_vq_cells(...) for the
+        static RPA."""
+        k1, k2, U, kv, f = _ee_args(self, k1st, k2nd, U, kvec, f, model)
+        pdm = pdm1 = None
+        if dm is not None:
+            dm = np.ascontiguousarray(dm, dtype=np.complex128).reshape(self.nk, self.nb * self.nb)
+            if dm1 is None:
+                dm1 = -dm.copy()
+                dm1[:, ::self.nb + 1] += 1.0
+            dm1 = np.ascontiguousarray(dm1, dtype=np.complex128).reshape(self.nk, self.nb * self.nb)
+            pdm, pdm1 = dm.ctypes.data, dm1.ctypes.data
+        vc = None if vq_cell is None else np.ascontiguousarray(vq_cell, dtype=np.float64)
+        call("dmd_scatt_add_ee_model_state", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             f.ctypes.data, pdm, pdm1, None if vc is None else vc.ctypes.data, ctypes.byref(model), float(scale),
+             int(bool(add)))
+
+    def add_Lsc_ee(self, k1st, k2nd, U, kvec, f_eq, model, scale=1.0, extra=0.0, coef34=1.0, dP_terms=True,
+                   vq_cell=None):
+        """The e-e terms of add_Lsc_contrib("ee") beyond the P1/P2 form (after
+        set_Lsc): extra x Lscij/Lscji of P_ee(f_eq) (1 under ee_antisymmetry),
+        and with dP_terms the P3/P4 (coef34 = 1: the reference's sign, -1: the
+        exact Jacobian) and P5/P6 terms (dmd_scatt_add_Lsc_ee)."""
+        k1, k2, U, kv, f = _ee_args(self, k1st, k2nd, U, kvec, f_eq, model)
+        vc = None if vq_cell is None else np.ascontiguousarray(vq_cell, dtype=np.float64)
+        call("dmd_scatt_add_Lsc_ee", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             f.ctypes.data, None if vc is None else vc.ctypes.data, ctypes.byref(model), float(scale), float(extra),
+             float(coef34), int(bool(dP_terms)))
+
     def download_P(self, first=0, count=None):
         count = self.n_pairs - first if count is None else count
         n4 = self.nb ** 4
@@ -196,6 +241,11 @@ class ScatteringOperator:
         P2 = np.empty((count, n4), np.complex128)
         call("dmd_scatt_download_P", self._h, int(first), int(count), P1.ctypes.data, P2.ctypes.data)
         return P1, P2
+
+    def copy_P_from(self, src, scale=1.0, add=False):
+        """P = scale x src's stored P (same pairs), converted on the device
+        between precisions (dmd_scatt_copy_P)."""
+        call("dmd_scatt_copy_P", self._h, src._h, float(scale), int(bool(add)))
 
     def generate_P(self, params):
         call("dmd_scatt_generate_P", self._h, ctypes.byref(params))
@@ -569,6 +619,41 @@ def synth_spin(p):
     s3 = np.empty((p.nk, 3, p.nb * p.nb), np.complex128)
     call("dmd_synth_spin", ctypes.byref(p), s3.ctypes.data)
     return s3
+
+
+def synth_initial_state(p, e, s3, axis=2, amp=1e-3, mu=-0.1 / 27.21138505):
+    """rho0 = f + (df/de) b S_axis, the first-order state of
+    electron::compute_dm_Bpert_1st (src_v4.5.7/electron.cpp:62-98, degenerate
+    limit f(f-1)/T) with b set so that Tr(S_axis drho) / sum Tr f = amp; f is
+    Fermi-Dirac at mu (default CBM - 0.1 eV).  Returns (rho0, f)."""
+    nk, nb = e.shape
+    T = p.temperature
+    f = 1.0 / (np.exp((e - mu) / T) + 1.0)
+    sa = s3[:, axis].reshape(nk, nb, nb)
+    de = e[:, :, None] - e[:, None, :]
+    fd = f[:, :, None] - f[:, None, :]
+    deg = np.abs(de) < 1e-8
+    favg = 0.5 * (f[:, :, None] + f[:, None, :])
+    dfde = np.where(deg, favg * (favg - 1) / T, fd / np.where(deg, 1.0, de))
+    d1 = dfde * sa  # drho for b = 1
+    s_per_b = np.real(np.einsum("kij,kji->", sa, d1))
+    b = amp * f.sum() / s_per_b
+    rho = (np.einsum("ki,ij->kij", f, np.eye(nb)) + b * d1).reshape(nk, nb * nb)
+    return rho.astype(np.complex128), f
+
+
+def synth_batch_states(p, e, s3, B):
+    """B initial states of the batched workload: spin along z, x, y in turn,
+    imbalance amplitudes log-uniform in [1e-4, 1e-2] (trajectory 0: z, 1e-3).
+    Returns (B x nk x nb^2 states, f)."""
+    states = []
+    f = None
+    for b in range(B):
+        axis = (2, 0, 1)[b % 3]
+        amp = 1e-3 if b == 0 else 10.0 ** (-4 + 2 * ((b * 0.6180339887) % 1.0))
+        rho, f = synth_initial_state(p, e, s3, axis, amp)
+        states.append(rho)
+    return np.stack(states), f
 
 
 def write_ldbd(dirname, p, nthreads=0):

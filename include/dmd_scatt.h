@@ -154,6 +154,41 @@ int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t 
  * scale = scale_ee. */
 int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
                            const double *kvec, const double *f, const dmd_eimp_model *m, double scale, int add);
+/* The e-e rebuild of reset_scatt for eeMode = Pee_update (ElectronPhonon.cpp:
+ * 15-42,44-113 with elecelec_model::calc_P at the density matrix,
+ * ElecElec_Model.h:185-262): P (+)= scale * P_ee(dm, dm1), with dm, dm1 the
+ * e-ph window of the state and of 1 - state in the Schroedinger picture
+ * (nk x nb^2, Hermitian, read from the upper triangle as the reference's
+ * zhemm does); dm = dm1 = NULL: at the occupations f (as
+ * dmd_scatt_add_ee_model).  vq_cell: v(q) per k-mesh cell from
+ * dmd_scatt_ee_vq_cells, required for the static RPA (unused otherwise). */
+int dmd_scatt_add_ee_model_state(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                                 const double *kvec, const double *f, const double *dm, const double *dm1,
+                                 const double *vq_cell, const dmd_eimp_model *m, double scale, int add);
+/* The static RPA v(q) of every k-mesh cell that is a difference of two k
+ * points (coulomb_model's q list over all (ik, jk), Coulomb_Model.h:690-744,
+ * 820-828), 0 at q = 0 and elsewhere: vq_cell holds kmesh[0]*kmesh[1]*kmesh[2]
+ * doubles, cell ((x m1 + y) m2 + z).  f: occupations at mu (nk x nb). */
+int dmd_scatt_ee_vq_cells(dmd_scatt *h, const double *U, const double *kvec, const double *f, const dmd_eimp_model *m,
+                          double *vq_cell);
+/* The e-e terms of add_Lsc_contrib("ee") beyond the P1/P2 form set_Lsc holds
+ * (ElectronPhonon_linear.cpp:38-104,183-253), added into the linearized
+ * operator of a handle after dmd_scatt_set_Lsc at f_eq: extra x the Lscij /
+ * Lscji of P_ee(f_eq) (extra = 1 under ee_antisymmetry, whose fac2 doubles
+ * them), and with dP_terms (eeMode other than Pee_fixed_at_eq) the P3/P4
+ * (ElecElec_Model_linear.cpp:3-57, none with exchange) and P5/P6 (:59-111)
+ * terms, all times scale (scale_ee).  coef34 = 1 subtracts the P3/P4 terms as
+ * the reference does; coef34 = -1 adds them, which makes the operator the
+ * exact Jacobian of the e-e right-hand side at f_eq. */
+int dmd_scatt_add_Lsc_ee(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                         const double *kvec, const double *f_eq, const double *vq_cell, const dmd_eimp_model *m,
+                         double scale, double extra, double coef34, int dP_terms);
+/* dst's P = scale x src's stored P (add = 1: added), pair by pair in file
+ * order, converted on the device between precisions and layouts (a reduced
+ * store's values into an FP64 handle, or FP64 P quantised into a reduced
+ * store).  Both handles hold the same stored pairs on one device.  No
+ * reference counterpart (the reference keeps one complex128 P). */
+int dmd_scatt_copy_P(dmd_scatt *dst, dmd_scatt *src, double scale, int add);
 /* P of stored pairs [first, first+count) in file order, as complex128. */
 int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2);
 /* Fills P on the device from the synthetic description (the handle's e must

@@ -660,7 +660,7 @@ static int64_t mesh_code(const double *k, const int *km)
 int oracle_eimp_rpa_vq(int64_t nk, int nb, int nw, const double *kvec, const int *kmesh, const double *U,
                        const double *e, const double *f, double T, const double *GGT, double prefac_vq,
                        double nk_full, int64_t npairs, const uint64_t *k1, const uint64_t *k2, double *vq_pair,
-                       double *qscr2_out, int64_t *nq_out)
+                       double *qscr2_out, int64_t *nq_out, double *vq_cell_out)
 {
     int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
     int64_t *kmap = (int64_t *)malloc(sizeof(int64_t) * cells), *qmap = (int64_t *)malloc(sizeof(int64_t) * cells);
@@ -720,6 +720,15 @@ int oracle_eimp_rpa_vq(int64_t nk, int nb, int nw, const double *kvec, const int
     }
     if (qscr2_out) memcpy(qscr2_out, qscr2, sizeof(double) * nq);
     if (nq_out) *nq_out = nq;
+    /* v(q) of every mesh cell that is a difference of two k points (0 at q = 0,
+     * as vq skips the Gamma point, :670-675); other cells 0 */
+    if (vq_cell_out)
+        for (int64_t c = 0; c < cells; c++) {
+            vq_cell_out[c] = 0.0;
+            if (qmap[c] < 0) continue;
+            double q2 = oracle_eimp_q2(GGT, qvec + 3 * qmap[c], (const double[3]){0, 0, 0});
+            vq_cell_out[c] = q2 < 1e-20 ? 0.0 : vq[qmap[c]];
+        }
     free(kmap); free(qmap); free(qvec); free(qscr2); free(vq);
     return 0;
 }
@@ -798,6 +807,51 @@ int oracle_eimp_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64
     return 0;
 }
 
+/* v(k_a - k_b) of the e-e model: the per-cell table (static RPA, the q list  */
+/* of every k-point difference) when given, else the closed-form screening   */
+/* (coulomb_model::vq, Coulomb_Model.h:632-662).                              */
+static double ee_vq(int64_t a, int64_t b, const double *kvec, const int *kmesh, const double *GGT,
+                    const double *vq_cell, int screening, double prefac_vq, double prefac_vq_bare,
+                    double qscr2_debye, double fac0, double fac2, double fac4)
+{
+    double q2 = oracle_eimp_q2(GGT, kvec + 3 * a, kvec + 3 * b);
+    if (vq_cell) {
+        if (q2 < 1e-20) return 0.0;
+        double q[3];
+        for (int d = 0; d < 3; d++) q[d] = kvec[3 * a + d] - kvec[3 * b + d];
+        return vq_cell[mesh_code(q, kmesh)];
+    }
+    return oracle_eimp_vq(screening, q2, prefac_vq, prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
+}
+
+/* calc_A1rho_dP (ElecElec_Model.h:271-281): with ddml[i][j] = dmleft[i][j] -  */
+/* fl[i] and ddmr[i][j] = dmright[i][j] - fr[i] (the reference subtracts the   */
+/* row's occupation from every element of the row, and its zhemm reads the   */
+/* upper triangle), out = fl o (A ddmr) + (ddml A) o fr, with fl[i] scaling   */
+/* rows and fr[j] columns.  fl/fr are f (or 1 - f when *_om is set).          */
+static void a1rho_dP(zc *out, const zc *A, const zc *dmleft, const double *fl0, int fl_om, const zc *dmright,
+                     const double *fr0, int fr_om, int nb)
+{
+    zc ddml[64], ddmr[64], t[64];
+    double fl[8], fr[8];
+    for (int i = 0; i < nb; i++) {
+        fl[i] = fl_om ? 1 - fl0[i] : fl0[i];
+        fr[i] = fr_om ? 1 - fr0[i] : fr0[i];
+    }
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) {
+            ddml[i * nb + j] = zsub(dmleft[i * nb + j], zmk(fl[i], 0));
+            ddmr[i * nb + j] = zsub(dmright[i * nb + j], zmk(fr[i], 0));
+        }
+    const zc c1 = zmk(1, 0), c0 = zmk(0, 0);
+    zhemm_upper(t, 0, ddmr, A, nb, c1, c0);
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) out[i * nb + j] = zscale(t[i * nb + j], fl[i]);
+    zhemm_upper(t, 1, ddml, A, nb, c1, c0);
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) out[i * nb + j] = zadd(out[i * nb + j], zscale(t[i * nb + j], fr[j]));
+}
+
 /* Model e-e P at the equilibrium occupations, elecelec_model::calc_P with  */
 /* dm = null and no exchange (scattering_model/ElecElec_Model.h:69-112,      */
 /* 144-184,185-262,285-296): for each stored pair (k1, k2), k1 != k2, and    */
@@ -811,7 +865,8 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                       const double *U, const double *kvec, const int *kmesh, const double *e, const double *f,
                       const double *vq_pair, int lindblad, double degauss, double ethr, double nk_full, double *P1,
                       double *P2, int antisym, const double *GGT, int screening, double prefac_vq, double prefac_vq_bare,
-                      double qscr2_debye, double fac0, double fac2, double fac4)
+                      double qscr2_debye, double fac0, double fac2, double fac4, const double *vq_cell,
+                      const double *dm, const double *dm1, double *dP1, double *dP2)
 {
     int nb2 = nb * nb, nb4 = nb2 * nb2;
     int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
@@ -828,10 +883,15 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
     zc *Ar = (zc *)malloc(sizeof(zc) * nb2), *Arb = (zc *)malloc(sizeof(zc) * nb2);
     zc *o14 = (zc *)malloc(sizeof(zc) * nb2), *o32 = (zc *)malloc(sizeof(zc) * nb2);
     double *dl = (double *)malloc(sizeof(double) * nb4);
+    const zc c1 = zmk(1, 0), c0 = zmk(0, 0);
+    zc *Ard = (zc *)malloc(sizeof(zc) * nb2), *Arbd = (zc *)malloc(sizeof(zc) * nb2);
     for (int64_t p = 0; p < npairs; p++) {
         int64_t a = (int64_t)k1[p], b = (int64_t)k2[p];
         zc *Q1 = (zc *)P1 + p * nb4, *Q2 = (zc *)P2 + p * nb4;
+        zc *D1 = dP1 ? (zc *)dP1 + p * nb4 : NULL, *D2 = dP2 ? (zc *)dP2 + p * nb4 : NULL;
         for (int q = 0; q < nb4; q++) Q1[q] = Q2[q] = zmk(0, 0);
+        if (D1)
+            for (int q = 0; q < nb4; q++) D1[q] = D2[q] = zmk(0, 0);
         if (a == b && !antisym) continue;
         const zc *Ua = (const zc *)U + a * nw * nb, *Ub = (const zc *)U + b * nw * nb;
         for (int i = 0; i < nb; i++)
@@ -865,8 +925,8 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                     o34[i * nb + j] = o;
                 }
             if (antisym) {  /* calc_ovlps' <1|v|4> and <3|2> (ElecElec_Model.h:69-75) */
-                double v14 = oracle_eimp_vq(screening, oracle_eimp_q2(GGT, kvec + 3 * a, kvec + 3 * c4), prefac_vq,
-                                            prefac_vq_bare, qscr2_debye, fac0, fac2, fac4);
+                double v14 = ee_vq(a, c4, kvec, kmesh, GGT, vq_cell, screening, prefac_vq, prefac_vq_bare,
+                                   qscr2_debye, fac0, fac2, fac4);
                 const zc *U2 = (const zc *)U + b * nw * nb;
                 for (int i = 0; i < nb; i++)
                     for (int j = 0; j < nb; j++) {
@@ -890,28 +950,261 @@ int oracle_ee_model_P(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t
                 A1[q] = lindblad ? A2[q] : mee;
             }
             const double *f3 = f + c3 * nb, *f4 = f + c4 * nb;
+            const int with_rho = dm != NULL && dP1 == NULL;
             for (int i1 = 0; i1 < nb; i1++)
                 for (int i3 = 0; i3 < nb; i3++) {
-                    for (int x = 0; x < nb; x++)
-                        for (int y = 0; y < nb; y++) {
-                            Ar[x * nb + y] = zscale(zscale(A1[(i1 * nb + i3) * nb2 + x * nb + y], 1 - f3[x]), f4[y]);
-                            Arb[x * nb + y] = zscale(zscale(A1[(i3 * nb + i1) * nb2 + x * nb + y], f3[x]), 1 - f4[y]);
-                        }
+                    if (with_rho) {  /* calc_A1rho (:266-270): dm1_k3 A1 dm_k4, dm_k3 A1 dm1_k4 */
+                        zc t[64];
+                        zhemm_upper(t, 0, (const zc *)dm + c4 * nb2, A1 + (i1 * nb + i3) * nb2, nb, c1, c0);
+                        zhemm_upper(Ar, 1, (const zc *)dm1 + c3 * nb2, t, nb, c1, c0);
+                        zhemm_upper(t, 0, (const zc *)dm1 + c4 * nb2, A1 + (i3 * nb + i1) * nb2, nb, c1, c0);
+                        zhemm_upper(Arb, 1, (const zc *)dm + c3 * nb2, t, nb, c1, c0);
+                    } else {  /* calc_A1f / calc_A1fbar (:282-293) */
+                        for (int x = 0; x < nb; x++)
+                            for (int y = 0; y < nb; y++) {
+                                Ar[x * nb + y] = zscale(zscale(A1[(i1 * nb + i3) * nb2 + x * nb + y], 1 - f3[x]), f4[y]);
+                                Arb[x * nb + y] = zscale(zscale(A1[(i3 * nb + i1) * nb2 + x * nb + y], f3[x]), 1 - f4[y]);
+                            }
+                    }
+                    if (dP1) {  /* calc_A1rho_dP (:271-281) */
+                        a1rho_dP(Ard, A1 + (i1 * nb + i3) * nb2, (const zc *)dm1 + c3 * nb2, f3, 1, (const zc *)dm + c4 * nb2,
+                                 f4, 0, nb);
+                        a1rho_dP(Arbd, A1 + (i3 * nb + i1) * nb2, (const zc *)dm + c3 * nb2, f3, 0, (const zc *)dm1 + c4 * nb2,
+                                 f4, 1, nb);
+                    }
                     for (int i2 = 0; i2 < nb; i2++)
                         for (int i4 = 0; i4 < nb; i4++) {
                             int q = ((i1 * nb + i2) * nb + i3) * nb + i4, n24 = (i2 * nb + i4) * nb2, n42 = (i4 * nb + i2) * nb2;
                             for (int xy = 0; xy < nb2; xy++) {
                                 Q1[q] = zadd(Q1[q], zmul(Ar[xy], zconj(A2[n24 + xy])));
                                 Q2[q] = zadd(Q2[q], zmul(Arb[xy], zconj(A2[n42 + xy])));
+                                if (dP1) {
+                                    D1[q] = zadd(D1[q], zmul(Ard[xy], zconj(A2[n24 + xy])));
+                                    D2[q] = zadd(D2[q], zmul(Arbd[xy], zconj(A2[n42 + xy])));
+                                }
                             }
                         }
                 }
         }
         double pf = (antisym ? 2 : 1) * pgauss / nk_full;
         for (int q = 0; q < nb4; q++) { Q1[q] = zscale(Q1[q], pf); Q2[q] = zscale(Q2[q], pf); }
+        if (dP1)
+            for (int q = 0; q < nb4; q++) { D1[q] = zscale(D1[q], pf); D2[q] = zscale(D2[q], pf); }
     }
-    free(o14); free(o32);
+    free(o14); free(o32); free(Ard); free(Arbd);
     free(kmap); free(o12); free(o34); free(A1); free(A2); free(Ar); free(Arb); free(dl);
+    return 0;
+}
+
+/* The e-e model's A tensor for the k quadruple (ka, kb, kc, kd) of calc_A   */
+/* (ElecElec_Model.h:144-166): mee = v(ka - kb) <a|b><c|d> (exchange: (mee -  */
+/* v(ka - kd) <a|d><c|b>) / 2), delta = exp(pexp de^2) inside ethr with       */
+/* de = e_a - e_b + e_c - e_d, A2 = delta mee, A1 = A2 (lindblad) or mee.      */
+/* Returns 0 when no band combination conserves energy (A untouched).        */
+static int ee_calc_A(int64_t ka, int64_t kb, int64_t kc, int64_t kd, int nb, int nw, const double *U,
+                     const double *kvec, const int *kmesh, const double *e, int lindblad, double pexp, double ethr,
+                     int antisym, const double *GGT, const double *vq_cell, int screening, double prefac_vq,
+                     double prefac_vq_bare, double qscr2_debye, double fac0, double fac2, double fac4, zc *A1, zc *A2)
+{
+    const int nb2 = nb * nb, nb4 = nb2 * nb2;
+    double dl[4096];
+    int conserved = 0;
+    for (int q = 0; q < nb4; q++) {
+        int i1 = q / (nb * nb2), i2 = (q / nb2) % nb, i3 = (q / nb) % nb, i4 = q % nb;
+        double de = e[ka * nb + i1] - e[kb * nb + i2] + e[kc * nb + i3] - e[kd * nb + i4];
+        dl[q] = 0;
+        if (fabs(de) < ethr) { dl[q] = exp(pexp * pow(de, 2)); conserved = 1; }
+    }
+    if (!conserved) return 0;
+    zc oab[64], ocd[64], oad[64], ocb[64];
+    const zc *Ua = (const zc *)U + ka * nw * nb, *Ub = (const zc *)U + kb * nw * nb;
+    const zc *Uc = (const zc *)U + kc * nw * nb, *Ud = (const zc *)U + kd * nw * nb;
+    double vab = ee_vq(ka, kb, kvec, kmesh, GGT, vq_cell, screening, prefac_vq, prefac_vq_bare, qscr2_debye, fac0,
+                       fac2, fac4);
+    double vad = antisym ? ee_vq(ka, kd, kvec, kmesh, GGT, vq_cell, screening, prefac_vq, prefac_vq_bare, qscr2_debye,
+                                 fac0, fac2, fac4)
+                         : 0.0;
+    for (int i = 0; i < nb; i++)
+        for (int j = 0; j < nb; j++) {
+            zc x = zmk(0, 0), y = zmk(0, 0), z = zmk(0, 0), w = zmk(0, 0);
+            for (int r = 0; r < nw; r++) {
+                x = zadd(x, zmul(zconj(Ua[r * nb + i]), Ub[r * nb + j]));
+                y = zadd(y, zmul(zconj(Uc[r * nb + i]), Ud[r * nb + j]));
+                if (antisym) {
+                    z = zadd(z, zmul(zconj(Ua[r * nb + i]), Ud[r * nb + j]));
+                    w = zadd(w, zmul(zconj(Uc[r * nb + i]), Ub[r * nb + j]));
+                }
+            }
+            oab[i * nb + j] = zscale(x, vab);
+            ocd[i * nb + j] = y;
+            oad[i * nb + j] = zscale(z, vad);
+            ocb[i * nb + j] = w;
+        }
+    for (int q = 0; q < nb4; q++) {
+        int i1 = q / (nb * nb2), i2 = (q / nb2) % nb, i3 = (q / nb) % nb, i4 = q % nb;
+        zc mee = zmul(oab[i1 * nb + i2], ocd[i3 * nb + i4]);
+        if (antisym) {
+            zc ex = zmul(oad[i1 * nb + i4], ocb[i3 * nb + i2]);
+            mee = zadd(zmul(zmk(-0.5, 0), ex), zmul(zmk(0.5, 0), mee));
+        }
+        A2[q] = zscale(mee, dl[q]);
+        A1[q] = lindblad ? A2[q] : mee;
+    }
+    return 1;
+}
+
+/* The linearised e-e P3..P6 of every stored pair (k1, ka) at the            */
+/* occupations f, elecelec_model::calc_P3P4 and calc_P5P6                    */
+/* (scattering_model/ElecElec_Model_linear.cpp:3-111):                        */
+/*  P3[(i1 i2),(ia ib)] = pf sum_k3 sum_43 A1[(i1 i4),(i3 ia)] (1-f3) f4       */
+/*                              conj(A2[(i2 i4),(i3 ib)]),  k4 = k1 + k3 - ka,  */
+/*  P4 the same with f3 (1-f4), A from calc_A(k1, k4, k3, ka);                 */
+/*  P5[(i1 i2),(ia ib)] = pf sum_k3 sum_34 A1[(i1 i3),(ib i4)] f3 f4           */
+/*                              conj(A2[(i2 i3),(ia i4)]),  k4 = k1 + ka - k3,  */
+/*  P6 the same with (1-f3)(1-f4), A from calc_A(k1, k3, ka, k4);              */
+/* pf = prefac_gauss / nk_full (x 2 with exchange), and the reference's k3     */
+/* exclusions.  Any of P3..P6 may be NULL only together with its partner.    */
+int oracle_ee_model_P3P6(int64_t nk, int nb, int nw, int64_t npairs, const uint64_t *k1, const uint64_t *k2,
+                         const double *U, const double *kvec, const int *kmesh, const double *e, const double *f,
+                         int lindblad, double degauss, double ethr, double nk_full, int antisym, const double *GGT,
+                         const double *vq_cell, int screening, double prefac_vq, double prefac_vq_bare,
+                         double qscr2_debye, double fac0, double fac2, double fac4, double *P3, double *P4,
+                         double *P5, double *P6)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 4) return -1;
+    int64_t cells = (int64_t)kmesh[0] * kmesh[1] * kmesh[2];
+    int64_t *kmap = (int64_t *)malloc(sizeof(int64_t) * cells);
+    for (int64_t c = 0; c < cells; c++) kmap[c] = -1;
+    for (int64_t ik = 0; ik < nk; ik++) {
+        int64_t c = mesh_code(kvec + 3 * ik, kmesh);
+        if (kmap[c] < 0) kmap[c] = ik;
+    }
+    double pgauss = 1. / (degauss * sqrt(2. * M_PI));
+    double pexp = lindblad ? -0.25 / (degauss * degauss) : -0.5 / (degauss * degauss);
+    double pf = (antisym ? 2 : 1) * pgauss / nk_full;
+    zc A1[256], A2[256], Ar[16], Arb[16];
+    for (int64_t p = 0; p < npairs; p++) {
+        int64_t a = (int64_t)k1[p], b = (int64_t)k2[p];
+        zc *Q3 = P3 ? (zc *)P3 + p * nb4 : NULL, *Q4 = P4 ? (zc *)P4 + p * nb4 : NULL;
+        zc *Q5 = P5 ? (zc *)P5 + p * nb4 : NULL, *Q6 = P6 ? (zc *)P6 + p * nb4 : NULL;
+        if (Q3) for (int q = 0; q < nb4; q++) Q3[q] = Q4[q] = zmk(0, 0);
+        if (Q5) for (int q = 0; q < nb4; q++) Q5[q] = Q6[q] = zmk(0, 0);
+        for (int64_t c3 = 0; c3 < nk && Q3; c3++) {  /* calc_P3P4(ik1 = a, ika = b) */
+            if (b == c3 && !antisym) continue;
+            if (antisym && b == c3 && a == b) continue;
+            double k4v[3];
+            for (int d = 0; d < 3; d++) k4v[d] = kvec[3 * a + d] + kvec[3 * c3 + d] - kvec[3 * b + d];
+            int64_t c4 = kmap[mesh_code(k4v, kmesh)];
+            if (c4 < 0) continue;
+            if (!ee_calc_A(a, c4, c3, b, nb, nw, U, kvec, kmesh, e, lindblad, pexp, ethr, antisym, GGT, vq_cell,
+                           screening, prefac_vq, prefac_vq_bare, qscr2_debye, fac0, fac2, fac4, A1, A2))
+                continue;
+            const double *f3 = f + c3 * nb, *f4 = f + c4 * nb;
+            for (int i1 = 0; i1 < nb; i1++)
+                for (int ia = 0; ia < nb; ia++) {
+                    for (int i4 = 0; i4 < nb; i4++)
+                        for (int i3 = 0; i3 < nb; i3++) {
+                            zc x = A1[(i1 * nb + i4) * nb2 + i3 * nb + ia];
+                            Ar[i4 * nb + i3] = zscale(zscale(x, 1 - f3[i3]), f4[i4]);
+                            Arb[i4 * nb + i3] = zscale(zscale(x, f3[i3]), 1 - f4[i4]);
+                        }
+                    for (int i2 = 0; i2 < nb; i2++)
+                        for (int ib = 0; ib < nb; ib++) {
+                            int q = (i1 * nb + i2) * nb2 + ia * nb + ib;
+                            for (int i4 = 0; i4 < nb; i4++)
+                                for (int i3 = 0; i3 < nb; i3++) {
+                                    zc y = zconj(A2[(i2 * nb + i4) * nb2 + i3 * nb + ib]);
+                                    Q3[q] = zadd(Q3[q], zmul(Ar[i4 * nb + i3], y));
+                                    Q4[q] = zadd(Q4[q], zmul(Arb[i4 * nb + i3], y));
+                                }
+                        }
+                }
+        }
+        for (int64_t c3 = 0; c3 < nk && Q5; c3++) {  /* calc_P5P6(ik1 = a, ika = b) */
+            if (a == c3 && !antisym) continue;
+            if (antisym && a == c3 && b == c3) continue;
+            double k4v[3];
+            for (int d = 0; d < 3; d++) k4v[d] = kvec[3 * a + d] + kvec[3 * b + d] - kvec[3 * c3 + d];
+            int64_t c4 = kmap[mesh_code(k4v, kmesh)];
+            if (c4 < 0) continue;
+            if (!ee_calc_A(a, c3, b, c4, nb, nw, U, kvec, kmesh, e, lindblad, pexp, ethr, antisym, GGT, vq_cell,
+                           screening, prefac_vq, prefac_vq_bare, qscr2_debye, fac0, fac2, fac4, A1, A2))
+                continue;
+            const double *f3 = f + c3 * nb, *f4 = f + c4 * nb;
+            for (int i1 = 0; i1 < nb; i1++)
+                for (int ib = 0; ib < nb; ib++) {
+                    for (int i3 = 0; i3 < nb; i3++)
+                        for (int i4 = 0; i4 < nb; i4++) {
+                            zc x = A1[(i1 * nb + i3) * nb2 + ib * nb + i4];
+                            Ar[i3 * nb + i4] = zscale(zscale(x, f3[i3]), f4[i4]);
+                            Arb[i3 * nb + i4] = zscale(zscale(x, 1 - f3[i3]), 1 - f4[i4]);
+                        }
+                    for (int i2 = 0; i2 < nb; i2++)
+                        for (int ia = 0; ia < nb; ia++) {
+                            int q = (i1 * nb + i2) * nb2 + ia * nb + ib;
+                            for (int i3 = 0; i3 < nb; i3++)
+                                for (int i4 = 0; i4 < nb; i4++) {
+                                    zc y = zconj(A2[(i2 * nb + i3) * nb2 + ia * nb + i4]);
+                                    Q5[q] = zadd(Q5[q], zmul(Ar[i3 * nb + i4], y));
+                                    Q6[q] = zadd(Q6[q], zmul(Arb[i3 * nb + i4], y));
+                                }
+                        }
+                }
+        }
+        if (Q3) for (int q = 0; q < nb4; q++) { Q3[q] = zscale(Q3[q], pf); Q4[q] = zscale(Q4[q], pf); }
+        if (Q5) for (int q = 0; q < nb4; q++) { Q5[q] = zscale(Q5[q], pf); Q6[q] = zscale(Q6[q], pf); }
+    }
+    free(kmap);
+    return 0;
+}
+
+/* The e-e terms of the linearised operator beyond set_Lsc's P1/P2 form,      */
+/* added into (Lscij, Lscji) of every stored pair (file order) at f_eq:       */
+/*  - extra x add_Lscij_from_P1_P2 of (Pee1, Pee2) (ElectronPhonon_linear.cpp:  */
+/*    183-207): the reference's fac2 = ee_antisymmetry doubles the e-e         */
+/*    Lscij/Lscji (not Lscii), i.e. extra = 1 with exchange, else 0;           */
+/*  - add_Lscij_from_P3_P4 (:209-231) and add_Lscij_from_P5_P6 (:233-253)      */
+/*    when P3..P6 are given (eeMode other than Pee_fixed_at_eq).               */
+int oracle_add_Lsc_ee(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2, const double *f_eq,
+                      double extra, const double *Pee1, const double *Pee2, const double *P3, const double *P4,
+                      const double *P5, const double *P6, double *Lij, double *Lji)
+{
+    const int nb2 = nb * nb, nb4 = nb2 * nb2;
+    if (nb < 1 || nb > 8 || nk < 0 || npairs < 0) return -1;
+    for (int64_t ip = 0; ip < npairs; ip++) {
+        const int64_t ik = (int64_t)k1[ip], jk = (int64_t)k2[ip];
+        if (ik >= nk || jk >= nk) return -2;
+        const double *fi = f_eq + ik * nb, *fj = f_eq + jk * nb;
+        zc *a = (zc *)Lij + (size_t)ip * nb4, *b = (zc *)Lji + (size_t)ip * nb4;
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                const int i12 = i1 * nb + i2, n12 = i12 * nb2, i21 = i2 * nb + i1;
+                for (int ia = 0; ia < nb; ia++)
+                    for (int ib = 0; ib < nb; ib++) {
+                        const int iab = ia * nb + ib, i12ab = n12 + iab, iab12 = iab * nb2 + i12;
+                        if (extra != 0 && Pee1) {
+                            const zc *p1 = (const zc *)Pee1 + (size_t)ip * nb4, *p2 = (const zc *)Pee2 + (size_t)ip * nb4;
+                            zc x = zadd(zscale(p1[i12ab], 1 - fi[i1]), zscale(p2[iab12], fi[i2]));
+                            zc y = zadd(zscale(zconj(p2[i12ab]), 1 - fj[i1]), zscale(zconj(p1[iab12]), fj[i2]));
+                            a[i12ab] = zadd(a[i12ab], zscale(x, extra));
+                            b[i12ab] = zadd(b[i12ab], zscale(y, extra));
+                        }
+                        if (P3) {
+                            const zc *p3 = (const zc *)P3 + (size_t)ip * nb4, *p4 = (const zc *)P4 + (size_t)ip * nb4;
+                            a[i12ab] = zsub(a[i12ab], zadd(zscale(p3[i12ab], 1 - fi[i1]), zscale(p4[i12ab], fi[i2])));
+                            b[i12ab] = zsub(b[i12ab], zadd(zscale(zconj(p4[iab12]), 1 - fj[i1]),
+                                                           zscale(zconj(p3[iab12]), fj[i2])));
+                        }
+                        if (P5) {
+                            const zc *p5 = (const zc *)P5 + (size_t)ip * nb4, *p6 = (const zc *)P6 + (size_t)ip * nb4;
+                            const int iba21 = (ib * nb + ia) * nb2 + i21;
+                            a[i12ab] = zsub(a[i12ab], zadd(zscale(p5[i12ab], 1 - fi[i1]), zscale(p6[i12ab], fi[i2])));
+                            b[i12ab] = zsub(b[i12ab], zadd(zscale(p5[iba21], 1 - fj[i1]), zscale(p6[iba21], fj[i2])));
+                        }
+                    }
+            }
+    }
     return 0;
 }
 

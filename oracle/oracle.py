@@ -51,10 +51,16 @@ def lib():
         L.oracle_ee_model_P.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d,
                                         ctypes.POINTER(ctypes.c_int), _d, _d, _d, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _d, _d, ctypes.c_int, _d, ctypes.c_int] + \
-            [ctypes.c_double] * 6
+            [ctypes.c_double] * 6 + [_d, _d, _d, _d, _d]
+        L.oracle_ee_model_P3P6.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _i64, _u64, _u64, _d, _d,
+                                           ctypes.POINTER(ctypes.c_int), _d, _d, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d, ctypes.c_int] + \
+            [ctypes.c_double] * 6 + [_d, _d, _d, _d]
+        L.oracle_add_Lsc_ee.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, ctypes.c_double, _d, _d, _d, _d,
+                                        _d, _d, _d, _d]
         L.oracle_eimp_rpa_vq.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _d, ctypes.POINTER(ctypes.c_int), _d, _d,
                                          _d, ctypes.c_double, _d, ctypes.c_double, ctypes.c_double, _i64, _u64, _u64,
-                                         _d, _d, ctypes.POINTER(ctypes.c_int64)]
+                                         _d, _d, ctypes.POINTER(ctypes.c_int64), _d]
         L.oracle_spin.argtypes = [_i64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d, _d, _d, _d,
                                   ctypes.c_double, ctypes.c_int, ctypes.c_double, _d]
         L.oracle_spin.restype = ctypes.c_double
@@ -228,9 +234,10 @@ def eimp_vq(m, q2):
                                 m["qscr2_debye"], m["fac0"], m["fac2"], m["fac4"])
 
 
-def eimp_rpa_vq(k1, k2, U, kvec, e, f, m):
+def eimp_rpa_vq(k1, k2, U, kvec, e, f, m, cells=False):
     """(v(q) per stored pair, q_scr^2 per q of the mesh's q list) of the static RPA
-    screening (Coulomb_Model.h:690-744,820-828)."""
+    screening (Coulomb_Model.h:690-744,820-828); with cells=True also v(q) of
+    every mesh cell (0 for cells that are no k-point difference, and at q = 0)."""
     e = _f64(e)
     nk, nb = e.shape
     U = _c128(U)
@@ -242,7 +249,10 @@ def eimp_rpa_vq(k1, k2, U, kvec, e, f, m):
     nq = ctypes.c_int64(0)
     lib().oracle_eimp_rpa_vq(nk, nb, U.shape[1], _p(_f64(kvec)), km, _p(U), _p(e), _p(_f64(f)),
                              float(m["temperature"]), _p(_f64(m["GGT"])), float(m["prefac_vq"]), float(m["nk_full"]),
-                             len(k1), _p(k1), _p(k2), _p(vq), _p(qs), ctypes.byref(nq))
+                             len(k1), _p(k1), _p(k2), _p(vq), _p(qs), ctypes.byref(nq),
+                             _p(qs_cell := np.zeros(int(np.prod(m["kmesh"])))))
+    if cells:
+        return vq, qs[:nq.value], qs_cell
     return vq, qs[:nq.value]
 
 
@@ -272,13 +282,20 @@ def eimp_model_P(k1, k2, U, kvec, e, m, f=None):
     return P1, P2
 
 
-def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, lindblad=True, antisym_model=None):
-    """elecelec_model::calc_P at the equilibrium f (ElecElec_Model.h:185-262);
-    vq_pair: v(k1 - k2) of each stored pair.  antisym_model: a screening dict
-    (eimp_model fields) to include the exchange term (ee_antisymmetry), whose
-    v(k1 - k4) is evaluated from it."""
-    m = antisym_model or {"GGT": np.eye(3), "screening": 0, "prefac_vq": 0.0, "prefac_vq_bare": 0.0,
-                          "qscr2_debye": 0.0, "fac0": 0.0, "fac2": 0.0, "fac4": 0.0}
+_NO_SCREENING = {"GGT": np.eye(3), "screening": 0, "prefac_vq": 0.0, "prefac_vq_bare": 0.0,
+                 "qscr2_debye": 0.0, "fac0": 0.0, "fac2": 0.0, "fac4": 0.0}
+
+
+def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, lindblad=True, antisym_model=None,
+               vq_cell=None, dm=None, dm1=None, with_dP=False):
+    """elecelec_model::calc_P (ElecElec_Model.h:185-262); vq_pair: v(k1 - k2)
+    of each stored pair.  antisym_model: a screening dict (eimp_model fields)
+    to include the exchange term (ee_antisymmetry), whose v(k1 - k4) is
+    evaluated from it, or from vq_cell (v per mesh cell, the static RPA) when
+    given.  dm, dm1 (nk x nb^2, Schroedinger picture): P at that state
+    (calc_A1rho) instead of at f; with with_dP (alg_linearize_dPee) P stays at
+    f (= f_eq) and (P1, P2, dP1, dP2) is returned, dP from calc_A1rho_dP."""
+    m = antisym_model or _NO_SCREENING
     e = _f64(e)
     nk, nb = e.shape
     U = _c128(U)
@@ -287,12 +304,69 @@ def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, li
     km = (ctypes.c_int * 3)(*[int(x) for x in kmesh])
     P1 = np.zeros((len(k1), nb ** 4), np.complex128)
     P2 = np.zeros_like(P1)
+    dP1 = np.zeros_like(P1) if with_dP else None
+    dP2 = np.zeros_like(P1) if with_dP else None
+    if (dm is None) != (dm1 is None) or (with_dP and dm is None):
+        raise ValueError("dm and dm1 go together (and with_dP needs them)")
     lib().oracle_ee_model_P(nk, nb, U.shape[1], len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), km, _p(e),
                             _p(_f64(f)), _p(_f64(vq_pair)), int(lindblad), float(degauss), float(ethr),
                             float(nk_full), _p(P1), _p(P2), int(antisym_model is not None), _p(_f64(m["GGT"])),
                             int(m["screening"]), float(m["prefac_vq"]), float(m["prefac_vq_bare"]),
-                            float(m["qscr2_debye"]), float(m["fac0"]), float(m["fac2"]), float(m["fac4"]))
+                            float(m["qscr2_debye"]), float(m["fac0"]), float(m["fac2"]), float(m["fac4"]),
+                            _p(None if vq_cell is None else _f64(vq_cell)),
+                            _p(None if dm is None else _c128(dm)), _p(None if dm1 is None else _c128(dm1)),
+                            _p(dP1), _p(dP2))
+    if with_dP:
+        return P1, P2, dP1, dP2
     return P1, P2
+
+
+def ee_model_P3P6(k1, k2, U, kvec, e, f, kmesh, degauss, ethr, nk_full, model, lindblad=True, antisym=False,
+                  vq_cell=None):
+    """elecelec_model::calc_P3P4 and calc_P5P6 at f for every stored pair
+    (scattering_model/ElecElec_Model_linear.cpp:3-111); model: a screening dict
+    (eimp_model fields) for the closed-form v(q), or vq_cell (v per mesh cell,
+    static RPA).  P3 and P4 are zero with antisym, as the reference skips them.
+    Returns (P3, P4, P5, P6)."""
+    e = _f64(e)
+    nk, nb = e.shape
+    U = _c128(U)
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    km = (ctypes.c_int * 3)(*[int(x) for x in kmesh])
+    out = [np.zeros((len(k1), nb ** 4), np.complex128) for _ in range(4)]
+    m = model
+    rc = lib().oracle_ee_model_P3P6(nk, nb, U.shape[1], len(k1), _p(k1), _p(k2), _p(U), _p(_f64(kvec)), km, _p(e),
+                                    _p(_f64(f)), int(lindblad), float(degauss), float(ethr), float(nk_full),
+                                    int(antisym), _p(_f64(m["GGT"])), _p(None if vq_cell is None else _f64(vq_cell)),
+                                    int(m["screening"]), float(m["prefac_vq"]), float(m["prefac_vq_bare"]),
+                                    float(m["qscr2_debye"]), float(m["fac0"]), float(m["fac2"]), float(m["fac4"]),
+                                    _p(None if antisym else out[0]), _p(None if antisym else out[1]), _p(out[2]),
+                                    _p(out[3]))
+    if rc != 0:
+        raise ValueError(f"oracle_ee_model_P3P6 failed: {rc}")
+    return tuple(out)
+
+
+def add_Lsc_ee(k1, k2, f_eq, Lij, Lji, extra=0.0, Pee=None, P3456=None):
+    """The e-e terms of set_Lsc beyond the P1/P2 form (ElectronPhonon_linear.cpp:
+    183-253): extra x add_Lscij_from_P1_P2 of Pee = (P1, P2) (extra = 1 under
+    ee_antisymmetry, whose fac2 doubles the e-e Lscij/Lscji), and the P3/P4,
+    P5/P6 terms.  Adds into copies of (Lij, Lji) and returns them."""
+    f_eq = _f64(f_eq)
+    nk, nb = f_eq.shape
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    Lij, Lji = _c128(Lij).copy(), _c128(Lji).copy()
+    P1, P2 = (None, None) if Pee is None else (_c128(Pee[0]), _c128(Pee[1]))
+    P3, P4, P5, P6 = (None,) * 4 if P3456 is None else tuple(_c128(x) for x in P3456)
+    if P3 is not None and not np.any(P3) and not np.any(P4):
+        P3 = P4 = None  # skipped under exchange
+    rc = lib().oracle_add_Lsc_ee(nk, nb, len(k1), _p(k1), _p(k2), _p(f_eq), float(extra), _p(P1), _p(P2), _p(P3),
+                                 _p(P4), _p(P5), _p(P6), _p(Lij), _p(Lji))
+    if rc != 0:
+        raise ValueError(f"oracle_add_Lsc_ee failed: {rc}")
+    return Lij, Lji
 
 
 def compute_P_eimp(A1, A2):

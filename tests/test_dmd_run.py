@@ -97,13 +97,28 @@ def test_dry_run_parses_continuations_units_and_substeps(tmp_path):
     assert "alg_picture = schrodinger" in out
 
 
+def test_dry_run_accepts_the_reference_eph_ei_ee_run(tmp_path):
+    """The reference's linearized e-ph + e-i + e-e example (its own param.in,
+    tests/golden/gaas_eph_ei_ee_param.in: alg_linearize = 1, eeMode =
+    Pee_update, impMode = model_ionized, scrMode = medium with scrFormula left
+    at its default RPA, the param.in's own "srcFormula" misspelling being an
+    unread key there too) with the GaAs ldbd_data: every key is carried."""
+    d = _gaas_folder(tmp_path)
+    shutil.copy(os.path.join(os.path.dirname(GOLD), "gaas_eph_ei_ee_param.in"), d / "param.in")
+    rc, out, err = _dry(d)
+    assert rc == 0, err
+    assert "alg_linearize = 1" in out and "eeMode = Pee_update" in out and "scrFormula = RPA" in out
+    assert "impurities = 1" in out and "model_impurities = 1" in out
+    assert 'key "srcFormula" is not read' in out
+
+
 @pytest.mark.parametrize("line,msg", [
-    ("eeMode = Pee_update\nscrMode = medium", "rebuilt from rho"),
+    ("eeMode = Pee_update\nscrMode = medium", "want to update Pee but set freq_update_ee_model as 0"),
+    ("eeMode = Pee_update\nscrMode = medium\nfreq_update_ee_model = -1", "freq_update_ee_model < 0"),
     ("eeMode = Pee_fixed_at_eq", "scrMode should not be none if considering electron-electron"),
-    ("eeMode = Pee_fixed_at_eq\nscrMode = medium\nee_antisymmetry = 1", "closed-form screenings"),
     ("alg_only_ee = 1\nalg_eph_enable = 0\nimpurity_density = 1e17\nimpMode = ab_neutral", "alg_only_ee is only possible"),
     ("eeMode = other", "eeMode must be none or Pee_fixed_at_eq"),
-    ("alg_linearize_dPee = 1", "linearized e-e operator"),
+    ("alg_linearize_dPee = 1", "linearize_dPee"),
     ("impurity_density = 1e17\nimpMode = model_ionized", "scrMode should not be none"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = lindhard", "built for the static RPA"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = heg\ndynamic_screening = model", "built for the static RPA"),
