@@ -218,6 +218,7 @@ PROTOTYPES = {
     "dmd_scatt_create_from_ldbd": [_cs, _cs, _d, _d, _i, ctypes.POINTER(_vp)],
     "dmd_scatt_load_sP": [_vp, _cs, _pLI, _cs, _d, _d],
     "dmd_scatt_threshold_P": [_vp, _d],
+    "dmd_scatt_compress_P": [_vp, _pi64],
     "dmd_scatt_mask_pairs": [_vp, _vp],
     "dmd_scatt_set_semiclassical": [_vp, _i],
     "dmd_scatt_set_phenom_relax": [_vp, _d, _i, _i, _vp],

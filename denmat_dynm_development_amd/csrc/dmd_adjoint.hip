@@ -283,6 +283,10 @@ using namespace dmd;
 
 extern "C" int dmd_scatt_vjp(dmd_scatt *h, double t, const double *rho, const double *v, double *g)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_vjp");
+        if (sg) return sg;
+    }
     if (!h || !rho || !v || !g) return fail(DMD_ERR_ARG, "dmd_scatt_vjp: bad arguments");
     int rc;
     if ((rc = vjp_ready(h))) return rc;
@@ -300,6 +304,10 @@ extern "C" int dmd_scatt_vjp(dmd_scatt *h, double t, const double *rho, const do
 
 extern "C" int dmd_scatt_combine_P(dmd_scatt *h, int ncomp, dmd_scatt *const *comps, const double *weights)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_combine_P");
+        if (sg) return sg;
+    }
     if (!h || ncomp < 1 || !comps || !weights) return fail(DMD_ERR_ARG, "dmd_scatt_combine_P: bad arguments");
     for (int c = 0; c < ncomp; c++) {
         const dmd_scatt *o = comps[c];
@@ -334,6 +342,10 @@ extern "C" int dmd_scatt_adjoint_rk4(dmd_scatt *h, double t0, double dt, int64_t
                                      int checkpoint_every, const double *rho0, const double *dL_dobs, int ncomp,
                                      dmd_scatt *const *comps, double *dL_dtheta, double *dL_drho0)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_adjoint_rk4");
+        if (sg) return sg;
+    }
     if (!h || !rho0 || n_steps < 1 || checkpoint_every < 1 || (ncomp > 0 && (!comps || !dL_dtheta)))
         return fail(DMD_ERR_ARG, "dmd_scatt_adjoint_rk4: bad arguments");
     if (h->semiclassical || h->ph_tau_inv != 0.0)

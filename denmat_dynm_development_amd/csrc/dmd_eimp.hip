@@ -381,6 +381,7 @@ extern "C" int dmd_scatt_add_eimp_model(dmd_scatt *h, const uint64_t *k1st, cons
 {
     if (!h || !m || (h->np > 0 && (!k1st || !k2nd || !U || !kvec)))
         return fail(DMD_ERR_ARG, "dmd_scatt_add_eimp_model: null argument");
+    if (const int sg = sp_guard(h, "dmd_scatt_add_eimp_model")) return sg;
     if (m->nb_wannier <= 0 || !(m->degauss > 0) || !(m->temperature > 0) || m->screening < DMD_SCR_UNSCREENED ||
         m->screening > DMD_SCR_LINDHARD)
         return fail(DMD_ERR_ARG, "dmd_scatt_add_eimp_model: bad model (nb_wannier, degauss, temperature, screening)");

@@ -194,6 +194,12 @@ struct dmd_scatt {
     int32_t *d_dk = nullptr;             // and rows
     double2 *d_adj = nullptr;            // 3 x nk x nb^2
     double *d_dotpart = nullptr;         // dot-product partials
+    // compressed P (dmd_scatt_compress_P, nb = 2 tiles): per-slot nonzero
+    // masks and value offsets, and the nonzero values; d_P is released while
+    // they are set
+    uint32_t *d_spmask = nullptr, *d_spofs = nullptr;
+    double2 *d_spv = nullptr;
+    int64_t sp_values = 0;  // stored values
     // linearized operator (alg_linearize): the P store holds Lscij | Lscji
     bool linear = false;
     int semiclassical = 0;  // alg_semiclassical: the integrators keep rho diagonal
@@ -243,6 +249,14 @@ inline int dalloc_t(T **p, size_t count, const char *what)
 }
 
 
+// refusal of the operations that need the dense P store once P is compressed
+inline int sp_guard(const dmd_scatt *h, const char *who)
+{
+    if (h && h->d_spv)
+        return fail(DMD_ERR_STATE, std::string(who) + ": P is stored compressed (dmd_scatt_compress_P); "
+                                                     "set P up again on a new handle to change it");
+    return DMD_OK;
+}
 int to_device(dmd_scatt *h, void *dst, const void *src, size_t bytes);
 int from_device(dmd_scatt *h, void *dst, const void *src, size_t bytes);
 int set_clock(dmd_scatt *h, double t0, double dt, long long step);

@@ -86,6 +86,7 @@ extern "C" int dmd_scatt_load_P(dmd_scatt *h, const char *dir, const dmd_ldbd_in
                                 int impurity, double scale, int add)
 {
     if (!h || !info || impurity < 0) return fail(DMD_ERR_ARG, "dmd_scatt_load_P: bad arguments");
+    if (const int sg = sp_guard(h, "dmd_scatt_load_P")) return sg;
     const int nbe = info->b_top_eph - info->b_bot_eph;
     if (nbe != h->nb || info->nkpair != h->np)
         return fail(DMD_ERR_ARG, "dmd_scatt_load_P: the files hold " + std::to_string(info->nkpair) + " pairs of " +
@@ -143,6 +144,7 @@ extern "C" int dmd_scatt_load_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_i
                                  double scale, double thr)
 {
     if (!h || !info || !(thr >= 0)) return fail(DMD_ERR_ARG, "dmd_scatt_load_sP: bad arguments");
+    if (const int sg = sp_guard(h, "dmd_scatt_load_sP")) return sg;
     const int nbe = info->b_top_eph - info->b_bot_eph;
     if (nbe != h->nb || info->nkpair != h->np)
         return fail(DMD_ERR_ARG, "dmd_scatt_load_sP: the files hold " + std::to_string(info->nkpair) + " pairs of " +
@@ -229,6 +231,10 @@ extern "C" int dmd_scatt_load_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_i
 extern "C" int dmd_ldbd_write_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, const char *scatt,
                                  double thr)
 {
+    {
+        const int sg = sp_guard(h, "dmd_ldbd_write_sP");
+        if (sg) return sg;
+    }
     if (!h || !info || !(thr >= 0)) return fail(DMD_ERR_ARG, "dmd_ldbd_write_sP: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator, not P");

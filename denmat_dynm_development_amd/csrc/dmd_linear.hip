@@ -233,6 +233,10 @@ using namespace dmd;
 // evolve_linear (evolve_eph_linear.cpp:3-45) from then on.  FP64 only.
 extern "C" int dmd_scatt_set_Lsc(dmd_scatt *h, const double *f_eq)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_set_Lsc");
+        if (sg) return sg;
+    }
     if (!h || !f_eq) return fail(DMD_ERR_ARG, "dmd_scatt_set_Lsc: bad arguments");
     if (h->precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "the linearized operator is FP64 only");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");

@@ -502,14 +502,22 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
 // LIN: the slot holds the linearized operator Lscij | Lscji and rhoS holds
 // drho_S; forward X = Lscij drho_k', reverse Y = Lscji drho_k, increments
 // X + X^+ and Y + Y^+ (evolve_eph_linear.cpp:26-37).
-template <bool DM1, bool LIN>
+// SP: P held compressed (dmd_scatt_compress_P): per slot a mask of its
+// nonzero elements (bit el < 16: P1 element el, bit 16 + el: P2 element el) and
+// the offset of its nonzero values, stored one after another in element order
+// (value j of slot s at P[gofs[s] + j]).  The lane rebuilds the slot's 32
+// elements in registers (zeros where the mask is clear) and runs the dense
+// arithmetic.
+template <bool DM1, bool LIN, bool SP = false>
 __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint32_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
                                                    const int32_t *__restrict__ gbeg,
                                                    const double2 *__restrict__ rhoS,
-                                                   const double2 *__restrict__ rho1S, double *__restrict__ hacc)
+                                                   const double2 *__restrict__ rho1S, double *__restrict__ hacc,
+                                                   const uint32_t *__restrict__ pmask = nullptr,
+                                                   const uint32_t *__restrict__ gofs = nullptr)
 {
     constexpr int NB2 = 4, NB4 = 16, EP = 32, R = DMD_TILE_R;  // nb = 2
     constexpr int W = 4, RPT = R / 256;  // rows per thread in the flush
@@ -575,10 +583,28 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             auto om_kp = [&](int q) { return DM1 ? r1kp[q] : one_minus(rkp, q); };
             const bool rev = ok && k < kp;
             double2 p1[NB4], p2[NB4];
+            if (SP) {
+                const uint32_t mask = ld_stream(pmask + s);
+                const double2 *gv = P + ld_stream(gofs + s);
+                int pos = 0;
 #pragma unroll
-            for (int el = 0; el < NB4; el++) p1[el] = ld_stream(g + el * 8);
+                for (int el = 0; el < NB4; el++) {
+                    const bool on = (mask >> el) & 1u;
+                    p1[el] = on ? ld_stream(gv + pos) : make_double2(0.0, 0.0);
+                    pos += on;
+                }
 #pragma unroll
-            for (int el = 0; el < NB4; el++) p2[el] = ld_stream(g + (NB4 + el) * 8);
+                for (int el = 0; el < NB4; el++) {
+                    const bool on = (mask >> (NB4 + el)) & 1u;
+                    p2[el] = on ? ld_stream(gv + pos) : make_double2(0.0, 0.0);
+                    pos += on;
+                }
+            } else {
+#pragma unroll
+                for (int el = 0; el < NB4; el++) p1[el] = ld_stream(g + el * 8);
+#pragma unroll
+                for (int el = 0; el < NB4; el++) p2[el] = ld_stream(g + (NB4 + el) * 8);
+            }
             if (LIN) {
                 double2 x[NB2], y[NB2];
 #pragma unroll
@@ -820,6 +846,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_vrec); dfree(d_vq); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
     dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB); dfree(d_adj); dfree(d_dotpart);
     dfree(d_dpos); dfree(d_dk); dfree(d_flin); dfree(d_ad); dfree(d_adpart); dfree(d_phfeq);
+    dfree(d_spmask); dfree(d_spofs); dfree(d_spv);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1104,6 +1131,10 @@ extern "C" int dmd_scatt_destroy(dmd_scatt *h)
 // (n x nb^4 complex128 each), scaled; add = 0 sets, 1 adds.  Stream-ordered.
 int dmd::place_P_dev(dmd_scatt *h, const double2 *a, const double2 *b, int64_t n, int64_t first, double scale, int add)
 {
+    {
+        const int sg = sp_guard(h, "P update");
+        if (sg) return sg;
+    }
     if (n <= 0) return DMD_OK;
     if (h->precision != DMD_PREC_FP64) return tiles4_place(h, a, b, n, first, scale, add);
     const int64_t tot = n * 2 * (int64_t)h->nb4;
@@ -1119,6 +1150,7 @@ static int put_P(dmd_scatt *h, int64_t first, int64_t count, const double *P1, c
                  int add)
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (const int sg = sp_guard(h, "P upload")) return sg;
     if (first < 0 || count < 0 || first + count > h->np) return fail(DMD_ERR_ARG, "pair range outside the stored pairs");
     if (h->linear && (add || first != 0 || count != h->np))
         return fail(DMD_ERR_STATE, "the handle holds the linearized operator: upload the whole P first");
@@ -1184,6 +1216,10 @@ extern "C" int dmd_scatt_upload_P_range(dmd_scatt *h, int64_t first, int64_t cou
 
 extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, double *P1, double *P2)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_download_P");
+        if (sg) return sg;
+    }
     if (!h || !P1 || !P2 || first < 0 || count < 0 || first + count > h->np)
         return fail(DMD_ERR_ARG, "dmd_scatt_download_P: bad arguments");
     if (count == 0) return DMD_OK;
@@ -1214,7 +1250,12 @@ extern "C" int dmd_scatt_download_P(dmd_scatt *h, int64_t first, int64_t count, 
 // quantised into a reduced one).  Both handles hold the same stored pairs.
 extern "C" int dmd_scatt_copy_P(dmd_scatt *dst, dmd_scatt *src, double scale, int add)
 {
+    {
+        const int sg = sp_guard(src, "dmd_scatt_copy_P");
+        if (sg) return sg;
+    }
     if (!dst || !src || dst == src) return fail(DMD_ERR_ARG, "dmd_scatt_copy_P: two distinct handles are needed");
+    if (const int sg = sp_guard(dst, "dmd_scatt_copy_P")) return sg;
     if (dst->nk != src->nk || dst->nb != src->nb || dst->np != src->np || dst->dev != src->dev)
         return fail(DMD_ERR_ARG, "dmd_scatt_copy_P: the handles differ in nk, nb, stored pairs or device");
     if (!src->has_P) return fail(DMD_ERR_STATE, "dmd_scatt_copy_P: the source holds no P");
@@ -1275,6 +1316,10 @@ __global__ void k_threshold(int64_t n, double2 *__restrict__ a, double2 *__restr
 // in chunks of 64 MiB per array: fetch into pair order, zero, place back.
 extern "C" int dmd_scatt_threshold_P(dmd_scatt *h, double thr)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_threshold_P");
+        if (sg) return sg;
+    }
     if (!h || !(thr >= 0)) return fail(DMD_ERR_ARG, "dmd_scatt_threshold_P: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator, not P");
@@ -1318,6 +1363,10 @@ __global__ void k_zero_pairs(int64_t n, int per, const uint8_t *__restrict__ kee
 // the pair filter of evolve (evolve_eph.cpp:52-58) applied to the stored P
 extern "C" int dmd_scatt_mask_pairs(dmd_scatt *h, const uint8_t *keep)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_mask_pairs");
+        if (sg) return sg;
+    }
     if (!h || (!keep && h->np > 0)) return fail(DMD_ERR_ARG, "dmd_scatt_mask_pairs: bad arguments");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     if (h->linear) return fail(DMD_ERR_STATE, "the handle holds the linearized operator, not P");
@@ -1359,6 +1408,10 @@ extern "C" int dmd_scatt_mask_pairs(dmd_scatt *h, const uint8_t *keep)
 
 extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
 {
+    {
+        const int sg = sp_guard(h, "dmd_scatt_generate_P");
+        if (sg) return sg;
+    }
     if (!h || !p) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: null argument");
     if (p->nb != h->nb || p->nk != h->nk) return fail(DMD_ERR_ARG, "dmd_scatt_generate_P: params do not match the handle");
     if (h->np == 0) { h->has_P = true; return DMD_OK; }
@@ -1385,6 +1438,8 @@ extern "C" int dmd_scatt_generate_P(dmd_scatt *h, const dmd_synth_params *p)
 extern "C" int dmd_scatt_set_flags(dmd_scatt *h, int expt, int schrodinger, int kernel)
 {
     if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (h->d_spv && kernel != DMD_KERNEL_AUTO && kernel != DMD_KERNEL_TILES)
+        return fail(DMD_ERR_STATE, "dmd_scatt_set_flags: the compressed P runs on the tile kernel only");
     if (kernel < DMD_KERNEL_AUTO || kernel > DMD_KERNEL_TILES) return fail(DMD_ERR_ARG, "unknown kernel");
     if (schrodinger && expt) return fail(DMD_ERR_ARG, "in the schrodinger picture expt must be 0 (parameters.cpp:452-453)");
     if (h->precision != DMD_PREC_FP64 && kernel != DMD_KERNEL_AUTO && kernel != DMD_KERNEL_TILES)
@@ -1522,6 +1577,14 @@ static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
         if (h->linear)
             hipLaunchKernelGGL((k_tiles_soa<false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
                                h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
+        else if (h->d_spv && dm1)
+            hipLaunchKernelGGL((k_tiles_soa<true, false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk,
+                               h->d_spv, h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S,
+                               h->d_acc_herm, h->d_spmask, h->d_spofs);
+        else if (h->d_spv)
+            hipLaunchKernelGGL((k_tiles_soa<false, false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk,
+                               h->d_spv, h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S,
+                               h->d_acc_herm, h->d_spmask, h->d_spofs);
         else if (dm1)
             hipLaunchKernelGGL((k_tiles_soa<true, false>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
                                h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
@@ -1659,6 +1722,8 @@ extern "C" int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, 
     if (!h) return fail(DMD_ERR_ARG, "null handle");
     if (n_pairs) *n_pairs = h->np;
     if (p_bytes) *p_bytes = h->np * 2 * h->nb4 * (int64_t)h->p_esize + (h->d_pscale ? h->np * 4 : 0);
+    if (p_bytes && h->d_spv)  // compressed: values, per-slot masks and offsets
+        *p_bytes = h->sp_values * 16 + h->nslots * 8;
     if (last_kernel_ms) *last_kernel_ms = h->last_ms;
     return DMD_OK;
 }

@@ -820,6 +820,10 @@ int run(const std::string &folder, bool dry)
         eph.set_phenom_relax(s.tau_phenom, s.bStart_tau, s.bEnd_tau, rows);
         std::printf("phenomenological relaxation: tau = %.10le on bands [%d, %d)\n", s.tau_phenom, s.bStart_tau, s.bEnd_tau);
     }
+    // the sparse modes stream P by its nonzero elements (nb = 2); Pee_update
+    // rebuilds P in place and keeps it dense
+    if ((s.sparseP || s.Pin_is_sparse) && !s.linearize && !pee_update && s.scatt_enable && eph.compress_P())
+        std::printf("sparse P: stored by its nonzero elements\n");
     dmd_scatt *h = eph.handle();
     // H_coh = H_BS + H_Ez (main.cpp:51-52, DenMat.cpp:7-14): H_BS = Btot . s with
     // Btot = B, or gs/2 B + Bso_k under alg_read_Bso, plus B/2 . l under needL

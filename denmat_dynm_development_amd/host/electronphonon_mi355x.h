@@ -200,6 +200,16 @@ public:
         if (write_files) check(dmd_ldbd_write_sP(h_, dir_.c_str(), &info_, scatt_.c_str(), thr_sparseP), "sP files");
     }
 
+    // The sparse modes' per-pair element lists (evolve_eph.cpp:69-86): P kept by
+    // its nonzero elements so the bytes streamed fall with the density (nb = 2
+    // FP64 tile layout; false where the store stays dense)
+    bool compress_P()
+    {
+        if (nb != 2) return false;
+        check(dmd_scatt_compress_P(h_, nullptr), "compressed P");
+        return true;
+    }
+
     // alg.semiclassical (parameters.cpp:101): the driver-side right-hand side
     // keeps rho diagonal (DensityMatrix_Dynamics.h:249,277)
     void set_semiclassical(bool on) { check(dmd_scatt_set_semiclassical(h_, on ? 1 : 0), "semiclassical"); }

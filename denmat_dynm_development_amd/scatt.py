@@ -148,6 +148,13 @@ class ScatteringOperator:
         call("dmd_scatt_load_sP", self._h, str(dirname).encode(), ctypes.byref(info), scatt.encode(), float(scale),
              float(thr))
 
+    def compress_P(self):
+        """Keep only the nonzero elements of P (dmd_scatt_compress_P); returns
+        the number of stored values (padding included)."""
+        n = ctypes.c_int64(0)
+        call("dmd_scatt_compress_P", self._h, ctypes.byref(n))
+        return n.value
+
     def threshold_P(self, thr):
         """alg_sparseP's sparsification (set_sparseP, ElectronPhonon.cpp:115-129):
         every stored element with |P| <= thr becomes 0."""

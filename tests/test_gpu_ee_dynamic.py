@@ -14,6 +14,10 @@ import pytest
 from helpers import FS, random_P, rel_err
 
 pytestmark = pytest.mark.gpu
+# the nb = 3 e-e generator is switched off (DMD_ERR_UNSUPPORTED): a device run of
+# it failed and took the device down; its cause is not found yet
+NB3_OFF = pytest.mark.skip(reason="nb = 3 e-e generator switched off after a device failure whose cause is open")
+NBS = [1, 2, pytest.param(3, marks=NB3_OFF), 4]
 
 
 def _mods():
@@ -63,7 +67,7 @@ def _tables(scatt, oracle, op, k1, k2, U, kv, e, f, m):
     return np.array([oracle.eimp_vq(d, oracle.eimp_q2(d["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)]), None
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4])
+@pytest.mark.parametrize("nb", NBS)
 def test_ee_P_at_a_state_matches_the_oracle(nb):
     scatt, oracle = _mods()
     rng, km, kv, e, f, U, k1, k2 = _case(nb, 80 + nb)
@@ -91,7 +95,7 @@ def test_ee_P_at_a_state_matches_the_oracle(nb):
                 assert np.abs(E1 - Q1).max() <= 1e-12 * np.abs(Q1).max()
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4])
+@pytest.mark.parametrize("nb", NBS)
 def test_ee_exchange_every_screening(nb):
     scatt, oracle = _mods()
     rng, km, kv, e, f, U, k1, k2 = _case(nb, 90 + nb)
@@ -114,7 +118,7 @@ def test_ee_exchange_every_screening(nb):
             assert s > 0 and np.abs(A1 - R1).max() <= 1e-12 * s and np.abs(A2 - R2).max() <= 1e-12 * s
 
 
-@pytest.mark.parametrize("nb", [1, 2, 3, 4])
+@pytest.mark.parametrize("nb", NBS)
 @pytest.mark.parametrize("exchange", [False, True])
 def test_linearized_ee_terms_match_the_oracle(nb, exchange):
     scatt, oracle = _mods()
