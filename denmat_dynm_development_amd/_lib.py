@@ -177,6 +177,7 @@ PROTOTYPES = {
     "dmd_scatt_set_split": [_vp, _vp],
     "dmd_scatt_set_Lsc": [_vp, _vp],
     "dmd_scatt_eval": [_vp, _d, _vp, _vp, _vp],
+    "dmd_scatt_eval_noalloc": [_vp, _d, _vp, _vp],
     "dmd_scatt_eval_batched": [_vp, _d, _i, _vp, _vp],
     "dmd_scatt_compute_ddm": [_vp, _vp, _vp, _vp],
     "dmd_scatt_set_window": [_vp, _i, _i, _vp],

@@ -322,6 +322,18 @@ class ScatteringOperator:
             out[...] = ko.reshape(out.shape)
         return out
 
+    def eval_noalloc(self, t, rho, out=None):
+        """dmd_scatt_eval_noalloc: eval with no device allocation; rho and out
+        are CUDA tensors (nk x nb^2 complex128)."""
+        if not _is_tensor(rho) or not rho.is_cuda:
+            raise TypeError("eval_noalloc takes device tensors")
+        if out is None:
+            out = torch.empty((self.nk, self.nb * self.nb), dtype=torch.complex128, device=rho.device)
+        pr, kr = _ptr(rho, np.complex128)
+        po, ko = _ptr(out, np.complex128)
+        call("dmd_scatt_eval_noalloc", self._h, float(t), pr, po)
+        return out
+
     def eval_batched(self, t, rho_B, out=None):
         B = rho_B.shape[0]
         if out is None:

@@ -231,6 +231,20 @@ int dmd_scatt_set_Lsc(dmd_scatt *h, const double *f_eq);
  * overwritten) from rho (nk x nb^2) at time t.  one_minus_rho may be NULL,
  * then 1-rho is formed as set_oneminusdm does (DenMat.cpp:249-261). */
 int dmd_scatt_eval(dmd_scatt *h, double t, const double *rho, const double *one_minus_rho, double *ddmdt);
+/* Allocation-free form of dmd_scatt_eval.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  rho and
+ * ddmdt must be device memory (nk x nb^2 complex, ddmdt fully overwritten,
+ * 1 - rho formed from rho); the rotated state, the row accumulators and the
+ * device clock live in one static device array of DMD_NOALLOC_SCRATCH_BYTES
+ * per GPU instead of the handle's buffers, and the call makes no hipMalloc
+ * or hipFree.  Operators whose scratch exceeds that size (about nk * nb^2 *
+ * 24 bytes plus 256; nk up to 65536 at nb = 2 and 32768 at nb = 4 fit) get
+ * DMD_ERR_ARG.  A reduced-precision handle needs one dmd_scatt_eval after its
+ * P or split occupations change (DMD_ERR_STATE otherwise).  Every call on one
+ * GPU shares the array: calls that overlap on one GPU race. */
+#define DMD_NOALLOC_SCRATCH_BYTES (16u << 20)
+int dmd_scatt_eval_noalloc(dmd_scatt *h, double t, const double *rho, double *ddmdt);
 /* evolve (evolve_eph.cpp:34-166) for B independent states: rho_B, ddmdt_B
  * are B x nk x nb^2 (one_minus_rho formed internally; no band window).  On
  * a DMD_PREC_C64 or DMD_PREC_BF16 nb = 4 handle the B states share one pass
