@@ -345,9 +345,12 @@ def mode_adjoint(args, torch, scatt):
         q.branch_mask = m
         params.append(q)
     K = args.steps
+    # one segment (every stage state kept, 4 x 1 MiB per step at nk = 16384)
+    # unless a checkpoint interval is given
+    C = args.checkpoint_every if args.checkpoint_every > 0 else K
     pr = SpinLifetimeProblem.from_device_generator(
         params, p.nk, p.nb, k1, k2, e, rho0=rho0, s_obs=s3[:, 2], f=f, i0=0, i1=p.nb, obs_prefac=1.0 / p.nk,
-        t0=0.0, dt=1.0 * FS, n_steps=K, report_every=1, checkpoint_every=64, nk_full=p.nk)
+        t0=0.0, dt=1.0 * FS, n_steps=K, report_every=1, checkpoint_every=C, nk_full=p.nk)
     theta = np.ones(len(masks))
     try:
         pr.tau(theta)  # warm (graph capture)
@@ -368,12 +371,22 @@ def mode_adjoint(args, torch, scatt):
         rel_norm = float(max(abs(grad[c] - fd[c]) for c in range(len(fd))) / gnorm)
     finally:
         pr.close()
+    fwd_ms, adj_ms = t_fwd / K * 1e3, t_adj / K * 1e3
+    # HBM streams of one adjoint step: the forward sweep's 4 evaluations and
+    # the reverse sweep's 4 one-pass VJPs read P (32 complex128 per pair), the
+    # parameter fold reads and writes Gc (same size) once per 4 steps; segments
+    # that are recomputed add 4 evaluations
+    p_bytes = len(k1) * 32 * 16
+    streams = 4 + 4 + 0.5 + (4 if C < K else 0)
     out = {"mode": "adjoint", "workload": f"preset 5: nb=2 nk={p.nk} stored_pairs={len(k1)}, 6 branches + e-i, "
-                                          f"RK4 dt=1fs {K} steps, checkpoints every 64",
+                                          f"RK4 dt=1fs {K} steps, checkpoints every {C}",
            "tau_s_ps": tau, "dtau_dtheta": list(map(float, grad)), "fd_central_h1e-6": fd,
            "adjoint_rel_err_vs_fd": rel, "adjoint_max_err_over_fd_norm": rel_norm,
-           "forward_s": t_fwd, "adjoint_s": t_adj,
-           "adjoint_ms_per_step": t_adj / K * 1e3}
+           "forward_s": t_fwd, "adjoint_s": t_adj, "forward_ms_per_step": fwd_ms,
+           "adjoint_ms_per_step": adj_ms, "adjoint_over_forward": adj_ms / fwd_ms,
+           "roofline": {"bound": "hbm", "achieved": streams * p_bytes / (adj_ms * 1e-3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": streams * p_bytes / (adj_ms * 1e-3) / 8e12,
+                        "traffic": None, "streams_per_step": streams, "p_bytes": p_bytes}}
     log(json.dumps(out))
     return out
 
@@ -549,6 +562,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--B", default="3,16,64")
     ap.add_argument("--fd-all", action="store_true", help="adjoint mode: check every parameter")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint mode: steps per checkpoint segment (0: one segment, every stage state kept)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()

@@ -31,10 +31,10 @@ class SpinLifetimeProblem:
         for P1, P2 in components:
             c = scatt.ScatteringOperator(nk, nb, k1, k2, e, nk_full=nk_full)
             c.upload_P(P1, P2)
-            c.set_flags(expt=expt, kernel=_lib.DMD_KERNEL_SINGLE_READ)
+            c.set_flags(expt=expt, kernel=_lib.DMD_KERNEL_AUTO)
             self.comps.append(c)
         self.op = scatt.ScatteringOperator(nk, nb, k1, k2, e, nk_full=nk_full)
-        self.op.set_flags(expt=expt, kernel=_lib.DMD_KERNEL_SINGLE_READ)
+        self.op.set_flags(expt=expt, kernel=_lib.DMD_KERNEL_AUTO)
         self.op.set_observable(s_obs, f, i0, i1, obs_prefac)
 
     @classmethod
@@ -44,7 +44,7 @@ class SpinLifetimeProblem:
         for q in params_by_component:
             c = scatt.ScatteringOperator(nk, nb, k1, k2, e, nk_full=kw.get("nk_full"))
             c.generate_P(q)
-            c.set_flags(expt=kw.get("expt", True), kernel=_lib.DMD_KERNEL_SINGLE_READ)
+            c.set_flags(expt=kw.get("expt", True), kernel=_lib.DMD_KERNEL_AUTO)
             self.comps.append(c)
         return self
 

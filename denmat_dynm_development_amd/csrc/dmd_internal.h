@@ -123,6 +123,39 @@ __device__ __forceinline__ double2 ccis(double x)
     return make_double2(c, s);
 }
 
+// nb = 2 tile helpers (shared by the forward and the adjoint tile kernels)
+__device__ __forceinline__ double2 one_minus(const double2 *r, int q)
+{
+    return make_double2((q % 3 == 0 ? 1.0 : 0.0) - r[q].x, -r[q].y);  // nb = 2: diagonal q = 0, 3
+}
+// The packed Hermitian part H = X + X^+ of a 2 x 2 matrix X (the linearized
+// operator's per-pair increment, evolve_eph_linear.cpp:47-53).
+__device__ __forceinline__ void herm_of(const double2 *c, double hv[4])
+{
+    hv[0] = 2.0 * c[0].x;
+    hv[1] = 2.0 * c[3].x;
+    hv[2] = c[1].x + c[2].x;
+    hv[3] = c[1].y - c[2].y;
+}
+
+// Rows [base_row, base_row + 64) of one wave: lane l holds the 4 doubles of
+// row base_row + l (zeros past nk); the wave regroups them so every atomic
+// instruction covers 16 consecutive rows' 32-byte records (4 full lines).
+__device__ __forceinline__ void flush_herm_wave(const double hv[4], int lane, int64_t base_row, int64_t nk,
+                                                double *__restrict__ hacc)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int src = 16 * q + (lane >> 2), d = lane & 3;
+        const double a0 = __shfl(hv[0], src), a1 = __shfl(hv[1], src), a2 = __shfl(hv[2], src),
+                     a3 = __shfl(hv[3], src);
+        const double v = d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
+        const int64_t row = base_row + src;
+        if (v != 0.0 && row < nk) unsafeAtomicAdd(hacc + row * 4 + d, v);
+    }
+}
+
+
 
 }  // namespace dmd
 
@@ -264,6 +297,9 @@ int set_clock(dmd_scatt *h, double t0, double dt, long long step);
 int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a, int semic = 0);
 // contraction + Hermitian epilogue on h->d_rhoS (and h->d_rho1S when dm1) into out (nk x nb^2)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
+// nb = 2: out = prefac/2 x the packed Hermitian rows of h->d_acc_herm, rotated to
+// the interaction picture at stage a (no ddmdt_eq); clears the accumulator
+int launch_assemble_herm(dmd_scatt *h, double a, double prefac, double2 *out);
 // DMD_KERNEL_ROWS, _SINGLE_READ or _TILES: the kernel launch_contract uses
 int contract_kind(const dmd_scatt *h);
 // nb = 4 tile path (dmd_tiles4.hip)

@@ -405,10 +405,6 @@ __global__ __launch_bounds__(256) void k_fwd_atomic(int64_t nk, const double2 *_
 // so the sum of the pairs' H is the row's H (the reference sums Pdm, dm1P
 // first and assembles once, evolve_eph.cpp:140-151; the initializer assembles
 // per pair, lindbladInit.../ElectronPhonon.h:855-862).
-__device__ __forceinline__ double2 one_minus(const double2 *r, int q)
-{
-    return make_double2((q % 3 == 0 ? 1.0 : 0.0) - r[q].x, -r[q].y);  // nb = 2: diagonal q = 0, 3
-}
 __device__ __forceinline__ void herm_inc(const double2 *pdm, const double2 *d1p, const double2 *r,
                                          const double2 *r1, double hv[4])
 {
@@ -435,33 +431,6 @@ __device__ __forceinline__ void herm_inc(const double2 *pdm, const double2 *d1p,
     hv[1] = 2.0 * c[3].x;     // H11
     hv[2] = c[1].x + c[2].x;  // H01 = C01 + conj(C10)
     hv[3] = c[1].y - c[2].y;
-}
-
-// The packed Hermitian part H = X + X^+ of a 2 x 2 matrix X (the linearized
-// operator's per-pair increment, evolve_eph_linear.cpp:47-53).
-__device__ __forceinline__ void herm_of(const double2 *c, double hv[4])
-{
-    hv[0] = 2.0 * c[0].x;
-    hv[1] = 2.0 * c[3].x;
-    hv[2] = c[1].x + c[2].x;
-    hv[3] = c[1].y - c[2].y;
-}
-
-// Rows [base_row, base_row + 64) of one wave: lane l holds the 4 doubles of
-// row base_row + l (zeros past nk); the wave regroups them so every atomic
-// instruction covers 16 consecutive rows' 32-byte records (4 full lines).
-__device__ __forceinline__ void flush_herm_wave(const double hv[4], int lane, int64_t base_row, int64_t nk,
-                                                double *__restrict__ hacc)
-{
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int src = 16 * q + (lane >> 2), d = lane & 3;
-        const double a0 = __shfl(hv[0], src), a1 = __shfl(hv[1], src), a2 = __shfl(hv[2], src),
-                     a3 = __shfl(hv[3], src);
-        const double v = d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
-        const int64_t row = base_row + src;
-        if (v != 0.0 && row < nk) unsafeAtomicAdd(hacc + row * 4 + d, v);
-    }
 }
 
 // Epilogue from the Hermitian accumulator (nb = 2): ddm = prefac/2 H,
@@ -1608,6 +1577,15 @@ int contract_kind(const dmd_scatt *h)
 }
 
 // the contraction + epilogue on the current rhoS (and rho1S when dm1)
+int launch_assemble_herm(dmd_scatt *h, double a, double prefac, double2 *out)
+{
+    if (h->nb != 2) return fail(DMD_ERR_STATE, "the packed Hermitian rows are an nb = 2 form");
+    hipLaunchKernelGGL(k_assemble_h, dim3(nblk(h->nk, 256)), dim3(256), 0, h->st, h->nk, h->d_acc_herm, h->d_e,
+                       h->d_clk, a, h->expt, 0, prefac, nullptr, out);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const int kind = contract_kind(h);

@@ -322,8 +322,10 @@ int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int64_t n_step
 /* Vector-Jacobian product of evolve (evolve_eph.cpp:34-166) at the state rho
  * (Hermitian, nk x nb^2) and time t: g = (d ddmdt / d rho)^T v under the real
  * pairing Re sum conj(a) b, projected on Hermitian matrices.  FP64, nb <= 2,
- * scattering term only; the handle switches to the single-read kernel.  No
- * reference counterpart (the reference has no derivatives). */
+ * scattering term only.  nb = 2 runs one pass over P on the tile layout (the
+ * handle switches to the tile kernel), nb = 1 two passes of the single-read
+ * kernel (the handle switches to it).  No reference counterpart (the
+ * reference has no derivatives). */
 int dmd_scatt_vjp(dmd_scatt *h, double t, const double *rho, const double *v, double *g);
 /* P = sum_c weights[c] P_c over component handles holding FP64 P of the same
  * pairs (e.g. per-branch and impurity parts; scale_eph generalised,
@@ -333,8 +335,15 @@ int dmd_scatt_combine_P(dmd_scatt *h, int ncomp, dmd_scatt *const *comps, const 
  * alone, FP64, nb <= 2): given dL/dS at the n_steps/report_every + 1 reports
  * of the observable, returns dL/dtheta_c = sum over stages of Re <Kbar,
  * f_c(Y)> for the component handles (the derivative with respect to the
- * weight of P_c) and dL/drho0.  States are recomputed from checkpoints every
- * checkpoint_every steps.  No reference counterpart. */
+ * weight of P_c) and dL/drho0.  The run is cut into segments of
+ * checkpoint_every steps: every segment's first state is kept, and the stage
+ * states of a segment are recomputed from it for the reverse sweep (none are
+ * recomputed for checkpoint_every >= n_steps; the stage states then take
+ * 4 n_steps nk nb^2 x 16 bytes of device memory).  nb = 2 on tiles folds the
+ * parameter cotangents into one per-pair sum dotted with every P_c at the
+ * end (components must hold P of the handle's pairs, FP64, same phase flag);
+ * other layouts evaluate every component at every stage.  No reference
+ * counterpart. */
 int dmd_scatt_adjoint_rk4(dmd_scatt *h, double t0, double dt, int64_t n_steps, int report_every,
                           int checkpoint_every, const double *rho0, const double *dL_dobs, int ncomp,
                           dmd_scatt *const *comps, double *dL_dtheta, double *dL_drho0);

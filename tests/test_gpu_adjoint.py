@@ -97,6 +97,43 @@ def test_adjoint_rk4_matches_forward_central_differences():
         pr.close()
 
 
+@pytest.mark.parametrize("nb", [1, 2])
+def test_adjoint_segment_lengths_agree(nb):
+    """One segment (every stage state kept), segments of 8 and a ragged last
+    segment (7 into 40) give the same gradients: the recomputed stage states
+    are the forward sweep's."""
+    scatt, oracle = _mods()
+    print(f"case adjoint segments nb={nb}", flush=True)
+    rng = np.random.default_rng(29 + nb)
+    nk, ncomp = 30, 2
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    comps = []
+    for _ in range(ncomp):
+        P1, P2 = random_P(len(k1), nb, rng)
+        comps.append((0.15 * P1, 0.15 * P2))
+    theta = np.array([0.8, 1.2])
+    f = np.sort(rng.uniform(0.1, 0.5, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho0 = np.zeros((nk, nb, nb), np.complex128)
+    rho0[:, np.arange(nb), np.arange(nb)] = f
+    rho0 = rho0.reshape(nk, nb * nb) + random_rho(nk, nb, rng, scale=0.02, diag=0.0)
+    s = random_rho(nk, nb, rng, scale=0.5, diag=0.0)
+    dt, n, rep = 2 * FS, 40, 5
+    w = rng.normal(size=n // rep + 1)
+    from denmat_dynm_development_amd.adjoint import SpinLifetimeProblem
+    got = []
+    for cpe in (1000, 8, 7):
+        pr = SpinLifetimeProblem(nk, nb, k1, k2, e, comps, rho0, s, f, 0, nb, 1.0, 0.0, dt, n, rep,
+                                 checkpoint_every=cpe)
+        try:
+            got.append(pr.grad_obs_functional(theta, w))
+        finally:
+            pr.close()
+    for dth, drho in got[1:]:
+        assert np.abs(dth - got[0][0]).max() <= 1e-12 * np.abs(got[0][0]).max()
+        assert np.abs(drho - got[0][1]).max() <= 1e-12 * np.abs(got[0][1]).max()
+
+
 def test_tau_gradcheck_nk256():
     """torch.autograd.gradcheck of tau_s(theta) over the six branch scales and
     the impurity density of a synthetic nk = 256 problem (FP64 forward)."""
