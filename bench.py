@@ -391,6 +391,60 @@ def mode_adjoint(args, torch, scatt):
     return out
 
 
+def mode_sparse(args, torch, scatt):
+    """The sparse modes' store (dmd_scatt_compress_P: per pair a nonzero mask,
+    an offset and the nonzero values) on the plain nb = 2 workload: P kept at
+    about 100, 50 and 25 % of its elements (alg_sparseP's threshold, chosen
+    from the magnitudes of a sample of pairs), bytes per pair and contraction
+    time of the compressed store against the dense one, and the compressed
+    evaluation against the dense store holding the same thresholded P."""
+    p = scatt.synth_params(args.preset)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    rho0, f = initial_state(p, e, s3)
+    n_pairs = len(k1)
+    rho_d = torch.from_numpy(rho0).cuda()
+    op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+    op.generate_P(p)
+    S1, S2 = op.download_P(0, min(n_pairs, 20000))
+    mags = np.abs(np.concatenate([S1.ravel(), S2.ravel()]))
+    op.eval(0.0, rho_d)
+    dense_ms, _ = op.time_eval(17 * FS, 10)
+    dense_bytes = op.info()["p_bytes"]
+    op.close()
+    rows = []
+    for keep in (1.0, 0.5, 0.25):
+        thr = 0.0 if keep == 1.0 else float(np.quantile(mags, 1.0 - keep))
+        ref = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+        ref.generate_P(p)
+        if thr > 0:
+            ref.threshold_P(thr)
+        want = ref.eval(17 * FS, rho_d).cpu().numpy()
+        ref.close()
+        op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk)
+        op.generate_P(p)
+        if thr > 0:
+            op.threshold_P(thr)
+        stored = op.compress_P()
+        got = op.eval(17 * FS, rho_d).cpu().numpy()
+        kernel_ms, _ = op.time_eval(17 * FS, 10)
+        nbytes = op.info()["p_bytes"]
+        op.close()
+        err = float(np.abs(got - want).max() / max(np.abs(want).max(), 1e-300))
+        r = {"kept_elements": stored / (n_pairs * 32), "threshold": thr, "compressed": nbytes != dense_bytes,
+             "p_bytes": nbytes,
+             "bytes_per_pair": nbytes / n_pairs, "kernel_ms": kernel_ms,
+             "GBps": nbytes / (kernel_ms * 1e-3) / 1e9, "vs_dense_time": kernel_ms / dense_ms,
+             "vs_dense_bytes": nbytes / dense_bytes, "max_rel_err_vs_dense_store": err}
+        log(f"sparse keep~{keep}: {r}")
+        rows.append(r)
+    return {"mode": "sparse", "workload": f"preset {args.preset}: nb={p.nb} nk={p.nk} stored_pairs={n_pairs}",
+            "dense": {"p_bytes": dense_bytes, "bytes_per_pair": dense_bytes / n_pairs, "kernel_ms": dense_ms,
+                      "GBps": dense_bytes / (dense_ms * 1e-3) / 1e9},
+            "compressed": rows}
+
+
 def mode_linear(args, torch, scatt):
     """The linearized operator (alg_linearize): set_Lsc at the Fermi occupations,
     then evolve_linear timed per evaluation and per RK4 step, FP64, at the
@@ -555,7 +609,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp"])
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp", "sparse"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
@@ -582,6 +636,9 @@ def main():
         return
     if args.mode == "adjoint":
         print(json.dumps(mode_adjoint(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "sparse":
+        print(json.dumps(mode_sparse(args, torch, scatt)), flush=True)
         return
     if args.mode == "linear":
         print(json.dumps(mode_linear(args, torch, scatt)), flush=True)

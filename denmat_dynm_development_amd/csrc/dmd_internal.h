@@ -228,9 +228,9 @@ struct dmd_scatt {
     double2 *d_adj = nullptr;            // 3 x nk x nb^2
     double *d_dotpart = nullptr;         // dot-product partials
     // compressed P (dmd_scatt_compress_P, nb = 2 tiles): per-slot nonzero
-    // masks and value offsets, and the nonzero values; d_P is released while
-    // they are set
-    uint32_t *d_spmask = nullptr, *d_spofs = nullptr;
+    // masks, per-granule value offsets and the nonzero values; d_P is
+    // released while they are set
+    uint32_t *d_spmask = nullptr, *d_spgofs = nullptr;
     double2 *d_spv = nullptr;
     int64_t sp_values = 0;  // stored values
     // linearized operator (alg_linearize): the P store holds Lscij | Lscji

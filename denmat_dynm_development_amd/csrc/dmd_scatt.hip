@@ -472,11 +472,11 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
 // drho_S; forward X = Lscij drho_k', reverse Y = Lscji drho_k, increments
 // X + X^+ and Y + Y^+ (evolve_eph_linear.cpp:26-37).
 // SP: P held compressed (dmd_scatt_compress_P): per slot a mask of its
-// nonzero elements (bit el < 16: P1 element el, bit 16 + el: P2 element el) and
-// the offset of its nonzero values, stored one after another in element order
-// (value j of slot s at P[gofs[s] + j]).  The lane rebuilds the slot's 32
-// elements in registers (zeros where the mask is clear) and runs the dense
-// arithmetic.
+// nonzero elements (bit el < 16: P1 element el, bit 16 + el: P2 element el),
+// per granule of 8 slots the offset of its values, stored rank-major (the
+// first nonzero P1 element of each of the 8 slots that has one, then the
+// second, ..., then P2 alike), so the lanes of one rank read consecutive
+// values.  Each nonzero adds its two GEMV terms directly.
 template <bool DM1, bool LIN, bool SP = false>
 __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint32_t *__restrict__ pk,
@@ -515,6 +515,13 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
         const int mq = (m + W - 1) / W;
         const int i0 = wv * mq, i1 = min(m, i0 + mq);
         uint32_t code_next = (i0 < i1 && lane * m + i0 < n) ? ld_stream(pk + b + (int64_t)i0 * 64 + lane) : 0u;
+        // SP: the slot's mask and its granule's value offset, one iteration ahead
+        uint32_t mask_next = 0u, gofs_next = 0u;
+        if (SP && i0 < i1) {
+            const int64_t s0 = b + (int64_t)i0 * 64 + lane;
+            mask_next = ld_stream(pmask + s0);
+            gofs_next = ld_stream(gofs + (s0 >> 3));
+        }
         // rho of the lane's current row k: a lane walks a run of one k, so it is
         // reloaded only where k changes
         int lo1_cur = -1;
@@ -523,10 +530,22 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
         for (int q = 0; q < NB2; q++) { rk[q] = make_double2(0, 0); r1k[q] = make_double2(0, 0); }
         for (int it = i0; it < i1; it++) {
             const bool ok = lane * m + it < n;
-            // idle lanes read the chunk's first slot (always valid) and add nothing
-            const int64_t s = b + (ok ? (int64_t)it * 64 + lane : 0);
+            // idle lanes read the chunk's first slot (always valid) and add
+            // nothing; with SP every lane keeps its own slot (a padding slot,
+            // mask 0, for idle lanes), so the 8 lanes of a granule are its 8 slots
+            const int64_t s = b + ((ok || SP) ? (int64_t)it * 64 + lane : 0);
             const uint32_t code = code_next;
             code_next = (it + 1 < i1 && lane * m + it + 1 < n) ? ld_stream(pk + b + (int64_t)(it + 1) * 64 + lane) : 0u;
+            uint32_t mask = 0u, vofs = 0u;
+            if (SP) {
+                mask = mask_next;
+                vofs = gofs_next;
+                if (it + 1 < i1) {
+                    const int64_t s1 = b + (int64_t)(it + 1) * 64 + lane;
+                    mask_next = ld_stream(pmask + s1);
+                    gofs_next = ld_stream(gofs + (s1 >> 3));
+                }
+            }
             const int lo1 = code % R, lo2 = code / R;
             const int64_t k = I0 + lo1, kp = J0 + lo2;
             const double2 *g = P + (s >> 3) * (8 * EP) + (s & 7);
@@ -551,93 +570,167 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             auto om_k = [&](int q) { return DM1 ? r1k[q] : one_minus(rk, q); };
             auto om_kp = [&](int q) { return DM1 ? r1kp[q] : one_minus(rkp, q); };
             const bool rev = ok && k < kp;
-            double2 p1[NB4], p2[NB4];
+            // forward Pdm_k = P1 rho_k', dm1P_k = (1-rho_k')^T P2 (f1, f2);
+            // reverse Pdm_k' = conj(P2) rho_k, dm1P_k' = (1-rho_k)^T conj(P1) (g1, g2)
+            double2 f1[NB2], f2[NB2], g1[NB2], g2[NB2];
             if (SP) {
-                const uint32_t mask = ld_stream(pmask + s);
-                const double2 *gv = P + ld_stream(gofs + s);
+#pragma unroll
+                for (int i = 0; i < NB2; i++) {
+                    f1[i] = make_double2(0, 0); f2[i] = make_double2(0, 0);
+                    g1[i] = make_double2(0, 0); g2[i] = make_double2(0, 0);
+                }
+                // rank-major values of the granule: for rank j = 0, 1, ... the
+                // j-th nonzero P1 element of every slot that has one, in slot
+                // order, then the same for P2.  A ballot per rank gives each
+                // lane its value's place, so all of a lane's loads are issued
+                // before any is used; its j-th value is the element at the
+                // j-th set bit of its mask and adds to one entry of two GEMVs.
+                const double2 *gv = P + vofs;
+                const int gsh = lane & 56;
+                const uint32_t below = (1u << (lane & 7)) - 1u;
                 int pos = 0;
 #pragma unroll
-                for (int el = 0; el < NB4; el++) {
-                    const bool on = (mask >> el) & 1u;
-                    p1[el] = on ? ld_stream(gv + pos) : make_double2(0.0, 0.0);
-                    pos += on;
-                }
+                for (int half = 0; half < 2; half++) {
+                    const uint32_t mh = half ? (mask >> 16) : (mask & 0xffffu);
+                    const int cnt = __popc(mh);
+                    double2 v[NB4];
+                    uint32_t live = 0;  // ranks some lane of the wavefront holds
 #pragma unroll
-                for (int el = 0; el < NB4; el++) {
-                    const bool on = (mask >> (NB4 + el)) & 1u;
-                    p2[el] = on ? ld_stream(gv + pos) : make_double2(0.0, 0.0);
-                    pos += on;
+                    for (int j = 0; j < NB4; j++) {
+                        const uint64_t bal = __ballot(cnt > j);
+                        const uint32_t gb = (uint32_t)(bal >> gsh) & 0xffu;
+                        v[j] = make_double2(0, 0);
+                        if (bal != 0) {
+                            live |= 1u << j;
+                            if (cnt > j) v[j] = ld_stream(gv + pos + __popc(gb & below));
+                        }
+                        pos += __popc(gb);
+                    }
+                    uint32_t rem = mh;
+#pragma unroll
+                    for (int j = 0; j < NB4; j++) {
+                        if (!((live >> j) & 1u)) continue;  // wavefront-uniform
+                        if (j >= cnt) continue;
+                        const int el = __builtin_ctz(rem);
+                        rem &= rem - 1u;
+                        const int ra = el >> 2, cb = el & 3;  // P[ra, cb]
+                        double wa[NB2], wb[NB2];
+#pragma unroll
+                        for (int t = 0; t < NB2; t++) {
+                            wa[t] = ra == t ? 1.0 : 0.0;
+                            wb[t] = cb == t ? 1.0 : 0.0;
+                        }
+                        double2 x1 = make_double2(0, 0), x2 = make_double2(0, 0);
+                        if (!half) {  // P1[ra, cb]: f1[ra] += P1 rho_k'[cb], g2[cb] += (1-rho_k)[ra] conj(P1)
+#pragma unroll
+                            for (int t = 0; t < NB2; t++) {
+                                const double2 a = rkp[t], c = om_k(t);
+                                x1 = make_double2(fma(wb[t], a.x, x1.x), fma(wb[t], a.y, x1.y));
+                                x2 = make_double2(fma(wa[t], c.x, x2.x), fma(wa[t], c.y, x2.y));
+                            }
+                            const double2 pf = cmul(v[j], x1), pr = cmulc(x2, v[j]);
+#pragma unroll
+                            for (int t = 0; t < NB2; t++) {
+                                f1[t] = make_double2(fma(wa[t], pf.x, f1[t].x), fma(wa[t], pf.y, f1[t].y));
+                                g2[t] = make_double2(fma(wb[t], pr.x, g2[t].x), fma(wb[t], pr.y, g2[t].y));
+                            }
+                        } else {  // P2[ra, cb]: f2[cb] += (1-rho_k')[ra] P2, g1[ra] += rho_k[cb] conj(P2)
+#pragma unroll
+                            for (int t = 0; t < NB2; t++) {
+                                const double2 a = om_kp(t), c = rk[t];
+                                x1 = make_double2(fma(wa[t], a.x, x1.x), fma(wa[t], a.y, x1.y));
+                                x2 = make_double2(fma(wb[t], c.x, x2.x), fma(wb[t], c.y, x2.y));
+                            }
+                            const double2 pf = cmul(x1, v[j]), pr = cmulc(x2, v[j]);
+#pragma unroll
+                            for (int t = 0; t < NB2; t++) {
+                                f2[t] = make_double2(fma(wb[t], pf.x, f2[t].x), fma(wb[t], pf.y, f2[t].y));
+                                g1[t] = make_double2(fma(wa[t], pr.x, g1[t].x), fma(wa[t], pr.y, g1[t].y));
+                            }
+                        }
+                    }
                 }
             } else {
+                double2 p1[NB4], p2[NB4];
 #pragma unroll
                 for (int el = 0; el < NB4; el++) p1[el] = ld_stream(g + el * 8);
 #pragma unroll
                 for (int el = 0; el < NB4; el++) p2[el] = ld_stream(g + (NB4 + el) * 8);
-            }
-            if (LIN) {
-                double2 x[NB2], y[NB2];
+                if (LIN) {
+                    double2 x[NB2], y[NB2];
 #pragma unroll
-                for (int i = 0; i < NB2; i++) {
-                    x[i] = make_double2(0, 0);
-                    y[i] = make_double2(0, 0);
+                    for (int i = 0; i < NB2; i++) {
+                        x[i] = make_double2(0, 0);
+                        y[i] = make_double2(0, 0);
 #pragma unroll
-                    for (int q = 0; q < NB2; q++) {
-                        cfma(x[i], p1[i * NB2 + q], rkp[q]);  // X_k[i]  += Lscij[i,q] drho_k'[q]
-                        cfma(y[i], p2[i * NB2 + q], rk[q]);   // Y_k'[i] += Lscji[i,q] drho_k[q]
+                        for (int q = 0; q < NB2; q++) {
+                            cfma(x[i], p1[i * NB2 + q], rkp[q]);  // X_k[i]  += Lscij[i,q] drho_k'[q]
+                            cfma(y[i], p2[i * NB2 + q], rk[q]);   // Y_k'[i] += Lscji[i,q] drho_k[q]
+                        }
+                    }
+                    double hf[4], hr[4];
+                    herm_of(x, hf);
+                    herm_of(y, hr);
+                    if (ok) {
+#pragma unroll
+                        for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
+                    }
+                    if (rev) {
+#pragma unroll
+                        for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
+                    }
+                    continue;
+                }
+                // forward (row k)
+                {
+#pragma unroll
+                    for (int i = 0; i < NB2; i++) {
+                        f1[i] = make_double2(0, 0);
+                        f2[i] = make_double2(0, 0);
+#pragma unroll
+                        for (int q = 0; q < NB2; q++) {
+                            cfma(f1[i], p1[i * NB2 + q], rkp[q]);      // Pdm_k[i]  += P1[i,q] rho_k'[q]
+                            cfma(f2[i], om_kp(q), p2[q * NB2 + i]);    // dm1P_k[i] += (1-rho_k')[q] P2[q,i]
+                        }
+                    }
+                    double hf[4];
+                    herm_inc(f1, f2, rk, DM1 ? r1k : nullptr, hf);
+                    if (ok) {
+#pragma unroll
+                        for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
                     }
                 }
-                double hf[4], hr[4];
-                herm_of(x, hf);
-                herm_of(y, hr);
-                if (ok) {
+                // reverse (row k')
+                {
 #pragma unroll
-                    for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
-                }
-                if (rev) {
+                    for (int i = 0; i < NB2; i++) {
+                        g1[i] = make_double2(0, 0);
+                        g2[i] = make_double2(0, 0);
 #pragma unroll
-                    for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
+                        for (int q = 0; q < NB2; q++) {
+                            cfmac(g1[i], rk[q], p2[i * NB2 + q]);      // Pdm_k'[i]  += rho_k[q] conj(P2[i,q])
+                            cfmac(g2[i], om_k(q), p1[q * NB2 + i]);    // dm1P_k'[i] += (1-rho_k)[q] conj(P1[q,i])
+                        }
+                    }
+                    double hr[4];
+                    herm_inc(g1, g2, rkp, DM1 ? r1kp : nullptr, hr);
+                    if (rev) {
+#pragma unroll
+                        for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
+                    }
                 }
                 continue;
             }
-            // forward (row k): Pdm_k = P1 rho_k', dm1P_k = (1-rho_k') P2
-            {
-                double2 f1[NB2], f2[NB2];
+            double hf[4], hr[4];
+            herm_inc(f1, f2, rk, DM1 ? r1k : nullptr, hf);
+            herm_inc(g1, g2, rkp, DM1 ? r1kp : nullptr, hr);
+            if (ok) {
 #pragma unroll
-                for (int i = 0; i < NB2; i++) {
-                    f1[i] = make_double2(0, 0);
-                    f2[i] = make_double2(0, 0);
-#pragma unroll
-                    for (int q = 0; q < NB2; q++) {
-                        cfma(f1[i], p1[i * NB2 + q], rkp[q]);      // Pdm_k[i]  += P1[i,q] rho_k'[q]
-                        cfma(f2[i], om_kp(q), p2[q * NB2 + i]);    // dm1P_k[i] += (1-rho_k')[q] P2[q,i]
-                    }
-                }
-                double hf[4];
-                herm_inc(f1, f2, rk, DM1 ? r1k : nullptr, hf);
-                if (ok) {
-#pragma unroll
-                    for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
-                }
+                for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
             }
-            // reverse (row k'): Pdm_k' = conj(P2) rho_k, dm1P_k' = (1-rho_k) conj(P1)
-            {
-                double2 g1[NB2], g2[NB2];
+            if (rev) {
 #pragma unroll
-                for (int i = 0; i < NB2; i++) {
-                    g1[i] = make_double2(0, 0);
-                    g2[i] = make_double2(0, 0);
-#pragma unroll
-                    for (int q = 0; q < NB2; q++) {
-                        cfmac(g1[i], rk[q], p2[i * NB2 + q]);      // Pdm_k'[i]  += rho_k[q] conj(P2[i,q])
-                        cfmac(g2[i], om_k(q), p1[q * NB2 + i]);    // dm1P_k'[i] += (1-rho_k)[q] conj(P1[q,i])
-                    }
-                }
-                double hr[4];
-                herm_inc(g1, g2, rkp, DM1 ? r1kp : nullptr, hr);
-                if (rev) {
-#pragma unroll
-                    for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
-                }
+                for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
             }
         }
         __syncthreads();  // block J done
@@ -815,7 +908,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_vrec); dfree(d_vq); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
     dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB); dfree(d_adj); dfree(d_dotpart);
     dfree(d_dpos); dfree(d_dk); dfree(d_flin); dfree(d_ad); dfree(d_adpart); dfree(d_phfeq);
-    dfree(d_spmask); dfree(d_spofs); dfree(d_spv);
+    dfree(d_spmask); dfree(d_spgofs); dfree(d_spv);
     for (void *p : extra) dfree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1549,11 +1642,11 @@ static void launch_tiles(dmd_scatt *h, bool dm1, double a, double2 *out)
         else if (h->d_spv && dm1)
             hipLaunchKernelGGL((k_tiles_soa<true, false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk,
                                h->d_spv, h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S,
-                               h->d_acc_herm, h->d_spmask, h->d_spofs);
+                               h->d_acc_herm, h->d_spmask, h->d_spgofs);
         else if (h->d_spv)
             hipLaunchKernelGGL((k_tiles_soa<false, false, true>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk,
                                h->d_spv, h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S,
-                               h->d_acc_herm, h->d_spmask, h->d_spofs);
+                               h->d_acc_herm, h->d_spmask, h->d_spgofs);
         else if (dm1)
             hipLaunchKernelGGL((k_tiles_soa<true, false>), dim3((unsigned)h->ngroups), dim3(256), 0, h->st, h->nk, h->d_P,
                                h->d_pk, h->d_cbeg, h->d_clen, h->d_cij, h->d_gbeg, h->d_rhoS, h->d_rho1S, h->d_acc_herm);
@@ -1701,7 +1794,7 @@ extern "C" int dmd_scatt_info(dmd_scatt *h, int64_t *n_pairs, int64_t *p_bytes, 
     if (n_pairs) *n_pairs = h->np;
     if (p_bytes) *p_bytes = h->np * 2 * h->nb4 * (int64_t)h->p_esize + (h->d_pscale ? h->np * 4 : 0);
     if (p_bytes && h->d_spv)  // compressed: values, per-slot masks and offsets
-        *p_bytes = h->sp_values * 16 + h->nslots * 8;
+        *p_bytes = h->sp_values * 16 + h->nslots * 4 + (h->nslots / 8) * 4;
     if (last_kernel_ms) *last_kernel_ms = h->last_ms;
     return DMD_OK;
 }

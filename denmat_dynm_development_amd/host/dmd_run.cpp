@@ -135,7 +135,8 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     s.semiclassical = p.get("alg_semiclassical", 0) != 0;  // parameters.cpp:101
     s.linearize = p.get("alg_linearize", 0) != 0;
     refuse(p, "alg_linearize_dPee", 0, "the linearized change of the e-e P inside evolve (linearize_dPee)");
-    // parameters.cpp:118-120; sparse P is kept dense in HBM with the small elements zeroed
+    // parameters.cpp:118-120; sparse P: the small elements are zeroed, and (nb = 2,
+    // FP64) P is then held by its nonzero elements
     s.Pin_is_sparse = p.get("alg_Pin_is_sparse", 0) != 0;
     s.sparseP = p.get("alg_sparseP", 0) != 0;
     s.thr_sparseP = p.get("alg_thr_sparseP", 1e-40);

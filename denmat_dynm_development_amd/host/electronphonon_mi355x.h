@@ -201,13 +201,19 @@ public:
     }
 
     // The sparse modes' per-pair element lists (evolve_eph.cpp:69-86): P kept by
-    // its nonzero elements so the bytes streamed fall with the density (nb = 2
-    // FP64 tile layout; false where the store stays dense)
+    // its nonzero elements so the bytes held and streamed fall with the density
+    // (nb = 2 FP64 tile layout; false where the store stays dense)
     bool compress_P()
     {
         if (nb != 2) return false;
-        check(dmd_scatt_compress_P(h_, nullptr), "compressed P");
-        return true;
+        int64_t np = 0, before = 0, after = 0;
+        double ms = 0;
+        check(dmd_scatt_info(h_, &np, &before, &ms), "info");
+        const int rc = dmd_scatt_compress_P(h_, nullptr);
+        if (rc == DMD_ERR_UNSUPPORTED) return false;  // another layout or precision: the store stays dense
+        check(rc, "compressed P");
+        check(dmd_scatt_info(h_, &np, &after, &ms), "info");
+        return after != before;
     }
 
     // alg.semiclassical (parameters.cpp:101): the driver-side right-hand side

@@ -500,13 +500,15 @@ int dmd_scatt_load_sP(dmd_scatt *h, const char *dir, const dmd_ldbd_info *info, 
                       double thr);
 /* Stores P by its nonzero elements (the reference's sparse modes alg_sparseP /
  * alg_Pin_is_sparse evolve over per-pair element lists, evolve_eph.cpp:69-86,
- * 194-203, comm/sparse_matrix.cpp:6-40): per pair a 32-bit nonzero mask, a
- * 32-bit offset and its nonzero values, 16 n + 8 bytes for n nonzeros, so the
- * bytes streamed per evaluation fall with the density; the dense P is released.
- * nb = 2 FP64 on the tile kernel.  Evaluation and the integrators run as
- * before; operations that change or read back P (upload, add, download,
- * threshold, masks, set_Lsc, the adjoint, kernel changes) return
- * DMD_ERR_STATE afterwards.  stored_values (optional): the nonzero values held. */
+ * 194-203, comm/sparse_matrix.cpp:6-40): per pair a 32-bit nonzero mask and
+ * its nonzero values, per 8 pairs a 32-bit offset, 16 n + 4.5 bytes for n
+ * nonzeros, so the bytes held and streamed per evaluation fall with the
+ * density; the dense P is released.  nb = 2 FP64 on the tile kernel.  With
+ * more than 3/4 of the elements nonzero the dense store is kept (DMD_OK).
+ * Evaluation and the integrators run as before; operations that change or
+ * read back P (upload, add, download, threshold, masks, set_Lsc, the
+ * adjoint, kernel changes) return DMD_ERR_STATE afterwards.  stored_values
+ * (optional): the nonzero values. */
 int dmd_scatt_compress_P(dmd_scatt *h, int64_t *stored_values);
 /* The sparsification of alg_sparseP and of the e-i fold-in under
  * alg_Pin_is_sparse (set_sparseP, ElectronPhonon.cpp:115-129;
