@@ -445,6 +445,37 @@ def mode_sparse(args, torch, scatt):
             "compressed": rows}
 
 
+def mode_panels(args, torch, scatt):
+    """The nb = 4 panel kernel alone (one trajectory, dmd_scatt_time_eval: HIP
+    events over 10 launches) in FP64, complex64 and BF16 on one preset, with
+    the algorithmic bytes of each store (the counters of a profiled run of
+    this mode back the per-precision bounds)."""
+    p = scatt.synth_params(args.preset)
+    e = scatt.synth_energies(p)
+    k1, k2 = scatt.synth_kpairs(p)
+    s3 = scatt.synth_spin(p)
+    rho0, f = initial_state(p, e, s3)
+    n_pairs = len(k1)
+    out = {"mode": "panels", "workload": f"preset {args.preset}: nb={p.nb} nk={p.nk} stored_pairs={n_pairs}",
+           "results": {}}
+    esize = {0: 16, 1: 8, 2: 4}
+    for prec in (0, 1, 2):
+        op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk, precision=prec)
+        op.generate_P(p)
+        if prec:
+            op.set_split(f)
+        rho_d = torch.from_numpy(rho0).cuda()
+        op.eval(17 * FS, rho_d)
+        kms, ems = op.time_eval(17 * FS, 10)
+        ab = alg_bytes(n_pairs, p.nk, p.nb, esize=esize[prec], extra=4 if prec == 2 else 0)
+        r = {"kernel_ms": kms, "eval_ms": ems, "alg_bytes": ab, "GBps": ab / kms / 1e6,
+             "roofline_frac": ab / kms / 1e6 / HBM_PEAK_GBPS}
+        out["results"][PREC_NAME[prec]] = r
+        log(f"{PREC_NAME[prec]}: {r}")
+        op.close()
+    return out
+
+
 def mode_linear(args, torch, scatt):
     """The linearized operator (alg_linearize): set_Lsc at the Fermi occupations,
     then evolve_linear timed per evaluation and per RK4 step, FP64, at the
@@ -609,7 +640,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
-    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp", "sparse"])
+    ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp", "sparse",
+                                                        "panels"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
@@ -639,6 +671,9 @@ def main():
         return
     if args.mode == "sparse":
         print(json.dumps(mode_sparse(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "panels":
+        print(json.dumps(mode_panels(args, torch, scatt)), flush=True)
         return
     if args.mode == "linear":
         print(json.dumps(mode_linear(args, torch, scatt)), flush=True)

@@ -214,6 +214,141 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
     }
 }
 
+// The same pass without visit records, for operators whose records do not
+// fit in device memory: the folded M of each visit is formed from the pair's
+// stored P (the blocked record of the tile store: BF16 with its scale, or
+// complex64) staged through LDS, so a visit streams the stored P (2 KiB per
+// visit for BF16, 4 KiB for complex64) and no 4 KiB-per-visit record array is
+// kept.  The forward visit reads rows of P1 and columns of P2, the reverse one
+// rows of P2 and columns of P1 (conjugated); LDS serves both.  M (scale,
+// sums, hi/lo split) is formed as k_build_visits forms it, so every visit's
+// products equal the record pass's; only the order of the row-sum atomics
+// differs.  vsd: slot << 1 | reverse.
+template <bool C64, int NBLK>
+__global__ __launch_bounds__(256) void k_batched_p(int64_t nseg, int Bp, const int4 *__restrict__ seg,
+                                                   const void *__restrict__ Pq, const float *__restrict__ pscale,
+                                                   const uint32_t *__restrict__ vsd, const int32_t *__restrict__ vq,
+                                                   const uint16_t *__restrict__ xb, float *__restrict__ F)
+{
+    constexpr int NCH = C64 ? 4 : 2;  // 16-byte chunks of a record per lane
+    __shared__ uint4 stage[4][64 * NCH];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t sg = (int64_t)blockIdx.x * 4 + wv;
+    if (sg >= nseg) return;
+    const int4 S = seg[sg];
+    const int64_t k = S.x;
+    const int64_t v0 = (int64_t)(uint32_t)S.y | ((int64_t)S.z << 32);
+    const int nv = S.w;
+    const int i = lane & 31, h = lane >> 5;
+    const int r = i & 15, part = i >> 4;
+    const int b0 = blockIdx.y * NBLK * 32;
+    floatx16 f1[NBLK], g2[NBLK];
+#pragma unroll
+    for (int nb = 0; nb < NBLK; nb++)
+#pragma unroll
+        for (int t = 0; t < 16; t++) { f1[nb][t] = 0.0f; g2[nb][t] = 0.0f; }
+    uint4 *buf = stage[wv];
+    auto rec_of = [&](uint32_t sd) {
+        const int64_t s = sd >> 1;
+        return C64 ? (const uint4 *)((const float2 *)Pq + s * 512) : (const uint4 *)((const uint32_t *)Pq + s * 512);
+    };
+    uint32_t sd_next = vsd[v0];
+    uint4 nxt[NCH];
+    {
+        const uint4 *rec = rec_of(sd_next);
+#pragma unroll
+        for (int q = 0; q < NCH; q++) nxt[q] = ld_stream_u4(rec + q * 64 + lane);
+    }
+    for (int u = 0; u < nv; u++) {
+        const int64_t v = v0 + u;
+        const uint32_t sd = sd_next;
+        uint4 cur[NCH];
+#pragma unroll
+        for (int q = 0; q < NCH; q++) cur[q] = nxt[q];
+        if (u + 1 < nv) {
+            sd_next = vsd[v + 1];
+            const uint4 *rec = rec_of(sd_next);
+#pragma unroll
+            for (int q = 0; q < NCH; q++) nxt[q] = ld_stream_u4(rec + q * 64 + lane);
+        }
+        __builtin_amdgcn_wave_barrier();  // the previous visit's reads of the stage are done
+#pragma unroll
+        for (int q = 0; q < NCH; q++) buf[q * 64 + lane] = cur[q];
+        __builtin_amdgcn_wave_barrier();
+        const bool rev = sd & 1u;
+        const float sc = C64 ? 1.0f : pscale[sd >> 1];
+        // complex element (row, col) of A1 = P1 | conj(P2) (mat 0) or A2^T = P2^T | conj(P1)^T (mat 1)
+        auto a_el = [&](int mat, int row, int col) {
+            const int src = mat == 0 ? (rev ? 1 : 0) : (rev ? 0 : 1);
+            const int rr = mat == 0 ? row : col, cc = mat == 0 ? col : row;
+            float2 w;
+            if (C64) {
+                w = ((const float2 *)buf)[blk_off(src, rr, cc, 2)];
+            } else {
+                const uint32_t x = ((const uint32_t *)buf)[blk_off(src, rr, cc, 4)];
+                w = make_float2(bflo(x) * sc, bfhi(x) * sc);
+            }
+            if (rev) w.y = -w.y;
+            return w;
+        };
+        uint32_t mh[2][4], ml[2][4];  // the lane's 8 K columns of row i, BF16 pairs
+#pragma unroll
+        for (int mat = 0; mat < 2; mat++)
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                const int p = 8 * h + t;
+                // K column p -> (ii, jj): 0..3 diagonal, 4..9 Re upper, 10..15 Im upper
+                const int ii = p < 4 ? p : (p < 10 ? (p < 7 ? 0 : (p < 9 ? 1 : 2)) : (p < 13 ? 0 : (p < 15 ? 1 : 2)));
+                const int jj = p < 4 ? p : (p < 10 ? (p < 7 ? p - 3 : (p < 9 ? p - 5 : 3)) : (p < 13 ? p - 9 : (p < 15 ? p - 11 : 3)));
+                const float2 aij = a_el(mat, r, 4 * ii + jj);
+                float2 m;
+                if (p < 4) {
+                    m = aij;
+                } else {
+                    const float2 aji = a_el(mat, r, 4 * jj + ii);
+                    m = p < 10 ? make_float2(aij.x + aji.x, aij.y + aji.y)
+                               : make_float2(-(aij.y - aji.y), aij.x - aji.x);
+                }
+                const float x = part ? m.y : m.x;
+                const uint32_t hi = f2bf(x), lo = f2bf(x - __uint_as_float(hi << 16));
+                if (t & 1) {
+                    mh[mat][t >> 1] |= hi << 16;
+                    ml[mat][t >> 1] |= lo << 16;
+                } else {
+                    mh[mat][t >> 1] = hi;
+                    ml[mat][t >> 1] = lo;
+                }
+            }
+        const bf16x8 M1h = as_bf8(make_uint4(mh[0][0], mh[0][1], mh[0][2], mh[0][3]));
+        const bf16x8 M2h = as_bf8(make_uint4(mh[1][0], mh[1][1], mh[1][2], mh[1][3]));
+        const bf16x8 M1l = as_bf8(make_uint4(ml[0][0], ml[0][1], ml[0][2], ml[0][3]));
+        const bf16x8 M2l = as_bf8(make_uint4(ml[1][0], ml[1][1], ml[1][2], ml[1][3]));
+        const uint16_t *xq = xb + (int64_t)vq[v] * 4 * (int64_t)Bp * 8;
+#pragma unroll
+        for (int nb = 0; nb < NBLK; nb++) {
+            const int b = b0 + nb * 32 + i;
+            const bf16x8 xh = as_bf8(*(const uint4 *)(xq + ((0 * 2 + h) * (int64_t)Bp + b) * 8));
+            const bf16x8 xl = as_bf8(*(const uint4 *)(xq + ((1 * 2 + h) * (int64_t)Bp + b) * 8));
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xh, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xh, g2[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xl, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xl, g2[nb], 0, 0, 0);
+            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1l, xh, f1[nb], 0, 0, 0);
+            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2l, xh, g2[nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NBLK; nb++) {
+        const int b = b0 + nb * 32 + i;
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int row = 8 * (t >> 2) + 4 * h + (t & 3);
+            unsafeAtomicAdd(F + (k * 64 + row) * Bp + b, f1[nb][t]);
+            unsafeAtomicAdd(F + (k * 64 + 32 + row) * Bp + b, g2[nb][t]);
+        }
+    }
+}
+
 // ddm of every trajectory: C = (1-rho)(F1_eq + F1) - (F2_eq - G) rho, ddm =
 // prefac/2 (C + C^+), rotated back, minus ddmdt_eq.  Clears F.  One thread
 // per (k, b) with b fastest, so each of the 64 row sums of k is one coalesced
@@ -343,23 +478,41 @@ static int build_visits(dmd_scatt *h)
     }
     h->h_vptr[nk] = v;
     int rc;
-    dfree(h->d_vrec); dfree(h->d_vq);
-    h->d_vrec = nullptr; h->d_vq = nullptr;
+    dfree(h->d_vrec); dfree(h->d_vq); dfree(h->d_vsd);
+    h->d_vrec = nullptr; h->d_vq = nullptr; h->d_vsd = nullptr;
     const bool c64 = h->precision == DMD_PREC_C64;
-    if ((rc = dalloc_t(&h->d_vrec, (size_t)nvis * 2048, "visit records"))) return rc;
+    if (h->nslots >= (1ll << 31)) return fail(DMD_ERR_UNSUPPORTED, "batched: 2^31 or more stored slots");
+    // Visit records (4 KiB per visit) when they fit in device memory: forming
+    // M per visit from the stored P reads half the bytes for BF16 P but costs
+    // more instructions than it saves (BF16, preset 4: 35 against 25 ms per
+    // evaluation at B = 16); without room for the records the pass forms M from
+    // P.  DMD_BATCH_RECORDS=0 forces the record-free pass (tests).
+    const char *renv = std::getenv("DMD_BATCH_RECORDS");
+    h->bdirect = renv && renv[0] == '0';
+    std::vector<uint32_t> vsd(nvis);
+    for (int64_t q = 0; q < nvis; q++) vsd[q] = (uint32_t)(vslot[q] << 1) | (vdir[q] ? 1u : 0u);
+    if (!h->bdirect && (rc = dalloc_t(&h->d_vrec, (size_t)nvis * 2048, "visit records"))) {
+        if (rc != DMD_ERR_NOMEM) return rc;
+        (void)hipGetLastError();  // the failed allocation: continue without records
+        h->bdirect = true;
+    }
     if ((rc = dalloc_t(&h->d_vq, (size_t)nvis, "visit partners"))) return rc;
+    if ((rc = dalloc_t(&h->d_vsd, (size_t)std::max<int64_t>(nvis, 1), "visit slots"))) return rc;
     int64_t *d_vslot = nullptr;
     uint8_t *d_vdir = nullptr;
-    if ((rc = dalloc_t(&d_vslot, (size_t)nvis, "visit slots"))) return rc;
-    if ((rc = dalloc_t(&d_vdir, (size_t)nvis, "visit directions"))) { dfree(d_vslot); return rc; }
+    if (!h->bdirect) {
+        if ((rc = dalloc_t(&d_vslot, (size_t)nvis, "visit slots"))) return rc;
+        if ((rc = dalloc_t(&d_vdir, (size_t)nvis, "visit directions"))) { dfree(d_vslot); return rc; }
+    }
     hipError_t e = hipSuccess;
     if (nvis) {
-        e = hipMemcpy(d_vslot, vslot.data(), nvis * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d_vdir, vdir.data(), nvis, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice);
+        e = hipMemcpy(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->d_vsd, vsd.data(), nvis * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !h->bdirect) e = hipMemcpy(d_vslot, vslot.data(), nvis * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !h->bdirect) e = hipMemcpy(d_vdir, vdir.data(), nvis, hipMemcpyHostToDevice);
     }
 
-    if (e == hipSuccess && nvis) {
+    if (e == hipSuccess && nvis && !h->bdirect) {
         if (c64)
             hipLaunchKernelGGL(k_build_visits<true>, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
                                (const void *)h->d_P, h->d_pscale, h->d_vrec);
@@ -498,6 +651,18 @@ static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
 {
     const int nb2 = (Bp / 32) % 2 == 0 && nblk_req == 2 ? 2 : 1;
     dim3 g(nblk(h->nseg, 4), Bp / (32 * nb2));
+    if (h->bdirect) {
+        const bool c64 = h->precision == DMD_PREC_C64;
+#define DMD_BP(C, N)                                                                                              \
+    hipLaunchKernelGGL((k_batched_p<C, N>), g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, (const void *)h->d_P,  \
+                       h->d_pscale, h->d_vsd, h->d_vq, (const uint16_t *)h->d_xb, h->d_Fb)
+        if (c64 && nb2 == 2) DMD_BP(true, 2);
+        else if (c64) DMD_BP(true, 1);
+        else if (nb2 == 2) DMD_BP(false, 2);
+        else DMD_BP(false, 1);
+#undef DMD_BP
+        return;
+    }
     if (nb2 == 2)
         hipLaunchKernelGGL(k_batched<2>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
                            (const uint16_t *)h->d_xb, h->d_Fb);

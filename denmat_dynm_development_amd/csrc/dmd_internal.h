@@ -203,6 +203,8 @@ struct dmd_scatt {
     // batched trajectories (BF16, nb = 4): visit records, segments, operands
     uint16_t *d_vrec = nullptr;          // visits x 2048 BF16: M1 hi | M2 hi | M1 lo | M2 lo
     int32_t *d_vq = nullptr;             // partner row of each visit
+    uint32_t *d_vsd = nullptr;           // slot << 1 | reverse of each visit
+    bool bdirect = false;                // batched pass forms M from the stored P (no visit records)
     int4 *d_seg = nullptr;               // segments: (row, first visit lo, hi, count)
     int64_t nseg = 0;
     bool bvis_ready = false, bvis_dirty = true;

@@ -78,6 +78,37 @@ def test_eval_batched_matches_oracle_and_unbatched(prec, B):
     assert worst < 5e-6 and d_one < 5e-6
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_record_free_pass_matches_the_record_pass(prec, monkeypatch):
+    """The pass that forms each visit's folded M from the stored P (used when
+    the visit records do not fit; DMD_BATCH_RECORDS=0 forces it) against the
+    record pass on the same handle set-up: the same products, so the results
+    differ only by the order of the FP32 row-sum atomics."""
+    scatt, oracle = _mods()
+    print(f"case batched record-free prec={prec}", flush=True)
+    rng = np.random.default_rng(7 + prec)
+    nk, nb, B = 250, 4, 40
+    k1, k2 = _hub_pairs(nk, 3, rng)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(B, nk, nb, f, rng)
+    t = 53 * FS
+    out = {}
+    for rec in ("1", "0"):
+        monkeypatch.setenv("DMD_BATCH_RECORDS", rec)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=prec) as op:
+            op.upload_P(P1, P2)
+            op.set_split(f)
+            rho_f = np.zeros((nk, nb * nb), np.complex128)
+            rho_f[:, ::nb + 1] = f
+            out[rec] = (op.eval_batched(t, rho), op.eval(t, rho_f))
+    (a, base), (b, _) = out["1"], out["0"]
+    err = rel_err(b - base[None], a - base[None])
+    print(f"prec={prec}: record-free vs records, drho part {err:.2e}")
+    assert err < 1e-6
+
+
 def test_eval_batched_fp64_handle_is_per_trajectory():
     scatt, oracle = _mods()
     rng = np.random.default_rng(7)
