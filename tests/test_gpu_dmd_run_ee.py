@@ -201,3 +201,53 @@ def test_dmd_run_reference_linearized_eph_ei_ee_keys(tmp_path):
     # the P3..P6 terms matter at that tolerance
     g0 = lambda t, y: oracle.evolve_linear(k1, k2, Lij, Lji, e, y, feq, t=t, prefac=prefac)
     assert np.abs(_rk4_run(g0, rho, 4, 1 * FS, spin) - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+def test_dmd_run_linearized_ee_with_exchange_doubles_the_pair_terms(tmp_path):
+    """alg_linearize = 1, eeMode = Pee_fixed_at_eq, ee_antisymmetry = 1 (Debye):
+    set_Lsc of P_eph + scale_ee P_ee (with the exchange term) at f_eq, plus
+    the fac2 doubling of the e-e Lscij / Lscji that add_Lsc_contrib("ee")
+    applies (add_Lscij_from_P1_P2 with fac2 = ee_antisymmetry,
+    ElectronPhonon_linear.cpp:79,180-199; Lscii once); S_z against the
+    oracle's RK4 of evolve_linear of that operator, and the doubling is
+    visible at that tolerance."""
+    from denmat_dynm_development_amd import scatt
+    from oracle import oracle
+    print("case dmd_run linearized e-e with exchange", flush=True)
+    d, info, U, e, kv, k1, k2, P1, P2, s3, R = _folder(tmp_path, 47)
+    nk, nb = e.shape
+    V = abs(np.linalg.det(R))
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    eps = 4.0
+    km = tuple(info.kmesh)
+    sig, ethr = info.degauss, info.degauss * info.ndegauss
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    m = scatt.eimp_model("debye", feq.sum() / (info.nk_full * V), T, V, scatt.reciprocal_metric(R), 5,
+                         n_ionized=1.0, epsilon_background=eps, degauss=sig, ethr=ethr, kmesh=km,
+                         nk_full=info.nk_full, exchange=True).as_dict()
+    vq = np.array([oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)])
+    Q1, Q2 = oracle.ee_model_P(k1, k2, U, kv, e, feq, vq, km, sig, ethr, info.nk_full, True, antisym_model=m)
+    scale_ee = float(f"{np.abs(P1).max() / np.abs(Q1).max():.6g}")
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 4\nfreq_measure = 1\n"
+                                "alg_linearize = 1\neeMode = Pee_fixed_at_eq\nee_antisymmetry = 1\n"
+                                f"scale_ee = {scale_ee}\nscrMode = medium\nscrFormula = debye\n"
+                                f"epsilon_background = {eps}\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-800:])
+    assert r.returncode == 0 and "ee_antisymmetry = 1" in r.stdout and "linearized at" in r.stdout
+    Lij, Lji = oracle.set_Lsc(k1, k2, P1 + scale_ee * Q1, P2 + scale_ee * Q2, feq)
+    Lij2, Lji2 = oracle.add_Lsc_ee(k1, k2, feq, Lij, Lji, extra=scale_ee, Pee=(Q1, Q2))
+    prefac = 2 * np.pi / info.nk_full
+    obs_prefac = 1.0 / (info.nk_full * V * BOHR2CM ** 3)
+    spin = lambda y, t: oracle.spin(s3[:, 2], e, f, y, t, 0, nb, obs_prefac)
+    g = lambda t, y: oracle.evolve_linear(k1, k2, Lij2, Lji2, e, y, feq, t=t, prefac=prefac)
+    ref = _rk4_run(g, rho, 4, 1 * FS, spin)
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", ref)
+    assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
+    g0 = lambda t, y: oracle.evolve_linear(k1, k2, Lij, Lji, e, y, feq, t=t, prefac=prefac)
+    assert np.abs(_rk4_run(g0, rho, 4, 1 * FS, spin) - ref).max() > 1e-6 * np.abs(ref).max()
