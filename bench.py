@@ -630,6 +630,33 @@ def mode_eimp(args, torch, scatt):
                                   "pairs_per_s": len(a1) / el, "pair_k3_terms_per_s": terms / el}
     log(f"ee debye: {out['results']['ee_debye']}")
     op.close()
+    # the Pee_update rebuild (P_ee at the current density matrix) on every stored pair of that workload,
+    # against the RK4 step it interleaves with (eeMode = Pee_update rebuilds every freq_update_ee_model steps)
+    a1, b1 = scatt.synth_kpairs(p1)
+    s3 = scatt.synth_spin(p1)
+    rho1, _ = initial_state(p1, e1, s3)
+    dm = rho1.copy()
+    dm[:, ::p1.nb + 1] = f1  # the window's state: f on the diagonal plus the spin imbalance
+    with scatt.ScatteringOperator(p1.nk, p1.nb, a1, b1, e1, nk_full=p1.nk) as op:
+        op.generate_P(p1)
+        op.add_ee_model_state(a1, b1, U1, g1, f1, m, dm=dm, add=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.add_ee_model_state(a1, b1, U1, g1, f1, m, dm=dm, add=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        rho_d = torch.from_numpy(rho1).cuda()
+        op.integrate_rk4(rho_d, 0.0, 1.0 * FS, 2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.integrate_rk4(rho_d, 0.0, 1.0 * FS, 20)
+        torch.cuda.synchronize()
+        step = (time.perf_counter() - t0) / 20
+    out["results"]["ee_rebuild"] = {"workload": f"nb={p1.nb} nk={p1.nk} (16^3 mesh), all {len(a1)} stored pairs, "
+                                                f"P_ee at a density matrix", "wall_s": el,
+                                    "pairs_per_s": len(a1) / el, "rk4_step_ms": step * 1e3,
+                                    "rebuild_in_rk4_steps": el / step}
+    log(f"ee rebuild: {out['results']['ee_rebuild']}")
     return out
 
 

@@ -247,6 +247,172 @@ __global__ __launch_bounds__(256) void k_ee_P(int64_t n, const uint64_t *__restr
     P2[idx] = make_double2(acc2.x * a.pfac, acc2.y * a.pfac);
 }
 
+// The same P for nb <= 2 with two wavefronts per stored pair, one for P1 and
+// one for P2 (one launch each; FIX < NB would split a form further by its
+// first band index): the lanes split the k3 sum, and per k3 a lane forms what
+// the wavefront's elements share once -- the energy-window test over all band
+// combinations, the overlaps <k3 x|k4 y> (and the exchange overlaps), the A2
+// blocks of every (p, q) with their Gaussians and, per (p, q), the A1 block
+// times the density matrices -- then adds its contribution to every element;
+// a wavefront reduction ends the pair.  The thread-per-element form evaluates
+// every shared factor once per element (16 times per (pair, k3) at nb = 2),
+// this one twice: the Pee_update rebuild of 964,065 pairs at nk = 4096 went
+// from 2.25 s to 0.89 s (with four wavefronts per pair, split by i1: 1.35 s).
+// WHICH 0: P1, 1: P2; FIX: the first band index i1, or NB for all of them
+template <int NB, int WHICH, int FIX>
+__global__ __launch_bounds__(256) void k_ee_pair(int64_t n, const uint64_t *__restrict__ k1,
+                                                 const uint64_t *__restrict__ k2, EeArgs a, EimpConst c,
+                                                 double2 *__restrict__ P1, double2 *__restrict__ P2)
+{
+    constexpr int NB2 = NB * NB, NB3 = NB2 * NB, NB4 = NB2 * NB2;
+    constexpr int which = WHICH, F0 = FIX == NB ? 0 : FIX, NF = FIX == NB ? NB : 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t fp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (fp >= n) return;  // whole wavefronts
+    const int64_t A = (int64_t)k1[fp], B = (int64_t)k2[fp];
+    const bool ex = c.m.exchange != 0, lb = c.m.lindblad != 0;
+    double2 acc[NF * NB3];
+#pragma unroll
+    for (int e = 0; e < NF * NB3; e++) acc[e] = make_double2(0.0, 0.0);
+    if (A != B || ex) {
+        const double v12 = a.vq_pair ? a.vq_pair[fp] : ee_v(a, c.m, A, B);
+        const double *eA = a.e + A * NB, *eB = a.e + B * NB;
+        double2 o12[NB2];
+        double d12[NB2], dmin = 1e300, dmax = -1e300;
+#pragma unroll
+        for (int p = 0; p < NB; p++)
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const double2 o = ee_ov<NB>(a, A, B, p, q);
+                o12[p * NB + q] = make_double2(o.x * v12, o.y * v12);
+                d12[p * NB + q] = eA[p] - eB[q];
+                dmin = fmin(dmin, d12[p * NB + q]);
+                dmax = fmax(dmax, d12[p * NB + q]);
+            }
+        for (int64_t c3 = lane; c3 < a.nk; c3 += 64) {
+            if (ex && A == B && B == c3) continue;
+            const int64_t c4 = ee_k4(a, A, B, c3);
+            if (c4 < 0) continue;
+            const double *e3 = a.e + c3 * NB, *e4 = a.e + c4 * NB;
+            double tlo = 1e300, thi = -1e300;
+#pragma unroll
+            for (int x = 0; x < NB; x++)
+#pragma unroll
+                for (int y = 0; y < NB; y++) {
+                    tlo = fmin(tlo, e3[x] - e4[y]);
+                    thi = fmax(thi, e3[x] - e4[y]);
+                }
+            // every A2 entry outside the window: nothing to add
+            if (dmin + tlo >= c.m.ethr || dmax + thi <= -c.m.ethr) continue;
+            double2 o34[NB2], o14[NB2], o32[NB2];
+#pragma unroll
+            for (int x = 0; x < NB; x++)
+#pragma unroll
+                for (int y = 0; y < NB; y++) o34[x * NB + y] = ee_ov<NB>(a, c3, c4, x, y);
+            if (ex) {
+                const double v14 = ee_v(a, c.m, A, c4);
+#pragma unroll
+                for (int p = 0; p < NB; p++)
+#pragma unroll
+                    for (int q = 0; q < NB; q++) {
+                        const double2 o = ee_ov<NB>(a, A, c4, p, q);
+                        o14[p * NB + q] = make_double2(o.x * v14, o.y * v14);
+                        o32[p * NB + q] = ee_ov<NB>(a, c3, B, p, q);
+                    }
+            }
+            auto mee = [&](int p, int q, int x, int y) {
+                double2 m = cmul(o12[p * NB + q], o34[x * NB + y]);
+                if (ex) {
+                    const double2 t = cmul(o14[p * NB + y], o32[x * NB + q]);
+                    m = make_double2(0.5 * m.x - 0.5 * t.x, 0.5 * m.y - 0.5 * t.y);
+                }
+                return m;
+            };
+            // A2 of every (p, q): the factor conjugated in the sum
+            double2 a2[NB2][NB2];
+#pragma unroll
+            for (int pq = 0; pq < NB2; pq++)
+#pragma unroll
+                for (int xy = 0; xy < NB2; xy++) {
+                    const int p = pq / NB, q = pq % NB, x = xy / NB, y = xy % NB;
+                    const double2 m = mee(p, q, x, y);
+                    const double dl = ee_dl(c, d12[pq] + e3[x] - e4[y]);
+                    a2[pq][xy] = make_double2(m.x * dl, m.y * dl);
+                }
+            // P1: dm1_k3 A1 dm_k4, P2: dm_k3 A1 dm1_k4 (upper-triangle Hermitian reads); null: diag f
+            const double2 *L = nullptr, *Rm = nullptr;
+            if (a.dm) {
+                L = which ? a.dm + c3 * NB2 : a.dm1 + c3 * NB2;
+                Rm = which ? a.dm1 + c4 * NB2 : a.dm + c4 * NB2;
+            }
+            const double *f3 = a.f + c3 * NB, *f4 = a.f + c4 * NB;
+#pragma unroll
+            for (int fv = 0; fv < NF * NB; fv++) {
+                // P1: (i1, i3) = (fix, v); P2: (i3, i1) = (v, fix)
+                const int fi = fv / NB, fix = F0 + fi, v = fv % NB;
+                const int p = which ? v : fix, q = which ? fix : v;
+                double2 a1[NB2], t[NB2];
+#pragma unroll
+                for (int xy = 0; xy < NB2; xy++) a1[xy] = lb ? a2[p * NB + q][xy] : mee(p, q, xy / NB, xy % NB);
+                if (L) {
+                    double2 u[NB2];
+#pragma unroll
+                    for (int x = 0; x < NB; x++)
+#pragma unroll
+                        for (int y = 0; y < NB; y++) {
+                            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int k = 0; k < NB; k++) s = cadd(s, cmul(a1[x * NB + k], ee_hu<NB>(Rm, k, y)));
+                            u[x * NB + y] = s;
+                        }
+#pragma unroll
+                    for (int x = 0; x < NB; x++)
+#pragma unroll
+                        for (int y = 0; y < NB; y++) {
+                            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                            for (int k = 0; k < NB; k++) s = cadd(s, cmul(ee_hu<NB>(L, x, k), u[k * NB + y]));
+                            t[x * NB + y] = s;
+                        }
+                } else {
+#pragma unroll
+                    for (int x = 0; x < NB; x++)
+#pragma unroll
+                        for (int y = 0; y < NB; y++) {
+                            const double w = which ? f3[x] * (1 - f4[y]) : (1 - f3[x]) * f4[y];
+                            t[x * NB + y] = make_double2(a1[x * NB + y].x * w, a1[x * NB + y].y * w);
+                        }
+                }
+                // P1 element (fix, r, v, s) with A2 of (i2, i4) = (r, s); P2 element
+                // (fix, s, v, r) with A2 of (i4, i2) = (r, s): local index within the
+                // wavefront's nb^3 block (i2 i3 i4)
+#pragma unroll
+                for (int r = 0; r < NB; r++)
+#pragma unroll
+                    for (int s = 0; s < NB; s++) {
+                        const int li = fi * NB3 + (which ? (s * NB + v) * NB + r : (r * NB + v) * NB + s);
+                        double2 sum = acc[li];
+#pragma unroll
+                        for (int xy = 0; xy < NB2; xy++) sum = cadd(sum, cmulc(t[xy], a2[r * NB + s][xy]));
+                        acc[li] = sum;
+                    }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < NF * NB3; e++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc[e].x += __shfl_down(acc[e].x, off, 64);
+            acc[e].y += __shfl_down(acc[e].y, off, 64);
+        }
+    if (lane == 0) {
+        double2 *out = (which ? P2 : P1) + fp * NB4 + F0 * NB3;
+#pragma unroll
+        for (int e = 0; e < NF * NB3; e++) out[e] = make_double2(acc[e].x * a.pfac, acc[e].y * a.pfac);
+    }
+}
+
 // P3..P6 of n stored pairs (k1, ka) at f, one thread per element (i1 i2 ia ib):
 //   P3/P4: k3 over k, k4 = k1 + k3 - ka, A from calc_A(k1, k4, k3, ka),
 //     P3 += sum_{i4 i3} A1[(i1 i4),(i3 ia)] (1-f3) f4 conj(A2[(i2 i4),(i3 ib)]),
@@ -531,11 +697,18 @@ EimpConst ee_const(const dmd_eimp_model &m)
     return c;
 }
 
+// nb <= 2: a wavefront per pair and form; nb > 2: a thread per element
 template <int NB>
 void launch_ee_P(dmd_scatt *h, int64_t n, const uint64_t *d1, const uint64_t *d2, const EeArgs &a, const EimpConst &c,
                  double2 *s1, double2 *s2)
 {
-    hipLaunchKernelGGL(k_ee_P<NB>, dim3(nblk(n * NB * NB * NB * NB, 256)), dim3(256), 0, h->st, n, d1, d2, a, c, s1, s2);
+    if constexpr (NB <= 2) {
+        const dim3 g(nblk(n, 4)), b(256);
+        hipLaunchKernelGGL((k_ee_pair<NB, 0, NB>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+        hipLaunchKernelGGL((k_ee_pair<NB, 1, NB>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+    } else
+        hipLaunchKernelGGL(k_ee_P<NB>, dim3(nblk(n * NB * NB * NB * NB, 256)), dim3(256), 0, h->st, n, d1, d2, a, c, s1,
+                           s2);
 }
 
 // The fold of scale x P_ee into P of every stored pair (add = 0: set), chunked.
@@ -705,7 +878,7 @@ extern "C" int dmd_scatt_add_Lsc_ee(dmd_scatt *h, const uint64_t *k1st, const ui
         const dim3 g(nblk(n * per, 256)), b(256);
 #define DMD_LSC_EE(NB_)                                                                                              \
     do {                                                                                                             \
-        if (extra != 0.0) hipLaunchKernelGGL(k_ee_P<NB_>, g, b, 0, h->st, n, d1, d2, a, c, Pe1, Pe2);                \
+        if (extra != 0.0) launch_ee_P<NB_>(h, n, d1, d2, a, c, Pe1, Pe2);                                             \
         if (dP_terms) hipLaunchKernelGGL(k_ee_P3456<NB_>, g, b, 0, h->st, n, d1, d2, a, c, P3, P4, P5, P6);         \
         hipLaunchKernelGGL(k_lsc_ee<NB_>, g, b, 0, h->st, n, d1, d2, a.f, extra, coef34, extra != 0.0 ? Pe1 : nullptr, \
                            Pe2, (dP_terms && !m->exchange) ? P3 : nullptr, P4, dP_terms ? P5 : nullptr, P6, L1, L2);    \
