@@ -20,6 +20,9 @@ spin lifetime of the same runs compared.
   python bench.py --mode linear     # the linearized operator (set_Lsc + evolve_linear), nb = 2 and nb = 4
   python bench.py --mode ode        # the adaptive rkf45 driver and compute_rhs against fixed-step RK4, nb = 2
   python bench.py --mode eimp       # model e-i P (Debye, static RPA) and e-e P generated on the device, nb = 2
+  python bench.py --mode sparse     # compressed P against the dense store at several densities, nb = 2
+  python bench.py --mode panels     # the nb = 4 panel kernel alone in FP64, complex64 and BF16
+  python bench.py --mode batched [--B 3,16,64] [--batched-forms c64,bf16]  # the batched forms alone
 """
 import argparse
 import json
@@ -191,15 +194,20 @@ def bench_plain(args, torch, scatt):
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": ab},
     }
     if args.copy_calib:
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads 1 GiB, writes 1 GiB
-        torch.cuda.synchronize()
-        log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
-        del src, dst
+        copy_calibration(torch)
     op.close()
     return out, (p, e, k1, k2, rho0)
+
+
+def copy_calibration(torch):
+    """one elementwise kernel of known traffic for scaling the HBM counters"""
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 1.0, out=dst)  # reads 1 GiB, writes 1 GiB
+    torch.cuda.synchronize()
+    log("copy calibration: 1 GiB read + 1 GiB written by one elementwise kernel")
+    del src, dst
 
 
 def run_traj(op, torch, rho, dt, steps, batched):
@@ -247,9 +255,12 @@ def bench_mixed_batched(args, torch, scatt):
     log(f"fp64: {el / K * 1e3:.1f} ms/step, kernel {kms:.2f} ms ({ab / kms / 1e6:.0f} GB/s), tau {tau64:.6g} ps")
     op.close()
     base = K / el
-    # the batched pass stores every visit's matrices as BF16 hi + lo words, so the
-    # complex64 form runs on P rounded to a 16-bit significand (exact for BF16 P)
-    for prec in (1, 2):
+    # the batched pass stores every visit's matrices as BF16 hi + lo words for the
+    # complex64 store (P rounded to a 16-bit significand) and as one FP16 word for
+    # the BF16 store (DMD_BATCH_HALF=0: hi + lo words for it too)
+    half = os.environ.get("DMD_BATCH_HALF", "1") != "0"
+    forms = {"c64": 1, "bf16": 2}
+    for prec in [forms[x] for x in args.batched_forms.split(",")]:
         name = {1: "c64_P_as_bf16_hi_lo", 2: "bf16"}[prec]
         try:
             op = scatt.ScatteringOperator(p.nk, p.nb, k1, k2, e, nk_full=p.nk, precision=prec)
@@ -265,12 +276,15 @@ def bench_mixed_batched(args, torch, scatt):
                 kms = op.time_eval_batched(0.0, 3)
                 # matrix-core work of one evaluation: every stored pair is two visits (k < k')
                 # or one (k = k'); per visit and 32 trajectories 6 v_mfma_f32_32x32x16_bf16
+                # (hi + lo records) or 4 v_mfma_f32_32x32x16_f16 (FP16 records)
                 n_visits = 2 * n_pairs - int(np.count_nonzero(k1 == k2))
-                mfma_flop = n_visits * ((B + 31) // 32) * 6 * 2 * 32 * 32 * 16
+                n_mfma = 4 if (prec == 2 and half) else 6
+                mfma_flop = n_visits * ((B + 31) // 32) * n_mfma * 2 * 32 * 32 * 16
                 per[str(B)] = {"traj_steps_per_s": B * K / el, "ms_per_step": el / K * 1e3,
                                "speedup_vs_fp64": B * K / el / base, "eval_ms": kms,
                                "mfma_tflops": mfma_flop / (kms * 1e-3) / 1e12,
-                               "mfma_frac_of_bf16_dense_peak": mfma_flop / (kms * 1e-3) / 2.5e15,
+                               "mfma_frac_of_dense_peak": mfma_flop / (kms * 1e-3) / 2.5e15,
+                               "records": "fp16" if n_mfma == 4 else "bf16_hi_lo",
                                "pair_traj_per_s": n_pairs * B / (kms * 1e-3),
                                "tau_s_ps": tau, "tau_rel_err_vs_fp64": abs(tau / tau64 - 1)}
                 log(f"{name} B={B}: {el / K * 1e3:.1f} ms/step, {B * K / el:.1f} traj-steps/s "
@@ -668,12 +682,13 @@ def main():
     ap.add_argument("--preset", type=int, default=2)
     ap.add_argument("--kernel", type=int, default=0)
     ap.add_argument("--mode", default="default", choices=["default", "accuracy", "adjoint", "linear", "ode", "eimp", "sparse",
-                                                        "panels"])
+                                                        "panels", "batched"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-preset", type=int, default=4)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--B", default="3,16,64")
+    ap.add_argument("--batched-forms", default="c64,bf16", help="batched stores to time (c64, bf16)")
     ap.add_argument("--fd-all", action="store_true", help="adjoint mode: check every parameter")
     ap.add_argument("--checkpoint-every", type=int, default=0,
                     help="adjoint mode: steps per checkpoint segment (0: one segment, every stage state kept)")
@@ -710,6 +725,11 @@ def main():
         return
     if args.mode == "eimp":
         print(json.dumps(mode_eimp(args, torch, scatt)), flush=True)
+        return
+    if args.mode == "batched":
+        if args.copy_calib:
+            copy_calibration(torch)
+        print(json.dumps({"mode": "batched", **bench_mixed_batched(args, torch, scatt)}), flush=True)
         return
     out, (p, e, k1, k2, rho0) = bench_plain(args, torch, scatt)
     if not args.no_cpu:

@@ -19,11 +19,23 @@
 // leading cross terms make 3 MFMAs per product, 6 per visit per 32
 // trajectories.
 //
+// FP16 records (BF16 store).  For P already rounded to BF16 (8-bit
+// significands) a second 16-bit word of M is wasted precision: M is kept as
+// one FP16 word (11-bit significand, relative rounding <= 2^-11, 8x below the
+// BF16 store's own 2^-8) with v_mfma_f32_32x32x16_f16, 2 KiB per visit, and x
+// as FP16 hi + lo (22-bit significand).  FP16's range is met by two powers of
+// two: M x G with G = 2^14 / max per-pair scale of the store (every entry
+// under 2^15), and x x s_b per trajectory with s_b = 2^14 / max |drho_b|
+// (every operand under 2^14); the row sums are divided by G s_b at assembly.
+// Entries below 2^-14 of those maxima lose relative precision (FP16
+// subnormals), with absolute error under 2^-25 of the maximum.  4 MFMAs per
+// visit per 32 trajectories (M xh, M xl for each product).
+//
 // Layouts in HBM.
 //  * Visit records, CSR by destination row: M1 (from A1) and M2 (from A2^T),
 //    32 x 16 BF16 row-major each, hi then lo: [M1 hi | M2 hi | M1 lo | M2 lo],
 //    4 KiB per visit (8 KiB per stored pair), conj and the per-pair scale of
-//    the BF16 store applied.
+//    the BF16 store applied; FP16 records [M1 | M2] x G, 2 KiB per visit.
 //  * X: per partner row q, [term][k-half][trajectory][8] BF16, so a
 //    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
 //    bytes (Bp x 64 bytes per row q).
@@ -36,6 +48,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -45,6 +58,7 @@
 namespace dmd {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 halfx8;
 typedef __attribute__((ext_vector_type(16))) float floatx16;
 
 __device__ __forceinline__ uint32_t f2bf(float f)  // round to nearest even (finite)
@@ -54,6 +68,19 @@ __device__ __forceinline__ uint32_t f2bf(float f)  // round to nearest even (fin
     return u >> 16;
 }
 __device__ __forceinline__ float bflo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ uint16_t f2h(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }  // nearest even
+__device__ __forceinline__ double h2d(uint16_t w) { return (double)__builtin_bit_cast(_Float16, w); }
+
+// FP16 operand scale of a trajectory from the bits of its max |drho|: a power
+// of two s with max |drho| s < 2^14 (1 for an all-zero or non-finite max).
+__device__ __forceinline__ double xscale_of(uint32_t bits)
+{
+    const float m = __uint_as_float(bits);
+    if (!(m > 0.0f) || !(m <= 3.0e38f)) return 1.0;
+    int ex;
+    (void)frexpf(m, &ex);  // m < 2^ex
+    return ldexp(1.0, min(14 - ex, 900));
+}
 __device__ __forceinline__ float bfhi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
 // Complex element (row, col) of A1 = P1 | conj(P2) or A2^T = P2^T | conj(P1)^T
@@ -75,20 +102,22 @@ __device__ __forceinline__ float2 a_elem(const void *rec, float sc, bool rev, in
 }
 
 // Visit records: M = [A_ii | A_ij + A_ji | i (A_ij - A_ji)] real-ified (32 x 16)
-// for A1 and A2^T, BF16 hi and lo.  One 64-lane group per visit; lane t
-// covers rows (t / 16) of the 4 row blocks and K column t % 16.
-template <bool C64>
-__global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const int64_t *__restrict__ vslot,
-                                                     const uint8_t *__restrict__ vdir, const void *__restrict__ Pq,
-                                                     const float *__restrict__ pscale, uint16_t *__restrict__ vrec)
+// for A1 and A2^T, BF16 hi and lo (HALF: M x gsc as one FP16 word).  One
+// 64-lane group per visit; lane t covers rows (t / 16) of the 4 row blocks and
+// K column t % 16.
+template <bool C64, bool HALF>
+__global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const uint32_t *__restrict__ vsd,
+                                                     const void *__restrict__ Pq, const float *__restrict__ pscale,
+                                                     double gsc, uint16_t *__restrict__ vrec)
 {
+    static_assert(!(C64 && HALF), "FP16 records are built from the BF16 store");
     const int64_t v = blockIdx.x;
     if (v >= nvis) return;
-    const int64_t s = vslot[v];
-    const bool rev = vdir[v] != 0;
+    const int64_t s = vsd[v] >> 1;
+    const bool rev = (vsd[v] & 1u) != 0;
     const void *rec = C64 ? (const void *)((const float2 *)Pq + s * 512) : (const void *)((const uint32_t *)Pq + s * 512);
-    const float sc = C64 ? 1.0f : pscale[s];
-    uint16_t *out = vrec + v * 2048;
+    const float sc = C64 ? 1.0f : (float)((double)pscale[s] * gsc);
+    uint16_t *out = vrec + v * (HALF ? 1024 : 2048);
     // K column p -> (i, j, kind): 0..3 diagonal, 4..9 Re upper, 10..15 Im upper
     const int pi[16] = { 0, 1, 2, 3, 0, 0, 0, 1, 1, 2, 0, 0, 0, 1, 1, 2 };
     const int pj[16] = { 0, 1, 2, 3, 1, 2, 3, 2, 3, 3, 1, 2, 3, 2, 3, 3 };
@@ -108,20 +137,65 @@ __global__ __launch_bounds__(64) void k_build_visits(int64_t nvis, const int64_t
         const float vals[2] = { m.x, m.y };
         for (int part = 0; part < 2; part++) {
             const float x = vals[part];
-            const uint32_t hi = f2bf(x), lo = f2bf(x - __uint_as_float(hi << 16));
             const int o = mat * 512 + (part * 16 + r) * 16 + p;
-            out[o] = (uint16_t)hi;
-            out[1024 + o] = (uint16_t)lo;
+            if (HALF) {
+                out[o] = f2h(x);
+            } else {
+                const uint32_t hi = f2bf(x), lo = f2bf(x - __uint_as_float(hi << 16));
+                out[o] = (uint16_t)hi;
+                out[1024 + o] = (uint16_t)lo;
+            }
         }
     }
 }
 
+__global__ void k_zero_u32(int n, uint32_t *__restrict__ a)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0u;
+}
+
+// Bits of an upper bound of max |x| of every trajectory (|rho_ij| for the
+// off-diagonal terms, which the rotation keeps; |rho_ii - f_i| on the
+// diagonal), for the FP16 operand scale.  Grid (k blocks of 256, B); one
+// atomic max per block on the FP32 bits (non-negative floats order as their
+// bits).
+__global__ __launch_bounds__(256) void k_xmax(int64_t nk, const double2 *__restrict__ rho,
+                                              const double *__restrict__ f, uint32_t *__restrict__ xmax)
+{
+    __shared__ float part[4];
+    const int b = blockIdx.y;
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float m = 0.0f;
+    if (k < nk) {
+        const double2 *r = rho + ((int64_t)b * nk + k) * 16;
+        double mx = 0.0;
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+            const int i = l >> 2, j = l & 3;
+            if (i > j) continue;
+            const double2 w = r[l];
+            const double v = i == j ? fabs(w.x - (f ? f[k * 4 + i] : 0.0)) : sqrt(w.x * w.x + w.y * w.y);
+            mx = fmax(mx, v);
+        }
+        m = (float)(mx * (1.0 + 1.0 / 1048576));  // FP32 rounding must not fall below the bound
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        if (m > 0.0f) atomicMax(xmax + b, __float_as_uint(m));
+    }
+}
+
 // x = (Re drho_ii; Re drho_ij, Im drho_ij, i < j) of every trajectory as the
-// hi/lo BF16 operand, and rho_S in FP64 for the assembly.  rho: B x nk x 16
-// (interaction picture).  One thread per (b, k, K column p).
+// hi/lo BF16 operand (with xmax: x s_b as FP16 hi/lo), and rho_S in FP64 for
+// the assembly.  rho: B x nk x 16 (interaction picture).  One thread per
+// (b, k, K column p).
 __global__ void k_prepare_x(int64_t nk, int B, int Bp, const double2 *__restrict__ rho, const double *__restrict__ e,
                             const DClock *__restrict__ clk, double a, int expt, const double *__restrict__ f,
-                            double2 *__restrict__ rhoS, uint16_t *__restrict__ xb)
+                            const uint32_t *__restrict__ xmax, double2 *__restrict__ rhoS, uint16_t *__restrict__ xb)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)Bp * nk * 16) return;
@@ -144,8 +218,15 @@ __global__ void k_prepare_x(int64_t nk, int B, int Bp, const double2 *__restrict
         if (l < 4) x = w.x - (f ? f[k * 4 + ii] : 0.0);
         else x = l < 10 ? w.x : w.y;
     }
-    const uint32_t hi = f2bf((float)x);
-    const uint32_t lo = f2bf((float)(x - (double)__uint_as_float(hi << 16)));
+    uint32_t hi, lo;
+    if (xmax) {
+        const double xs = x * xscale_of(xmax[b]);
+        hi = f2h((float)xs);
+        lo = f2h((float)(xs - h2d((uint16_t)hi)));
+    } else {
+        hi = f2bf((float)x);
+        lo = f2bf((float)(x - (double)__uint_as_float(hi << 16)));
+    }
     const int half = l >> 3, el = l & 7;
     // [q][term][k-half][b][8]
     xb[(((k * 2 + 0) * 2 + half) * Bp + b) * 8 + el] = (uint16_t)hi;
@@ -158,15 +239,32 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 u)
     __builtin_memcpy(&r, &u, 16);
     return r;
 }
+__device__ __forceinline__ halfx8 as_h8(uint4 u)
+{
+    halfx8 r;
+    __builtin_memcpy(&r, &u, 16);
+    return r;
+}
+
+// Visits per segment at most (the autotuner's segment lengths are 64, 128
+// and 256).
+constexpr int kSegMax = 256;
 
 // One wavefront per segment; NBLK blocks of 32 trajectories per pass (the
-// grid's y index selects the pass).  Products M_hi x_hi + M_hi x_lo + M_lo
-// x_hi (the M_lo x_lo term is below the 16-bit significands).
-template <int NBLK>
+// grid's y index selects the pass).  BF16 hi + lo records (HALF = false):
+// M_hi x_hi + M_hi x_lo + M_lo x_hi (the M_lo x_lo term is below the 16-bit
+// significands), 6 MFMAs per visit per block; FP16 records (HALF): M x_hi +
+// M x_lo, 4 MFMAs.  Timing variants that staged the segment's partner rows in
+// LDS and kept two visits' records and operands in flight per wavefront ran
+// no faster (the extra registers cost a wave per SIMD), so a visit's loads
+// are issued when it starts.
+template <bool HALF, int NBLK>
 __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
                                                  const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
                                                  const uint16_t *__restrict__ xb, float *__restrict__ F)
 {
+    constexpr int NR = HALF ? 2 : 4;            // 16-byte record words per lane
+    constexpr int64_t RS = HALF ? 1024 : 2048;  // record stride (16-bit words)
     const int lane = threadIdx.x & 63;
     const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= nseg) return;
@@ -181,24 +279,34 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
     for (int nb = 0; nb < NBLK; nb++)
 #pragma unroll
         for (int t = 0; t < 16; t++) { f1[nb][t] = 0.0f; g2[nb][t] = 0.0f; }
-    const int oa = i * 16 + 8 * h;  // the lane's 8 K columns of row i
+    // the lane's 8 K columns of row i in each 512-word matrix of a record
+    const uint16_t *rb = vrec + v0 * RS + i * 16 + 8 * h;
     for (int u = 0; u < nv; u++) {
-        const int64_t v = v0 + u;
-        const uint16_t *rec = vrec + v * 2048;
-        const bf16x8 M1h = as_bf8(ld_stream_u4(rec + oa)), M2h = as_bf8(ld_stream_u4(rec + 512 + oa));
-        const bf16x8 M1l = as_bf8(ld_stream_u4(rec + 1024 + oa)), M2l = as_bf8(ld_stream_u4(rec + 1536 + oa));
-        const uint16_t *xq = xb + (int64_t)vq[v] * 4 * (int64_t)Bp * 8;
+        uint4 m[NR];
+#pragma unroll
+        for (int w = 0; w < NR; w++) m[w] = ld_stream_u4(rb + u * RS + w * 512);
+        const uint16_t *xq = xb + (int64_t)vq[v0 + u] * 4 * (int64_t)Bp * 8;
 #pragma unroll
         for (int nb = 0; nb < NBLK; nb++) {
             const int b = b0 + nb * 32 + i;
-            const bf16x8 xh = as_bf8(*(const uint4 *)(xq + ((0 * 2 + h) * (int64_t)Bp + b) * 8));
-            const bf16x8 xl = as_bf8(*(const uint4 *)(xq + ((1 * 2 + h) * (int64_t)Bp + b) * 8));
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xh, f1[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xh, g2[nb], 0, 0, 0);
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, xl, f1[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, xl, g2[nb], 0, 0, 0);
-            f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1l, xh, f1[nb], 0, 0, 0);
-            g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2l, xh, g2[nb], 0, 0, 0);
+            const uint4 xh = *(const uint4 *)(xq + ((0 * 2 + h) * (int64_t)Bp + b) * 8);
+            const uint4 xl = *(const uint4 *)(xq + ((1 * 2 + h) * (int64_t)Bp + b) * 8);
+            if (HALF) {
+                const halfx8 M1 = as_h8(m[0]), M2 = as_h8(m[1]), ah = as_h8(xh), al = as_h8(xl);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M1, ah, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M2, ah, g2[nb], 0, 0, 0);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M1, al, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M2, al, g2[nb], 0, 0, 0);
+            } else {
+                const bf16x8 M1h = as_bf8(m[0]), M2h = as_bf8(m[1]), M1l = as_bf8(m[2]), M2l = as_bf8(m[3]);
+                const bf16x8 ah = as_bf8(xh), al = as_bf8(xl);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, ah, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, ah, g2[nb], 0, 0, 0);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, al, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, al, g2[nb], 0, 0, 0);
+                f1[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1l, ah, f1[nb], 0, 0, 0);
+                g2[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2l, ah, g2[nb], 0, 0, 0);
+            }
         }
     }
     // D layout: element t of lane l is row 8 (t / 4) + 4 h + t % 4, column (trajectory) l % 32
@@ -355,6 +463,7 @@ __global__ __launch_bounds__(256) void k_batched_p(int64_t nseg, int Bp, const i
 // row across the trajectories; the thread owns its 64 sums, reads and clears
 // them, and writes its 16 outputs.
 __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, float *__restrict__ F,
+                                                    const uint32_t *__restrict__ xmax, double gsc,
                                                     const double *__restrict__ feq, const double2 *__restrict__ rhoS,
                                                     const double *__restrict__ e, const DClock *__restrict__ clk,
                                                     double a, int expt, int schrodinger, double prefac,
@@ -368,6 +477,7 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
     if (b < B) {
         const int64_t bk = (int64_t)b * nk + k;
         const double *fe = feq + k * 64;
+        const double fs = xmax ? 1.0 / (gsc * xscale_of(xmax[b])) : 1.0;  // FP16 operands: undo G s_b
         double2 rk[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) rk[q] = rhoS[bk * 16 + q];
@@ -381,7 +491,8 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
 #pragma unroll
             for (int jj = 0; jj < 4; jj++) {
                 const int r = 4 * m + jj;  // F1[m][jj] = F1_eq + F
-                const double2 f1 = make_double2(fe[2 * r] + (double)Fk[r * Bp], fe[2 * r + 1] + (double)Fk[(16 + r) * Bp]);
+                const double2 f1 = make_double2(fe[2 * r] + fs * (double)Fk[r * Bp],
+                                                fe[2 * r + 1] + fs * (double)Fk[(16 + r) * Bp]);
 #pragma unroll
                 for (int ii = 0; ii < 4; ii++) {
                     double2 r1;
@@ -397,8 +508,8 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const int r = 4 * ii + m;  // F2[ii][m] = F2_eq - G
-                const double2 g = make_double2(fe[32 + 2 * r] - (double)Fk[(32 + r) * Bp],
-                                               fe[32 + 2 * r + 1] - (double)Fk[(48 + r) * Bp]);
+                const double2 g = make_double2(fe[32 + 2 * r] - fs * (double)Fk[(32 + r) * Bp],
+                                               fe[32 + 2 * r + 1] - fs * (double)Fk[(48 + r) * Bp]);
                 const double2 ng = make_double2(-g.x, -g.y);
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
@@ -431,9 +542,75 @@ __global__ __launch_bounds__(256) void k_assemble_b(int64_t nk, int B, int Bp, f
     for (int r = 0; r < 64; r++) Fk[r * Bp] = 0.0f;
 }
 
+// max of non-negative floats as bits (one atomic per block)
+__global__ __launch_bounds__(256) void k_umax_f(int64_t n, const float *__restrict__ a, uint32_t *__restrict__ out)
+{
+    __shared__ float part[4];
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) m = fmaxf(m, a[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        if (m > 0.0f) atomicMax(out, __float_as_uint(m));
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+
+// G = 2^14 / (power of two above the largest per-pair scale of the BF16
+// store), so that every FP16 record entry |M| <= 2 sc stays under 2^15.
+static int half_record_factor(dmd_scatt *h, double *g)
+{
+    uint32_t *d = nullptr, bits = 0;
+    int rc;
+    if ((rc = dalloc_t(&d, 1, "scale max"))) return rc;
+    hipError_t e = hipMemsetAsync(d, 0, 4, h->st);
+    if (e == hipSuccess && h->nslots > 0) {
+        hipLaunchKernelGGL(k_umax_f, dim3(1024), dim3(256), 0, h->st, h->nslots, (const float *)h->d_pscale, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bits, d, 4, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    dfree(d);
+    if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("record scale: ") + hipGetErrorString(e));
+    float m;
+    std::memcpy(&m, &bits, 4);
+    int ex = 0;
+    if (m > 0.0f && std::isfinite(m)) (void)std::frexp((double)m, &ex);  // m < 2^ex
+    *g = std::ldexp(1.0, 14 - ex);
+    return DMD_OK;
+}
+
+// The visits' partner rows and slots on the device, and their records built
+// there from the stored P (when records are kept).
+static int upload_visits(dmd_scatt *h, const std::vector<int32_t> &vq, const std::vector<uint32_t> &vsd)
+{
+    const int64_t nvis = (int64_t)vq.size();
+    if (!nvis) return DMD_OK;
+    DMD_HIP(hipStreamSynchronize(h->st));  // no pass may still read the arrays
+    DMD_HIP(hipMemcpyAsync(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice, h->st));
+    DMD_HIP(hipMemcpyAsync(h->d_vsd, vsd.data(), nvis * 4, hipMemcpyHostToDevice, h->st));
+    if (!h->bdirect) {
+        const dim3 g((unsigned)nvis);
+        if (h->precision == DMD_PREC_C64)
+            hipLaunchKernelGGL((k_build_visits<true, false>), g, dim3(64), 0, h->st, nvis, h->d_vsd,
+                               (const void *)h->d_P, h->d_pscale, 1.0, h->d_vrec);
+        else if (h->bhalf)
+            hipLaunchKernelGGL((k_build_visits<false, true>), g, dim3(64), 0, h->st, nvis, h->d_vsd,
+                               (const void *)h->d_P, h->d_pscale, h->bhalf_g, h->d_vrec);
+        else
+            hipLaunchKernelGGL((k_build_visits<false, false>), g, dim3(64), 0, h->st, nvis, h->d_vsd,
+                               (const void *)h->d_P, h->d_pscale, 1.0, h->d_vrec);
+        DMD_HIP(hipGetLastError());
+    }
+    DMD_HIP(hipStreamSynchronize(h->st));  // the host arrays may go now
+    return DMD_OK;
+}
 
 // visit records, segments (built once per P; rebuilt when P changes)
 static int build_visits(dmd_scatt *h)
@@ -453,77 +630,61 @@ static int build_visits(dmd_scatt *h)
         DMD_HIP(hipMemcpy(revpos.data(), h->d_revpos, h->nrev * 8, hipMemcpyDeviceToHost));
         DMD_HIP(hipMemcpy(revcol.data(), h->d_revcol, h->nrev * 4, hipMemcpyDeviceToHost));
     }
+    if (h->nslots >= (1ll << 31)) return fail(DMD_ERR_UNSUPPORTED, "batched: 2^31 or more stored slots");
     const int64_t nvis = h->np + h->nrev;
-    std::vector<int64_t> vslot(nvis);
-    std::vector<uint8_t> vdir(nvis);
+    std::vector<uint32_t> vsd(nvis);  // slot << 1 | reverse
     std::vector<int32_t> vq(nvis);
-    std::vector<int4> seg;
     int64_t v = 0;
     h->h_vptr.assign(nk + 1, 0);
     std::vector<int64_t> perm;
+    std::vector<uint32_t> s2;
+    std::vector<int32_t> q2;
     for (int64_t k = 0; k < nk; k++) {
         h->h_vptr[k] = v;
         const int64_t v0 = v;
-        for (int64_t p = rowptr[k]; p < rowptr[k + 1]; p++, v++) { vslot[v] = fwdpos[p]; vdir[v] = 0; vq[v] = col[p]; }
-        for (int64_t p = revptr[k]; p < revptr[k + 1]; p++, v++) { vslot[v] = revpos[p]; vdir[v] = 1; vq[v] = revcol[p]; }
+        for (int64_t p = rowptr[k]; p < rowptr[k + 1]; p++, v++) { vsd[v] = (uint32_t)(fwdpos[p] << 1); vq[v] = col[p]; }
+        for (int64_t p = revptr[k]; p < revptr[k + 1]; p++, v++) { vsd[v] = (uint32_t)(revpos[p] << 1) | 1u; vq[v] = revcol[p]; }
         // a row's visits by partner, so that a partner-block segment order is contiguous
         perm.resize(v - v0);
         for (int64_t q = 0; q < v - v0; q++) perm[q] = v0 + q;
         std::stable_sort(perm.begin(), perm.end(), [&](int64_t x, int64_t y) { return vq[x] < vq[y]; });
-        std::vector<int64_t> s2(perm.size());
-        std::vector<uint8_t> d2(perm.size());
-        std::vector<int32_t> q2(perm.size());
-        for (size_t q = 0; q < perm.size(); q++) { s2[q] = vslot[perm[q]]; d2[q] = vdir[perm[q]]; q2[q] = vq[perm[q]]; }
-        for (size_t q = 0; q < perm.size(); q++) { vslot[v0 + q] = s2[q]; vdir[v0 + q] = d2[q]; vq[v0 + q] = q2[q]; }
+        s2.resize(perm.size());
+        q2.resize(perm.size());
+        for (size_t q = 0; q < perm.size(); q++) { s2[q] = vsd[perm[q]]; q2[q] = vq[perm[q]]; }
+        for (size_t q = 0; q < perm.size(); q++) { vsd[v0 + q] = s2[q]; vq[v0 + q] = q2[q]; }
     }
     h->h_vptr[nk] = v;
     int rc;
-    dfree(h->d_vrec); dfree(h->d_vq); dfree(h->d_vsd);
-    h->d_vrec = nullptr; h->d_vq = nullptr; h->d_vsd = nullptr;
+    dfree(h->d_vrec); dfree(h->d_vq); dfree(h->d_vsd); dfree(h->d_seg);
+    h->d_vrec = nullptr; h->d_vq = nullptr; h->d_vsd = nullptr; h->d_seg = nullptr;
+    h->nseg = 0;
     const bool c64 = h->precision == DMD_PREC_C64;
-    if (h->nslots >= (1ll << 31)) return fail(DMD_ERR_UNSUPPORTED, "batched: 2^31 or more stored slots");
-    // Visit records (4 KiB per visit) when they fit in device memory: forming
-    // M per visit from the stored P reads half the bytes for BF16 P but costs
-    // more instructions than it saves (BF16, preset 4: 35 against 25 ms per
-    // evaluation at B = 16); without room for the records the pass forms M from
-    // P.  DMD_BATCH_RECORDS=0 forces the record-free pass (tests).
+    // Visit records (4 KiB per visit; 2 KiB as FP16) when they fit in device
+    // memory: forming M per visit from the stored P reads half the bytes for
+    // BF16 P but costs more instructions than it saves (BF16, preset 4: 35
+    // against 25 ms per evaluation at B = 16 with 4 KiB records); without room
+    // for the records the pass forms M from P.  DMD_BATCH_RECORDS=0 forces
+    // the record-free pass (tests).
     const char *renv = std::getenv("DMD_BATCH_RECORDS");
     h->bdirect = renv && renv[0] == '0';
-    std::vector<uint32_t> vsd(nvis);
-    for (int64_t q = 0; q < nvis; q++) vsd[q] = (uint32_t)(vslot[q] << 1) | (vdir[q] ? 1u : 0u);
-    if (!h->bdirect && (rc = dalloc_t(&h->d_vrec, (size_t)nvis * 2048, "visit records"))) {
+    // The BF16 store's records are one FP16 word per entry (2 KiB per visit);
+    // DMD_BATCH_HALF=0 keeps BF16 hi + lo records for it (tests, timing).
+    const char *henv = std::getenv("DMD_BATCH_HALF");
+    const bool want_half = !c64 && !(henv && henv[0] == '0');
+    h->bhalf = false;
+    h->bhalf_g = 1.0;
+    if (!h->bdirect && (rc = dalloc_t(&h->d_vrec, (size_t)nvis * (want_half ? 1024 : 2048), "visit records"))) {
         if (rc != DMD_ERR_NOMEM) return rc;
         (void)hipGetLastError();  // the failed allocation: continue without records
         h->bdirect = true;
     }
-    if ((rc = dalloc_t(&h->d_vq, (size_t)nvis, "visit partners"))) return rc;
+    if (!h->bdirect && want_half) {
+        if ((rc = half_record_factor(h, &h->bhalf_g))) return rc;
+        h->bhalf = true;
+    }
+    if ((rc = dalloc_t(&h->d_vq, (size_t)std::max<int64_t>(nvis, 1), "visit partners"))) return rc;
     if ((rc = dalloc_t(&h->d_vsd, (size_t)std::max<int64_t>(nvis, 1), "visit slots"))) return rc;
-    int64_t *d_vslot = nullptr;
-    uint8_t *d_vdir = nullptr;
-    if (!h->bdirect) {
-        if ((rc = dalloc_t(&d_vslot, (size_t)nvis, "visit slots"))) return rc;
-        if ((rc = dalloc_t(&d_vdir, (size_t)nvis, "visit directions"))) { dfree(d_vslot); return rc; }
-    }
-    hipError_t e = hipSuccess;
-    if (nvis) {
-        e = hipMemcpy(h->d_vq, vq.data(), nvis * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(h->d_vsd, vsd.data(), nvis * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess && !h->bdirect) e = hipMemcpy(d_vslot, vslot.data(), nvis * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess && !h->bdirect) e = hipMemcpy(d_vdir, vdir.data(), nvis, hipMemcpyHostToDevice);
-    }
-
-    if (e == hipSuccess && nvis && !h->bdirect) {
-        if (c64)
-            hipLaunchKernelGGL(k_build_visits<true>, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
-                               (const void *)h->d_P, h->d_pscale, h->d_vrec);
-        else
-            hipLaunchKernelGGL(k_build_visits<false>, dim3((unsigned)nvis), dim3(64), 0, h->st, nvis, d_vslot, d_vdir,
-                               (const void *)h->d_P, h->d_pscale, h->d_vrec);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(h->st);
-    }
-    dfree(d_vslot); dfree(d_vdir);
-    if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("building visit records: ") + hipGetErrorString(e));
+    if ((rc = upload_visits(h, vq, vsd))) return rc;
     h->h_vq = std::move(vq);
     h->bvis_ready = true;
     h->bvis_dirty = false;
@@ -539,6 +700,7 @@ static int build_visits(dmd_scatt *h)
 // wavefronts share the L2 (a block of X at B trajectories is 64 B x B per row).
 static int build_segments(dmd_scatt *h, int seg_len, int pblock)
 {
+    if (seg_len < 1 || seg_len > kSegMax) return fail(DMD_ERR_ARG, "batched: segment length outside 1 .. 256");
     if (h->seg_len == seg_len && h->seg_pblock == pblock && h->d_seg) return DMD_OK;
     std::vector<int4> seg;
     std::vector<int32_t> blk;
@@ -566,7 +728,7 @@ static int build_segments(dmd_scatt *h, int seg_len, int pblock)
     dfree(h->d_seg);
     h->d_seg = nullptr;
     int rc;
-    if ((rc = dalloc_t(&h->d_seg, seg.size(), "segments"))) return rc;
+    if ((rc = dalloc_t(&h->d_seg, std::max<size_t>(seg.size(), 1), "segments"))) return rc;
     if (!seg.empty()) DMD_HIP(hipMemcpy(h->d_seg, seg.data(), seg.size() * sizeof(int4), hipMemcpyHostToDevice));
     h->nseg = (int64_t)seg.size();
     h->seg_len = seg_len;
@@ -606,10 +768,10 @@ static int autotune(dmd_scatt *h, int Bp)
                 if (ms < best) { best = ms; best_seg = sl; best_nb = nb; }
             }
         }
-        // partner blocks that fit the L2 of one XCD (4 MiB), and 4x and 16x that
+        // partner blocks that fit the L2 of one XCD (4 MiB), and 2x .. 16x that
         if (forced_pb < 0) {
             const int p0 = std::max(256, 65536 / Bp);
-            for (int pb : { p0, 4 * p0, 16 * p0 }) {
+            for (int pb : { p0, 2 * p0, 4 * p0, 8 * p0, 16 * p0 }) {
                 if ((rc = build_segments(h, best_seg, pb))) return rc;
                 if ((rc = trial(best_nb, &ms))) return rc;
                 if (ms < best) { best = ms; best_pb = pb; }
@@ -632,9 +794,10 @@ int batched_prepare(dmd_scatt *h, int B)
     if ((rc = build_visits(h))) return rc;
     const int Bp = (B + 31) / 32 * 32;
     if (Bp > h->b_cap) {
-        dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB);
-        h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr;
+        dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB); dfree(h->d_bxmax);
+        h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr; h->d_bxmax = nullptr;
         h->b_cap = 0;
+        if ((rc = dalloc_t(&h->d_bxmax, (size_t)Bp, "batched operand scales"))) return rc;
         if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 32 * Bp, "batched X"))) return rc;
         if ((rc = dalloc_t(&h->d_Fb, (size_t)h->nk * 64 * Bp, "batched row sums"))) return rc;
         if ((rc = dalloc_t(&h->d_rhoSB, (size_t)h->nk * 16 * Bp, "batched rho_S"))) return rc;
@@ -663,12 +826,14 @@ static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
 #undef DMD_BP
         return;
     }
-    if (nb2 == 2)
-        hipLaunchKernelGGL(k_batched<2>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
-                           (const uint16_t *)h->d_xb, h->d_Fb);
-    else
-        hipLaunchKernelGGL(k_batched<1>, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
-                           (const uint16_t *)h->d_xb, h->d_Fb);
+#define DMD_BR(H, N)                                                                                              \
+    hipLaunchKernelGGL((k_batched<H, N>), g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,      \
+                       (const uint16_t *)h->d_xb, h->d_Fb)
+    if (h->bhalf && nb2 == 2) DMD_BR(true, 2);
+    else if (h->bhalf) DMD_BR(true, 1);
+    else if (nb2 == 2) DMD_BR(false, 2);
+    else DMD_BR(false, 1);
+#undef DMD_BR
 }
 
 // ddm of B trajectories (rho: B x nk x 16 device, interaction picture at the
@@ -679,12 +844,18 @@ int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *o
     const int Bp = (B + 31) / 32 * 32;
     if (Bp > h->b_cap) return fail(DMD_ERR_STATE, "batched buffers too small");
     const int64_t nx = (int64_t)Bp * h->nk * 16;
+    const uint32_t *xmax = h->bhalf ? h->d_bxmax : nullptr;
+    if (h->bhalf) {  // the FP16 operand scale of every trajectory
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, h->st, Bp, h->d_bxmax);
+        hipLaunchKernelGGL(k_xmax, dim3(nblk(h->nk, 256), B), dim3(256), 0, h->st, h->nk, rho, h->d_fsplit,
+                           h->d_bxmax);
+    }
     hipLaunchKernelGGL(k_prepare_x, dim3(nblk(nx, 256)), dim3(256), 0, h->st, h->nk, B, Bp, rho, h->d_e, h->d_clk, a,
-                       h->expt, h->d_fsplit, h->d_rhoSB, (uint16_t *)h->d_xb);
+                       h->expt, h->d_fsplit, xmax, h->d_rhoSB, (uint16_t *)h->d_xb);
     if (h->nseg > 0) launch_batched_core(h, Bp, h->batch_nblk);
     const int64_t nout = (int64_t)Bp * h->nk;
-    hipLaunchKernelGGL(k_assemble_b, dim3(nblk(nout, 256)), dim3(256), 0, h->st, h->nk, B, Bp, h->d_Fb, h->d_feq,
-                       h->d_rhoSB, h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac,
+    hipLaunchKernelGGL(k_assemble_b, dim3(nblk(nout, 256)), dim3(256), 0, h->st, h->nk, B, Bp, h->d_Fb, xmax,
+                       h->bhalf_g, h->d_feq, h->d_rhoSB, h->d_e, h->d_clk, a, h->expt, h->schrodinger, h->prefac,
                        h->has_eq ? h->d_eq : nullptr, out);
     DMD_HIP(hipGetLastError());
     return DMD_OK;

@@ -201,7 +201,10 @@ struct dmd_scatt {
     double *d_hacc16 = nullptr;          // nk x 16 packed Hermitian row accumulator (nb = 4)
     bool feq_dirty = true;
     // batched trajectories (BF16, nb = 4): visit records, segments, operands
-    uint16_t *d_vrec = nullptr;          // visits x 2048 BF16: M1 hi | M2 hi | M1 lo | M2 lo
+    uint16_t *d_vrec = nullptr;          // visits x 2048 BF16: M1 hi | M2 hi | M1 lo | M2 lo (bhalf: x 1024 FP16 M1 | M2)
+    bool bhalf = false;                  // BF16 store: FP16 records of M x bhalf_g, FP16 hi/lo operand
+    double bhalf_g = 1.0;                // power-of-two factor that puts every record entry under 2^15
+    uint32_t *d_bxmax = nullptr;         // Bp: bits of max |drho| of every trajectory (bhalf)
     int32_t *d_vq = nullptr;             // partner row of each visit
     uint32_t *d_vsd = nullptr;           // slot << 1 | reverse of each visit
     bool bdirect = false;                // batched pass forms M from the stored P (no visit records)

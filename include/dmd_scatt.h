@@ -248,9 +248,11 @@ int dmd_scatt_eval_noalloc(dmd_scatt *h, double t, const double *rho, double *dd
 /* evolve (evolve_eph.cpp:34-166) for B independent states: rho_B, ddmdt_B
  * are B x nk x nb^2 (one_minus_rho formed internally; no band window).  On
  * a DMD_PREC_C64 or DMD_PREC_BF16 nb = 4 handle the B states share one pass
- * over P on the matrix cores (P and drho as two BF16 terms each, FP32 sums,
- * FP64 equilibrium part and assembly); on other handles it evaluates the
- * states one after another.  No reference counterpart for B > 1. */
+ * over P on the matrix cores (C64: P and drho as two BF16 terms each; BF16:
+ * the folded P entries as one FP16 word, drho as two FP16 terms, each scaled
+ * by a power of two; FP32 sums, FP64 equilibrium part and assembly); on
+ * other handles it evaluates the states one after another.  No reference
+ * counterpart for B > 1. */
 int dmd_scatt_eval_batched(dmd_scatt *h, double t, int B, const double *rho_B, double *ddmdt_B);
 /* The initializer's one-step form over all stored pairs
  * (lindbladInit.../ElectronPhonon.h:774-862 without the file reads): ddm +=
