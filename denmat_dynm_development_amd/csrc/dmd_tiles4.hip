@@ -350,18 +350,15 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
                 cfmac_t(g1[i], xk[j], p2[4 * i + j]);   // Pdm_k'[4a+i]  += rho_k[4b+j] conj(P2[4a+i][4b+j])
                 cfmac_t(g2[j], yk[i], p1[4 * i + j]);   // dm1P_k'[4b+j] += (1-rho_k)[4a+i] conj(P1[4a+i][4b+j])
             }
-        if (pscale) {
+        V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
+        V F2 = reduce4(f2, lane, 8, 4);  // output 4b + a
+        V G1 = reduce4(g1, lane, 2, 1);
+        V G2 = reduce4(g2, lane, 8, 4);
+        if (pscale) {  // a power of two: scaling the reduced sums is exact, 8 products instead of 32
             const CT sc = (CT)pscale[s];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                f1[q].x *= sc; f1[q].y *= sc; f2[q].x *= sc; f2[q].y *= sc;
-                g1[q].x *= sc; g1[q].y *= sc; g2[q].x *= sc; g2[q].y *= sc;
-            }
+            F1.x *= sc; F1.y *= sc; F2.x *= sc; F2.y *= sc;
+            G1.x *= sc; G1.y *= sc; G2.x *= sc; G2.y *= sc;
         }
-        const V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
-        const V F2 = reduce4(f2, lane, 8, 4);  // output 4b + a
-        const V G1 = reduce4(g1, lane, 2, 1);
-        const V G2 = reduce4(g2, lane, 8, 4);
         if (ok) {
             double *r1 = accI + lo1 * RS4;
             atomicAdd(r1 + 2 * (4 * a + b), (double)F1.x);
