@@ -41,10 +41,12 @@
 //    bytes (Bp x 64 bytes per row q).
 //  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
 // Uneven rows: every row's visits are cut into segments of at most seg_len
-// visits, one wavefront each, so a 5000-partner row is as many independent
-// work items; segments of one row meet in FP32 atomics.  seg_len and the
-// trajectory blocks per wavefront are picked by timing at the first batched
-// call for a padded B (autotune).
+// visits, so a 5000-partner row is as many independent work items; segments
+// of one row meet in FP32 atomics.  A persistent pass of resident wavefronts
+// takes the segments from a queue counter, longest first within each block
+// of partner rows.  seg_len, the partner-block size and the trajectory blocks
+// per wavefront are picked by timing at the first batched call for a padded B
+// (autotune).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -250,30 +252,26 @@ __device__ __forceinline__ halfx8 as_h8(uint4 u)
 // and 256).
 constexpr int kSegMax = 256;
 
-// One wavefront per segment; NBLK blocks of 32 trajectories per pass (the
-// grid's y index selects the pass).  BF16 hi + lo records (HALF = false):
-// M_hi x_hi + M_hi x_lo + M_lo x_hi (the M_lo x_lo term is below the 16-bit
-// significands), 6 MFMAs per visit per block; FP16 records (HALF): M x_hi +
-// M x_lo, 4 MFMAs.  Timing variants that staged the segment's partner rows in
-// LDS and kept two visits' records and operands in flight per wavefront ran
-// no faster (the extra registers cost a wave per SIMD), so a visit's loads
-// are issued when it starts.
+// One segment of the record pass (a wavefront): the row sums of its visits
+// over NBLK blocks of 32 trajectories from b0, added into F at its end.  BF16
+// hi + lo records (HALF = false): M_hi x_hi + M_hi x_lo + M_lo x_hi (the
+// M_lo x_lo term is below the 16-bit significands), 6 MFMAs per visit per
+// block; FP16 records (HALF): M x_hi + M x_lo, 4 MFMAs.  Timing variants that
+// staged the segment's partner rows in LDS and kept two visits' records and
+// operands in flight per wavefront ran no faster (the extra registers cost a
+// wave per SIMD), so a visit's loads are issued when it starts.
 template <bool HALF, int NBLK>
-__global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
-                                                 const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
-                                                 const uint16_t *__restrict__ xb, float *__restrict__ F)
+__device__ __forceinline__ void batched_segment(int4 S, int b0, int Bp, const uint16_t *__restrict__ vrec,
+                                                const int32_t *__restrict__ vq, const uint16_t *__restrict__ xb,
+                                                float *__restrict__ F)
 {
     constexpr int NR = HALF ? 2 : 4;            // 16-byte record words per lane
     constexpr int64_t RS = HALF ? 1024 : 2048;  // record stride (16-bit words)
     const int lane = threadIdx.x & 63;
-    const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (sg >= nseg) return;
-    const int4 S = seg[sg];  // (k, v0 lo, v0 hi, count)
-    const int64_t k = S.x;
+    const int64_t k = S.x;  // (k, v0 lo, v0 hi, count)
     const int64_t v0 = (int64_t)(uint32_t)S.y | ((int64_t)S.z << 32);
     const int nv = S.w;
     const int i = lane & 31, h = lane >> 5;  // A row (real-ified), K half
-    const int b0 = blockIdx.y * NBLK * 32;
     floatx16 f1[NBLK], g2[NBLK];
 #pragma unroll
     for (int nb = 0; nb < NBLK; nb++)
@@ -319,6 +317,38 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
             unsafeAtomicAdd(F + (k * 64 + row) * Bp + b, f1[nb][t]);
             unsafeAtomicAdd(F + (k * 64 + 32 + row) * Bp + b, g2[nb][t]);
         }
+    }
+}
+
+// One wavefront per segment (the grid's y index selects the trajectory pass).
+template <bool HALF, int NBLK>
+__global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
+                                                 const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
+                                                 const uint16_t *__restrict__ xb, float *__restrict__ F)
+{
+    const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sg >= nseg) return;
+    batched_segment<HALF, NBLK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
+}
+
+// Persistent form: as many wavefronts as the GPU holds at once, each taking
+// the next segment from a queue counter (one per trajectory pass, zero at
+// launch) until the segments run out, so long and short rows balance over
+// the wavefronts that are resident.  Every wavefront leaves once the counter
+// passes nseg.
+template <bool HALF, int NBLK>
+__global__ __launch_bounds__(256) void k_batched_q(int64_t nseg, int Bp, const int4 *__restrict__ seg,
+                                                   const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
+                                                   const uint16_t *__restrict__ xb, float *__restrict__ F,
+                                                   uint32_t *__restrict__ queue)
+{
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(queue + blockIdx.y, 1u);
+        const int64_t sg = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(t);
+        if (sg >= nseg) return;
+        batched_segment<HALF, NBLK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
     }
 }
 
@@ -737,6 +767,7 @@ static int build_segments(dmd_scatt *h, int seg_len, int pblock)
 }
 
 static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req);
+static void persistent_residency(dmd_scatt *h);
 
 // Autotune: segment length x trajectory blocks per wavefront, by timing the
 // matrix-core pass on the current operands; the row sums are cleared after.
@@ -792,12 +823,14 @@ int batched_prepare(dmd_scatt *h, int B)
     int rc;
     if ((rc = tiles4_prepare(h))) return rc;
     if ((rc = build_visits(h))) return rc;
+    if (!h->bq_resident[0]) persistent_residency(h);
     const int Bp = (B + 31) / 32 * 32;
     if (Bp > h->b_cap) {
-        dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB); dfree(h->d_bxmax);
-        h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr; h->d_bxmax = nullptr;
+        dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB); dfree(h->d_bxmax); dfree(h->d_bqueue);
+        h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr; h->d_bxmax = nullptr; h->d_bqueue = nullptr;
         h->b_cap = 0;
         if ((rc = dalloc_t(&h->d_bxmax, (size_t)Bp, "batched operand scales"))) return rc;
+        if ((rc = dalloc_t(&h->d_bqueue, (size_t)(Bp / 32), "segment queues"))) return rc;
         if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 32 * Bp, "batched X"))) return rc;
         if ((rc = dalloc_t(&h->d_Fb, (size_t)h->nk * 64 * Bp, "batched row sums"))) return rc;
         if ((rc = dalloc_t(&h->d_rhoSB, (size_t)h->nk * 16 * Bp, "batched rho_S"))) return rc;
@@ -826,14 +859,44 @@ static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
 #undef DMD_BP
         return;
     }
-#define DMD_BR(H, N)                                                                                              \
-    hipLaunchKernelGGL((k_batched<H, N>), g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,      \
-                       (const uint16_t *)h->d_xb, h->d_Fb)
-    if (h->bhalf && nb2 == 2) DMD_BR(true, 2);
-    else if (h->bhalf) DMD_BR(true, 1);
-    else if (nb2 == 2) DMD_BR(false, 2);
-    else DMD_BR(false, 1);
-#undef DMD_BR
+    // DMD_BATCH_PERSIST=0: one wavefront per segment instead of the persistent pass
+    const char *penv = std::getenv("DMD_BATCH_PERSIST");
+    const bool persist = !(penv && penv[0] == '0') && h->nseg < (1ll << 31);
+    if (persist) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, h->st, (int)g.y, h->d_bqueue);
+    auto launch = [&](auto kq, auto kg, int resident) {
+        if (!persist || resident <= 0) {
+            hipLaunchKernelGGL(kg, g, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                               (const uint16_t *)h->d_xb, h->d_Fb);
+            return;
+        }
+        const dim3 gq((unsigned)std::min<int64_t>(g.x, resident), g.y);
+        hipLaunchKernelGGL(kq, gq, dim3(256), 0, h->st, h->nseg, Bp, h->d_seg, h->d_vrec, h->d_vq,
+                           (const uint16_t *)h->d_xb, h->d_Fb, h->d_bqueue);
+    };
+    const int *res = h->bq_resident;
+    if (h->bhalf && nb2 == 2) launch(k_batched_q<true, 2>, k_batched<true, 2>, res[3]);
+    else if (h->bhalf) launch(k_batched_q<true, 1>, k_batched<true, 1>, res[2]);
+    else if (nb2 == 2) launch(k_batched_q<false, 2>, k_batched<false, 2>, res[1]);
+    else launch(k_batched_q<false, 1>, k_batched<false, 1>, res[0]);
+}
+
+// Workgroups of each persistent pass instance the GPU holds at once (set up
+// outside any stream capture; 0 leaves that instance on the per-segment grid).
+static void persistent_residency(dmd_scatt *h)
+{
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 0;
+    auto one = [&](auto kq) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kq, 256, 0) != hipSuccess) per_cu = 0;
+        return cus > 0 && per_cu > 0 ? per_cu * cus : 0;
+    };
+    h->bq_resident[0] = one(k_batched_q<false, 1>);
+    h->bq_resident[1] = one(k_batched_q<false, 2>);
+    h->bq_resident[2] = one(k_batched_q<true, 1>);
+    h->bq_resident[3] = one(k_batched_q<true, 2>);
+    (void)hipGetLastError();
 }
 
 // ddm of B trajectories (rho: B x nk x 16 device, interaction picture at the

@@ -905,7 +905,7 @@ dmd_scatt::~dmd_scatt()
     dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows); dfree(d_acc_herm);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
     dfree(d_pscale); dfree(d_fsplit); dfree(d_feq); dfree(d_x32); dfree(d_hacc16);
-    dfree(d_vrec); dfree(d_bxmax); dfree(d_vq); dfree(d_vsd); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
+    dfree(d_vrec); dfree(d_bxmax); dfree(d_bqueue); dfree(d_vq); dfree(d_vsd); dfree(d_seg); dfree(d_xb); dfree(d_Fb); dfree(d_rhoSB); dfree(d_bin); dfree(d_bout);
     dfree(d_yB0); dfree(d_yBa); dfree(d_yBb); dfree(d_accB); dfree(d_wB); dfree(d_adj); dfree(d_dotpart);
     dfree(d_dpos); dfree(d_dk); dfree(d_flin); dfree(d_ad); dfree(d_adpart); dfree(d_phfeq);
     dfree(d_spmask); dfree(d_spgofs); dfree(d_spv);

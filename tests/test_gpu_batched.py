@@ -248,3 +248,38 @@ def test_partner_block_segment_orders_agree(monkeypatch):
             assert rel_err(r[b], ref - ref_f) < 5e-6
     for r in results[1:]:
         assert rel_err(r, results[0]) < 2e-6
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_persistent_pass_matches_the_per_segment_grid(prec, monkeypatch):
+    """The persistent pass (resident wavefronts taking segments from a queue
+    counter, the default) against one wavefront per segment
+    (DMD_BATCH_PERSIST=0): the same products per segment, so the results
+    differ only by the order of the FP32 row-sum atomics; two evaluations in a
+    row check that the queue counters start from zero every time."""
+    scatt, oracle = _mods()
+    print(f"case batched persistent prec={prec}", flush=True)
+    rng = np.random.default_rng(31 + prec)
+    nk, nb, B = 230, 4, 40
+    k1, k2 = _hub_pairs(nk, 3, rng)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(B, nk, nb, f, rng)
+    t = 41 * FS
+    rho_f = np.zeros((nk, nb * nb), np.complex128)
+    rho_f[:, ::nb + 1] = f
+    out = {}
+    for persist in ("1", "0"):
+        monkeypatch.setenv("DMD_BATCH_PERSIST", persist)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=prec) as op:
+            op.upload_P(P1, P2)
+            op.set_split(f)
+            base = op.eval(t, rho_f)
+            first = op.eval_batched(t, rho) - base[None]
+            second = op.eval_batched(t, rho) - base[None]
+            out[persist] = (first, second)
+    (a1, a2), (g1, _) = out["1"], out["0"]
+    err, rep = rel_err(a1, g1), rel_err(a2, a1)
+    print(f"prec={prec}: persistent vs grid {err:.2e}, repeat {rep:.2e}")
+    assert np.linalg.norm(g1) > 0 and err < 1e-6 and rep < 1e-6
