@@ -451,7 +451,13 @@ __global__ __launch_bounds__(256) void k_adj_G(int64_t nslots, const int32_t *__
                                                double2 *__restrict__ Gc)
 {
     constexpr int NB2 = 4, NB4 = 16, EP = 32;
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Workgroups run on XCD w % 8 (the hardware's round-robin), so XCD x
+    // (workgroups x, x + 8, ...) walks the x-th eighth of the slots in order
+    // and its L2 holds the stage rows of the few tiles it is on.  The grid
+    // is a multiple of 8 workgroups, which makes the remap a bijection.
+    const int64_t per = gridDim.x / 8;
+    const int64_t bid = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const int64_t s = bid * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
     const int32_t ka = k1[s];
     if (ka < 0) return;  // padding slot
@@ -733,8 +739,8 @@ extern "C" int dmd_scatt_adjoint_rk4(dmd_scatt *h, double t0, double dt, int64_t
     int pending = 0;  // stages held in rS / uS
     auto fold_G = [&]() -> int {
         if (gwords && pending > 0 && h->nslots > 0)
-            hipLaunchKernelGGL(k_adj_G, dim3(nblk(h->nslots, 256)), dim3(256), 0, h->st, h->nslots, h->d_k1, h->d_k2,
-                               pending, (int64_t)n, rS, uS, Gc);
+            hipLaunchKernelGGL(k_adj_G, dim3((nblk(h->nslots, 256) + 7) / 8 * 8), dim3(256), 0, h->st, h->nslots,
+                               h->d_k1, h->d_k2, pending, (int64_t)n, rS, uS, Gc);
         pending = 0;
         DMD_HIP(hipGetLastError());
         return DMD_OK;
