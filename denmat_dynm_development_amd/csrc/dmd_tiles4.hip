@@ -276,6 +276,7 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
     // per 16-lane group: the pair's reverse sums G1 | G2 and the rows rho_k', 1 - rho_k'
     __shared__ double2 gsc[16][32];
     __shared__ double2 rsc[16][16], r1sc[DM1 ? 16 : 1][16];
+    __shared__ double2 csc[16][16];  // per 16-lane group: the pair's reverse C' elements
     const int c = blockIdx.x;
     const int I = cij[c].x;
     const int64_t I0 = (int64_t)I * R;
@@ -384,7 +385,10 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
             if (DM1) r1sc[gi][l] = rho1S[kp * 16 + l];
             __builtin_amdgcn_wave_barrier();
             const double2 *r1p = DM1 ? r1sc[gi] : nullptr;
-            const double v = herm_pack(c_elem(gsc[gi], rsc[gi], r1p, a, b), c_elem(gsc[gi], rsc[gi], r1p, b, a), a, b);
+            const double2 cab = c_elem(gsc[gi], rsc[gi], r1p, a, b);  // lane (a, b) forms C'_ab once
+            csc[gi][l] = cab;
+            __builtin_amdgcn_wave_barrier();
+            const double v = herm_pack(cab, csc[gi][4 * b + a], a, b);  // C'_ba from lane (b, a)
             __builtin_amdgcn_wave_barrier();  // the group's LDS rows are read before the next pair writes them
             if (rev && v != 0.0) unsafeAtomicAdd(out + kp * 16 + l, v);
         }
