@@ -697,7 +697,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the scattering operator runs on a single MI355X here "
-              "(multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+              "(sharding over several GPUs is a design, not launched by this benchmark)", file=sys.stderr)
         sys.exit(2)
 
     import torch

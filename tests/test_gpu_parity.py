@@ -1,7 +1,7 @@
 """Parity of the HIP scattering operator (through the C-ABI) with the CPU oracle.
 
-Tolerance: FP64, norm-wise relative error <= 1e-12 (the north-star bar for
-ddm); the kernel sums pairs in a different order than the reference loop and
+Tolerance: FP64, norm-wise relative error <= 1e-12 (the parity bar for ddm
+against the reference's CPU compute_ddm); the kernel sums pairs in a different order than the reference loop and
 applies the interaction-picture phase in factorised form, so results agree to
 a few ulp, not bit for bit.
 """
