@@ -280,11 +280,15 @@ def bench_mixed_batched(args, torch, scatt):
                 n_visits = 2 * n_pairs - int(np.count_nonzero(k1 == k2))
                 n_mfma = 4 if (prec == 2 and half) else 6
                 mfma_flop = n_visits * ((B + 31) // 32) * n_mfma * 2 * 32 * 32 * 16
+                # the pass is bound by its record stream: 2 KiB (FP16) or 4 KiB (hi + lo) per visit
+                rec_bytes = n_visits * (2048 if n_mfma == 4 else 4096)
                 per[str(B)] = {"traj_steps_per_s": B * K / el, "ms_per_step": el / K * 1e3,
                                "speedup_vs_fp64": B * K / el / base, "eval_ms": kms,
                                "mfma_tflops": mfma_flop / (kms * 1e-3) / 1e12,
                                "mfma_frac_of_dense_peak": mfma_flop / (kms * 1e-3) / 2.5e15,
                                "records": "fp16" if n_mfma == 4 else "bf16_hi_lo",
+                               "record_GBps": rec_bytes / (kms * 1e-3) / 1e9,
+                               "record_frac_of_hbm_peak": rec_bytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                                "pair_traj_per_s": n_pairs * B / (kms * 1e-3),
                                "tau_s_ps": tau, "tau_rel_err_vs_fp64": abs(tau / tau64 - 1)}
                 log(f"{name} B={B}: {el / K * 1e3:.1f} ms/step, {B * K / el:.1f} traj-steps/s "
