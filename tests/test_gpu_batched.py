@@ -283,3 +283,24 @@ def test_persistent_pass_matches_the_per_segment_grid(prec, monkeypatch):
     err, rep = rel_err(a1, g1), rel_err(a2, a1)
     print(f"prec={prec}: persistent vs grid {err:.2e}, repeat {rep:.2e}")
     assert np.linalg.norm(g1) > 0 and err < 1e-6 and rep < 1e-6
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batched_with_no_stored_pairs(prec):
+    """An operator with no stored pairs: no visits, no segments, an empty
+    queue for the persistent pass; every trajectory's ddm is zero, as the
+    unbatched evaluation's is."""
+    scatt, oracle = _mods()
+    print(f"case batched no pairs prec={prec}", flush=True)
+    rng = np.random.default_rng(3)
+    nk, nb, B = 12, 4, 5
+    e = random_energies(nk, nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(B, nk, nb, f, rng)
+    empty = np.zeros(0, np.uint64)
+    with scatt.ScatteringOperator(nk, nb, empty, empty, e, precision=prec) as op:
+        op.upload_P(np.zeros((0, nb ** 4), np.complex128), np.zeros((0, nb ** 4), np.complex128))
+        op.set_split(f)
+        got = op.eval_batched(13 * FS, rho)
+        one = op.eval(13 * FS, rho[0])
+    assert got.shape == (B, nk, nb * nb) and np.all(got == 0) and np.all(one == 0)
