@@ -1,4 +1,4 @@
-// dmd_batched.hip -- B trajectories sharing one pass over P (nb = 4, BF16 P):
+// dmd_batched.hip -- B trajectories sharing one pass over P (nb = 4, complex64 or BF16 P):
 // the per-pair GEMVs of electronphonon::evolve (src_v4.5.7/evolve_eph.cpp:99-106)
 // become GEMMs over the trajectory index and run on the matrix cores.
 //
@@ -21,8 +21,10 @@
 //
 // FP16 records (BF16 store).  For P already rounded to BF16 (8-bit
 // significands) a second 16-bit word of M is wasted precision: M is kept as
-// one FP16 word (11-bit significand, relative rounding <= 2^-11, 8x below the
-// BF16 store's own 2^-8) with v_mfma_f32_32x32x16_f16, 2 KiB per visit, and x
+// one FP16 word (11-bit significand: a BF16 word, or the sum or difference of
+// two whose exponents differ by at most 2, is exact; otherwise the rounding is
+// <= 2^-11 relative, 8x below the BF16 store's own 2^-8) with
+// v_mfma_f32_32x32x16_f16, 2 KiB per visit, and x
 // as FP16 hi + lo (22-bit significand).  FP16's range is met by two powers of
 // two: M x G with G = 2^14 / max per-pair scale of the store (every entry
 // under 2^15), and x x s_b per trajectory with s_b = 2^14 / max |drho_b|
@@ -36,7 +38,8 @@
 //    32 x 16 BF16 row-major each, hi then lo: [M1 hi | M2 hi | M1 lo | M2 lo],
 //    4 KiB per visit (8 KiB per stored pair), conj and the per-pair scale of
 //    the BF16 store applied; FP16 records [M1 | M2] x G, 2 KiB per visit.
-//  * X: per partner row q, [term][k-half][trajectory][8] BF16, so a
+//  * X: per partner row q, [term][k-half][trajectory][8] BF16 (FP16 with FP16
+//    records; term 0 hi, 1 lo), so a
 //    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
 //    bytes (Bp x 64 bytes per row q).
 //  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
