@@ -473,12 +473,13 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
 // X + X^+ and Y + Y^+ (evolve_eph_linear.cpp:26-37).
 // SP: P held compressed (dmd_scatt_compress_P): per slot a mask of its
 // nonzero elements (bit el < 16: P1 element el, bit 16 + el: P2 element el),
-// per granule of 8 slots the offset of its values, stored rank-major (the
-// first nonzero P1 element of each of the 8 slots that has one, then the
-// second, ..., then P2 alike), so the lanes of one rank read consecutive
-// values.  Each nonzero adds its two GEMV terms directly.
+// per granule of 8 slots the offset of its values, stored element-major (for
+// each element, the values of the granule's slots that have it, in slot
+// order), so the lanes of a granule read one element's values together.  The
+// lane rebuilds its pair's dense P1, P2 in registers (zeros where absent) and
+// runs the dense GEMVs.
 template <bool DM1, bool LIN, bool SP = false>
-__global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
+__global__ __launch_bounds__(256, 2) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint32_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
@@ -573,89 +574,34 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
             // forward Pdm_k = P1 rho_k', dm1P_k = (1-rho_k')^T P2 (f1, f2);
             // reverse Pdm_k' = conj(P2) rho_k, dm1P_k' = (1-rho_k)^T conj(P1) (g1, g2)
             double2 f1[NB2], f2[NB2], g1[NB2], g2[NB2];
+            double2 p1[NB4], p2[NB4];
             if (SP) {
-#pragma unroll
-                for (int i = 0; i < NB2; i++) {
-                    f1[i] = make_double2(0, 0); f2[i] = make_double2(0, 0);
-                    g1[i] = make_double2(0, 0); g2[i] = make_double2(0, 0);
-                }
-                // rank-major values of the granule: for rank j = 0, 1, ... the
-                // j-th nonzero P1 element of every slot that has one, in slot
-                // order, then the same for P2.  A ballot per rank gives each
-                // lane its value's place, so all of a lane's loads are issued
-                // before any is used; its j-th value is the element at the
-                // j-th set bit of its mask and adds to one entry of two GEMVs.
+                // element-major values of the granule: for element el = 0 .. 31
+                // (P1 then P2) the values of the granule's slots that have it,
+                // in slot order.  One ballot per element gives each lane its
+                // place; an element a slot does not have is zero, so the dense
+                // GEMVs below run unchanged.
                 const double2 *gv = P + vofs;
                 const int gsh = lane & 56;
                 const uint32_t below = (1u << (lane & 7)) - 1u;
-                int pos = 0;
+                int base = 0;
 #pragma unroll
-                for (int half = 0; half < 2; half++) {
-                    const uint32_t mh = half ? (mask >> 16) : (mask & 0xffffu);
-                    const int cnt = __popc(mh);
-                    double2 v[NB4];
-                    uint32_t live = 0;  // ranks some lane of the wavefront holds
-#pragma unroll
-                    for (int j = 0; j < NB4; j++) {
-                        const uint64_t bal = __ballot(cnt > j);
-                        const uint32_t gb = (uint32_t)(bal >> gsh) & 0xffu;
-                        v[j] = make_double2(0, 0);
-                        if (bal != 0) {
-                            live |= 1u << j;
-                            if (cnt > j) v[j] = ld_stream(gv + pos + __popc(gb & below));
-                        }
-                        pos += __popc(gb);
-                    }
-                    uint32_t rem = mh;
-#pragma unroll
-                    for (int j = 0; j < NB4; j++) {
-                        if (!((live >> j) & 1u)) continue;  // wavefront-uniform
-                        if (j >= cnt) continue;
-                        const int el = __builtin_ctz(rem);
-                        rem &= rem - 1u;
-                        const int ra = el >> 2, cb = el & 3;  // P[ra, cb]
-                        double wa[NB2], wb[NB2];
-#pragma unroll
-                        for (int t = 0; t < NB2; t++) {
-                            wa[t] = ra == t ? 1.0 : 0.0;
-                            wb[t] = cb == t ? 1.0 : 0.0;
-                        }
-                        double2 x1 = make_double2(0, 0), x2 = make_double2(0, 0);
-                        if (!half) {  // P1[ra, cb]: f1[ra] += P1 rho_k'[cb], g2[cb] += (1-rho_k)[ra] conj(P1)
-#pragma unroll
-                            for (int t = 0; t < NB2; t++) {
-                                const double2 a = rkp[t], c = om_k(t);
-                                x1 = make_double2(fma(wb[t], a.x, x1.x), fma(wb[t], a.y, x1.y));
-                                x2 = make_double2(fma(wa[t], c.x, x2.x), fma(wa[t], c.y, x2.y));
-                            }
-                            const double2 pf = cmul(v[j], x1), pr = cmulc(x2, v[j]);
-#pragma unroll
-                            for (int t = 0; t < NB2; t++) {
-                                f1[t] = make_double2(fma(wa[t], pf.x, f1[t].x), fma(wa[t], pf.y, f1[t].y));
-                                g2[t] = make_double2(fma(wb[t], pr.x, g2[t].x), fma(wb[t], pr.y, g2[t].y));
-                            }
-                        } else {  // P2[ra, cb]: f2[cb] += (1-rho_k')[ra] P2, g1[ra] += rho_k[cb] conj(P2)
-#pragma unroll
-                            for (int t = 0; t < NB2; t++) {
-                                const double2 a = om_kp(t), c = rk[t];
-                                x1 = make_double2(fma(wa[t], a.x, x1.x), fma(wa[t], a.y, x1.y));
-                                x2 = make_double2(fma(wb[t], c.x, x2.x), fma(wb[t], c.y, x2.y));
-                            }
-                            const double2 pf = cmul(x1, v[j]), pr = cmulc(x2, v[j]);
-#pragma unroll
-                            for (int t = 0; t < NB2; t++) {
-                                f2[t] = make_double2(fma(wb[t], pf.x, f2[t].x), fma(wb[t], pf.y, f2[t].y));
-                                g1[t] = make_double2(fma(wa[t], pr.x, g1[t].x), fma(wa[t], pr.y, g1[t].y));
-                            }
-                        }
-                    }
+                for (int el = 0; el < 2 * NB4; el++) {
+                    const bool has = (mask >> el) & 1u;
+                    const uint32_t gb = (uint32_t)(__ballot(has) >> gsh) & 0xffu;
+                    double2 v = make_double2(0, 0);
+                    if (has) v = ld_stream(gv + base + __popc(gb & below));
+                    base += __popc(gb);
+                    if (el < NB4) p1[el] = v;
+                    else p2[el - NB4] = v;
                 }
             } else {
-                double2 p1[NB4], p2[NB4];
 #pragma unroll
                 for (int el = 0; el < NB4; el++) p1[el] = ld_stream(g + el * 8);
 #pragma unroll
                 for (int el = 0; el < NB4; el++) p2[el] = ld_stream(g + (NB4 + el) * 8);
+            }
+            {
                 if (LIN) {
                     double2 x[NB2], y[NB2];
 #pragma unroll
@@ -719,18 +665,6 @@ __global__ __launch_bounds__(256) void k_tiles_soa(int64_t nk, const double2 *__
                         for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
                     }
                 }
-                continue;
-            }
-            double hf[4], hr[4];
-            herm_inc(f1, f2, rk, DM1 ? r1k : nullptr, hf);
-            herm_inc(g1, g2, rkp, DM1 ? r1kp : nullptr, hr);
-            if (ok) {
-#pragma unroll
-                for (int d = 0; d < 4; d++) atomicAdd(&accA[d * R + lo1], hf[d]);
-            }
-            if (rev) {
-#pragma unroll
-                for (int d = 0; d < 4; d++) atomicAdd(&aJ[d * R + lo2], hr[d]);
             }
         }
         __syncthreads();  // block J done

@@ -6,13 +6,13 @@
 // Layout (nb = 2, the tile kernel's slot order and 8-slot granules): per slot
 // a 32-bit mask of its nonzero elements (bit el: P1 element el, bit 16 + el:
 // P2 element el); per granule the offset of its values, which follow
-// rank-major: for j = 0, 1, ... the j-th nonzero P1 element of every slot of
-// the granule that has a j-th one, in slot order, then P2 the same way.  A
-// pair with n nonzeros streams 16 n + 4.5 bytes against 512 dense.  The 8
-// lanes of a granule read rank j together, so one load instruction of a
-// wavefront covers 8 runs of consecutive values (the dense granules' line
-// pattern); k_tiles_soa<.., SP = true> finds each lane's place in rank j with
-// one ballot and adds the value's two GEMV terms.
+// element-major: for el = 0 .. 31 the values of the granule's slots that have
+// element el, in slot order.  A pair with n nonzeros streams 16 n + 4.5 bytes
+// against 512 dense.  The 8 lanes of a granule read one element together, so
+// one load instruction of a wavefront covers 8 runs of consecutive values (the
+// dense granules' line pattern); k_tiles_soa<.., SP = true> finds each lane's
+// place with one ballot per element, rebuilds the pair's dense P in registers
+// and runs the dense GEMVs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,7 +37,7 @@ __global__ void k_sp_mask(int64_t nslots, const double2 *__restrict__ P, uint32_
     mask[s] = m;
 }
 
-// one thread per granule: its values, rank-major per half, from its offset
+// one thread per granule: its values, element-major, from its offset
 __global__ void k_sp_pack(int64_t ngran, const double2 *__restrict__ P, const uint32_t *__restrict__ mask,
                           const uint32_t *__restrict__ gofs, double2 *__restrict__ V)
 {
@@ -45,18 +45,12 @@ __global__ void k_sp_pack(int64_t ngran, const double2 *__restrict__ P, const ui
     if (gi >= ngran) return;
     const double2 *g = P + gi * (8 * 32);
     double2 *o = V + gofs[gi];
+    uint32_t m[8];
+    for (int t = 0; t < 8; t++) m[t] = mask[gi * 8 + t];
     int64_t pos = 0;
-    for (int half = 0; half < 2; half++) {
-        uint32_t rem[8];
-        for (int t = 0; t < 8; t++) rem[t] = half ? (mask[gi * 8 + t] >> 16) : (mask[gi * 8 + t] & 0xffffu);
-        for (int j = 0; j < 16; j++)
-            for (int t = 0; t < 8; t++) {
-                if (!rem[t]) continue;
-                const int el = 16 * half + __builtin_ctz(rem[t]);
-                rem[t] &= rem[t] - 1u;
-                o[pos++] = g[el * 8 + t];
-            }
-    }
+    for (int el = 0; el < 32; el++)
+        for (int t = 0; t < 8; t++)
+            if ((m[t] >> el) & 1u) o[pos++] = g[el * 8 + t];
 }
 
 }  // namespace dmd
