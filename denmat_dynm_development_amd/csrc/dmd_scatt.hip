@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
 // lane rebuilds its pair's dense P1, P2 in registers (zeros where absent) and
 // runs the dense GEMVs.
 template <bool DM1, bool LIN, bool SP = false>
-__global__ __launch_bounds__(256, 2) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
+__global__ __launch_bounds__(256, SP ? 2 : 1) void k_tiles_soa(int64_t nk, const double2 *__restrict__ P,
                                                    const uint32_t *__restrict__ pk,
                                                    const int64_t *__restrict__ cbeg,
                                                    const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
