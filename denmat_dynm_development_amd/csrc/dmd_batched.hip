@@ -909,6 +909,8 @@ int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *o
 {
     const int Bp = (B + 31) / 32 * 32;
     if (Bp > h->b_cap) return fail(DMD_ERR_STATE, "batched buffers too small");
+    if (h->has_dP)  // built from one state at a rebuild: not a term the trajectories share
+        return fail(DMD_ERR_UNSUPPORTED, "the batched forms do not carry the linearize_dPee term");
     const int64_t nx = (int64_t)Bp * h->nk * 16;
     const uint32_t *xmax = h->bhalf ? h->d_bxmax : nullptr;
     if (h->bhalf) {  // the FP16 operand scale of every trajectory

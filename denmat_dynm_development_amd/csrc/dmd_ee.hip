@@ -3,7 +3,9 @@
 // ElecElec_Model.h:185-262) at the occupations f (eeMode = Pee_fixed_at_eq) or
 // at the current density matrix (eeMode = Pee_update, rebuilt by reset_scatt,
 // ElectronPhonon.cpp:15-42), folded into P as add_scatt_contrib("ee") does
-// (:44-113); and the e-e terms of the linearized operator, calc_P3P4 /
+// (:44-113); under alg_linearize_dPee the change dP of the P at f_eq towards
+// the state (calc_A1rho_dP, ElecElec_Model.h:213-217,271-281) and its row term
+// in evolve (evolve_eph.cpp:107-148); and the e-e terms of the linearized operator, calc_P3P4 /
 // calc_P5P6 (ElecElec_Model_linear.cpp:3-111) turned into Lscij / Lscji as
 // add_Lsc_contrib("ee") does (ElectronPhonon_linear.cpp:38-104,183-253).
 //
@@ -53,6 +55,7 @@ struct EeArgs {
     int64_t nk;
     int nw, m0, m1, m2;
     double pfac;             // prefac_gauss / nk_full (x 2 with exchange)
+    int dP;                  // with dm: the change dP (calc_A1rho_dP at f = f_eq) instead of P(dm)
 };
 
 // mesh cell of k_a + sc (k_c - k_b)
@@ -100,6 +103,37 @@ __device__ __forceinline__ double2 ee_hu(const double2 *M, int i, int j)
     if (i < j) return M[i * NB + j];
     if (i > j) return cconj(M[j * NB + i]);
     return make_double2(M[i * NB + i].x, 0.0);
+}
+// The row-subtracted Hermitian read of calc_A1rho_dP (ElecElec_Model.h:271-281):
+// there ddm[i][j] = dm[i][j] - g[i] for every j (row i's occupation taken from
+// each element of row i), and the zhemm that follows reads the upper triangle,
+// so element (i, j) is dm[i][j] - g[i] above the diagonal and its conjugate
+// mirror dm[i][j] - g[j] below (real diagonal).
+template <int NB>
+__device__ __forceinline__ double2 ee_hus(const double2 *M, const double *g, int i, int j)
+{
+    if (i < j) return make_double2(M[i * NB + j].x - g[i], M[i * NB + j].y);
+    if (i > j) return make_double2(M[j * NB + i].x - g[j], -M[j * NB + i].y);
+    return make_double2(M[i * NB + i].x - g[i], 0.0);
+}
+// calc_A1rho_dP: out = fl o (A1 ddmr) + (ddml A1) o fr, fl scaling rows and fr
+// columns, with ddml = L - fl and ddmr = R - fr read as above.
+template <int NB>
+__device__ __forceinline__ void ee_a1rho_dP(const double2 *A1, const double2 *L, const double *fl, const double2 *R,
+                                            const double *fr, double2 *out)
+{
+#pragma unroll
+    for (int x = 0; x < NB; x++)
+#pragma unroll
+        for (int y = 0; y < NB; y++) {
+            double2 s = make_double2(0.0, 0.0), u = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                s = cadd(s, cmul(A1[x * NB + k], ee_hus<NB>(R, fr, k, y)));
+                u = cadd(u, cmul(ee_hus<NB>(L, fl, x, k), A1[k * NB + y]));
+            }
+            out[x * NB + y] = make_double2(fl[x] * s.x + u.x * fr[y], fl[x] * s.y + u.y * fr[y]);
+        }
 }
 // can e_p - e_q + (e3 - e4 over the bands) fall inside (-ethr, ethr)?
 template <int NB>
@@ -195,7 +229,17 @@ __global__ __launch_bounds__(256) void k_ee_P(int64_t n, const uint64_t *__restr
                     b42[xy] = make_double2(m.x * dl, m.y * dl);
                 }
             double2 t1[NB2], t2[NB2];
-            if (a.dm) {  // calc_A1rho (ElecElec_Model.h:266-270): dm1_k3 A1 dm_k4, dm_k3 A1 dm1_k4
+            if (a.dm && a.dP) {  // calc_A1rho_dP (:216-217): (dm1_k3, 1 - f3 | dm_k4, f4), (dm_k3, f3 | dm1_k4, 1 - f4)
+                const double *f3 = a.f + c3 * NB, *f4 = a.f + c4 * NB;
+                double g3[NB], g4[NB];
+#pragma unroll
+                for (int x = 0; x < NB; x++) {
+                    g3[x] = 1 - f3[x];
+                    g4[x] = 1 - f4[x];
+                }
+                ee_a1rho_dP<NB>(b13, a.dm1 + c3 * NB2, g3, a.dm + c4 * NB2, f4, t1);
+                ee_a1rho_dP<NB>(b31, a.dm + c3 * NB2, f3, a.dm1 + c4 * NB2, g4, t2);
+            } else if (a.dm) {  // calc_A1rho (ElecElec_Model.h:266-270): dm1_k3 A1 dm_k4, dm_k3 A1 dm1_k4
                 const double2 *r3 = a.dm + c3 * NB2, *r4 = a.dm + c4 * NB2, *s3 = a.dm1 + c3 * NB2,
                               *s4 = a.dm1 + c4 * NB2;
                 double2 u1[NB2], u2[NB2];
@@ -354,7 +398,15 @@ __global__ __launch_bounds__(256) void k_ee_pair(int64_t n, const uint64_t *__re
                 double2 a1[NB2], t[NB2];
 #pragma unroll
                 for (int xy = 0; xy < NB2; xy++) a1[xy] = lb ? a2[p * NB + q][xy] : mee(p, q, xy / NB, xy % NB);
-                if (L) {
+                if (L && a.dP) {  // calc_A1rho_dP: P1 (dm1_k3, 1 - f3 | dm_k4, f4), P2 (dm_k3, f3 | dm1_k4, 1 - f4)
+                    double fl[NB], fr[NB];
+#pragma unroll
+                    for (int x = 0; x < NB; x++) {
+                        fl[x] = which ? f3[x] : 1 - f3[x];
+                        fr[x] = which ? 1 - f4[x] : f4[x];
+                    }
+                    ee_a1rho_dP<NB>(a1, L, fl, Rm, fr, t);
+                } else if (L) {
                     double2 u[NB2];
 #pragma unroll
                     for (int x = 0; x < NB; x++)
@@ -603,6 +655,90 @@ __global__ __launch_bounds__(256) void k_lsc_ee(int64_t n, const uint64_t *__res
     dLji[idx] = y;
 }
 
+// alg_linearize_dPee's row sums of n stored pairs' dP (evolve_eph.cpp:107-131):
+//   dPdm_k[i12]  += sum_i3 dP1[(i12),(i3 i3)] f_k'[i3]
+//   dm1dP_k[i12] += sum_i3 (1 - f_k'[i3]) dP2[(i3 i3),(i12)]
+// and, for k < k', with P1_next = conj(dP2) and P2_next = conj(dP1),
+//   dPdm_k'[i12]  += sum_i3 conj(dP2[(i12),(i3 i3)]) f_k[i3]
+//   dm1dP_k'[i12] += sum_i3 (1 - f_k[i3]) conj(dP1[(i3 i3),(i12)]).
+// One thread per (pair, i12); acc holds dPdm then dm1dP (nk x nb^2 complex each).
+template <int NB>
+__global__ __launch_bounds__(256) void k_dP_rows(int64_t n, const uint64_t *__restrict__ k1,
+                                                 const uint64_t *__restrict__ k2, const double2 *__restrict__ dP1,
+                                                 const double2 *__restrict__ dP2, const double *__restrict__ f,
+                                                 int64_t nk, double *__restrict__ acc)
+{
+    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * NB2) return;
+    const int64_t fp = idx / NB2;
+    const int i12 = (int)(idx - fp * NB2);
+    const int64_t k = (int64_t)k1[fp], kp = (int64_t)k2[fp];
+    const double2 *D1 = dP1 + fp * NB4, *D2 = dP2 + fp * NB4;
+    double2 a = make_double2(0.0, 0.0), b = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i3 = 0; i3 < NB; i3++) {
+        const int d = i3 * NB + i3;
+        const double fq = f[kp * NB + i3];
+        const double2 x = D1[i12 * NB2 + d], y = D2[d * NB2 + i12];
+        a = make_double2(a.x + x.x * fq, a.y + x.y * fq);
+        b = make_double2(b.x + y.x * (1 - fq), b.y + y.y * (1 - fq));
+    }
+    double *pd = acc + 2 * (k * NB2 + i12), *pm = acc + 2 * ((nk + k) * NB2 + i12);
+    unsafeAtomicAdd(pd, a.x);
+    unsafeAtomicAdd(pd + 1, a.y);
+    unsafeAtomicAdd(pm, b.x);
+    unsafeAtomicAdd(pm + 1, b.y);
+    if (k < kp) {
+        a = make_double2(0.0, 0.0);
+        b = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int i3 = 0; i3 < NB; i3++) {
+            const int d = i3 * NB + i3;
+            const double fq = f[k * NB + i3];
+            const double2 x = D2[i12 * NB2 + d], y = D1[d * NB2 + i12];
+            a = make_double2(a.x + x.x * fq, a.y - x.y * fq);
+            b = make_double2(b.x + y.x * (1 - fq), b.y - y.y * (1 - fq));
+        }
+        pd = acc + 2 * (kp * NB2 + i12);
+        pm = acc + 2 * ((nk + kp) * NB2 + i12);
+        unsafeAtomicAdd(pd, a.x);
+        unsafeAtomicAdd(pd + 1, a.y);
+        unsafeAtomicAdd(pm, b.x);
+        unsafeAtomicAdd(pm + 1, b.y);
+    }
+}
+
+// The row term of evolve_eph.cpp:144-151 without its phase: D_ij = (1 - f_i)
+// dPdm_ij - dm1dP_ij f_j, and H = prefac/2 (D + D^+), the part the assembly's
+// Hermitian symmetrization keeps (cis((e_i - e_j) t) is common to D_ij and
+// conj(D_ji cis((e_j - e_i) t)), so the phase multiplies H afterwards).
+template <int NB>
+__global__ __launch_bounds__(256) void k_dP_herm(int64_t nk, const double *__restrict__ acc,
+                                                 const double *__restrict__ f, double prefac, double2 *__restrict__ H)
+{
+    constexpr int NB2 = NB * NB;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nk) return;
+    const double2 *pd = (const double2 *)acc + k * NB2, *pm = (const double2 *)acc + (nk + k) * NB2;
+    const double *fk = f + k * NB;
+    double2 D[NB2];
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const double2 x = pd[i * NB + j], y = pm[i * NB + j];
+            D[i * NB + j] = make_double2((1 - fk[i]) * x.x - y.x * fk[j], (1 - fk[i]) * x.y - y.y * fk[j]);
+        }
+    const double s = 0.5 * prefac;
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            H[k * NB2 + i * NB + j] =
+                make_double2(s * (D[i * NB + j].x + D[j * NB + i].x), s * (D[i * NB + j].y - D[j * NB + i].y));
+}
+
 namespace {
 
 // Device copies of what every e-e call needs, released together.
@@ -713,8 +849,10 @@ void launch_ee_P(dmd_scatt *h, int64_t n, const uint64_t *d1, const uint64_t *d2
 
 // The fold of scale x P_ee into P of every stored pair (add = 0: set), chunked.
 // vq_pair_host: v(k1 - k2) of every stored pair (or null: a.vq_cell / closed form).
+// rows (device, 2 nk nb^2 complex, zeroed): the chunks' dP go into the
+// alg_linearize_dPee row sums (k_dP_rows) instead, and P is left as it is.
 int ee_fold(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, EeArgs a, const EimpConst &c,
-            const std::vector<double> *vq_pair_host, double scale, int add)
+            const std::vector<double> *vq_pair_host, double scale, int add, double *rows = nullptr)
 {
     const int64_t per = h->nb4;
     const int64_t chunk = std::min<int64_t>(h->np, std::max<int64_t>(1, (64ll << 20) / per));
@@ -746,14 +884,26 @@ int ee_fold(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, EeArgs a, 
         default: launch_ee_P<4>(h, n, d1, d2, a, c, s1, s2); break;
         }
         if ((e = hipGetLastError()) != hipSuccess) break;
-        rc = place_P_dev(h, s1, s2, n, first, scale, add ? 1 : 0);
+        if (rows) {
+            const dim3 g(nblk(n * h->nb2, 256)), b(256);
+            switch (h->nb) {
+            case 1: hipLaunchKernelGGL(k_dP_rows<1>, g, b, 0, h->st, n, d1, d2, s1, s2, a.f, h->nk, rows); break;
+            case 2: hipLaunchKernelGGL(k_dP_rows<2>, g, b, 0, h->st, n, d1, d2, s1, s2, a.f, h->nk, rows); break;
+            case 3: hipLaunchKernelGGL(k_dP_rows<3>, g, b, 0, h->st, n, d1, d2, s1, s2, a.f, h->nk, rows); break;
+            default: hipLaunchKernelGGL(k_dP_rows<4>, g, b, 0, h->st, n, d1, d2, s1, s2, a.f, h->nk, rows); break;
+            }
+            e = hipGetLastError();
+        } else {
+            rc = place_P_dev(h, s1, s2, n, first, scale, add ? 1 : 0);
+        }
         // the next chunk's pair copies come from pageable host memory
-        if (rc == DMD_OK) e = hipStreamSynchronize(h->st);
+        if (rc == DMD_OK && e == hipSuccess) e = hipStreamSynchronize(h->st);
     }
     hipError_t e2 = hipStreamSynchronize(h->st);
     release();
     if (rc == DMD_OK && e == hipSuccess) e = e2;
     if (rc == DMD_OK && e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    if (rows) return rc;
     if (rc) {
         h->has_P = false;
         return rc;
@@ -839,6 +989,89 @@ extern "C" int dmd_scatt_add_ee_model_state(dmd_scatt *h, const uint64_t *k1st, 
     EeArgs a;
     if ((rc = ee_setup(h, U, kvec, f, dm, dm1, m->screening == DMD_SCR_RPA ? vq_cell : nullptr, *m, s, a))) return rc;
     return ee_fold(h, k1st, k2nd, a, ee_const(*m), nullptr, scale, add);
+}
+
+// alg_linearize_dPee: the inputs of the dP build (add_scatt_contrib("ee") with
+// dP1ee, dP2ee, ElectronPhonon.cpp:84; elecelec_model::calc_P with dP,
+// ElecElec_Model.h:185-262,271-281), checked and uploaded.
+static int ee_dP_setup(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U, const double *kvec,
+                       const double *f_eq, const double *dm, const double *dm1, const double *vq_cell,
+                       const dmd_eimp_model *m, const char *who, EeSetup &s, EeArgs &a)
+{
+    int rc;
+    if ((rc = ee_check(h, k1st, k2nd, U, kvec, f_eq, m, who))) return rc;
+    if (!dm || !dm1) return fail(DMD_ERR_ARG, std::string(who) + ": dP needs the state (dm and dm1)");
+    if (m->screening == DMD_SCR_RPA && !vq_cell)
+        return fail(DMD_ERR_ARG, std::string(who) + ": the static RPA needs the per-cell v(q) table (dmd_scatt_ee_vq_cells)");
+    if (h->linear) return fail(DMD_ERR_STATE, std::string(who) + ": alg_linearize_dPee excludes the linearized operator");
+    if ((rc = ee_setup(h, U, kvec, f_eq, dm, dm1, m->screening == DMD_SCR_RPA ? vq_cell : nullptr, *m, s, a))) return rc;
+    a.dP = 1;
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_add_ee_dP(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                                   const double *kvec, const double *f_eq, const double *dm, const double *dm1,
+                                   const double *vq_cell, const dmd_eimp_model *m, double scale, int add)
+{
+    if (h && add && !h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated (add = 0 sets it)");
+    if (h && h->np == 0 && h->nb >= 1) {
+        int rc = ee_check(h, k1st, k2nd, U, kvec, f_eq, m, "dmd_scatt_add_ee_dP");
+        if (rc == DMD_OK) h->has_P = true;
+        return rc;
+    }
+    EeSetup s;
+    EeArgs a;
+    int rc;
+    if ((rc = ee_dP_setup(h, k1st, k2nd, U, kvec, f_eq, dm, dm1, vq_cell, m, "dmd_scatt_add_ee_dP", s, a))) return rc;
+    return ee_fold(h, k1st, k2nd, a, ee_const(*m), nullptr, scale, add);
+}
+
+extern "C" int dmd_scatt_set_ee_dP(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
+                                   const double *kvec, const double *f_eq, const double *dm, const double *dm1,
+                                   const double *vq_cell, const dmd_eimp_model *m)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    if (!dm && !dm1) {
+        h->has_dP = false;
+        return DMD_OK;
+    }
+    EeSetup s;
+    EeArgs a;
+    int rc;
+    if ((rc = ee_dP_setup(h, k1st, k2nd, U, kvec, f_eq, dm, dm1, vq_cell, m, "dmd_scatt_set_ee_dP", s, a))) return rc;
+    const size_t nrow = (size_t)h->nk * h->nb2;
+    if (!h->d_dPH && (rc = dalloc_t(&h->d_dPH, nrow, "linearize_dPee rows"))) return rc;
+    double *acc = nullptr;
+    if ((rc = dalloc_t(&acc, 4 * nrow, "linearize_dPee row sums"))) return rc;
+    hipError_t e = hipMemsetAsync(acc, 0, 4 * nrow * 8, h->st);
+    if (e != hipSuccess) {
+        dfree(acc);
+        return fail(DMD_ERR_HIP, hipGetErrorString(e));
+    }
+    if (h->np > 0) rc = ee_fold(h, k1st, k2nd, a, ee_const(*m), nullptr, 1.0, 1, acc);
+    if (rc == DMD_OK) {
+        const dim3 g(nblk(h->nk, 256)), b(256);
+        switch (h->nb) {
+        case 1: hipLaunchKernelGGL(k_dP_herm<1>, g, b, 0, h->st, h->nk, acc, a.f, h->prefac, h->d_dPH); break;
+        case 2: hipLaunchKernelGGL(k_dP_herm<2>, g, b, 0, h->st, h->nk, acc, a.f, h->prefac, h->d_dPH); break;
+        case 3: hipLaunchKernelGGL(k_dP_herm<3>, g, b, 0, h->st, h->nk, acc, a.f, h->prefac, h->d_dPH); break;
+        default: hipLaunchKernelGGL(k_dP_herm<4>, g, b, 0, h->st, h->nk, acc, a.f, h->prefac, h->d_dPH); break;
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+        if (e != hipSuccess) rc = fail(DMD_ERR_HIP, hipGetErrorString(e));
+    }
+    dfree(acc);
+    h->has_dP = rc == DMD_OK;
+    return rc;
+}
+
+extern "C" int dmd_scatt_get_ee_dP(dmd_scatt *h, double *H)
+{
+    if (!h || !H) return fail(DMD_ERR_ARG, "dmd_scatt_get_ee_dP: null argument");
+    if (!h->has_dP) return fail(DMD_ERR_STATE, "dmd_scatt_get_ee_dP: no linearize_dPee term is set");
+    DMD_HIP(hipMemcpy(H, h->d_dPH, (size_t)h->nk * h->nb2 * 16, hipMemcpyDefault));
+    return DMD_OK;
 }
 
 extern "C" int dmd_scatt_add_Lsc_ee(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,

@@ -253,6 +253,10 @@ struct dmd_scatt {
     double *d_adpart = nullptr;
     int64_t ad_cap = 0;
     double2 *d_rhoS = nullptr, *d_rho1S = nullptr, *d_out = nullptr, *d_eq = nullptr;
+    // alg_linearize_dPee: the Hermitian row term of the e-e P's change, added
+    // to every evaluation times cis((e_i - e_j) t) (dmd_scatt_set_ee_dP)
+    double2 *d_dPH = nullptr;
+    bool has_dP = false;
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
     // driver: coherent H, observable, RK4 buffers (allocated on first use)
@@ -304,6 +308,8 @@ int set_clock(dmd_scatt *h, double t0, double dt, long long step);
 int launch_rotate(dmd_scatt *h, const double2 *in, const double2 *in1, int nbd, int b0, double a, int semic = 0);
 // contraction + Hermitian epilogue on h->d_rhoS (and h->d_rho1S when dm1) into out (nk x nb^2)
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out);
+// adds the alg_linearize_dPee row term at the clock time a into out (nk x nb^2)
+int launch_dP_term(dmd_scatt *h, double a, double2 *out);
 // nb = 2: out = prefac/2 x the packed Hermitian rows of h->d_acc_herm, rotated to
 // the interaction picture at stage a (no ddmdt_eq); clears the accumulator
 int launch_assemble_herm(dmd_scatt *h, double a, double prefac, double2 *out);

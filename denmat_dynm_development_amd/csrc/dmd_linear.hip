@@ -262,5 +262,6 @@ extern "C" int dmd_scatt_set_Lsc(dmd_scatt *h, const double *f_eq)
     if (e != hipSuccess) return fail(DMD_ERR_HIP, std::string("set_Lsc: ") + hipGetErrorString(e));
     h->linear = true;
     h->has_eq = false;  // evolve_linear subtracts no ddmdt_eq
+    h->has_dP = false;  // nor adds a linearize_dPee term (the two modes exclude each other)
     return DMD_OK;
 }

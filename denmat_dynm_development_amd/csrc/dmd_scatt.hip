@@ -468,6 +468,24 @@ __global__ __launch_bounds__(256) void k_assemble_h(int64_t nk, double *__restri
     for (int ij = 0; ij < NB2; ij++) out[k * NB2 + ij] = d[ij];
 }
 
+// alg_linearize_dPee (evolve_eph.cpp:144-151): out[k][ij] += H[k][ij]
+// cis((e_i - e_j) t), H = prefac/2 (D + D^+) the Hermitian part of the row term
+// D_ij = (1 - f_i) dPdm_ij - dm1dP_ij f_j formed when dP was built; the phase
+// is applied in either picture, as the reference does.
+__global__ __launch_bounds__(256) void k_dP_term(int64_t n, int nb, const double2 *__restrict__ H,
+                                                 const double *__restrict__ e, const DClock *__restrict__ clk,
+                                                 double a, double2 *__restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int nb2 = nb * nb;
+    const int64_t k = q / nb2;
+    const int ij = (int)(q - k * nb2), i = ij / nb, j = ij % nb;
+    const double t = clock_t_at(clk, a);
+    const double2 v = i == j ? H[q] : cmul(H[q], ccis((e[k * nb + i] - e[k * nb + j]) * t));
+    out[q] = cadd(out[q], v);
+}
+
 // LIN: the slot holds the linearized operator Lscij | Lscji and rhoS holds
 // drho_S; forward X = Lscij drho_k', reverse Y = Lscji drho_k, increments
 // X + X^+ and Y + Y^+ (evolve_eph_linear.cpp:26-37).
@@ -834,7 +852,7 @@ dmd_scatt::~dmd_scatt()
 {
     dfree(d_e); dfree(d_edm); dfree(d_P); dfree(d_inv); dfree(d_rowptr); dfree(d_revptr); dfree(d_revpos);
     dfree(d_col); dfree(d_revcol); dfree(d_k1); dfree(d_k2); dfree(d_fwdpos); dfree(d_pk); dfree(d_cbeg);
-    dfree(d_clen); dfree(d_cij); dfree(d_gbeg); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq);
+    dfree(d_clen); dfree(d_cij); dfree(d_gbeg); dfree(d_rhoS); dfree(d_rho1S); dfree(d_out); dfree(d_eq); dfree(d_dPH);
     dfree(d_in); dfree(d_in1); dfree(d_outw); dfree(d_clk);
     dfree(d_H); dfree(d_s); dfree(d_f); dfree(d_obs); dfree(d_part); dfree(d_acc_rows); dfree(d_acc_herm);
     dfree(d_y0); dfree(d_ya); dfree(d_yb); dfree(d_acc); dfree(d_w);
@@ -1613,7 +1631,25 @@ int launch_assemble_herm(dmd_scatt *h, double a, double prefac, double2 *out)
     return DMD_OK;
 }
 
+int launch_dP_term(dmd_scatt *h, double a, double2 *out)
+{
+    const int64_t n = h->nk * h->nb2;
+    hipLaunchKernelGGL(k_dP_term, dim3(nblk(n, 256)), dim3(256), 0, h->st, n, h->nb, h->d_dPH, h->d_e, h->d_clk, a,
+                       out);
+    DMD_HIP(hipGetLastError());
+    return DMD_OK;
+}
+
+static int launch_contract_core(dmd_scatt *h, bool dm1, double a, double2 *out);
+
 int launch_contract(dmd_scatt *h, bool dm1, double a, double2 *out)
+{
+    int rc = launch_contract_core(h, dm1, a, out);
+    if (rc == DMD_OK && h->has_dP && !h->linear) rc = launch_dP_term(h, a, out);
+    return rc;
+}
+
+static int launch_contract_core(dmd_scatt *h, bool dm1, double a, double2 *out)
 {
     const int kind = contract_kind(h);
     if (h->linear && kind != DMD_KERNEL_TILES) return launch_linear(h, a, out);
@@ -1679,14 +1715,16 @@ extern "C" int dmd_scatt_compute_ddm(dmd_scatt *h, const double *rho, const doub
 {
     if (!h || !ddm) return fail(DMD_ERR_ARG, "dmd_scatt_compute_ddm: bad arguments");
     int save_expt = h->expt;
-    bool save_eq = h->has_eq;
+    bool save_eq = h->has_eq, save_dP = h->has_dP;
     h->expt = 0;
     h->has_eq = false;
+    h->has_dP = false;
     const size_t n = (size_t)h->nk * h->nb2 * 2;
     std::vector<double> tmp(n);
     int rc = eval_core(h, 0.0, rho, rho1, tmp.data(), false);
     h->expt = save_expt;
     h->has_eq = save_eq;
+    h->has_dP = save_dP;
     if (rc) return rc;
     if (is_device_ptr(ddm)) {
         std::vector<double> cur(n);
@@ -1714,10 +1752,13 @@ extern "C" int dmd_scatt_compute_ddmdt_eq(dmd_scatt *h, const double *f)
             dm[o] = fh[k * h->nb + i];
             dm1[o] = 1 - fh[k * h->nb + i];
         }
-    bool save_eq = h->has_eq;
+    // evolve with compute_eq: no ddmdt_eq and no alg_linearize_dPee term (:107,144)
+    bool save_eq = h->has_eq, save_dP = h->has_dP;
     h->has_eq = false;
+    h->has_dP = false;
     int rc = eval_core(h, 0.0, dm.data(), dm1.data(), out.data(), false);
     h->has_eq = save_eq;
+    h->has_dP = save_dP;
     if (rc) return rc;
     return dmd_scatt_set_ddmdt_eq(h, out.data());
 }

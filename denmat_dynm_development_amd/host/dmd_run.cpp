@@ -68,7 +68,7 @@ struct Settings {
     double scale_Ez = 0;
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
-    bool ddmdteq = false, expt = true, linearize = false;
+    bool ddmdteq = false, expt = true, linearize = false, linearize_dPee = false;
     double scale_eph = 1, scale_ei = 1;
     double t0 = 0, tend = 0, tstep = 0, hmax = 0, hstart = 0, hmin = 0, epsabs = 1e-8;
     std::string ode = "rkf45";
@@ -134,7 +134,7 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     refuse(p, "alg_summode", 1, "the A-matrix (non-summode) form");
     s.semiclassical = p.get("alg_semiclassical", 0) != 0;  // parameters.cpp:101
     s.linearize = p.get("alg_linearize", 0) != 0;
-    refuse(p, "alg_linearize_dPee", 0, "the linearized change of the e-e P inside evolve (linearize_dPee)");
+    s.linearize_dPee = p.get("alg_linearize_dPee", 0) != 0;
     // parameters.cpp:118-120; sparse P: the small elements are zeroed, and (nb = 2,
     // FP64) P is then held by its nonzero elements
     s.Pin_is_sparse = p.get("alg_Pin_is_sparse", 0) != 0;
@@ -288,6 +288,23 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
         throw std::runtime_error("if only e-imp or only e-e, e-ph should not be enabled. Set alg_eph_enable = 0");
     if (s.only_ee && !ee) throw std::runtime_error("alg_only_ee is only possible if eeMode is not none");
     if (s.only_eimp && ee) throw std::runtime_error("if you want only e-i scattering, please set eeMode to be none");
+    if (s.linearize && s.linearize_dPee)  // parameters.cpp:436-443
+        throw std::runtime_error("alg_linearize_dPee and alg.linearize cannot be both true");
+    if (s.linearize_dPee && (s.Pin_is_sparse || s.sparseP))
+        throw std::runtime_error("linearization does not support sparse matrices");
+    if (s.linearize_dPee && !s.ddmdteq) throw std::runtime_error("linearize_dPee must be used ddmdteq");
+    if (!ee && s.linearize_dPee)
+        throw std::runtime_error("alg_linearize_dPee is only active when e-e scattering is active");
+    // set_eph's e-e fold-in (ElectronPhonon.cpp:12) calls calc_P's dP branch with
+    // no state, which reads dm1[ik3] (ElectronPhonon.cpp:84, ElecElec_Model.h:216):
+    // the reference stops there under this flag.  What its code defines past that
+    // point is run when a rebuild from the state replaces that P before the first
+    // step: Pee_update with freq_update_ee_model > 0 (the rebuild at it = 1).
+    if (s.linearize_dPee && s.eeMode != "Pee_update")
+        throw std::runtime_error("alg_linearize_dPee with eeMode = " + s.eeMode +
+                                 ": the reference's only e-e build is set_eph's, which passes calc_P's dP branch "
+                                 "no density matrix (ElectronPhonon.cpp:12,84; ElecElec_Model.h:216); use "
+                                 "eeMode = Pee_update");
     if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee == 0)  // parameters.cpp:483-484
         throw std::runtime_error("want to update Pee but set freq_update_ee_model as 0");
     if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee < 0)
@@ -517,9 +534,9 @@ int run(const std::string &folder, bool dry)
     if (s.ode == "rkf45")
         std::printf("rkf45: ode_hstart = %.10le ode_hmin = %.10le ode_hmax = %.10le ode_epsabs = %lg\n", s.hstart, s.hmin,
                     s.hmax, s.epsabs);
-    std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_linearize = %d alg_expt = %d alg_picture = %s "
-                "alg_scatt = %s\n", s.scale_eph, s.scale_ei, (int)s.ddmdteq, (int)s.linearize, (int)s.expt,
-                s.picture.c_str(), s.scatt.c_str());
+    std::printf("scale_eph = %lg scale_ei = %lg alg_ddmdteq = %d alg_linearize = %d alg_linearize_dPee = %d "
+                "alg_expt = %d alg_picture = %s alg_scatt = %s\n", s.scale_eph, s.scale_ei, (int)s.ddmdteq,
+                (int)s.linearize, (int)s.linearize_dPee, (int)s.expt, s.picture.c_str(), s.scatt.c_str());
     std::printf("Bpert = %.10le %.10le %.10le B = %.10le %.10le %.10le impurities = %zu restart = %d\n", s.Bpert[0],
                 s.Bpert[1], s.Bpert[2], s.B[0], s.B[1], s.B[2], s.ni.size(), (int)s.restart);
     std::printf("alg_sparseP = %d alg_Pin_is_sparse = %d alg_thr_sparseP = %lg\n", (int)s.sparseP, (int)s.Pin_is_sparse,
@@ -728,6 +745,7 @@ int run(const std::string &folder, bool dry)
               "RPA v(q) table");
     }
     if (pee_update) {
+        eph.linearize_dPee = s.linearize_dPee;
         eph.keep_base(true, ee_vq_cell);
         if (s.update_screening && (s.scrFormula == "debye" || s.scrFormula == "Bechstedt")) {
             // coulomb_model::init(dm): n_free from the window's occupations
@@ -771,7 +789,9 @@ int run(const std::string &folder, bool dry)
         dmd_eimp_model m = screened();
         m.degauss = s.degauss_ee;
         m.exchange = s.ee_exchange;
-        const std::vector<double> fee = eph_cols(s.linearize ? feq : fdm);
+        // under alg_linearize_dPee the e-e P is taken at f_eq (calc_A1f with f_eq,
+        // ElecElec_Model.h:213-215), as every rebuild takes it
+        const std::vector<double> fee = eph_cols(s.linearize || s.linearize_dPee ? feq : fdm);
         std::printf("model e-e (%s): degauss_ee = %lg scale_ee = %lg\n", s.eeMode.c_str(), m.degauss, s.scale_ee);
         eph.add_ee_model(mk1.data(), mk2.data(), mU.data(), mkv.data(), fee.data(), m, s.scale_ee, ee_vq_cell);
     }
@@ -811,6 +831,11 @@ int run(const std::string &folder, bool dry)
                 eph.add_Lsc_ee(fw.data(), s.eeMode != "Pee_fixed_at_eq", 1.0, ee_vq_cell);
             }
             std::printf("linearized at mu_e = %.12le mu_h = %.12le\n", mue, muh);
+        } else if (s.ddmdteq && s.linearize_dPee) {
+            // DensityMatrix_Dynamics.h:60-62: neither reset_scatt at dm_eq nor
+            // compute_ddmdt_eq under alg_linearize_dPee (they run again only
+            // for a pump's e-i update, :282-288): ddmdt_eq stays zero
+            std::printf("ddmdt_eq not computed (alg_linearize_dPee): zero\n");
         } else if (s.ddmdteq) {
             eph.compute_ddmdt_eq(rows);
             std::printf("ddmdt_eq at mu_e = %.12le mu_h = %.12le\n", mue, muh);
@@ -996,7 +1021,8 @@ int run(const std::string &folder, bool dry)
         auto update_ee = [&]() {
             if (!pee_update || (it - 1) % s.freq_update_ee != 0) return;
             cplx *rows_dm[1] = {dm.data()};
-            eph.reset_scatt(false, true, rows_dm, nullptr, t, nullptr);
+            double *rows_feq[1] = {feq.data()};
+            eph.reset_scatt(false, true, rows_dm, nullptr, t, rows_feq);
         };
         if (s.ode == "euler") update_ee();
         report_tau(it);

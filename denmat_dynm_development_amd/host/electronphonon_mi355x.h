@@ -280,12 +280,16 @@ public:
     // e-e term takes the window of dm (and dm1; null: 1 - dm) turned from the
     // interaction to the Schroedinger picture as add_scatt_contrib("ee") does
     // (:56-65): the upper triangle times cis((e_j - e_i) t), the lower its
-    // conjugate, dm1 = -dm off the diagonal.
+    // conjugate, dm1 = -dm off the diagonal.  Under linearize_dPee the e-e P
+    // is taken at f_eq_expand (required; cut to the window as :28-31 does) and
+    // its change dP towards the state is built, whose row term every later
+    // evolve adds (calc_P with dP, ElectronPhonon.cpp:84; evolve_eph.cpp:107-148).
     void reset_scatt(bool reset_eimp, bool reset_ee, complex_t **dm_expand, complex_t **dm1_expand, double t,
                      double **f_eq_expand)
     {
-        (void)f_eq_expand;
         if ((!reset_eimp && !reset_ee) || !base_ || (eimp_models_.empty() && !has_ee_)) return;
+        if (linearize_dPee && has_ee_ && !f_eq_expand)
+            throw std::runtime_error("reset_scatt: alg_linearize_dPee needs f_eq");
         const long nk = nk_glob;
         const int nb2 = nb * nb, nbx = nb_expand;
         std::vector<double> dm((size_t)nk * nb2 * 2), dm1((size_t)nk * nb2 * 2), occ((size_t)nk * nb);
@@ -324,7 +328,20 @@ public:
                                            mf_.empty() ? nullptr : mf_.data(), &m, em.scale, 1),
                   "model e-i P of defect " + std::to_string(em.iD + 1));
         }
-        if (has_ee_) {
+        if (has_ee_ && linearize_dPee) {
+            dmd_eimp_model m = ee_m_;
+            if (rescreen) rescreen(occ, m);
+            std::vector<double> feq((size_t)nk * nb);
+            for (long k = 0; k < nk; k++)
+                for (int i = 0; i < nb; i++) feq[(size_t)k * nb + i] = f_eq_expand[0][(size_t)k * nbx + bStart + i];
+            const double *vc = ee_vq_cell_.empty() ? nullptr : ee_vq_cell_.data();
+            check(dmd_scatt_add_ee_model_state(h_, mk1_.data(), mk2_.data(), (const double *)mU_.data(), mkv_.data(),
+                                               feq.data(), nullptr, nullptr, vc, &m, scale_ee_, 1),
+                  "model e-e P at f_eq");
+            check(dmd_scatt_set_ee_dP(h_, mk1_.data(), mk2_.data(), (const double *)mU_.data(), mkv_.data(), feq.data(),
+                                      dm.data(), dm1.data(), vc, &m),
+                  "linearize_dPee: dP of the model e-e P");
+        } else if (has_ee_) {
             dmd_eimp_model m = ee_m_;
             if (rescreen) rescreen(occ, m);
             const bool at_state = update_ee_ && reset_ee;
@@ -335,6 +352,9 @@ public:
         }
         if (Pin_is_sparse || sparse_on_) check(dmd_scatt_threshold_P(h_, thr_sparseP), "sparsification");
     }
+
+    // alg_linearize_dPee (set before the first reset_scatt)
+    bool linearize_dPee = false;
 
     dmd_scatt *handle() const { return h_; }
     const dmd_ldbd_info &info() const { return info_; }

@@ -229,6 +229,43 @@ class ScatteringOperator:
              f.ctypes.data, pdm, pdm1, None if vc is None else vc.ctypes.data, ctypes.byref(model), float(scale),
              int(bool(add)))
 
+    def _dP_state(self, dm, dm1):
+        dm = np.ascontiguousarray(dm, dtype=np.complex128).reshape(self.nk, self.nb * self.nb)
+        if dm1 is None:
+            dm1 = -dm.copy()
+            dm1[:, ::self.nb + 1] += 1.0
+        return dm, np.ascontiguousarray(dm1, dtype=np.complex128).reshape(self.nk, self.nb * self.nb)
+
+    def add_ee_dP(self, k1st, k2nd, U, kvec, f_eq, model, dm, dm1=None, vq_cell=None, scale=1.0, add=True):
+        """P (+)= scale x dP, the alg_linearize_dPee change of the e-e P at f_eq
+        towards the state dm (nk, nb^2, Schroedinger picture; dm1 = 1 - dm by
+        default): the dP1ee, dP2ee tensors (dmd_scatt_add_ee_dP)."""
+        k1, k2, U, kv, f = _ee_args(self, k1st, k2nd, U, kvec, f_eq, model)
+        dm, dm1 = self._dP_state(dm, dm1)
+        vc = None if vq_cell is None else np.ascontiguousarray(vq_cell, dtype=np.float64)
+        call("dmd_scatt_add_ee_dP", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             f.ctypes.data, dm.ctypes.data, dm1.ctypes.data, None if vc is None else vc.ctypes.data,
+             ctypes.byref(model), float(scale), int(bool(add)))
+
+    def set_ee_dP(self, k1st, k2nd, U, kvec, f_eq, model, dm, dm1=None, vq_cell=None):
+        """Builds dP at the state dm and keeps its row term, which every later
+        evaluation adds with its phase (dmd_scatt_set_ee_dP); dm None clears it."""
+        if dm is None:
+            call("dmd_scatt_set_ee_dP", self._h, None, None, None, None, None, None, None, None, None)
+            return
+        k1, k2, U, kv, f = _ee_args(self, k1st, k2nd, U, kvec, f_eq, model)
+        dm, dm1 = self._dP_state(dm, dm1)
+        vc = None if vq_cell is None else np.ascontiguousarray(vq_cell, dtype=np.float64)
+        call("dmd_scatt_set_ee_dP", self._h, k1.ctypes.data, k2.ctypes.data, U.ctypes.data, kv.ctypes.data,
+             f.ctypes.data, dm.ctypes.data, dm1.ctypes.data, None if vc is None else vc.ctypes.data,
+             ctypes.byref(model))
+
+    def get_ee_dP(self):
+        """The kept row term prefac/2 (D + D^+), (nk, nb^2) complex."""
+        out = np.zeros((self.nk, self.nb * self.nb), np.complex128)
+        call("dmd_scatt_get_ee_dP", self._h, out.ctypes.data)
+        return out
+
     def add_Lsc_ee(self, k1st, k2nd, U, kvec, f_eq, model, scale=1.0, extra=0.0, coef34=1.0, dP_terms=True,
                    vq_cell=None):
         """The e-e terms of add_Lsc_contrib("ee") beyond the P1/P2 form (after

@@ -165,6 +165,34 @@ int dmd_scatt_add_ee_model(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k
 int dmd_scatt_add_ee_model_state(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U,
                                  const double *kvec, const double *f, const double *dm, const double *dm1,
                                  const double *vq_cell, const dmd_eimp_model *m, double scale, int add);
+/* alg_linearize_dPee.  The reference's reset_scatt builds, with the e-e P at
+ * f_eq, the change dP1ee, dP2ee of that P towards the state (ElectronPhonon.cpp:
+ * 28-31,84; elecelec_model::calc_P with dP, ElecElec_Model.h:185-262, whose
+ * calc_A1rho_dP, :271-281, takes f_eq from the state's rows: ddm[i][j] =
+ * dm[i][j] - f_eq[i] for every j, read Hermitian from the upper triangle), and
+ * evolve adds (evolve_eph.cpp:107-131,138,144-148)
+ *   ddmdt_contrib[ij] += ((1 - f_eq_i) dPdm_ij - dm1dP_ij f_eq_j) cis((e_i - e_j) t),
+ *   dPdm_k[i12] = sum_k' sum_i3 dP1[(i12),(i3 i3)] f_eq_k'[i3] (and the conjugate
+ *   reverse terms for k < k'), dm1dP_k likewise from dP2 with 1 - f_eq,
+ * unscaled by scale_ee, and not when computing ddmdt_eq.
+ * dmd_scatt_add_ee_dP: P (+)= scale * dP of every stored pair (the dP1ee,
+ * dP2ee tensors themselves).  f_eq: nk x nb occupations of the window; dm, dm1:
+ * the state and 1 - state in the Schroedinger picture (nk x nb^2); vq_cell,
+ * m: as for dmd_scatt_add_ee_model_state. */
+int dmd_scatt_add_ee_dP(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U, const double *kvec,
+                        const double *f_eq, const double *dm, const double *dm1, const double *vq_cell,
+                        const dmd_eimp_model *m, double scale, int add);
+/* dmd_scatt_set_ee_dP: builds dP at the state and keeps its row term on the
+ * handle; every later evaluation (dmd_scatt_eval, evolve_driver, the
+ * integrators) adds prefac/2 (D + D^+)_ij cis((e_i - e_j) t) with D the row term
+ * above; compute_ddmdt_eq and compute_ddm do not.  P is left as it is.  dm =
+ * dm1 = NULL clears the term.  The batched forms and the adjoint refuse a handle
+ * holding it.  dmd_scatt_get_ee_dP: that Hermitian row term prefac/2 (D + D^+)
+ * (nk x nb^2 complex). */
+int dmd_scatt_set_ee_dP(dmd_scatt *h, const uint64_t *k1st, const uint64_t *k2nd, const double *U, const double *kvec,
+                        const double *f_eq, const double *dm, const double *dm1, const double *vq_cell,
+                        const dmd_eimp_model *m);
+int dmd_scatt_get_ee_dP(dmd_scatt *h, double *H);
 /* The static RPA v(q) of every k-mesh cell that is a difference of two k
  * points (coulomb_model's q list over all (ik, jk), Coulomb_Model.h:690-744,
  * 820-828), 0 at q = 0 and elsewhere: vq_cell holds kmesh[0]*kmesh[1]*kmesh[2]

@@ -49,6 +49,13 @@
  *                          balance), P2(k,k') = conj(P1(k',k)), the reshaped
  *                          P1[(i1 i3),(i2 i4)] is Hermitian PSD of rank 1
  *                          (2 with detailed balance).
+ *   oracle_ee_model_P      (with dP, alg_linearize_dPee) dP is the exact
+ *     dP part              derivative (central differences of a quadratic) of
+ *                          P at the state along the row-subtracted directions
+ *                          calc_A1rho_dP forms; for nb = 1, where no row
+ *                          subtraction enters, the derivative along rho - f_eq.
+ *   oracle_ee_dP_term      evolve of dP at diag(f_eq), diag(1 - f_eq) without
+ *                          phases, times cis((e_i - e_j) t) elementwise.
  *   oracle_ee_model_P      the P diagonal is the e-e golden rule
  *                          (1/nk_full) sum_k3 |M|^2 (1-f3) f4 delta(de) with a
  *                          normalised Gaussian, for both scattering forms;
@@ -421,6 +428,64 @@ int oracle_set_Lsc(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const
             for (int i = 0; i < nb4; i++) { a[i] = zadd(a[i], l[i]); b[i] = zadd(b[i], l[i]); }
         }
     free(Lii);
+    return 0;
+}
+
+/* The alg_linearize_dPee part of electronphonon::evolve (compute_eq false),  */
+/* src_v4.5.7/evolve_eph.cpp:107-131 (row sums), 138, 144-151 (row term and  */
+/* the Hermitian assembly): for every stored pair (k, k')                     */
+/*   dPdm[k][i12]  += sum_i3 dP1[(i12),(i3 i3)] f_eq[k'][i3]                  */
+/*   dm1dP[k][i12] += sum_i3 f1_eq[k'][i3] dP2[(i3 i3),(i12)]                 */
+/* and for k < k' the same from P2_next = conj(dP1), P1_next = conj(dP2) into */
+/* row k'; then contrib[ij] = (f1_eq[k][i] dPdm[ij] - dm1dP[ij] f_eq[k][j])   */
+/* cis((e_i - e_j) t) and out[ij] += prefac/2 (contrib[ij] + conj(contrib[ji])).  */
+/* f1_eq = 1 - f_eq (ElectronPhonon.cpp:28-31).  out: nk x nb^2, added to.    */
+int oracle_ee_dP_term(int64_t nk, int nb, int64_t npairs, const uint64_t *k1, const uint64_t *k2, const double *dP1,
+                      const double *dP2, const double *e, const double *f_eq, double t, double prefac, double *out)
+{
+    int nb2 = nb * nb, nb4 = nb2 * nb2;
+    zc *dPdm = (zc *)calloc((size_t)nk * nb2, sizeof(zc)), *dm1dP = (zc *)calloc((size_t)nk * nb2, sizeof(zc));
+    zc *P1n = (zc *)malloc(sizeof(zc) * nb4), *P2n = (zc *)malloc(sizeof(zc) * nb4), *c = (zc *)malloc(sizeof(zc) * nb2);
+    if (!dPdm || !dm1dP || !P1n || !P2n || !c) { free(dPdm); free(dm1dP); free(P1n); free(P2n); free(c); return 1; }
+    for (int64_t p = 0; p < npairs; p++) {
+        int64_t k = (int64_t)k1[p], kp = (int64_t)k2[p];
+        const zc *D1 = (const zc *)dP1 + p * nb4, *D2 = (const zc *)dP2 + p * nb4;
+        for (int i1 = 0; i1 < nb; i1++)
+            for (int i2 = 0; i2 < nb; i2++) {
+                int i12 = i1 * nb + i2, n12 = i12 * nb2;
+                for (int i3 = 0; i3 < nb; i3++) {
+                    dPdm[k * nb2 + i12] = zadd(dPdm[k * nb2 + i12], zscale(D1[n12 + i3 * nb + i3], f_eq[kp * nb + i3]));
+                    dm1dP[k * nb2 + i12] = zadd(dm1dP[k * nb2 + i12],
+                                                zscale(D2[(i3 * nb + i3) * nb2 + i12], 1 - f_eq[kp * nb + i3]));
+                }
+            }
+        if (k < kp) {
+            for (int q = 0; q < nb4; q++) { P2n[q] = zconj(D1[q]); P1n[q] = zconj(D2[q]); }
+            for (int i1 = 0; i1 < nb; i1++)
+                for (int i2 = 0; i2 < nb; i2++) {
+                    int i12 = i1 * nb + i2, n12 = i12 * nb2;
+                    for (int i3 = 0; i3 < nb; i3++) {
+                        dPdm[kp * nb2 + i12] = zadd(dPdm[kp * nb2 + i12], zscale(P1n[n12 + i3 * nb + i3], f_eq[k * nb + i3]));
+                        dm1dP[kp * nb2 + i12] = zadd(dm1dP[kp * nb2 + i12],
+                                                     zscale(P2n[(i3 * nb + i3) * nb2 + i12], 1 - f_eq[k * nb + i3]));
+                    }
+                }
+        }
+    }
+    for (int64_t k = 0; k < nk; k++) {
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) {
+                zc v = zsub(zscale(dPdm[k * nb2 + i * nb + j], 1 - f_eq[k * nb + i]),
+                            zscale(dm1dP[k * nb2 + i * nb + j], f_eq[k * nb + j]));
+                c[i * nb + j] = zmul(v, zcis((e[k * nb + i] - e[k * nb + j]) * t));
+            }
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < nb; j++) {
+                size_t o = (size_t)k * nb2 + i * nb + j;
+                ZW(out, o) = zadd(Z(out, o), zscale(zadd(c[i * nb + j], zconj(c[j * nb + i])), 0.5 * prefac));
+            }
+    }
+    free(dPdm); free(dm1dP); free(P1n); free(P2n); free(c);
     return 0;
 }
 

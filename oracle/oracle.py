@@ -71,6 +71,8 @@ def lib():
                                  ctypes.c_double, _i64, ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_double, ctypes.c_int, _d, _d]
         L.oracle_set_Lsc.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d]
+        L.oracle_ee_dP_term.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, ctypes.c_double,
+                                        ctypes.c_double, _d]
         L.oracle_evolve_linear.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, _d]
         _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -319,6 +321,23 @@ def ee_model_P(k1, k2, U, kvec, e, f, vq_pair, kmesh, degauss, ethr, nk_full, li
     if with_dP:
         return P1, P2, dP1, dP2
     return P1, P2
+
+
+def ee_dP_term(k1, k2, dP1, dP2, e, f_eq, t=0.0, prefac=1.0):
+    """The alg_linearize_dPee part of electronphonon::evolve
+    (src_v4.5.7/evolve_eph.cpp:107-131,138,144-151): the row sums of dP at
+    f_eq, 1 - f_eq, the row term with cis((e_i - e_j) t) and its Hermitian
+    assembly times prefac/2.  Returns (nk, nb^2) complex."""
+    e = _f64(e)
+    nk, nb = e.shape
+    k1 = np.ascontiguousarray(k1, dtype=np.uint64)
+    k2 = np.ascontiguousarray(k2, dtype=np.uint64)
+    out = np.zeros((nk, nb * nb), np.complex128)
+    rc = lib().oracle_ee_dP_term(nk, nb, len(k1), _p(k1), _p(k2), _p(_c128(dP1)), _p(_c128(dP2)), _p(e),
+                                 _p(_f64(f_eq)), float(t), float(prefac), _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle_ee_dP_term failed: {rc}")
+    return out
 
 
 def ee_model_P3P6(k1, k2, U, kvec, e, f, kmesh, degauss, ethr, nk_full, model, lindblad=True, antisym=False,

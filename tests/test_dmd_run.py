@@ -112,13 +112,25 @@ def test_dry_run_accepts_the_reference_eph_ei_ee_run(tmp_path):
     assert 'key "srcFormula" is not read' in out
 
 
+def test_dry_run_accepts_linearize_dPee_with_Pee_update(tmp_path):
+    d = _gaas_folder(tmp_path, "tstep = 1\ntend = 10\nalg_linearize_dPee = 1\nalg_ddmdteq = 1\neeMode = Pee_update\n"
+                               "freq_update_ee_model = 2\nscrMode = medium\n")
+    rc, out, err = _dry(d)
+    assert rc == 0, err
+    assert "alg_linearize_dPee = 1" in out and "eeMode = Pee_update" in out
+
+
 @pytest.mark.parametrize("line,msg", [
     ("eeMode = Pee_update\nscrMode = medium", "want to update Pee but set freq_update_ee_model as 0"),
     ("eeMode = Pee_update\nscrMode = medium\nfreq_update_ee_model = -1", "freq_update_ee_model < 0"),
     ("eeMode = Pee_fixed_at_eq", "scrMode should not be none if considering electron-electron"),
     ("alg_only_ee = 1\nalg_eph_enable = 0\nimpurity_density = 1e17\nimpMode = ab_neutral", "alg_only_ee is only possible"),
     ("eeMode = other", "eeMode must be none or Pee_fixed_at_eq"),
-    ("alg_linearize_dPee = 1", "linearize_dPee"),
+    ("alg_linearize_dPee = 1", "linearize_dPee must be used ddmdteq"),
+    ("alg_linearize_dPee = 1\nalg_ddmdteq = 1", "only active when e-e scattering is active"),
+    ("alg_linearize_dPee = 1\nalg_linearize = 1", "cannot be both true"),
+    ("alg_linearize_dPee = 1\nalg_ddmdteq = 1\nalg_sparseP = 1", "linearization does not support sparse"),
+    ("alg_linearize_dPee = 1\nalg_ddmdteq = 1\neeMode = Pee_fixed_at_eq\nscrMode = medium", "use eeMode = Pee_update"),
     ("impurity_density = 1e17\nimpMode = model_ionized", "scrMode should not be none"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = lindhard", "built for the static RPA"),
     ("impurity_density = 1e17\nscrMode = medium\nscrFormula = heg\ndynamic_screening = model", "built for the static RPA"),

@@ -10,7 +10,10 @@ oracle's run of the same physics:
   (alg_linearize = 1, eeMode = Pee_update, impMode = model_ionized, scrFormula
   at its default, the static RPA): set_Lsc of P_eph + P_imp + P_ee at f_eq
   with the P3..P6 terms in the reference's sign (ElectronPhonon_linear.cpp:
-  3-104), RPA q_scr^2 at the occupations of mu."""
+  3-104), RPA q_scr^2 at the occupations of mu.
+- alg_linearize_dPee with Pee_update: the e-e P at f_eq and the row term of
+  its change dP towards the state added to every right-hand side
+  (evolve_eph.cpp:107-148), ddmdt_eq zero."""
 import os
 import subprocess
 
@@ -142,6 +145,86 @@ def test_dmd_run_pee_update_rebuilds_P_from_the_state(tmp_path):
     P["1"], P["2"] = P1 + scale_ei * M1 + scale_ee * Q1, P2 + scale_ei * M2 + scale_ee * Q2
     fixed = _rk4_run(g, rho, nsteps, 1 * FS, spin)
     assert np.abs(fixed - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
+    """alg_linearize_dPee with eeMode = Pee_update: every rebuild takes the
+    e-e P at f_eq and its change dP towards the state (calc_A1rho_dP), and
+    every right-hand side adds dP's row term with its phase
+    (evolve_eph.cpp:107-148); ddmdt_eq stays zero (DensityMatrix_Dynamics.h:
+    60-62)."""
+    from denmat_dynm_development_amd import scatt
+    from oracle import oracle
+    print("case dmd_run linearize_dPee", flush=True)
+    d, info, U, e, kv, k1, k2, P1, P2, s3, R = _folder(tmp_path, 47)
+    nk, nb = e.shape
+    V = abs(np.linalg.det(R))
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    eps = 4.0
+    km = tuple(info.kmesh)
+    sig, ethr = info.degauss, info.degauss * info.ndegauss
+
+    def model(n_free, n_i=1.0):
+        return scatt.eimp_model("debye", n_free, T, V, scatt.reciprocal_metric(R), 5, n_ionized=n_i,
+                                epsilon_background=eps, degauss=sig, ethr=ethr, kmesh=km,
+                                nk_full=info.nk_full).as_dict()
+
+    def vq_of(m):
+        return np.array([oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)])
+
+    n0 = f.sum() / (info.nk_full * V)
+    E1, _ = oracle.ee_model_P(k1, k2, U, kv, e, f, vq_of(model(n0)), km, sig, ethr, info.nk_full, True)
+    scale_ee = float(f"{np.abs(P1).max() / np.abs(E1).max():.6g}")
+    I1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(n0, 1e-6))
+    n_i = 1e-6 * 0.5 * np.abs(P1).max() / np.abs(I1).max()
+    scale_ei, freq, nsteps = 0.7, 2, 5
+    (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 5\nfreq_measure = 1\n"
+                                "alg_ddmdteq = 1\nalg_linearize_dPee = 1\n"
+                                f"eeMode = Pee_update\nfreq_update_ee_model = {freq}\nscale_ee = {scale_ee}\n"
+                                f"scale_ei = {scale_ei}\nimpurity_density = {n_i / BOHR2CM ** 3:.17g}\n"
+                                "impMode = model_ionized\nscrMode = medium\nscrFormula = debye\n"
+                                f"epsilon_background = {eps}\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-800:])
+    assert r.returncode == 0 and "alg_linearize_dPee = 1" in r.stdout and "ddmdt_eq not computed" in r.stdout
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    target = rho.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real.sum()
+    feq = _fermi(T, _mu_conserving(target, e, T), e)
+    prefac = 2 * np.pi / info.nk_full
+    obs_prefac = 1.0 / (info.nk_full * V * BOHR2CM ** 3)
+    spin = lambda y, t: oracle.spin(s3[:, 2], e, f, y, t, 0, nb, obs_prefac)
+    P = {}
+
+    def rebuild(it, t, y):
+        if (it - 1) % freq:
+            return
+        occ = y.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real
+        m = model(occ.sum() / (info.nk_full * V), n_i)
+        M1, M2 = oracle.eimp_model_P(k1, k2, U, kv, e, m)
+        S = y.reshape(nk, nb, nb).copy()
+        for i in range(nb):
+            for j in range(i + 1, nb):
+                S[:, i, j] *= np.exp(1j * (e[:, j] - e[:, i]) * t)
+                S[:, j, i] = S[:, i, j].conj()
+        S1 = -S.copy()
+        S1[:, np.arange(nb), np.arange(nb)] += 1.0
+        Q1, Q2, D1, D2 = oracle.ee_model_P(k1, k2, U, kv, e, feq, vq_of(m), km, sig, ethr, info.nk_full, True,
+                                           dm=S.reshape(nk, -1), dm1=S1.reshape(nk, -1), with_dP=True)
+        P["1"], P["2"] = P1 + scale_ei * M1 + scale_ee * Q1, P2 + scale_ei * M2 + scale_ee * Q2
+        P["D1"], P["D2"] = D1, D2
+
+    g = lambda t, y: (oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
+                      + oracle.ee_dP_term(k1, k2, P["D1"], P["D2"], e, feq, t=t, prefac=prefac))
+    ref = _rk4_run(g, rho, nsteps, 1 * FS, spin, rebuild)
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", ref)
+    assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
+    # the dP term matters: without it the run differs well beyond that
+    g0 = lambda t, y: oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
+    plain = _rk4_run(g0, rho, nsteps, 1 * FS, spin, rebuild)
+    assert np.abs(plain - ref).max() > 1e-6 * np.abs(ref).max()
 
 
 def test_dmd_run_reference_linearized_eph_ei_ee_keys(tmp_path):

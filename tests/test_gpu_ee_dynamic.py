@@ -159,3 +159,65 @@ def test_linearized_ee_terms_match_the_oracle(nb, exchange):
             got = op.eval(t, rho)
             ref = oracle.evolve_linear(k1, k2, Lij, Lji, e, rho, f, t=t, prefac=prefac)
             assert rel_err(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("nb", NBS)
+def test_ee_dP_and_its_evolve_term_match_the_oracle(nb):
+    """alg_linearize_dPee: the dP tensors (dmd_scatt_add_ee_dP) for Debye, the
+    static RPA and exchange, both forms; the row term kept by
+    dmd_scatt_set_ee_dP; evaluations in both pictures with it added;
+    compute_ddmdt_eq without it; and the term cleared."""
+    scatt, oracle = _mods()
+    rng, km, kv, e, f, U, k1, k2 = _case(nb, 120 + nb, nkeep=20)
+    nk = len(kv)
+    dm = _state(rng, f)
+    dm1 = -dm.copy()
+    dm1[:, ::nb + 1] += 1.0
+    dm1 = dm1 + _state(rng, np.zeros_like(f), 0.02)  # the two sides independent
+    P1, P2 = random_P(len(k1), nb, rng)
+    P1, P2 = 1e-3 * P1, 1e-3 * P2
+    rho = _state(rng, f, 0.02)
+    t = 23 * FS
+    prefac = 2 * np.pi / nk
+    for scr, kind, exchange in (("debye", "lindblad", False), ("RPA", "conventional", False),
+                                ("debye", "lindblad", True)):
+        print(f"case ee dP nb={nb} {scr} {kind} exchange={exchange}", flush=True)
+        m = _model(scatt, scr, U.shape[1], km, kind, exchange=exchange)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+            vq, cells = _tables(scatt, oracle, op, k1, k2, U, kv, e, f, m)
+            _, _, D1, D2 = oracle.ee_model_P(k1, k2, U, kv, e, f, vq, km, 4e-4, 1.2e-3, 36.0, kind == "lindblad",
+                                             antisym_model=m.as_dict() if exchange else None, vq_cell=cells,
+                                             dm=dm, dm1=dm1, with_dP=True)
+            op.add_ee_dP(k1, k2, U, kv, f, m, dm, dm1=dm1, vq_cell=cells, scale=0.6, add=False)
+            A1, A2 = op.download_P()
+            s = np.abs(D1).max()
+            assert s > 0 and np.abs(A1 - 0.6 * D1).max() <= 1e-12 * s and np.abs(A2 - 0.6 * D2).max() <= 1e-12 * s
+            op.upload_P(P1, P2)
+            op.set_ee_dP(k1, k2, U, kv, f, m, dm, dm1=dm1, vq_cell=cells)
+            H = op.get_ee_dP()
+            Href = oracle.ee_dP_term(k1, k2, D1, D2, e, f, t=0.0, prefac=prefac)
+            assert np.abs(H - Href).max() <= 1e-12 * np.abs(Href).max()
+            term = oracle.ee_dP_term(k1, k2, D1, D2, e, f, t=t, prefac=prefac)
+            for schro in (False, True):  # the Schroedinger picture runs with expt = 0 (parameters.cpp:452-453)
+                op.set_flags(expt=not schro, schrodinger=schro)
+                got = op.eval(t, rho)
+                ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=prefac, expt=not schro,
+                                    schrodinger=schro) + term
+                assert rel_err(got, ref) < 1e-12
+            op.set_flags(expt=True, schrodinger=False)
+            # compute_ddmdt_eq is evolve with compute_eq: no dP term in it
+            op.compute_ddmdt_eq(f)
+            eq = oracle.evolve(k1, k2, P1, P2, e, _diag(f), dm1=_diag(1 - f), t=0.0, prefac=prefac)
+            got = op.eval(t, rho)
+            ref = oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=prefac, ddmdt_eq=eq) + term
+            assert rel_err(got, ref) < 1e-12
+            op.set_ddmdt_eq(None)
+            op.set_ee_dP(k1, k2, U, kv, f, m, None)
+            got = op.eval(t, rho)
+            assert rel_err(got, oracle.evolve(k1, k2, P1, P2, e, rho, t=t, prefac=prefac)) < 1e-12
+
+
+def _diag(v):
+    d = np.zeros((v.shape[0], v.shape[1] ** 2), complex)
+    d[:, ::v.shape[1] + 1] = v
+    return d
