@@ -232,14 +232,9 @@ __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t 
             v = herm_pack(make_double2(F[2 * (4 * i + j)], F[2 * (4 * i + j) + 1]),
                           make_double2(F[2 * (4 * j + i)], F[2 * (4 * j + i) + 1]), i, j);
         } else if (row < nk) {
-            double2 rk[16], r1k[16];
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                rk[q] = rhoS[row * 16 + q];
-                if (DM1) r1k[q] = rho1S[row * 16 + q];
-            }
+            // rho of the row read where it lies (an indexed private copy would sit in scratch)
             const double *F = acc + r * RS4;
-            const double2 *r1 = DM1 ? r1k : nullptr;
+            const double2 *rk = rhoS + row * 16, *r1 = DM1 ? rho1S + row * 16 : nullptr;
             v = herm_pack(c_elem(F, rk, r1, i, j), c_elem(F, rk, r1, j, i), i, j);
         }
         __syncthreads();  // every lane of the row has read F before it is cleared
@@ -259,8 +254,10 @@ __device__ __forceinline__ void flush_block4(double *acc, int64_t base, int64_t 
 // raw row sums F1|F2 instead (F_eq).  LIN: the record holds the linearized
 // operator Lscij | Lscji and X is drho_S: forward X_k += Lscij drho_k',
 // reverse Y_k' += Lscji drho_k, increments X + X^+ (evolve_eph_linear.cpp:26-53).
+// FP32 compute: three waves per SIMD (at most 168 VGPRs); the FP64 form keeps
+// its allocation (under that bound it spills).
 template <typename PT, typename CT, bool RAW, bool YNEG, bool DM1, int R, bool LIN = false>
-__global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restrict__ P,
+__global__ __launch_bounds__(256, sizeof(CT) == 4 ? 3 : 1) void k_panels4(int64_t nk, const PT *__restrict__ P,
                                                  const float *__restrict__ pscale, const int32_t *__restrict__ k1s,
                                                  const int32_t *__restrict__ k2s, const int64_t *__restrict__ cbeg,
                                                  const int32_t *__restrict__ clen, const int2 *__restrict__ cij,
@@ -277,6 +274,7 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
     __shared__ double2 gsc[16][32];
     __shared__ double2 rsc[16][16], r1sc[DM1 ? 16 : 1][16];
     __shared__ double2 csc[16][16];  // per 16-lane group: the pair's reverse C' elements
+    __shared__ V xsc[16][16], ysc[DM1 ? 16 : 1][16];  // per 16-lane group: the partner row of X (and Y)
     const int c = blockIdx.x;
     const int I = cij[c].x;
     const int64_t I0 = (int64_t)I * R;
@@ -298,29 +296,17 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
         const int lo1 = ok ? (int)(kk - I0) : 0;
         V p1[16], p2[16];
         Loader<PT, V>::load(P + s * 512, l, p1, p2);
-        // x, y slices: columns 4b.. of x (rows b of the matrices), rows 4a.. of y
-        V xkp[4], ykp[4], xk[4], yk[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xkp[q] = X[kp * 16 + 4 * b + q];
-            xk[q] = xI[lo1 * 16 + 4 * b + q];
-            if (LIN) continue;
-            const V u = X[kp * 16 + 4 * a + q], w = xI[lo1 * 16 + 4 * a + q];
-            if (DM1) {
-                ykp[q] = Y[kp * 16 + 4 * a + q];
-                yk[q] = yI[lo1 * 16 + 4 * a + q];
-            } else if (YNEG) {
-                ykp[q].x = -u.x; ykp[q].y = -u.y;
-                yk[q].x = -w.x; yk[q].y = -w.y;
-            } else {
-                ykp[q].x = (a == q ? 1 : 0) - u.x; ykp[q].y = -u.y;
-                yk[q].x = (a == q ? 1 : 0) - w.x; yk[q].y = -w.y;
-            }
-        }
-        V f1[4], f2[4], g1[4], g2[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) { f1[q] = cz<V>(); f2[q] = cz<V>(); g1[q] = cz<V>(); g2[q] = cz<V>(); }
         if (LIN) {
+            // x slices: columns 4b.. of x (rows b of the matrices)
+            V xkp[4], xk[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                xkp[q] = X[kp * 16 + 4 * b + q];
+                xk[q] = xI[lo1 * 16 + 4 * b + q];
+            }
+            V f1[4], g1[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) { f1[q] = cz<V>(); g1[q] = cz<V>(); }
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -342,19 +328,67 @@ __global__ __launch_bounds__(256) void k_panels4(int64_t nk, const PT *__restric
             if (ok && kk < kp && v != 0.0) unsafeAtomicAdd(out + kp * 16 + l, v);
             continue;
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                cfma_t(f1[i], p1[4 * i + j], xkp[j]);   // Pdm_k[4a+i]   += P1[4a+i][4b+j] rho_k'[4b+j]
-                cfma_t(f2[j], ykp[i], p2[4 * i + j]);   // dm1P_k[4b+j]  += (1-rho_k')[4a+i] P2[4a+i][4b+j]
-                cfmac_t(g1[i], xk[j], p2[4 * i + j]);   // Pdm_k'[4a+i]  += rho_k[4b+j] conj(P2[4a+i][4b+j])
-                cfmac_t(g2[j], yk[i], p1[4 * i + j]);   // dm1P_k'[4b+j] += (1-rho_k)[4a+i] conj(P1[4a+i][4b+j])
+        // the partner row k' staged per group; the GEMVs read x and y from LDS
+        // and run one after another, so few registers beyond P stay live
+        const V xv = X[kp * 16 + l];
+        __builtin_amdgcn_wave_barrier();  // the previous pair's reads of the group's rows come first
+        xsc[gi][l] = xv;
+        if (DM1) ysc[gi][l] = Y[kp * 16 + l];
+        __builtin_amdgcn_wave_barrier();
+        const V *xr = xsc[gi], *xl = xI + lo1 * 16, *yr = DM1 ? ysc[gi] : nullptr, *yl = DM1 ? yI + lo1 * 16 : nullptr;
+        // y = 1 - x of the row (YNEG: -x; DM1: the given Y) at (4a + q)
+        auto yof = [&](const V *x, const V *y, int q) {
+            V r;
+            if (DM1) {
+                r = y[4 * a + q];
+            } else {
+                const V u = x[4 * a + q];
+                r.x = (YNEG ? 0 : (a == q ? 1 : 0)) - u.x;
+                r.y = -u.y;
             }
-        V F1 = reduce4(f1, lane, 2, 1);  // output 4a + b
-        V F2 = reduce4(f2, lane, 8, 4);  // output 4b + a
-        V G1 = reduce4(g1, lane, 2, 1);
-        V G2 = reduce4(g2, lane, 8, 4);
+            return r;
+        };
+        V f[4];
+        // Pdm_k[4a+i] += P1[4a+i][4b+j] rho_k'[4b+j]
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[q] = cz<V>();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const V x = xr[4 * b + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++) cfma_t(f[i], p1[4 * i + j], x);
+        }
+        V F1 = reduce4(f, lane, 2, 1);  // output 4a + b
+        // dm1P_k'[4b+j] += (1-rho_k)[4a+i] conj(P1[4a+i][4b+j])
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[q] = cz<V>();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const V y = yof(xl, yl, i);
+#pragma unroll
+            for (int j = 0; j < 4; j++) cfmac_t(f[j], y, p1[4 * i + j]);
+        }
+        V G2 = reduce4(f, lane, 8, 4);  // output 4b + a
+        // dm1P_k[4b+j] += (1-rho_k')[4a+i] P2[4a+i][4b+j]
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[q] = cz<V>();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const V y = yof(xr, yr, i);
+#pragma unroll
+            for (int j = 0; j < 4; j++) cfma_t(f[j], y, p2[4 * i + j]);
+        }
+        V F2 = reduce4(f, lane, 8, 4);
+        // Pdm_k'[4a+i] += rho_k[4b+j] conj(P2[4a+i][4b+j])
+#pragma unroll
+        for (int q = 0; q < 4; q++) f[q] = cz<V>();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const V x = xl[4 * b + j];
+#pragma unroll
+            for (int i = 0; i < 4; i++) cfmac_t(f[i], x, p2[4 * i + j]);
+        }
+        V G1 = reduce4(f, lane, 2, 1);
         if (pscale) {  // a power of two: scaling the reduced sums is exact, 8 products instead of 32
             const CT sc = (CT)pscale[s];
             F1.x *= sc; F1.y *= sc; F2.x *= sc; F2.y *= sc;
