@@ -276,10 +276,13 @@ def bench_mixed_batched(args, torch, scatt):
                 kms = op.time_eval_batched(0.0, 3)
                 # matrix-core work of one evaluation: every stored pair is two visits (k < k')
                 # or one (k = k'); per visit and 32 trajectories 6 v_mfma_f32_32x32x16_bf16
-                # (hi + lo records) or 4 v_mfma_f32_32x32x16_f16 (FP16 records)
+                # (hi + lo records) or 4 v_mfma_f32_32x32x16_f16 (FP16 records); for B <= 16
+                # the packed pass forms x_hi and x_lo of 16 trajectories in one MFMA: 4 or 2
                 n_visits = 2 * n_pairs - int(np.count_nonzero(k1 == k2))
                 n_mfma = 4 if (prec == 2 and half) else 6
-                mfma_flop = n_visits * ((B + 31) // 32) * n_mfma * 2 * 32 * 32 * 16
+                packed = B <= 16 and os.environ.get("DMD_BATCH_PACK", "1") != "0"
+                n_mfma_run = n_mfma - 2 if packed else n_mfma
+                mfma_flop = n_visits * (1 if packed else (B + 31) // 32) * n_mfma_run * 2 * 32 * 32 * 16
                 # the pass is bound by its record stream: 2 KiB (FP16) or 4 KiB (hi + lo) per visit
                 rec_bytes = n_visits * (2048 if n_mfma == 4 else 4096)
                 per[str(B)] = {"traj_steps_per_s": B * K / el, "ms_per_step": el / K * 1e3,

@@ -43,6 +43,12 @@
 //    wavefront's 32 trajectories of one 8-element operand are 512 contiguous
 //    bytes (Bp x 64 bytes per row q).
 //  * Row sums: FP32 [k][64][B] (F1 re, F1 im, G re, G im).
+// Packed pass (B <= 16 with visit records, Bp = 16): the 32 columns of one
+// MFMA carry x_hi of the 16 trajectories and x_lo, so one MFMA per product and
+// record word forms both terms (2 MFMAs per visit with FP16 records instead of
+// 4 at 32 padded trajectories, 4 instead of 6 with hi + lo records) and the
+// partner operand is half as large; the two column halves meet in a lane
+// exchange before the row-sum atomics.
 // Uneven rows: every row's visits are cut into segments of at most seg_len
 // visits, so a 5000-partner row is as many independent work items; segments
 // of one row meet in FP32 atomics.  A persistent pass of resident wavefronts
@@ -263,7 +269,10 @@ constexpr int kSegMax = 256;
 // staged the segment's partner rows in LDS and kept two visits' records and
 // operands in flight per wavefront ran no faster (the extra registers cost a
 // wave per SIMD), so a visit's loads are issued when it starts.
-template <bool HALF, int NBLK>
+// PK (B <= 16, Bp = 16, NBLK = 1): the 32 MFMA columns carry x_hi of the 16
+// trajectories (columns 0-15) and x_lo (16-31), so one MFMA per product and
+// record word forms both terms; the two column halves are added at the flush.
+template <bool HALF, int NBLK, bool PK = false>
 __device__ __forceinline__ void batched_segment(int4 S, int b0, int Bp, const uint16_t *__restrict__ vrec,
                                                 const int32_t *__restrict__ vq, const uint16_t *__restrict__ xb,
                                                 float *__restrict__ F)
@@ -287,6 +296,22 @@ __device__ __forceinline__ void batched_segment(int4 S, int b0, int Bp, const ui
 #pragma unroll
         for (int w = 0; w < NR; w++) m[w] = ld_stream_u4(rb + u * RS + w * 512);
         const uint16_t *xq = xb + (int64_t)vq[v0 + u] * 4 * (int64_t)Bp * 8;
+        if (PK) {
+            const uint4 xv = *(const uint4 *)(xq + (((i >> 4) * 2 + h) * (int64_t)Bp + (i & 15)) * 8);
+            if (HALF) {
+                const halfx8 M1 = as_h8(m[0]), M2 = as_h8(m[1]), a = as_h8(xv);
+                f1[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M1, a, f1[0], 0, 0, 0);
+                g2[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(M2, a, g2[0], 0, 0, 0);
+            } else {  // M_hi (x_hi | x_lo) + M_lo (x_hi | 0)
+                const bf16x8 M1h = as_bf8(m[0]), M2h = as_bf8(m[1]), M1l = as_bf8(m[2]), M2l = as_bf8(m[3]);
+                const bf16x8 a = as_bf8(xv), az = as_bf8(i < 16 ? xv : make_uint4(0u, 0u, 0u, 0u));
+                f1[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1h, a, f1[0], 0, 0, 0);
+                g2[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2h, a, g2[0], 0, 0, 0);
+                f1[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M1l, az, f1[0], 0, 0, 0);
+                g2[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M2l, az, g2[0], 0, 0, 0);
+            }
+            continue;
+        }
 #pragma unroll
         for (int nb = 0; nb < NBLK; nb++) {
             const int b = b0 + nb * 32 + i;
@@ -311,6 +336,18 @@ __device__ __forceinline__ void batched_segment(int4 S, int b0, int Bp, const ui
         }
     }
     // D layout: element t of lane l is row 8 (t / 4) + 4 h + t % 4, column (trajectory) l % 32
+    if (PK) {  // column i + 16 (x_lo) of the same row sits in lane l ^ 16
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int row = 8 * (t >> 2) + 4 * h + (t & 3);
+            const float a = f1[0][t] + __shfl_xor(f1[0][t], 16), c = g2[0][t] + __shfl_xor(g2[0][t], 16);
+            if (i < 16) {
+                unsafeAtomicAdd(F + (k * 64 + row) * Bp + i, a);
+                unsafeAtomicAdd(F + (k * 64 + 32 + row) * Bp + i, c);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int nb = 0; nb < NBLK; nb++) {
         const int b = b0 + nb * 32 + i;
@@ -324,14 +361,14 @@ __device__ __forceinline__ void batched_segment(int4 S, int b0, int Bp, const ui
 }
 
 // One wavefront per segment (the grid's y index selects the trajectory pass).
-template <bool HALF, int NBLK>
+template <bool HALF, int NBLK, bool PK = false>
 __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int4 *__restrict__ seg,
                                                  const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
                                                  const uint16_t *__restrict__ xb, float *__restrict__ F)
 {
     const int64_t sg = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sg >= nseg) return;
-    batched_segment<HALF, NBLK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
+    batched_segment<HALF, NBLK, PK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
 }
 
 // Persistent form: as many wavefronts as the GPU holds at once, each taking
@@ -339,7 +376,7 @@ __global__ __launch_bounds__(256) void k_batched(int64_t nseg, int Bp, const int
 // launch) until the segments run out, so long and short rows balance over
 // the wavefronts that are resident.  Every wavefront leaves once the counter
 // passes nseg.
-template <bool HALF, int NBLK>
+template <bool HALF, int NBLK, bool PK = false>
 __global__ __launch_bounds__(256) void k_batched_q(int64_t nseg, int Bp, const int4 *__restrict__ seg,
                                                    const uint16_t *__restrict__ vrec, const int32_t *__restrict__ vq,
                                                    const uint16_t *__restrict__ xb, float *__restrict__ F,
@@ -351,7 +388,7 @@ __global__ __launch_bounds__(256) void k_batched_q(int64_t nseg, int Bp, const i
         if (lane == 0) t = atomicAdd(queue + blockIdx.y, 1u);
         const int64_t sg = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane(t);
         if (sg >= nseg) return;
-        batched_segment<HALF, NBLK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
+        batched_segment<HALF, NBLK, PK>(seg[sg], blockIdx.y * NBLK * 32, Bp, vrec, vq, xb, F);
     }
 }
 
@@ -821,19 +858,28 @@ static int autotune(dmd_scatt *h, int Bp)
     return DMD_OK;
 }
 
+// The padded trajectory count: 16 (the packed pass) for B <= 16 with visit
+// records, else B rounded up to 32 (DMD_BATCH_PACK=0: always 32-wide).
+static int padded_B(const dmd_scatt *h, int B)
+{
+    const char *env = std::getenv("DMD_BATCH_PACK");
+    if (B <= 16 && !h->bdirect && !(env && env[0] == '0')) return 16;
+    return (B + 31) / 32 * 32;
+}
+
 int batched_prepare(dmd_scatt *h, int B)
 {
     int rc;
     if ((rc = tiles4_prepare(h))) return rc;
     if ((rc = build_visits(h))) return rc;
     if (!h->bq_resident[0]) persistent_residency(h);
-    const int Bp = (B + 31) / 32 * 32;
+    const int Bp = padded_B(h, B);
     if (Bp > h->b_cap) {
         dfree(h->d_xb); dfree(h->d_Fb); dfree(h->d_rhoSB); dfree(h->d_bxmax); dfree(h->d_bqueue);
         h->d_xb = nullptr; h->d_Fb = nullptr; h->d_rhoSB = nullptr; h->d_bxmax = nullptr; h->d_bqueue = nullptr;
         h->b_cap = 0;
         if ((rc = dalloc_t(&h->d_bxmax, (size_t)Bp, "batched operand scales"))) return rc;
-        if ((rc = dalloc_t(&h->d_bqueue, (size_t)(Bp / 32), "segment queues"))) return rc;
+        if ((rc = dalloc_t(&h->d_bqueue, (size_t)std::max(1, Bp / 32), "segment queues"))) return rc;
         if ((rc = dalloc_t(&h->d_xb, (size_t)h->nk * 32 * Bp, "batched X"))) return rc;
         if ((rc = dalloc_t(&h->d_Fb, (size_t)h->nk * 64 * Bp, "batched row sums"))) return rc;
         if ((rc = dalloc_t(&h->d_rhoSB, (size_t)h->nk * 16 * Bp, "batched rho_S"))) return rc;
@@ -848,8 +894,9 @@ int batched_prepare(dmd_scatt *h, int B)
 // the matrix-core pass alone (X prepared, F accumulated)
 static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
 {
-    const int nb2 = (Bp / 32) % 2 == 0 && nblk_req == 2 ? 2 : 1;
-    dim3 g(nblk(h->nseg, 4), Bp / (32 * nb2));
+    const bool pk = Bp == 16;
+    const int nb2 = !pk && (Bp / 32) % 2 == 0 && nblk_req == 2 ? 2 : 1;
+    dim3 g(nblk(h->nseg, 4), std::max(1, Bp / (32 * nb2)));
     if (h->bdirect) {
         const bool c64 = h->precision == DMD_PREC_C64;
 #define DMD_BP(C, N)                                                                                              \
@@ -877,7 +924,9 @@ static void launch_batched_core(dmd_scatt *h, int Bp, int nblk_req)
                            (const uint16_t *)h->d_xb, h->d_Fb, h->d_bqueue);
     };
     const int *res = h->bq_resident;
-    if (h->bhalf && nb2 == 2) launch(k_batched_q<true, 2>, k_batched<true, 2>, res[3]);
+    if (pk && h->bhalf) launch(k_batched_q<true, 1, true>, k_batched<true, 1, true>, res[5]);
+    else if (pk) launch(k_batched_q<false, 1, true>, k_batched<false, 1, true>, res[4]);
+    else if (h->bhalf && nb2 == 2) launch(k_batched_q<true, 2>, k_batched<true, 2>, res[3]);
     else if (h->bhalf) launch(k_batched_q<true, 1>, k_batched<true, 1>, res[2]);
     else if (nb2 == 2) launch(k_batched_q<false, 2>, k_batched<false, 2>, res[1]);
     else launch(k_batched_q<false, 1>, k_batched<false, 1>, res[0]);
@@ -899,6 +948,8 @@ static void persistent_residency(dmd_scatt *h)
     h->bq_resident[1] = one(k_batched_q<false, 2>);
     h->bq_resident[2] = one(k_batched_q<true, 1>);
     h->bq_resident[3] = one(k_batched_q<true, 2>);
+    h->bq_resident[4] = one(k_batched_q<false, 1, true>);
+    h->bq_resident[5] = one(k_batched_q<true, 1, true>);
     (void)hipGetLastError();
 }
 
@@ -907,7 +958,7 @@ static void persistent_residency(dmd_scatt *h)
 // for the padded B the last batched_prepare saw.
 int launch_batched(dmd_scatt *h, int B, const double2 *rho, double a, double2 *out)
 {
-    const int Bp = (B + 31) / 32 * 32;
+    const int Bp = padded_B(h, B);
     if (Bp > h->b_cap) return fail(DMD_ERR_STATE, "batched buffers too small");
     if (h->has_dP)  // built from one state at a rebuild: not a term the trajectories share
         return fail(DMD_ERR_UNSUPPORTED, "the batched forms do not carry the linearize_dPee term");

@@ -206,7 +206,7 @@ struct dmd_scatt {
     double bhalf_g = 1.0;                // power-of-two factor that puts every record entry under 2^15
     uint32_t *d_bxmax = nullptr;         // Bp: bits of max |drho| of every trajectory (bhalf)
     uint32_t *d_bqueue = nullptr;        // Bp / 32: segment queue counters of the persistent pass
-    int bq_resident[4] = { 0, 0, 0, 0 }; // workgroups resident at once, persistent pass (HALF, NBLK = 2)
+    int bq_resident[6] = { 0, 0, 0, 0, 0, 0 }; // workgroups resident at once, persistent pass instances
     int32_t *d_vq = nullptr;             // partner row of each visit
     uint32_t *d_vsd = nullptr;           // slot << 1 | reverse of each visit
     bool bdirect = false;                // batched pass forms M from the stored P (no visit records)

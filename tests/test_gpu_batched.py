@@ -285,6 +285,40 @@ def test_persistent_pass_matches_the_per_segment_grid(prec, monkeypatch):
     assert np.linalg.norm(g1) > 0 and err < 1e-6 and rep < 1e-6
 
 
+@pytest.mark.parametrize("prec,half", [(1, "1"), (2, "1"), (2, "0")])
+def test_packed_pass_matches_the_32_wide_pass(prec, half, monkeypatch):
+    """B <= 16: the packed pass (x_hi and x_lo of 16 trajectories in the 32
+    MFMA columns, halves added at the flush) against the 32-wide pass
+    (DMD_BATCH_PACK=0) on the same stored P, for FP16 and hi + lo records; B = 16
+    and B = 5 on one handle (buffers sized for 16 serve 5), then B = 40 (the
+    32-wide pass) after them."""
+    scatt, oracle = _mods()
+    print(f"case batched packed prec={prec} half={half}", flush=True)
+    monkeypatch.setenv("DMD_BATCH_HALF", half)
+    rng = np.random.default_rng(61 + prec)
+    nk, nb = 230, 4
+    k1, k2 = _hub_pairs(nk, 3, rng)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    f = np.sort(rng.uniform(0.05, 0.6, (nk, nb)), axis=1)[:, ::-1].copy()
+    rho = _states(40, nk, nb, f, rng)
+    t = 41 * FS
+    rho_f = np.zeros((nk, nb * nb), np.complex128)
+    rho_f[:, ::nb + 1] = f
+    out = {}
+    for pack in ("1", "0"):
+        monkeypatch.setenv("DMD_BATCH_PACK", pack)
+        with scatt.ScatteringOperator(nk, nb, k1, k2, e, precision=prec) as op:
+            op.upload_P(P1, P2)
+            op.set_split(f)
+            base = op.eval(t, rho_f)
+            out[pack] = [op.eval_batched(t, rho[:B]) - base[None] for B in (16, 5, 40)]
+    for B, a, g in zip((16, 5, 40), out["1"], out["0"]):
+        err = rel_err(a, g)
+        print(f"prec={prec} half={half} B={B}: packed vs 32-wide {err:.2e}")
+        assert np.linalg.norm(g) > 0 and err < 1e-6
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_batched_with_no_stored_pairs(prec):
     """An operator with no stored pairs: no visits, no segments, an empty
