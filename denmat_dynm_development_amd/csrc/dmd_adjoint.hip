@@ -524,9 +524,9 @@ static int vjp_ready(dmd_scatt *h)
 {
     if (h->nb > 2) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is built for nb <= 2");
     if (h->linear) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is built for the P form, not the linearized operator");
-    if (h->nbd != h->nb || h->has_H || h->has_eq || h->has_dP || h->schrodinger)
+    if (h->nbd != h->nb || h->has_H || h->has_eq || h->has_dP || h->rhs_hook || h->schrodinger)
         return fail(DMD_ERR_UNSUPPORTED, "the adjoint covers the scattering term alone (no window, coherent term, "
-                                         "ddmdt_eq or linearize_dPee term)");
+                                         "ddmdt_eq, linearize_dPee term or right-hand-side hook)");
     if (h->precision != DMD_PREC_FP64) return fail(DMD_ERR_UNSUPPORTED, "the adjoint is FP64");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     // nb = 2 with tiles: the one-pass tile VJP on the forward kernel's layout;

@@ -275,8 +275,11 @@ static int launch_obs(dmd_scatt *h, const double2 *y, double *out, int idx)
     return DMD_OK;
 }
 
-// one RK4 step on the stream (captured into a graph by the caller)
-static int launch_step(dmd_scatt *h, const StageArgs &A)
+static int run_rhs_hook(dmd_scatt *h, double t, const double2 *y);
+
+// one RK4 step on the stream (captured into a graph by the caller); with a
+// right-hand-side hook (not captured) it runs before every stage at t0 + (it + a) dt
+static int launch_step(dmd_scatt *h, const StageArgs &A, int64_t it = 0, double t0 = 0.0, double dt = 0.0)
 {
     const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
     dim3 g(nblk(n, 256)), b(256);
@@ -285,6 +288,7 @@ static int launch_step(dmd_scatt *h, const StageArgs &A)
     double2 *yout[3] = { h->d_ya, h->d_yb, h->d_ya };
     int rc;
     for (int s = 0; s < 4; s++) {
+        if (h->rhs_hook && (rc = run_rhs_hook(h, t0 + ((double)it + a_s[s]) * dt, yin[s]))) return rc;
         if ((rc = launch_contract(h, false, a_s[s], h->d_w))) return rc;
         if (s < 3)
             hipLaunchKernelGGL(k_rk_stage, g, b, 0, h->st, A, s, yin[s], h->d_w, h->d_y0, h->d_acc, yout[s], h->d_rhoS);
@@ -426,7 +430,7 @@ extern "C" int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64
     int iobs = 0;
     if (want_obs && (rc = launch_obs(h, h->d_y0, d_obs, iobs++))) return rc;
     const char *env = std::getenv("DMD_GRAPH");
-    const bool use_graph = !(env && env[0] == '0') && n_steps > 1;
+    const bool use_graph = !(env && env[0] == '0') && n_steps > 1 && !h->rhs_hook;
     hipGraphExec_t exec = nullptr;
     if (use_graph) {
         hipGraph_t graph = nullptr;
@@ -443,7 +447,7 @@ extern "C" int dmd_scatt_integrate_rk4(dmd_scatt *h, double t0, double dt, int64
         if (exec) {
             hipError_t e = hipGraphLaunch(exec, h->st);
             if (e != hipSuccess) { (void)hipGraphExecDestroy(exec); return fail(DMD_ERR_HIP, hipGetErrorString(e)); }
-        } else if ((rc = launch_step(h, A))) {
+        } else if ((rc = launch_step(h, A, it, t0, dt))) {
             return rc;
         }
         if (want_obs && (it + 1) % report_every == 0 && (rc = launch_obs(h, h->d_y0, d_obs, iobs++))) {
@@ -567,6 +571,7 @@ extern "C" int dmd_scatt_integrate_rk4_batched(dmd_scatt *h, int B, double t0, d
                                                int report_every, double *rho_B, double *obs_B)
 {
     if (!h || B < 1 || !rho_B || n_steps < 0) return fail(DMD_ERR_ARG, "dmd_scatt_integrate_rk4_batched: bad arguments");
+    if (h->rhs_hook) return fail(DMD_ERR_UNSUPPORTED, "a right-hand-side hook acts on one trajectory's state");
     if (!h->has_P) return fail(DMD_ERR_STATE, "P has not been uploaded or generated");
     if (h->semiclassical || h->ph_tau_inv != 0.0)
         return fail(DMD_ERR_UNSUPPORTED, "the batched integrator has no semiclassical mode or phenomenological relaxation");
@@ -769,10 +774,26 @@ static int ensure_adaptive(dmd_scatt *h)
     return DMD_OK;
 }
 
+// update_scatt_inside: the caller's hook on the stage state y at time t, with
+// the stream idle; the reduced forms' F_eq follows a changed P
+static int run_rhs_hook(dmd_scatt *h, double t, const double2 *y)
+{
+    const size_t n = (size_t)h->nk * h->nbd * h->nbd;
+    if (h->h_hook.size() < 2 * n) h->h_hook.resize(2 * n);
+    DMD_HIP(hipMemcpyAsync(h->h_hook.data(), y, n * 16, hipMemcpyDeviceToHost, h->st));
+    DMD_HIP(hipStreamSynchronize(h->st));
+    if (h->rhs_hook(h->rhs_ctx, t, h->h_hook.data()) != 0)
+        return fail(DMD_ERR_STATE, "the right-hand-side hook stopped the integrator at t = " + std::to_string(t));
+    return tiles4_prepare(h);
+}
+
 // k = f(t + a hstep, y) with the clock at (t, hstep)
 static int full_rhs(dmd_scatt *h, const StageArgs &A, const double2 *y, double a, double2 *kout)
 {
     int rc;
+    if (h->rhs_hook &&
+        (rc = run_rhs_hook(h, h->h_clk.t0 + ((double)h->h_clk.step + a) * h->h_clk.dt, y)))
+        return rc;
     if ((rc = launch_rotate(h, y, nullptr, h->nbd, h->b0, a, h->semiclassical))) return rc;
     if ((rc = launch_contract(h, false, a, h->d_w))) return rc;
     const int64_t n = h->nk * (int64_t)h->nbd * h->nbd;
@@ -923,6 +944,14 @@ extern "C" int dmd_scatt_compute_rhs(dmd_scatt *h, double t, const double *rho, 
     if ((rc = full_rhs(h, A, h->d_ad, 0.0, h->d_ad + 3 * n))) return rc;
     if ((rc = from_device(h, ddmdt, h->d_ad + 3 * n, (size_t)n * 16))) return rc;
     DMD_HIP(hipStreamSynchronize(h->st));
+    return DMD_OK;
+}
+
+extern "C" int dmd_scatt_set_rhs_hook(dmd_scatt *h, dmd_rhs_hook fn, void *ctx)
+{
+    if (!h) return fail(DMD_ERR_ARG, "null handle");
+    h->rhs_hook = fn;
+    h->rhs_ctx = fn ? ctx : nullptr;
     return DMD_OK;
 }
 

@@ -257,6 +257,10 @@ struct dmd_scatt {
     // to every evaluation times cis((e_i - e_j) t) (dmd_scatt_set_ee_dP)
     double2 *d_dPH = nullptr;
     bool has_dP = false;
+    // host callback before every integrator right-hand side (dmd_scatt_set_rhs_hook)
+    dmd_rhs_hook rhs_hook = nullptr;
+    void *rhs_ctx = nullptr;
+    std::vector<double> h_hook;  // host copy of the stage state handed to the hook
     double2 *d_in = nullptr, *d_in1 = nullptr, *d_outw = nullptr;
     DClock *d_clk = nullptr;
     // driver: coherent H, observable, RK4 buffers (allocated on first use)

@@ -109,6 +109,40 @@ void refuse(const ParamIn &p, const std::string &key, double off_value, const st
         throw std::runtime_error(key + " = " + p.str(key, "") + ": " + why + " is not carried by this operator");
 }
 
+// update_scatt_inside's e-e rebuild from a stage state (host copy, nk x nbd^2):
+// compute() zeroes the off-diagonal elements first under alg_semiclassical
+// (DensityMatrix_Dynamics.h:249), then reset_scatt runs on the state and f_eq
+struct InsideRebuild {
+    electronphonon_mi355x<cplx> *eph;
+    std::vector<cplx> dm;
+    double *feq;
+    bool semiclassical;
+    long nk;
+    int nbd;
+    std::string err;
+};
+int inside_rebuild(void *ctx, double t, const double *rho)
+{
+    InsideRebuild &c = *(InsideRebuild *)ctx;
+    try {
+        const size_t n = (size_t)c.nk * c.nbd * c.nbd;
+        c.dm.assign((const cplx *)rho, (const cplx *)rho + n);
+        if (c.semiclassical)
+            for (long k = 0; k < c.nk; k++)
+                for (int i = 0; i < c.nbd; i++)
+                    for (int j = 0; j < c.nbd; j++)
+                        if (i != j) c.dm[(size_t)k * c.nbd * c.nbd + i * c.nbd + j] = {0, 0};
+        cplx *rows[1] = {c.dm.data()};
+        double *frows[1] = {c.feq};
+        c.eph->reset_scatt(false, true, rows, nullptr, t, frows);
+    } catch (const std::exception &e) {
+        c.err = e.what();
+        std::fprintf(stderr, "dmd_run: rebuild inside the right-hand side: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
 Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, const std::string &base_dir)
 {
     using namespace units;
@@ -307,10 +341,6 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
                                  "eeMode = Pee_update");
     if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee == 0)  // parameters.cpp:483-484
         throw std::runtime_error("want to update Pee but set freq_update_ee_model as 0");
-    if (!s.linearize && s.eeMode == "Pee_update" && s.freq_update_ee < 0)
-        throw std::runtime_error("freq_update_ee_model < 0 (the e-e P rebuilt inside every right-hand side) is not "
-                                 "carried: the integrators here run whole steps on the device; use a positive "
-                                 "freq_update_ee_model");
     if (ee && (s.scrFormula == "lindhard" || s.scrFormula == "heg" || s.scr_dynamic != "static"))
         throw std::runtime_error("eeMode: the model e-e term is built for the static RPA, unscreened, debye and "
                                  "Bechstedt screening");
@@ -851,6 +881,14 @@ int run(const std::string &folder, bool dry)
     if ((s.sparseP || s.Pin_is_sparse) && !s.linearize && !pee_update && s.scatt_enable && eph.compress_P())
         std::printf("sparse P: stored by its nonzero elements\n");
     dmd_scatt *h = eph.handle();
+    // freq_update_ee_model < 0: the e-e P rebuilt from every right-hand side's
+    // stage state (update_scatt_inside in compute, DensityMatrix_Dynamics.h:262,
+    // 281-289, 307-310), through the integrators' hook
+    InsideRebuild inside{&eph, {}, feq.data(), s.semiclassical, nk, nbd, {}};
+    if (pee_update && s.freq_update_ee < 0) {
+        check(dmd_scatt_set_rhs_hook(h, inside_rebuild, &inside), "right-hand-side hook");
+        std::printf("the e-e P is rebuilt inside every right-hand side (freq_update_ee_model < 0)\n");
+    }
     // H_coh = H_BS + H_Ez (main.cpp:51-52, DenMat.cpp:7-14): H_BS = Btot . s with
     // Btot = B, or gs/2 B + Bso_k under alg_read_Bso, plus B/2 . l under needL
     // (electron.cpp:19-42); H_Ez = scale_Ez x ldbd_HEzmat.bin when the file exists (:44-61)
@@ -1019,7 +1057,7 @@ int run(const std::string &folder, bool dry)
         // rebuilt from the state at it = 1, 1 + freq, ...; evolve_euler_one_step
         // rebuilds before report_tau, evolve_gsl after it
         auto update_ee = [&]() {
-            if (!pee_update || (it - 1) % s.freq_update_ee != 0) return;
+            if (!pee_update || s.freq_update_ee <= 0 || (it - 1) % s.freq_update_ee != 0) return;
             cplx *rows_dm[1] = {dm.data()};
             double *rows_feq[1] = {feq.data()};
             eph.reset_scatt(false, true, rows_dm, nullptr, t, rows_feq);

@@ -344,6 +344,20 @@ typedef struct {
  * (its source is not part of the reference). */
 int dmd_scatt_rkf45_apply(dmd_scatt *h, double *t, double t1, double *h_step, double hmin, double hmax,
                           double eps_abs, double eps_rel, double *rho_inout, dmd_ode_stats *stats);
+/* A host callback run before every right-hand side the integrators
+ * (dmd_scatt_integrate_rk4, dmd_scatt_rkf45_apply, dmd_scatt_integrate_euler)
+ * and dmd_scatt_compute_rhs evaluate: the place where compute() runs
+ * update_scatt_inside (DensityMatrix_Dynamics.h:243-262,281-289; eeMode =
+ * Pee_update with freq_update_ee_model < 0 rebuilds P from every stage state).
+ * t: the stage time; rho: a host copy of the stage state (nk x nb_dm^2
+ * complex, interaction picture), valid for the duration of the call; the
+ * stream is idle.  The callback may change the handle's P (dmd_scatt_copy_P,
+ * dmd_scatt_add_*, dmd_scatt_set_ee_dP) and returns 0, or nonzero to stop the
+ * integrator with DMD_ERR_STATE.  With a hook, RK4 runs its stages without a
+ * captured graph.  fn = NULL removes it.  No counterpart as a function: the
+ * reference calls update_scatt_inside from compute. */
+typedef int (*dmd_rhs_hook)(void *ctx, double t, const double *rho);
+int dmd_scatt_set_rhs_hook(dmd_scatt *h, dmd_rhs_hook fn, void *ctx);
 /* evolve_euler (DensityMatrix_Dynamics.h:177-187,227-234): n_steps of
  * rho += dt (ddmdt + ddmdt^+)/2 (update_dm_euler, DenMat.cpp:236-248) from t0. */
 int dmd_scatt_integrate_euler(dmd_scatt *h, double t0, double dt, int64_t n_steps, double *rho_inout);

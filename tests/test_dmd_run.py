@@ -112,6 +112,13 @@ def test_dry_run_accepts_the_reference_eph_ei_ee_run(tmp_path):
     assert 'key "srcFormula" is not read' in out
 
 
+def test_dry_run_accepts_the_rebuild_inside_every_rhs(tmp_path):
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\neeMode = Pee_update\nscrMode = medium\n"
+                                               "freq_update_ee_model = -1\n"))
+    assert rc == 0, err
+    assert "eeMode = Pee_update" in out
+
+
 def test_dry_run_accepts_linearize_dPee_with_Pee_update(tmp_path):
     d = _gaas_folder(tmp_path, "tstep = 1\ntend = 10\nalg_linearize_dPee = 1\nalg_ddmdteq = 1\neeMode = Pee_update\n"
                                "freq_update_ee_model = 2\nscrMode = medium\n")
@@ -122,7 +129,6 @@ def test_dry_run_accepts_linearize_dPee_with_Pee_update(tmp_path):
 
 @pytest.mark.parametrize("line,msg", [
     ("eeMode = Pee_update\nscrMode = medium", "want to update Pee but set freq_update_ee_model as 0"),
-    ("eeMode = Pee_update\nscrMode = medium\nfreq_update_ee_model = -1", "freq_update_ee_model < 0"),
     ("eeMode = Pee_fixed_at_eq", "scrMode should not be none if considering electron-electron"),
     ("alg_only_ee = 1\nalg_eph_enable = 0\nimpurity_density = 1e17\nimpMode = ab_neutral", "alg_only_ee is only possible"),
     ("eeMode = other", "eeMode must be none or Pee_fixed_at_eq"),

@@ -147,6 +147,84 @@ def test_dmd_run_pee_update_rebuilds_P_from_the_state(tmp_path):
     assert np.abs(fixed - ref).max() > 1e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("ode", ["rk4", "euler"])
+def test_dmd_run_pee_update_inside_every_rhs(tmp_path, ode):
+    """freq_update_ee_model < 0: compute() rebuilds P from every stage state
+    (update_scatt_inside, DensityMatrix_Dynamics.h:262,281-289,307-310): P =
+    P_eph + scale_ei P_imp + scale_ee P_ee(stage state) with the screening
+    re-derived from the stage state, before each right-hand side of RK4 and of
+    Euler (update_dm_euler)."""
+    from denmat_dynm_development_amd import scatt
+    from oracle import oracle
+    print(f"case dmd_run Pee_update inside every rhs ({ode})", flush=True)
+    d, info, U, e, kv, k1, k2, P1, P2, s3, R = _folder(tmp_path, 53)
+    nk, nb = e.shape
+    V = abs(np.linalg.det(R))
+    T = info.temperature
+    f = _fermi(T, info.mu, e)
+    eps = 4.0
+    km = tuple(info.kmesh)
+    sig, ethr = info.degauss, info.degauss * info.ndegauss
+
+    def model(n_free, n_i=1.0):
+        return scatt.eimp_model("debye", n_free, T, V, scatt.reciprocal_metric(R), 5, n_ionized=n_i,
+                                epsilon_background=eps, degauss=sig, ethr=ethr, kmesh=km,
+                                nk_full=info.nk_full).as_dict()
+
+    def vq_of(m):
+        return np.array([oracle.eimp_vq(m, oracle.eimp_q2(m["GGT"], kv[a], kv[b])) for a, b in zip(k1, k2)])
+
+    n0 = f.sum() / (info.nk_full * V)
+    E1, _ = oracle.ee_model_P(k1, k2, U, kv, e, f, vq_of(model(n0)), km, sig, ethr, info.nk_full, True)
+    scale_ee = float(f"{np.abs(P1).max() / np.abs(E1).max():.6g}")
+    I1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(n0, 1e-6))
+    n_i = 1e-6 * 0.5 * np.abs(P1).max() / np.abs(I1).max()
+    scale_ei, nsteps = 0.7, 4
+    (d / "param.in").write_text(f"alg_ode_method = {ode}\nBzpert = 2\ntstep = 1\ntend = {nsteps}\n"
+                                "freq_measure = 1\neeMode = Pee_update\nfreq_update_ee_model = -1\n"
+                                f"scale_ee = {scale_ee}\nscale_ei = {scale_ei}\n"
+                                f"impurity_density = {n_i / BOHR2CM ** 3:.17g}\n"
+                                "impMode = model_ionized\nscrMode = medium\nscrFormula = debye\n"
+                                f"epsilon_background = {eps}\n")
+    r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-800:])
+    assert r.returncode == 0 and "rebuilt inside every right-hand side" in r.stdout
+    rho = oracle.dm_Bpert_1st(e, f, s3, np.array([0, 0, 2 * TESLA]), T, 0.0, True).reshape(nk, nb, nb)
+    rho = (0.5 * (rho + rho.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
+    prefac = 2 * np.pi / info.nk_full
+    obs_prefac = 1.0 / (info.nk_full * V * BOHR2CM ** 3)
+    spin = lambda y, t: oracle.spin(s3[:, 2], e, f, y, t, 0, nb, obs_prefac)
+
+    def g(t, y):  # reset_scatt at the stage state, then evolve
+        occ = y.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real
+        m = model(occ.sum() / (info.nk_full * V), n_i)
+        M1, M2 = oracle.eimp_model_P(k1, k2, U, kv, e, m)
+        S = y.reshape(nk, nb, nb).copy()
+        for i in range(nb):
+            for j in range(i + 1, nb):
+                S[:, i, j] *= np.exp(1j * (e[:, j] - e[:, i]) * t)
+                S[:, j, i] = S[:, i, j].conj()
+        S1 = -S.copy()
+        S1[:, np.arange(nb), np.arange(nb)] += 1.0
+        Q1, Q2 = oracle.ee_model_P(k1, k2, U, kv, e, f, vq_of(m), km, sig, ethr, info.nk_full, True,
+                                   dm=S.reshape(nk, -1), dm1=S1.reshape(nk, -1))
+        return oracle.evolve(k1, k2, P1 + scale_ei * M1 + scale_ee * Q1, P2 + scale_ei * M2 + scale_ee * Q2, e, y,
+                             t=t, prefac=prefac)
+
+    if ode == "rk4":
+        ref = _rk4_run(g, rho, nsteps, 1 * FS, spin)
+    else:  # update_dm_euler: rho += dt (ddmdt + ddmdt^+) / 2
+        y, ref = rho.copy(), [spin(rho, 0.0)]
+        for i in range(nsteps):
+            k = g(i * FS, y).reshape(nk, nb, nb)
+            y = y + 0.5 * FS * (k + k.conj().transpose(0, 2, 1)).reshape(nk, -1)
+            ref.append(spin(y, (i + 1) * FS))
+        ref = np.array(ref)
+    rows = np.loadtxt(d / "sz_elec_tot.out")
+    print("S_z runner", rows[:, 1], "oracle", ref)
+    assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
+
+
 def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
     """alg_linearize_dPee with eeMode = Pee_update: every rebuild takes the
     e-e P at f_eq and its change dP towards the state (calc_A1rho_dP), and
