@@ -171,6 +171,7 @@ PROTOTYPES = {
     "dmd_scatt_add_ee_dP": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _i],
     "dmd_scatt_set_ee_dP": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel)],
     "dmd_scatt_get_ee_dP": [_vp, _vp],
+    "dmd_scatt_set_rhs_hook": [_vp, _vp, _vp],
     "dmd_scatt_ee_vq_cells": [_vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _vp],
     "dmd_scatt_add_Lsc_ee": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(EimpModel), _d, _d, _d, _i],
     "dmd_scatt_generate_P": [_vp, _pSP],
