@@ -21,10 +21,12 @@
 // nk_full, doubled with exchange.  The density matrices are Hermitian and
 // read from their upper triangle, as the reference's zhemm does.
 //
-// MI355X mapping: one thread per P element of a stored pair; the thread walks
-// k3 and skips, before any overlap work, every k3 whose band energies cannot
-// bring the A2 entries it multiplies inside the energy window (the rebuild's
-// cost is the k3 sum, so the window test goes first).  The chunk's P goes into
+// MI355X mapping: nb <= 2, one wavefront per stored pair (k_ee_pair2): the
+// lanes test 64 k3 at a time against the energy window and queue the
+// survivors in LDS, so every term runs with all lanes busy, and one term forms
+// the overlaps and A2 blocks once for P1 and P2; nb > 2, one thread per P
+// element (k_ee_P), which skips, before any overlap work, every k3 whose band
+// energies cannot bring the A2 entries it multiplies inside the window.  The chunk's P goes into
 // the handle's layout through place_P_dev (every precision), or, for the
 // linearized operator, the chunk's Lscij / Lscji increments are added into
 // the converted store.
@@ -291,33 +293,148 @@ __global__ __launch_bounds__(256) void k_ee_P(int64_t n, const uint64_t *__restr
     P2[idx] = make_double2(acc2.x * a.pfac, acc2.y * a.pfac);
 }
 
-// The same P for nb <= 2 with two wavefronts per stored pair, one for P1 and
-// one for P2 (one launch each; FIX < NB would split a form further by its
-// first band index): the lanes split the k3 sum, and per k3 a lane forms what
-// the wavefront's elements share once -- the energy-window test over all band
-// combinations, the overlaps <k3 x|k4 y> (and the exchange overlaps), the A2
-// blocks of every (p, q) with their Gaussians and, per (p, q), the A1 block
-// times the density matrices -- then adds its contribution to every element;
-// a wavefront reduction ends the pair.  The thread-per-element form evaluates
-// every shared factor once per element (16 times per (pair, k3) at nb = 2),
-// this one twice: the Pee_update rebuild of 964,065 pairs at nk = 4096 went
-// from 2.25 s to 0.89 s (with four wavefronts per pair, split by i1: 1.35 s).
-// WHICH 0: P1, 1: P2; FIX: the first band index i1, or NB for all of them
-template <int NB, int WHICH, int FIX>
-__global__ __launch_bounds__(256) void k_ee_pair(int64_t n, const uint64_t *__restrict__ k1,
-                                                 const uint64_t *__restrict__ k2, EeArgs a, EimpConst c,
-                                                 double2 *__restrict__ P1, double2 *__restrict__ P2)
+// One (pair, k3) term for nb <= 2 (the body of k_ee_pair2): the overlaps
+// <k3 x|k4 y> (and the exchange overlaps), the A2 blocks of every (p, q) with
+// their Gaussians, then per form and (p, q) the A1 block times the density
+// matrices (or f), added into every element.  W: 0 P1, 1 P2, 2 both (the A2
+// blocks serve both forms).
+template <int NB, int W>
+__device__ __forceinline__ void ee_pair_term(const EeArgs &a, const EimpConst &c, int64_t A, int64_t B,
+                                             const double2 *o12, const double *d12, int64_t c3, int64_t c4,
+                                             double2 *acc1, double2 *acc2)
 {
-    constexpr int NB2 = NB * NB, NB3 = NB2 * NB, NB4 = NB2 * NB2;
-    constexpr int which = WHICH, F0 = FIX == NB ? 0 : FIX, NF = FIX == NB ? NB : 1;
-    const int lane = threadIdx.x & 63;
-    const int64_t fp = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (fp >= n) return;  // whole wavefronts
-    const int64_t A = (int64_t)k1[fp], B = (int64_t)k2[fp];
+    constexpr int NB2 = NB * NB, NB3 = NB2 * NB;
     const bool ex = c.m.exchange != 0, lb = c.m.lindblad != 0;
-    double2 acc[NF * NB3];
+    const double *e3 = a.e + c3 * NB, *e4 = a.e + c4 * NB;
+    double2 o34[NB2], o14[NB2], o32[NB2];
 #pragma unroll
-    for (int e = 0; e < NF * NB3; e++) acc[e] = make_double2(0.0, 0.0);
+    for (int x = 0; x < NB; x++)
+#pragma unroll
+        for (int y = 0; y < NB; y++) o34[x * NB + y] = ee_ov<NB>(a, c3, c4, x, y);
+    if (ex) {
+        const double v14 = ee_v(a, c.m, A, c4);
+#pragma unroll
+        for (int p = 0; p < NB; p++)
+#pragma unroll
+            for (int q = 0; q < NB; q++) {
+                const double2 o = ee_ov<NB>(a, A, c4, p, q);
+                o14[p * NB + q] = make_double2(o.x * v14, o.y * v14);
+                o32[p * NB + q] = ee_ov<NB>(a, c3, B, p, q);
+            }
+    }
+    auto mee = [&](int p, int q, int x, int y) {
+        double2 m = cmul(o12[p * NB + q], o34[x * NB + y]);
+        if (ex) {
+            const double2 t = cmul(o14[p * NB + y], o32[x * NB + q]);
+            m = make_double2(0.5 * m.x - 0.5 * t.x, 0.5 * m.y - 0.5 * t.y);
+        }
+        return m;
+    };
+    double2 a2[NB2][NB2];
+#pragma unroll
+    for (int pq = 0; pq < NB2; pq++)
+#pragma unroll
+        for (int xy = 0; xy < NB2; xy++) {
+            const int p = pq / NB, q = pq % NB, x = xy / NB, y = xy % NB;
+            const double2 m = mee(p, q, x, y);
+            const double dl = ee_dl(c, d12[pq] + e3[x] - e4[y]);
+            a2[pq][xy] = make_double2(m.x * dl, m.y * dl);
+        }
+    const double *f3 = a.f + c3 * NB, *f4 = a.f + c4 * NB;
+#pragma unroll
+    for (int which = (W == 1 ? 1 : 0); which < (W == 0 ? 1 : 2); which++) {
+        // P1: dm1_k3 A1 dm_k4, P2: dm_k3 A1 dm1_k4 (upper-triangle Hermitian reads); null: diag f
+        const double2 *L = nullptr, *Rm = nullptr;
+        if (a.dm) {
+            L = which ? a.dm + c3 * NB2 : a.dm1 + c3 * NB2;
+            Rm = which ? a.dm1 + c4 * NB2 : a.dm + c4 * NB2;
+        }
+        double2 *acc = (which && W == 2) ? acc2 : acc1;
+#pragma unroll
+        for (int fv = 0; fv < NB * NB; fv++) {
+            // P1: (i1, i3) = (fix, v); P2: (i3, i1) = (v, fix)
+            const int fix = fv / NB, v = fv % NB;
+            const int p = which ? v : fix, q = which ? fix : v;
+            double2 a1[NB2], t[NB2];
+#pragma unroll
+            for (int xy = 0; xy < NB2; xy++) a1[xy] = lb ? a2[p * NB + q][xy] : mee(p, q, xy / NB, xy % NB);
+            if (L && a.dP) {  // calc_A1rho_dP: P1 (dm1_k3, 1 - f3 | dm_k4, f4), P2 (dm_k3, f3 | dm1_k4, 1 - f4)
+                double fl[NB], fr[NB];
+#pragma unroll
+                for (int x = 0; x < NB; x++) {
+                    fl[x] = which ? f3[x] : 1 - f3[x];
+                    fr[x] = which ? 1 - f4[x] : f4[x];
+                }
+                ee_a1rho_dP<NB>(a1, L, fl, Rm, fr, t);
+            } else if (L) {
+                double2 u[NB2];
+#pragma unroll
+                for (int x = 0; x < NB; x++)
+#pragma unroll
+                    for (int y = 0; y < NB; y++) {
+                        double2 sm = make_double2(0.0, 0.0);
+#pragma unroll
+                        for (int k = 0; k < NB; k++) sm = cadd(sm, cmul(a1[x * NB + k], ee_hu<NB>(Rm, k, y)));
+                        u[x * NB + y] = sm;
+                    }
+#pragma unroll
+                for (int x = 0; x < NB; x++)
+#pragma unroll
+                    for (int y = 0; y < NB; y++) {
+                        double2 sm = make_double2(0.0, 0.0);
+#pragma unroll
+                        for (int k = 0; k < NB; k++) sm = cadd(sm, cmul(ee_hu<NB>(L, x, k), u[k * NB + y]));
+                        t[x * NB + y] = sm;
+                    }
+            } else {
+#pragma unroll
+                for (int x = 0; x < NB; x++)
+#pragma unroll
+                    for (int y = 0; y < NB; y++) {
+                        const double w = which ? f3[x] * (1 - f4[y]) : (1 - f3[x]) * f4[y];
+                        t[x * NB + y] = make_double2(a1[x * NB + y].x * w, a1[x * NB + y].y * w);
+                    }
+            }
+            // P1 element (fix, r, v, s) with A2 of (i2, i4) = (r, s); P2 element
+            // (fix, s, v, r) with A2 of (i4, i2) = (r, s)
+#pragma unroll
+            for (int r = 0; r < NB; r++)
+#pragma unroll
+                for (int s = 0; s < NB; s++) {
+                    const int li = fix * NB3 + (which ? (s * NB + v) * NB + r : (r * NB + v) * NB + s);
+                    double2 sum = acc[li];
+#pragma unroll
+                    for (int xy = 0; xy < NB2; xy++) sum = cadd(sum, cmulc(t[xy], a2[r * NB + s][xy]));
+                    acc[li] = sum;
+                }
+        }
+    }
+}
+
+// P1 and P2 for nb <= 2, one wavefront per stored pair.  The lanes test 64
+// k3 at a time (k4 on the mesh, some A2 entry inside the energy window) and
+// append the survivors to a per-wavefront LDS queue (ballot and popcount);
+// every 64 queued k3 run the term with all lanes busy, so the lanes do not
+// idle through the terms of k3 outside the window, and the overlaps and A2
+// blocks serve both forms.  A wavefront reduction ends the pair.
+template <int NB, int W>
+__global__ __launch_bounds__(256) void k_ee_pair2(int64_t n, const uint64_t *__restrict__ k1,
+                                                  const uint64_t *__restrict__ k2, EeArgs a, EimpConst c,
+                                                  double2 *__restrict__ P1, double2 *__restrict__ P2)
+{
+    constexpr int NB2 = NB * NB, NB4 = NB2 * NB2;
+    __shared__ int2 queue[4][128];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t fp = (int64_t)blockIdx.x * 4 + wv;
+    if (fp >= n) return;  // whole wavefronts; only wavefront-level synchronisation below
+    const int64_t A = (int64_t)k1[fp], B = (int64_t)k2[fp];
+    const bool ex = c.m.exchange != 0;
+    constexpr int NA2 = W == 2 ? NB4 : 1;  // acc1: P1 (W = 0, 2) or P2 (W = 1); acc2: P2 when W = 2
+    double2 acc1[NB4], acc2[NA2];
+#pragma unroll
+    for (int e = 0; e < NB4; e++) acc1[e] = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < NA2; e++) acc2[e] = make_double2(0.0, 0.0);
     if (A != B || ex) {
         const double v12 = a.vq_pair ? a.vq_pair[fp] : ee_v(a, c.m, A, B);
         const double *eA = a.e + A * NB, *eB = a.e + B * NB;
@@ -333,135 +450,71 @@ __global__ __launch_bounds__(256) void k_ee_pair(int64_t n, const uint64_t *__re
                 dmin = fmin(dmin, d12[p * NB + q]);
                 dmax = fmax(dmax, d12[p * NB + q]);
             }
-        for (int64_t c3 = lane; c3 < a.nk; c3 += 64) {
-            if (ex && A == B && B == c3) continue;
-            const int64_t c4 = ee_k4(a, A, B, c3);
-            if (c4 < 0) continue;
-            const double *e3 = a.e + c3 * NB, *e4 = a.e + c4 * NB;
-            double tlo = 1e300, thi = -1e300;
-#pragma unroll
-            for (int x = 0; x < NB; x++)
-#pragma unroll
-                for (int y = 0; y < NB; y++) {
-                    tlo = fmin(tlo, e3[x] - e4[y]);
-                    thi = fmax(thi, e3[x] - e4[y]);
-                }
-            // every A2 entry outside the window: nothing to add
-            if (dmin + tlo >= c.m.ethr || dmax + thi <= -c.m.ethr) continue;
-            double2 o34[NB2], o14[NB2], o32[NB2];
-#pragma unroll
-            for (int x = 0; x < NB; x++)
-#pragma unroll
-                for (int y = 0; y < NB; y++) o34[x * NB + y] = ee_ov<NB>(a, c3, c4, x, y);
-            if (ex) {
-                const double v14 = ee_v(a, c.m, A, c4);
-#pragma unroll
-                for (int p = 0; p < NB; p++)
-#pragma unroll
-                    for (int q = 0; q < NB; q++) {
-                        const double2 o = ee_ov<NB>(a, A, c4, p, q);
-                        o14[p * NB + q] = make_double2(o.x * v14, o.y * v14);
-                        o32[p * NB + q] = ee_ov<NB>(a, c3, B, p, q);
-                    }
-            }
-            auto mee = [&](int p, int q, int x, int y) {
-                double2 m = cmul(o12[p * NB + q], o34[x * NB + y]);
-                if (ex) {
-                    const double2 t = cmul(o14[p * NB + y], o32[x * NB + q]);
-                    m = make_double2(0.5 * m.x - 0.5 * t.x, 0.5 * m.y - 0.5 * t.y);
-                }
-                return m;
-            };
-            // A2 of every (p, q): the factor conjugated in the sum
-            double2 a2[NB2][NB2];
-#pragma unroll
-            for (int pq = 0; pq < NB2; pq++)
-#pragma unroll
-                for (int xy = 0; xy < NB2; xy++) {
-                    const int p = pq / NB, q = pq % NB, x = xy / NB, y = xy % NB;
-                    const double2 m = mee(p, q, x, y);
-                    const double dl = ee_dl(c, d12[pq] + e3[x] - e4[y]);
-                    a2[pq][xy] = make_double2(m.x * dl, m.y * dl);
-                }
-            // P1: dm1_k3 A1 dm_k4, P2: dm_k3 A1 dm1_k4 (upper-triangle Hermitian reads); null: diag f
-            const double2 *L = nullptr, *Rm = nullptr;
-            if (a.dm) {
-                L = which ? a.dm + c3 * NB2 : a.dm1 + c3 * NB2;
-                Rm = which ? a.dm1 + c4 * NB2 : a.dm + c4 * NB2;
-            }
-            const double *f3 = a.f + c3 * NB, *f4 = a.f + c4 * NB;
-#pragma unroll
-            for (int fv = 0; fv < NF * NB; fv++) {
-                // P1: (i1, i3) = (fix, v); P2: (i3, i1) = (v, fix)
-                const int fi = fv / NB, fix = F0 + fi, v = fv % NB;
-                const int p = which ? v : fix, q = which ? fix : v;
-                double2 a1[NB2], t[NB2];
-#pragma unroll
-                for (int xy = 0; xy < NB2; xy++) a1[xy] = lb ? a2[p * NB + q][xy] : mee(p, q, xy / NB, xy % NB);
-                if (L && a.dP) {  // calc_A1rho_dP: P1 (dm1_k3, 1 - f3 | dm_k4, f4), P2 (dm_k3, f3 | dm1_k4, 1 - f4)
-                    double fl[NB], fr[NB];
-#pragma unroll
-                    for (int x = 0; x < NB; x++) {
-                        fl[x] = which ? f3[x] : 1 - f3[x];
-                        fr[x] = which ? 1 - f4[x] : f4[x];
-                    }
-                    ee_a1rho_dP<NB>(a1, L, fl, Rm, fr, t);
-                } else if (L) {
-                    double2 u[NB2];
+        int qlen = 0;  // wavefront-uniform
+        const uint64_t below = (1ull << lane) - 1ull;
+        for (int64_t base = 0; base < a.nk; base += 64) {
+            const int64_t c3 = base + lane;
+            int64_t c4 = -1;
+            bool ok = false;
+            if (c3 < a.nk && !(ex && A == B && B == c3)) {
+                c4 = ee_k4(a, A, B, c3);
+                if (c4 >= 0) {
+                    const double *e3 = a.e + c3 * NB, *e4 = a.e + c4 * NB;
+                    double tlo = 1e300, thi = -1e300;
 #pragma unroll
                     for (int x = 0; x < NB; x++)
 #pragma unroll
                         for (int y = 0; y < NB; y++) {
-                            double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-                            for (int k = 0; k < NB; k++) s = cadd(s, cmul(a1[x * NB + k], ee_hu<NB>(Rm, k, y)));
-                            u[x * NB + y] = s;
+                            tlo = fmin(tlo, e3[x] - e4[y]);
+                            thi = fmax(thi, e3[x] - e4[y]);
                         }
-#pragma unroll
-                    for (int x = 0; x < NB; x++)
-#pragma unroll
-                        for (int y = 0; y < NB; y++) {
-                            double2 s = make_double2(0.0, 0.0);
-#pragma unroll
-                            for (int k = 0; k < NB; k++) s = cadd(s, cmul(ee_hu<NB>(L, x, k), u[k * NB + y]));
-                            t[x * NB + y] = s;
-                        }
-                } else {
-#pragma unroll
-                    for (int x = 0; x < NB; x++)
-#pragma unroll
-                        for (int y = 0; y < NB; y++) {
-                            const double w = which ? f3[x] * (1 - f4[y]) : (1 - f3[x]) * f4[y];
-                            t[x * NB + y] = make_double2(a1[x * NB + y].x * w, a1[x * NB + y].y * w);
-                        }
+                    // some A2 entry inside the window
+                    ok = dmin + tlo < c.m.ethr && dmax + thi > -c.m.ethr;
                 }
-                // P1 element (fix, r, v, s) with A2 of (i2, i4) = (r, s); P2 element
-                // (fix, s, v, r) with A2 of (i4, i2) = (r, s): local index within the
-                // wavefront's nb^3 block (i2 i3 i4)
-#pragma unroll
-                for (int r = 0; r < NB; r++)
-#pragma unroll
-                    for (int s = 0; s < NB; s++) {
-                        const int li = fi * NB3 + (which ? (s * NB + v) * NB + r : (r * NB + v) * NB + s);
-                        double2 sum = acc[li];
-#pragma unroll
-                        for (int xy = 0; xy < NB2; xy++) sum = cadd(sum, cmulc(t[xy], a2[r * NB + s][xy]));
-                        acc[li] = sum;
-                    }
             }
+            const uint64_t m = __ballot(ok);
+            if (ok) queue[wv][qlen + __popcll(m & below)] = make_int2((int)c3, (int)c4);
+            qlen += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            if (qlen >= 64) {
+                const int2 q = queue[wv][lane];
+                ee_pair_term<NB, W>(a, c, A, B, o12, d12, q.x, q.y, acc1, acc2);
+                const int rest = qlen - 64;
+                int2 r = make_int2(0, 0);
+                if (lane < rest) r = queue[wv][64 + lane];
+                __builtin_amdgcn_wave_barrier();
+                if (lane < rest) queue[wv][lane] = r;
+                __builtin_amdgcn_wave_barrier();
+                qlen = rest;
+            }
+        }
+        if (lane < qlen) {
+            const int2 q = queue[wv][lane];
+            ee_pair_term<NB, W>(a, c, A, B, o12, d12, q.x, q.y, acc1, acc2);
         }
     }
 #pragma unroll
-    for (int e = 0; e < NF * NB3; e++)
+    for (int e = 0; e < NB4; e++)
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-            acc[e].x += __shfl_down(acc[e].x, off, 64);
-            acc[e].y += __shfl_down(acc[e].y, off, 64);
+            acc1[e].x += __shfl_down(acc1[e].x, off, 64);
+            acc1[e].y += __shfl_down(acc1[e].y, off, 64);
+        }
+#pragma unroll
+    for (int e = 0; e < NA2; e++)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            acc2[e].x += __shfl_down(acc2[e].x, off, 64);
+            acc2[e].y += __shfl_down(acc2[e].y, off, 64);
         }
     if (lane == 0) {
-        double2 *out = (which ? P2 : P1) + fp * NB4 + F0 * NB3;
+        double2 *o1 = (W == 1 ? P2 : P1) + fp * NB4;
 #pragma unroll
-        for (int e = 0; e < NF * NB3; e++) out[e] = make_double2(acc[e].x * a.pfac, acc[e].y * a.pfac);
+        for (int e = 0; e < NB4; e++) o1[e] = make_double2(acc1[e].x * a.pfac, acc1[e].y * a.pfac);
+        if (W == 2) {
+#pragma unroll
+            for (int e = 0; e < NA2; e++) P2[fp * NB4 + e] = make_double2(acc2[e].x * a.pfac, acc2[e].y * a.pfac);
+        }
     }
 }
 
@@ -840,8 +893,15 @@ void launch_ee_P(dmd_scatt *h, int64_t n, const uint64_t *d1, const uint64_t *d2
 {
     if constexpr (NB <= 2) {
         const dim3 g(nblk(n, 4)), b(256);
-        hipLaunchKernelGGL((k_ee_pair<NB, 0, NB>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
-        hipLaunchKernelGGL((k_ee_pair<NB, 1, NB>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+        // both forms in one pass (the A2 blocks serve both; twice the sums held);
+        // DMD_EE_FORMS=1: one pass per form
+        const char *env = std::getenv("DMD_EE_FORMS");
+        if (!(env && env[0] == '1')) {
+            hipLaunchKernelGGL((k_ee_pair2<NB, 2>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+        } else {
+            hipLaunchKernelGGL((k_ee_pair2<NB, 0>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+            hipLaunchKernelGGL((k_ee_pair2<NB, 1>), g, b, 0, h->st, n, d1, d2, a, c, s1, s2);
+        }
     } else
         hipLaunchKernelGGL(k_ee_P<NB>, dim3(nblk(n * NB * NB * NB * NB, 256)), dim3(256), 0, h->st, n, d1, d2, a, c, s1,
                            s2);
