@@ -95,6 +95,29 @@ def test_ee_P_at_a_state_matches_the_oracle(nb):
                 assert np.abs(E1 - Q1).max() <= 1e-12 * np.abs(Q1).max()
 
 
+@pytest.mark.parametrize("exchange", [False, True])
+def test_ee_one_pass_per_form_matches_both_forms(exchange, monkeypatch):
+    """The nb <= 2 generator forms P1 and P2 in one pass by default; one pass
+    per form (DMD_EE_FORMS=1) gives the same P, at f and at a state."""
+    scatt, oracle = _mods()
+    print(f"case ee forms exchange={exchange}", flush=True)
+    rng, km, kv, e, f, U, k1, k2 = _case(2, 140 + exchange)
+    nk = len(kv)
+    dm = _state(rng, f)
+    m = _model(scatt, "debye", U.shape[1], km, "lindblad", exchange=exchange)
+    out = {}
+    for forms in ("2", "1"):
+        monkeypatch.setenv("DMD_EE_FORMS", forms)
+        with scatt.ScatteringOperator(nk, 2, k1, k2, e) as op:
+            op.add_ee_model_state(k1, k2, U, kv, f, m, dm=dm, add=False)
+            S = op.download_P()
+            op.add_ee_model(k1, k2, U, kv, f, m, add=False)
+            out[forms] = (S, op.download_P())
+    for (a1, a2), (b1, b2) in zip(out["2"], out["1"]):
+        s = np.abs(b1).max()
+        assert s > 0 and np.abs(a1 - b1).max() <= 1e-13 * s and np.abs(a2 - b2).max() <= 1e-13 * s
+
+
 @pytest.mark.parametrize("nb", NBS)
 def test_ee_exchange_every_screening(nb):
     scatt, oracle = _mods()
