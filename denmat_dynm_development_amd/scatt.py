@@ -464,6 +464,31 @@ class ScatteringOperator:
         call("dmd_scatt_integrate_euler", self._h, float(t0), float(dt), int(n_steps), py)
         return y
 
+    def set_rhs_hook(self, fn):
+        """fn(t, rho) before every right-hand side the integrators evaluate
+        (dmd_scatt_set_rhs_hook; update_scatt_inside in compute): rho is a
+        host copy (nk, nb_dm^2) of the stage state; fn may change P (e.g. an
+        e-e rebuild at that state); an exception stops the integrator.  None
+        removes the hook."""
+        if fn is None:
+            call("dmd_scatt_set_rhs_hook", self._h, None, None)
+            self._hook = None
+            return
+        shape = (self.nk, self.nb_dm * self.nb_dm)
+
+        def tramp(ctx, t, rho):
+            try:
+                arr = np.ctypeslib.as_array(rho, shape=(shape[0] * shape[1] * 2,)).view(np.complex128).reshape(shape)
+                fn(t, arr.copy())
+                return 0
+            except Exception as e:  # reported through the integrator's error
+                self._hook_error = e
+                return 1
+
+        cb = _RHS_HOOK(tramp)
+        call("dmd_scatt_set_rhs_hook", self._h, ctypes.cast(cb, ctypes.c_void_p), None)
+        self._hook = cb  # the C side keeps only the pointer
+
     def integrate_rk4_batched(self, rho0_B, t0, dt, n_steps, report_every=0):
         B = rho0_B.shape[0]
         rho = np.array(rho0_B, dtype=np.complex128, copy=True).reshape(B, self.nk, self.nb_dm * self.nb_dm)
@@ -549,6 +574,9 @@ class ElectronPhonon(ScatteringOperator):
 
     def evolve(self, t, dm, dm1=None):
         return self.eval(t, dm, dm1)
+
+
+_RHS_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double))
 
 
 def _ee_args(op, k1st, k2nd, U, kvec, f, model):

@@ -248,3 +248,70 @@ def test_phenomenological_relaxation_term():
         op.set_phenom_relax(0.0, 0, 0, None)  # off
         inner_only = op.compute_rhs(9 * FS, rho0)
         assert rel_err(inner_only, rhs(9 * FS, rho0) + 0) > 1e-6
+
+
+def test_rhs_hook_rebuilds_P_from_every_stage_state():
+    """A right-hand-side hook (update_scatt_inside) that sets P from the stage
+    state: P = (1 + 2 Re rho_0[0,0]) P0 before every stage.  RK4 (stages at
+    t0 + (n + a) dt, uncaptured), Euler and compute_rhs against the oracle
+    applying the same rule; rkf45 calls it once per stage evaluation; a hook
+    that fails stops the integrator."""
+    _lib, scatt, oracle = _mods()
+    print("case rhs hook", flush=True)
+    rng = np.random.default_rng(91)
+    nk, nb = 40, 2
+    k1, k2 = random_pairs(nk, 5, rng, diag=True)
+    e = random_energies(nk, nb, rng)
+    P1, P2 = random_P(len(k1), nb, rng)
+    rho0 = random_rho(nk, nb, rng)
+    prefac = 2 * np.pi / nk
+    dt, n = 0.05 * FS, 4
+    scale = lambda y: 1.0 + 2.0 * y.reshape(nk, nb * nb)[0, 0].real
+    g = lambda t, y: oracle.evolve(k1, k2, scale(y) * P1, scale(y) * P2, e, y, t=t, prefac=prefac)
+    with scatt.ScatteringOperator(nk, nb, k1, k2, e) as op:
+        op.upload_P(P1, P2)
+        calls = []
+
+        def hook(t, rho):
+            calls.append(t)
+            op.upload_P(scale(rho) * P1, scale(rho) * P2)
+
+        op.set_rhs_hook(hook)
+        got, _ = op.integrate_rk4(rho0, 0.5 * FS, dt, n)
+        assert len(calls) == 4 * n
+        assert np.allclose(calls, [0.5 * FS + (i + a) * dt for i in range(n) for a in (0, 0.5, 0.5, 1)],
+                           rtol=0, atol=1e-9)
+        y = rho0.copy()
+        for i in range(n):
+            t = 0.5 * FS + i * dt
+            a1 = g(t, y)
+            a2 = g(t + dt / 2, y + dt / 2 * a1)
+            a3 = g(t + dt / 2, y + dt / 2 * a2)
+            a4 = g(t + dt, y + dt * a3)
+            y = y + dt / 6 * (a1 + 2 * a2 + 2 * a3 + a4)
+            yy = y.reshape(nk, nb, nb)
+            y = (0.5 * (yy + yy.conj().transpose(0, 2, 1))).reshape(nk, -1)
+        assert rel_err(got, y) < 1e-12
+        calls.clear()
+        got = op.integrate_euler(rho0, 0.0, dt, n)
+        y = rho0.copy()
+        for i in range(n):
+            k = g(i * dt, y).reshape(nk, nb, nb)
+            y = y + 0.5 * dt * (k + k.conj().transpose(0, 2, 1)).reshape(nk, -1)
+        assert len(calls) == n and rel_err(got, y) < 1e-12
+        calls.clear()
+        assert rel_err(op.compute_rhs(7 * FS, rho0), g(7 * FS, rho0)) < 1e-12 and len(calls) == 1
+        calls.clear()
+        _, _, _, st = op.rkf45_apply(rho0, 0.0, 2 * dt, dt, 0.0, dt, 1e-6, 0.0)
+        assert len(calls) == st["rhs_evals"] > 0
+
+        def bad(t, rho):
+            raise RuntimeError("stop")
+
+        op.set_rhs_hook(bad)
+        with pytest.raises(_lib.DMDError):
+            op.integrate_rk4(rho0, 0.0, dt, 2)
+        op.set_rhs_hook(None)
+        op.upload_P(P1, P2)
+        got, _ = op.integrate_rk4(rho0, 0.0, dt, 2)  # no hook: the captured-graph path again
+        assert np.all(np.isfinite(got))
