@@ -257,9 +257,10 @@ __device__ __forceinline__ halfx8 as_h8(uint4 u)
     return r;
 }
 
-// Visits per segment at most (the autotuner's segment lengths are 64, 128
-// and 256).
-constexpr int kSegMax = 256;
+// Visits per segment at most (the autotuner's segment lengths are 64 to
+// 1024; longer segments mean fewer row-sum flushes, whose FP32 atomics share
+// the memory system with the record stream).
+constexpr int kSegMax = 1024;
 
 // One segment of the record pass (a wavefront): the row sums of its visits
 // over NBLK blocks of 32 trajectories from b0, added into F at its end.  BF16
@@ -770,7 +771,7 @@ static int build_visits(dmd_scatt *h)
 // wavefronts share the L2 (a block of X at B trajectories is 64 B x B per row).
 static int build_segments(dmd_scatt *h, int seg_len, int pblock)
 {
-    if (seg_len < 1 || seg_len > kSegMax) return fail(DMD_ERR_ARG, "batched: segment length outside 1 .. 256");
+    if (seg_len < 1 || seg_len > kSegMax) return fail(DMD_ERR_ARG, "batched: segment length outside 1 .. 1024");
     if (h->seg_len == seg_len && h->seg_pblock == pblock && h->d_seg) return DMD_OK;
     std::vector<int4> seg;
     std::vector<int32_t> blk;
@@ -831,7 +832,7 @@ static int autotune(dmd_scatt *h, int Bp)
     if (tune && h->np > 0) {
         float best = 1e30f, ms = 0;
         int rc;
-        for (int sl : { 64, 128, 256 }) {
+        for (int sl : { 64, 128, 256, 512, 1024 }) {
             if ((rc = build_segments(h, sl, best_pb))) return rc;
             for (int nb : { 1, 2 }) {
                 if (nb == 2 && Bp % 64) continue;
