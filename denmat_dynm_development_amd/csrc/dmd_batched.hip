@@ -849,6 +849,17 @@ static int autotune(dmd_scatt *h, int Bp)
                 if (ms < best) { best = ms; best_pb = pb; }
             }
         }
+        // the segment length again within the chosen partner blocks (blocks cut
+        // segments, so the best length can differ from the row order's)
+        if (best_pb > 0) {
+            const int seg0 = best_seg;
+            for (int sl : { 64, 128, 256, 512, 1024 }) {
+                if (sl == seg0) continue;
+                if ((rc = build_segments(h, sl, best_pb))) return rc;
+                if ((rc = trial(best_nb, &ms))) return rc;
+                if (ms < best) { best = ms; best_seg = sl; }
+            }
+        }
     }
     int rc = build_segments(h, best_seg, best_pb);
     if (rc) return rc;
