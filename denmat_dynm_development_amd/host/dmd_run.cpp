@@ -298,7 +298,11 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
                                          ": the model e-i term is built for the static RPA, unscreened, debye and "
                                          "Bechstedt screening");
             // occ_of_impurity_level reads lng, which parameters.cpp never fills (Scatt_Param.h:88-93)
-            refuse(p, "partial_ionized" + sfx, 0, "partially ionized impurities");
+            // the impurity level's occupation reads lng[iD] (Scatt_Param.h:90), a vector
+            // parameters.cpp never fills (its push_back is commented out, :219)
+            refuse(p, "partial_ionized" + sfx, 0,
+                   "partially ionized impurities (the reference's occupation of the impurity level reads an "
+                   "unfilled lng vector, Scatt_Param.h:90, parameters.cpp:219)");
             if (n < 0) throw std::runtime_error("eip.ni_ionized[iD] must be postive");  // ElecImp_Model.h:33
         }
         if (Z <= 0) throw std::runtime_error("for ionized impurity, Z is defined as a positive value in this code");
