@@ -225,15 +225,17 @@ def test_dmd_run_pee_update_inside_every_rhs(tmp_path, ode):
     assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
 
 
-def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
+@pytest.mark.parametrize("freq", [2, -1])
+def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path, freq):
     """alg_linearize_dPee with eeMode = Pee_update: every rebuild takes the
     e-e P at f_eq and its change dP towards the state (calc_A1rho_dP), and
     every right-hand side adds dP's row term with its phase
     (evolve_eph.cpp:107-148); ddmdt_eq stays zero (DensityMatrix_Dynamics.h:
-    60-62)."""
+    60-62).  freq_update_ee_model = -1: the rebuild (P and dP) at every stage
+    state."""
     from denmat_dynm_development_amd import scatt
     from oracle import oracle
-    print("case dmd_run linearize_dPee", flush=True)
+    print(f"case dmd_run linearize_dPee freq={freq}", flush=True)
     d, info, U, e, kv, k1, k2, P1, P2, s3, R = _folder(tmp_path, 47)
     nk, nb = e.shape
     V = abs(np.linalg.det(R))
@@ -256,7 +258,7 @@ def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
     scale_ee = float(f"{np.abs(P1).max() / np.abs(E1).max():.6g}")
     I1, _ = oracle.eimp_model_P(k1, k2, U, kv, e, model(n0, 1e-6))
     n_i = 1e-6 * 0.5 * np.abs(P1).max() / np.abs(I1).max()
-    scale_ei, freq, nsteps = 0.7, 2, 5
+    scale_ei, nsteps = 0.7, 5
     (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\ntstep = 1\ntend = 5\nfreq_measure = 1\n"
                                 "alg_ddmdteq = 1\nalg_linearize_dPee = 1\n"
                                 f"eeMode = Pee_update\nfreq_update_ee_model = {freq}\nscale_ee = {scale_ee}\n"
@@ -276,7 +278,7 @@ def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
     P = {}
 
     def rebuild(it, t, y):
-        if (it - 1) % freq:
+        if freq > 0 and (it - 1) % freq:
             return
         occ = y.reshape(nk, nb, nb)[:, np.arange(nb), np.arange(nb)].real
         m = model(occ.sum() / (info.nk_full * V), n_i)
@@ -293,15 +295,23 @@ def test_dmd_run_linearize_dPee_adds_the_dP_term(tmp_path):
         P["1"], P["2"] = P1 + scale_ei * M1 + scale_ee * Q1, P2 + scale_ei * M2 + scale_ee * Q2
         P["D1"], P["D2"] = D1, D2
 
-    g = lambda t, y: (oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
-                      + oracle.ee_dP_term(k1, k2, P["D1"], P["D2"], e, feq, t=t, prefac=prefac))
-    ref = _rk4_run(g, rho, nsteps, 1 * FS, spin, rebuild)
+    def g(t, y):
+        if freq < 0:  # update_scatt_inside: the rebuild at the stage state
+            rebuild(1, t, y)
+        return (oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
+                + oracle.ee_dP_term(k1, k2, P["D1"], P["D2"], e, feq, t=t, prefac=prefac))
+
+    ref = _rk4_run(g, rho, nsteps, 1 * FS, spin, rebuild if freq > 0 else None)
     rows = np.loadtxt(d / "sz_elec_tot.out")
     print("S_z runner", rows[:, 1], "oracle", ref)
     assert np.abs(rows[:, 1] - ref).max() < 1e-10 * np.abs(ref).max()
     # the dP term matters: without it the run differs well beyond that
-    g0 = lambda t, y: oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
-    plain = _rk4_run(g0, rho, nsteps, 1 * FS, spin, rebuild)
+    def g0(t, y):
+        if freq < 0:
+            rebuild(1, t, y)
+        return oracle.evolve(k1, k2, P["1"], P["2"], e, y, t=t, prefac=prefac)
+
+    plain = _rk4_run(g0, rho, nsteps, 1 * FS, spin, rebuild if freq > 0 else None)
     assert np.abs(plain - ref).max() > 1e-6 * np.abs(ref).max()
 
 
