@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -66,6 +67,8 @@ struct Settings {
     bool excess = false;  // carrier_density_means_excess_density
     bool read_Bso = false, needL = false, set_scv_zero = false;
     double scale_Ez = 0;
+    bool gfac_normal = false;  // gfac_normal_dist: k-resolved random g factors
+    double gfac_mean = 2, gfac_sigma = 0, gfac_cap = 0;
     double thr_sparseP = 1e-40;
     std::string scatt = "lindblad", picture = "interaction";
     bool ddmdteq = false, expt = true, linearize = false, linearize_dPee = false;
@@ -101,6 +104,41 @@ double fermi(double T, double mu, double e)
     if (x < -46) return 1;
     if (x > 46) return 0;
     return 1. / (std::exp(x) + 1);
+}
+
+// random_normal_array(gfack, nk, mean, sigma, cap) of electron::set_gfac
+// (electron.cpp:7-18, comm/myarray.cpp:299-321, comm/Random.cpp:24-45): a
+// 64-bit Mersenne Twister seeded with n, normal draws mean + sigma z (each
+// redrawn while |sigma z| > cap when cap > 0), then shifted to the exact mean
+// and scaled to the exact population sigma about it.  The engine and the
+// distribution are the standard library's, as the reference's are.
+std::vector<double> gfac_normal_array(long n, double mean, double sigma, double cap)
+{
+    std::vector<double> a((size_t)n, 0.0);
+    if (n == 0) return a;
+    if (n == 1) { a[0] = mean; return a; }
+    std::mt19937_64 gen((uint64_t)n);
+    std::normal_distribution<double> z;
+    for (;;) {
+        for (long i = 0; i < n; i++) {
+            double r = sigma * z(gen);
+            if (cap > 0.0)
+                while (std::fabs(r) > cap) r = sigma * z(gen);
+            a[i] = mean + r;
+        }
+        if (sigma == 0) return a;
+        double sum = 0;
+        for (long i = 0; i < n; i++) sum += a[i];
+        const double mean1 = sum / n;
+        for (long i = 0; i < n; i++) a[i] = a[i] + mean - mean1;
+        double ss = 0;
+        for (long i = 0; i < n; i++) ss += (a[i] - mean) * (a[i] - mean);
+        const double sigma1 = std::sqrt(ss / n);
+        if (sigma1 == 0) continue;
+        const double fac = sigma / sigma1;
+        for (long i = 0; i < n; i++) a[i] = mean + fac * (a[i] - mean);
+        return a;
+    }
 }
 
 void refuse(const ParamIn &p, const std::string &key, double off_value, const std::string &why)
@@ -220,7 +258,13 @@ Settings read_settings(const ParamIn &p, const dmd_ldbd_info &info, int dim, con
     s.needL = p.get("needL", 0) != 0;
     s.scale_Ez = p.get("scale_Ez", 0.);
     s.set_scv_zero = p.get("alg_set_scv_zero", 0) != 0;
-    refuse(p, "gfac_normal_dist", 0, "k-resolved random g factors");
+    // k-resolved random g factors (parameters.cpp:179-183,524-525; electron.cpp:7-31)
+    s.gfac_normal = p.get("gfac_normal_dist", 0) != 0;
+    if (p.get("gfac_k_resolved", 1) == 0)
+        throw std::runtime_error("currently, g factor should not have band dependence (gfac_k_resolved = 0)");
+    s.gfac_mean = p.get("gfac_mean", 2.);
+    s.gfac_sigma = p.get("gfac_sigma", 0.);
+    s.gfac_cap = p.get("gfac_cap", 0.);
     s.excess = p.get("carrier_density_means_excess_density", 0) != 0;  // parameters.cpp:170
     // e-e (parameters.cpp:226-231,479-492): the P of set_eph at the equilibrium
     // occupations, kept fixed (Pee_fixed_at_eq), or rebuilt from the state every
@@ -588,6 +632,8 @@ int run(const std::string &folder, bool dry)
     }
     std::printf("alg_read_Bso = %d needL = %d scale_Ez = %lg alg_set_scv_zero = %d\n", (int)s.read_Bso, (int)s.needL,
                 s.scale_Ez, (int)s.set_scv_zero);
+    std::printf("gfac_normal_dist = %d gfac_mean = %lg gfac_sigma = %lg gfac_cap = %lg\n", (int)s.gfac_normal,
+                s.gfac_mean, s.gfac_sigma, s.gfac_cap);
     std::printf("alg_phenom_relax = %d tau_phenom = %.10le bStart_tau = %d bEnd_tau = %d "
                 "carrier_density_means_excess_density = %d\n", (int)s.phenom, s.tau_phenom, s.bStart_tau, s.bEnd_tau,
                 (int)s.excess);
@@ -895,12 +941,25 @@ int run(const std::string &folder, bool dry)
     }
     // H_coh = H_BS + H_Ez (main.cpp:51-52, DenMat.cpp:7-14): H_BS = Btot . s with
     // Btot = B, or gs/2 B + Bso_k under alg_read_Bso, plus B/2 . l under needL
-    // (electron.cpp:19-42); H_Ez = scale_Ez x ldbd_HEzmat.bin when the file exists (:44-61)
+    // (electron.cpp:19-42); H_Ez = scale_Ez x ldbd_HEzmat.bin when the file exists (:44-61).
+    // Under gfac_normal_dist gs becomes g_k and the orbital term is not added
+    // (electron.cpp:25-29).
     const bool has_BS = std::sqrt(s.B[0] * s.B[0] + s.B[1] * s.B[1] + s.B[2] * s.B[2]) > 1e-10 || s.read_Bso;
     std::vector<double> bso;
     if (s.read_Bso) {
         bso.resize((size_t)nk * 3);
         check(dmd_ldbd_read_Bso(ldbd.c_str(), &info, bso.data()), "ldbd_Bso.bin");
+    }
+    std::vector<double> gfack;
+    if (has_BS && s.gfac_normal) {
+        gfack = gfac_normal_array(nk, s.gfac_mean, s.gfac_sigma, s.gfac_cap);
+        const std::string fn = base + "/gfack_normal.out";
+        FILE *gf = std::fopen(fn.c_str(), "w");
+        if (!gf) throw std::runtime_error("cannot open " + fn);
+        std::fprintf(gf, "gfack:\n");
+        for (long k = 0; k < nk; k++) std::fprintf(gf, "%14.10lf\n", gfack[k]);
+        std::fprintf(gf, "\n");
+        std::fclose(gf);
     }
     std::vector<cplx> hez;
     if (std::fabs(s.scale_Ez) > 1e-10) {
@@ -914,15 +973,16 @@ int run(const std::string &folder, bool dry)
         std::vector<cplx> H((size_t)nk * nbd2, cplx{0, 0});
         for (long k = 0; k < nk; k++) {
             double Bt[3];
+            const double gk = s.gfac_normal ? gfack[k] : 2.0023193043625635;
             for (int c = 0; c < 3; c++)
-                Bt[c] = s.read_Bso ? 0.5 * 2.0023193043625635 * s.B[c] + bso[(size_t)k * 3 + c] : s.B[c];
+                Bt[c] = s.read_Bso ? 0.5 * gk * s.B[c] + bso[(size_t)k * 3 + c] : s.B[c];
             for (size_t q = 0; q < nbd2; q++) {
                 cplx v = {0, 0};
                 if (has_BS)
                     for (int c = 0; c < 3; c++) {
                         v.x += Bt[c] * smat[((size_t)k * 3 + c) * nbd2 + q].x;
                         v.y += Bt[c] * smat[((size_t)k * 3 + c) * nbd2 + q].y;
-                        if (s.needL) {
+                        if (s.needL && !s.gfac_normal) {
                             v.x += 0.5 * s.B[c] * lmat[((size_t)k * 3 + c) * nbd2 + q].x;
                             v.y += 0.5 * s.B[c] * lmat[((size_t)k * 3 + c) * nbd2 + q].y;
                         }

@@ -71,6 +71,14 @@
  *   oracle_evolve_linear   central differences of a quadratic map) when every
  *                          k has its diagonal pair; Lscii dropped for rows
  *                          without one, as the reference does; linearity.
+ *   oracle_mt19937_64      the C++ standard's known answer (10000th output
+ *                          of the default seed 5489 is 9981545732273789042).
+ *   oracle_normal_draws    bit for bit against the host C++ library's
+ *                          std::normal_distribution over std::mt19937_64.
+ *   oracle_gfac_normal     (g - mean)/sigma is the raw draws of a generator
+ *                          seeded with n standardised by their own mean and
+ *                          population sigma, with the draws beyond the cap
+ *                          skipped; n = 0, 1 and sigma = 0.
  * The bit-level rounding of the reference's CBLAS zgemm/zhemm (GSL 2.6 or MKL)
  * and libm sincos is not reproduced bit for bit ("parity unpinned" at the last
  * ulp); summation orders follow the reference's loops so results agree to a
@@ -1477,4 +1485,118 @@ int oracle_max_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-resolved random g factors, gfac_normal_dist (src_v4.5.7/electron.cpp:7-18,  */
+/* comm/myarray.cpp:299-321, comm/Random.cpp:24-45).  The reference draws from   */
+/* a std::mt19937_64 seeded with n and a std::normal_distribution<double>; both */
+/* are restated here from their published definitions: the 64-bit Mersenne     */
+/* Twister of the C++ standard ([rand.eng.mers], [rand.predef]) and the         */
+/* Marsaglia polar method as libstdc++ implements normal_distribution, with     */
+/* uniforms in [0, 1) from generate_canonical<double, 53> (one 64-bit draw      */
+/* divided by 2^64).                                                            */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint64_t mt[312]; int i; } omt64;
+
+static void omt64_seed(omt64 *g, uint64_t seed)
+{
+    g->mt[0] = seed;
+    for (int i = 1; i < 312; i++)
+        g->mt[i] = 6364136223846793005ULL * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+    g->i = 312;
+}
+
+static uint64_t omt64_next(omt64 *g)
+{
+    if (g->i >= 312) {
+        const uint64_t up = ~0ULL << 31, lo = (1ULL << 31) - 1;
+        for (int i = 0; i < 312; i++) {
+            uint64_t x = (g->mt[i] & up) | (g->mt[(i + 1) % 312] & lo);
+            uint64_t xa = x >> 1;
+            if (x & 1) xa ^= 0xB5026F5AA96619E9ULL;
+            g->mt[i] = g->mt[(i + 156) % 312] ^ xa;
+        }
+        g->i = 0;
+    }
+    uint64_t y = g->mt[g->i++];
+    y ^= (y >> 29) & 0x5555555555555555ULL;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+    y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+    y ^= y >> 43;
+    return y;
+}
+
+/* Raw engine outputs after seeding (for the standard's known answer). */
+void oracle_mt19937_64(uint64_t seed, int64_t n, uint64_t *out)
+{
+    omt64 g;
+    omt64_seed(&g, seed);
+    for (int64_t i = 0; i < n; i++) out[i] = omt64_next(&g);
+}
+
+typedef struct { omt64 g; double saved; int has_saved; } onorm;
+
+static double onorm_uniform(onorm *s)
+{
+    double u = (double)omt64_next(&s->g) / 18446744073709551616.0;
+    return u >= 1.0 ? nextafter(1.0, 0.0) : u;
+}
+
+static double onorm_next(onorm *s)
+{
+    if (s->has_saved) { s->has_saved = 0; return s->saved; }
+    double x, y, r2;
+    do {
+        x = 2.0 * onorm_uniform(s) - 1.0;
+        y = 2.0 * onorm_uniform(s) - 1.0;
+        r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    const double mult = sqrt(-2.0 * log(r2) / r2);
+    s->saved = x * mult;
+    s->has_saved = 1;
+    return y * mult;
+}
+
+/* n standard normal draws from a generator seeded with seed. */
+void oracle_normal_draws(uint64_t seed, int64_t n, double *out)
+{
+    onorm s;
+    omt64_seed(&s.g, seed);
+    s.has_saved = 0;
+    for (int64_t i = 0; i < n; i++) out[i] = onorm_next(&s);
+}
+
+/* random_normal_array(a, n, mean, sigma, cap) with no weights: n draws of
+ * mean + sigma z (each z redrawn while |sigma z| > cap when cap > 0), then
+ * shifted to the exact mean and scaled to the exact sigma (population form,
+ * about the target mean); a draw whose sigma comes out zero is repeated from
+ * the running generator. */
+void oracle_gfac_normal(int64_t n, double mean, double sigma, double cap, double *a)
+{
+    if (n == 0) return;
+    if (n == 1) { a[0] = mean; return; }
+    onorm s;
+    omt64_seed(&s.g, (uint64_t)n);
+    s.has_saved = 0;
+    for (;;) {
+        for (int64_t i = 0; i < n; i++) {
+            double r = sigma * onorm_next(&s);
+            if (cap > 0.0)
+                while (fabs(r) > cap) r = sigma * onorm_next(&s);
+            a[i] = mean + r;
+        }
+        if (sigma == 0) return;
+        double sum = 0;
+        for (int64_t i = 0; i < n; i++) sum += a[i];
+        const double mean1 = sum / (double)n;
+        for (int64_t i = 0; i < n; i++) a[i] = a[i] + mean - mean1;
+        double ss = 0;
+        for (int64_t i = 0; i < n; i++) ss += (a[i] - mean) * (a[i] - mean);
+        const double sigma1 = sqrt(ss / (double)n);
+        if (sigma1 == 0) continue;
+        const double fac = sigma / sigma1;
+        for (int64_t i = 0; i < n; i++) a[i] = mean + fac * (a[i] - mean);
+        return;
+    }
 }

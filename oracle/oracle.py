@@ -73,6 +73,9 @@ def lib():
         L.oracle_set_Lsc.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d]
         L.oracle_ee_dP_term.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, ctypes.c_double,
                                         ctypes.c_double, _d]
+        L.oracle_mt19937_64.argtypes = [ctypes.c_uint64, _i64, _u64]
+        L.oracle_normal_draws.argtypes = [ctypes.c_uint64, _i64, _d]
+        L.oracle_gfac_normal.argtypes = [_i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _d]
         L.oracle_evolve_linear.argtypes = [_i64, ctypes.c_int, _i64, _u64, _u64, _d, _d, _d, _d, _d,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, _d]
         _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -337,6 +340,31 @@ def ee_dP_term(k1, k2, dP1, dP2, e, f_eq, t=0.0, prefac=1.0):
                                  _p(_f64(f_eq)), float(t), float(prefac), _p(out))
     if rc != 0:
         raise ValueError(f"oracle_ee_dP_term failed: {rc}")
+    return out
+
+
+def mt19937_64(seed, n):
+    """The first n outputs of a 64-bit Mersenne Twister seeded with seed (the
+    engine behind src_v4.5.7/comm/Random.cpp:24)."""
+    out = np.zeros(n, np.uint64)
+    lib().oracle_mt19937_64(int(seed), int(n), _p(out))
+    return out
+
+
+def normal_draws(seed, n):
+    """n standard normal draws of the polar method over that engine
+    (src_v4.5.7/comm/Random.cpp:25,41-45 with sigma 1, mean 0, no cap)."""
+    out = np.zeros(n, np.float64)
+    lib().oracle_normal_draws(int(seed), int(n), _p(out))
+    return out
+
+
+def gfac_normal(n, mean, sigma, cap):
+    """The k-resolved g factors of gfac_normal_dist: random_normal_array(gfack,
+    nk, gfac_mean, gfac_sigma, gfac_cap) (src_v4.5.7/electron.cpp:7-18,
+    comm/myarray.cpp:299-321), generator seeded with n."""
+    out = np.zeros(n, np.float64)
+    lib().oracle_gfac_normal(int(n), float(mean), float(sigma), float(cap), _p(out))
     return out
 
 

@@ -158,7 +158,7 @@ def test_dry_run_accepts_linearize_dPee_with_Pee_update(tmp_path):
     ("alg_eph_enable = 0\nalg_only_eimp = 1", "only possible if impurity_density"),
     ("alg_only_eimp = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "Set alg_eph_enable = 0"),
     ("alg_sparseP = 1\nimpurity_density = 1e17\nimpMode = ab_neutral", "does not support sparse"),
-    ("gfac_normal_dist = 1", "random g factors"),
+    ("gfac_k_resolved = 0", "g factor should not have band dependence"),
     ("alg_eph_need_elec = 0\nalg_eph_need_hole = 0", "either alg_eph_need_elec or alg_eph_need_hole"),
 ])
 def test_dry_run_refuses_what_the_operator_does_not_carry(tmp_path, line, msg):
@@ -252,6 +252,16 @@ def test_dry_run_coherent_hamiltonian_inputs(tmp_path):
                                                    "scale_Ez = 0.25\nalg_set_scv_zero = 1\n"))
     assert rc == 0, err
     assert "alg_read_Bso = 1 needL = 1 scale_Ez = 0.25 alg_set_scv_zero = 1" in out
+    assert "not read by this runner" not in out
+    assert "gfac_normal_dist = 0 gfac_mean = 2 gfac_sigma = 0 gfac_cap = 0" in out
+
+
+def test_dry_run_k_resolved_random_g_factors(tmp_path):
+    """gfac_normal_dist with its mean, sigma and cap (parameters.cpp:179-183)."""
+    rc, out, err = _dry(_gaas_folder(tmp_path, "tstep = 1\ntend = 10\ngfac_normal_dist = 1\ngfac_mean = 2.1\n"
+                                                "gfac_sigma = 0.3\ngfac_cap = 0.5\ngfac_k_resolved = 1\n"))
+    assert rc == 0, err
+    assert "gfac_normal_dist = 1 gfac_mean = 2.1 gfac_sigma = 0.3 gfac_cap = 0.5" in out
     assert "not read by this runner" not in out
 
 

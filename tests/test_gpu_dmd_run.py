@@ -555,16 +555,19 @@ def _herm_fields(rng, nk, nb, ncomp, scale):
     return (scale * 0.5 * (a + a.conj().transpose(0, 1, 3, 2))).reshape(nk, ncomp, nb * nb)
 
 
-@pytest.mark.parametrize("scv_zero", [0, 1])
-def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
+@pytest.mark.parametrize("scv_zero,gfac", [(0, 0), (1, 0), (0, 1)])
+def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero, gfac):
     """needL, alg_read_Bso and scale_Ez together (electron.cpp:19-61, 62-98,
     DenMat.cpp:7-14): the first-order state from (gs Bpert.s + Bpert.l)/2, the
     coherent H = (gs/2 B + Bso_k).s + B/2.l + scale_Ez H_Ez, and S_z, L_z per
     report against the oracle's RK4 with that H; with alg_set_scv_zero (and
-    nv = 1) the valence-conduction elements of s and l are zeroed first."""
+    nv = 1) the valence-conduction elements of s and l are zeroed first.  With
+    gfac_normal_dist the g factor of H is the oracle's g_k (mean 2.1, sigma 0.3,
+    cap 0.5), written to gfack_normal.out, and H has no orbital term
+    (electron.cpp:7-31)."""
     from denmat_dynm_development_amd import ldbd, scatt
     from oracle import oracle
-    print(f"case dmd_run needL/read_Bso/scale_Ez scv_zero={scv_zero}", flush=True)
+    print(f"case dmd_run needL/read_Bso/scale_Ez scv_zero={scv_zero} gfac={gfac}", flush=True)
     p = scatt.synth_params(0)
     d = tmp_path / "run"
     L = d / "ldbd_data"
@@ -588,7 +591,9 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
     scale_ez, B = 0.7, np.array([0.5, 0.0, 1.5])
     (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\nBxpert = 0.5\nBx = 0.5\nBz = 1.5\nneedL = 1\n"
                                 f"alg_read_Bso = 1\nscale_Ez = {scale_ez}\nalg_set_scv_zero = {scv_zero}\n"
-                                "tstep = 1\ntend = 4\nfreq_measure = 1\n")
+                                "tstep = 1\ntend = 4\nfreq_measure = 1\n"
+                                + ("gfac_normal_dist = 1\ngfac_mean = 2.1\ngfac_sigma = 0.3\ngfac_cap = 0.5\n"
+                                   if gfac else ""))
     r = subprocess.run([RUN, str(d)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-600:], r.stderr[-600:])
     assert r.returncode == 0
@@ -611,8 +616,17 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero):
     rho0 = oracle.dm_Bpert_1st(e, f, 0.5 * (gs * s3 + l3), np.array([0.5 * TESLA, 0, 2 * TESLA]), T, 0.0, True)
     rho0 = rho0.reshape(nk, nb, nb)
     rho0 = (0.5 * (rho0 + rho0.conj().transpose(0, 2, 1))).reshape(nk, nb * nb)
-    Bt = 0.5 * gs * B * TESLA + bso
-    H = np.einsum("kc,kcq->kq", Bt, s3) + 0.5 * TESLA * np.einsum("c,kcq->kq", B, l3) + scale_ez * hez
+    if gfac:
+        gk = oracle.gfac_normal(nk, 2.1, 0.3, 0.5)
+        lines = (d / "gfack_normal.out").read_text().split("\n")
+        assert lines[0] == "gfack:" and len(lines) == nk + 3
+        assert np.abs(np.array([float(x) for x in lines[1:nk + 1]]) - gk).max() < 6e-11
+        Bt = 0.5 * gk[:, None] * B * TESLA + bso
+        H = np.einsum("kc,kcq->kq", Bt, s3) + scale_ez * hez
+    else:
+        assert not (d / "gfack_normal.out").exists()
+        Bt = 0.5 * gs * B * TESLA + bso
+        H = np.einsum("kc,kcq->kq", Bt, s3) + 0.5 * TESLA * np.einsum("c,kcq->kq", B, l3) + scale_ez * hez
     obs_prefac = 1.0 / (info.nk_full * abs(np.linalg.det(R)) * BOHR2CM ** 3)
     for name, m in (("sz", s3), ("lz", l3)):
         _, obs = oracle.rk4(k1, k2, P1, P2, e, rho0, 1 * FS, 4, prefac=2 * np.pi / info.nk_full, H=H, report_every=1,
