@@ -17,7 +17,10 @@
 // equilibrium rate at the carrier-conserving f_eq (singdenmat_k::set_dm_eq);
 // B (Bx, By, Bz) adds the coherent H_BS = B . s (with needL, alg_read_Bso and
 // scale_Ez: (gs/2 B + Bso_k) . s + B/2 . l + scale_Ez H_Ez, electron.cpp:19-61,
-// and alg_set_scv_zero); alg_linearize replaces the
+// and alg_set_scv_zero; gfac_normal_dist draws g_k per k, electron.cpp:7-31);
+// eeMode Pee_update rebuilds the e-e P from the state every
+// freq_update_ee_model steps, or inside every right-hand side when it is
+// negative, and alg_linearize_dPee adds its linearized change; alg_linearize replaces the
 // operator by its linearization at that f_eq (set_Lsc).  The evolution is the
 // on-device integrator alg_ode_method picks: rkf45 (the reference's default,
 // adaptive with ode_hstart / ode_hmin / ode_hmax / ode_epsabs), euler, or
@@ -26,9 +29,9 @@
 // alg_sparseP zeroes |P| <= alg_thr_sparseP and writes the sparse sP files;
 // alg_Pin_is_sparse reads P from them.  alg_semiclassical, alg_phenom_relax,
 // the valley k-pair filters and the scattering switches act as in the
-// reference.  Features whose physics this operator does not carry (e-e
-// rebuilt from rho, lindhard or electron-gas screening, pump pulses, k-resolved random g
-// factors) stop the run with a message rather than being ignored.
+// reference.  Features whose physics this operator does not carry (lindhard
+// or electron-gas screening, partial_ionized impurities, pump pulses) stop the
+// run with a message rather than being ignored.
 //
 //   dmd_run [--dry-run] [folder]
 #include <algorithm>
