@@ -555,7 +555,7 @@ def _herm_fields(rng, nk, nb, ncomp, scale):
     return (scale * 0.5 * (a + a.conj().transpose(0, 1, 3, 2))).reshape(nk, ncomp, nb * nb)
 
 
-@pytest.mark.parametrize("scv_zero,gfac", [(0, 0), (1, 0), (0, 1)])
+@pytest.mark.parametrize("scv_zero,gfac", [(0, 0), (1, 0), (0, 1), (0, 2)])
 def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero, gfac):
     """needL, alg_read_Bso and scale_Ez together (electron.cpp:19-61, 62-98,
     DenMat.cpp:7-14): the first-order state from (gs Bpert.s + Bpert.l)/2, the
@@ -564,7 +564,8 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero, gfac):
     nv = 1) the valence-conduction elements of s and l are zeroed first.  With
     gfac_normal_dist the g factor of H is the oracle's g_k (mean 2.1, sigma 0.3,
     cap 0.5), written to gfack_normal.out, and H has no orbital term
-    (electron.cpp:7-31)."""
+    (electron.cpp:7-31); without alg_read_Bso (gfac = 2) the field on s is B
+    itself and the g factors are still drawn and written."""
     from denmat_dynm_development_amd import ldbd, scatt
     from oracle import oracle
     print(f"case dmd_run needL/read_Bso/scale_Ez scv_zero={scv_zero} gfac={gfac}", flush=True)
@@ -590,7 +591,7 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero, gfac):
     hez.tofile(L / "ldbd_HEzmat.bin")
     scale_ez, B = 0.7, np.array([0.5, 0.0, 1.5])
     (d / "param.in").write_text("alg_ode_method = rk4\nBzpert = 2\nBxpert = 0.5\nBx = 0.5\nBz = 1.5\nneedL = 1\n"
-                                f"alg_read_Bso = 1\nscale_Ez = {scale_ez}\nalg_set_scv_zero = {scv_zero}\n"
+                                f"alg_read_Bso = {int(gfac != 2)}\nscale_Ez = {scale_ez}\nalg_set_scv_zero = {scv_zero}\n"
                                 "tstep = 1\ntend = 4\nfreq_measure = 1\n"
                                 + ("gfac_normal_dist = 1\ngfac_mean = 2.1\ngfac_sigma = 0.3\ngfac_cap = 0.5\n"
                                    if gfac else ""))
@@ -621,7 +622,7 @@ def test_dmd_run_orbital_bso_and_ez_terms(tmp_path, scv_zero, gfac):
         lines = (d / "gfack_normal.out").read_text().split("\n")
         assert lines[0] == "gfack:" and len(lines) == nk + 3
         assert np.abs(np.array([float(x) for x in lines[1:nk + 1]]) - gk).max() < 6e-11
-        Bt = 0.5 * gk[:, None] * B * TESLA + bso
+        Bt = 0.5 * gk[:, None] * B * TESLA + bso if gfac == 1 else np.broadcast_to(B * TESLA, (nk, 3))
         H = np.einsum("kc,kcq->kq", Bt, s3) + scale_ez * hez
     else:
         assert not (d / "gfack_normal.out").exists()
